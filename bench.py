@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark: TopoUSM Fast over a 16384 x 16384 float32 fBm DEM on one MI355X.
+
+Workload (the headline configuration): seeded spectral fBm (Hurst 0.75,
+relief 0-3776 m, 5 m cells), radii 2, 8, 32, 128, 512, 2048 with the 2^n
+weight profile normalised to 1, 4096 px tiles with the reference halo, spatial
+mode; outputs float32 and uint8.  One step is the whole job with the DEM
+resident in HBM: NoData scan, overview field for the large radii, the
+stratified global-statistics pre-pass (p99 on the GPU), and the tiled main
+pass writing both encodings.
+
+Prints ONE JSON line.  ``roofline`` is measured with HIP events around every
+launch of the dominant kernel (k_usm_main) inside the timed steps;
+``cpu_baseline`` times the CPU oracle (NumPy/SciPy restatement of the
+reference's tile compute) on a bounded set of the same tile windows.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Gpix/s & %HBM peak per algorithm; surrogate vs exact openness at ≤1° MAE"
+HBM_PEAK_GBS = 8000.0
+N_PX = 16384
+RADII = [2, 8, 32, 128, 512, 2048]
+WEIGHTS = [2.0 ** (5 - i) / 63.0 for i in range(6)]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=N_PX)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-tiles", type=int, default=0, help="tiles in the CPU sample (0 = auto)")
+    return ap.parse_args()
+
+
+def make_dem(n, device):
+    from fujishadergpu_amd import synth
+    return synth.fbm(n, n, 0.75, 3776.0, seed=2, device=device)
+
+
+def run_step(dem, out_u8, out_f32):
+    from fujishadergpu_amd.scheduler import process_dem
+    return process_dem(dem, "topousm_fast", tile_size=4096, pixel_size=5.0, mode="spatial",
+                       radii=RADII, weights=WEIGHTS, output_dtype="uint8", out=out_u8,
+                       out_f32=out_f32, copy_input=False, return_plan=True)
+
+
+def cpu_baseline(dem, plan, stats_scale, gpu_f32, n_tiles):
+    """Oracle per-tile compute on the host, ThreadPool over tile windows."""
+    import numpy as np
+    from oracle import ref
+
+    host = dem.cpu().numpy()
+    lr, lw = plan.extras["large"]
+    coarse, scale = ref.overview_block_mean(host)
+    field = ref.topousm_large_coarse_field(coarse, large_radii=lr, large_weights=lw,
+                                           decimation=scale)
+    p = plan.params
+    tiles = plan.tiles
+    interior = [t for t in tiles if 0 < t[0] < max(x[0] for x in tiles)
+                and 0 < t[1] < max(x[1] for x in tiles)]
+    cores = len(os.sched_getaffinity(0))
+    if n_tiles <= 0:
+        n_tiles = 4 if cores < 16 else min(len(tiles), 8)
+    sample = (interior + [t for t in tiles if t not in interior])[:n_tiles]
+
+    def one(t):
+        ty, tx, cx, cy, cw, ch, wx, wy, ww, wh = t
+        win = host[wy:wy + wh, wx:wx + ww]
+        out = ref.topousm_direct(win, radii=p["radii"], weights=p["weights"], pixel_size=5.0,
+                                 coarse_field=field, small_radii=p["_topousm_fast_small_radii"],
+                                 small_weights=p["_topousm_fast_small_weights"],
+                                 w_large=p["_topousm_fast_w_large"], field_offset=(wy, wx),
+                                 full_shape=(host.shape[0], host.shape[1]),
+                                 stats=(stats_scale,))
+        return t, out[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw]
+
+    threads = max(1, min(cores, len(sample)))
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, sample))
+    wall = time.perf_counter() - t0
+    px = sum(t[4] * t[5] for t in sample)
+    # parity of the GPU float32 output on the sampled cores
+    g = gpu_f32.cpu().numpy()
+    maxd = 0.0
+    for t, core in res:
+        ty, tx, cx, cy, cw, ch = t[:6]
+        d = np.abs(g[cy:cy + ch, cx:cx + cw].astype(np.float64) - core)
+        maxd = max(maxd, float(np.nanmax(d)))
+    return {
+        "value": px / wall / 1e9, "unit": "Gpix/s", "cores": threads, "kind": "port",
+        "sample": f"{len(sample)} tile windows (cores {sample[0][4]}x{sample[0][5]} px, "
+                  f"halo {plan.padding}) of the 16384^2 job, oracle topousm_direct with the "
+                  f"GPU-computed p99 scale; ThreadPool of {threads}",
+        "seconds": wall, "max_abs_diff_vs_gpu": maxd,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
+              "is design-only in this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from fujishadergpu_amd import _native as N
+
+    dev = torch.device("cuda:0")
+    n = args.size
+    dem = make_dem(n, dev)
+    out_u8 = torch.empty((n, n), dtype=torch.uint8, device=dev)
+    out_f32 = torch.empty((n, n), dtype=torch.float32, device=dev)
+    L = N.lib()
+    for _ in range(args.warmup):
+        run_step(dem, out_u8, out_f32)
+    torch.cuda.synchronize()
+    L.fsg_timing_read(b"k_usm_main", ctypes.byref(ctypes.c_double()), ctypes.byref(ctypes.c_int64()))
+    L.fsg_timing_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, results, plan = run_step(dem, out_u8, out_f32)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    L.fsg_timing_enable(0)
+    tot_ms, cnt = ctypes.c_double(), ctypes.c_int64()
+    N.check(L.fsg_timing_read(b"k_usm_main", ctypes.byref(tot_ms), ctypes.byref(cnt)))
+    ms_step = wall * 1e3 / args.steps
+    gpix = n * n / 1e9 / (ms_step / 1e3)
+
+    # algorithmic bytes of k_usm_main per step: main pass = core px x (4 B read
+    # + 4 B f32 + 1 B u8 written); pre-pass = pooled px x (4 B read + 4 B write)
+    from fujishadergpu_amd.params import norm_stat_window_geometry, stratified_windows
+    main_px = sum(t[4] * t[5] for t in plan.tiles)
+    margin, tile = norm_stat_window_geometry("topousm_fast", {"radii": RADII})
+    wins = stratified_windows(n, n, 0, n, 0, n, grid=3, tile=min(tile, n))
+    pre_px = sum((th - 2 * min(margin, th // 3, tw // 3)) * (tw - 2 * min(margin, th // 3, tw // 3))
+                 for _, _, tw, th in wins)
+    bytes_step = main_px * 9 + pre_px * 8
+    launches_step = cnt.value / args.steps
+    avg_ms = tot_ms.value / max(1, cnt.value)
+    achieved = bytes_step / (tot_ms.value / args.steps / 1e3) / 1e9
+    stats_scale = float(plan.stats[0].item())
+
+    line = {
+        "metric": METRIC, "value": round(gpix, 3), "unit": "Gpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"topousm_fast spatial, {n}x{n} f32 fBm DEM (H=0.75, 0-3776 m, 5 m "
+                        f"cells, seed 2), radii {RADII}, 2^n weights, tile 4096 + halo "
+                        f"{plan.padding}, outputs float32+uint8, stratified p99 pre-pass on GPU",
+            "tiles": len(plan.tiles), "overview_decimation": plan.extras.get("overview_scale"),
+            "p99_scale": stats_scale,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_usm_main", "launches_per_step": launches_step,
+            "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_step": bytes_step,
+            "kernel_share_of_step": round(tot_ms.value / args.steps / ms_step, 3),
+        },
+        "main_pass_tiles": len(plan.tiles),
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(dem, plan, stats_scale, out_f32, args.cpu_tiles)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+"""ctypes binding of libfsg.so (include/fsg.h).
+
+The library is the product: there is no CPU fallback.  ``lib()`` raises
+``NativeLibraryError`` when libfsg.so is missing or lacks a symbol, and every
+wrapper raises ``FsgError`` on a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("FSG_LIBRARY", os.path.join(_PKG, "libfsg.so"))
+
+FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 3, 4
+OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
+POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
+MAX_RADII = 16
+
+
+class NativeLibraryError(RuntimeError):
+    """libfsg.so is missing, stale or incomplete."""
+
+
+class FsgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[fsg status {code}] {msg}")
+        self.code = code
+
+
+class fsg_window(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("ld", C.c_int64), ("h", C.c_int32), ("w", C.c_int32)]
+
+
+class fsg_region(C.Structure):
+    _fields_ = [("r0", C.c_int32), ("c0", C.c_int32), ("h", C.c_int32), ("w", C.c_int32)]
+
+
+class fsg_output(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p), ("ld", C.c_int64), ("dtype", C.c_int32), ("clip01", C.c_int32),
+        ("a_coef", C.c_float), ("b_coef", C.c_float), ("dn_min", C.c_int32), ("dn_max", C.c_int32),
+        ("post_kind", C.c_int32), ("pad0", C.c_int32), ("host_stats", C.c_double * 2),
+        ("dev_stats", C.c_void_p),
+    ]
+
+
+class fsg_shade_params(C.Structure):
+    _fields_ = [
+        ("step_x", C.c_double), ("step_y", C.c_double), ("sign_x", C.c_double),
+        ("sign_y", C.c_double), ("z_factor", C.c_double), ("azimuth_deg", C.c_double),
+        ("altitude_deg", C.c_double), ("slope_unit", C.c_int32), ("nan_hint", C.c_int32),
+    ]
+
+
+class fsg_curvature_params(C.Structure):
+    _fields_ = [("step_x", C.c_double), ("step_y", C.c_double),
+                ("curvature_type", C.c_int32), ("nan_hint", C.c_int32)]
+
+
+class fsg_ray_sample(C.Structure):
+    _fields_ = [("dx", C.c_int32), ("dy", C.c_int32), ("skip", C.c_int32), ("pad0", C.c_int32),
+                ("phys", C.c_double)]
+
+
+class fsg_openness_params(C.Structure):
+    _fields_ = [("openness_type", C.c_int32), ("n_dirs", C.c_int32), ("n_dist", C.c_int32),
+                ("pad0", C.c_int32), ("samples", C.POINTER(fsg_ray_sample))]
+
+
+class fsg_ao_sample(C.Structure):
+    _fields_ = [("dx", C.c_int32), ("dy", C.c_int32), ("skip", C.c_int32), ("pad0", C.c_int32),
+                ("phys", C.c_double), ("dist_factor", C.c_double)]
+
+
+class fsg_ao_params(C.Structure):
+    _fields_ = [("n_samples", C.c_int32), ("pad_d", C.c_int32), ("intensity", C.c_double),
+                ("samples", C.POINTER(fsg_ao_sample))]
+
+
+class fsg_topousm_params(C.Structure):
+    _fields_ = [
+        ("n_radii", C.c_int32), ("radii", C.c_int32 * MAX_RADII), ("weights", C.c_float * MAX_RADII),
+        ("ds", C.c_int32 * MAX_RADII), ("field", C.c_void_p), ("field_h", C.c_int32),
+        ("field_w", C.c_int32), ("full_h", C.c_int32), ("full_w", C.c_int32),
+        ("win_r0", C.c_int32), ("win_c0", C.c_int32), ("w_large", C.c_float), ("nan_hint", C.c_int32),
+    ]
+
+
+P = C.POINTER
+vp = C.c_void_p
+_SIGS = {
+    "fsg_abi_version": (C.c_int, []),
+    "fsg_last_error": (C.c_char_p, []),
+    "fsg_launch_count": (C.c_int64, []),
+    "fsg_timing_enable": (C.c_int, [C.c_int32]),
+    "fsg_timing_read": (C.c_int, [C.c_char_p, P(C.c_double), P(C.c_int64)]),
+    "fsg_nodata_to_nan": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                    P(C.c_int32), C.c_int32, vp, vp]),
+    "fsg_hillshade_slope": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_shade_params),
+                                      P(fsg_output), P(fsg_output), vp]),
+    "fsg_curvature": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_curvature_params),
+                                P(fsg_output), vp]),
+    "fsg_openness": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_openness_params),
+                               P(fsg_output), vp]),
+    "fsg_ambient_occlusion_workspace_size": (C.c_size_t, [P(fsg_window), P(fsg_region)]),
+    "fsg_ambient_occlusion": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_ao_params),
+                                        P(fsg_output), vp, C.c_size_t, vp]),
+    "fsg_topousm_workspace_size": (C.c_size_t, [P(fsg_window), P(fsg_topousm_params)]),
+    "fsg_topousm": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_topousm_params),
+                              P(fsg_output), P(fsg_output), vp, C.c_size_t, vp]),
+    "fsg_block_mean": (C.c_int, [P(fsg_window), C.c_int32, vp, C.c_int64, vp]),
+    "fsg_topousm_field_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "fsg_topousm_coarse_field": (C.c_int, [P(fsg_window), C.c_int32, P(C.c_int32), P(C.c_float),
+                                           vp, C.c_int64, vp, C.c_size_t, vp]),
+    "fsg_percentile_workspace_size": (C.c_size_t, [C.c_int64]),
+    "fsg_percentile": (C.c_int, [vp, C.c_int64, C.c_int32, P(C.c_double), C.c_int32, vp, vp, vp,
+                                 vp, C.c_size_t, vp]),
+    "fsg_finalize_stats": (C.c_int, [C.c_int32, vp, vp, vp, vp, vp]),
+    "fsg_copy_region": (C.c_int, [P(fsg_window), P(fsg_region), vp, vp]),
+    "fsg_encode": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_output), vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def declared_symbols():
+    """Names of every entry point the C header declares (parsed from include/fsg.h)."""
+    import re
+    hdr = os.path.join(os.path.dirname(_PKG), "include", "fsg.h")
+    text = open(hdr, encoding="utf-8").read()
+    return sorted(set(re.findall(r"\b(fsg_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Load libfsg.so once; raise NativeLibraryError if it is missing/incomplete."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryError(
+                f"{LIB_PATH} not found: build it with `python -m fujishadergpu_amd.build` "
+                "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+        try:
+            L = C.CDLL(LIB_PATH)
+        except OSError as exc:
+            raise NativeLibraryError(f"cannot load {LIB_PATH}: {exc}") from exc
+        for name, (res, args) in _SIGS.items():
+            try:
+                fn = getattr(L, name)
+            except AttributeError as exc:
+                raise NativeLibraryError(f"{LIB_PATH} lacks symbol {name}") from exc
+            fn.restype = res
+            fn.argtypes = args
+        if L.fsg_abi_version() != 1:
+            raise NativeLibraryError("libfsg.so ABI version mismatch")
+        _lib = L
+        return _lib
+
+
+def check(code: int) -> None:
+    if code != FSG_OK:
+        msg = lib().fsg_last_error()
+        raise FsgError(code, msg.decode("utf-8", "replace") if msg else "")

@@ -1,0 +1,374 @@
+// Library plumbing: error reporting, validation, numpy-compatible host math,
+// NoData preparation, region copies and block-mean decimation.
+#include <algorithm>
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "fsg_common.hpp"
+
+namespace fsg {
+
+static thread_local std::string g_last_error;
+static thread_local int64_t g_launches = 0;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return FSG_OK;
+  return fail(FSG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void count_launch(int n) { g_launches += n; }
+
+// ------------------------------------------------------------ kernel timing
+struct EvPair {
+  hipEvent_t a, b;
+};
+static std::mutex g_tmu;
+static std::atomic<bool> g_timing{false};
+static std::map<std::string, std::vector<EvPair>> g_events;
+static std::map<std::string, hipEvent_t> g_open;
+
+bool timing_on() { return g_timing.load(std::memory_order_relaxed); }
+
+void timing_start(const char* name, hipStream_t s) {
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEventRecord(e, s);
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_open[name] = e;
+}
+
+void timing_stop(const char* name, hipStream_t s) {
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEventRecord(e, s);
+  std::lock_guard<std::mutex> lk(g_tmu);
+  auto it = g_open.find(name);
+  if (it == g_open.end()) {
+    hipEventDestroy(e);
+    return;
+  }
+  g_events[name].push_back(EvPair{it->second, e});
+  g_open.erase(it);
+}
+
+int validate_window(const fsg_window* w, const char* who) {
+  if (!w) return fail(FSG_E_INVALID, std::string(who) + ": window is NULL");
+  if (w->h < 0 || w->w < 0 || (w->h > 0 && w->w > 0 && w->data == nullptr))
+    return fail(FSG_E_INVALID, std::string(who) + ": bad window");
+  if (w->ld < w->w) return fail(FSG_E_INVALID, std::string(who) + ": window ld < w");
+  return FSG_OK;
+}
+
+int validate_region(const fsg_window* w, const fsg_region* r, const char* who) {
+  if (!r) return fail(FSG_E_INVALID, std::string(who) + ": region is NULL");
+  if (r->r0 < 0 || r->c0 < 0 || r->h < 0 || r->w < 0 || (int64_t)r->r0 + r->h > w->h ||
+      (int64_t)r->c0 + r->w > w->w)
+    return fail(FSG_E_INVALID, std::string(who) + ": region outside window");
+  return FSG_OK;
+}
+
+int validate_output(const fsg_output* o, const fsg_region* r, const char* who) {
+  if (!o) return FSG_OK;
+  if (r->h > 0 && r->w > 0 && o->data == nullptr)
+    return fail(FSG_E_INVALID, std::string(who) + ": output data is NULL");
+  if (o->ld < r->w) return fail(FSG_E_INVALID, std::string(who) + ": output ld < region w");
+  if (o->dtype < FSG_OUT_F32 || o->dtype > FSG_OUT_U8)
+    return fail(FSG_E_INVALID, std::string(who) + ": bad output dtype");
+  if (o->post_kind < FSG_POST_NONE || o->post_kind > FSG_POST_STRETCH)
+    return fail(FSG_E_INVALID, std::string(who) + ": bad post_kind");
+  return FSG_OK;
+}
+
+double np_pairwise_sum(const double* a, int64_t n) {
+  if (n < 8) {
+    double res = 0.;
+    for (int64_t i = 0; i < n; i++) res += a[i];
+    return res;
+  } else if (n <= 128) {
+    double r[8];
+    for (int j = 0; j < 8; j++) r[j] = a[j];
+    int64_t i;
+    for (i = 8; i < n - (n % 8); i += 8)
+      for (int j = 0; j < 8; j++) r[j] += a[i + j];
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; i++) res += a[i];
+    return res;
+  } else {
+    int64_t n2 = n / 2;
+    n2 -= n2 % 8;
+    return np_pairwise_sum(a, n2) + np_pairwise_sum(a + n2, n - n2);
+  }
+}
+
+void gaussian_weights(double sigma, int radius, double* w) {
+  const double sigma2 = sigma * sigma;
+  const int n = 2 * radius + 1;
+  std::vector<double> phi(n);
+  for (int i = 0; i < n; i++) {
+    const double x = (double)(i - radius);
+    phi[i] = std::exp(-0.5 / sigma2 * (x * x));
+  }
+  const double s = np_pairwise_sum(phi.data(), n);
+  for (int i = 0; i < n; i++) w[i] = phi[i] / s;
+}
+
+}  // namespace fsg
+
+using namespace fsg;
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_nodata_to_nan(float* __restrict__ ras, int64_t ld, int h, int w,
+                                int has_nodata, double nodata) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (c >= w || r >= h) return;
+  float* p = ras + (int64_t)r * ld + c;
+  const float v = *p;
+  if (has_nodata && !isnan(v)) {
+    if (fabs((double)v - nodata) <= 1e-6) *p = __builtin_nanf("");
+  }
+}
+
+// Counts the NaN cells of each window: blockIdx.z = window, blockIdx.y = row
+// group, x = columns (no per-element division).
+struct WinChunk {
+  int4 w[32];  // r0, c0, h, w
+};
+
+__global__ void __launch_bounds__(256)
+k_count_nan(const float* __restrict__ ras, int64_t ld, WinChunk wins, int n_windows,
+            unsigned long long* __restrict__ counts) {
+  const int wi = blockIdx.z;
+  if (wi >= n_windows) return;
+  const int4 wd = wins.w[wi];
+  unsigned long long local = 0;
+  for (int r = blockIdx.y; r < wd.z; r += gridDim.y) {
+    const float* row = ras + (int64_t)(wd.x + r) * ld + wd.y;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < wd.w; c += gridDim.x * 256)
+      local += isnan(row[c]) ? 1ull : 0ull;
+  }
+  for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(&counts[wi], local);
+}
+
+__global__ void k_copy_region(const float* __restrict__ src, int64_t ld, int r0, int c0,
+                              int h, int w, float* __restrict__ dst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (c >= w || r >= h) return;
+  dst[(int64_t)r * w + c] = src[(int64_t)(r0 + r) * ld + c0 + c];
+}
+
+__global__ void k_encode(const float* __restrict__ src, int64_t ld, int r0, int c0, int h, int w,
+                         OutDev out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (c >= w || r >= h) return;
+  const PostK pp = load_post(out);
+  store_out(out, r, c, apply_post(pp, src[(int64_t)(r0 + r) * ld + c0 + c]));
+}
+
+// NaN-aware f x f block mean: sum in f64 (exact for cell sizes used here),
+// rounded once to f32, then f32 total / f32 count (_nan_utils.py:638-643).
+__global__ void k_block_mean(const float* __restrict__ src, int64_t ld, int h, int w, int f,
+                             float* __restrict__ out, int64_t ldo, int oh, int ow) {
+  const int oc = blockIdx.x * blockDim.x + threadIdx.x;
+  const int orow = blockIdx.y;
+  if (oc >= ow || orow >= oh) return;
+  const int r0 = orow * f, c0 = oc * f;
+  const int r1 = min(r0 + f, h), c1 = min(c0 + f, w);
+  double tot = 0.0;
+  int cnt = 0;
+  for (int r = r0; r < r1; r++) {
+    const float* row = src + (int64_t)r * ld;
+    for (int c = c0; c < c1; c++) {
+      const float v = row[c];
+      if (isfinite(v)) {
+        tot += (double)v;
+        cnt++;
+      }
+    }
+  }
+  float res;
+  if (cnt > 0) {
+    res = (float)tot / (float)cnt;
+  } else {
+    res = __builtin_nanf("");
+  }
+  out[(int64_t)orow * ldo + oc] = res;
+}
+
+// Power-of-two factor: lanes sum their column over the f rows, then groups of
+// f lanes reduce by shuffles (coalesced reads).
+__global__ void __launch_bounds__(256)
+k_block_mean_p2(const float* __restrict__ src, int64_t ld, int h, int w, int f,
+                float* __restrict__ out, int64_t ldo, int oh, int ow) {
+  const int orow = blockIdx.y;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int r0 = orow * f, r1 = min(r0 + f, h);
+  double tot = 0.0;
+  int cnt = 0;
+  if (c < w) {
+    for (int r = r0; r < r1; r++) {
+      const float v = src[(int64_t)r * ld + c];
+      if (isfinite(v)) {
+        tot += (double)v;
+        cnt++;
+      }
+    }
+  }
+  for (int off = 1; off < f; off <<= 1) {
+    tot += __shfl_down(tot, off, 64);
+    cnt += __shfl_down(cnt, off, 64);
+  }
+  if ((threadIdx.x & (f - 1)) == 0) {
+    const int oc = c / f;
+    if (oc < ow && orow < oh)
+      out[(int64_t)orow * ldo + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
+  }
+}
+
+// -------------------------------------------------------------------- API
+
+extern "C" int fsg_abi_version(void) { return FSG_ABI_VERSION; }
+
+extern "C" const char* fsg_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int64_t fsg_launch_count(void) { return g_launches; }
+
+extern "C" int fsg_timing_enable(int32_t on) {
+  g_timing.store(on != 0);
+  return FSG_OK;
+}
+
+extern "C" int fsg_timing_read(const char* name, double* total_ms, int64_t* count) {
+  if (!name || !total_ms || !count) return fail(FSG_E_INVALID, "fsg_timing_read: NULL argument");
+  std::vector<EvPair> evs;
+  {
+    std::lock_guard<std::mutex> lk(g_tmu);
+    auto it = g_events.find(name);
+    if (it != g_events.end()) {
+      evs.swap(it->second);
+      g_events.erase(it);
+    }
+  }
+  double tot = 0.0;
+  for (auto& p : evs) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(p.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+    if (e != hipSuccess) return check_hip(e, "fsg_timing_read");
+    tot += ms;
+  }
+  *total_ms = tot;
+  *count = (int64_t)evs.size();
+  return FSG_OK;
+}
+
+extern "C" int fsg_nodata_to_nan(float* raster, int64_t ld, int32_t h, int32_t w,
+                                 int32_t has_nodata, double nodata,
+                                 const int32_t* windows, int32_t n_windows,
+                                 int64_t* counts, void* stream) {
+  if (h < 0 || w < 0 || ld < w || (h > 0 && w > 0 && !raster))
+    return fail(FSG_E_INVALID, "fsg_nodata_to_nan: bad raster");
+  if (n_windows < 0 || (n_windows > 0 && (!windows || !counts)))
+    return fail(FSG_E_INVALID, "fsg_nodata_to_nan: bad windows");
+  hipStream_t s = as_stream(stream);
+  if (has_nodata && h > 0 && w > 0) {
+    dim3 blk(256), grd((w + 255) / 256, h);
+    if (std::isnan(nodata)) has_nodata = 0;  // NaN sentinel: isnan only
+    if (has_nodata) {
+      hipLaunchKernelGGL(k_nodata_to_nan, grd, blk, 0, s, raster, ld, h, w, has_nodata, nodata);
+      FSG_CHECK_LAUNCH("k_nodata_to_nan");
+    }
+  }
+  for (int base = 0; base < n_windows; base += 32) {
+    WinChunk chunk{};
+    const int n = std::min(32, n_windows - base);
+    for (int i = 0; i < n; i++) {
+      const int32_t* q = windows + 4 * (base + i);
+      if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[3] < 0 || q[0] + q[2] > h || q[1] + q[3] > w)
+        return fail(FSG_E_INVALID, "fsg_nodata_to_nan: window outside raster");
+      chunk.w[i] = make_int4(q[0], q[1], q[2], q[3]);
+    }
+    int maxw = 1, maxh = 1;
+    for (int i = 0; i < n; i++) {
+      maxw = std::max(maxw, chunk.w[i].w);
+      maxh = std::max(maxh, chunk.w[i].z);
+    }
+    dim3 blk(256), grd(std::min(16, (maxw + 255) / 256), std::min(256, maxh), n);
+    hipLaunchKernelGGL(k_count_nan, grd, blk, 0, s, raster, ld, chunk, n,
+                       (unsigned long long*)(counts + base));
+    FSG_CHECK_LAUNCH("k_count_nan");
+  }
+  return FSG_OK;
+}
+
+extern "C" int fsg_copy_region(const fsg_window* win, const fsg_region* reg, float* dst,
+                               void* stream) {
+  int rc = validate_window(win, "fsg_copy_region");
+  if (rc) return rc;
+  rc = validate_region(win, reg, "fsg_copy_region");
+  if (rc) return rc;
+  if (reg->h == 0 || reg->w == 0) return FSG_OK;
+  if (!dst) return fail(FSG_E_INVALID, "fsg_copy_region: dst is NULL");
+  dim3 blk(256), grd((reg->w + 255) / 256, reg->h);
+  hipLaunchKernelGGL(k_copy_region, grd, blk, 0, as_stream(stream), win->data, win->ld,
+                     reg->r0, reg->c0, reg->h, reg->w, dst);
+  FSG_CHECK_LAUNCH("k_copy_region");
+  return FSG_OK;
+}
+
+extern "C" int fsg_encode(const fsg_window* src, const fsg_region* reg, const fsg_output* out,
+                          void* stream) {
+  int rc = validate_window(src, "fsg_encode");
+  if (rc) return rc;
+  rc = validate_region(src, reg, "fsg_encode");
+  if (rc) return rc;
+  if (!out) return fail(FSG_E_INVALID, "fsg_encode: output NULL");
+  if ((rc = validate_output(out, reg, "fsg_encode"))) return rc;
+  if (reg->h == 0 || reg->w == 0) return FSG_OK;
+  dim3 blk(256), grd((reg->w + 255) / 256, reg->h);
+  hipLaunchKernelGGL(k_encode, grd, blk, 0, as_stream(stream), src->data, src->ld, reg->r0,
+                     reg->c0, reg->h, reg->w, make_out(out));
+  FSG_CHECK_LAUNCH("k_encode");
+  return FSG_OK;
+}
+
+extern "C" int fsg_block_mean(const fsg_window* src, int32_t factor, float* out,
+                              int64_t ld_out, void* stream) {
+  int rc = validate_window(src, "fsg_block_mean");
+  if (rc) return rc;
+  if (factor < 1) return fail(FSG_E_INVALID, "fsg_block_mean: factor < 1");
+  const int oh = std::max(1, (src->h + factor - 1) / factor);
+  const int ow = std::max(1, (src->w + factor - 1) / factor);
+  if (src->h == 0 || src->w == 0) return fail(FSG_E_INVALID, "fsg_block_mean: empty source");
+  if (!out || ld_out < ow) return fail(FSG_E_INVALID, "fsg_block_mean: bad output");
+  if (factor <= 64 && (factor & (factor - 1)) == 0) {
+    dim3 blk(256), grd((src->w + 255) / 256, oh);
+    hipLaunchKernelGGL(k_block_mean_p2, grd, blk, 0, as_stream(stream), src->data, src->ld,
+                       src->h, src->w, factor, out, ld_out, oh, ow);
+    FSG_CHECK_LAUNCH("k_block_mean_p2");
+    return FSG_OK;
+  }
+  dim3 blk(128), grd((ow + 127) / 128, oh);
+  hipLaunchKernelGGL(k_block_mean, grd, blk, 0, as_stream(stream), src->data, src->ld, src->h,
+                     src->w, factor, out, ld_out, oh, ow);
+  FSG_CHECK_LAUNCH("k_block_mean");
+  return FSG_OK;
+}

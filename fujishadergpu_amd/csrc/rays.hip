@@ -1,0 +1,280 @@
+// Ray marches: sparse Yokoyama openness and stylised ambient occlusion.
+//
+// Openness (compute_openness_vectorized, _impl_openness.py:31-132): per
+// direction the horizon extreme of atan((z_s - z) / d) over the valid
+// in-window samples.  atan is monotone, so the kernel takes the extreme of the
+// slope ratio and evaluates one atanf per direction instead of one per sample.
+// The ratio uses f32(1/phys) multiplication (within 1.5 ulp of the reference's
+// division by f32(phys)).
+//
+// Ambient occlusion (compute_ambient_occlusion_block,
+// _impl_ambient_occlusion.py:32-121): edge-padded samples, clamp(atan,0,pi/4),
+// ring weighting, mean over valid samples, then a sigma=1 'nearest' Gaussian
+// (scipy two-pass order) of where(nan, 1, ao), gamma, NaN restored.
+#include <algorithm>
+#include <cmath>
+
+#include "fsg_common.hpp"
+
+namespace fsg {
+
+struct WinDevR {
+  const float* p;
+  int64_t ld;
+  int h, w;
+};
+
+constexpr int OPEN_MAX = 320;  // n_dirs * n_dist entries passed by value
+
+struct OpenTab {
+  int n_dirs, n_dist;
+  int negative;
+  float half_pi;   // f32(pi/2)
+  float gamma;     // f32(1/2.2)
+  short2 off[OPEN_MAX];
+  float inv[OPEN_MAX];
+};
+
+__global__ void __launch_bounds__(256)
+k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (c >= rw || r >= rh) return;
+  const int i = r0 + r, j = c0 + c;
+  const PostK pp = load_post(out);
+  const float z = W.p[(int64_t)i * W.ld + j];
+  if (isnan(z)) {
+    store_out(out, r, c, apply_post(pp, __builtin_nanf("")));
+    return;
+  }
+  float angle_sum = 0.0f, dir_count = 0.0f;
+  for (int d = 0; d < T.n_dirs; d++) {
+    float ext = T.negative ? INFINITY : -INFINITY;
+    bool any = false;
+    for (int k = 0; k < T.n_dist; k++) {
+      const int e = d * T.n_dist + k;
+      const short2 o = T.off[e];
+      if (o.x == 0 && o.y == 0) continue;
+      const int si = i + o.y, sj = j + o.x;
+      if (si < 0 || si >= W.h || sj < 0 || sj >= W.w) continue;
+      const float s = W.p[(int64_t)si * W.ld + sj];
+      if (isnan(s)) continue;
+      const float q = (s - z) * T.inv[e];
+      ext = T.negative ? fminf(ext, q) : fmaxf(ext, q);
+      any = true;
+    }
+    if (any) {
+      const float a = atanf(ext);
+      const float da = T.negative ? (T.half_pi + a) : (T.half_pi - a);
+      angle_sum = angle_sum + da;
+      dir_count = dir_count + 1.0f;
+    }
+  }
+  float op = angle_sum / fmaxf(dir_count, 1.0f);
+  op = op / T.half_pi;
+  op = fminf(fmaxf(op, 0.0f), 1.0f);
+  const float res = powf(op, T.gamma);
+  store_out(out, r, c, apply_post(pp, res));
+}
+
+constexpr int AO_MAX = 256;
+
+struct AoTab {
+  int n;
+  float intensity;  // f32(intensity)
+  float quarter_pi; // f32(pi/4)
+  float gamma;
+  short2 off[AO_MAX];
+  float inv[AO_MAX];
+  float df[AO_MAX];
+};
+
+// Pass 1: AO before the Gaussian on the expanded region, stored as
+// where(isnan(z), 1, ao) -- the array the reference smooths.
+__global__ void __launch_bounds__(256)
+k_ao_raw(WinDevR W, int er0, int ec0, int eh, int ew, AoTab T, float* __restrict__ dst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (c >= ew || r >= eh) return;
+  const int i = er0 + r, j = ec0 + c;
+  const float z = W.p[(int64_t)i * W.ld + j];
+  float res;
+  if (isnan(z)) {
+    res = 1.0f;
+  } else {
+    float total = 0.0f, count = 0.0f;
+    for (int e = 0; e < T.n; e++) {
+      const short2 o = T.off[e];
+      if (o.x == 0 && o.y == 0) continue;
+      const int si = clampi(i + o.y, W.h), sj = clampi(j + o.x, W.w);
+      const float s = W.p[(int64_t)si * W.ld + sj];
+      if (isnan(s)) continue;
+      const float a = atanf((s - z) * T.inv[e]);
+      float occ = fmaxf(0.0f, a) / T.quarter_pi;
+      occ = fminf(occ, 1.0f);
+      total = total + occ * T.df[e];
+      count = count + 1.0f;
+    }
+    count = fmaxf(count, 1.0f);
+    const float mean = total / count;
+    float ao = 1.0f - mean * T.intensity;
+    res = fminf(fmaxf(ao, 0.0f), 1.0f);
+  }
+  dst[(int64_t)r * ew + c] = res;
+}
+
+struct G9 {
+  double g[5];
+};
+
+// Pass 2: sigma=1 Gaussian ('nearest' at the window edge), gamma, NaN restore.
+__global__ void __launch_bounds__(256)
+k_ao_smooth(WinDevR W, int r0, int c0, int rh, int rw, int er0, int ec0, int eh, int ew,
+            const float* __restrict__ src, G9 G, float gamma, OutDev out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (c >= rw || r >= rh) return;
+  const int i = r0 + r, j = c0 + c;
+  const PostK pp = load_post(out);
+  const float z = W.p[(int64_t)i * W.ld + j];
+  if (isnan(z)) {
+    store_out(out, r, c, apply_post(pp, __builtin_nanf("")));
+    return;
+  }
+  float v[9];
+  for (int k = 0; k < 9; k++) {
+    const int jj = clampi(j + k - 4, W.w) - ec0;
+    const float* colp = src + jj;
+    double s = (double)colp[(int64_t)(i - er0) * ew] * G.g[0];
+    for (int d = 4; d >= 1; d--) {
+      const int ia = clampi(i - d, W.h) - er0, ib = clampi(i + d, W.h) - er0;
+      s += ((double)colp[(int64_t)ia * ew] + (double)colp[(int64_t)ib * ew]) * G.g[d];
+    }
+    v[k] = (float)s;
+  }
+  double s = (double)v[4] * G.g[0];
+  for (int d = 4; d >= 1; d--) s += ((double)v[4 - d] + (double)v[4 + d]) * G.g[d];
+  const float ao = (float)s;
+  store_out(out, r, c, apply_post(pp, powf(ao, gamma)));
+}
+
+static void expanded(const fsg_window* win, const fsg_region* reg, int* er0, int* ec0, int* eh,
+                     int* ew) {
+  *er0 = std::max(0, reg->r0 - 4);
+  *ec0 = std::max(0, reg->c0 - 4);
+  const int er1 = std::min(win->h, reg->r0 + reg->h + 4);
+  const int ec1 = std::min(win->w, reg->c0 + reg->w + 4);
+  *eh = er1 - *er0;
+  *ew = ec1 - *ec0;
+}
+
+}  // namespace fsg
+
+using namespace fsg;
+
+extern "C" int fsg_openness(const fsg_window* win, const fsg_region* reg,
+                            const fsg_openness_params* p, const fsg_output* out, void* stream) {
+  int rc = validate_window(win, "fsg_openness");
+  if (rc) return rc;
+  rc = validate_region(win, reg, "fsg_openness");
+  if (rc) return rc;
+  if (!p || !out) return fail(FSG_E_INVALID, "fsg_openness: params/output NULL");
+  if ((rc = validate_output(out, reg, "fsg_openness"))) return rc;
+  if (p->n_dirs < 0 || p->n_dist < 0 || (p->n_dirs * p->n_dist > 0 && !p->samples))
+    return fail(FSG_E_INVALID, "fsg_openness: bad ray table");
+  if (p->n_dirs * p->n_dist > OPEN_MAX)
+    return fail(FSG_E_UNSUPPORTED, "fsg_openness: ray table larger than 320 samples");
+  if (reg->h == 0 || reg->w == 0) return FSG_OK;
+  OpenTab T{};
+  T.n_dirs = p->n_dirs;
+  T.n_dist = p->n_dist;
+  T.negative = p->openness_type != 0;
+  T.half_pi = (float)(M_PI / 2);
+  T.gamma = (float)(1.0 / 2.2);
+  for (int e = 0; e < p->n_dirs * p->n_dist; e++) {
+    const fsg_ray_sample& s = p->samples[e];
+    if (s.skip || (s.dx == 0 && s.dy == 0)) {
+      T.off[e] = make_short2(0, 0);
+      T.inv[e] = 0.f;
+      continue;
+    }
+    if (std::abs(s.dx) > 32767 || std::abs(s.dy) > 32767)
+      return fail(FSG_E_UNSUPPORTED, "fsg_openness: offset beyond int16");
+    T.off[e] = make_short2((short)s.dx, (short)s.dy);
+    T.inv[e] = 1.0f / (float)s.phys;
+  }
+  WinDevR W{win->data, win->ld, win->h, win->w};
+  dim3 blk(64, 4), grd((reg->w + 63) / 64, (reg->h + 3) / 4);
+  TimedScope ts("k_openness", as_stream(stream));
+  hipLaunchKernelGGL(k_openness, grd, blk, 0, as_stream(stream), W, reg->r0, reg->c0, reg->h,
+                     reg->w, T, make_out(out));
+  FSG_CHECK_LAUNCH("k_openness");
+  return FSG_OK;
+}
+
+extern "C" size_t fsg_ambient_occlusion_workspace_size(const fsg_window* win,
+                                                       const fsg_region* reg) {
+  if (!win || !reg) return 0;
+  int er0, ec0, eh, ew;
+  expanded(win, reg, &er0, &ec0, &eh, &ew);
+  if (eh <= 0 || ew <= 0) return 0;
+  return sizeof(float) * (size_t)eh * (size_t)ew;
+}
+
+extern "C" int fsg_ambient_occlusion(const fsg_window* win, const fsg_region* reg,
+                                     const fsg_ao_params* p, const fsg_output* out,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = validate_window(win, "fsg_ambient_occlusion");
+  if (rc) return rc;
+  rc = validate_region(win, reg, "fsg_ambient_occlusion");
+  if (rc) return rc;
+  if (!p || !out) return fail(FSG_E_INVALID, "fsg_ambient_occlusion: params/output NULL");
+  if ((rc = validate_output(out, reg, "fsg_ambient_occlusion"))) return rc;
+  if (p->n_samples < 0 || (p->n_samples > 0 && !p->samples))
+    return fail(FSG_E_INVALID, "fsg_ambient_occlusion: bad sample table");
+  if (p->n_samples > AO_MAX)
+    return fail(FSG_E_UNSUPPORTED, "fsg_ambient_occlusion: more than 256 samples");
+  if (reg->h == 0 || reg->w == 0) return FSG_OK;
+  const size_t need = fsg_ambient_occlusion_workspace_size(win, reg);
+  if (!workspace || workspace_bytes < need)
+    return fail(FSG_E_WORKSPACE, "fsg_ambient_occlusion: workspace too small");
+  AoTab T{};
+  T.n = p->n_samples;
+  T.intensity = (float)p->intensity;
+  T.quarter_pi = (float)(M_PI / 4);
+  T.gamma = (float)(1.0 / 2.2);
+  for (int e = 0; e < p->n_samples; e++) {
+    const fsg_ao_sample& s = p->samples[e];
+    if (s.skip || (s.dx == 0 && s.dy == 0)) {
+      T.off[e] = make_short2(0, 0);
+      continue;
+    }
+    if (std::abs(s.dx) > 32767 || std::abs(s.dy) > 32767)
+      return fail(FSG_E_UNSUPPORTED, "fsg_ambient_occlusion: offset beyond int16");
+    T.off[e] = make_short2((short)s.dx, (short)s.dy);
+    T.inv[e] = 1.0f / (float)s.phys;
+    T.df[e] = (float)s.dist_factor;
+  }
+  int er0, ec0, eh, ew;
+  expanded(win, reg, &er0, &ec0, &eh, &ew);
+  WinDevR W{win->data, win->ld, win->h, win->w};
+  hipStream_t s = as_stream(stream);
+  float* raw = (float*)workspace;
+  {
+    dim3 blk(64, 4), grd((ew + 63) / 64, (eh + 3) / 4);
+    TimedScope ts("k_ao_raw", s);
+    hipLaunchKernelGGL(k_ao_raw, grd, blk, 0, s, W, er0, ec0, eh, ew, T, raw);
+    FSG_CHECK_LAUNCH("k_ao_raw");
+  }
+  G9 G;
+  double w9[9];
+  gaussian_weights(1.0, 4, w9);
+  for (int d = 0; d <= 4; d++) G.g[d] = w9[4 + d];
+  {
+    dim3 blk(64, 4), grd((reg->w + 63) / 64, (reg->h + 3) / 4);
+    hipLaunchKernelGGL(k_ao_smooth, grd, blk, 0, s, W, reg->r0, reg->c0, reg->h, reg->w, er0,
+                       ec0, eh, ew, (const float*)raw, G, T.gamma, make_out(out));
+    FSG_CHECK_LAUNCH("k_ao_smooth");
+  }
+  return FSG_OK;
+}

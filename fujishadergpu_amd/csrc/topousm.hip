@@ -1,0 +1,962 @@
+// TopoUSM Fast: multiscale box-mean unsharp mask.
+//
+// Per window (the reference's tile window), for radii r_i with weights w_i
+// (compute_topousm_fast_efficient_block, _impl_topousm_fast.py:49-100):
+//   ds_i == 1, r_i >= 2 : NaN-aware box mean of size 2r+1, mode 'reflect'
+//   ds_i == 1, r_i <= 1 : NaN-aware Gaussian sigma=1, mode 'nearest'
+//   ds_i  > 1           : NaN-aware ds x ds block mean (+ enclosed void fill),
+//                         box (or sigma=1 Gaussian) on that grid, then a
+//                         corner-aligned bilinear zoom back to the window
+//   sum_i w_i * (z - mean_i), NaN restored, + (w_large * z - bilinear(field))
+//   for the large-radius overview field, then / f32(p99 scale).
+//
+// Box means follow scipy.ndimage.uniform_filter's structure: axis-0 means
+// rounded to float32, then axis-1 means of those, with exact (float64)
+// window sums; NaN-aware means are where(sw > 0, sv / sw, 0) in float32.
+//
+// Main kernel layout (k_usm_main): one workgroup of 256 threads owns a strip
+// of SW output columns x SH output rows.  Threads first walk down their column
+// (SW + 2R columns, R = largest full-resolution box radius) keeping running
+// float64 window sums per radius; every K rows the axis-0 means land in LDS,
+// then 256 threads re-partition as K rows x S segments and slide running sums
+// along each row segment.  Results are staged in LDS and written coalesced.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fsg_common.hpp"
+
+namespace fsg {
+
+struct WinDev {
+  const float* p;
+  int64_t ld;
+  int h, w;
+};
+
+struct Gauss9 {
+  double g[5];
+};
+
+Gauss9 make_gauss9_t() {
+  double w[9];
+  gaussian_weights(1.0, 4, w);
+  Gauss9 G;
+  for (int d = 0; d <= 4; d++) G.g[d] = w[4 + d];
+  return G;
+}
+
+__device__ __forceinline__ float rawv(const float* p, int64_t ld, int r, int c) {
+  return p[(int64_t)r * ld + c];
+}
+
+// NaN-aware sigma=1 Gaussian mean at (r, c) of an h x w grid, scipy order,
+// mode 'nearest': where(sw > 0, sv / sw, 0)  (_nan_utils.py:18-31).  Without
+// NaN this equals gaussian_filter(block) (sw rounds to 1.0f).
+__device__ __noinline__ float gauss9_mean(const float* p, int64_t ld, int h, int w, int r, int c,
+                                          Gauss9 G) {
+  float vsv[9], vsw[9];
+  for (int k = 0; k < 9; k++) {
+    const int cc = clampi(c + k - 4, w);
+    const float x0 = rawv(p, ld, r, cc);
+    const bool v0 = !isnan(x0);
+    double sv = (v0 ? (double)x0 : 0.0) * G.g[0];
+    double sw = (v0 ? 1.0 : 0.0) * G.g[0];
+    for (int d = 4; d >= 1; d--) {
+      const float a = rawv(p, ld, clampi(r - d, h), cc);
+      const float b = rawv(p, ld, clampi(r + d, h), cc);
+      const double fa = isnan(a) ? 0.0 : (double)a, fb = isnan(b) ? 0.0 : (double)b;
+      const double wa = isnan(a) ? 0.0 : 1.0, wb = isnan(b) ? 0.0 : 1.0;
+      sv += (fa + fb) * G.g[d];
+      sw += (wa + wb) * G.g[d];
+    }
+    vsv[k] = (float)sv;
+    vsw[k] = (float)sw;
+  }
+  double sv = (double)vsv[4] * G.g[0];
+  double sw = (double)vsw[4] * G.g[0];
+  for (int d = 4; d >= 1; d--) {
+    sv += ((double)vsv[4 - d] + (double)vsv[4 + d]) * G.g[d];
+    sw += ((double)vsw[4 - d] + (double)vsw[4 + d]) * G.g[d];
+  }
+  const float fsv = (float)sv, fsw = (float)sw;
+  return fsw > 0.0f ? fsv / fsw : 0.0f;
+}
+
+// ------------------------------------------------------------ coarse kernels
+
+// _downsample_nan_aware cell means + "any NaN cell" flag.  One block covers
+// 256 consecutive source columns of one coarse row (f source rows): each lane
+// sums its column, then groups of f lanes reduce by shuffles (f is a power of
+// two <= 16).  Cell sums are exact in float64, so the order does not matter.
+__global__ void __launch_bounds__(256)
+k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __restrict__ nan_flag) {
+  const int orow = blockIdx.y;
+  const int c = blockIdx.x * 256 + threadIdx.x;  // source column
+  const int r0 = orow * f, r1 = min(r0 + f, W.h);
+  double tot = 0.0;
+  int cnt = 0;
+  if (c < W.w) {
+    for (int r = r0; r < r1; r++) {
+      const float v = W.p[(int64_t)r * W.ld + c];
+      if (isfinite(v)) {
+        tot += (double)v;
+        cnt++;
+      }
+    }
+  }
+  for (int off = 1; off < f; off <<= 1) {
+    tot += __shfl_down(tot, off, 64);
+    cnt += __shfl_down(cnt, off, 64);
+  }
+  if ((threadIdx.x & (f - 1)) == 0) {
+    const int oc = c / f;
+    if (oc < ow && orow < oh) {
+      float res;
+      if (cnt > 0) {
+        res = (float)tot / (float)cnt;
+      } else {
+        res = __builtin_nanf("");
+        atomicOr(nan_flag, 1);
+      }
+      out[(int64_t)orow * ow + oc] = res;
+    }
+  }
+}
+
+// Separable Gaussian pass of the enclosed-void fill, mode 'nearest', scipy
+// symmetric order.  Pass 0: axis 0 of (where(nan,0,x), valid) -> (tv, tw);
+// pass 1: axis 1 of (tv, tw) -> (sv, sw).  Gated on the NaN flag.
+constexpr int VF_MAX_RAD = 320;
+struct VFWeights {
+  int rad;
+  int by_value;    // 1: g[] holds the scipy weights; 0: evaluate on device
+  double sigma2, inv_sum;
+  double g[VF_MAX_RAD + 1];
+};
+
+__device__ __forceinline__ double vf_w(const VFWeights& V, int d) {
+  return V.by_value ? V.g[d] : exp(-0.5 / V.sigma2 * (double)(d * d)) * V.inv_sum;
+}
+
+__global__ void k_voidfill_pass(const float* __restrict__ a, const float* __restrict__ b,
+                                float* __restrict__ oa, float* __restrict__ ob, int h, int w,
+                                VFWeights V, int axis, const int* __restrict__ nan_flag) {
+  if (*nan_flag == 0) return;
+  const int rad = V.rad;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)h * w;
+       e += (int64_t)gridDim.x * blockDim.x) {
+  const int r = (int)(e / w), c = (int)(e - (int64_t)r * w);
+  double sv, sw;
+  if (axis == 0) {
+    // a = coarse (NaN -> value 0, weight 0)
+    const float x0 = a[(int64_t)r * w + c];
+    sv = (isnan(x0) ? 0.0 : (double)x0) * vf_w(V, 0);
+    sw = (isnan(x0) ? 0.0 : 1.0) * vf_w(V, 0);
+    for (int d = rad; d >= 1; d--) {
+      const float xa = a[(int64_t)clampi(r - d, h) * w + c];
+      const float xb = a[(int64_t)clampi(r + d, h) * w + c];
+      sv += ((isnan(xa) ? 0.0 : (double)xa) + (isnan(xb) ? 0.0 : (double)xb)) * vf_w(V, d);
+      sw += ((isnan(xa) ? 0.0 : 1.0) + (isnan(xb) ? 0.0 : 1.0)) * vf_w(V, d);
+    }
+  } else {
+    const int64_t row = (int64_t)r * w;
+    sv = (double)a[row + c] * vf_w(V, 0);
+    sw = (double)b[row + c] * vf_w(V, 0);
+    for (int d = rad; d >= 1; d--) {
+      sv += ((double)a[row + clampi(c - d, w)] + (double)a[row + clampi(c + d, w)]) * vf_w(V, d);
+      sw += ((double)b[row + clampi(c - d, w)] + (double)b[row + clampi(c + d, w)]) * vf_w(V, d);
+    }
+  }
+  oa[(int64_t)r * w + c] = (float)sv;
+  ob[(int64_t)r * w + c] = (float)sw;
+  }
+}
+
+__global__ void k_zero_int(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// coarse = where(isnan & sw > 0.5, sv / max(sw, 1e-6), coarse)
+__global__ void k_voidfill_apply(float* __restrict__ coarse, const float* __restrict__ sv,
+                                 const float* __restrict__ sw, int n,
+                                 const int* __restrict__ nan_flag) {
+  if (*nan_flag == 0) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float x = coarse[i];
+    if (isnan(x) && sw[i] > 0.5f) coarse[i] = sv[i] / fmaxf(sw[i], 1e-6f);
+  }
+}
+
+// NaN-aware box mean on a small grid as two transposing axis-0 passes.  A
+// 64-lane block owns 64 columns x 64 rows: each lane keeps a running float64
+// window sum down its column, the 64x64 results go through LDS and are written
+// transposed, so both passes read and write coalesced rows.
+//   pass 0: x (h x w, NaN-aware)  -> tv^T, tw^T (w x h): float32 axis-0 means
+//   pass 1: tv^T, tw^T (w x h)    -> out (h x w): axis-1 means of those and
+//           where(sw > 0, sv / sw, 0)   (_nan_utils.py:34-47 structure)
+__global__ void __launch_bounds__(64)
+k_box_t(const float* __restrict__ a, const float* __restrict__ b, int64_t lda, int h, int w,
+        int r, double inv, float* __restrict__ oa, float* __restrict__ ob, int64_t ldo,
+        int pass) {
+  __shared__ float ta[64][65];
+  __shared__ float tb[64][65];
+  const int lane = threadIdx.x;
+  const int c = blockIdx.x * 64 + lane;  // column of the (h x w) input
+  const int y0 = blockIdx.y * 64;
+  const int y1 = min(y0 + 64, h);
+  if (c < w) {
+    const float* pa = a + c;
+    const float* pb = b ? b + c : nullptr;
+    double s = 0.0, sw = 0.0;
+    for (int d = -r; d <= r; d++) {
+      const int64_t o = (int64_t)refl(y0 + d, h) * lda;
+      if (pass == 0) {
+        const float v = pa[o];
+        if (!isnan(v)) { s += (double)v; sw += 1.0; }
+      } else {
+        s += (double)pa[o];
+        sw += (double)pb[o];
+      }
+    }
+    for (int y = y0; y < y1; y++) {
+      if (y > y0) {
+        const int64_t on = (int64_t)refl(y + r, h) * lda, oo = (int64_t)refl(y - r - 1, h) * lda;
+        if (pass == 0) {
+          const float vn = pa[on], vo = pa[oo];
+          if (!isnan(vn)) { s += (double)vn; sw += 1.0; }
+          if (!isnan(vo)) { s -= (double)vo; sw -= 1.0; }
+        } else {
+          s += (double)pa[on] - (double)pa[oo];
+          sw += (double)pb[on] - (double)pb[oo];
+        }
+      }
+      const float fv = (float)(s * inv), fw = (float)(sw * inv);
+      if (pass == 0) {
+        ta[y - y0][lane] = fv;
+        tb[y - y0][lane] = fw;
+      } else {
+        ta[y - y0][lane] = fw > 0.0f ? fv / fw : 0.0f;
+      }
+    }
+  }
+  __syncthreads();
+  // transposed write: output row = input column, output column = input row
+  const int ncol = min(64, w - blockIdx.x * 64);
+  for (int cc = 0; cc < ncol; cc++) {
+    const int y = y0 + lane;
+    if (y < y1) {
+      const int64_t o = (int64_t)(blockIdx.x * 64 + cc) * ldo + y;
+      oa[o] = ta[lane][cc];
+      if (pass == 0) ob[o] = tb[lane][cc];
+    }
+  }
+}
+
+__global__ void k_gauss9_grid(const float* __restrict__ x, int64_t ldx, int h, int w,
+                              float* __restrict__ out, int64_t ldo, Gauss9 G) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (c >= w || r >= h) return;
+  out[(int64_t)r * ldo + c] = gauss9_mean(x, ldx, h, w, r, c, G);
+}
+
+// field = (first ? 0 : field) + f32(w) * mean, in radius order
+__global__ void k_field_accum(float* __restrict__ field, int64_t ldf, const float* __restrict__ m,
+                              int64_t ldm, int h, int w, float wt, int first) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (c >= w || r >= h) return;
+  const float t = wt * m[(int64_t)r * ldm + c];
+  float* p = field + (int64_t)r * ldf + c;
+  *p = first ? t : (*p + t);
+}
+
+// ------------------------------------------------------------ main kernel
+
+enum { TERM_BOX = 0, TERM_GAUSS = 1, TERM_COARSE = 2 };
+constexpr int MAX_BOX = 6;
+constexpr int MAX_SRC = 5;  // bilinear sources staged in LDS (coarse means + field)
+
+// A bilinear source: a coarse mean grid sampled with the corner-aligned zoom
+// (mode 0, window coordinates, float64 coordinates), or the overview field
+// sampled at raster coordinates with float32 coordinates (mode 1), both with
+// 'nearest' clamping (_nan_utils.py:255-281, :671-698).
+struct SrcRef {
+  const float* p;
+  int64_t ld;
+  int h, w;
+  int mode;
+  int oy, ox;           // raster origin of the window (mode 1)
+  double zy, zx;        // zoom factors (mode 0)
+  float fy, fx;         // f32 ratios (mode 1)
+  int rows_max, cols_max, lds_off;
+};
+
+struct UsmArgs {
+  WinDev W;
+  int r0, c0, rh, rw;  // region in window coordinates
+  int n_terms;
+  int kind[FSG_MAX_RADII];
+  int slot[FSG_MAX_RADII];  // box slot, or source index for coarse terms
+  float wt[FSG_MAX_RADII];
+  int nr;                   // number of full-resolution box radii
+  int rad[MAX_BOX];
+  double inv[MAX_BOX];      // 1 / (2r + 1)
+  int R;                    // largest box radius (strip halo)
+  int SW, SH;               // strip width, rows per workgroup
+  int n_strips;
+  int n_src;
+  SrcRef src[MAX_SRC];
+  int field_src;            // source index of the overview field, or -1
+  float wl;                 // f32(w_large)
+  int src_floats;           // LDS floats of all staging tiles
+  Gauss9 G;
+  OutDev o1, o2;
+};
+
+__device__ __forceinline__ int reflf(int i, int n) {
+  return (i >= 0 && i < n) ? i : refl(i, n);
+}
+
+__host__ __device__ inline void src_axis(const SrcRef& S, int row, int idx, int& i0, int& i1,
+                                         double& t) {
+  const int n = row ? S.h : S.w;
+  double c;
+  if (S.mode == 0) {
+    c = (double)idx * (row ? S.zy : S.zx);
+  } else {
+    const float f = ((float)(idx + (row ? S.oy : S.ox)) + 0.5f) * (row ? S.fy : S.fx) - 0.5f;
+    c = (double)f;
+  }
+  c = fmin(fmax(c, 0.0), (double)(n - 1));
+  i0 = (int)floor(c);
+  i1 = (i0 + 1 < n - 1) ? i0 + 1 : n - 1;
+  t = c - (double)i0;
+}
+
+// bilinear value from a staged tile (rows from r_lo, cols from c_lo)
+__device__ __forceinline__ float src_sample(const SrcRef& S, const float* tile, int r_lo, int c_lo,
+                                            int yw, int xw) {
+  int a0, a1, b0, b1;
+  double ty, tx;
+  src_axis(S, 1, yw, a0, a1, ty);
+  src_axis(S, 0, xw, b0, b1, tx);
+  a0 -= r_lo; a1 -= r_lo; b0 -= c_lo; b1 -= c_lo;
+  const double a = tile[a0 * S.cols_max + b0], b = tile[a0 * S.cols_max + b1];
+  const double c = tile[a1 * S.cols_max + b0], d = tile[a1 * S.cols_max + b1];
+  const double top = a * (1.0 - tx) + b * tx;
+  const double bot = c * (1.0 - tx) + d * tx;
+  return (float)(top * (1.0 - ty) + bot * ty);
+}
+
+template <int NR, bool NANAW, int K, bool GAUSS>
+__global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
+  constexpr int KS = 8;           // rows per batch of column loads
+  constexpr int NRA = NR > 0 ? NR : 1;
+  extern __shared__ float lds[];
+  const int NC = A.SW + 2 * A.R;
+  const int LDC = NC + 1;         // padded LDS row stride
+  float* Vs = lds;                                   // [NR][K][LDC] axis-0 means
+  float* Ws = Vs + NR * K * LDC;                     // [NR][K][LDC] validity means
+  float* Zs = NANAW ? (Ws + NR * K * LDC) : Ws;      // [K][LDC] centre values / results
+  float* Src = Zs + K * LDC;                         // staging tiles of bilinear sources
+
+  __shared__ int s_kind[FSG_MAX_RADII], s_slot[FSG_MAX_RADII];
+  __shared__ float s_wt[FSG_MAX_RADII];
+  __shared__ SrcRef s_src[MAX_SRC];
+  __shared__ int s_clo[MAX_SRC], s_ncol[MAX_SRC];
+
+  const int t = threadIdx.x;
+  const int strip = blockIdx.x % A.n_strips;
+  const int seg = blockIdx.x / A.n_strips;
+  const int x0 = strip * A.SW;           // first output column (region coordinates)
+  const int y0 = seg * A.SH;             // first output row (region coordinates)
+  const int rows = min(A.SH, A.rh - y0);
+  const int cols = min(A.SW, A.rw - x0);
+  if (rows <= 0 || cols <= 0) return;
+  const int wx0 = A.c0 + x0, wy0 = A.r0 + y0;  // window coordinates
+
+  if (t < FSG_MAX_RADII) {
+    s_kind[t] = A.kind[t];
+    s_slot[t] = A.slot[t];
+    s_wt[t] = A.wt[t];
+  }
+  if (t < A.n_src) {
+    const SrcRef S = A.src[t];
+    s_src[t] = S;
+    int a0, a1, b0, b1;
+    double tt;
+    src_axis(S, 0, wx0, a0, a1, tt);
+    src_axis(S, 0, wx0 + cols - 1, b0, b1, tt);
+    s_clo[t] = a0;
+    s_ncol[t] = min(b1 - a0 + 1, S.cols_max);
+  }
+
+  // column owned in the vertical stage
+  const bool has_col = t < NC;
+  const int jw = refl(wx0 - A.R + t, A.W.w);
+  const float* colp = A.W.p + jw;
+  const int H = A.W.h;
+  const int64_t ld = A.W.ld;
+
+  // running window sums for the row before the first output row
+  double S[NRA];
+  int Cn[NRA];
+#pragma unroll
+  for (int i = 0; i < NR; i++) {
+    double s = 0.0;
+    int n = 0;
+    if (has_col) {
+      const int r = A.rad[i];
+      for (int d = -r; d <= r; d++) {
+        const float v = colp[(int64_t)reflf(wy0 - 1 + d, H) * ld];
+        if (NANAW) {
+          if (!isnan(v)) { s += (double)v; n++; }
+        } else {
+          s += (double)v;
+        }
+      }
+    }
+    S[i] = s;
+    Cn[i] = n;
+  }
+  __syncthreads();
+
+  const int n_chunks = (rows + K - 1) / K;
+  const int segs = 256 / K;
+  const int L = A.SW / segs;
+
+  for (int ch = 0; ch < n_chunks; ch++) {
+    const int wyc = wy0 + ch * K;
+    const int nrow = min(K, rows - ch * K);
+    // ---------------- axis-0 stage: batched column loads, running sums
+    if (has_col) {
+      for (int kb = 0; kb < K; kb += KS) {
+        float vz[KS], vn[NRA][KS], vo[NRA][KS];
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+          const int yw = wyc + kb + k;
+          vz[k] = colp[(int64_t)min(yw, H - 1) * ld];
+#pragma unroll
+          for (int i = 0; i < NR; i++) {
+            vn[i][k] = colp[(int64_t)reflf(yw + A.rad[i], H) * ld];
+            vo[i][k] = colp[(int64_t)reflf(yw - A.rad[i] - 1, H) * ld];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+#pragma unroll
+          for (int i = 0; i < NR; i++) {
+            if (NANAW) {
+              if (!isnan(vn[i][k])) { S[i] += (double)vn[i][k]; Cn[i]++; }
+              if (!isnan(vo[i][k])) { S[i] -= (double)vo[i][k]; Cn[i]--; }
+              Ws[(i * K + kb + k) * LDC + t] = (float)((double)Cn[i] * A.inv[i]);
+            } else {
+              S[i] += (double)vn[i][k];
+              S[i] -= (double)vo[i][k];
+            }
+            Vs[(i * K + kb + k) * LDC + t] = (float)(S[i] * A.inv[i]);
+          }
+          Zs[(kb + k) * LDC + t] = vz[k];
+        }
+      }
+    }
+    // ---------------- stage the bilinear sources for these rows
+    for (int s = 0; s < A.n_src; s++) {
+      const SrcRef& SR = s_src[s];
+      int a0, a1, b0, b1;
+      double tt;
+      src_axis(SR, 1, wyc, a0, a1, tt);
+      src_axis(SR, 1, wyc + nrow - 1, b0, b1, tt);
+      const int nr = min(b1 - a0 + 1, SR.rows_max);
+      const int nc = s_ncol[s], clo = s_clo[s];
+      float* tile = Src + SR.lds_off;
+      for (int e = t; e < nr * nc; e += 256) {
+        const int rr = e / nc, cc = e - rr * nc;
+        tile[rr * SR.cols_max + cc] = SR.p[(int64_t)(a0 + rr) * SR.ld + clo + cc];
+      }
+    }
+    __syncthreads();
+
+    // ---------------- axis-1 stage + combine
+    {
+      const int k = t / segs;
+      const int s = t % segs;
+      if (k < nrow) {
+        const int yw = wyc + k;
+        const int cbase = A.R + s * L;  // LDS column of this segment's first output
+        double Hs[NRA], HW[NRA];
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+          const float* vrow = Vs + (i * K + k) * LDC;
+          const float* wrow = Ws + (i * K + k) * LDC;
+          double h = 0.0, hw = 0.0;
+          for (int d = -A.rad[i]; d <= A.rad[i]; d++) {
+            h += (double)vrow[cbase + d];
+            if (NANAW) hw += (double)wrow[cbase + d];
+          }
+          Hs[i] = h;
+          HW[i] = hw;
+        }
+        // per-source tile origins for this chunk (recomputed, uniform per row)
+        for (int m = 0; m < L; m++) {
+          const int lc = cbase + m;
+          float bm[NRA];
+#pragma unroll
+          for (int i = 0; i < NR; i++) {
+            if (m > 0) {
+              const float* vrow = Vs + (i * K + k) * LDC;
+              const int r = A.rad[i];
+              Hs[i] += (double)vrow[lc + r] - (double)vrow[lc - r - 1];
+              if (NANAW) {
+                const float* wrow = Ws + (i * K + k) * LDC;
+                HW[i] += (double)wrow[lc + r] - (double)wrow[lc - r - 1];
+              }
+            }
+            const float fv = (float)(Hs[i] * A.inv[i]);
+            if (NANAW) {
+              const float fw = (float)(HW[i] * A.inv[i]);
+              bm[i] = fw > 0.0f ? fv / fw : 0.0f;
+            } else {
+              bm[i] = fv;
+            }
+          }
+          const int xr = x0 + s * L + m;  // region column
+          float res = 0.0f;
+          if (xr < A.rw) {
+            const int xw = A.c0 + xr;
+            const float z = Zs[k * LDC + lc];
+            float acc = 0.0f;
+#pragma unroll 1
+            for (int q = 0; q < A.n_terms; q++) {
+              float mean = 0.0f;
+              const int kd = s_kind[q];
+              if (kd == TERM_BOX) {
+                const int sl = s_slot[q];
+#pragma unroll
+                for (int i = 0; i < NR; i++)
+                  if (i == sl) mean = bm[i];
+              } else if (GAUSS && kd == TERM_GAUSS) {
+                mean = gauss9_mean(A.W.p, A.W.ld, A.W.h, A.W.w, yw, xw, A.G);
+              } else {
+                const int si = s_slot[q];
+                const SrcRef& SR = s_src[si];
+                int a0, a1;
+                double tt;
+                src_axis(SR, 1, wyc, a0, a1, tt);
+                mean = src_sample(SR, Src + SR.lds_off, a0, s_clo[si], yw, xw);
+              }
+              const float diff = s_wt[q] * (z - mean);
+              acc = (q == 0) ? diff : acc + diff;
+            }
+            if (isnan(z)) acc = __builtin_nanf("");
+            if (A.field_src >= 0) {
+              const SrcRef& SR = s_src[A.field_src];
+              int a0, a1;
+              double tt;
+              src_axis(SR, 1, wyc, a0, a1, tt);
+              const float up = src_sample(SR, Src + SR.lds_off, a0, s_clo[A.field_src], yw, xw);
+              const float large = A.wl * z - up;
+              acc = (A.n_terms > 0) ? acc + large : large;
+            }
+            res = acc;
+          }
+          // the thread's own Zs cell is read above only by this thread
+          Zs[k * LDC + lc] = res;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---------------- coalesced write of the chunk's results
+    const PostK p1 = load_post(A.o1), p2 = load_post(A.o2);
+    for (int idx = t; idx < nrow * A.SW; idx += 256) {
+      const int k = idx / A.SW, c = idx - k * A.SW;
+      if (c >= cols) continue;
+      const float v = Zs[k * LDC + A.R + c];
+      const int yr = y0 + ch * K + k, xr = x0 + c;
+      if (A.o1.data) store_out(A.o1, yr, xr, apply_post(p1, v));
+      if (A.o2.data) store_out(A.o2, yr, xr, apply_post(p2, v));
+    }
+    __syncthreads();
+  }
+}
+
+template <int NR, bool NANAW>
+static int launch_usm(const UsmArgs& A, int n_blocks, hipStream_t s) {
+  constexpr int K = (NR <= 3) ? 16 : 8;
+  const int LDC = A.SW + 2 * A.R + 1;
+  const size_t lds = sizeof(float) * ((size_t)NR * K * LDC * (NANAW ? 2 : 1) + (size_t)K * LDC +
+                                      (size_t)A.src_floats);
+  if (lds > 150 * 1024) return fail(FSG_E_UNSUPPORTED, "k_usm_main: LDS budget exceeded");
+  bool gauss = false;
+  for (int q = 0; q < A.n_terms; q++) gauss |= (A.kind[q] == TERM_GAUSS);
+  TimedScope ts("k_usm_main", s);
+  if (gauss)
+    hipLaunchKernelGGL((k_usm_main<NR, NANAW, K, true>), dim3(n_blocks), dim3(256), lds, s, A);
+  else
+    hipLaunchKernelGGL((k_usm_main<NR, NANAW, K, false>), dim3(n_blocks), dim3(256), lds, s, A);
+  FSG_CHECK_LAUNCH("k_usm_main");
+  return FSG_OK;
+}
+
+static int dispatch_usm(const UsmArgs& A, bool nanaw, int n_blocks, hipStream_t s) {
+#define FSG_USM_CASE(N)                                                   \
+  case N:                                                                 \
+    return nanaw ? launch_usm<N, true>(A, n_blocks, s) : launch_usm<N, false>(A, n_blocks, s);
+  switch (A.nr) {
+    FSG_USM_CASE(0)
+    FSG_USM_CASE(1)
+    FSG_USM_CASE(2)
+    FSG_USM_CASE(3)
+    FSG_USM_CASE(4)
+    FSG_USM_CASE(5)
+    FSG_USM_CASE(6)
+    default:
+      return fail(FSG_E_UNSUPPORTED, "fsg_topousm: more than 6 full-resolution box radii");
+  }
+#undef FSG_USM_CASE
+}
+
+// Exact staging-tile bounds: rows spanned by any K-row chunk of the region and
+// columns spanned by any SW-column strip (same arithmetic as the device).
+static void src_bounds(SrcRef& S, int r0, int rh, int c0, int rw, int K, int SW, int SH) {
+  int rmax = 1, cmax = 1, a0, a1, b0, b1;
+  double tt;
+  for (int y0 = 0; y0 < rh; y0 += SH) {
+    const int rows = std::min(SH, rh - y0);
+    for (int ch = 0; ch * K < rows; ch++) {
+      const int first = r0 + y0 + ch * K, last = first + std::min(K, rows - ch * K) - 1;
+      src_axis(S, 1, first, a0, a1, tt);
+      src_axis(S, 1, last, b0, b1, tt);
+      rmax = std::max(rmax, b1 - a0 + 1);
+    }
+  }
+  for (int x0 = 0; x0 < rw; x0 += SW) {
+    const int cols = std::min(SW, rw - x0);
+    src_axis(S, 0, c0 + x0, a0, a1, tt);
+    src_axis(S, 0, c0 + x0 + cols - 1, b0, b1, tt);
+    cmax = std::max(cmax, b1 - a0 + 1);
+  }
+  S.rows_max = rmax;
+  S.cols_max = cmax;
+}
+
+// ------------------------------------------------------------ host helpers
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct CoarsePlan {
+  int n_ds = 0;
+  int ds_val[FSG_MAX_RADII];
+  int ds_of_radius[FSG_MAX_RADII];  // index into ds_val, or -1
+  int hc[FSG_MAX_RADII], wc[FSG_MAX_RADII];
+  size_t grid_off[FSG_MAX_RADII];   // per distinct ds: decimated grid
+  size_t mean_off[FSG_MAX_RADII];   // per radius: coarse mean grid
+  size_t flag_off;
+  size_t tmp_off[4];
+  size_t total = 0;
+  int max_cells = 0;
+  int max_vrad = 0;
+};
+
+static int void_fill_radius(int hc, int wc, double* sigma_out) {
+  const double sigma = std::max(1.0, (double)std::min(hc, wc) / 64.0);
+  *sigma_out = sigma;
+  return gaussian_radius(sigma);
+}
+
+static CoarsePlan plan_coarse(const fsg_window* win, const fsg_topousm_params* p) {
+  CoarsePlan P;
+  size_t off = 0;
+  for (int i = 0; i < p->n_radii; i++) {
+    P.ds_of_radius[i] = -1;
+    const int ds = p->ds[i];
+    if (ds <= 1) continue;
+    int idx = -1;
+    for (int j = 0; j < P.n_ds; j++)
+      if (P.ds_val[j] == ds) idx = j;
+    if (idx < 0) {
+      idx = P.n_ds++;
+      P.ds_val[idx] = ds;
+      P.hc[idx] = std::max(1, (win->h + ds - 1) / ds);
+      P.wc[idx] = std::max(1, (win->w + ds - 1) / ds);
+      P.grid_off[idx] = off;
+      off += align256(sizeof(float) * (size_t)P.hc[idx] * P.wc[idx]);
+      P.max_cells = std::max(P.max_cells, P.hc[idx] * P.wc[idx]);
+      double sg;
+      P.max_vrad = std::max(P.max_vrad, void_fill_radius(P.hc[idx], P.wc[idx], &sg));
+    }
+    P.ds_of_radius[i] = idx;
+    P.mean_off[i] = off;
+    off += align256(sizeof(float) * (size_t)P.hc[idx] * P.wc[idx]);
+  }
+  P.flag_off = off;
+  off += align256(sizeof(int) * FSG_MAX_RADII);
+  for (int k = 0; k < 4; k++) {
+    P.tmp_off[k] = off;
+    off += align256(sizeof(float) * (size_t)P.max_cells);
+  }
+  P.total = off;
+  return P;
+}
+
+// Box (r >= 2) or sigma=1 Gaussian (r <= 1) NaN-aware mean of an h x w grid.
+static int coarse_mean(const float* x, int64_t ldx, int h, int w, int r, float* out, int64_t ldo,
+                       float* tv, float* tw, hipStream_t s) {
+  if (r <= 1) {
+    dim3 blk(128), grd((w + 127) / 128, h);
+    hipLaunchKernelGGL(k_gauss9_grid, grd, blk, 0, s, x, ldx, h, w, out, ldo, make_gauss9_t());
+    FSG_CHECK_LAUNCH("k_gauss9_grid");
+    return FSG_OK;
+  }
+  const double inv = 1.0 / (double)(2 * r + 1);
+  // pass 0: (h x w) -> transposed (w x h) axis-0 means
+  hipLaunchKernelGGL(k_box_t, dim3((w + 63) / 64, (h + 63) / 64), dim3(64), 0, s, x, nullptr,
+                     ldx, h, w, r, inv, tv, tw, (int64_t)h, 0);
+  FSG_CHECK_LAUNCH("k_box_t");
+  // pass 1: axis 0 of the transposed grid (= axis 1), transposed back
+  hipLaunchKernelGGL(k_box_t, dim3((h + 63) / 64, (w + 63) / 64), dim3(64), 0, s, tv, tw,
+                     (int64_t)h, w, h, r, inv, out, nullptr, ldo, 1);
+  FSG_CHECK_LAUNCH("k_box_t");
+  return FSG_OK;
+}
+
+// Decimated grid with the enclosed-void fill (_nan_utils.py:604-668).
+static int decimate(const fsg_window* win, int ds, float* grid, int hc, int wc, int* flag,
+                    float* t0, float* t1, float* t2, float* t3, bool nan_free, hipStream_t s) {
+  if (ds > 16 || (ds & (ds - 1)) != 0)
+    return fail(FSG_E_UNSUPPORTED, "fsg_topousm: downsample factor must be a power of two <= 16");
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, flag, 1);
+  FSG_CHECK_LAUNCH("k_zero_int");
+  WinDev W{win->data, win->ld, win->h, win->w};
+  {
+    dim3 blk(256), grd((win->w + 255) / 256, hc);
+    hipLaunchKernelGGL(k_decimate, grd, blk, 0, s, W, ds, grid, hc, wc, flag);
+    FSG_CHECK_LAUNCH("k_decimate");
+  }
+  // a NaN-free window has no all-NaN cell: the enclosed-void fill is a no-op
+  if (nan_free) return FSG_OK;
+  double sigma;
+  const int rad = void_fill_radius(hc, wc, &sigma);
+  VFWeights V{};
+  V.rad = rad;
+  V.sigma2 = sigma * sigma;
+  {
+    std::vector<double> full(2 * rad + 1);
+    gaussian_weights(sigma, rad, full.data());
+    if (rad <= VF_MAX_RAD) {
+      V.by_value = 1;
+      for (int d = 0; d <= rad; d++) V.g[d] = full[rad + d];
+    } else {
+      std::vector<double> phi(2 * rad + 1);
+      for (int i = 0; i <= 2 * rad; i++) {
+        const double x = (double)(i - rad);
+        phi[i] = std::exp(-0.5 / V.sigma2 * (x * x));
+      }
+      V.by_value = 0;
+      V.inv_sum = 1.0 / np_pairwise_sum(phi.data(), 2 * rad + 1);
+    }
+  }
+  const int n = hc * wc;
+  dim3 blk(256), grd(std::min(1024, (n + 255) / 256));
+  hipLaunchKernelGGL(k_voidfill_pass, grd, blk, 0, s, grid, grid, t0, t1, hc, wc, V, 0, flag);
+  FSG_CHECK_LAUNCH("k_voidfill_pass");
+  hipLaunchKernelGGL(k_voidfill_pass, grd, blk, 0, s, t0, t1, t2, t3, hc, wc, V, 1, flag);
+  FSG_CHECK_LAUNCH("k_voidfill_pass");
+  hipLaunchKernelGGL(k_voidfill_apply, grd, blk, 0, s, grid, t2, t3, n, flag);
+  FSG_CHECK_LAUNCH("k_voidfill_apply");
+  return FSG_OK;
+}
+
+}  // namespace fsg
+
+using namespace fsg;
+
+extern "C" size_t fsg_topousm_workspace_size(const fsg_window* win,
+                                             const fsg_topousm_params* p) {
+  if (!win || !p || p->n_radii < 0 || p->n_radii > FSG_MAX_RADII) return 0;
+  return plan_coarse(win, p).total;
+}
+
+extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
+                           const fsg_topousm_params* p, const fsg_output* out,
+                           const fsg_output* out2, void* workspace, size_t workspace_bytes,
+                           void* stream) {
+  int rc = validate_window(win, "fsg_topousm");
+  if (rc) return rc;
+  rc = validate_region(win, reg, "fsg_topousm");
+  if (rc) return rc;
+  if (!p || !out) return fail(FSG_E_INVALID, "fsg_topousm: params/output NULL");
+  if ((rc = validate_output(out, reg, "fsg_topousm"))) return rc;
+  if ((rc = validate_output(out2, reg, "fsg_topousm"))) return rc;
+  if (p->n_radii < 0 || p->n_radii > FSG_MAX_RADII)
+    return fail(FSG_E_INVALID, "fsg_topousm: n_radii out of range");
+  if (p->n_radii == 0 && !p->field)
+    return fail(FSG_E_INVALID, "fsg_topousm: no radii and no large field");
+  if (win->h == 0 || win->w == 0) return fail(FSG_E_INVALID, "fsg_topousm: empty window");
+  if (reg->h == 0 || reg->w == 0) return FSG_OK;
+  hipStream_t s = as_stream(stream);
+
+  const CoarsePlan P = plan_coarse(win, p);
+  if (P.total > 0 && (!workspace || workspace_bytes < P.total))
+    return fail(FSG_E_WORKSPACE, "fsg_topousm: workspace too small");
+  char* ws = (char*)workspace;
+
+  UsmArgs A{};
+  A.W = WinDev{win->data, win->ld, win->h, win->w};
+  A.r0 = reg->r0;
+  A.c0 = reg->c0;
+  A.rh = reg->h;
+  A.rw = reg->w;
+  A.n_terms = p->n_radii;
+  A.nr = 0;
+  A.R = 0;
+  for (int i = 0; i < p->n_radii; i++) {
+    const int r = p->radii[i];
+    A.wt[i] = p->weights[i];
+    if (p->ds[i] > 1) {
+      A.kind[i] = TERM_COARSE;
+      A.slot[i] = i;
+    } else if (r <= 1) {
+      A.kind[i] = TERM_GAUSS;
+      A.slot[i] = 0;
+    } else {
+      int slot = -1;
+      for (int j = 0; j < A.nr; j++)
+        if (A.rad[j] == r) slot = j;
+      if (slot < 0) {
+        if (A.nr >= MAX_BOX)
+          return fail(FSG_E_UNSUPPORTED, "fsg_topousm: more than 6 full-resolution box radii");
+        slot = A.nr++;
+        A.rad[slot] = r;
+        A.inv[slot] = 1.0 / (double)(2 * r + 1);
+        A.R = std::max(A.R, r);
+      }
+      A.kind[i] = TERM_BOX;
+      A.slot[i] = slot;
+    }
+  }
+  if (A.R > 96)
+    return fail(FSG_E_UNSUPPORTED,
+                "fsg_topousm: full-resolution box radius > 96 (the downsample rule caps it "
+                "near 41)");
+
+  // coarse grids for ds > 1 radii
+  int* flags = (int*)(ws + P.flag_off);
+  float* tmp[4];
+  for (int k = 0; k < 4; k++) tmp[k] = (float*)(ws + P.tmp_off[k]);
+  for (int j = 0; j < P.n_ds; j++) {
+    rc = decimate(win, P.ds_val[j], (float*)(ws + P.grid_off[j]), P.hc[j], P.wc[j], flags + j,
+                  tmp[0], tmp[1], tmp[2], tmp[3], p->nan_hint == 0, s);
+    if (rc) return rc;
+  }
+  // strip geometry (needed for the staging bounds)
+  const int K = (A.nr <= 3) ? 16 : 8;
+  int SW = ((256 - 2 * A.R) / 32) * 32;
+  if (SW < 32) return fail(FSG_E_UNSUPPORTED, "fsg_topousm: box radius too large");
+  A.SW = SW;
+  A.SH = 256;
+  A.n_strips = (reg->w + SW - 1) / SW;
+  A.n_src = 0;
+  A.field_src = -1;
+  int src_floats = 0;
+  for (int i = 0; i < p->n_radii; i++) {
+    const int j = P.ds_of_radius[i];
+    if (j < 0) continue;
+    const int ds = P.ds_val[j];
+    const int r_small = std::max(1, (int)std::nearbyint((double)p->radii[i] / (double)ds));
+    float* mean = (float*)(ws + P.mean_off[i]);
+    rc = coarse_mean((float*)(ws + P.grid_off[j]), P.wc[j], P.hc[j], P.wc[j], r_small, mean,
+                     P.wc[j], tmp[0], tmp[1], s);
+    if (rc) return rc;
+    // scipy zoom output shape round(hc * (H / hc)) must be the window shape
+    const double zoom_y = (double)win->h / (double)P.hc[j];
+    const double zoom_x = (double)win->w / (double)P.wc[j];
+    if ((int)std::nearbyint(P.hc[j] * zoom_y) != win->h ||
+        (int)std::nearbyint(P.wc[j] * zoom_x) != win->w)
+      return fail(FSG_E_UNSUPPORTED, "fsg_topousm: zoom output shape differs from window");
+    if (A.n_src >= MAX_SRC)
+      return fail(FSG_E_UNSUPPORTED, "fsg_topousm: more than 5 decimated radii");
+    SrcRef& S = A.src[A.n_src];
+    S = SrcRef{};
+    S.p = mean;
+    S.ld = P.wc[j];
+    S.h = P.hc[j];
+    S.w = P.wc[j];
+    S.mode = 0;
+    S.zy = (win->h != 1) ? (double)(P.hc[j] - 1) / (double)(win->h - 1) : 1.0;
+    S.zx = (win->w != 1) ? (double)(P.wc[j] - 1) / (double)(win->w - 1) : 1.0;
+    src_bounds(S, reg->r0, reg->h, reg->c0, reg->w, K, A.SW, A.SH);
+    S.lds_off = src_floats;
+    src_floats += S.rows_max * S.cols_max;
+    A.slot[i] = A.n_src++;
+  }
+  if (p->field) {
+    if (p->field_h <= 0 || p->field_w <= 0 || p->full_h <= 0 || p->full_w <= 0)
+      return fail(FSG_E_INVALID, "fsg_topousm: bad field geometry");
+    if (A.n_src >= MAX_SRC) return fail(FSG_E_UNSUPPORTED, "fsg_topousm: too many sources");
+    SrcRef& S = A.src[A.n_src];
+    S = SrcRef{};
+    S.p = p->field;
+    S.ld = p->field_w;
+    S.h = p->field_h;
+    S.w = p->field_w;
+    S.mode = 1;
+    S.oy = p->win_r0;
+    S.ox = p->win_c0;
+    S.fy = (float)((double)p->field_h / (double)p->full_h);
+    S.fx = (float)((double)p->field_w / (double)p->full_w);
+    src_bounds(S, reg->r0, reg->h, reg->c0, reg->w, K, A.SW, A.SH);
+    S.lds_off = src_floats;
+    src_floats += S.rows_max * S.cols_max;
+    A.field_src = A.n_src++;
+    A.wl = p->w_large;
+  }
+  A.src_floats = src_floats;
+  A.G = make_gauss9_t();
+  A.o1 = make_out(out);
+  A.o2 = out2 ? make_out(out2) : OutDev{};
+  const int n_segs = (reg->h + A.SH - 1) / A.SH;
+  // The NaN-aware variant is exact for every window; nan_hint == 0 (the caller
+  // knows the window holds no NaN) selects the cheaper plain-mean variant,
+  // whose results are identical on NaN-free windows.
+  return dispatch_usm(A, /*nanaw=*/p->nan_hint != 0, A.n_strips * n_segs, s);
+}
+
+extern "C" size_t fsg_topousm_field_workspace_size(int32_t h, int32_t w) {
+  if (h <= 0 || w <= 0) return 0;
+  return 3 * align256(sizeof(float) * (size_t)h * w);
+}
+
+extern "C" int fsg_topousm_coarse_field(const fsg_window* coarse, int32_t n_radii,
+                                        const int32_t* radii_coarse, const float* weights,
+                                        float* out, int64_t ld_out, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  int rc = validate_window(coarse, "fsg_topousm_coarse_field");
+  if (rc) return rc;
+  if (n_radii <= 0 || !radii_coarse || !weights || !out || ld_out < coarse->w)
+    return fail(FSG_E_INVALID, "fsg_topousm_coarse_field: bad arguments");
+  const int h = coarse->h, w = coarse->w;
+  if (h == 0 || w == 0) return fail(FSG_E_INVALID, "fsg_topousm_coarse_field: empty grid");
+  const size_t need = fsg_topousm_field_workspace_size(h, w);
+  if (!workspace || workspace_bytes < need)
+    return fail(FSG_E_WORKSPACE, "fsg_topousm_coarse_field: workspace too small");
+  char* ws = (char*)workspace;
+  float* tv = (float*)ws;
+  float* tw = (float*)(ws + align256(sizeof(float) * (size_t)h * w));
+  float* mean = (float*)(ws + 2 * align256(sizeof(float) * (size_t)h * w));
+  hipStream_t s = as_stream(stream);
+  for (int i = 0; i < n_radii; i++) {
+    const int r = std::max(1, (int)radii_coarse[i]);
+    rc = coarse_mean(coarse->data, coarse->ld, h, w, r, mean, w, tv, tw, s);
+    if (rc) return rc;
+    dim3 blk(128), grd((w + 127) / 128, h);
+    hipLaunchKernelGGL(k_field_accum, grd, blk, 0, s, out, ld_out, mean, (int64_t)w, h, w,
+                       weights[i], i == 0 ? 1 : 0);
+    FSG_CHECK_LAUNCH("k_field_accum");
+  }
+  return FSG_OK;
+}

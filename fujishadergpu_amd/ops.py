@@ -1,0 +1,300 @@
+"""Device-level wrappers over libfsg.so for torch CUDA (HIP) tensors.
+
+torch supplies device memory and the current stream; every computation is a
+libfsg kernel.  Windows are 2-D float32 tensors whose rows may be strided
+(a view into a larger raster), i.e. ``stride(1) == 1``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+_DTYPE_CODE = {"float32": N.OUT_F32, "int16": N.OUT_I16, "uint8": N.OUT_U8}
+_TORCH_DTYPE = {"float32": torch.float32, "int16": torch.int16, "uint8": torch.uint8}
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _check_window(t: torch.Tensor, name="window") -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor on the GPU, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must live on the GPU (got device {t.device})")
+    if t.dtype != torch.float32 or t.dim() != 2:
+        raise ValueError(f"{name} must be a 2-D float32 tensor, got {t.dtype} {tuple(t.shape)}")
+    if t.shape[0] > 0 and t.shape[1] > 1 and t.stride(1) != 1:
+        raise ValueError(f"{name} rows must be contiguous (stride(1) == 1)")
+
+
+def window(t: torch.Tensor) -> N.fsg_window:
+    _check_window(t)
+    ld = t.stride(0) if t.shape[0] > 1 else max(1, t.shape[1])
+    return N.fsg_window(t.data_ptr(), int(ld), int(t.shape[0]), int(t.shape[1]))
+
+
+def region(r0: int, c0: int, h: int, w: int) -> N.fsg_region:
+    return N.fsg_region(int(r0), int(c0), int(h), int(w))
+
+
+def full_region(t: torch.Tensor) -> N.fsg_region:
+    return region(0, 0, t.shape[0], t.shape[1])
+
+
+def output(t: torch.Tensor, *, qp: Optional[dict] = None, clip01: bool = False,
+           post: int = N.POST_NONE, host_stats: Sequence[float] = (0.0, 0.0),
+           dev_stats: Optional[torch.Tensor] = None) -> N.fsg_output:
+    """Output descriptor for a (region-shaped) tensor of float32/int16/uint8."""
+    if not t.is_cuda or t.dim() != 2:
+        raise ValueError("output must be a 2-D GPU tensor")
+    if t.shape[0] > 0 and t.shape[1] > 1 and t.stride(1) != 1:
+        raise ValueError("output rows must be contiguous")
+    code = {torch.float32: N.OUT_F32, torch.int16: N.OUT_I16, torch.uint8: N.OUT_U8}.get(t.dtype)
+    if code is None:
+        raise ValueError(f"unsupported output dtype {t.dtype}")
+    o = N.fsg_output()
+    o.data = t.data_ptr()
+    o.ld = int(t.stride(0) if t.shape[0] > 1 else max(1, t.shape[1]))
+    o.dtype = code
+    o.clip01 = int(bool(clip01))
+    if code != N.OUT_F32:
+        if qp is None:
+            raise ValueError("integer output needs quantisation params")
+        o.a_coef = float(np.float32(qp["a_coef"]))
+        o.b_coef = float(np.float32(qp["b_coef"]))
+        o.dn_min, o.dn_max = int(qp["dn_min"]), int(qp["dn_max"])
+    o.post_kind = int(post)
+    o.host_stats[0] = float(host_stats[0]) if len(host_stats) > 0 else 0.0
+    o.host_stats[1] = float(host_stats[1]) if len(host_stats) > 1 else 0.0
+    if dev_stats is not None:
+        if dev_stats.dtype != torch.float64 or not dev_stats.is_cuda:
+            raise ValueError("dev_stats must be a float64 GPU tensor")
+        o.dev_stats = dev_stats.data_ptr()
+    return o
+
+
+def empty_out(h: int, w: int, dtype: str = "float32", device=None) -> torch.Tensor:
+    return torch.empty((h, w), dtype=_TORCH_DTYPE[dtype], device=device or "cuda")
+
+
+def _lib():
+    return N.lib()
+
+
+# ------------------------------------------------------------------ stencils
+
+def hillshade_slope(win_t: torch.Tensor, reg: N.fsg_region, *, step_x: float, step_y: float,
+                    sign_x: float = 1.0, sign_y: float = 1.0, z_factor: float = 1.0,
+                    azimuth: float = 315.0, altitude: float = 45.0, unit: str = "degree",
+                    hs_out: Optional[N.fsg_output] = None,
+                    sl_out: Optional[N.fsg_output] = None, nan_free: bool = False) -> None:
+    p = N.fsg_shade_params(float(step_x), float(step_y), float(sign_x), float(sign_y),
+                           float(z_factor), float(azimuth), float(altitude),
+                           {"degree": 0, "percent": 1}.get(unit, 2), 0 if nan_free else 1)
+    w = window(win_t)
+    N.check(_lib().fsg_hillshade_slope(C.byref(w), C.byref(reg), C.byref(p),
+                                       C.byref(hs_out) if hs_out is not None else None,
+                                       C.byref(sl_out) if sl_out is not None else None,
+                                       _stream()))
+
+
+def curvature(win_t, reg, *, step_x: float, step_y: float, curvature_type: str,
+              out: N.fsg_output, nan_free: bool = False) -> None:
+    ct = {"mean": 0, "gaussian": 1, "planform": 2}.get(curvature_type, 3)
+    p = N.fsg_curvature_params(float(step_x), float(step_y), ct, 0 if nan_free else 1)
+    w = window(win_t)
+    N.check(_lib().fsg_curvature(C.byref(w), C.byref(reg), C.byref(p), C.byref(out), _stream()))
+
+
+# ------------------------------------------------------------------ rays
+
+def openness(win_t, reg, *, table, openness_type: str, out: N.fsg_output) -> None:
+    """table: list over directions of lists over distances of (ox, oy, phys)|None."""
+    n_dirs = len(table)
+    n_dist = max((len(r) for r in table), default=0)
+    arr = (N.fsg_ray_sample * max(1, n_dirs * n_dist))()
+    for d, row in enumerate(table):
+        for k in range(n_dist):
+            e = arr[d * n_dist + k]
+            entry = row[k] if k < len(row) else None
+            if entry is None:
+                e.skip = 1
+            else:
+                e.dx, e.dy, e.phys = int(entry[0]), int(entry[1]), float(entry[2])
+    p = N.fsg_openness_params(0 if openness_type == "positive" else 1, n_dirs, n_dist, 0,
+                              C.cast(arr, C.POINTER(N.fsg_ray_sample)))
+    w = window(win_t)
+    N.check(_lib().fsg_openness(C.byref(w), C.byref(reg), C.byref(p), C.byref(out), _stream()))
+
+
+def ambient_occlusion(win_t, reg, *, table, intensity: float, pad_d: int,
+                      out: N.fsg_output) -> None:
+    """table: list of (dx, dy, phys, dist_factor)|None in reference order."""
+    arr = (N.fsg_ao_sample * max(1, len(table)))()
+    for i, entry in enumerate(table):
+        if entry is None:
+            arr[i].skip = 1
+        else:
+            arr[i].dx, arr[i].dy = int(entry[0]), int(entry[1])
+            arr[i].phys, arr[i].dist_factor = float(entry[2]), float(entry[3])
+    p = N.fsg_ao_params(len(table), int(pad_d), float(intensity),
+                        C.cast(arr, C.POINTER(N.fsg_ao_sample)))
+    w = window(win_t)
+    L = _lib()
+    nbytes = L.fsg_ambient_occlusion_workspace_size(C.byref(w), C.byref(reg))
+    ws = torch.empty(max(1, nbytes), dtype=torch.uint8, device=win_t.device)
+    N.check(L.fsg_ambient_occlusion(C.byref(w), C.byref(reg), C.byref(p), C.byref(out),
+                                    ws.data_ptr(), nbytes, _stream()))
+
+
+# ------------------------------------------------------------------ topousm
+
+def topousm_params(radii: Sequence[int], weights: Sequence[float], ds: Sequence[int],
+                   field: Optional[torch.Tensor] = None, full_shape=None, win_origin=(0, 0),
+                   w_large: float = 0.0) -> N.fsg_topousm_params:
+    if len(radii) > N.MAX_RADII:
+        raise ValueError(f"at most {N.MAX_RADII} radii are supported")
+    if len(weights) != len(radii):
+        raise ValueError(
+            f"Length of weights ({len(weights)}) must match length of radii ({len(radii)})")
+    p = N.fsg_topousm_params()
+    p.nan_hint = 1
+    p.n_radii = len(radii)
+    for i, (r, wt, d) in enumerate(zip(radii, weights, ds)):
+        p.radii[i] = int(r)
+        p.weights[i] = float(np.float32(wt))
+        p.ds[i] = int(d)
+    if field is not None:
+        _check_window(field, "field")
+        if not field.is_contiguous():
+            raise ValueError("field must be contiguous")
+        p.field = field.data_ptr()
+        p.field_h, p.field_w = int(field.shape[0]), int(field.shape[1])
+        p.full_h, p.full_w = int(full_shape[0]), int(full_shape[1])
+        p.win_r0, p.win_c0 = int(win_origin[0]), int(win_origin[1])
+        p.w_large = float(np.float32(w_large))
+    return p
+
+
+class Workspace:
+    """Grow-only device scratch buffer (one per caller/stream)."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, nbytes: int, device) -> torch.Tensor:
+        if self.buf is None or self.buf.numel() < nbytes or self.buf.device != torch.device(device):
+            self.buf = torch.empty(max(256, int(nbytes)), dtype=torch.uint8, device=device)
+        return self.buf
+
+
+def topousm(win_t, reg, params: N.fsg_topousm_params, out: N.fsg_output,
+            ws: Optional[Workspace] = None, out2: Optional[N.fsg_output] = None) -> None:
+    w = window(win_t)
+    L = _lib()
+    nbytes = L.fsg_topousm_workspace_size(C.byref(w), C.byref(params))
+    buf = (ws or Workspace()).get(nbytes, win_t.device)
+    N.check(L.fsg_topousm(C.byref(w), C.byref(reg), C.byref(params), C.byref(out),
+                          C.byref(out2) if out2 is not None else None,
+                          buf.data_ptr(), buf.numel(), _stream()))
+
+
+def block_mean(src_t: torch.Tensor, factor: int) -> torch.Tensor:
+    h, w = src_t.shape
+    oh, ow = max(1, -(-h // factor)), max(1, -(-w // factor))
+    out = torch.empty((oh, ow), dtype=torch.float32, device=src_t.device)
+    win = window(src_t)
+    N.check(_lib().fsg_block_mean(C.byref(win), int(factor), out.data_ptr(), ow, _stream()))
+    return out
+
+
+def topousm_coarse_field(coarse_t: torch.Tensor, radii_coarse: Sequence[int],
+                         weights: Sequence[float]) -> torch.Tensor:
+    h, w = coarse_t.shape
+    out = torch.empty((h, w), dtype=torch.float32, device=coarse_t.device)
+    L = _lib()
+    nbytes = L.fsg_topousm_field_workspace_size(h, w)
+    ws = torch.empty(max(1, nbytes), dtype=torch.uint8, device=coarse_t.device)
+    rr = (C.c_int32 * len(radii_coarse))(*[int(r) for r in radii_coarse])
+    ww = (C.c_float * len(weights))(*[float(np.float32(x)) for x in weights])
+    win = window(coarse_t)
+    N.check(L.fsg_topousm_coarse_field(C.byref(win), len(radii_coarse), rr, ww, out.data_ptr(),
+                                       w, ws.data_ptr(), nbytes, _stream()))
+    return out
+
+
+# ------------------------------------------------------------------ stats
+
+PCT_ABS, PCT_FINITE = 1, 2
+
+
+def percentile(values: torch.Tensor, qs: Sequence[float], flags: int):
+    """Exact percentiles on the GPU.  Returns device tensors (pct[nq] f64,
+    n_valid[1] i64, moments[2] f64) without synchronising."""
+    if values.dtype != torch.float32 or not values.is_cuda:
+        raise ValueError("values must be a float32 GPU tensor")
+    v = values.reshape(-1)
+    if not v.is_contiguous():
+        v = v.contiguous()
+    dev = values.device
+    q = torch.tensor(list(qs), dtype=torch.float64, device=dev)
+    out = torch.empty(len(qs), dtype=torch.float64, device=dev)
+    nval = torch.empty(1, dtype=torch.int64, device=dev)
+    mom = torch.empty(2, dtype=torch.float64, device=dev)
+    L = _lib()
+    nbytes = L.fsg_percentile_workspace_size(v.numel())
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    N.check(L.fsg_percentile(v.data_ptr(), v.numel(), int(flags),
+                             C.cast(q.data_ptr(), C.POINTER(C.c_double)), len(qs),
+                             out.data_ptr(), nval.data_ptr(), mom.data_ptr(), ws.data_ptr(),
+                             nbytes, _stream()))
+    # keep q alive until the kernels that read it have run
+    out._fsg_keep = (q, ws)
+    return out, nval, mom
+
+
+def finalize_stats(kind: int, pct, moments, n_valid) -> torch.Tensor:
+    stats = torch.empty(2, dtype=torch.float64, device=pct.device)
+    N.check(_lib().fsg_finalize_stats(int(kind), pct.data_ptr(), moments.data_ptr(),
+                                      n_valid.data_ptr(), stats.data_ptr(), _stream()))
+    return stats
+
+
+def copy_region(win_t, reg, dst: torch.Tensor) -> None:
+    w = window(win_t)
+    N.check(_lib().fsg_copy_region(C.byref(w), C.byref(reg), dst.data_ptr(), _stream()))
+
+
+def encode(src_t: torch.Tensor, reg, out: N.fsg_output) -> None:
+    w = window(src_t)
+    N.check(_lib().fsg_encode(C.byref(w), C.byref(reg), C.byref(out), _stream()))
+
+
+def quantize(t: torch.Tensor, qp: dict, dtype: str) -> torch.Tensor:
+    """quantize_array on the GPU for a 1-D or 2-D float32 tensor."""
+    src = t.to(torch.float32)
+    shape = src.shape
+    src2 = src.reshape(1, -1) if src.dim() != 2 else src
+    src2 = src2.contiguous()
+    dst = torch.empty(src2.shape, dtype=_TORCH_DTYPE[dtype], device=src2.device)
+    encode(src2, full_region(src2), output(dst, qp=qp))
+    return dst.reshape(shape)
+
+
+def nodata_to_nan(raster: torch.Tensor, nodata: Optional[float], windows=()) -> torch.Tensor:
+    """NoData sentinel -> NaN in place; returns per-window NoData counts (device)."""
+    _check_window(raster, "raster")
+    h, w = raster.shape
+    counts = torch.zeros(max(1, len(windows)), dtype=torch.int64, device=raster.device)
+    flat = (C.c_int32 * max(1, 4 * len(windows)))(*[int(v) for wd in windows for v in wd])
+    has = nodata is not None
+    N.check(_lib().fsg_nodata_to_nan(raster.data_ptr(), int(raster.stride(0)), h, w, int(has),
+                                     float(nodata) if has else 0.0, flat, len(windows),
+                                     counts.data_ptr(), _stream()))
+    return counts

@@ -1,0 +1,287 @@
+"""Device-resident tile scheduler: process_dem_tiles without the file I/O.
+
+Restates the orchestration of process_dem_tiles / process_single_tile
+(core/tile_processor.py:822-1745) for a raster that already lives in HBM:
+output encoding, mode and radii resolution, the TopoUSM large-radius split and
+overview field, halo sizing, the global-statistics pre-pass, the tile grid,
+NoData skipping, per-tile geographic scales, and the per-tile compute.  Each
+tile computes only its core region, straight into the output raster, with the
+crop, NoData restore, normalisation and quantisation fused into the kernel
+epilogue.  GDAL decode/encode, per-tile GeoTIFFs and the COG build are not
+part of this path.
+"""
+from __future__ import annotations
+
+import logging
+import math
+from dataclasses import dataclass, field
+from typing import Any, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import ops
+from .algorithms import ALGORITHMS, SUPPORTED, OutSpec, UnsupportedOnBackend, load_algorithm
+from .encoding import SUPPORTED_OUTPUT_DTYPES, quantize_params, resolve_output_range
+from .params import (LOCAL_RADII, LOCAL_WEIGHTS, MULTISCALE_REQUIRED_ALGOS,
+                     RADII_DRIVEN_ALGOS, SPATIAL_TILE_ALGORITHMS, auto_spatial_profile,
+                     normalize_topousm_radii_weights, overview_geometry, required_padding,
+                     sanitize_spatial_radii_weights_for_tile, split_radii_by_threshold,
+                     tile_grid, topousm_large_radius_threshold)
+from .raster_info import metric_pixel_scales, tile_geographic_scales
+from .stats import norm_stats_prepass
+
+logger = logging.getLogger(__name__)
+
+
+@dataclass
+class TileResult:
+    """utils/types.py:7-14 (success / skipped / error per tile)."""
+    tile_y: int
+    tile_x: int
+    success: bool
+    filename: Optional[str] = None
+    error_message: Optional[str] = None
+    skipped_reason: Optional[str] = None
+
+
+@dataclass
+class RunPlan:
+    algorithm: str
+    params: Dict[str, Any]
+    tile_size: int
+    padding: int
+    tiles: List[tuple]
+    output_dtype: str
+    qp: Optional[dict]
+    stats: Any = None
+    extras: Dict[str, Any] = field(default_factory=dict)
+
+
+def _validate_sizes(tile_size, padding):
+    """core/tile_processor.py:1053-1073."""
+    if tile_size is not None:
+        try:
+            parsed = int(tile_size)
+        except (TypeError, ValueError) as exc:
+            raise ValueError(f"tile_size must be a positive integer, got {tile_size!r}") from exc
+        if isinstance(tile_size, bool) or parsed != tile_size or parsed <= 0:
+            raise ValueError(f"tile_size must be a positive integer, got {tile_size!r}")
+        tile_size = parsed
+    if padding is not None:
+        try:
+            parsed = int(padding)
+        except (TypeError, ValueError) as exc:
+            raise ValueError(f"padding must be >= 0, got {padding!r}") from exc
+        if isinstance(padding, bool) or parsed != padding or parsed < 0:
+            raise ValueError(f"padding must be >= 0, got {padding!r}")
+        padding = parsed
+    return tile_size, padding
+
+
+def plan(raster: torch.Tensor, algorithm: str, *, tile_size: Optional[int] = 4096,
+         padding: Optional[int] = None, sigma: float = 10.0, multiscale_mode: bool = True,
+         pixel_size: Optional[float] = None, transform: Optional[Sequence[float]] = None,
+         is_geographic: Optional[bool] = False, nodata: Optional[float] = None,
+         **algo_params) -> RunPlan:
+    """Parameter resolution of process_dem_tiles (core/tile_processor.py:1050-1590).
+
+    ``transform`` = affine (a, b, c, d, e, f); without it the raster is taken as
+    projected, north-up, with ``pixel_size`` cells."""
+    if algorithm not in ALGORITHMS:
+        raise ValueError(f"Algorithm {algorithm} not found or not available on this platform")
+    if algorithm not in SUPPORTED:
+        ALGORITHMS[algorithm].process(raster)  # raises UnsupportedOnBackend
+    tile_size, padding = _validate_sizes(tile_size, padding)
+    H, W = int(raster.shape[0]), int(raster.shape[1])
+    algo_params = dict(algo_params)
+    user_radii = algo_params.get("radii") is not None
+    user_weights = algo_params.get("weights") is not None
+
+    output_dtype = str(algo_params.pop("output_dtype", "float32") or "float32").lower()
+    output_range = algo_params.pop("output_range", None)
+    if output_dtype not in SUPPORTED_OUTPUT_DTYPES:
+        raise ValueError(
+            f"Unsupported output_dtype={output_dtype!r}. Choose from {SUPPORTED_OUTPUT_DTYPES}.")
+    qp = None
+    if output_dtype in ("int16", "uint8"):
+        vr = resolve_output_range(algorithm, params=algo_params, override=output_range)
+        if vr is None:
+            logger.warning("No fixed output range for %s (unit=%s); writing float32 instead of %s.",
+                           algorithm, algo_params.get("unit", ""), output_dtype)
+            output_dtype = "float32"
+        else:
+            qp = quantize_params(float(vr[0]), float(vr[1]), output_dtype)
+
+    if pixel_size is None:
+        pixel_size = abs(float(transform[0])) if transform is not None else 1.0
+    if algorithm == "hillshade":
+        algo_params.setdefault("z_factor", 1.0)
+
+    mode_now = str(algo_params.get("mode", "spatial")).lower()
+    is_local = mode_now == "local" or (algorithm == "topousm_fast" and not multiscale_mode)
+    if is_local and algorithm in MULTISCALE_REQUIRED_ALGOS:
+        algo_params["mode"] = "spatial"
+        mode_now, is_local = "spatial", False
+    if is_local:
+        algo_params["radii"] = list(LOCAL_RADII)
+        algo_params["weights"] = list(LOCAL_WEIGHTS)
+    elif algorithm in RADII_DRIVEN_ALGOS and not algo_params.get("radii") and mode_now == "spatial":
+        r, w = auto_spatial_profile(min(W, H))
+        algo_params["radii"] = r
+        if not algo_params.get("weights"):
+            algo_params["weights"] = w
+
+    mode_norm = str(algo_params.get("mode", "spatial")).lower()
+    if mode_norm == "spatial" and algorithm in SPATIAL_TILE_ALGORITHMS:
+        if algo_params.get("radii") is None:
+            r, w = auto_spatial_profile(None)
+            algo_params["radii"] = r
+            if algo_params.get("weights") is None and w:
+                algo_params["weights"] = w
+        ar, aw, _ = sanitize_spatial_radii_weights_for_tile(
+            algorithm, algo_params.get("radii"), algo_params.get("weights"), int(tile_size))
+        if ar is not None:
+            algo_params["radii"] = ar
+            if aw is not None:
+                algo_params["weights"] = aw
+            elif algo_params.get("weights") is not None:
+                algo_params["weights"] = None
+
+    extras: Dict[str, Any] = {}
+    if algorithm == "topousm_fast":
+        full_r, full_w = normalize_topousm_radii_weights(
+            None, None, float(pixel_size), manual_radii=algo_params.get("radii"),
+            manual_weights=algo_params.get("weights"))
+        if full_r:
+            thr = topousm_large_radius_threshold(int(tile_size))
+            sr, sw, lr, lw = split_radii_by_threshold(full_r, full_w, thr)
+            if lr:
+                extras["large"] = (lr, lw)
+                algo_params["_topousm_fast_full_radii"] = list(full_r)
+                algo_params["_topousm_fast_full_weights"] = list(full_w) if full_w else None
+                algo_params["radii"] = sr
+                algo_params["weights"] = sw
+                algo_params["_topousm_fast_small_radii"] = sr
+                algo_params["_topousm_fast_small_weights"] = sw
+                algo_params["_topousm_fast_w_large"] = float(sum(lw))
+                algo_params["_topousm_fast_full_shape"] = (H, W)
+
+    algo_params.setdefault("is_geographic_dem", bool(is_geographic))
+    req = required_padding(algorithm, algo_params, sigma, float(pixel_size), None, int(tile_size))
+    padding = req if padding is None else max(padding, req)
+
+    # small DEM -> local (core/tile_processor.py:1442-1452)
+    if (str(algo_params.get("mode", "spatial")).lower() == "spatial" and not user_radii
+            and not user_weights and min(W, H) <= 1024):
+        algo_params["mode"] = "local"
+
+    if transform is not None:
+        a, e = float(transform[0]), float(transform[4])
+        left, top = float(transform[2]), float(transform[5])
+        bounds = (left, top + H * e, left + W * a, top)
+        psx, psy, _, is_geo, _ = metric_pixel_scales(transform_a=a, transform_e=e,
+                                                     is_geographic=is_geographic, bounds=bounds)
+        algo_params["pixel_scale_x"], algo_params["pixel_scale_y"] = psx, psy
+        algo_params["is_geographic_dem"] = bool(is_geo)
+        extras["transform"] = tuple(float(t) for t in transform)
+    else:
+        algo_params.setdefault("pixel_scale_x", float(pixel_size))
+        algo_params.setdefault("pixel_scale_y", -float(pixel_size))
+    algo_params["pixel_size"] = float(pixel_size)
+    tiles = tile_grid(W, H, int(tile_size), int(padding))
+    return RunPlan(algorithm, algo_params, int(tile_size), int(padding), tiles, output_dtype,
+                   qp, extras=extras)
+
+
+def overview_field(raster: torch.Tensor, large_radii, large_weights) -> Tuple[torch.Tensor, float]:
+    """Large-radius coarse field from the overview (core/tile_processor.py:552-602,
+    _impl_topousm_fast.py:133-155).  The overview read is GDAL average
+    resampling; for an integer decimation that is the NaN-aware block mean."""
+    H, W = raster.shape
+    scale, sh, sw = overview_geometry(W, H)
+    f = int(round(scale))
+    if abs(scale - f) > 1e-12 or sh * f != H or sw * f != W:
+        raise UnsupportedOnBackend(
+            f"overview decimation {scale:.4f} for a {W}x{H} raster is not an integer block "
+            "mean; GDAL's fractional average resampling is not restated")
+    coarse = ops.block_mean(raster, f) if f > 1 else raster.contiguous().clone()
+    rc = [max(1, int(round(float(r) / max(scale, 1.0)))) for r in large_radii]
+    return ops.topousm_coarse_field(coarse, rc, large_weights), scale
+
+
+def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float] = None,
+                nodata_threshold: float = 1.0, out: Optional[torch.Tensor] = None,
+                out_f32: Optional[torch.Tensor] = None, global_stats=None,
+                copy_input: bool = True, return_plan: bool = False, **kwargs):
+    """Run one algorithm over a device-resident raster, tile by tile.
+
+    Returns the output raster (float32 with NaN NoData, or int16/uint8 with 0
+    NoData), and the per-tile TileResult list.  ``out_f32`` optionally also
+    receives the float32 result when the main output is integer-encoded."""
+    P = plan(raster, algorithm, nodata=nodata, **kwargs)
+    H, W = raster.shape
+    src = raster.clone() if copy_input else raster
+    counts = ops.nodata_to_nan(src, nodata, [(t[7], t[6], t[9], t[8]) for t in P.tiles])
+    if "large" in P.extras:
+        lr, lw = P.extras["large"]
+        fld, scale = overview_field(src, lr, lw)
+        P.params["_topousm_fast_coarse_field"] = fld
+        P.extras["overview_scale"] = scale
+    stats = global_stats if global_stats is not None else P.params.get("global_stats")
+    if stats is None:
+        stats = norm_stats_prepass(src, algorithm, P.params)
+    if stats is not None:
+        P.params["global_stats"] = stats
+    P.stats = stats
+    dt = {"float32": torch.float32, "int16": torch.int16, "uint8": torch.uint8}[P.output_dtype]
+    if out is None:
+        out = torch.empty((H, W), dtype=dt, device=raster.device)
+    nan_counts = counts.cpu().numpy()
+    algo = load_algorithm(algorithm)
+    results: List[TileResult] = []
+    transform = P.extras.get("transform")
+    for t, nn in zip(P.tiles, nan_counts):
+        ty, tx, cx, cy, cw, ch, wx, wy, ww, wh = t
+        ratio = float(nn) / float(ww * wh) if ww * wh else 1.0
+        o_view = out[cy:cy + ch, cx:cx + cw]
+        f_view = out_f32[cy:cy + ch, cx:cx + cw] if out_f32 is not None else None
+        if nn > 0 and ratio >= nodata_threshold:
+            o_view.fill_(float("nan") if dt == torch.float32 else 0)
+            if f_view is not None:
+                f_view.fill_(float("nan"))
+            results.append(TileResult(ty, tx, False, skipped_reason=(
+                f"NoData covers {ratio:.1%} (threshold: {nodata_threshold:.1%})")))
+            continue
+        p = dict(P.params)
+        p["_window_nan_free"] = int(nn) == 0
+        if algorithm == "topousm_fast" and "_topousm_fast_coarse_field" in p:
+            p["_topousm_fast_field_offset"] = (int(wy), int(wx))
+        if transform is not None and p.get("is_geographic_dem"):
+            a, _, _, _, e, f = transform
+            top = f + cy * e
+            bottom = f + (cy + ch) * e
+            p["pixel_scale_x"], p["pixel_scale_y"] = tile_geographic_scales(
+                a, e, 0.5 * (top + bottom))
+        win = src[wy:wy + wh, wx:wx + ww]
+        reg = ops.region(cy - wy, cx - wx, ch, cw)
+        spec = OutSpec(f32=o_view if dt == torch.float32 else f_view,
+                       q=o_view if dt != torch.float32 else None, qp=P.qp,
+                       clip01=(algorithm == "hillshade"))
+        try:
+            algo.run_region(win, reg, spec, p)
+            results.append(TileResult(ty, tx, True))
+        except UnsupportedOnBackend:
+            raise
+        except Exception as exc:  # per-tile failure record (core/tile_processor.py:1022-1024)
+            logger.error("Tile (%d, %d) processing failed: %s", ty, tx, exc)
+            results.append(TileResult(ty, tx, False, error_message=str(exc)))
+    errors = [r for r in results if not r.success and not r.skipped_reason]
+    if errors:
+        detail = "\n".join(f"Tile ({r.tile_y}, {r.tile_x}): {r.error_message}" for r in errors[:3])
+        raise RuntimeError(f"Tile processing error:\n{detail}")
+    if not any(r.success for r in results):
+        raise ValueError("No tiles were processed")
+    if return_plan:
+        return out, results, P
+    return out, results

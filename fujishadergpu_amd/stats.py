@@ -1,0 +1,119 @@
+"""Global normalisation statistics on the GPU (the stratified pre-pass).
+
+Restates _compute_norm_stats_tiled / inject_global_stats
+(algorithms/_norm_stats.py:176-350) for an in-memory raster already resident
+in HBM: the valid-data bounding box, the 3x3 stratified windows, the raw block
+function on each window (normalisation off), the margin trim, the pooled
+values and the stat function -- all on the device.  The statistic stays on the
+device (float64) and feeds the normalisation epilogue of the main pass
+directly; nothing is copied back unless the caller asks.
+"""
+from __future__ import annotations
+
+import logging
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import ops
+from .algorithms import (AmbientOcclusionAlgorithm, OpennessAlgorithm, OutSpec,
+                         TopoUSMFastAlgorithm)
+from .params import NORM_STAT_ALGOS, norm_stat_window_geometry, stratified_windows
+
+logger = logging.getLogger(__name__)
+
+# Defaults merged before the block-function parameters (ALGORITHMS[..].get_default_params()).
+_DEFAULTS = {
+    "topousm_fast": TopoUSMFastAlgorithm().get_default_params,
+    "openness": OpennessAlgorithm().get_default_params,
+    "ambient_occlusion": AmbientOcclusionAlgorithm().get_default_params,
+}
+# Parameters each raw block function accepts (inspect.signature in the reference).
+_BLOCK_KEYS = {
+    "topousm_fast": ("radii", "weights", "pixel_size"),
+    "openness": ("openness_type", "num_directions", "max_distance", "pixel_size",
+                 "pixel_scale_x", "pixel_scale_y"),
+    "ambient_occlusion": ("num_samples", "radius", "intensity", "pixel_size",
+                          "pixel_scale_x", "pixel_scale_y"),
+}
+
+
+def valid_bbox(raster: torch.Tensor):
+    """Valid-data bounding box from a ~512 px nearest-neighbour overview
+    (_norm_stats.py:254-264).  GDAL's nearest pick for non-integer ratios is
+    unpinned; the overview samples pixel centres floor((i + 0.5) * H / oh)."""
+    H, W = raster.shape
+    cov = max(1, max(W, H) // 512)
+    oh, ow = max(1, H // cov), max(1, W // cov)
+    ri = torch.clamp(((torch.arange(oh, device=raster.device, dtype=torch.float64) + 0.5) * (H / oh))
+                     .floor().long(), max=H - 1)
+    ci = torch.clamp(((torch.arange(ow, device=raster.device, dtype=torch.float64) + 0.5) * (W / ow))
+                     .floor().long(), max=W - 1)
+    ov = raster.index_select(0, ri).index_select(1, ci)
+    vm = torch.isfinite(ov)
+    if not bool(vm.any()):
+        return None
+    ys = torch.nonzero(vm.any(dim=1)).flatten()
+    xs = torch.nonzero(vm.any(dim=0)).flatten()
+    by0, by1 = int(ys.min()) * cov, min(H, (int(ys.max()) + 1) * cov)
+    bx0, bx1 = int(xs.min()) * cov, min(W, (int(xs.max()) + 1) * cov)
+    return by0, by1, bx0, bx1
+
+
+def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
+                       grid: int = 3, max_tile: int = 4096, min_valid_frac: float = 0.02,
+                       return_pool: bool = False):
+    """Device-side _compute_norm_stats_tiled.  ``raster`` holds NaN at NoData.
+    Returns a float64 GPU tensor (stats) or None (no valid window)."""
+    if algorithm not in NORM_STAT_ALGOS:
+        return None
+    merged = {**_DEFAULTS[algorithm](), **(params or {})}
+    if merged.get("_topousm_fast_full_radii"):
+        merged["radii"] = list(merged["_topousm_fast_full_radii"])
+        if merged.get("_topousm_fast_full_weights"):
+            merged["weights"] = list(merged["_topousm_fast_full_weights"])
+    kw = {k: merged[k] for k in _BLOCK_KEYS[algorithm] if merged.get(k) is not None}
+    margin, tile = norm_stat_window_geometry(algorithm, merged, max_tile)
+    H, W = raster.shape
+    bbox = valid_bbox(raster)
+    if bbox is None:
+        return None
+    wins = stratified_windows(W, H, *bbox, grid=grid, tile=min(tile, max(W, H)))
+    # valid fraction of every window in one device pass, one host sync
+    counts = ops.nodata_to_nan(raster, None, [(wy, wx, th, tw) for wy, wx, tw, th in wins])
+    nan_counts = counts.cpu().numpy()
+    regions = []
+    for (wy, wx, tw, th), nn in zip(wins, nan_counts):
+        if (th * tw - int(nn)) / float(th * tw) < min_valid_frac:
+            continue
+        m = int(min(margin, th // 3, tw // 3))
+        regions.append((wy, wx, th, tw, m, int(nn) == 0))
+    if not regions:
+        return None
+    total = sum((th - 2 * m) * (tw - 2 * m) for _, _, th, tw, m, _ in regions)
+    pool = torch.empty(total, dtype=torch.float32, device=raster.device)
+    algo = {"topousm_fast": TopoUSMFastAlgorithm, "openness": OpennessAlgorithm,
+            "ambient_occlusion": AmbientOcclusionAlgorithm}[algorithm]()
+    bp = dict(kw)
+    off = 0
+    for wy, wx, th, tw, m, nan_free in regions:
+        win = raster[wy:wy + th, wx:wx + tw]
+        rh, rw = th - 2 * m, tw - 2 * m
+        dst = pool[off:off + rh * rw].view(rh, rw)
+        reg = ops.region(m, m, rh, rw)
+        if algorithm == "topousm_fast":
+            algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "_window_nan_free": nan_free},
+                            normalize=False)
+        else:
+            algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "mode": "local"}, stretch=False)
+        off += rh * rw
+    if algorithm == "topousm_fast":
+        pct, nval, mom = ops.percentile(pool, [99.0], ops.PCT_ABS)
+        stats = ops.finalize_stats(0, pct, mom, nval)
+    else:
+        pct, nval, mom = ops.percentile(pool, [1.0, 99.0], ops.PCT_FINITE)
+        stats = ops.finalize_stats(1, pct, mom, nval)
+    if return_pool:
+        return stats, pool
+    return stats

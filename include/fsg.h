@@ -1,0 +1,339 @@
+/*
+ * fsg.h -- C ABI of the MI355X-native FujiShaderGPU raster hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers, sizes and POD
+ * structs, and never allocates or frees caller memory.  Device buffers are
+ * owned by the caller (e.g. torch tensors, via data_ptr()).  All work is
+ * enqueued on the caller's HIP stream (`stream`, a hipStream_t passed as
+ * void*; NULL = the default stream) and is stream-ordered; nothing blocks the
+ * host except where a function says so.
+ *
+ * Return codes: FSG_OK (0) or a positive FSG_E_* code; fsg_last_error()
+ * returns a thread-local message for the last failure.  No C++ exception
+ * crosses this boundary.
+ *
+ * Geometry convention: a *window* is the array a per-tile call receives (tile
+ * core + halo, clipped at the raster edge), exactly as the reference's
+ * TileAlgorithm.process(dem_gpu) receives it.  A *region* is the sub-rectangle
+ * of the window whose output is written; the reference always computes the
+ * whole window and crops the core afterwards
+ * (core/tile_processor.py:944-971), so region == whole window reproduces
+ * process() and region == core fuses the crop.  Results inside the region are
+ * identical either way: every window-edge rule (one-sided gradients, reflect /
+ * nearest padding, out-of-window rays) is evaluated against the window.
+ */
+#ifndef FSG_H
+#define FSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSG_ABI_VERSION 1
+
+enum {
+  FSG_OK = 0,
+  FSG_E_INVALID = 1,     /* bad argument (shape, pointer, parameter) */
+  FSG_E_HIP = 2,         /* a HIP runtime call failed */
+  FSG_E_UNSUPPORTED = 3, /* valid request this build does not implement */
+  FSG_E_WORKSPACE = 4    /* workspace too small (see *_workspace_size) */
+};
+
+/* Output encodings (io/output_encoding.py:75 SUPPORTED_OUTPUT_DTYPES). */
+enum { FSG_OUT_F32 = 0, FSG_OUT_I16 = 1, FSG_OUT_U8 = 2 };
+
+/* Post-processing applied to the float result before encoding. */
+enum {
+  FSG_POST_NONE = 0,
+  FSG_POST_DIV = 1,     /* v / f32(scale)          (_normalization.py:35-41)   */
+  FSG_POST_STRETCH = 2  /* max((v - f32(lo)) / f32(scale), 0), no-op unless
+                           scale > 1e-12          (tile/dask_bridge.py:173-187) */
+};
+
+typedef struct fsg_window {
+  const float* data; /* device pointer to window element (0, 0) */
+  int64_t ld;        /* row stride in elements (>= w) */
+  int32_t h, w;      /* window rows, columns */
+} fsg_window;
+
+typedef struct fsg_region {
+  int32_t r0, c0, h, w; /* output rectangle, in window coordinates */
+} fsg_region;
+
+/*
+ * Output descriptor: post-processing + encoding + NoData policy.
+ *   float32: NaN stays NaN (NoData), optional [0,1] re-clip
+ *            (core/tile_processor.py:606-624 _format_algorithm_output).
+ *   int16/uint8: DN = clip(rint(f32(a)*v + f32(b)), dn_min, dn_max) with two
+ *            rounded f32 operations and ties-to-even; non-finite v -> 0
+ *            (io/output_encoding.py:178-190, core/dask_processor.py:997-1008).
+ * post_kind uses host_stats, or dev_stats (device doubles) when non-NULL, so a
+ * statistic computed on the GPU is consumed without a host round trip.
+ */
+typedef struct fsg_output {
+  void* data;     /* device pointer to region element (0, 0) */
+  int64_t ld;     /* row stride in elements of the output dtype */
+  int32_t dtype;  /* FSG_OUT_* */
+  int32_t clip01; /* 1: clip finite values to [0, 1] before encoding */
+  float a_coef, b_coef;
+  int32_t dn_min, dn_max;
+  int32_t post_kind;        /* FSG_POST_* */
+  int32_t pad0;
+  double host_stats[2];     /* DIV: {scale}; STRETCH: {lo, scale} */
+  const double* dev_stats;  /* optional device copy of host_stats */
+} fsg_output;
+
+/* ------------------------------------------------------------------ */
+/* Library                                                             */
+/* ------------------------------------------------------------------ */
+
+int fsg_abi_version(void);
+const char* fsg_last_error(void);
+/* Number of kernels launched by this thread since load (testing aid). */
+int64_t fsg_launch_count(void);
+/*
+ * Kernel timing for measurement: while enabled, HIP events are recorded on
+ * the launch stream around the named hot kernels ("k_usm_main",
+ * "k_hillshade_slope", "k_curvature", "k_openness", "k_ao_raw", ...).
+ * fsg_timing_read synchronises those events, returns the summed milliseconds
+ * and launch count for one name, and forgets them.  No reference counterpart
+ * (the reference has no kernels of its own to time).
+ */
+int fsg_timing_enable(int32_t on);
+int fsg_timing_read(const char* name, double* total_ms, int64_t* count);
+
+/* ------------------------------------------------------------------ */
+/* Input preparation                                                   */
+/* ------------------------------------------------------------------ */
+
+/*
+ * NoData sentinel -> NaN, in place, and per-tile NoData counts.
+ * Replaces _build_nodata_mask + dem_tile[mask] = NaN
+ * (core/tile_processor.py:185-196, :854-885): a cell is NoData when it is NaN
+ * or |f64(v) - nodata| <= 1e-6 (numpy isclose, rtol=0, atol=1e-6).
+ * has_nodata = 0 converts nothing (NaN cells still count).
+ * counts (device int64[n_tiles], optional) receives the NoData count of each
+ * window listed in windows (host array of int32[4] = {r0, c0, h, w} in raster
+ * coordinates); counts are accumulated, so zero them first.
+ */
+int fsg_nodata_to_nan(float* raster, int64_t ld, int32_t h, int32_t w,
+                      int32_t has_nodata, double nodata,
+                      const int32_t* windows, int32_t n_windows,
+                      int64_t* counts, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Gradient stencils                                                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct fsg_shade_params {
+  /* Derivative steps: |pixel_scale| (or pixel_size when the scale is None or
+     below 1e-9), exactly as handle_nan_for_gradient resolves them
+     (algorithms/_nan_utils.py:65-73). */
+  double step_x, step_y;
+  double sign_x, sign_y; /* +1/-1 east/north orientation (_impl_hillshade.py:39-42) */
+  double z_factor;       /* elevation scale before the gradient (hillshade) */
+  double azimuth_deg, altitude_deg;
+  int32_t slope_unit;    /* 0 degree, 1 percent, 2 radian (_impl_slope.py:27-33) */
+  int32_t nan_hint;      /* 0: window known NaN-free (no fill path); else NaN-aware */
+} fsg_shade_params;
+
+/*
+ * Hillshade and/or slope from the central-difference gradient with
+ * edge_order=2 window edges and the NaN-aware sigma=1 fill.
+ * Replaces compute_hillshade_block (_impl_hillshade.py:20-54) and
+ * compute_slope_block (_impl_slope.py:19-35); both outputs come from one
+ * read of the window.  Either output may be NULL.  The window needs h, w >= 3
+ * (numpy/cupy gradient edge_order=2 requirement).
+ */
+int fsg_hillshade_slope(const fsg_window* win, const fsg_region* reg,
+                        const fsg_shade_params* p,
+                        const fsg_output* hillshade, const fsg_output* slope,
+                        void* stream);
+
+typedef struct fsg_curvature_params {
+  /* Signed steps (pixel_scale or pixel_size, with the 1e-9 fallback) as
+     compute_curvature_block resolves them (_impl_curvature.py:23-28); the
+     first derivatives use |step|, the second ones the signed step. */
+  double step_x, step_y;
+  int32_t curvature_type; /* 0 mean, 1 gaussian, 2 planform, 3 profile */
+  int32_t nan_hint;       /* 0: window known NaN-free (no fill path); else NaN-aware */
+} fsg_curvature_params;
+
+/* Replaces compute_curvature_block (_impl_curvature.py:19-57). */
+int fsg_curvature(const fsg_window* win, const fsg_region* reg,
+                  const fsg_curvature_params* p, const fsg_output* out,
+                  void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Ray marches                                                          */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Openness ray table, built by the caller exactly as
+ * compute_openness_vectorized builds it (_impl_openness.py:58-109):
+ * for direction k (0..n_dirs-1) the samples k*n_dist .. k*n_dist+n_dist-1 hold
+ * integer offsets (dx = column, dy = row) and the physical distance
+ * max(hypot(dx*|sx|, dy*|sy|), 1e-9).  Samples with dx == dy == 0 must be
+ * marked skip = 1.
+ */
+typedef struct fsg_ray_sample {
+  int32_t dx, dy;
+  int32_t skip, pad0;
+  double phys;
+} fsg_ray_sample;
+
+typedef struct fsg_openness_params {
+  int32_t openness_type; /* 0 positive, 1 negative */
+  int32_t n_dirs, n_dist;
+  int32_t pad0;
+  const fsg_ray_sample* samples; /* host pointer, n_dirs*n_dist entries */
+} fsg_openness_params;
+
+/* Replaces compute_openness_vectorized (_impl_openness.py:31-132). */
+int fsg_openness(const fsg_window* win, const fsg_region* reg,
+                 const fsg_openness_params* p, const fsg_output* out,
+                 void* stream);
+
+/*
+ * Ambient-occlusion sample table (_impl_ambient_occlusion.py:288-339): ring
+ * r (0..3) sample i holds the np.round offsets, skip when both are 0, the
+ * physical distance and the ring's distance factor 1 - 0.3*r_factor.
+ */
+typedef struct fsg_ao_sample {
+  int32_t dx, dy;
+  int32_t skip, pad0;
+  double phys;
+  double dist_factor;
+} fsg_ao_sample;
+
+typedef struct fsg_ao_params {
+  int32_t n_samples; /* total table entries (4 rings x num_samples) */
+  int32_t pad_d;     /* D = max(1, int(round(radius))) edge padding */
+  double intensity;
+  const fsg_ao_sample* samples; /* host pointer */
+} fsg_ao_params;
+
+/* Replaces compute_ambient_occlusion_block (_impl_ambient_occlusion.py:32-121). */
+size_t fsg_ambient_occlusion_workspace_size(const fsg_window* win,
+                                            const fsg_region* reg);
+int fsg_ambient_occlusion(const fsg_window* win, const fsg_region* reg,
+                          const fsg_ao_params* p, const fsg_output* out,
+                          void* workspace, size_t workspace_bytes,
+                          void* stream);
+
+/* ------------------------------------------------------------------ */
+/* TopoUSM Fast                                                         */
+/* ------------------------------------------------------------------ */
+
+#define FSG_MAX_RADII 16
+
+typedef struct fsg_topousm_params {
+  int32_t n_radii;
+  int32_t radii[FSG_MAX_RADII];     /* small (in-window) radii, in order */
+  float weights[FSG_MAX_RADII];     /* f32 weights (cp.array(weights, f32)) */
+  int32_t ds[FSG_MAX_RADII];        /* _radius_to_downsample_factor per radius */
+  /* Large-radius overview field (tile/dask_bridge.py:237-249); field == NULL
+     disables it.  field is a device f32 grid of field_h x field_w. */
+  const float* field;
+  int32_t field_h, field_w;
+  int32_t full_h, full_w;           /* raster shape the field spans */
+  int32_t win_r0, win_c0;           /* window origin in raster coordinates */
+  float w_large;                    /* f32(sum of the large weights) */
+  /* 0: the caller knows the window holds no NaN (enables the plain-mean
+     variant); any other value: NaN-aware means (always correct). */
+  int32_t nan_hint;
+} fsg_topousm_params;
+
+/*
+ * Replaces compute_topousm_fast_efficient_block (_impl_topousm_fast.py:49-100)
+ * plus the large-radius part and normalisation of _direct_topousm_fast
+ * (tile/dask_bridge.py:225-271).  Normalisation is out->post_kind = DIV.
+ * out2 (optional) receives the same values in a second encoding (e.g. the
+ * float32 and the uint8 product of one pass).
+ */
+size_t fsg_topousm_workspace_size(const fsg_window* win,
+                                  const fsg_topousm_params* p);
+int fsg_topousm(const fsg_window* win, const fsg_region* reg,
+                const fsg_topousm_params* p, const fsg_output* out,
+                const fsg_output* out2, /* optional second encoding */
+                void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * NaN-aware block mean (area average for an integer factor), the restatement
+ * of the overview read with Resampling.average
+ * (core/tile_processor.py:575-590) and of _downsample_nan_aware's cell mean
+ * (_nan_utils.py:620-643).  out is ceil(h/f) x ceil(w/f); ragged cells
+ * average their in-raster cells; all-NaN cells become NaN.
+ */
+int fsg_block_mean(const fsg_window* src, int32_t factor, float* out,
+                   int64_t ld_out, void* stream);
+
+/*
+ * Large-radius coarse field sum_i w_i * mean_{r_i}(coarse)
+ * (_impl_topousm_fast.py:133-155).  radii are already scaled into the coarse
+ * grid (max(1, round(r / decimation))).  out must not alias coarse.
+ */
+size_t fsg_topousm_field_workspace_size(int32_t h, int32_t w);
+int fsg_topousm_coarse_field(const fsg_window* coarse, int32_t n_radii,
+                             const int32_t* radii_coarse,
+                             const float* weights, float* out, int64_t ld_out,
+                             void* workspace, size_t workspace_bytes,
+                             void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Statistics                                                           */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Order statistics of the finite values of a device f32 array, exactly:
+ * radix selection on the order-preserving bit pattern, then numpy's
+ * method='linear' interpolation in FP64 (cp.percentile as called by
+ * topousm_fast_stat_func, _normalization.py:22-32, and
+ * robust_unsigned_stretch_stat_func, _global_stats.py:181-203).
+ * flags: FSG_PCT_ABS selects on |v| (topousm_fast_stat_func); FSG_PCT_FINITE
+ * keeps only finite values (robust_unsigned_stretch_stat_func uses isfinite,
+ * topousm_fast_stat_func only drops NaN).  q (device double[nq], nq <= 2) are
+ * percentages in [0, 100]; out (device double[nq]) receives the percentiles;
+ * n_valid (device int64, optional) the selected count; moments (device
+ * double[2], optional) the FP64 mean and population std of the selected
+ * (signed) values -- the std fallback of topousm_fast_stat_func.
+ */
+enum { FSG_PCT_ABS = 1, FSG_PCT_FINITE = 2 };
+size_t fsg_percentile_workspace_size(int64_t n);
+int fsg_percentile(const float* values, int64_t n, int32_t flags,
+                   const double* q, int32_t nq, double* out, int64_t* n_valid,
+                   double* moments, void* workspace, size_t workspace_bytes,
+                   void* stream);
+
+/*
+ * Finalise device statistics in place for the normalisation epilogue:
+ *   kind 0 (topousm): stats[0] = p99 > 1e-9 ? p99 : (std > 1e-9 ? std : 1.0)
+ *   kind 1 (stretch): (lo, scale) = (p1, p99 - p1); scale 0 (stretch off) when
+ *                     not > 1e-9, as the pre-pass drops such stats
+ *                     (_norm_stats.py:288-289)
+ * pct holds the percentiles from fsg_percentile, moments its moments.
+ */
+int fsg_finalize_stats(int32_t kind, const double* pct, const double* moments,
+                       const int64_t* n_valid, double* stats, void* stream);
+
+/*
+ * Post-process + encode a float32 array region into an output: the
+ * drop-in for quantize_array / _quantize_block_cp
+ * (io/output_encoding.py:178-190, core/dask_processor.py:997-1008), the
+ * display stretch (_apply_display_stretch_block, tile/dask_bridge.py:173-187)
+ * and the normalisation divide (_global_stats.py:123-153).
+ */
+int fsg_encode(const fsg_window* src, const fsg_region* reg,
+               const fsg_output* out, void* stream);
+
+/* Gather the region of a window into a dense f32 pool (stats pre-pass). */
+int fsg_copy_region(const fsg_window* win, const fsg_region* reg, float* dst,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSG_H */

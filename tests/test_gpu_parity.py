@@ -1,0 +1,243 @@
+"""GPU parity: each HIP kernel (through the C ABI) against the CPU oracle.
+
+Tolerances (float32 outputs):
+* hillshade: bit-exact except where a NaN fill enters (|d| <= 1e-6);
+* slope (degrees): |d| <= 2e-5 + 1e-6*|v|  (atanf implementations differ by ulps);
+* curvature display value: |d| <= 4e-4, and |d| <= 2e-6 in the tanh domain
+  (one tanh ulp at saturation becomes 3.8e-4 after the 1/2.2 gamma);
+* topousm (normalised units): |d| <= 2e-4 (float32 box means of ~3000 m
+  elevations; one f32 ulp of a mean is ~2.4e-4 m);
+* openness / AO display values: |d| <= 1e-5.
+NaN masks must match exactly; integer encodings must equal the oracle's
+quantisation of the GPU float values bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _dem(h, w, seed, relief=3000.0, hurst=0.8, nan_discs=0):
+    from fujishadergpu_amd import synth
+    d = synth.fbm(h, w, hurst, relief, seed=seed, device="cpu")
+    if nan_discs:
+        rng = np.random.default_rng(seed + 99)
+        for _ in range(nan_discs):
+            cy, cx, r = int(rng.integers(0, h)), int(rng.integers(0, w)), int(rng.integers(2, 9))
+            yy, xx = np.ogrid[:h, :w]
+            m = (yy - cy) ** 2 + (xx - cx) ** 2 <= r * r
+            d[torch.from_numpy(m)] = float("nan")
+    return d
+
+
+def _cmp(name, got, exp, atol, rtol=0.0):
+    got = got.detach().cpu().numpy() if isinstance(got, torch.Tensor) else got
+    nm_g, nm_e = np.isnan(got), np.isnan(exp)
+    assert np.array_equal(nm_g, nm_e), f"{name}: NaN mask differs ({(nm_g != nm_e).sum()} px)"
+    d = np.abs(got - exp)[~nm_e]
+    lim = atol + rtol * np.abs(exp[~nm_e])
+    bad = d > lim
+    assert not bad.any(), (f"{name}: {bad.sum()} px over tolerance, max |d| = {d.max():.3g}")
+    return float(d.max()) if d.size else 0.0, float((d == 0).mean()) if d.size else 1.0
+
+
+@pytest.mark.parametrize("nan_discs", [0, 6])
+@pytest.mark.parametrize("psx,psy", [(10.0, -10.0), (None, None), (3.0, 7.5)])
+def test_hillshade_slope_window(cuda, nan_discs, psx, psy):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(131, 157, 3, nan_discs=nan_discs)
+    g = dem.to(cuda)
+    kw = dict(pixel_size=10.0, pixel_scale_x=psx, pixel_scale_y=psy)
+    hs = ALGORITHMS["hillshade"].process(g, **kw)
+    sl = ALGORITHMS["slope"].process(g, **kw)
+    a = dem.numpy()
+    mx, exact = _cmp("hillshade", hs, ref.hillshade(a, **kw), 1e-6)
+    if nan_discs == 0:
+        assert exact == 1.0, f"hillshade not bit-exact ({exact:.4f}, max {mx})"
+    _cmp("slope", sl, ref.slope(a, **kw), 2e-5, 1e-6)
+
+
+@pytest.mark.parametrize("z,az,alt,unit", [(2.5, 45.0, 30.0, "percent"), (1.0, 200.0, 60.0, "radian")])
+def test_hillshade_slope_params(cuda, z, az, alt, unit):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(64, 70, 5)
+    g = dem.to(cuda)
+    hs = ALGORITHMS["hillshade"].process(g, z_factor=z, azimuth=az, altitude=alt, pixel_size=5.0)
+    sl = ALGORITHMS["slope"].process(g, unit=unit, pixel_size=5.0)
+    a = dem.numpy()
+    _cmp("hillshade", hs, ref.hillshade(a, z_factor=z, azimuth=az, altitude=alt, pixel_size=5.0), 1e-6)
+    exp = ref.slope(a, unit=unit, pixel_size=5.0)
+    if unit == "percent":
+        # tan(atan(m)) * 100 has condition number ~m on steep cells: compare the
+        # angle atan(v / 100), where one ulp of atan/tan is 1e-7 rad.
+        got = sl.cpu().numpy()
+        _cmp("slope%", np.arctan(got.astype(np.float64) / 100.0),
+             np.arctan(exp.astype(np.float64) / 100.0), 1e-6)
+    else:
+        _cmp("slope", sl, exp, 2e-5, 2e-6)
+
+
+@pytest.mark.parametrize("ctype", ["mean", "gaussian", "planform", "profile"])
+@pytest.mark.parametrize("nan_discs", [0, 5])
+def test_curvature_window(cuda, ctype, nan_discs):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(97, 121, 7, relief=2500.0, hurst=0.85, nan_discs=nan_discs)
+    g = dem.to(cuda)
+    kw = dict(curvature_type=ctype, pixel_size=10.0, pixel_scale_x=10.0, pixel_scale_y=-10.0)
+    out = ALGORITHMS["curvature"].process(g, **kw).cpu().numpy()
+    exp = ref.curvature(dem.numpy(), **kw)
+    _cmp(f"curvature/{ctype}", out, exp, 4e-4)
+    # The display map v = ((t + 1) / 2) ** (1 / 2.2) turns one ulp of tanh at
+    # saturation (t = -1 vs -1 + 6e-8) into 3.8e-4 in v; in the tanh domain
+    # t = 2 v**2.2 - 1 the two implementations agree to 2e-6.
+    tg = 2.0 * np.power(out.astype(np.float64), 2.2) - 1.0
+    te = 2.0 * np.power(exp.astype(np.float64), 2.2) - 1.0
+    _cmp(f"curvature/{ctype}/tanh", tg, te, 2e-6)
+    d = np.abs(out - exp)[~np.isnan(exp)]
+    assert (d <= 1e-6).mean() >= 0.9, f"only {(d <= 1e-6).mean():.3f} within 1e-6"
+
+
+@pytest.mark.parametrize("radii,weights,px", [
+    ([2, 8, 32], [0.5, 0.3, 0.2], 5.0),
+    ([2, 8, 32, 128], [8 / 15, 4 / 15, 2 / 15, 1 / 15], 5.0),
+    ([1], [1.0], 10.0),
+    ([4, 64], None, 10.0),
+])
+@pytest.mark.parametrize("nan_discs", [0, 8])
+def test_topousm_window(cuda, radii, weights, px, nan_discs):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(389, 301, 11, relief=3776.0, hurst=0.75, nan_discs=nan_discs)
+    g = dem.to(cuda)
+    out = ALGORITHMS["topousm_fast"].process(g, radii=radii, weights=weights, pixel_size=px,
+                                             global_stats=(17.0,))
+    exp = ref.topousm_direct(dem.numpy(), radii=radii, weights=weights, pixel_size=px,
+                             stats=(17.0,))
+    _cmp("topousm", out, exp, 2e-4)
+
+
+def test_topousm_field_and_two_outputs(cuda):
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.algorithms import OutSpec, TopoUSMFastAlgorithm
+    from fujishadergpu_amd.encoding import quantize_params
+    H = W = 512
+    dem = _dem(H, W, 13, relief=3776.0, hurst=0.75)
+    g = dem.to(cuda)
+    coarse_np, scale = ref.overview_block_mean(dem.numpy(), sample_max=128)
+    field_np = ref.topousm_large_coarse_field(coarse_np, large_radii=[300, 512],
+                                              large_weights=[0.05, 0.02], decimation=scale)
+    coarse = ops.block_mean(g, int(scale))
+    field = ops.topousm_coarse_field(coarse, [max(1, int(round(r / scale))) for r in (300, 512)],
+                                     [0.05, 0.02])
+    _cmp("overview", coarse, coarse_np, 0.0)
+    _cmp("field", field, field_np, 2e-4)
+    # one tile window (rows 100..400, cols 60..460) with the field
+    wy, wx, wh, ww = 100, 60, 300, 400
+    win = g[wy:wy + wh, wx:wx + ww]
+    p = dict(radii=[2, 8, 32], weights=[0.4, 0.3, 0.2], pixel_size=5.0,
+             _topousm_fast_coarse_field=field, _topousm_fast_small_radii=[2, 8, 32],
+             _topousm_fast_small_weights=[0.4, 0.3, 0.2], _topousm_fast_w_large=0.07,
+             _topousm_fast_full_shape=(H, W), _topousm_fast_field_offset=(wy, wx),
+             global_stats=(9.0,))
+    algo = TopoUSMFastAlgorithm()
+    qp = quantize_params(-1.176, 1.176, "uint8")
+    f32 = torch.empty((wh, ww), dtype=torch.float32, device=cuda)
+    u8 = torch.empty((wh, ww), dtype=torch.uint8, device=cuda)
+    algo.run_region(win, ops.full_region(win), OutSpec(f32=f32, q=u8, qp=qp), p)
+    exp = ref.topousm_direct(dem.numpy()[wy:wy + wh, wx:wx + ww], radii=[2, 8, 32],
+                             weights=[0.4, 0.3, 0.2], pixel_size=5.0, coarse_field=field_np,
+                             small_radii=[2, 8, 32], small_weights=[0.4, 0.3, 0.2],
+                             w_large=0.07, field_offset=(wy, wx), full_shape=(H, W),
+                             stats=(9.0,))
+    _cmp("topousm+field", f32, exp, 2e-4)
+    dn = ref.quantize(f32.cpu().numpy(), qp["a_coef"], qp["b_coef"], qp["dn_min"], qp["dn_max"],
+                      "uint8")
+    assert np.array_equal(u8.cpu().numpy(), dn)
+
+
+@pytest.mark.parametrize("otype", ["positive", "negative"])
+@pytest.mark.parametrize("nan_discs", [0, 6])
+def test_openness_window(cuda, otype, nan_discs):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(150, 170, 17, relief=2500.0, hurst=0.85, nan_discs=nan_discs)
+    g = dem.to(cuda)
+    kw = dict(openness_type=otype, num_directions=8, max_distance=64, pixel_size=10.0,
+              pixel_scale_x=10.0, pixel_scale_y=-10.0)
+    out = ALGORITHMS["openness"].process(g, **kw)
+    _cmp(f"openness/{otype}", out, ref.openness(dem.numpy(), **kw), 1e-5)
+    st = (0.62, 0.21)
+    out2 = ALGORITHMS["openness"].process(g, global_stats=st, **kw)
+    _cmp("openness+stretch", out2, ref.display_stretch(ref.openness(dem.numpy(), **kw), st), 1e-4)
+
+
+@pytest.mark.parametrize("nan_discs", [0, 6])
+def test_ambient_occlusion_window(cuda, nan_discs):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(120, 140, 19, nan_discs=nan_discs)
+    g = dem.to(cuda)
+    kw = dict(num_samples=16, radius=10.0, intensity=1.0, pixel_size=10.0)
+    out = ALGORITHMS["ambient_occlusion"].process(g, **kw)
+    _cmp("ao", out, ref.ambient_occlusion(dem.numpy(), **kw), 1e-5)
+
+
+@pytest.mark.parametrize("n", [1, 2, 1000, 123457])
+def test_percentile_matches_numpy(cuda, n):
+    from fujishadergpu_amd import ops
+    rng = np.random.default_rng(n)
+    v = (rng.standard_normal(n) * 7.0).astype(np.float32)
+    if n > 1:
+        v[rng.integers(0, n, size=max(1, n // 50))] = np.nan
+    t = torch.from_numpy(v).to(cuda)
+    pct, nval, mom = ops.percentile(t, [99.0], ops.PCT_ABS)
+    fin = v[~np.isnan(v)]
+    assert int(nval.item()) == fin.size
+    exp = np.percentile(np.abs(fin).astype(np.float64), 99.0)
+    assert abs(pct.item() - exp) <= 1e-12 * max(1.0, abs(exp))
+    pct2, _, _ = ops.percentile(t, [1.0, 99.0], ops.PCT_FINITE)
+    exp2 = np.percentile(fin.astype(np.float64), [1.0, 99.0])
+    assert np.allclose(pct2.cpu().numpy(), exp2, rtol=1e-12, atol=1e-12)
+    assert abs(mom[0].item() - fin.astype(np.float64).mean()) < 1e-9 * max(1, abs(fin).max())
+
+
+def test_percentile_ties_and_reference_vectors(cuda):
+    from fujishadergpu_amd import ops
+    # tests/test_topousm_fast_normalization.py:27-36: 800 x 0.01 + 200 x 0.5 -> p99 = 0.5
+    data = np.concatenate([np.full(800, 0.01, np.float32), np.full(200, 0.5, np.float32)])
+    pct, _, mom = ops.percentile(torch.from_numpy(data).to(cuda), [99.0], ops.PCT_ABS)
+    stats = ops.finalize_stats(0, pct, mom, torch.tensor([1000], device=cuda))
+    assert stats[0].item() == pytest.approx(0.5, rel=1e-2)
+    # all-equal values: p99 is the value
+    pct, _, _ = ops.percentile(torch.full((4096,), 3.25, device=cuda), [99.0], ops.PCT_ABS)
+    assert pct.item() == 3.25
+
+
+def test_encode_quantize_bit_exact(cuda):
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.encoding import quantize_params
+    rng = np.random.default_rng(0)
+    v = np.concatenate([rng.uniform(-1.5, 1.5, 200000).astype(np.float32),
+                        np.array([np.inf, -np.inf, np.nan, 1.0, 0.0, -0.0], np.float32)])
+    # exact .5 ties after a*v+b
+    v = np.concatenate([v, (np.arange(-300, 300, dtype=np.float32) + 0.5) / np.float32(127.0)])
+    t = torch.from_numpy(v).to(cuda)
+    for lo, hi, dt in ((-1.176, 1.176, "uint8"), (-1.176, 1.176, "int16"), (0.0, 1.0, "uint8"),
+                       (0.0, 90.0, "int16")):
+        qp = quantize_params(lo, hi, dt)
+        got = ops.quantize(t, qp, dt).cpu().numpy()
+        exp = ref.quantize(v, qp["a_coef"], qp["b_coef"], qp["dn_min"], qp["dn_max"], dt)
+        assert np.array_equal(got, exp), (lo, hi, dt)
+    # tests/test_audit_p1_regressions.py:42-49
+    qp = {"a_coef": 100.0, "b_coef": 0.0, "dn_min": 0, "dn_max": 255}
+    got = ops.quantize(torch.tensor([np.inf, -np.inf, np.nan, 1.0], device=cuda), qp, "uint8")
+    assert got.cpu().tolist() == [0, 0, 0, 100]
+
+
+def test_nodata_to_nan_and_counts(cuda):
+    from fujishadergpu_amd import ops
+    data = torch.tensor([[1.0, -9999.0, 5.0], [float("nan"), 2.0, -9998.9]], device=cuda)
+    counts = ops.nodata_to_nan(data, -9999.0, [(0, 0, 2, 3), (0, 1, 2, 2)])
+    mask = torch.isnan(data).cpu().numpy()
+    assert mask.tolist() == [[False, True, False], [True, False, False]]
+    assert counts.cpu().tolist() == [2, 1]

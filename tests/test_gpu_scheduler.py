@@ -1,0 +1,125 @@
+"""GPU parity of the device-resident tile pipeline against the oracle tile loop.
+
+The oracle side runs the reference's per-tile semantics on the CPU (full
+window compute, core crop, NoData restore, encoding) over the same tile grid;
+the GPU side computes only each core, straight into the output raster.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _raster(h, w, seed, nodata_discs=0, relief=3000.0, hurst=0.8):
+    from fujishadergpu_amd import synth
+    d = synth.fbm(h, w, hurst, relief, seed=seed, device="cpu")
+    if nodata_discs:
+        discs = synth.nodata_discs(h, w, 0.01, seed=seed + 1)[:nodata_discs]
+        synth.apply_discs(d, discs, -9999.0)
+    return d
+
+
+def _oracle_tiles(host, P, algorithm, nodata, stats=None, field=None):
+    """Reference tile loop (core/tile_processor.py:822-1024) with the oracle."""
+    H, W = host.shape
+    src = host.copy()
+    if nodata is not None:
+        src[ref.nodata_mask(src, nodata)] = np.nan
+    out = np.full((H, W), np.nan, np.float32)
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+        win = src[wy:wy + wh, wx:wx + ww]
+        p = {k: v for k, v in P.params.items() if not isinstance(v, torch.Tensor)}
+        if stats is not None:
+            p["global_stats"] = stats
+        if field is not None:
+            p["_topousm_fast_coarse_field"] = field
+            p["_topousm_fast_field_offset"] = (wy, wx)
+        res = ref.run_window(algorithm, win, p)
+        core = res[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw]
+        out[cy:cy + ch, cx:cx + cw] = ref.format_output(core, algorithm)
+    return out
+
+
+@pytest.mark.parametrize("algorithm,kw", [
+    ("hillshade", dict(mode="local")),
+    ("slope", dict(mode="local")),
+    ("curvature", dict(mode="local", curvature_type="mean")),
+    ("openness", dict(mode="local", num_directions=8, max_distance=32)),
+    ("ambient_occlusion", dict(mode="local")),
+])
+@pytest.mark.parametrize("nodata_discs", [0, 12])
+def test_scheduler_matches_oracle_tile_loop(cuda, algorithm, kw, nodata_discs):
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _raster(700, 900, 21, nodata_discs=nodata_discs)
+    g = host_t.to(cuda)
+    nodata = -9999.0 if nodata_discs else None
+    out, results, P = process_dem(g, algorithm, tile_size=256, pixel_size=10.0, nodata=nodata,
+                                  return_plan=True, **kw)
+    assert all(r.success for r in results)
+    stats = None
+    if P.stats is not None:
+        stats = tuple(float(x) for x in P.stats.cpu().numpy())
+    exp = _oracle_tiles(host_t.numpy(), P, algorithm, nodata, stats=stats)
+    got = out.cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(exp))
+    m = ~np.isnan(exp)
+    tol = {"hillshade": 1e-6, "slope": 3e-5, "curvature": 4e-4, "openness": 1e-4,
+           "ambient_occlusion": 1e-4}[algorithm]
+    assert np.abs(got[m] - exp[m]).max() <= tol
+
+
+def test_scheduler_topousm_overview_prepass_and_uint8(cuda):
+    from fujishadergpu_amd.scheduler import process_dem
+    H, W = 1100, 1300
+    host_t = _raster(H, W, 23, relief=3776.0, hurst=0.75)
+    g = host_t.to(cuda)
+    radii, weights = [2, 8, 32, 300], [0.4, 0.3, 0.2, 0.1]
+    f32 = torch.empty((H, W), dtype=torch.float32, device=cuda)
+    out, results, P = process_dem(g, "topousm_fast", tile_size=512, pixel_size=5.0,
+                                  mode="spatial", radii=radii, weights=weights,
+                                  output_dtype="uint8", out_f32=f32, return_plan=True)
+    assert P.extras["large"][0] == [300]
+    assert P.extras["large"][1] == pytest.approx([0.1])
+    assert P.padding == 64  # max(32 + 16, 50) -> 64
+    host = host_t.numpy()
+    # overview field (scale 1: no decimation below 2048 px)
+    coarse, scale = ref.overview_block_mean(host)
+    field = ref.topousm_large_coarse_field(coarse, large_radii=[300], large_weights=[0.1],
+                                           decimation=scale)
+    # oracle pre-pass over the same stratified windows, full radii
+    from fujishadergpu_amd.params import norm_stat_window_geometry, stratified_windows
+    margin, tile = norm_stat_window_geometry("topousm_fast", {"radii": radii})
+    pool = []
+    for wy, wx, tw, th in stratified_windows(W, H, 0, H, 0, W, grid=3, tile=min(tile, max(W, H))):
+        raw = ref.topousm_fast_block(host[wy:wy + th, wx:wx + tw], radii=radii, weights=weights,
+                                     pixel_size=5.0)
+        m = int(min(margin, th // 3, tw // 3))
+        raw = raw[m:-m, m:-m] if m > 0 else raw
+        pool.append(raw[~np.isnan(raw)])
+    stats = ref.topousm_stat(np.concatenate(pool))
+    assert float(P.stats[0]) == pytest.approx(stats[0], rel=1e-6)
+    exp = _oracle_tiles(host, P, "topousm_fast", None, stats=(float(P.stats[0]),), field=field)
+    got = f32.cpu().numpy()
+    assert np.abs(got - exp).max() <= 2e-4
+    qp = P.qp
+    dn = ref.quantize(got, qp["a_coef"], qp["b_coef"], qp["dn_min"], qp["dn_max"], "uint8")
+    assert np.array_equal(out.cpu().numpy(), dn)
+
+
+def test_scheduler_skips_all_nodata_tiles(cuda):
+    """A window (core + halo) that is entirely NoData is skipped
+    (core/tile_processor.py:864-874); NoData there reads as DN 0."""
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _raster(300, 600, 29)
+    host_t[:, :214] = -9999.0   # covers the windows of tile columns 0 (cols 0..213)
+    out, results, P = process_dem(host_t.to(cuda), "hillshade", tile_size=150, pixel_size=10.0,
+                                  nodata=-9999.0, mode="local", output_dtype="uint8",
+                                  return_plan=True)
+    assert P.padding == 64
+    skipped = sorted((r.tile_y, r.tile_x) for r in results if r.skipped_reason)
+    assert skipped == [(0, 0), (1, 0)]
+    assert int(out[:, :150].max()) == 0
+    assert int(out[:, 214:].min()) >= 1
