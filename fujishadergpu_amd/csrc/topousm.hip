@@ -320,6 +320,13 @@ __device__ __forceinline__ int reflf(int i, int n) {
   return (i >= 0 && i < n) ? i : refl(i, n);
 }
 
+// Row pointer of a block-uniform row index: the index is forced into a scalar
+// register so loads become (scalar row base)[per-lane column offset].
+__device__ __forceinline__ const float* urow(const float* base, int row, int64_t ld) {
+  const int r = __builtin_amdgcn_readfirstlane(row);
+  return base + (int64_t)r * ld;
+}
+
 __host__ __device__ inline void src_axis(const SrcRef& S, int row, int idx, int& i0, int& i1,
                                          double& t) {
   const int n = row ? S.h : S.w;
@@ -362,10 +369,12 @@ __global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
   float* Ws = Vs + NR * K * LDC;                     // [NR][K][LDC] validity means
   float* Zs = NANAW ? (Ws + NR * K * LDC) : Ws;      // [K][LDC] centre values / results
   float* Src = Zs + K * LDC;                         // staging tiles of bilinear sources
+  // bilinear coefficient tables: per source, K row entries and SW column entries
+  int2* RowI = (int2*)(Src + A.src_floats);
+  double* RowT = (double*)(RowI + MAX_SRC * K);
+  int2* ColI = (int2*)(RowT + MAX_SRC * K);
+  double* ColT = (double*)(ColI + A.n_src * A.SW);
 
-  __shared__ int s_kind[FSG_MAX_RADII], s_slot[FSG_MAX_RADII];
-  __shared__ float s_wt[FSG_MAX_RADII];
-  __shared__ SrcRef s_src[MAX_SRC];
   __shared__ int s_clo[MAX_SRC], s_ncol[MAX_SRC];
 
   const int t = threadIdx.x;
@@ -377,45 +386,58 @@ __global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
   const int cols = min(A.SW, A.rw - x0);
   if (rows <= 0 || cols <= 0) return;
   const int wx0 = A.c0 + x0, wy0 = A.r0 + y0;  // window coordinates
+  const int H = A.W.h;
+  const int64_t ld = A.W.ld;
 
-  if (t < FSG_MAX_RADII) {
-    s_kind[t] = A.kind[t];
-    s_slot[t] = A.slot[t];
-    s_wt[t] = A.wt[t];
-  }
-  if (t < A.n_src) {
-    const SrcRef S = A.src[t];
-    s_src[t] = S;
+  // per-source column ranges and column coefficient tables (once per block)
+  for (int s = 0; s < A.n_src; s++) {
+    const SrcRef S = A.src[s];
     int a0, a1, b0, b1;
     double tt;
     src_axis(S, 0, wx0, a0, a1, tt);
     src_axis(S, 0, wx0 + cols - 1, b0, b1, tt);
-    s_clo[t] = a0;
-    s_ncol[t] = min(b1 - a0 + 1, S.cols_max);
+    if (t == 0) {
+      s_clo[s] = a0;
+      s_ncol[s] = min(b1 - a0 + 1, S.cols_max);
+    }
+    // column indices relative to the staged tile's first column a0
+    for (int c = t; c < A.SW; c += 256) {
+      int i0, i1;
+      double tx;
+      src_axis(S, 0, wx0 + min(c, cols - 1), i0, i1, tx);
+      ColI[s * A.SW + c] = make_int2(i0 - a0, i1 - a0);
+      ColT[s * A.SW + c] = tx;
+    }
   }
 
-  // column owned in the vertical stage
+  // column owned in the vertical stage; every row address is block-uniform,
+  // so loads are (uniform row pointer)[per-lane column]
   const bool has_col = t < NC;
-  const int jw = refl(wx0 - A.R + t, A.W.w);
-  const float* colp = A.W.p + jw;
-  const int H = A.W.h;
-  const int64_t ld = A.W.ld;
+  const int jw = has_col ? refl(wx0 - A.R + t, A.W.w) : 0;
+  const float* Wp = A.W.p;
 
-  // running window sums for the row before the first output row
   double S[NRA];
   int Cn[NRA];
 #pragma unroll
   for (int i = 0; i < NR; i++) {
     double s = 0.0;
     int n = 0;
-    if (has_col) {
-      const int r = A.rad[i];
-      for (int d = -r; d <= r; d++) {
-        const float v = colp[(int64_t)reflf(wy0 - 1 + d, H) * ld];
+    const int r = A.rad[i];
+    // rows wy0-1-r .. wy0-1+r in batches of 8 independent loads (exact sums)
+    for (int d0 = -r; d0 <= r; d0 += 8) {
+      float v[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int d = min(d0 + e, r);
+        v[e] = urow(Wp, reflf(wy0 - 1 + d, H), ld)[jw];
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        if (d0 + e > r) break;
         if (NANAW) {
-          if (!isnan(v)) { s += (double)v; n++; }
+          if (!isnan(v[e])) { s += (double)v[e]; n++; }
         } else {
-          s += (double)v;
+          s += (double)v[e];
         }
       }
     }
@@ -432,19 +454,19 @@ __global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
     const int wyc = wy0 + ch * K;
     const int nrow = min(K, rows - ch * K);
     // ---------------- axis-0 stage: batched column loads, running sums
-    if (has_col) {
-      for (int kb = 0; kb < K; kb += KS) {
-        float vz[KS], vn[NRA][KS], vo[NRA][KS];
+    for (int kb = 0; kb < K; kb += KS) {
+      float vz[KS], vn[NRA][KS], vo[NRA][KS];
 #pragma unroll
-        for (int k = 0; k < KS; k++) {
-          const int yw = wyc + kb + k;
-          vz[k] = colp[(int64_t)min(yw, H - 1) * ld];
+      for (int k = 0; k < KS; k++) {
+        const int yw = wyc + kb + k;
+        vz[k] = urow(Wp, min(yw, H - 1), ld)[jw];
 #pragma unroll
-          for (int i = 0; i < NR; i++) {
-            vn[i][k] = colp[(int64_t)reflf(yw + A.rad[i], H) * ld];
-            vo[i][k] = colp[(int64_t)reflf(yw - A.rad[i] - 1, H) * ld];
-          }
+        for (int i = 0; i < NR; i++) {
+          vn[i][k] = urow(Wp, reflf(yw + A.rad[i], H), ld)[jw];
+          vo[i][k] = urow(Wp, reflf(yw - A.rad[i] - 1, H), ld)[jw];
         }
+      }
+      if (has_col) {
 #pragma unroll
         for (int k = 0; k < KS; k++) {
 #pragma unroll
@@ -463,9 +485,9 @@ __global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
         }
       }
     }
-    // ---------------- stage the bilinear sources for these rows
+    // ---------------- stage the bilinear sources and their row coefficients
     for (int s = 0; s < A.n_src; s++) {
-      const SrcRef& SR = s_src[s];
+      const SrcRef& SR = A.src[s];
       int a0, a1, b0, b1;
       double tt;
       src_axis(SR, 1, wyc, a0, a1, tt);
@@ -477,22 +499,29 @@ __global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
         const int rr = e / nc, cc = e - rr * nc;
         tile[rr * SR.cols_max + cc] = SR.p[(int64_t)(a0 + rr) * SR.ld + clo + cc];
       }
+      if (t < K) {
+        int i0, i1;
+        double ty;
+        src_axis(SR, 1, wyc + min(t, nrow - 1), i0, i1, ty);
+        RowI[s * K + t] = make_int2((i0 - a0) * SR.cols_max, (i1 - a0) * SR.cols_max);
+        RowT[s * K + t] = ty;
+      }
     }
     __syncthreads();
 
     // ---------------- axis-1 stage + combine
     {
       const int k = t / segs;
-      const int s = t % segs;
+      const int sg = t % segs;
       if (k < nrow) {
-        const int yw = wyc + k;
-        const int cbase = A.R + s * L;  // LDS column of this segment's first output
+        const int cbase = A.R + sg * L;  // LDS column of this segment's first output
         double Hs[NRA], HW[NRA];
 #pragma unroll
         for (int i = 0; i < NR; i++) {
           const float* vrow = Vs + (i * K + k) * LDC;
           const float* wrow = Ws + (i * K + k) * LDC;
           double h = 0.0, hw = 0.0;
+#pragma unroll 4
           for (int d = -A.rad[i]; d <= A.rad[i]; d++) {
             h += (double)vrow[cbase + d];
             if (NANAW) hw += (double)wrow[cbase + d];
@@ -500,7 +529,6 @@ __global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
           Hs[i] = h;
           HW[i] = hw;
         }
-        // per-source tile origins for this chunk (recomputed, uniform per row)
         for (int m = 0; m < L; m++) {
           const int lc = cbase + m;
           float bm[NRA];
@@ -523,62 +551,74 @@ __global__ void __launch_bounds__(256) k_usm_main(UsmArgs A) {
               bm[i] = fv;
             }
           }
-          const int xr = x0 + s * L + m;  // region column
-          float res = 0.0f;
-          if (xr < A.rw) {
-            const int xw = A.c0 + xr;
-            const float z = Zs[k * LDC + lc];
-            float acc = 0.0f;
-#pragma unroll 1
-            for (int q = 0; q < A.n_terms; q++) {
+          const int cl = sg * L + m;      // strip column
+          const float z = Zs[k * LDC + lc];
+          float acc = 0.0f;
+          // term table unrolled with compile-time indices: kind/slot/weight
+          // come from the kernel arguments (scalar registers)
+#pragma unroll
+          for (int q = 0; q < FSG_MAX_RADII; q++) {
+            if (q < A.n_terms) {
               float mean = 0.0f;
-              const int kd = s_kind[q];
+              const int kd = A.kind[q];
               if (kd == TERM_BOX) {
-                const int sl = s_slot[q];
 #pragma unroll
                 for (int i = 0; i < NR; i++)
-                  if (i == sl) mean = bm[i];
+                  if (i == A.slot[q]) mean = bm[i];
               } else if (GAUSS && kd == TERM_GAUSS) {
-                mean = gauss9_mean(A.W.p, A.W.ld, A.W.h, A.W.w, yw, xw, A.G);
+                mean = gauss9_mean(A.W.p, A.W.ld, A.W.h, A.W.w, wyc + k, A.c0 + x0 + cl, A.G);
               } else {
-                const int si = s_slot[q];
-                const SrcRef& SR = s_src[si];
-                int a0, a1;
-                double tt;
-                src_axis(SR, 1, wyc, a0, a1, tt);
-                mean = src_sample(SR, Src + SR.lds_off, a0, s_clo[si], yw, xw);
+                const int s = A.slot[q];
+                const int2 ri = RowI[s * K + k];
+                const double ty = RowT[s * K + k];
+                const int2 ci = ColI[s * A.SW + cl];
+                const double tx = ColT[s * A.SW + cl];
+                const float* tile = Src + A.src[s].lds_off;
+                const double a = tile[ri.x + ci.x], b = tile[ri.x + ci.y];
+                const double c = tile[ri.y + ci.x], d = tile[ri.y + ci.y];
+                const double top = a * (1.0 - tx) + b * tx;
+                const double bot = c * (1.0 - tx) + d * tx;
+                mean = (float)(top * (1.0 - ty) + bot * ty);
               }
-              const float diff = s_wt[q] * (z - mean);
+              const float diff = A.wt[q] * (z - mean);
               acc = (q == 0) ? diff : acc + diff;
             }
-            if (isnan(z)) acc = __builtin_nanf("");
-            if (A.field_src >= 0) {
-              const SrcRef& SR = s_src[A.field_src];
-              int a0, a1;
-              double tt;
-              src_axis(SR, 1, wyc, a0, a1, tt);
-              const float up = src_sample(SR, Src + SR.lds_off, a0, s_clo[A.field_src], yw, xw);
-              const float large = A.wl * z - up;
-              acc = (A.n_terms > 0) ? acc + large : large;
-            }
-            res = acc;
+          }
+          if (isnan(z)) acc = __builtin_nanf("");
+          if (A.field_src >= 0) {
+            const int s = A.field_src;
+            const int2 ri = RowI[s * K + k];
+            const double ty = RowT[s * K + k];
+            const int2 ci = ColI[s * A.SW + cl];
+            const double tx = ColT[s * A.SW + cl];
+            const float* tile = Src + A.src[s].lds_off;
+            const double a = tile[ri.x + ci.x], b = tile[ri.x + ci.y];
+            const double c = tile[ri.y + ci.x], d = tile[ri.y + ci.y];
+            const double top = a * (1.0 - tx) + b * tx;
+            const double bot = c * (1.0 - tx) + d * tx;
+            const float up = (float)(top * (1.0 - ty) + bot * ty);
+            const float large = A.wl * z - up;
+            acc = (A.n_terms > 0) ? acc + large : large;
           }
           // the thread's own Zs cell is read above only by this thread
-          Zs[k * LDC + lc] = res;
+          Zs[k * LDC + lc] = acc;
         }
       }
     }
     __syncthreads();
 
-    // ---------------- coalesced write of the chunk's results
-    const PostK p1 = load_post(A.o1), p2 = load_post(A.o2);
-    for (int idx = t; idx < nrow * A.SW; idx += 256) {
-      const int k = idx / A.SW, c = idx - k * A.SW;
-      if (c >= cols) continue;
-      const float v = Zs[k * LDC + A.R + c];
-      const int yr = y0 + ch * K + k, xr = x0 + c;
-      if (A.o1.data) store_out(A.o1, yr, xr, apply_post(p1, v));
-      if (A.o2.data) store_out(A.o2, yr, xr, apply_post(p2, v));
+    // ---------------- coalesced write: one region row per iteration
+    {
+      const PostK p1 = load_post(A.o1);
+      const bool two = A.o2.data != nullptr;
+      for (int k = 0; k < nrow; k++) {
+        if (t < cols) {
+          const float v = apply_post(p1, Zs[k * LDC + A.R + t]);
+          const int yr = y0 + ch * K + k, xr = x0 + t;
+          store_out(A.o1, yr, xr, v);
+          if (two) store_out(A.o2, yr, xr, v);
+        }
+      }
     }
     __syncthreads();
   }
@@ -588,8 +628,10 @@ template <int NR, bool NANAW>
 static int launch_usm(const UsmArgs& A, int n_blocks, hipStream_t s) {
   constexpr int K = (NR <= 3) ? 16 : 8;
   const int LDC = A.SW + 2 * A.R + 1;
-  const size_t lds = sizeof(float) * ((size_t)NR * K * LDC * (NANAW ? 2 : 1) + (size_t)K * LDC +
-                                      (size_t)A.src_floats);
+  size_t lds = sizeof(float) * ((size_t)NR * K * LDC * (NANAW ? 2 : 1) + (size_t)K * LDC +
+                                (size_t)A.src_floats);
+  lds = (lds + 15) & ~(size_t)15;
+  lds += (sizeof(int2) + sizeof(double)) * ((size_t)MAX_SRC * K + (size_t)A.n_src * A.SW);
   if (lds > 150 * 1024) return fail(FSG_E_UNSUPPORTED, "k_usm_main: LDS budget exceeded");
   bool gauss = false;
   for (int q = 0; q < A.n_terms; q++) gauss |= (A.kind[q] == TERM_GAUSS);

@@ -251,8 +251,9 @@ typedef struct fsg_topousm_params {
  * Replaces compute_topousm_fast_efficient_block (_impl_topousm_fast.py:49-100)
  * plus the large-radius part and normalisation of _direct_topousm_fast
  * (tile/dask_bridge.py:225-271).  Normalisation is out->post_kind = DIV.
- * out2 (optional) receives the same values in a second encoding (e.g. the
- * float32 and the uint8 product of one pass).
+ * out2 (optional) receives the same post-processed values (out's post_kind
+ * and statistics apply to both) in a second encoding, e.g. the float32 and the
+ * uint8 product of one pass.
  */
 size_t fsg_topousm_workspace_size(const fsg_window* win,
                                   const fsg_topousm_params* p);
