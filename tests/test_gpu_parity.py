@@ -105,6 +105,10 @@ def test_curvature_window(cuda, ctype, nan_discs):
     ([2, 8, 32, 128], [8 / 15, 4 / 15, 2 / 15, 1 / 15], 5.0),
     ([1], [1.0], 10.0),
     ([4, 64], None, 10.0),
+    # unsorted: a box radius after a decimated one (general kernel)
+    ([32, 2, 128, 8], [0.2, 0.4, 0.1, 0.3], 5.0),
+    # decimated radii only (no full-resolution box term)
+    ([128, 300], [0.6, 0.4], 5.0),
 ])
 @pytest.mark.parametrize("nan_discs", [0, 8])
 def test_topousm_window(cuda, radii, weights, px, nan_discs):
