@@ -196,10 +196,10 @@ __global__ void k_voidfill_apply(float* __restrict__ coarse, const float* __rest
 //   pass 0: x (h x w, NaN-aware)  -> tv^T, tw^T (w x h): float32 axis-0 means
 //   pass 1: tv^T, tw^T (w x h)    -> out (h x w): axis-1 means of those and
 //           where(sw > 0, sv / sw, 0)   (_nan_utils.py:34-47 structure)
+template <int PASS>
 __global__ void __launch_bounds__(64)
 k_box_t(const float* __restrict__ a, const float* __restrict__ b, int64_t lda, int h, int w,
-        int r, double inv, float* __restrict__ oa, float* __restrict__ ob, int64_t ldo,
-        int pass) {
+        int r, double inv, float* __restrict__ oa, float* __restrict__ ob, int64_t ldo) {
   __shared__ float ta[64][65];
   __shared__ float tb[64][65];
   const int lane = threadIdx.x;
@@ -208,36 +208,62 @@ k_box_t(const float* __restrict__ a, const float* __restrict__ b, int64_t lda, i
   const int y1 = min(y0 + 64, h);
   if (c < w) {
     const float* pa = a + c;
-    const float* pb = b ? b + c : nullptr;
+    const float* pb = PASS ? b + c : nullptr;
     double s = 0.0, sw = 0.0;
-    for (int d = -r; d <= r; d++) {
-      const int64_t o = (int64_t)refl(y0 + d, h) * lda;
-      if (pass == 0) {
-        const float v = pa[o];
-        if (!isnan(v)) { s += (double)v; sw += 1.0; }
-      } else {
-        s += (double)pa[o];
-        sw += (double)pb[o];
+    // window rows y0-r .. y0+r, 16 independent loads per batch
+    for (int d0 = -r; d0 <= r; d0 += 16) {
+      float va[16], vb[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int64_t o = (int64_t)refl(y0 + min(d0 + e, r), h) * lda;
+        va[e] = pa[o];
+        if (PASS) vb[e] = pb[o];
       }
-    }
-    for (int y = y0; y < y1; y++) {
-      if (y > y0) {
-        const int64_t on = (int64_t)refl(y + r, h) * lda, oo = (int64_t)refl(y - r - 1, h) * lda;
-        if (pass == 0) {
-          const float vn = pa[on], vo = pa[oo];
-          if (!isnan(vn)) { s += (double)vn; sw += 1.0; }
-          if (!isnan(vo)) { s -= (double)vo; sw -= 1.0; }
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        if (d0 + e > r) break;
+        if (PASS == 0) {
+          if (!isnan(va[e])) { s += (double)va[e]; sw += 1.0; }
         } else {
-          s += (double)pa[on] - (double)pa[oo];
-          sw += (double)pb[on] - (double)pb[oo];
+          s += (double)va[e];
+          sw += (double)vb[e];
         }
       }
-      const float fv = (float)(s * inv), fw = (float)(sw * inv);
-      if (pass == 0) {
-        ta[y - y0][lane] = fv;
-        tb[y - y0][lane] = fw;
-      } else {
-        ta[y - y0][lane] = fw > 0.0f ? fv / fw : 0.0f;
+    }
+    // running sums down the block's rows, 8 rows of loads per batch
+    for (int yb = y0; yb < y1; yb += 8) {
+      float na[8], oa_[8], nb[8], obv[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int y = min(yb + e, y1 - 1);
+        const int64_t on = (int64_t)refl(y + r, h) * lda, oo = (int64_t)refl(y - r - 1, h) * lda;
+        na[e] = pa[on];
+        oa_[e] = pa[oo];
+        if (PASS) {
+          nb[e] = pb[on];
+          obv[e] = pb[oo];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int y = yb + e;
+        if (y >= y1) break;
+        if (y > y0) {
+          if (PASS == 0) {
+            if (!isnan(na[e])) { s += (double)na[e]; sw += 1.0; }
+            if (!isnan(oa_[e])) { s -= (double)oa_[e]; sw -= 1.0; }
+          } else {
+            s += (double)na[e] - (double)oa_[e];
+            sw += (double)nb[e] - (double)obv[e];
+          }
+        }
+        const float fv = (float)(s * inv), fw = (float)(sw * inv);
+        if (PASS == 0) {
+          ta[y - y0][lane] = fv;
+          tb[y - y0][lane] = fw;
+        } else {
+          ta[y - y0][lane] = fw > 0.0f ? fv / fw : 0.0f;
+        }
       }
     }
   }
@@ -249,7 +275,7 @@ k_box_t(const float* __restrict__ a, const float* __restrict__ b, int64_t lda, i
     if (y < y1) {
       const int64_t o = (int64_t)(blockIdx.x * 64 + cc) * ldo + y;
       oa[o] = ta[lane][cc];
-      if (pass == 0) ob[o] = tb[lane][cc];
+      if (PASS == 0) ob[o] = tb[lane][cc];
     }
   }
 }
@@ -311,7 +337,7 @@ struct UsmArgs {
   SrcRef src[MAX_SRC];
   int field_src;            // source index of the overview field, or -1
   float wl;                 // f32(w_large)
-  int src_floats;           // LDS floats of all staging tiles
+  int src_floats;           // LDS entries of all staging tiles (float64 in k_usm_fast)
   Gauss9 G;
   OutDev o1, o2;
 };
@@ -679,6 +705,11 @@ static int dispatch_usm(const UsmArgs& A, bool nanaw, int n_blocks, hipStream_t 
 constexpr int FAST_SH = 64;
 constexpr int FAST_K = 8;
 
+// scipy 'reflect' for an index at most n outside [0, n)
+__device__ __forceinline__ int refl1(int i, int n) {
+  return i < 0 ? -1 - i : (i >= n ? 2 * n - 1 - i : i);
+}
+
 template <int NR, bool NANAW>
 __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int per_xcd) {
   constexpr int K = FAST_K;
@@ -690,14 +721,12 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
   float* Vs = lds;                                   // [NR][K][LDC]
   float* Ws = Vs + NR * K * LDC;                     // [NR][K][LDC] (NaN-aware only)
   float* Zs = NANAW ? (Ws + NR * K * LDC) : Ws;      // [K][LDC] centre -> partial sum
-  float* Src = Zs + K * LDC;
-  // 8-byte coefficient tables from a 16-byte boundary: per source K row
-  // entries, then per source SW column entries
-  int2* RowI = (int2*)(lds + ((NR * K * LDC * (NANAW ? 2 : 1) + K * LDC + A.src_floats + 3) & ~3));
-  double* RowT = (double*)(RowI + MAX_SRC * K);
-  int2* ColI = (int2*)(RowT + MAX_SRC * K);
-  double* ColT = (double*)(ColI + A.n_src * A.SW);
-  __shared__ int s_clo[MAX_SRC], s_ncol[MAX_SRC];
+  // float64 tables from a 16-byte boundary: per source, rows_max staged
+  // coarse rows of horizontally interpolated values [row][SW]; then per source
+  // K row entries (staged-row offsets, ty, 1 - ty)
+  double* Hs_t = (double*)(lds + ((NR * K * LDC * (NANAW ? 2 : 1) + K * LDC + 3) & ~3));
+  int2* RowI = (int2*)(Hs_t + A.src_floats);
+  double2* RowT = (double2*)(RowI + MAX_SRC * K);
 
   // XCD-aware order: consecutive tiles of one band land on one XCD (L2)
   const int b = blockIdx.x;
@@ -717,26 +746,6 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
   const float* Wp = A.W.p;
   const int nz = A.n_src - (A.field_src >= 0 ? 1 : 0);  // decimated terms
 
-  // staged-tile column range and column coefficients per source (once)
-  for (int s = 0; s < A.n_src; s++) {
-    const SrcRef& S = A.src[s];
-    int a0, a1, b0, b1;
-    double tt;
-    src_axis(S, 0, wx0, a0, a1, tt);
-    if (t == 0) {
-      src_axis(S, 0, wx0 + cols - 1, b0, b1, tt);
-      s_clo[s] = a0;
-      s_ncol[s] = min(b1 - a0 + 1, S.cols_max);
-    }
-    for (int c = t; c < A.SW; c += 256) {
-      int i0, i1;
-      double tx;
-      src_axis(S, 0, wx0 + min(c, cols - 1), i0, i1, tx);
-      ColI[s * A.SW + c] = make_int2(i0 - a0, i1 - a0);
-      ColT[s * A.SW + c] = tx;
-    }
-  }
-
   // vertical running sums over rows wy0-1-r .. wy0-1+r (16 loads in flight)
   const bool has_col = t < NC;
   const int jw = has_col ? refl(wx0 - A.R + t, A.W.w) : 0;
@@ -751,7 +760,7 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
       float v[16];
 #pragma unroll
       for (int e = 0; e < 16; e++)
-        v[e] = urow(Wp, reflf(wy0 - 1 + min(d0 + e, r), H), ld)[jw];
+        v[e] = urow(Wp, refl1(wy0 - 1 + min(d0 + e, r), H), ld)[jw];
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         if (d0 + e > r) break;
@@ -769,6 +778,7 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
   const int segs = 256 / K;
   const int L = A.SW / segs;
   const int n_chunks = (rows + K - 1) / K;
+  const int tc = min(t, cols - 1);  // this lane's output column (clamped)
 
   for (int ch = 0; ch < n_chunks; ch++) {
     const int wyc = wy0 + ch * K;
@@ -784,8 +794,8 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
         vz[k] = urow(Wp, min(yw, H - 1), ld)[jw];
 #pragma unroll
         for (int i = 0; i < NR; i++) {
-          vn[i][k] = urow(Wp, reflf(yw + A.rad[i], H), ld)[jw];
-          vo[i][k] = urow(Wp, reflf(yw - A.rad[i] - 1, H), ld)[jw];
+          vn[i][k] = urow(Wp, refl1(yw + A.rad[i], H), ld)[jw];
+          vo[i][k] = urow(Wp, refl1(yw - A.rad[i] - 1, H), ld)[jw];
         }
       }
       if (has_col) {
@@ -807,7 +817,8 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
         }
       }
     }
-    // ---------------- staging of the bilinear sources for this chunk
+    // ---------------- staging: horizontal lerps of the coarse rows this
+    // chunk samples, top/bot = a * (1 - tx) + b * tx per (coarse row, column)
     for (int s = 0; s < A.n_src; s++) {
       const SrcRef& SR = A.src[s];
       int a0, a1, b0, b1;
@@ -815,18 +826,25 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
       src_axis(SR, 1, wyc, a0, a1, tt);
       src_axis(SR, 1, wyc + nrow - 1, b0, b1, tt);
       const int nr = min(b1 - a0 + 1, SR.rows_max);
-      const int nc = s_ncol[s], clo = s_clo[s];
-      float* tl = Src + SR.lds_off;
-      for (int e = t; e < nr * nc; e += 256) {
-        const int rr = e / nc, cc = e - rr * nc;
-        tl[rr * SR.cols_max + cc] = SR.p[(int64_t)(a0 + rr) * SR.ld + clo + cc];
+      if (t < A.SW) {
+        int i0, i1;
+        double tx;
+        src_axis(SR, 0, wx0 + tc, i0, i1, tx);
+        const double mx = 1.0 - tx;
+        double* hrow = Hs_t + SR.lds_off + t;
+        const float* prow = SR.p + (int64_t)a0 * SR.ld;
+        for (int rr = 0; rr < nr; rr++) {
+          const double va = prow[i0], vb = prow[i1];
+          hrow[rr * A.SW] = va * mx + vb * tx;
+          prow += SR.ld;
+        }
       }
       if (t < K) {
         int i0, i1;
         double ty;
         src_axis(SR, 1, wyc + min(t, nrow - 1), i0, i1, ty);
-        RowI[s * K + t] = make_int2((i0 - a0) * SR.cols_max, (i1 - a0) * SR.cols_max);
-        RowT[s * K + t] = ty;
+        RowI[s * K + t] = make_int2((i0 - a0) * A.SW, (i1 - a0) * A.SW);
+        RowT[s * K + t] = make_double2(ty, 1.0 - ty);
       }
     }
     __syncthreads();
@@ -837,7 +855,7 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
       const int sg = t % segs;
       if (k < nrow) {
         const int cbase = A.R + sg * L;
-        double Hs[NRA], HW[NRA];
+        double Hsum[NRA], HW[NRA];
 #pragma unroll
         for (int i = 0; i < NR; i++) {
           const float* vrow = Vs + (i * K + k) * LDC + cbase;
@@ -849,7 +867,7 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
             h += (double)vrow[d];
             if (NANAW) hw += (double)wrow[d];
           }
-          Hs[i] = h;
+          Hsum[i] = h;
           HW[i] = hw;
         }
         for (int m = 0; m < L; m++) {
@@ -861,13 +879,13 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
             const int r = A.rad[i];
             if (m > 0) {
               const float* vrow = Vs + (i * K + k) * LDC;
-              Hs[i] += (double)vrow[lc + r] - (double)vrow[lc - r - 1];
+              Hsum[i] += (double)vrow[lc + r] - (double)vrow[lc - r - 1];
               if (NANAW) {
                 const float* wrow = Ws + (i * K + k) * LDC;
                 HW[i] += (double)wrow[lc + r] - (double)wrow[lc - r - 1];
               }
             }
-            const float fv = (float)(Hs[i] * A.inv[i]);
+            const float fv = (float)(Hsum[i] * A.inv[i]);
             float bm;
             if (NANAW) {
               const float fw = (float)(HW[i] * A.inv[i]);
@@ -899,15 +917,9 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
           float acc = (NR > 0) ? Zs[k * LDC + A.R + t] : 0.0f;
           for (int j = 0; j < nz; j++) {
             const int2 ri = RowI[j * K + k];
-            const double ty = RowT[j * K + k];
-            const int2 ci = ColI[j * A.SW + t];
-            const double tx = ColT[j * A.SW + t];
-            const float* tl = Src + A.src[j].lds_off;
-            const double a = tl[ri.x + ci.x], bb = tl[ri.x + ci.y];
-            const double c = tl[ri.y + ci.x], d = tl[ri.y + ci.y];
-            const double top = a * (1.0 - tx) + bb * tx;
-            const double bot = c * (1.0 - tx) + d * tx;
-            const float mean = (float)(top * (1.0 - ty) + bot * ty);
+            const double2 ty = RowT[j * K + k];
+            const double* h = Hs_t + A.src[j].lds_off + t;
+            const float mean = (float)(h[ri.x] * ty.y + h[ri.y] * ty.x);
             const float diff = A.wt[NR + j] * (z - mean);
             acc = (NR == 0 && j == 0) ? diff : acc + diff;
           }
@@ -915,15 +927,9 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
           if (A.field_src >= 0) {
             const int s = A.field_src;
             const int2 ri = RowI[s * K + k];
-            const double ty = RowT[s * K + k];
-            const int2 ci = ColI[s * A.SW + t];
-            const double tx = ColT[s * A.SW + t];
-            const float* tl = Src + A.src[s].lds_off;
-            const double a = tl[ri.x + ci.x], bb = tl[ri.x + ci.y];
-            const double c = tl[ri.y + ci.x], d = tl[ri.y + ci.y];
-            const double top = a * (1.0 - tx) + bb * tx;
-            const double bot = c * (1.0 - tx) + d * tx;
-            const float up = (float)(top * (1.0 - ty) + bot * ty);
+            const double2 ty = RowT[s * K + k];
+            const double* h = Hs_t + A.src[s].lds_off + t;
+            const float up = (float)(h[ri.x] * ty.y + h[ri.y] * ty.x);
             const float large = A.wl * z - up;
             acc = (A.n_terms > 0) ? acc + large : large;
           }
@@ -940,11 +946,10 @@ __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int pe
 
 static size_t fast_lds_bytes(const UsmArgs& A, bool nanaw) {
   const int LDC = A.SW + 2 * A.R + 1;
-  size_t lds = sizeof(float) * ((size_t)A.nr * FAST_K * LDC * (nanaw ? 2 : 1) +
-                                (size_t)FAST_K * LDC + (size_t)A.src_floats);
-  lds = (lds + 15) & ~(size_t)15;
-  return lds + (sizeof(int2) + sizeof(double)) * ((size_t)MAX_SRC * FAST_K +
-                                                   (size_t)A.n_src * A.SW);
+  size_t lds = sizeof(float) * (((size_t)A.nr * FAST_K * LDC * (nanaw ? 2 : 1) +
+                                 (size_t)FAST_K * LDC + 3) & ~(size_t)3);
+  lds += sizeof(double) * (size_t)A.src_floats;  // staged rows (float64)
+  return lds + (sizeof(int2) + sizeof(double2)) * (size_t)MAX_SRC * FAST_K;
 }
 
 template <int NR, bool NANAW>
@@ -1071,12 +1076,12 @@ static int coarse_mean(const float* x, int64_t ldx, int h, int w, int r, float* 
   }
   const double inv = 1.0 / (double)(2 * r + 1);
   // pass 0: (h x w) -> transposed (w x h) axis-0 means
-  hipLaunchKernelGGL(k_box_t, dim3((w + 63) / 64, (h + 63) / 64), dim3(64), 0, s, x, nullptr,
-                     ldx, h, w, r, inv, tv, tw, (int64_t)h, 0);
+  hipLaunchKernelGGL(k_box_t<0>, dim3((w + 63) / 64, (h + 63) / 64), dim3(64), 0, s, x,
+                     nullptr, ldx, h, w, r, inv, tv, tw, (int64_t)h);
   FSG_CHECK_LAUNCH("k_box_t");
   // pass 1: axis 0 of the transposed grid (= axis 1), transposed back
-  hipLaunchKernelGGL(k_box_t, dim3((h + 63) / 64, (w + 63) / 64), dim3(64), 0, s, tv, tw,
-                     (int64_t)h, w, h, r, inv, out, nullptr, ldo, 1);
+  hipLaunchKernelGGL(k_box_t<1>, dim3((h + 63) / 64, (w + 63) / 64), dim3(64), 0, s, tv, tw,
+                     (int64_t)h, w, h, r, inv, out, nullptr, ldo);
   FSG_CHECK_LAUNCH("k_box_t");
   return FSG_OK;
 }
@@ -1214,7 +1219,9 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
   // radii]: the order of every ascending radius list.  Other orders (a sigma=1
   // Gaussian term, a repeated radius, a decimated radius before a box radius)
   // use the general kernel.  Both accumulate in term order.
-  bool fast = true;
+  // (single reflection of window rows: the last chunk's rows reach K - 1
+  // past the region, so the window must hold at least R + K rows)
+  bool fast = A.R + FAST_K <= win->h;
   for (int q = 0; q < A.n_terms; q++)
     fast = fast && (q < A.nr ? (A.kind[q] == TERM_BOX && A.slot[q] == q)
                              : A.kind[q] == TERM_COARSE);
@@ -1256,7 +1263,7 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
     S.zx = (win->w != 1) ? (double)(P.wc[j] - 1) / (double)(win->w - 1) : 1.0;
     src_bounds(S, reg->r0, reg->h, reg->c0, reg->w, K, A.SW, A.SH);
     S.lds_off = src_floats;
-    src_floats += S.rows_max * S.cols_max;
+    src_floats += S.rows_max * (fast ? A.SW : S.cols_max);
     A.slot[i] = A.n_src++;
   }
   if (p->field) {
@@ -1276,7 +1283,7 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
     S.fx = (float)((double)p->field_w / (double)p->full_w);
     src_bounds(S, reg->r0, reg->h, reg->c0, reg->w, K, A.SW, A.SH);
     S.lds_off = src_floats;
-    src_floats += S.rows_max * S.cols_max;
+    src_floats += S.rows_max * (fast ? A.SW : S.cols_max);
     A.field_src = A.n_src++;
     A.wl = p->w_large;
   }
