@@ -17,6 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
+ABI_VERSION = 2  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -85,6 +86,7 @@ class fsg_topousm_params(C.Structure):
         ("ds", C.c_int32 * MAX_RADII), ("field", C.c_void_p), ("field_h", C.c_int32),
         ("field_w", C.c_int32), ("full_h", C.c_int32), ("full_w", C.c_int32),
         ("win_r0", C.c_int32), ("win_c0", C.c_int32), ("w_large", C.c_float), ("nan_hint", C.c_int32),
+        ("mean", C.c_void_p * MAX_RADII), ("mean_ld", C.c_int64 * MAX_RADII),
     ]
 
 
@@ -111,6 +113,9 @@ _SIGS = {
     "fsg_topousm": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_topousm_params),
                               P(fsg_output), P(fsg_output), vp, C.c_size_t, vp]),
     "fsg_block_mean": (C.c_int, [P(fsg_window), C.c_int32, vp, C.c_int64, vp]),
+    "fsg_topousm_coarse_means_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32]),
+    "fsg_topousm_coarse_means": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, P(C.c_int32),
+                                           P(vp), C.c_int64, vp, C.c_size_t, vp]),
     "fsg_topousm_field_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32]),
     "fsg_topousm_coarse_field": (C.c_int, [P(fsg_window), C.c_int32, P(C.c_int32), P(C.c_float),
                                            vp, C.c_int64, vp, C.c_size_t, vp]),
@@ -157,7 +162,7 @@ def lib():
                 raise NativeLibraryError(f"{LIB_PATH} lacks symbol {name}") from exc
             fn.restype = res
             fn.argtypes = args
-        if L.fsg_abi_version() != 1:
+        if L.fsg_abi_version() != ABI_VERSION:
             raise NativeLibraryError("libfsg.so ABI version mismatch")
         _lib = L
         return _lib

@@ -24,8 +24,9 @@ import torch
 
 from . import _native as N
 from . import ops
-from .params import (DEFAULT_ALTITUDE, DEFAULT_AZIMUTH, normalize_topousm_radii_weights,
-                     radius_to_downsample_factor, topousm_default_radii)
+from .params import (DEFAULT_ALTITUDE, DEFAULT_AZIMUTH, coarse_radius,
+                     normalize_topousm_radii_weights, radius_to_downsample_factor,
+                     shared_coarse_ok, topousm_default_radii)
 
 logger = logging.getLogger(__name__)
 
@@ -325,6 +326,58 @@ class AmbientOcclusionAlgorithm(NativeTileAlgorithm):
                                   pad_d=max(1, int(round(float(radius)))), out=o)
 
 
+class CoarsePyramid:
+    """Raster-wide coarse means for topousm_fast's decimated radii, resident
+    in HBM and shared by every window whose own coarse means they equal.
+
+    Each window of compute_topousm_fast_efficient_block
+    (_impl_topousm_fast.py:72-91) decimates itself and box-filters the result;
+    for a NaN-free raster, windows on the ds lattice whose sampled cells stay
+    clear of their edges (shared_coarse_ok) read the same values from one
+    raster-wide grid per (ds, coarse radius) instead.
+    """
+
+    def __init__(self, raster: torch.Tensor):
+        self.raster = raster
+        self.grids: Dict[Tuple[int, int], torch.Tensor] = {}
+
+    @staticmethod
+    def pairs(prm) -> List[Tuple[int, int]]:
+        return [(int(prm.ds[i]), coarse_radius(int(prm.radii[i]), int(prm.ds[i])))
+                for i in range(prm.n_radii) if prm.ds[i] > 1]
+
+    def require(self, pairs) -> None:
+        by_ds: Dict[int, List[int]] = {}
+        for ds, rs in pairs:
+            if (ds, rs) not in self.grids and rs not in by_ds.setdefault(ds, []):
+                by_ds[ds].append(rs)
+        for ds, rss in by_ds.items():
+            if rss:
+                g = ops.topousm_coarse_means(self.raster, ds, rss)
+                for i, rs in enumerate(rss):
+                    self.grids[(ds, rs)] = g[i]
+
+    def attach(self, prm, win_origin, win_shape, reg) -> int:
+        """Point prm.mean[i] at the shared grid wherever that is exact;
+        returns how many radii were attached."""
+        H, W = self.raster.shape
+        n = 0
+        for i in range(prm.n_radii):
+            ds = int(prm.ds[i])
+            if ds <= 1:
+                continue
+            rs = coarse_radius(int(prm.radii[i]), ds)
+            g = self.grids.get((ds, rs))
+            if g is None or not shared_coarse_ok(win_origin, win_shape,
+                                                 (reg.r0, reg.c0, reg.h, reg.w), ds, rs, (H, W)):
+                continue
+            v = g[int(win_origin[0]) // ds:, int(win_origin[1]) // ds:]
+            prm.mean[i] = v.data_ptr()
+            prm.mean_ld[i] = int(g.stride(0))
+            n += 1
+        return n
+
+
 class TopoUSMFastAlgorithm(NativeTileAlgorithm):
     """_direct_topousm_fast (tile/dask_bridge.py:225-271)."""
     name = "topousm_fast"
@@ -372,6 +425,9 @@ class TopoUSMFastAlgorithm(NativeTileAlgorithm):
         # may use the plain-mean kernel variant (identical results)
         if p.get("_window_nan_free", False):
             prm.nan_hint = 0
+        pyr = p.get("_topousm_fast_pyramid")
+        if pyr is not None and p.get("_window_origin") is not None:
+            pyr.attach(prm, p["_window_origin"], tuple(win.shape), reg)
         if prm.field and p.get("_topousm_fast_full_shape") is None:
             prm.full_h, prm.full_w = int(win.shape[0]), int(win.shape[1])
         stats = p.get("global_stats", None) if normalize else None

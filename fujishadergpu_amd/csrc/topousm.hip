@@ -1036,7 +1036,7 @@ static CoarsePlan plan_coarse(const fsg_window* win, const fsg_topousm_params* p
   for (int i = 0; i < p->n_radii; i++) {
     P.ds_of_radius[i] = -1;
     const int ds = p->ds[i];
-    if (ds <= 1) continue;
+    if (ds <= 1 || p->mean[i]) continue;  // full resolution, or supplied by the caller
     int idx = -1;
     for (int j = 0; j < P.n_ds; j++)
       if (P.ds_val[j] == ds) idx = j;
@@ -1236,31 +1236,39 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
   A.field_src = -1;
   int src_floats = 0;
   for (int i = 0; i < p->n_radii; i++) {
-    const int j = P.ds_of_radius[i];
-    if (j < 0) continue;
-    const int ds = P.ds_val[j];
-    const int r_small = std::max(1, (int)std::nearbyint((double)p->radii[i] / (double)ds));
-    float* mean = (float*)(ws + P.mean_off[i]);
-    rc = coarse_mean((float*)(ws + P.grid_off[j]), P.wc[j], P.hc[j], P.wc[j], r_small, mean,
-                     P.wc[j], tmp[0], tmp[1], s);
-    if (rc) return rc;
+    const int ds = p->ds[i];
+    if (ds <= 1) continue;
+    const int hc = std::max(1, (win->h + ds - 1) / ds), wc = std::max(1, (win->w + ds - 1) / ds);
+    const float* mean = p->mean[i];
+    int64_t mean_ld = p->mean_ld[i];
+    if (!mean) {
+      const int j = P.ds_of_radius[i];
+      const int r_small = std::max(1, (int)std::nearbyint((double)p->radii[i] / (double)ds));
+      float* m = (float*)(ws + P.mean_off[i]);
+      rc = coarse_mean((float*)(ws + P.grid_off[j]), P.wc[j], P.hc[j], P.wc[j], r_small, m,
+                       P.wc[j], tmp[0], tmp[1], s);
+      if (rc) return rc;
+      mean = m;
+      mean_ld = P.wc[j];
+    } else if (mean_ld < wc) {
+      return fail(FSG_E_INVALID, "fsg_topousm: mean_ld smaller than the coarse grid width");
+    }
     // scipy zoom output shape round(hc * (H / hc)) must be the window shape
-    const double zoom_y = (double)win->h / (double)P.hc[j];
-    const double zoom_x = (double)win->w / (double)P.wc[j];
-    if ((int)std::nearbyint(P.hc[j] * zoom_y) != win->h ||
-        (int)std::nearbyint(P.wc[j] * zoom_x) != win->w)
+    const double zoom_y = (double)win->h / (double)hc;
+    const double zoom_x = (double)win->w / (double)wc;
+    if ((int)std::nearbyint(hc * zoom_y) != win->h || (int)std::nearbyint(wc * zoom_x) != win->w)
       return fail(FSG_E_UNSUPPORTED, "fsg_topousm: zoom output shape differs from window");
     if (A.n_src >= MAX_SRC)
       return fail(FSG_E_UNSUPPORTED, "fsg_topousm: more than 5 decimated radii");
     SrcRef& S = A.src[A.n_src];
     S = SrcRef{};
     S.p = mean;
-    S.ld = P.wc[j];
-    S.h = P.hc[j];
-    S.w = P.wc[j];
+    S.ld = mean_ld;
+    S.h = hc;
+    S.w = wc;
     S.mode = 0;
-    S.zy = (win->h != 1) ? (double)(P.hc[j] - 1) / (double)(win->h - 1) : 1.0;
-    S.zx = (win->w != 1) ? (double)(P.wc[j] - 1) / (double)(win->w - 1) : 1.0;
+    S.zy = (win->h != 1) ? (double)(hc - 1) / (double)(win->h - 1) : 1.0;
+    S.zx = (win->w != 1) ? (double)(wc - 1) / (double)(win->w - 1) : 1.0;
     src_bounds(S, reg->r0, reg->h, reg->c0, reg->w, K, A.SW, A.SH);
     S.lds_off = src_floats;
     src_floats += S.rows_max * (fast ? A.SW : S.cols_max);
@@ -1297,6 +1305,46 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
   // whose results are identical on NaN-free windows.
   if (fast) return dispatch_fast(A, /*nanaw=*/p->nan_hint != 0, A.n_strips * n_segs, s);
   return dispatch_usm(A, /*nanaw=*/p->nan_hint != 0, A.n_strips * n_segs, s);
+}
+
+extern "C" size_t fsg_topousm_coarse_means_workspace_size(const fsg_window* win, int32_t ds) {
+  if (!win || win->h <= 0 || win->w <= 0 || ds < 2) return 0;
+  const size_t cells = (size_t)((win->h + ds - 1) / ds) * (size_t)((win->w + ds - 1) / ds);
+  // decimated grid, flag, four void-fill temporaries (two reused by the box passes)
+  return 5 * align256(sizeof(float) * cells) + align256(sizeof(int) * 4);
+}
+
+extern "C" int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32_t n_radii,
+                                        const int32_t* radii_coarse, float* const* outs,
+                                        int64_t ld_out, void* workspace, size_t workspace_bytes,
+                                        void* stream) {
+  int rc = validate_window(win, "fsg_topousm_coarse_means");
+  if (rc) return rc;
+  if (win->h == 0 || win->w == 0) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means: empty");
+  if (ds < 2 || n_radii <= 0 || !radii_coarse || !outs)
+    return fail(FSG_E_INVALID, "fsg_topousm_coarse_means: bad arguments");
+  const int hc = (win->h + ds - 1) / ds, wc = (win->w + ds - 1) / ds;
+  if (ld_out < wc) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means: ld_out < grid width");
+  for (int i = 0; i < n_radii; i++)
+    if (!outs[i]) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means: output NULL");
+  const size_t need = fsg_topousm_coarse_means_workspace_size(win, ds);
+  if (!workspace || workspace_bytes < need)
+    return fail(FSG_E_WORKSPACE, "fsg_topousm_coarse_means: workspace too small");
+  char* ws = (char*)workspace;
+  const size_t g = align256(sizeof(float) * (size_t)hc * wc);
+  float* grid = (float*)ws;
+  float* tmp[4];
+  for (int k = 0; k < 4; k++) tmp[k] = (float*)(ws + (1 + k) * g);
+  int* flag = (int*)(ws + 5 * g);
+  hipStream_t s = as_stream(stream);
+  rc = decimate(win, ds, grid, hc, wc, flag, tmp[0], tmp[1], tmp[2], tmp[3], false, s);
+  if (rc) return rc;
+  for (int i = 0; i < n_radii; i++) {
+    rc = coarse_mean(grid, wc, hc, wc, std::max(1, (int)radii_coarse[i]), outs[i], ld_out, tmp[0],
+                     tmp[1], s);
+    if (rc) return rc;
+  }
+  return FSG_OK;
 }
 
 extern "C" size_t fsg_topousm_field_workspace_size(int32_t h, int32_t w) {

@@ -214,6 +214,24 @@ def block_mean(src_t: torch.Tensor, factor: int) -> torch.Tensor:
     return out
 
 
+def topousm_coarse_means(win_t: torch.Tensor, ds: int, radii_coarse: Sequence[int]) -> torch.Tensor:
+    """Coarse means of one decimation level over a whole window (typically the
+    raster): a (len(radii_coarse), ceil(h/ds), ceil(w/ds)) float32 tensor."""
+    h, w = win_t.shape
+    hc, wc = -(-h // ds), -(-w // ds)
+    n = len(radii_coarse)
+    out = torch.empty((n, hc, wc), dtype=torch.float32, device=win_t.device)
+    L = _lib()
+    win = window(win_t)
+    nbytes = L.fsg_topousm_coarse_means_workspace_size(C.byref(win), int(ds))
+    ws = torch.empty(max(1, nbytes), dtype=torch.uint8, device=win_t.device)
+    rr = (C.c_int32 * n)(*[int(r) for r in radii_coarse])
+    outs = (C.c_void_p * n)(*[out[i].data_ptr() for i in range(n)])
+    N.check(L.fsg_topousm_coarse_means(C.byref(win), int(ds), n, rr, outs, wc, ws.data_ptr(),
+                                       nbytes, _stream()))
+    return out
+
+
 def topousm_coarse_field(coarse_t: torch.Tensor, radii_coarse: Sequence[int],
                          weights: Sequence[float]) -> torch.Tensor:
     h, w = coarse_t.shape

@@ -325,3 +325,44 @@ def norm_stat_window_geometry(algorithm: str, merged: dict, max_tile: int = 4096
 
 
 NORM_STAT_ALGOS = frozenset({"topousm_fast", "openness", "ambient_occlusion"})
+
+
+def coarse_radius(radius: int, ds: int) -> int:
+    """Radius on the ds-grid (_impl_topousm_fast.py:72-91): max(1, round(r / ds))."""
+    return max(1, int(round(float(radius) / float(ds))))
+
+
+def _axis_shared_ok(o: int, n: int, full: int, rs: int, rl: int, ds: int, margin: int) -> bool:
+    if o % ds or n % ds or n < 1:
+        return False
+    nc = n // ds
+    z = (nc - 1) / (n - 1) if n != 1 else 1.0
+    lo = min(max(rs * z, 0.0), nc - 1.0)
+    hi = min(max((rs + rl - 1) * z, 0.0), nc - 1.0)
+    i_lo = int(math.floor(lo))
+    i_hi = min(int(math.floor(hi)) + 1, nc - 1)
+    if o == 0 and o + n == full:
+        return True                      # the window spans the raster on this axis
+    lo_ok = i_lo - margin >= 0 or (o == 0 and margin < nc)
+    hi_ok = i_hi + margin <= nc - 1 or (o + n == full and margin < nc)
+    return lo_ok and hi_ok
+
+
+def shared_coarse_ok(win_origin, win_shape, region, ds: int, r_coarse: int, raster_shape) -> bool:
+    """Whether a raster-wide coarse mean grid (ds, r_coarse) holds exactly the
+    values a window would build for itself over the cells its region samples.
+
+    True when the window origin and shape are multiples of ds (its ds-grid is a
+    sub-grid of the raster's, cell for cell) and the box (or the radius-4
+    Gaussian for r_coarse <= 1) around every sampled cell stays inside the
+    window's grid, except across a window edge that is also a raster edge,
+    where both grids reflect alike.  The raster must hold no NaN (no
+    enclosed-void fill on either grid); the caller checks that.
+    region = (r0, c0, h, w) in window coordinates.
+    """
+    margin = r_coarse if r_coarse >= 2 else 4
+    r0, c0, rh, rw = region
+    return (_axis_shared_ok(int(win_origin[0]), int(win_shape[0]), int(raster_shape[0]), r0, rh,
+                            ds, margin)
+            and _axis_shared_ok(int(win_origin[1]), int(win_shape[1]), int(raster_shape[1]), c0,
+                                rw, ds, margin))

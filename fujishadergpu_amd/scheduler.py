@@ -21,7 +21,8 @@ import numpy as np
 import torch
 
 from . import ops
-from .algorithms import ALGORITHMS, SUPPORTED, OutSpec, UnsupportedOnBackend, load_algorithm
+from .algorithms import (ALGORITHMS, SUPPORTED, CoarsePyramid, OutSpec, TopoUSMFastAlgorithm,
+                         UnsupportedOnBackend, load_algorithm)
 from .encoding import SUPPORTED_OUTPUT_DTYPES, quantize_params, resolve_output_range
 from .params import (LOCAL_RADII, LOCAL_WEIGHTS, MULTISCALE_REQUIRED_ALGOS,
                      RADII_DRIVEN_ALGOS, SPATIAL_TILE_ALGORITHMS, auto_spatial_profile,
@@ -223,21 +224,28 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     H, W = raster.shape
     src = raster.clone() if copy_input else raster
     counts = ops.nodata_to_nan(src, nodata, [(t[7], t[6], t[9], t[8]) for t in P.tiles])
+    nan_counts = counts.cpu().numpy()
+    # the tile windows cover the raster: all zero means the raster holds no NaN
+    raster_nan_free = bool((nan_counts == 0).all())
     if "large" in P.extras:
         lr, lw = P.extras["large"]
         fld, scale = overview_field(src, lr, lw)
         P.params["_topousm_fast_coarse_field"] = fld
         P.extras["overview_scale"] = scale
+    pyramid = None
+    if algorithm == "topousm_fast" and raster_nan_free:
+        pyramid = CoarsePyramid(src)
+        pyramid.require(CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(P.params)))
     stats = global_stats if global_stats is not None else P.params.get("global_stats")
     if stats is None:
-        stats = norm_stats_prepass(src, algorithm, P.params)
+        stats = norm_stats_prepass(src, algorithm, P.params, raster_nan_free=raster_nan_free,
+                                   pyramid=pyramid)
     if stats is not None:
         P.params["global_stats"] = stats
     P.stats = stats
     dt = {"float32": torch.float32, "int16": torch.int16, "uint8": torch.uint8}[P.output_dtype]
     if out is None:
         out = torch.empty((H, W), dtype=dt, device=raster.device)
-    nan_counts = counts.cpu().numpy()
     algo = load_algorithm(algorithm)
     results: List[TileResult] = []
     transform = P.extras.get("transform")
@@ -255,8 +263,12 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
             continue
         p = dict(P.params)
         p["_window_nan_free"] = int(nn) == 0
-        if algorithm == "topousm_fast" and "_topousm_fast_coarse_field" in p:
-            p["_topousm_fast_field_offset"] = (int(wy), int(wx))
+        if algorithm == "topousm_fast":
+            if "_topousm_fast_coarse_field" in p:
+                p["_topousm_fast_field_offset"] = (int(wy), int(wx))
+            if pyramid is not None:
+                p["_topousm_fast_pyramid"] = pyramid
+                p["_window_origin"] = (int(wy), int(wx))
         if transform is not None and p.get("is_geographic_dem"):
             a, _, _, _, e, f = transform
             top = f + cy * e

@@ -63,9 +63,14 @@ def valid_bbox(raster: torch.Tensor):
 
 def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
                        grid: int = 3, max_tile: int = 4096, min_valid_frac: float = 0.02,
-                       return_pool: bool = False):
+                       return_pool: bool = False, raster_nan_free: bool = False,
+                       pyramid=None):
     """Device-side _compute_norm_stats_tiled.  ``raster`` holds NaN at NoData.
-    Returns a float64 GPU tensor (stats) or None (no valid window)."""
+    Returns a float64 GPU tensor (stats) or None (no valid window).
+
+    ``raster_nan_free`` (known from the NoData scan) skips the per-window NaN
+    count; ``pyramid`` (a CoarsePyramid of this raster, NaN-free rasters only)
+    lets topousm_fast windows share raster-wide coarse means where exact."""
     if algorithm not in NORM_STAT_ALGOS:
         return None
     merged = {**_DEFAULTS[algorithm](), **(params or {})}
@@ -81,8 +86,11 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
         return None
     wins = stratified_windows(W, H, *bbox, grid=grid, tile=min(tile, max(W, H)))
     # valid fraction of every window in one device pass, one host sync
-    counts = ops.nodata_to_nan(raster, None, [(wy, wx, th, tw) for wy, wx, tw, th in wins])
-    nan_counts = counts.cpu().numpy()
+    if raster_nan_free:
+        nan_counts = np.zeros(len(wins), dtype=np.int64)
+    else:
+        counts = ops.nodata_to_nan(raster, None, [(wy, wx, th, tw) for wy, wx, tw, th in wins])
+        nan_counts = counts.cpu().numpy()
     regions = []
     for (wy, wx, tw, th), nn in zip(wins, nan_counts):
         if (th * tw - int(nn)) / float(th * tw) < min_valid_frac:
@@ -96,6 +104,8 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
     algo = {"topousm_fast": TopoUSMFastAlgorithm, "openness": OpennessAlgorithm,
             "ambient_occlusion": AmbientOcclusionAlgorithm}[algorithm]()
     bp = dict(kw)
+    if pyramid is not None and algorithm == "topousm_fast":
+        pyramid.require(pyramid.pairs(algo.params_struct(bp)))
     off = 0
     for wy, wx, th, tw, m, nan_free in regions:
         win = raster[wy:wy + th, wx:wx + tw]
@@ -103,8 +113,11 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
         dst = pool[off:off + rh * rw].view(rh, rw)
         reg = ops.region(m, m, rh, rw)
         if algorithm == "topousm_fast":
-            algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "_window_nan_free": nan_free},
-                            normalize=False)
+            wp = {**bp, "_window_nan_free": nan_free}
+            if pyramid is not None:
+                wp["_topousm_fast_pyramid"] = pyramid
+                wp["_window_origin"] = (wy, wx)
+            algo.run_region(win, reg, OutSpec(f32=dst), wp, normalize=False)
         else:
             algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "mode": "local"}, stretch=False)
         off += rh * rw

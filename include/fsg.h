@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 1
+#define FSG_ABI_VERSION 2
 
 enum {
   FSG_OK = 0,
@@ -245,6 +245,16 @@ typedef struct fsg_topousm_params {
   /* 0: the caller knows the window holds no NaN (enables the plain-mean
      variant); any other value: NaN-aware means (always correct). */
   int32_t nan_hint;
+  /* Optional precomputed coarse means, per decimated radius (NULL: built
+     here from the window).  mean[i] points at cell (0, 0) of this window's
+     ds-grid; ceil(h/ds) x ceil(w/ds) cells are read with leading dimension
+     mean_ld[i].  The caller vouches that the values equal the window's own
+     coarse means, e.g. views of fsg_topousm_coarse_means over the raster
+     when the raster holds no NaN, the window origin and shape are multiples
+     of ds and the region's samples never reach a window edge that is not a
+     raster edge. */
+  const float* mean[FSG_MAX_RADII];
+  int64_t mean_ld[FSG_MAX_RADII];
 } fsg_topousm_params;
 
 /*
@@ -261,6 +271,22 @@ int fsg_topousm(const fsg_window* win, const fsg_region* reg,
                 const fsg_topousm_params* p, const fsg_output* out,
                 const fsg_output* out2, /* optional second encoding */
                 void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * The coarse means fsg_topousm builds per window for its decimated radii,
+ * for a whole window (typically the raster): the NaN-aware ds x ds cell means
+ * with the enclosed-void fill of _downsample_nan_aware (_nan_utils.py:604-668),
+ * then per radius the NaN-aware box mean of radius radii_coarse[i]
+ * (max(1, round(r / ds)); <= 1 selects the sigma=1 Gaussian) with 'reflect'
+ * edges, as compute_topousm_fast_efficient_block does
+ * (_impl_topousm_fast.py:72-91).  outs[i] are ceil(h/ds) x ceil(w/ds) grids
+ * with leading dimension ld_out; ds is a power of two <= 16.
+ */
+size_t fsg_topousm_coarse_means_workspace_size(const fsg_window* win, int32_t ds);
+int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32_t n_radii,
+                             const int32_t* radii_coarse, float* const* outs,
+                             int64_t ld_out, void* workspace, size_t workspace_bytes,
+                             void* stream);
 
 /*
  * NaN-aware block mean (area average for an integer factor), the restatement

@@ -161,6 +161,38 @@ def test_topousm_field_and_two_outputs(cuda):
     assert np.array_equal(u8.cpu().numpy(), dn)
 
 
+def test_topousm_shared_coarse_means(cuda):
+    """A window reading raster-wide coarse means (CoarsePyramid) equals the
+    same window building its own, and the oracle."""
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.algorithms import CoarsePyramid, OutSpec, TopoUSMFastAlgorithm
+    H = W = 2048
+    dem = _dem(H, W, 31, relief=3776.0, hurst=0.75)
+    g = dem.to(cuda)
+    radii, weights = [2, 8, 32, 128, 200], [0.4, 0.25, 0.15, 0.12, 0.08]
+    p = dict(radii=radii, weights=weights, pixel_size=5.0, global_stats=(20.0,),
+             _window_nan_free=True)
+    algo = TopoUSMFastAlgorithm()
+    pyr = CoarsePyramid(g)
+    pyr.require(pyr.pairs(algo.params_struct(p)))
+    assert sorted(pyr.grids) == [(4, 32), (8, 25)]
+    wy, wx, wh, ww = 256, 512, 1536, 1536
+    win = g[wy:wy + wh, wx:wx + ww]
+    reg = ops.region(256, 256, 1024, 1024)
+    assert pyr.attach(algo.params_struct(p), (wy, wx), (wh, ww), reg) == 2
+    own = torch.empty((1024, 1024), dtype=torch.float32, device=cuda)
+    shared = torch.empty_like(own)
+    algo.run_region(win, reg, OutSpec(f32=own), p)
+    algo.run_region(win, reg, OutSpec(f32=shared),
+                    {**p, "_topousm_fast_pyramid": pyr, "_window_origin": (wy, wx)})
+    d = float((own - shared).abs().max())
+    print(f"shared vs own coarse means: max |d| = {d:.3g}", flush=True)
+    assert d <= 1e-6
+    exp = ref.topousm_direct(dem.numpy()[wy:wy + wh, wx:wx + ww], radii=radii, weights=weights,
+                             pixel_size=5.0, stats=(20.0,))[256:1280, 256:1280]
+    _cmp("topousm/shared", shared, exp, 2e-4)
+
+
 @pytest.mark.parametrize("otype", ["positive", "negative"])
 @pytest.mark.parametrize("nan_discs", [0, 6])
 def test_openness_window(cuda, otype, nan_discs):
