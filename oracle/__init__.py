@@ -11,33 +11,43 @@ beside the GPU.  It is never imported by the product package
 Modules
 -------
 ref    float32-faithful restatement of the reference block functions
-fp64   float64 / closed-form oracles used to pin ``ref`` (and exact modes)
 
-Pinning (what checks each oracle function; see tests/test_oracle_*.py):
+Pinning (tests/test_oracle_pins.py checks each function against something
+other than itself; the GPU tests then compare the HIP path with this oracle):
 
 ==========================  ==================================================
-gradient / hillshade/slope  closed forms on planes; flat hillshade = sin(alt);
-                            np.gradient on quadratics (exact); FP64 oracle
-curvature                   quadratic surfaces (closed-form kappa everywhere);
-                            reference tests/test_curvature_analytic.py values;
-                            signed-step fixture
-NaN Gaussian fill           brute-force FP64 9x9 double loop on small grids
-box / topousm               z = x^2 closed form -sum w r(r+1)/3; planes -> 0;
-                            brute-force box means on small grids
-decimate / zoom / sample    brute-force loops on tiny grids (block means,
-                            corner-aligned bilinear)
-openness                    apex closed forms (flat, cone, pit); reference
-                            tests/test_openness_yokoyama.py; brute-force FP64
-ambient occlusion           flat -> 1; brute-force FP64 loop on small grids
-percentile / stats          numpy.percentile(method='linear') in FP64;
-                            reference tests/test_topousm_fast_normalization.py
-quantize / NoData           reference tests/test_audit_p1_regressions.py
-                            vectors; exhaustive-style sweeps vs the formula
+gradient / hillshade/slope  exact on planes with anisotropic steps and on
+                            quadratics (edge_order=2); flat hillshade =
+                            sin(alt); Lambert closed form on planes, both row
+                            orientations; slope in every unit on planes
+curvature                   reference tests/test_curvature_analytic.py values;
+                            closed-form mean (bowl) and Gaussian (elliptic)
+                            curvature at an apex
+NaN Gaussian / box means    brute-force FP64 normalised convolutions with the
+                            'nearest' and 'reflect' index rules
+topousm                     z = x^2 -> -sum w r(r+1)/3; planes and constants
+                            -> 0; downsample-factor table
+decimate / zoom / sample    brute-force cell means, enclosed-void fill,
+                            corner-aligned and pixel-centre bilinear loops;
+                            integer overview block mean
+openness                    reference tests/test_openness_yokoyama.py
+                            orderings and flat = 1; cone-apex closed forms
+                            (pi/2 - atan k), physical pixel distance
+ambient occlusion           flat -> 1; FP64 ring average on a plane
+stats / stretch             reference tests/test_topousm_fast_normalization
+                            vectors; linear order statistics; stretch
+quantize / NoData           reference tests/test_audit_p1_regressions.py and
+                            tests/test_output_nodata_policy.py vectors;
+                            round-half-even and clip rules
 ==========================  ==================================================
 
-Bit-level parity with CuPy's own float32 filter accumulation order
-(gaussian_filter, uniform_filter, zoom, reductions) is not reproducible
-without CuPy and an NVIDIA GPU; those calls are restated with SciPy's
-semantics (double accumulation, float32 output per pass) and compared under a
-stated tolerance.
+Parity unpinned (no pin exists without CuPy on an NVIDIA GPU, or GDAL):
+the exact float32 bits of CuPy's gaussian_filter / uniform_filter / zoom
+accumulation and of its float32 cell-sum reduction in _downsample_nan_aware
+(restated with SciPy semantics: double accumulation, float32 per pass), the
+precision of the 0-d promotion in compute_hillshade_block (restated in
+float64, checked only against the closed form to 1e-6), and GDAL's
+average-resampled overview read for a non-integer decimation
+(overview_block_mean raises NotImplementedError there).  The GPU parity
+tests state the tolerance that covers each.
 """

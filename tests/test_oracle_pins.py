@@ -1,0 +1,454 @@
+"""Pins for the CPU oracle (oracle/ref.py), CPU only.
+
+Each oracle function is checked against something other than itself:
+closed forms, brute-force FP64 loops written independently of SciPy, and the
+values the reference's own tests hold (cited by file).  Every check is chosen
+so that a plausible slip -- a sign, a transposed axis, a dropped term, a wrong
+boundary mode or rounding rule -- fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ref
+
+F32 = np.float32
+G = 1.0 / 2.2
+ZERO_DISP = 0.5 ** G          # display value of zero curvature / half openness
+
+
+# --------------------------------------------------------------- gradient (A1)
+
+def test_gradient_exact_on_planes_with_anisotropic_steps():
+    y, x = np.mgrid[0:12, 0:15].astype(F32)
+    z = (F32(3) * x - F32(2) * y).astype(F32)
+    drow, dcol = ref.gradient_f32(z, 2.0, 4.0)        # step_y = 2, step_x = 4
+    assert np.all(dcol == F32(0.75)) and np.all(drow == F32(-1.0))
+
+
+def test_gradient_edge_order_two_is_exact_on_quadratics():
+    y, x = np.mgrid[0:9, 0:11].astype(F32)
+    drow, dcol = ref.gradient_f32(x * x, 1.0, 1.0)
+    assert np.array_equal(dcol, 2 * x) and np.all(drow == 0)   # edges included
+    drow, dcol = ref.gradient_f32(y * y, 1.0, 1.0)
+    assert np.array_equal(drow, 2 * y) and np.all(dcol == 0)
+
+
+# ------------------------------------------------------ hillshade, slope (A2-3)
+
+def _plane(a, b, h=16, w=16):
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    return (a * x + b * y).astype(F32)
+
+
+def test_flat_hillshade_is_sin_altitude():
+    out = ref.hillshade(np.zeros((8, 8), F32), altitude=45.0)
+    assert np.allclose(out, math.sin(math.radians(45.0)), atol=1e-7, rtol=0)
+    out = ref.hillshade(np.zeros((8, 8), F32), altitude=30.0)
+    assert np.allclose(out, 0.5, atol=1e-7, rtol=0)
+
+
+@pytest.mark.parametrize("sy", [None, -1.0])
+def test_hillshade_on_a_plane_matches_the_lambert_closed_form(sy):
+    a, b = 0.5, 0.25
+    out = ref.hillshade(_plane(a, b), azimuth=315.0, altitude=45.0, pixel_size=1.0,
+                        pixel_scale_y=sy)
+    az, alt = math.radians(315.0), math.radians(45.0)
+    lx, ly, lz = math.sin(az) * math.cos(alt), math.cos(az) * math.cos(alt), math.sin(alt)
+    n = b * (1.0 if sy is None else -1.0)        # north-up rasters flip the row axis
+    exp = (-a * lx - n * ly + lz) / math.sqrt(1.0 + a * a + b * b)
+    assert np.allclose(out, exp, atol=1e-6, rtol=0)
+
+
+def test_slope_on_a_plane_in_every_unit():
+    z = _plane(0.5, 0.25)
+    g = math.hypot(0.5 / 2.0, 0.25 / 2.0)        # pixel size 2
+    assert np.allclose(ref.slope(z, unit="degree", pixel_size=2.0), math.degrees(math.atan(g)),
+                       atol=2e-5, rtol=0)
+    assert np.allclose(ref.slope(z, unit="radians", pixel_size=2.0), math.atan(g),
+                       atol=1e-6, rtol=0)
+    assert np.allclose(ref.slope(z, unit="percent", pixel_size=2.0), 100.0 * g, atol=1e-4, rtol=0)
+
+
+def test_nan_fill_does_not_spread_nan():
+    z = _plane(0.5, 0.25)
+    z[5, 7] = np.nan
+    for f in (ref.hillshade, ref.slope, ref.curvature):
+        out = f(z)
+        assert np.array_equal(np.isnan(out), np.isnan(z))
+
+
+# ----------------------------------------------------------------- curvature (A4)
+
+def _grid(h=64, w=64):
+    return np.mgrid[0:h, 0:w].astype(F32)
+
+
+def test_planform_zero_profile_nonzero_on_a_parabolic_cylinder():
+    # reference tests/test_curvature_analytic.py:18-41
+    _, x = _grid()
+    z = (F32(0.01) * (x - F32(32.0)) ** 2).astype(F32)
+    plan = ref.curvature(z, curvature_type="planform")[20:44, 20:44]
+    prof = ref.curvature(z, curvature_type="profile")[20:44, 20:44]
+    assert np.abs(plan - ZERO_DISP).max() < 0.02
+    assert np.abs(prof - ZERO_DISP).max() > 0.05
+
+
+def test_dome_has_plan_and_profile_curvature():
+    # reference tests/test_curvature_analytic.py:44-54
+    y, x = _grid()
+    dome = (F32(-0.01) * ((x - F32(32)) ** 2 + (y - F32(32)) ** 2)).astype(F32)
+    for ct in ("planform", "profile"):
+        assert np.abs(ref.curvature(dome, curvature_type=ct)[20:44, 20:44] - ZERO_DISP).max() > 0.05
+
+
+def _disp(k):
+    return ((math.tanh(100.0 * k) + 1.0) / 2.0) ** G
+
+
+def test_mean_curvature_at_a_bowl_apex():
+    # z = c (x^2 + y^2): p = q = s = 0, r = t = 2c -> H = -(r + t) / 2 = -2c
+    c = 0.001
+    y, x = _grid(33, 33)
+    z = (F32(c) * ((x - 16) ** 2 + (y - 16) ** 2)).astype(F32)
+    out = ref.curvature(z, curvature_type="mean")
+    assert out[16, 16] == pytest.approx(_disp(-2 * c), abs=1e-6)
+
+
+def test_gaussian_curvature_at_an_elliptic_apex():
+    # z = c x^2 + d y^2 at the apex: K = r t - s^2 = 4 c d
+    c, d = 0.05, 0.03
+    y, x = _grid(33, 33)
+    z = (F32(c) * (x - 16) ** 2 + F32(d) * (y - 16) ** 2).astype(F32)
+    out = ref.curvature(z, curvature_type="gaussian")
+    assert out[16, 16] == pytest.approx(_disp(4 * c * d), abs=1e-6)
+
+
+# -------------------------------------------------- NaN-aware filters (A1, A5)
+
+def _gauss_w(sigma=1.0, truncate=4.0):
+    r = int(truncate * sigma + 0.5)
+    k = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * k * k / (sigma * sigma))
+    return k.astype(int), w / w.sum()
+
+
+def _idx(i, n, mode):
+    if mode == "nearest":
+        return min(max(i, 0), n - 1)
+    # scipy 'reflect': d c b a | a b c d | d c b a
+    period = 2 * n
+    i %= period
+    return i if i < n else period - 1 - i
+
+
+def _brute_sep(x, taps, weights, mode):
+    """Separable 2D correlation by explicit loops in float64."""
+    h, w = x.shape
+    out = np.zeros((h, w))
+    for i in range(h):
+        for j in range(w):
+            s = 0.0
+            for ki, wi in zip(taps, weights):
+                for kj, wj in zip(taps, weights):
+                    s += wi * wj * x[_idx(i + ki, h, mode), _idx(j + kj, w, mode)]
+            out[i, j] = s
+    return out
+
+
+def _nan_block(seed, h=9, w=11):
+    rng = np.random.default_rng(seed)
+    b = (rng.random((h, w)) * 100).astype(F32)
+    b[2, 3] = b[6, 0] = b[8, 10] = np.nan
+    return b
+
+
+def test_nan_gaussian_fill_is_the_normalised_convolution():
+    b = _nan_block(1)
+    valid = ~np.isnan(b)
+    taps, wts = _gauss_w(1.0)
+    sv = _brute_sep(np.where(valid, b, 0).astype(np.float64), taps, wts, "nearest")
+    sw = _brute_sep(valid.astype(np.float64), taps, wts, "nearest")
+    got, mask = ref.handle_nan_with_gaussian(b, 1.0, mode="nearest")
+    assert np.array_equal(mask, ~valid)
+    assert np.allclose(got, sv / sw, atol=2e-4, rtol=0)
+
+
+@pytest.mark.parametrize("mode", ["reflect", "nearest"])
+def test_nan_box_mean_matches_brute_force(mode):
+    b = _nan_block(2)
+    valid = ~np.isnan(b)
+    r = 2
+    taps, wts = np.arange(-r, r + 1), np.full(2 * r + 1, 1.0 / (2 * r + 1))
+    sv = _brute_sep(np.where(valid, b, 0).astype(np.float64), taps, wts, mode)
+    sw = _brute_sep(valid.astype(np.float64), taps, wts, mode)
+    got, _ = ref.handle_nan_with_uniform(b, 2 * r + 1, mode=mode)
+    assert np.allclose(got, sv / sw, atol=2e-4, rtol=0)
+
+
+# ------------------------------------------------------------ TopoUSM (A5-A7)
+
+def test_topousm_on_a_parabola_is_minus_weighted_r_r_plus_1_over_3():
+    # box mean of x^2 over [x-r, x+r] is x^2 + r(r+1)/3
+    _, x = np.mgrid[0:24, 0:41].astype(F32)
+    z = ((x - F32(20)) ** 2).astype(F32)
+    out = ref.topousm_fast_block(z, radii=[2, 5], weights=[0.6, 0.4], pixel_size=100.0)
+    exp = -(0.6 * 2 * 3 / 3 + 0.4 * 5 * 6 / 3)
+    assert np.allclose(out[:, 6:35], exp, atol=1e-3, rtol=0)
+
+
+def test_topousm_is_zero_on_planes_and_constants():
+    z = _plane(3.0, -2.0, 32, 40)
+    out = ref.topousm_fast_block(z, radii=[2, 4], weights=[0.5, 0.5], pixel_size=100.0)
+    assert np.abs(out[5:-5, 5:-5]).max() < 1e-3
+    c = np.full((20, 30), 123.25, F32)
+    assert np.all(ref.topousm_fast_block(c, radii=[2, 9, 40], weights=[0.2, 0.3, 0.5],
+                                         pixel_size=5.0) == 0)
+
+
+def test_downsample_factor_table():
+    cases = {(2, 5.0): 1, (32, 5.0): 1, (128, 5.0): 4, (200, 5.0): 8, (512, 5.0): 16,
+             (2048, 5.0): 16, (48, 0.5): 2, (48, 1.0): 2, (24, 1.0): 1}
+    from fujishadergpu_amd.params import radius_to_downsample_factor as product
+    for (r, px), f in cases.items():
+        assert ref.radius_to_downsample_factor(r, pixel_size=px, algorithm_name="topousm_fast") == f
+        assert product(r, pixel_size=px, algorithm_name="topousm_fast") == f
+
+
+def test_decimation_is_the_nan_aware_cell_mean():
+    rng = np.random.default_rng(3)
+    b = (rng.random((10, 13)) * 50).astype(F32)
+    b[0, 0] = b[5, 7] = np.nan
+    got = ref.downsample_nan_aware(b, 4)
+    assert got.shape == (3, 4)
+    for i in range(3):
+        for j in range(4):
+            cell = b[4 * i:4 * i + 4, 4 * j:4 * j + 4].astype(np.float64)
+            assert got[i, j] == pytest.approx(np.nanmean(cell), rel=1e-6)
+
+
+def test_enclosed_empty_cell_is_filled_from_its_neighbours():
+    b = np.arange(12 * 12, dtype=F32).reshape(12, 12)
+    b[4:8, 4:8] = np.nan                            # one all-NaN cell at factor 4
+    got = ref.downsample_nan_aware(b, 4)
+    coarse = np.array([[np.nanmean(b[4 * i:4 * i + 4, 4 * j:4 * j + 4].astype(np.float64))
+                        if not np.isnan(b[4 * i:4 * i + 4, 4 * j:4 * j + 4]).all() else np.nan
+                        for j in range(3)] for i in range(3)])
+    valid = ~np.isnan(coarse)
+    taps, wts = _gauss_w(1.0)                       # sigma = max(1, 3 / 64)
+    sv = _brute_sep(np.where(valid, coarse, 0), taps, wts, "nearest")
+    sw = _brute_sep(valid.astype(np.float64), taps, wts, "nearest")
+    assert sw[1, 1] > 0.5
+    assert got[1, 1] == pytest.approx(sv[1, 1] / sw[1, 1], rel=1e-5)
+
+
+def test_zoom_up_is_corner_aligned_bilinear():
+    rng = np.random.default_rng(4)
+    c = (rng.random((4, 5)) * 10).astype(F32)
+    th, tw = 10, 13
+    got = ref.upsample_to_shape(c, (th, tw))
+    for i in range(th):
+        for j in range(tw):
+            y, x = i * 3 / (th - 1), j * 4 / (tw - 1)
+            y0, x0 = min(int(y), 2), min(int(x), 3)
+            ty, tx = y - y0, x - x0
+            v = ((1 - ty) * ((1 - tx) * c[y0, x0] + tx * c[y0, x0 + 1])
+                 + ty * ((1 - tx) * c[y0 + 1, x0] + tx * c[y0 + 1, x0 + 1]))
+            assert got[i, j] == pytest.approx(v, abs=2e-6)
+
+
+def test_overview_field_sample_is_pixel_centre_bilinear_with_clamp():
+    rng = np.random.default_rng(5)
+    c = (rng.random((6, 8)) * 10).astype(F32)
+    H, W = 24, 32                                   # coarse = 1/4 of the raster
+    got = ref.bilinear_sample_coarse(c, 3, 11, 0, 32, H, W)
+    for i, r in enumerate(range(3, 11)):
+        for j in range(32):
+            y = min(max((r + 0.5) * 6 / H - 0.5, 0.0), 5.0)
+            x = min(max((j + 0.5) * 8 / W - 0.5, 0.0), 7.0)
+            y0, x0 = min(int(y), 4), min(int(x), 6)
+            ty, tx = y - y0, x - x0
+            v = ((1 - ty) * ((1 - tx) * c[y0, x0] + tx * c[y0, x0 + 1])
+                 + ty * ((1 - tx) * c[y0 + 1, x0] + tx * c[y0 + 1, x0 + 1]))
+            assert got[i, j] == pytest.approx(v, abs=2e-6)
+
+
+def test_overview_integer_decimation_is_a_block_mean():
+    rng = np.random.default_rng(6)
+    r = (rng.random((12, 12)) * 7).astype(F32)
+    r[1, 1] = np.nan
+    got, scale = ref.overview_block_mean(r, sample_max=4)
+    assert scale == 3.0 and got.shape == (4, 4)
+    f = 3
+    for i in range(got.shape[0]):
+        for j in range(got.shape[1]):
+            assert got[i, j] == pytest.approx(np.nanmean(
+                r[f * i:f * i + f, f * j:f * j + f].astype(np.float64)), rel=1e-6)
+
+
+def test_p99_stat_reference_vectors():
+    # reference tests/test_topousm_fast_normalization.py:6-33
+    rng = np.random.default_rng(0)
+    base = rng.normal(0.0, 0.01, size=2048).astype(F32)
+    base[:3] = [10.0, -8.0, 12.0]
+    s = ref.topousm_stat(base)[0]
+    assert 0.0 < s < 0.2
+    data = np.concatenate([np.full(800, 0.01, F32), np.full(200, 0.5, F32)])
+    assert ref.topousm_stat(data)[0] == pytest.approx(0.5, rel=1e-2)
+    # linear in-between order statistic (method='linear')
+    assert ref.topousm_stat(np.arange(101, dtype=F32))[0] == pytest.approx(99.0)
+    assert ref.topousm_stat(np.arange(11, dtype=F32))[0] == pytest.approx(9.9)
+
+
+def test_normalisation_is_a_plain_divide():
+    # reference tests/test_topousm_fast_normalization.py:36-44
+    out = ref.normalize_div(np.array([-4.0, -1.0, 0.0, 1.0, 4.0], F32), 2.0)
+    assert out.tolist() == [-2.0, -0.5, 0.0, 0.5, 2.0]
+
+
+# --------------------------------------------------------- openness, AO (A8-A10)
+
+def _radial(n=101):
+    yy, xx = np.mgrid[0:n, 0:n]
+    return np.sqrt((xx - n // 2) ** 2 + (yy - n // 2) ** 2)
+
+
+def test_openness_reference_orderings_and_flat():
+    # reference tests/test_openness_yokoyama.py:21-47
+    r = _radial()
+    kw = dict(num_directions=16, max_distance=40)
+    peak, pit = (50 - r).astype(F32), (r - 50).astype(F32)
+    assert (ref.openness(peak, openness_type="positive", **kw)[50, 50]
+            > ref.openness(pit, openness_type="positive", **kw)[50, 50])
+    assert (ref.openness(pit, openness_type="negative", **kw)[50, 50]
+            > ref.openness(peak, openness_type="negative", **kw)[50, 50])
+    flat = np.zeros((101, 101), F32)
+    for t in ("positive", "negative"):
+        assert ref.openness(flat, openness_type=t, **kw)[50, 50] == pytest.approx(1.0, abs=1e-3)
+
+
+@pytest.mark.parametrize("k", [1.0, 2.0, 0.5])
+def test_openness_at_a_cone_apex(k):
+    # every sample of a cone of slope k sits at elevation angle atan(k)
+    r = _radial(81)
+    pit = (F32(k) * r).astype(F32)
+    kw = dict(num_directions=8, max_distance=30)
+    ang = (math.pi / 2 - math.atan(k)) / (math.pi / 2)
+    assert ref.openness(pit, openness_type="positive", **kw)[40, 40] == pytest.approx(
+        ang ** G, abs=2e-6)
+    peak = (-F32(k) * r).astype(F32)
+    assert ref.openness(peak, openness_type="negative", **kw)[40, 40] == pytest.approx(
+        ang ** G, abs=2e-6)
+
+
+def test_openness_uses_physical_distance():
+    # pixel size 2 halves every elevation angle's tangent on the same cone
+    r = _radial(81)
+    pit = (F32(2.0) * r).astype(F32)
+    out = ref.openness(pit, openness_type="positive", num_directions=8, max_distance=30,
+                       pixel_size=2.0)
+    ang = (math.pi / 2 - math.atan(1.0)) / (math.pi / 2)
+    assert out[40, 40] == pytest.approx(ang ** G, abs=2e-6)
+
+
+def test_ambient_occlusion_flat_and_plane():
+    flat = np.zeros((40, 40), F32)
+    assert np.allclose(ref.ambient_occlusion(flat, num_samples=8, radius=6.0), 1.0, atol=1e-7)
+    # a plane z = k x: every pixel sees the same ring, so the interior is the
+    # FP64 ring average below (smoothing a constant field leaves it constant)
+    k, n_s, rad = 0.4, 8, 6.0
+    y, x = np.mgrid[0:48, 0:48]
+    z = (k * x).astype(F32)
+    out = ref.ambient_occlusion(z, num_samples=n_s, radius=rad)
+    tot = cnt = 0.0
+    for rf in (0.25, 0.5, 0.75, 1.0):
+        for i in range(n_s):
+            a = 2 * math.pi * i / n_s
+            dx, dy = round(rad * rf * math.cos(a)), round(rad * rf * math.sin(a))
+            if dx == 0 and dy == 0:
+                continue
+            occ = min(max(0.0, math.atan(k * dx / math.hypot(dx, dy))) / (math.pi / 4), 1.0)
+            tot += occ * (1.0 - 0.3 * rf)
+            cnt += 1
+    exp = max(0.0, min(1.0, 1.0 - tot / cnt)) ** G
+    assert np.allclose(out[12:36, 12:36], exp, atol=2e-6)
+
+
+def test_display_stretch_percentiles():
+    lo, sc = ref.stretch_stat(np.arange(101, dtype=F32))
+    assert (lo, sc) == (pytest.approx(1.0), pytest.approx(98.0))
+    out = ref.display_stretch(np.array([0.0, 1.0, 50.0, 99.0, 200.0], F32), (lo, sc))
+    assert np.allclose(out, [0.0, 0.0, 49 / 98, 1.0, 199 / 98])
+    assert ref.stretch_stat(np.full(10, 3.0, F32)) == (3.0, 0.0)
+
+
+# ---------------------------------------------------- encoding, NoData (A12-13)
+
+def test_quantize_reference_vector_and_rounding_rule():
+    # reference tests/test_audit_p1_regressions.py:44-50
+    dn = ref.quantize(np.array([np.inf, -np.inf, np.nan, 1.0], F32), 100.0, 0.0, 0, 255, "uint8")
+    assert dn.tolist() == [0, 0, 0, 100]
+    # rint is round-half-to-even, then clip to the code range
+    dn = ref.quantize(np.array([0.5, 1.5, 2.5, -0.5, 300.0], F32), 1.0, 0.0, 0, 255, "uint8")
+    assert dn.tolist() == [0, 2, 2, 0, 255]
+    dn = ref.quantize(np.array([-1.0, 0.0, 1.0], F32), 32767.0, 0.0, -32767, 32767, "int16")
+    assert dn.tolist() == [-32767, 0, 32767]
+
+
+def test_product_quantizer_matches_the_oracle():
+    from fujishadergpu_amd.encoding import quantize_params, resolve_output_range
+    qp = quantize_params(-1.176, 1.176, "uint8")
+    x = np.linspace(-1.5, 1.5, 4001, dtype=F32)
+    exp = ref.quantize(x, qp["a_coef"], qp["b_coef"], qp["dn_min"], qp["dn_max"], "uint8")
+    assert exp.min() == qp["dn_min"] and exp.max() == qp["dn_max"]
+    assert np.all(np.diff(exp.astype(int)) >= 0)
+    with pytest.raises(ValueError, match="high > low"):
+        resolve_output_range("hillshade", override=(1.0, 0.0))   # test_audit_p5:103-107
+
+
+def test_nodata_mask_reference_vectors():
+    # reference tests/test_audit_p1_regressions.py:20-38
+    d = np.array([[1.0, -9999.0], [np.nan, 2.0]], F32)
+    assert ref.nodata_mask(d, -9999.0).tolist() == [[False, True], [True, False]]
+    d = np.array([-9999.0, -9998.9, 0.0], F32)
+    assert ref.nodata_mask(d, -9999.0).tolist() == [True, False, False]
+
+
+def test_hillshade_output_keeps_nan_through_the_clip():
+    # reference tests/test_output_nodata_policy.py:20-41
+    out = ref.format_output(np.array([[np.nan, 0.5], [1.2, -0.1]], F32), "hillshade")
+    assert np.isnan(out[0, 0]) and out[1, 0] == 1.0 and out[1, 1] == 0.0 and out[0, 1] == 0.5
+
+
+# ------------------------------------------------- host parameter logic (A15-16)
+
+@pytest.mark.parametrize("short_side,expected", [
+    (30000, [2, 8, 32, 128, 512, 2048]), (20480, [2, 8, 32, 128, 512, 2048]),
+    (10000, [2, 8, 32, 128, 512]), (5120, [2, 8, 32, 128, 512]), (5119, [2, 8, 32, 128]),
+    (300, [2, 8]), (15, [2]),
+])
+def test_auto_radii_reference_table(short_side, expected):
+    # reference tests/test_spatial_auto_radii.py:30-43
+    from fujishadergpu_amd.params import auto_spatial_radii
+    assert auto_spatial_radii(short_side) == expected
+
+
+def test_auto_weights_and_dedup_reference_vectors():
+    from fujishadergpu_amd.params import (auto_spatial_profile, auto_spatial_weights,
+                                          deduplicate_radii_weights)
+    for n in (1, 2, 3, 4, 6):
+        w = auto_spatial_weights(n)
+        assert sum(w) == pytest.approx(1.0)
+        assert all(w[i] == pytest.approx(2 * w[i + 1]) for i in range(n - 1))
+    r, w = auto_spatial_profile(10000, radii=[4, 16, 64, 256])
+    assert r == [4, 16, 64, 256] and w == pytest.approx([8 / 15, 4 / 15, 2 / 15, 1 / 15])
+    r, w = deduplicate_radii_weights([4, 4, 8], [0.2, 0.3, 0.5])   # test_audit_p5:20-25
+    assert r == [4, 8] and w == pytest.approx([0.5, 0.5])
+
+
+def test_meters_per_degree_at_35_north():
+    # reference tests/test_audit_p1_regressions.py:99-107
+    from fujishadergpu_amd.raster_info import meters_per_degree
+    m_lon, m_lat = meters_per_degree(35.0)
+    assert m_lat == pytest.approx(110940.5, abs=15) and m_lon == pytest.approx(91288.0, abs=15)
