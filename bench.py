@@ -43,7 +43,27 @@ def parse():
     ap.add_argument("--size", type=int, default=N_PX)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=0, help="tiles in the CPU sample (0 = auto)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed steps, copy a 8192x8192 float32 region once with "
+                         "k_copy_region (256 MiB read + 256 MiB written): the reference point "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_usm_main_topousm_16384_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_usm_main launch from the committed counter summary
+    (the line 'traffic_bytes_per_launch: N'), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].strip().split()[0])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def make_dem(n, device):
@@ -156,6 +176,14 @@ def main():
     avg_ms = tot_ms.value / max(1, cnt.value)
     achieved = bytes_step / (tot_ms.value / args.steps / 1e3) / 1e9
     stats_scale = float(plan.stats[0].item())
+    if args.calib_copy:
+        from fujishadergpu_amd import ops
+        m = min(8192, n)
+        dst = torch.empty((m, m), dtype=torch.float32, device=dev)
+        ops.copy_region(dem, ops.region(0, 0, m, m), dst)
+        torch.cuda.synchronize()
+        del dst
+    traffic = measured_traffic()
 
     line = {
         "metric": METRIC, "value": round(gpix, 3), "unit": "Gpix/s", "n_gpus": 1,
@@ -171,7 +199,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(traffic) if traffic is not None else None,
             "kernel": "k_usm_main", "launches_per_step": launches_step,
             "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_step": bytes_step,
             "kernel_share_of_step": round(tot_ms.value / args.steps / ms_step, 3),

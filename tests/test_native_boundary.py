@@ -57,3 +57,34 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     finally:
         monkeypatch.delenv("FSG_LIBRARY")
         importlib.reload(N)
+
+
+def test_ctypes_layout_matches_the_c_header(tmp_path):
+    """gcc compiles include/fsg.h and prints sizeof/offsetof of every field of
+    every struct the ctypes mirror declares; both must agree."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {n: getattr(N, n) for n in dir(N)
+               if n.startswith("fsg_") and isinstance(getattr(N, n), type)
+               and issubclass(getattr(N, n), ctypes.Structure)}
+    assert {"fsg_window", "fsg_output", "fsg_topousm_params"} <= set(structs)
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fsg.h"', "int main(void){"]
+    for name, st in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f in st._fields_:
+            lines.append(f'printf("{name}.{f[0]} %zu\\n", offsetof({name}, {f[0]}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    inc = os.path.join(os.path.dirname(N.__file__), "..", "include")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in
+               subprocess.run([str(exe)], check=True, capture_output=True, text=True)
+               .stdout.splitlines())
+    for name, st in structs.items():
+        assert int(got[f"{name} size"]) == ctypes.sizeof(st), name
+        for f in st._fields_:
+            assert int(got[f"{name}.{f[0]}"]) == getattr(st, f[0]).offset, f"{name}.{f[0]}"
