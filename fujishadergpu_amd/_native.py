@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 2  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 3  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -90,6 +90,19 @@ class fsg_topousm_params(C.Structure):
     ]
 
 
+class fsg_batch_tile(C.Structure):
+    _fields_ = [
+        ("win", fsg_window), ("reg", fsg_region), ("hillshade", fsg_output), ("slope", fsg_output),
+        ("curvature", fsg_output), ("topousm", fsg_output), ("topousm2", fsg_output),
+        ("shade", fsg_shade_params), ("curv", fsg_curvature_params), ("usm", fsg_topousm_params),
+    ]
+
+
+class fsg_batch_config(C.Structure):
+    _fields_ = [("wg_per_cu", C.c_int32), ("order", C.c_int32), ("n_cu", C.c_int32),
+                ("pad0", C.c_int32)]
+
+
 P = C.POINTER
 vp = C.c_void_p
 _SIGS = {
@@ -116,6 +129,12 @@ _SIGS = {
     "fsg_topousm_coarse_means_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32]),
     "fsg_topousm_coarse_means": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, P(C.c_int32),
                                            P(vp), C.c_int64, vp, C.c_size_t, vp]),
+    "fsg_batch_create": (C.c_int, [P(fsg_batch_tile), C.c_int32, P(vp)]),
+    "fsg_batch_run": (C.c_int, [vp, P(fsg_batch_config), vp]),
+    "fsg_batch_capture": (C.c_int, [vp, P(fsg_batch_config), vp]),
+    "fsg_batch_replay": (C.c_int, [vp, vp]),
+    "fsg_batch_info": (C.c_int, [vp, P(C.c_int64)]),
+    "fsg_batch_destroy": (None, [vp]),
     "fsg_topousm_field_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32]),
     "fsg_topousm_coarse_field": (C.c_int, [P(fsg_window), C.c_int32, P(C.c_int32), P(C.c_float),
                                            vp, C.c_int64, vp, C.c_size_t, vp]),

@@ -1,0 +1,450 @@
+// Batched fused launch: many independent tiles (ragged sizes, each its own
+// window) and, per tile, hillshade + slope + curvature + TopoUSM in ONE
+// persistent kernel that drains a device work queue.
+//
+// The reference runs one tile at a time and one algorithm at a time
+// (core/tile_processor.py:1635-1683 calling run_tile_algorithm per tile);
+// per-tile, per-algorithm launches are the "unfused" comparison here.
+//
+// Work items are TopoUSM main-pass blocks (SW x FAST_SH of a tile region,
+// the heavier items) and stencil blocks (64 x 16 output pixels).  A stencil
+// item stages the block's filled window values with a 3-pixel halo in LDS
+// once and derives hillshade, slope and curvature from them; both item kinds
+// run the same device functions as the per-window kernels (stencil.hpp,
+// usm.hpp), so fused and unfused outputs are identical bit for bit.
+// Per-tile coarse-grid preparation (decimation, coarse box means) is
+// enqueued before the persistent kernel; the whole sequence can be captured
+// once in a hipGraph and replayed.
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "stencil.hpp"
+#include "usm.hpp"
+
+namespace fsg {
+
+constexpr int ST_TW = 64, ST_TH = 16, ST_HALO = 3;
+constexpr int ST_LT = ST_TW + 2 * ST_HALO + 1;  // LDS row stride (floats)
+constexpr int ST_ROWS = ST_TH + 2 * ST_HALO;
+
+struct StencilTile {
+  WinDev W;
+  int r0, c0, rh, rw;
+  int nbx;  // stencil blocks per block row
+  int nanaw;
+  int want_hs, want_sl, want_cv;
+  ShadeK shade;
+  CurvK curv;
+  OutDev hs, sl, cv;
+};
+
+template <bool NANAW>
+__device__ __forceinline__ void stencil_block(const StencilTile& S, int blk, const Gauss9& G,
+                                              float* lds) {
+  const int by = blk / S.nbx, bx = blk - by * S.nbx;
+  const int rl = S.r0 + by * ST_TH, cl = S.c0 + bx * ST_TW;  // window coords of the block
+  const int tr0 = max(rl - ST_HALO, 0), tr1 = min(rl + ST_TH + ST_HALO, S.W.h);
+  const int tc0 = max(cl - ST_HALO, 0), tc1 = min(cl + ST_TW + ST_HALO, S.W.w);
+  const int nr = tr1 - tr0, nc = tc1 - tc0;
+  const int t = threadIdx.x;
+  for (int e = t; e < nr * nc; e += 256) {
+    const int rr = e / nc, cc = e - rr * nc;
+    lds[rr * ST_LT + cc] = filled<NANAW>(S.W, tr0 + rr, tc0 + cc, G);
+  }
+  __syncthreads();
+  // every derivative footprint of a block pixel lies within 3 rows/columns
+  // of it and inside the window, i.e. inside the staged tile
+  auto f = [&](int r, int c) { return lds[(r - tr0) * ST_LT + (c - tc0)]; };
+  const int c = t & 63;
+  const PostK phs = load_post(S.hs), psl = load_post(S.sl), pcv = load_post(S.cv);
+  for (int k = t >> 6; k < ST_TH; k += 4) {
+    const int rr = by * ST_TH + k, cc = bx * ST_TW + c;  // region coordinates
+    if (rr < S.rh && cc < S.rw) {
+      const int wr = S.r0 + rr, wc = S.c0 + cc;
+      const float z = NANAW ? raw(S.W, wr, wc) : f(wr, wc);
+      if (isnan(z)) {
+        const float qn = __builtin_nanf("");
+        if (S.want_hs) store_out(S.hs, rr, cc, apply_post(phs, qn));
+        if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, qn));
+        if (S.want_cv) store_out(S.cv, rr, cc, apply_post(pcv, qn));
+      } else {
+        if (S.want_hs || S.want_sl) {
+          float dx, dy;
+          grad_at(S.shade.ax, S.shade.ay, wr, wc, S.W.h, S.W.w, f, dx, dy);
+          if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, slope_from(S.shade, dx, dy)));
+          if (S.want_hs)
+            store_out(S.hs, rr, cc,
+                      apply_post(phs, hillshade_from(S.shade, wr, wc, S.W.h, S.W.w, f, dx, dy)));
+        }
+        if (S.want_cv)
+          store_out(S.cv, rr, cc,
+                    apply_post(pcv, curvature_at(S.curv, wr, wc, S.W.h, S.W.w, f)));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+struct BatchDev {
+  const UsmArgs* usm;
+  const int* usm_pre;  // prefix of main-pass blocks per fused TopoUSM tile (n + 1)
+  int n_usm_tiles, n_usm_items;
+  const StencilTile* st;
+  const int* st_pre;   // prefix of stencil blocks per tile (n + 1)
+  int n_st_tiles, n_st_items;
+  int* queue;
+  int order;           // 0: TopoUSM items first, 1: stencil items first
+  Gauss9 G;
+};
+
+// largest t with pre[t] <= item (pre[0] = 0, pre strictly covers the items)
+__device__ __forceinline__ int find_tile(const int* pre, int n, int item) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int NR>
+__global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
+  extern __shared__ float lds[];
+  __shared__ int s_item;
+  const int total = B.n_usm_items + B.n_st_items;
+  // every workgroup leaves once the queue passes the item count
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(B.queue, 1);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= total) break;
+    bool is_usm;
+    int it;
+    if (B.order == 0) {
+      is_usm = item < B.n_usm_items;
+      it = is_usm ? item : item - B.n_usm_items;
+    } else {
+      is_usm = item >= B.n_st_items;
+      it = is_usm ? item - B.n_st_items : item;
+    }
+    if (is_usm) {
+      const int tl = find_tile(B.usm_pre, B.n_usm_tiles, it);
+      usm_fast_block<NR, false>(B.usm[tl], it - B.usm_pre[tl], lds);
+    } else {
+      const int tl = find_tile(B.st_pre, B.n_st_tiles, it);
+      const StencilTile& S = B.st[tl];
+      if (S.nanaw)
+        stencil_block<true>(S, it - B.st_pre[tl], B.G, lds);
+      else
+        stencil_block<false>(S, it - B.st_pre[tl], B.G, lds);
+    }
+  }
+}
+
+__global__ void k_batch_reset(int* __restrict__ q) {
+  if (threadIdx.x == 0) q[0] = 0;
+}
+
+}  // namespace fsg
+
+using namespace fsg;
+
+struct fsg_batch {
+  std::vector<fsg_batch_tile> tiles;
+  std::vector<size_t> ws_off;         // per tile: TopoUSM workspace offset, or SIZE_MAX
+  std::vector<int> usm_fused;         // tile indices in the fused kernel
+  std::vector<int> usm_fallback;      // tile indices run with per-tile launches
+  int nr = -1;
+  size_t lds = 0;
+  int n_usm_items = 0, n_st_items = 0, n_st_tiles = 0;
+  char* dev = nullptr;                // one allocation: workspaces, descriptors, queue
+  size_t dev_bytes = 0;
+  UsmArgs* d_usm = nullptr;
+  int* d_usm_pre = nullptr;
+  StencilTile* d_st = nullptr;
+  int* d_st_pre = nullptr;
+  int* d_queue = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static bool wants_usm(const fsg_batch_tile& t) {
+  return t.topousm.data != nullptr && (t.usm.n_radii > 0 || t.usm.field != nullptr);
+}
+static bool wants_stencil(const fsg_batch_tile& t) {
+  return t.hillshade.data || t.slope.data || t.curvature.data;
+}
+static const fsg_output* opt(const fsg_output& o) { return o.data ? &o : nullptr; }
+
+extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batch** out) {
+  if (!out) return fail(FSG_E_INVALID, "fsg_batch_create: out NULL");
+  *out = nullptr;
+  if (n <= 0 || !tiles) return fail(FSG_E_INVALID, "fsg_batch_create: no tiles");
+  fsg_batch* b = new fsg_batch();
+  b->tiles.assign(tiles, tiles + n);
+  b->ws_off.assign(n, SIZE_MAX);
+  // validation (the per-tile entry points' rules) and TopoUSM workspaces
+  size_t off = 0;
+  for (int i = 0; i < n; i++) {
+    const fsg_batch_tile& t = b->tiles[i];
+    int rc = validate_window(&t.win, "fsg_batch_create");
+    if (!rc) rc = validate_region(&t.win, &t.reg, "fsg_batch_create");
+    for (const fsg_output* o : {opt(t.hillshade), opt(t.slope), opt(t.curvature), opt(t.topousm),
+                                opt(t.topousm2)})
+      if (!rc) rc = validate_output(o, &t.reg, "fsg_batch_create");
+    if (!rc && wants_stencil(t)) {
+      if (t.win.h < 3 || t.win.w < 3)
+        rc = fail(FSG_E_INVALID, "fsg_batch_create: window smaller than 3 x 3");
+      else if ((t.hillshade.data || t.slope.data) && !(t.shade.step_x > 0 && t.shade.step_y > 0))
+        rc = fail(FSG_E_INVALID, "fsg_batch_create: shade steps must be positive");
+      else if (t.curvature.data && (t.curv.step_x == 0.0 || t.curv.step_y == 0.0 ||
+                                    t.curv.curvature_type < 0 || t.curv.curvature_type > 3))
+        rc = fail(FSG_E_INVALID, "fsg_batch_create: bad curvature parameters");
+    }
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    if (wants_usm(t)) {
+      b->ws_off[i] = off;
+      off += al256(std::max<size_t>(256, fsg_topousm_workspace_size(&t.win, &t.usm)));
+    }
+  }
+  const size_t ws_total = off;
+  const size_t usm_bytes = al256(sizeof(UsmArgs) * n), pre_bytes = al256(sizeof(int) * (n + 1));
+  const size_t st_bytes = al256(sizeof(StencilTile) * n);
+  b->dev_bytes = ws_total + usm_bytes + 2 * pre_bytes + st_bytes + 256;
+  if (int rc = check_hip(hipMalloc((void**)&b->dev, b->dev_bytes), "fsg_batch_create")) {
+    delete b;
+    return rc;
+  }
+  char* p = b->dev + ws_total;
+  b->d_usm = (UsmArgs*)p;
+  p += usm_bytes;
+  b->d_usm_pre = (int*)p;
+  p += pre_bytes;
+  b->d_st = (StencilTile*)p;
+  p += st_bytes;
+  b->d_st_pre = (int*)p;
+  p += pre_bytes;
+  b->d_queue = (int*)p;
+
+  // TopoUSM arguments (no launches): fused when on the fast path, NaN-free,
+  // and with the batch's common number of box radii
+  std::vector<UsmArgs> args(n);
+  std::vector<int> blocks(n, 0), fast_ok(n, 0);
+  std::map<int, int> nr_count;
+  for (int i = 0; i < n; i++) {
+    const fsg_batch_tile& t = b->tiles[i];
+    if (!wants_usm(t)) continue;
+    bool fast = false;
+    int rc = usm_prepare(&t.win, &t.reg, &t.usm, &t.topousm, opt(t.topousm2),
+                         b->dev + b->ws_off[i],
+                         fsg_topousm_workspace_size(&t.win, &t.usm), nullptr, &args[i], &fast,
+                         &blocks[i], /*launch_prep=*/false);
+    if (rc) {
+      fsg_batch_destroy(b);
+      return rc;
+    }
+    if (fast && t.usm.nan_hint == 0 && blocks[i] > 0 &&
+        fast_lds_bytes(args[i], false) <= 150 * 1024) {
+      fast_ok[i] = 1;
+      nr_count[args[i].nr]++;
+    }
+  }
+  int best = -1, best_n = 0;
+  for (auto& kv : nr_count)
+    if (kv.second > best_n) best = kv.first, best_n = kv.second;
+  b->nr = best;
+  std::vector<UsmArgs> fused_args;
+  std::vector<int> usm_pre{0};
+  size_t lds = sizeof(float) * ST_ROWS * ST_LT;
+  for (int i = 0; i < n; i++) {
+    if (!wants_usm(b->tiles[i])) continue;
+    if (fast_ok[i] && args[i].nr == best) {
+      b->usm_fused.push_back(i);
+      fused_args.push_back(args[i]);
+      usm_pre.push_back(usm_pre.back() + blocks[i]);
+      lds = std::max(lds, fast_lds_bytes(args[i], false));
+    } else {
+      b->usm_fallback.push_back(i);
+    }
+  }
+  b->n_usm_items = usm_pre.back();
+  // stencil descriptors
+  std::vector<StencilTile> st;
+  std::vector<int> st_pre{0};
+  for (int i = 0; i < n; i++) {
+    const fsg_batch_tile& t = b->tiles[i];
+    if (!wants_stencil(t) || t.reg.h == 0 || t.reg.w == 0) continue;
+    StencilTile S{};
+    S.W = WinDev{t.win.data, t.win.ld, t.win.h, t.win.w};
+    S.r0 = t.reg.r0;
+    S.c0 = t.reg.c0;
+    S.rh = t.reg.h;
+    S.rw = t.reg.w;
+    S.nbx = (t.reg.w + ST_TW - 1) / ST_TW;
+    S.want_hs = t.hillshade.data != nullptr;
+    S.want_sl = t.slope.data != nullptr;
+    S.want_cv = t.curvature.data != nullptr;
+    S.nanaw = (S.want_hs || S.want_sl ? t.shade.nan_hint != 0 : 0) ||
+              (S.want_cv ? t.curv.nan_hint != 0 : 0);
+    if (S.want_hs || S.want_sl) S.shade = make_shadek(&t.shade, S.want_hs, S.want_sl);
+    if (S.want_cv) S.curv = make_curvk(&t.curv);
+    S.hs = S.want_hs ? make_out(&t.hillshade) : OutDev{};
+    S.sl = S.want_sl ? make_out(&t.slope) : OutDev{};
+    S.cv = S.want_cv ? make_out(&t.curvature) : OutDev{};
+    st.push_back(S);
+    st_pre.push_back(st_pre.back() + S.nbx * ((t.reg.h + ST_TH - 1) / ST_TH));
+  }
+  b->n_st_tiles = (int)st.size();
+  b->n_st_items = st_pre.back();
+  b->lds = lds;
+  int rc = FSG_OK;
+  if (!fused_args.empty())
+    rc = check_hip(hipMemcpy(b->d_usm, fused_args.data(), sizeof(UsmArgs) * fused_args.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc)
+    rc = check_hip(hipMemcpy(b->d_usm_pre, usm_pre.data(), sizeof(int) * usm_pre.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc && !st.empty())
+    rc = check_hip(hipMemcpy(b->d_st, st.data(), sizeof(StencilTile) * st.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc)
+    rc = check_hip(hipMemcpy(b->d_st_pre, st_pre.data(), sizeof(int) * st_pre.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
+  if (rc) {
+    fsg_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return FSG_OK;
+}
+
+template <int NR>
+static void launch_fused(const BatchDev& D, int grid, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_fused<NR>, dim3(grid), dim3(256), lds, s, D);
+}
+
+static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t s) {
+  // coarse grids of the fused TopoUSM tiles (decimation, coarse box means)
+  for (int i : b->usm_fused) {
+    const fsg_batch_tile& t = b->tiles[i];
+    UsmArgs A;
+    bool fast = false;
+    int nb = 0;
+    const int rc = usm_prepare(&t.win, &t.reg, &t.usm, &t.topousm, opt(t.topousm2),
+                               b->dev + b->ws_off[i], fsg_topousm_workspace_size(&t.win, &t.usm),
+                               s, &A, &fast, &nb, /*launch_prep=*/true);
+    if (rc) return rc;
+  }
+  // tiles outside the fused kernel: their own complete launches
+  for (int i : b->usm_fallback) {
+    const fsg_batch_tile& t = b->tiles[i];
+    const int rc = fsg_topousm(&t.win, &t.reg, &t.usm, &t.topousm, opt(t.topousm2),
+                               b->dev + b->ws_off[i], fsg_topousm_workspace_size(&t.win, &t.usm),
+                               s);
+    if (rc) return rc;
+  }
+  const int total = b->n_usm_items + b->n_st_items;
+  if (total == 0) return FSG_OK;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_queue);
+  FSG_CHECK_LAUNCH("k_batch_reset");
+  int n_cu = cfg ? cfg->n_cu : 0;
+  if (n_cu <= 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      n_cu = prop.multiProcessorCount;
+    if (n_cu <= 0) n_cu = 256;
+  }
+  const int per_cu = (cfg && cfg->wg_per_cu > 0) ? cfg->wg_per_cu : 3;
+  const int grid = std::max(1, std::min(total, n_cu * per_cu));
+  BatchDev D;
+  D.usm = b->d_usm;
+  D.usm_pre = b->d_usm_pre;
+  D.n_usm_tiles = (int)b->usm_fused.size();
+  D.n_usm_items = b->n_usm_items;
+  D.st = b->d_st;
+  D.st_pre = b->d_st_pre;
+  D.n_st_tiles = b->n_st_tiles;
+  D.n_st_items = b->n_st_items;
+  D.queue = b->d_queue;
+  D.order = cfg ? cfg->order : 0;
+  D.G = make_gauss9();
+  const int nr = b->usm_fused.empty() ? 0 : b->nr;
+  TimedScope ts("k_batch_fused", s);
+  switch (nr) {
+    case 0: launch_fused<0>(D, grid, b->lds, s); break;
+    case 1: launch_fused<1>(D, grid, b->lds, s); break;
+    case 2: launch_fused<2>(D, grid, b->lds, s); break;
+    case 3: launch_fused<3>(D, grid, b->lds, s); break;
+    case 4: launch_fused<4>(D, grid, b->lds, s); break;
+    case 5: launch_fused<5>(D, grid, b->lds, s); break;
+    case 6: launch_fused<6>(D, grid, b->lds, s); break;
+    default: return fail(FSG_E_UNSUPPORTED, "fsg_batch_run: more than 6 box radii");
+  }
+  FSG_CHECK_LAUNCH("k_batch_fused");
+  return FSG_OK;
+}
+
+extern "C" int fsg_batch_run(fsg_batch* b, const fsg_batch_config* cfg, void* stream) {
+  if (!b) return fail(FSG_E_INVALID, "fsg_batch_run: NULL batch");
+  return batch_enqueue(b, cfg, as_stream(stream));
+}
+
+extern "C" int fsg_batch_capture(fsg_batch* b, const fsg_batch_config* cfg, void* stream) {
+  if (!b) return fail(FSG_E_INVALID, "fsg_batch_capture: NULL batch");
+  hipStream_t s = as_stream(stream);
+  if (!s) return fail(FSG_E_INVALID, "fsg_batch_capture: capture needs a non-default stream");
+  if (b->exec) {
+    hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  int rc = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "fsg_batch_capture");
+  if (rc) return rc;
+  const int erc = batch_enqueue(b, cfg, s);
+  hipGraph_t g = nullptr;
+  rc = check_hip(hipStreamEndCapture(s, &g), "fsg_batch_capture");
+  if (erc) {
+    if (g) hipGraphDestroy(g);
+    return erc;
+  }
+  if (rc) return rc;
+  b->graph = g;
+  return check_hip(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "fsg_batch_capture");
+}
+
+extern "C" int fsg_batch_replay(fsg_batch* b, void* stream) {
+  if (!b || !b->exec) return fail(FSG_E_INVALID, "fsg_batch_replay: nothing captured");
+  return check_hip(hipGraphLaunch(b->exec, as_stream(stream)), "fsg_batch_replay");
+}
+
+extern "C" int fsg_batch_info(const fsg_batch* b, int64_t* info) {
+  if (!b || !info) return fail(FSG_E_INVALID, "fsg_batch_info: NULL argument");
+  info[0] = b->n_usm_items;
+  info[1] = b->n_st_items;
+  info[2] = (int64_t)b->usm_fused.size();
+  info[3] = (int64_t)b->usm_fallback.size();
+  info[4] = b->n_st_tiles;
+  info[5] = (int64_t)b->lds;
+  info[6] = b->nr;
+  info[7] = (int64_t)b->dev_bytes;
+  return FSG_OK;
+}
+
+extern "C" void fsg_batch_destroy(fsg_batch* b) {
+  if (!b) return;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->dev) hipFree(b->dev);
+  delete b;
+}

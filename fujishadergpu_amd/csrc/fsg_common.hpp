@@ -167,4 +167,25 @@ inline int gaussian_radius(double sigma, double truncate = 4.0) {
   return (int)(truncate * sigma + 0.5);
 }
 
+// ---------------------------------------------- shared device-side types
+// A window as the kernels see it (row-major float32 view of the raster).
+struct WinDev {
+  const float* p;
+  int64_t ld;
+  int h, w;
+};
+
+// Gaussian sigma=1, truncate=4 weights: g[0] centre, g[d] distance d.
+struct Gauss9 {
+  double g[5];
+};
+
+inline Gauss9 make_gauss9() {
+  double w[9];
+  gaussian_weights(1.0, 4, w);
+  Gauss9 G;
+  for (int d = 0; d <= 4; d++) G.g[d] = w[4 + d];
+  return G;
+}
+
 }  // namespace fsg

@@ -1,0 +1,241 @@
+// Gradient-stencil device code shared by the per-window kernels
+// (stencil.hip) and the batched fused launch (batch.hip).
+//
+// Derivatives are numpy/cupy gradient(f, h_row, h_col, edge_order=2) on the
+// window: interior (f[i+1] - f[i-1]) / f32(2h), window edges
+// a*f0 + b*f1 + c*f2 with f32 coefficients (-1.5/h, 2/h, -0.5/h) and the
+// mirror (0.5/h, -2/h, 1.5/h), every operation rounded in f32.  NaN cells are
+// replaced by the NaN-aware sigma=1 Gaussian fill of handle_nan_for_gradient
+// (_nan_utils.py:50-74), evaluated on demand in scipy's two-pass order.
+//
+// The per-pixel functions take an accessor f(r, c) returning the filled
+// value at window coordinates, so a kernel reading global memory and one
+// reading an LDS tile run the same float32 operations in the same order.
+#pragma once
+#include <cmath>
+
+#include "fsg_common.hpp"
+
+namespace fsg {
+
+__device__ __forceinline__ float raw(const WinDev& W, int r, int c) {
+  return W.p[(int64_t)r * W.ld + c];
+}
+
+// NaN-aware local fill at (r, c): scipy gaussian_filter(mode='nearest') of
+// filled*valid and of valid, axis 0 then axis 1 with float32 intermediates,
+// symmetric accumulation order (centre, then distance 4..1), then
+// where(sw > 0, sv / sw, 0) in float32 (_nan_utils.py:18-31).
+static __device__ __noinline__ float nan_fill(const WinDev W, int r, int c, const Gauss9 G) {
+  float vsv[9], vsw[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int cc = clampi(c + k - 4, W.w);
+    const float x0 = raw(W, r, cc);
+    const bool v0 = !isnan(x0);
+    double sv = (v0 ? (double)x0 : 0.0) * G.g[0];
+    double sw = (v0 ? 1.0 : 0.0) * G.g[0];
+#pragma unroll
+    for (int d = 4; d >= 1; d--) {
+      const float a = raw(W, clampi(r - d, W.h), cc);
+      const float b = raw(W, clampi(r + d, W.h), cc);
+      const double fa = isnan(a) ? 0.0 : (double)a, fb = isnan(b) ? 0.0 : (double)b;
+      const double wa = isnan(a) ? 0.0 : 1.0, wb = isnan(b) ? 0.0 : 1.0;
+      sv += (fa + fb) * G.g[d];
+      sw += (wa + wb) * G.g[d];
+    }
+    vsv[k] = (float)sv;
+    vsw[k] = (float)sw;
+  }
+  double sv = (double)vsv[4] * G.g[0];
+  double sw = (double)vsw[4] * G.g[0];
+#pragma unroll
+  for (int d = 4; d >= 1; d--) {
+    sv += ((double)vsv[4 - d] + (double)vsv[4 + d]) * G.g[d];
+    sw += ((double)vsw[4 - d] + (double)vsw[4 + d]) * G.g[d];
+  }
+  const float fsv = (float)sv, fsw = (float)sw;
+  return fsw > 0.0f ? fsv / fsw : 0.0f;
+}
+
+// Filled value (NaN replaced by the local fill).  NANAW = false is the variant
+// for windows known to hold no NaN: no fill call, identical results there.
+template <bool NANAW>
+__device__ __forceinline__ float filled(const WinDev& W, int r, int c, const Gauss9& G) {
+  const float v = raw(W, r, c);
+  if (!NANAW) return v;
+  return isnan(v) ? nan_fill(W, r, c, G) : v;
+}
+
+// numpy gradient coefficients for one axis (all float32, from a Python step).
+struct Axis {
+  float two_h;       // f32(2*h)
+  float a0, b0, c0;  // f32(-1.5/h), f32(2/h), f32(-0.5/h)
+  float a1, b1, c1;  // f32(0.5/h),  f32(-2/h), f32(1.5/h)
+};
+
+inline Axis make_axis(double h) {
+  Axis a;
+  a.two_h = (float)(2. * h);
+  a.a0 = (float)(-1.5 / h);
+  a.b0 = (float)(2. / h);
+  a.c0 = (float)(-0.5 / h);
+  a.a1 = (float)(0.5 / h);
+  a.b1 = (float)(-2. / h);
+  a.c1 = (float)(1.5 / h);
+  return a;
+}
+
+// One derivative along an axis of length n at index i from accessor f(k).
+template <class F>
+__device__ __forceinline__ float deriv(const Axis& A, int i, int n, F f) {
+  if (i == 0) {
+    const float t0 = A.a0 * f(0);
+    const float t1 = A.b0 * f(1);
+    const float s = t0 + t1;
+    const float t2 = A.c0 * f(2);
+    return s + t2;
+  }
+  if (i == n - 1) {
+    const float t0 = A.a1 * f(n - 3);
+    const float t1 = A.b1 * f(n - 2);
+    const float s = t0 + t1;
+    const float t2 = A.c1 * f(n - 1);
+    return s + t2;
+  }
+  const float d = f(i + 1) - f(i - 1);
+  return d / A.two_h;
+}
+
+// ------------------------------------------------------------ hillshade+slope
+
+struct ShadeK {
+  Axis ax, ay;    // |step| derivative coefficients
+  float zf;       // f32(z_factor) (hillshade only)
+  float sgx, sgy; // f32 orientation signs
+  double lx, ly, lz;
+  int unit;
+  int need_hs, need_sl;
+  int z_is_one;
+};
+
+struct CurvK {
+  Axis ax, ay;    // |step| (first derivatives)
+  Axis sx, sy;    // signed steps (second derivatives)
+  int type;
+  float gamma;
+};
+
+// Kernel constants from the C-ABI parameters (host side).
+inline ShadeK make_shadek(const fsg_shade_params* p, bool need_hs, bool need_sl) {
+  ShadeK K;
+  K.ax = make_axis(p->step_x);
+  K.ay = make_axis(p->step_y);
+  K.zf = (float)p->z_factor;
+  K.z_is_one = (K.zf == 1.0f);
+  K.sgx = (float)p->sign_x;
+  K.sgy = (float)p->sign_y;
+  const double alt = p->altitude_deg * (M_PI / 180.0);
+  const double az = p->azimuth_deg * (M_PI / 180.0);
+  K.lx = std::sin(az) * std::cos(alt);
+  K.ly = std::cos(az) * std::cos(alt);
+  K.lz = std::sin(alt);
+  K.unit = p->slope_unit;
+  K.need_hs = need_hs;
+  K.need_sl = need_sl;
+  return K;
+}
+
+inline CurvK make_curvk(const fsg_curvature_params* p) {
+  CurvK K;
+  K.ax = make_axis(std::fabs(p->step_x));
+  K.ay = make_axis(std::fabs(p->step_y));
+  K.sx = make_axis(p->step_x);
+  K.sy = make_axis(p->step_y);
+  K.type = p->curvature_type;
+  K.gamma = (float)(1.0 / 2.2);
+  return K;
+}
+
+// Gradient (d/dcol, d/drow) at window (i, j) of an h x w window.
+template <class F>
+__device__ __forceinline__ void grad_at(const Axis& ax, const Axis& ay, int i, int j, int h,
+                                        int w, F f, float& dx, float& dy) {
+  dy = deriv(ay, i, h, [&](int k) { return f(k, j); });
+  dx = deriv(ax, j, w, [&](int k) { return f(i, k); });
+}
+
+// compute_slope_block's unit conversion (_impl_slope.py:23-33).
+__device__ __forceinline__ float slope_from(const ShadeK& K, float dx, float dy) {
+  const float m = sqrtf(dx * dx + dy * dy);
+  const float rad = atanf(m);
+  if (K.unit == 0) return (float)((double)rad * (180.0 / M_PI));
+  if (K.unit == 1) return tanf(rad) * 100.0f;
+  return rad;
+}
+
+// compute_hillshade_block's shading from the (unscaled) gradient; a z factor
+// other than 1 re-derives from the scaled values as the reference does.
+template <class F>
+__device__ __forceinline__ float hillshade_from(const ShadeK& K, int i, int j, int h, int w, F f,
+                                                float dx, float dy) {
+  float hdy = dy, hdx = dx;
+  if (!K.z_is_one) {
+    hdy = deriv(K.ay, i, h, [&](int k) { return f(k, j) * K.zf; });
+    hdx = deriv(K.ax, j, w, [&](int k) { return f(i, k) * K.zf; });
+  }
+  const float e = hdx * K.sgx;
+  const float n = hdy * K.sgy;
+  const float norm = sqrtf(e * e + n * n + 1.0f);
+  // float64 from here: the light vector is a 0-d float64 array.
+  double hs = (double)(-e) * K.lx;
+  hs = hs + (double)(-n) * K.ly;
+  hs = hs + K.lz;
+  hs = hs / (double)norm;
+  hs = fmin(fmax(hs, 0.0), 1.0);
+  return (float)hs;
+}
+
+// compute_curvature_block (_impl_curvature.py:19-57) at window (i, j):
+// first derivatives with |step|, second derivatives of those with the signed
+// steps, the curvature formula, tanh(100 c), (x + 1) / 2, gamma.
+template <class F>
+__device__ __forceinline__ float curvature_at(const CurvK& K, int i, int j, int h, int w, F f) {
+  const float p = deriv(K.ax, j, w, [&](int k) { return f(i, k); });  // dx
+  const float q = deriv(K.ay, i, h, [&](int k) { return f(k, j); });  // dy
+  // dyy, dyx = gradient(dy, step_y, step_x); dxy, dxx = gradient(dx, ...)
+  const float dxx = deriv(K.sx, j, w, [&](int k) {
+    return deriv(K.ax, k, w, [&](int m) { return f(i, m); });
+  });
+  const float dxy = deriv(K.sy, i, h, [&](int k) {
+    return deriv(K.ax, j, w, [&](int m) { return f(k, m); });
+  });
+  const float dyy = deriv(K.sy, i, h, [&](int k) {
+    return deriv(K.ay, k, h, [&](int m) { return f(m, j); });
+  });
+  const float dyx = deriv(K.sx, j, w, [&](int k) {
+    return deriv(K.ay, i, h, [&](int m) { return f(m, k); });
+  });
+  float cv;
+  const float one = 1.0f, two = 2.0f;
+  if (K.type == 0) {
+    const float s = (dxy + dyx) / two;
+    const float den = powf(one + p * p + q * q, 1.5f);
+    const float num = (one + q * q) * dxx - two * p * q * s + (one + p * p) * dyy;
+    cv = -num / (two * den + 1e-10f);
+  } else if (K.type == 1) {
+    const float b = one + p * p + q * q;
+    cv = (dxx * dyy - dxy * dxy) / (b * b);
+  } else if (K.type == 2) {
+    cv = (-(q * q * dxx - two * p * q * dxy + p * p * dyy)) /
+         (powf(p * p + q * q, 1.5f) + 1e-10f);
+  } else {
+    cv = (-(p * p * dxx + two * p * q * dxy + q * q * dyy)) /
+         ((p * p + q * q) * powf(one + p * p + q * q, 1.5f) + 1e-10f);
+  }
+  const float cn = tanhf(cv * 100.0f);
+  float res = (cn + one) / two;
+  return powf(res, K.gamma);
+}
+
+}  // namespace fsg

@@ -24,27 +24,9 @@
 #include <cmath>
 #include <vector>
 
-#include "fsg_common.hpp"
+#include "usm.hpp"
 
 namespace fsg {
-
-struct WinDev {
-  const float* p;
-  int64_t ld;
-  int h, w;
-};
-
-struct Gauss9 {
-  double g[5];
-};
-
-Gauss9 make_gauss9_t() {
-  double w[9];
-  gaussian_weights(1.0, 4, w);
-  Gauss9 G;
-  for (int d = 0; d <= 4; d++) G.g[d] = w[4 + d];
-  return G;
-}
 
 __device__ __forceinline__ float rawv(const float* p, int64_t ld, int r, int c) {
   return p[(int64_t)r * ld + c];
@@ -300,74 +282,6 @@ __global__ void k_field_accum(float* __restrict__ field, int64_t ldf, const floa
 }
 
 // ------------------------------------------------------------ main kernel
-
-enum { TERM_BOX = 0, TERM_GAUSS = 1, TERM_COARSE = 2 };
-constexpr int MAX_BOX = 6;
-constexpr int MAX_SRC = 5;  // bilinear sources staged in LDS (coarse means + field)
-
-// A bilinear source: a coarse mean grid sampled with the corner-aligned zoom
-// (mode 0, window coordinates, float64 coordinates), or the overview field
-// sampled at raster coordinates with float32 coordinates (mode 1), both with
-// 'nearest' clamping (_nan_utils.py:255-281, :671-698).
-struct SrcRef {
-  const float* p;
-  int64_t ld;
-  int h, w;
-  int mode;
-  int oy, ox;           // raster origin of the window (mode 1)
-  double zy, zx;        // zoom factors (mode 0)
-  float fy, fx;         // f32 ratios (mode 1)
-  int rows_max, cols_max, lds_off;
-};
-
-struct UsmArgs {
-  WinDev W;
-  int r0, c0, rh, rw;  // region in window coordinates
-  int n_terms;
-  int kind[FSG_MAX_RADII];
-  int slot[FSG_MAX_RADII];  // box slot, or source index for coarse terms
-  float wt[FSG_MAX_RADII];
-  int nr;                   // number of full-resolution box radii
-  int rad[MAX_BOX];
-  double inv[MAX_BOX];      // 1 / (2r + 1)
-  int R;                    // largest box radius (strip halo)
-  int SW, SH;               // strip width, rows per workgroup
-  int n_strips;
-  int n_src;
-  SrcRef src[MAX_SRC];
-  int field_src;            // source index of the overview field, or -1
-  float wl;                 // f32(w_large)
-  int src_floats;           // LDS entries of all staging tiles (float64 in k_usm_fast)
-  Gauss9 G;
-  OutDev o1, o2;
-};
-
-__device__ __forceinline__ int reflf(int i, int n) {
-  return (i >= 0 && i < n) ? i : refl(i, n);
-}
-
-// Row pointer of a block-uniform row index: the index is forced into a scalar
-// register so loads become (scalar row base)[per-lane column offset].
-__device__ __forceinline__ const float* urow(const float* base, int row, int64_t ld) {
-  const int r = __builtin_amdgcn_readfirstlane(row);
-  return base + (int64_t)r * ld;
-}
-
-__host__ __device__ inline void src_axis(const SrcRef& S, int row, int idx, int& i0, int& i1,
-                                         double& t) {
-  const int n = row ? S.h : S.w;
-  double c;
-  if (S.mode == 0) {
-    c = (double)idx * (row ? S.zy : S.zx);
-  } else {
-    const float f = ((float)(idx + (row ? S.oy : S.ox)) + 0.5f) * (row ? S.fy : S.fx) - 0.5f;
-    c = (double)f;
-  }
-  c = fmin(fmax(c, 0.0), (double)(n - 1));
-  i0 = (int)floor(c);
-  i1 = (i0 + 1 < n - 1) ? i0 + 1 : n - 1;
-  t = c - (double)i0;
-}
 
 // bilinear value from a staged tile (rows from r_lo, cols from c_lo)
 __device__ __forceinline__ float src_sample(const SrcRef& S, const float* tile, int r_lo, int c_lo,
@@ -690,266 +604,16 @@ static int dispatch_usm(const UsmArgs& A, bool nanaw, int n_blocks, hipStream_t 
 }
 
 // ------------------------------------------------------------ fast main kernel
-//
-// Term order [distinct box radii ascending in slot order][decimated radii]
-// [field] -- the order every sorted radius list produces.  A workgroup covers
-// SW output columns x FAST_SH rows; each K-row chunk runs three phases:
-//   1. vertical running sums (one LDS column per thread, all row loads of the
-//      chunk in flight together) -> Vs, centre values -> Zs;
-//   2. horizontal running sums over row segments and the box terms in order,
-//      the partial sum written over the centre value in Zs;
-//   3. one output row per step with lane = column: the decimated terms from
-//      staged coarse tiles (column coefficients held in registers), the field
-//      term, post-processing and both encodings, coalesced.
-// Accumulation order is the term order, so results equal k_usm_main's.
-constexpr int FAST_SH = 64;
-constexpr int FAST_K = 8;
 
-// scipy 'reflect' for an index at most n outside [0, n)
-__device__ __forceinline__ int refl1(int i, int n) {
-  return i < 0 ? -1 - i : (i >= n ? 2 * n - 1 - i : i);
-}
-
+// One launch per window: XCD-aware order, consecutive blocks of one band of
+// the region land on one XCD (L2).
 template <int NR, bool NANAW>
 __global__ void __launch_bounds__(256) k_usm_fast(UsmArgs A, int n_tiles, int per_xcd) {
-  constexpr int K = FAST_K;
-  constexpr int KB = 4;            // rows per batch of vertical loads
-  constexpr int NRA = NR > 0 ? NR : 1;
   extern __shared__ float lds[];
-  const int NC = A.SW + 2 * A.R;
-  const int LDC = NC + 1;
-  float* Vs = lds;                                   // [NR][K][LDC]
-  float* Ws = Vs + NR * K * LDC;                     // [NR][K][LDC] (NaN-aware only)
-  float* Zs = NANAW ? (Ws + NR * K * LDC) : Ws;      // [K][LDC] centre -> partial sum
-  // float64 tables from a 16-byte boundary: per source, rows_max staged
-  // coarse rows of horizontally interpolated values [row][SW]; then per source
-  // K row entries (staged-row offsets, ty, 1 - ty)
-  double* Hs_t = (double*)(lds + ((NR * K * LDC * (NANAW ? 2 : 1) + K * LDC + 3) & ~3));
-  int2* RowI = (int2*)(Hs_t + A.src_floats);
-  double2* RowT = (double2*)(RowI + MAX_SRC * K);
-
-  // XCD-aware order: consecutive tiles of one band land on one XCD (L2)
   const int b = blockIdx.x;
   const int tile = (b & 7) * per_xcd + (b >> 3);
   if (tile >= n_tiles) return;
-  const int t = threadIdx.x;
-  const int strip = tile % A.n_strips;
-  const int seg = tile / A.n_strips;
-  const int x0 = strip * A.SW;
-  const int y0 = seg * FAST_SH;
-  const int rows = min(FAST_SH, A.rh - y0);
-  const int cols = min(A.SW, A.rw - x0);
-  if (rows <= 0 || cols <= 0) return;
-  const int wx0 = A.c0 + x0, wy0 = A.r0 + y0;
-  const int H = A.W.h;
-  const int64_t ld = A.W.ld;
-  const float* Wp = A.W.p;
-  const int nz = A.n_src - (A.field_src >= 0 ? 1 : 0);  // decimated terms
-
-  // vertical running sums over rows wy0-1-r .. wy0-1+r (16 loads in flight)
-  const bool has_col = t < NC;
-  const int jw = has_col ? refl(wx0 - A.R + t, A.W.w) : 0;
-  double S[NRA];
-  int Cn[NRA];
-#pragma unroll
-  for (int i = 0; i < NR; i++) {
-    double s = 0.0;
-    int n = 0;
-    const int r = A.rad[i];
-    for (int d0 = -r; d0 <= r; d0 += 16) {
-      float v[16];
-#pragma unroll
-      for (int e = 0; e < 16; e++)
-        v[e] = urow(Wp, refl1(wy0 - 1 + min(d0 + e, r), H), ld)[jw];
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        if (d0 + e > r) break;
-        if (NANAW) {
-          if (!isnan(v[e])) { s += (double)v[e]; n++; }
-        } else {
-          s += (double)v[e];
-        }
-      }
-    }
-    S[i] = s;
-    Cn[i] = n;
-  }
-
-  const int segs = 256 / K;
-  const int L = A.SW / segs;
-  const int n_chunks = (rows + K - 1) / K;
-  const int tc = min(t, cols - 1);  // this lane's output column (clamped)
-
-  for (int ch = 0; ch < n_chunks; ch++) {
-    const int wyc = wy0 + ch * K;
-    const int nrow = min(K, rows - ch * K);
-
-    // ---------------- phase 1: vertical running sums of K rows
-#pragma unroll
-    for (int kb = 0; kb < K; kb += KB) {
-      float vz[KB], vn[NRA][KB], vo[NRA][KB];
-#pragma unroll
-      for (int k = 0; k < KB; k++) {
-        const int yw = wyc + kb + k;
-        vz[k] = urow(Wp, min(yw, H - 1), ld)[jw];
-#pragma unroll
-        for (int i = 0; i < NR; i++) {
-          vn[i][k] = urow(Wp, refl1(yw + A.rad[i], H), ld)[jw];
-          vo[i][k] = urow(Wp, refl1(yw - A.rad[i] - 1, H), ld)[jw];
-        }
-      }
-      if (has_col) {
-#pragma unroll
-        for (int k = 0; k < KB; k++) {
-#pragma unroll
-          for (int i = 0; i < NR; i++) {
-            if (NANAW) {
-              if (!isnan(vn[i][k])) { S[i] += (double)vn[i][k]; Cn[i]++; }
-              if (!isnan(vo[i][k])) { S[i] -= (double)vo[i][k]; Cn[i]--; }
-              Ws[(i * K + kb + k) * LDC + t] = (float)((double)Cn[i] * A.inv[i]);
-            } else {
-              S[i] += (double)vn[i][k];
-              S[i] -= (double)vo[i][k];
-            }
-            Vs[(i * K + kb + k) * LDC + t] = (float)(S[i] * A.inv[i]);
-          }
-          Zs[(kb + k) * LDC + t] = vz[k];
-        }
-      }
-    }
-    // ---------------- staging: horizontal lerps of the coarse rows this
-    // chunk samples, top/bot = a * (1 - tx) + b * tx per (coarse row, column)
-    for (int s = 0; s < A.n_src; s++) {
-      const SrcRef& SR = A.src[s];
-      int a0, a1, b0, b1;
-      double tt;
-      src_axis(SR, 1, wyc, a0, a1, tt);
-      src_axis(SR, 1, wyc + nrow - 1, b0, b1, tt);
-      const int nr = min(b1 - a0 + 1, SR.rows_max);
-      if (t < A.SW) {
-        int i0, i1;
-        double tx;
-        src_axis(SR, 0, wx0 + tc, i0, i1, tx);
-        const double mx = 1.0 - tx;
-        double* hrow = Hs_t + SR.lds_off + t;
-        const float* prow = SR.p + (int64_t)a0 * SR.ld;
-        for (int rr = 0; rr < nr; rr++) {
-          const double va = prow[i0], vb = prow[i1];
-          hrow[rr * A.SW] = va * mx + vb * tx;
-          prow += SR.ld;
-        }
-      }
-      if (t < K) {
-        int i0, i1;
-        double ty;
-        src_axis(SR, 1, wyc + min(t, nrow - 1), i0, i1, ty);
-        RowI[s * K + t] = make_int2((i0 - a0) * A.SW, (i1 - a0) * A.SW);
-        RowT[s * K + t] = make_double2(ty, 1.0 - ty);
-      }
-    }
-    __syncthreads();
-
-    // ---------------- phase 2: horizontal running sums + box terms in order
-    if (NR > 0) {
-      const int k = t / segs;
-      const int sg = t % segs;
-      if (k < nrow) {
-        const int cbase = A.R + sg * L;
-        double Hsum[NRA], HW[NRA];
-#pragma unroll
-        for (int i = 0; i < NR; i++) {
-          const float* vrow = Vs + (i * K + k) * LDC + cbase;
-          const float* wrow = Ws + (i * K + k) * LDC + cbase;
-          double h = 0.0, hw = 0.0;
-          const int r = A.rad[i];
-#pragma unroll 8
-          for (int d = -r; d <= r; d++) {
-            h += (double)vrow[d];
-            if (NANAW) hw += (double)wrow[d];
-          }
-          Hsum[i] = h;
-          HW[i] = hw;
-        }
-        for (int m = 0; m < L; m++) {
-          const int lc = cbase + m;
-          const float z = Zs[k * LDC + lc];
-          float acc = 0.0f;
-#pragma unroll
-          for (int i = 0; i < NR; i++) {
-            const int r = A.rad[i];
-            if (m > 0) {
-              const float* vrow = Vs + (i * K + k) * LDC;
-              Hsum[i] += (double)vrow[lc + r] - (double)vrow[lc - r - 1];
-              if (NANAW) {
-                const float* wrow = Ws + (i * K + k) * LDC;
-                HW[i] += (double)wrow[lc + r] - (double)wrow[lc - r - 1];
-              }
-            }
-            const float fv = (float)(Hsum[i] * A.inv[i]);
-            float bm;
-            if (NANAW) {
-              const float fw = (float)(HW[i] * A.inv[i]);
-              bm = fw > 0.0f ? fv / fw : 0.0f;
-            } else {
-              bm = fv;
-            }
-            const float diff = A.wt[i] * (z - bm);
-            acc = (i == 0) ? diff : acc + diff;
-          }
-          // each output column belongs to exactly one thread: in-place is safe
-          Zs[k * LDC + lc] = acc;
-        }
-      }
-      __syncthreads();
-    }
-
-    // ---------------- phase 3: decimated terms, field, post, store
-    if (t < cols) {
-      const PostK p1 = load_post(A.o1);
-      const bool two = A.o2.data != nullptr;
-      float zq[K];
-#pragma unroll
-      for (int k = 0; k < K; k++) zq[k] = urow(Wp, min(wyc + k, H - 1), ld)[wx0 + t];
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        if (k < nrow) {
-          const float z = zq[k];
-          float acc = (NR > 0) ? Zs[k * LDC + A.R + t] : 0.0f;
-          for (int j = 0; j < nz; j++) {
-            const int2 ri = RowI[j * K + k];
-            const double2 ty = RowT[j * K + k];
-            const double* h = Hs_t + A.src[j].lds_off + t;
-            const float mean = (float)(h[ri.x] * ty.y + h[ri.y] * ty.x);
-            const float diff = A.wt[NR + j] * (z - mean);
-            acc = (NR == 0 && j == 0) ? diff : acc + diff;
-          }
-          if (isnan(z)) acc = __builtin_nanf("");
-          if (A.field_src >= 0) {
-            const int s = A.field_src;
-            const int2 ri = RowI[s * K + k];
-            const double2 ty = RowT[s * K + k];
-            const double* h = Hs_t + A.src[s].lds_off + t;
-            const float up = (float)(h[ri.x] * ty.y + h[ri.y] * ty.x);
-            const float large = A.wl * z - up;
-            acc = (A.n_terms > 0) ? acc + large : large;
-          }
-          const float v = apply_post(p1, acc);
-          const int yr = y0 + ch * K + k, xr = x0 + t;
-          store_out(A.o1, yr, xr, v);
-          if (two) store_out(A.o2, yr, xr, v);
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
-static size_t fast_lds_bytes(const UsmArgs& A, bool nanaw) {
-  const int LDC = A.SW + 2 * A.R + 1;
-  size_t lds = sizeof(float) * (((size_t)A.nr * FAST_K * LDC * (nanaw ? 2 : 1) +
-                                 (size_t)FAST_K * LDC + 3) & ~(size_t)3);
-  lds += sizeof(double) * (size_t)A.src_floats;  // staged rows (float64)
-  return lds + (sizeof(int2) + sizeof(double2)) * (size_t)MAX_SRC * FAST_K;
+  usm_fast_block<NR, NANAW>(A, tile, lds);
 }
 
 template <int NR, bool NANAW>
@@ -1070,7 +734,7 @@ static int coarse_mean(const float* x, int64_t ldx, int h, int w, int r, float* 
                        float* tv, float* tw, hipStream_t s) {
   if (r <= 1) {
     dim3 blk(128), grd((w + 127) / 128, h);
-    hipLaunchKernelGGL(k_gauss9_grid, grd, blk, 0, s, x, ldx, h, w, out, ldo, make_gauss9_t());
+    hipLaunchKernelGGL(k_gauss9_grid, grd, blk, 0, s, x, ldx, h, w, out, ldo, make_gauss9());
     FSG_CHECK_LAUNCH("k_gauss9_grid");
     return FSG_OK;
   }
@@ -1143,10 +807,12 @@ extern "C" size_t fsg_topousm_workspace_size(const fsg_window* win,
   return plan_coarse(win, p).total;
 }
 
-extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
-                           const fsg_topousm_params* p, const fsg_output* out,
-                           const fsg_output* out2, void* workspace, size_t workspace_bytes,
-                           void* stream) {
+int fsg::usm_prepare(const fsg_window* win, const fsg_region* reg, const fsg_topousm_params* p,
+                     const fsg_output* out, const fsg_output* out2, void* workspace,
+                     size_t workspace_bytes, hipStream_t s, UsmArgs* A_out, bool* fast_out,
+                     int* n_blocks_out, bool launch_prep) {
+  *n_blocks_out = 0;
+  *fast_out = false;
   int rc = validate_window(win, "fsg_topousm");
   if (rc) return rc;
   rc = validate_region(win, reg, "fsg_topousm");
@@ -1160,7 +826,6 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
     return fail(FSG_E_INVALID, "fsg_topousm: no radii and no large field");
   if (win->h == 0 || win->w == 0) return fail(FSG_E_INVALID, "fsg_topousm: empty window");
   if (reg->h == 0 || reg->w == 0) return FSG_OK;
-  hipStream_t s = as_stream(stream);
 
   const CoarsePlan P = plan_coarse(win, p);
   if (P.total > 0 && (!workspace || workspace_bytes < P.total))
@@ -1210,7 +875,7 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
   int* flags = (int*)(ws + P.flag_off);
   float* tmp[4];
   for (int k = 0; k < 4; k++) tmp[k] = (float*)(ws + P.tmp_off[k]);
-  for (int j = 0; j < P.n_ds; j++) {
+  for (int j = 0; j < P.n_ds && launch_prep; j++) {
     rc = decimate(win, P.ds_val[j], (float*)(ws + P.grid_off[j]), P.hc[j], P.wc[j], flags + j,
                   tmp[0], tmp[1], tmp[2], tmp[3], p->nan_hint == 0, s);
     if (rc) return rc;
@@ -1245,9 +910,11 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
       const int j = P.ds_of_radius[i];
       const int r_small = std::max(1, (int)std::nearbyint((double)p->radii[i] / (double)ds));
       float* m = (float*)(ws + P.mean_off[i]);
-      rc = coarse_mean((float*)(ws + P.grid_off[j]), P.wc[j], P.hc[j], P.wc[j], r_small, m,
-                       P.wc[j], tmp[0], tmp[1], s);
-      if (rc) return rc;
+      if (launch_prep) {
+        rc = coarse_mean((float*)(ws + P.grid_off[j]), P.wc[j], P.hc[j], P.wc[j], r_small, m,
+                         P.wc[j], tmp[0], tmp[1], s);
+        if (rc) return rc;
+      }
       mean = m;
       mean_ld = P.wc[j];
     } else if (mean_ld < wc) {
@@ -1296,15 +963,32 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
     A.wl = p->w_large;
   }
   A.src_floats = src_floats;
-  A.G = make_gauss9_t();
+  A.G = make_gauss9();
   A.o1 = make_out(out);
   A.o2 = out2 ? make_out(out2) : OutDev{};
   const int n_segs = (reg->h + A.SH - 1) / A.SH;
+  *A_out = A;
+  *fast_out = fast;
+  *n_blocks_out = A.n_strips * n_segs;
+  return FSG_OK;
+}
+
+extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
+                           const fsg_topousm_params* p, const fsg_output* out,
+                           const fsg_output* out2, void* workspace, size_t workspace_bytes,
+                           void* stream) {
+  hipStream_t s = as_stream(stream);
+  UsmArgs A;
+  bool fast = false;
+  int n_blocks = 0;
+  const int rc =
+      usm_prepare(win, reg, p, out, out2, workspace, workspace_bytes, s, &A, &fast, &n_blocks);
+  if (rc || n_blocks == 0) return rc;
   // The NaN-aware variant is exact for every window; nan_hint == 0 (the caller
   // knows the window holds no NaN) selects the cheaper plain-mean variant,
   // whose results are identical on NaN-free windows.
-  if (fast) return dispatch_fast(A, /*nanaw=*/p->nan_hint != 0, A.n_strips * n_segs, s);
-  return dispatch_usm(A, /*nanaw=*/p->nan_hint != 0, A.n_strips * n_segs, s);
+  if (fast) return dispatch_fast(A, /*nanaw=*/p->nan_hint != 0, n_blocks, s);
+  return dispatch_usm(A, /*nanaw=*/p->nan_hint != 0, n_blocks, s);
 }
 
 extern "C" size_t fsg_topousm_coarse_means_workspace_size(const fsg_window* win, int32_t ds) {

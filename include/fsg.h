@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 2
+#define FSG_ABI_VERSION 3
 
 enum {
   FSG_OK = 0,
@@ -359,6 +359,61 @@ int fsg_encode(const fsg_window* src, const fsg_region* reg,
 /* Gather the region of a window into a dense f32 pool (stats pre-pass). */
 int fsg_copy_region(const fsg_window* win, const fsg_region* reg, float* dst,
                     void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Batched fused launch                                                 */
+/* ------------------------------------------------------------------ */
+
+/*
+ * One tile of a batch: an independent window, the region computed, and the
+ * outputs wanted from it (an output with data == NULL is not computed;
+ * topousm needs n_radii > 0 or a field).  The parameters mean what they mean
+ * for fsg_hillshade_slope, fsg_curvature and fsg_topousm.
+ */
+typedef struct fsg_batch_tile {
+  fsg_window win;
+  fsg_region reg;
+  fsg_output hillshade, slope, curvature, topousm, topousm2;
+  fsg_shade_params shade;
+  fsg_curvature_params curv;
+  fsg_topousm_params usm;
+} fsg_batch_tile;
+
+/* Launch shape of the persistent kernel (what the tuner searches). */
+typedef struct fsg_batch_config {
+  int32_t wg_per_cu;  /* persistent workgroups per CU (0: 3) */
+  int32_t order;      /* 0: TopoUSM blocks first, 1: stencil blocks first */
+  int32_t n_cu;       /* CUs to size the grid for (0: the device's) */
+  int32_t pad0;
+} fsg_batch_config;
+
+typedef struct fsg_batch fsg_batch;
+
+/*
+ * The batch replaces the reference's per-tile, per-algorithm loop
+ * (core/tile_processor.py:1635-1683 -> run_tile_algorithm,
+ * core/tile_compute.py:93-129) for hillshade, slope, curvature and
+ * topousm_fast: every tile's work goes through ONE persistent kernel that
+ * drains a device queue of TopoUSM main-pass blocks and fused stencil blocks,
+ * after the per-tile coarse-grid preparation.  Outputs equal the per-tile
+ * entry points' bit for bit.  TopoUSM tiles off the fast path, or with NaN
+ * (nan_hint != 0), or with a box-radius count other than the batch's most
+ * common one run their own fsg_topousm launches inside the same sequence.
+ *
+ * fsg_batch_create plans the batch (host work, one device allocation for the
+ * coarse grids and descriptors, a synchronous upload); the tiles' buffers must
+ * outlive it.  fsg_batch_run enqueues one pass on a stream; fsg_batch_capture
+ * records one pass into a hipGraph (non-default stream; per-kernel timing
+ * off) and fsg_batch_replay launches it.  fsg_batch_info fills int64[8]:
+ * TopoUSM items, stencil items, fused TopoUSM tiles, fallback tiles, stencil
+ * tiles, LDS bytes, box radii of the fused kernel, device bytes held.
+ */
+int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batch** out);
+int fsg_batch_run(fsg_batch* b, const fsg_batch_config* cfg, void* stream);
+int fsg_batch_capture(fsg_batch* b, const fsg_batch_config* cfg, void* stream);
+int fsg_batch_replay(fsg_batch* b, void* stream);
+int fsg_batch_info(const fsg_batch* b, int64_t* info);
+void fsg_batch_destroy(fsg_batch* b);
 
 #ifdef __cplusplus
 }
