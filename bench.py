@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--size", type=int, default=N_PX)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=0, help="tiles in the CPU sample (0 = auto)")
+    ap.add_argument("--only-c2", action="store_true",
+                    help="skip the per-algorithm workloads (C1 hillshade+slope, C3 curvature and "
+                         "openness, C5 fused batch)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy a 8192x8192 float32 region once with "
                          "k_copy_region (256 MiB read + 256 MiB written): the reference point "
@@ -130,6 +133,183 @@ def cpu_baseline(dem, plan, stats_scale, gpu_f32, n_tiles):
     }
 
 
+# ---------------------------------------------------------------- per algorithm
+
+def _gpu_time(fn, reps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def bench_c1(dev):
+    """Hillshade + slope, local, 2048^2 fBm (H 0.8, 0-3000 m, 10 m, seed 1):
+    one fused launch; the oracle's hillshade + slope timed on the host."""
+    import numpy as np
+    import torch
+    from fujishadergpu_amd import ops, synth
+    from fujishadergpu_amd.algorithms import HillshadeSlopeFused, OutSpec
+    n = 2048
+    dem = synth.fbm(n, n, 0.8, 3000.0, seed=1, device=dev)
+    hs = torch.empty_like(dem)
+    sl = torch.empty_like(dem)
+    algo = HillshadeSlopeFused()
+    p = {"pixel_size": 10.0, "azimuth": 315.0, "altitude": 45.0, "_window_nan_free": True}
+    run = lambda: algo.run_region(dem, ops.full_region(dem), OutSpec(f32=hs, clip01=True), p,  # noqa
+                                  out_slope=OutSpec(f32=sl))
+    run()
+    t = _gpu_time(run, 50)
+    from oracle import ref
+    host = dem.cpu().numpy()
+    t0 = time.perf_counter()
+    ehs = ref.hillshade(host, pixel_size=10.0)
+    esl = ref.slope(host, pixel_size=10.0)
+    tc = time.perf_counter() - t0
+    d = max(float(np.abs(hs.cpu().numpy() - ehs).max()), float(np.abs(sl.cpu().numpy() - esl).max()))
+    px = n * n
+    return {"workload": "hillshade+slope fused, local, 2048x2048 f32 fBm (H=0.8, 0-3000 m, 10 m, "
+                        "seed 1), no NoData", "gpix_s": round(px / t / 1e9, 3),
+            "ms": round(t * 1e3, 4), "hbm_frac": round(12 * px / t / (HBM_PEAK_GBS * 1e9), 4),
+            "alg_bytes_per_px": 12,
+            "cpu_baseline": {"value": round(px / tc / 1e9, 4), "unit": "Gpix/s", "cores": 1,
+                             "kind": "port", "sample": "whole 2048^2 raster, oracle hillshade + "
+                                                       "slope (NumPy, one thread of calls)"},
+            "max_abs_diff_vs_oracle": d}
+
+
+def bench_c3(dev, n=32768):
+    """Curvature (mean) and openness (positive, N=8, D=64), local mode, on a
+    32768^2 fBm (H 0.85, 0-2500 m, 10 m, seed 3) with 1% NoData discs (seed 33),
+    tiles 4096 + the local 64 px halo; the tile size is picked by measuring."""
+    import torch
+    from fujishadergpu_amd import synth
+    from fujishadergpu_amd.scheduler import process_dem
+    from fujishadergpu_amd.tuner import tune_tile_size
+    dem = synth.fbm(n, n, 0.85, 2500.0, seed=3, device=dev)
+    discs = synth.nodata_discs(n, n, 0.01, seed=33)
+    synth.apply_discs(dem, discs, -9999.0)
+    out = torch.empty_like(dem)
+    res = {}
+    for algo, kw in (("curvature", dict(curvature_type="mean")),
+                     ("openness", dict(openness_type="positive", num_directions=8,
+                                       max_distance=64))):
+        def run(ts, _a=algo, _kw=kw):
+            process_dem(dem, _a, tile_size=ts, pixel_size=10.0, nodata=-9999.0, mode="local",
+                        out=out, copy_input=False, **_kw)
+        print(f"#   c3 {algo}: tuning the tile size", file=sys.stderr, flush=True)
+        best, times = tune_tile_size(run, (2048, 4096, 8192), reps=1)
+        t = _gpu_time(lambda: run(best), 2)
+        px = n * n
+        res[algo] = {"gpix_s": round(px / t / 1e9, 3), "ms": round(t * 1e3, 2),
+                     "hbm_frac": round(8 * px / t / (HBM_PEAK_GBS * 1e9), 4),
+                     "alg_bytes_per_px": 8, "tile_size": best,
+                     "tuner_ms": {str(k): round(v * 1e3, 2) for k, v in times.items()}}
+    res["workload"] = (f"{n}x{n} f32 fBm (H=0.85, 0-2500 m, 10 m, seed 3), 1% NoData discs "
+                       "r 5-50 px (seed 33, sentinel -9999), local mode, whole job per algorithm "
+                       "(NoData scan, tiles, openness p1/p99 stretch pre-pass)")
+    del dem, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def c5_jobs(dev, n_tiles=256):
+    """256 independent tiles, side log-uniform in [256, 4096] (seeds 50000+i),
+    fBm H ~ U[0.6, 0.9], relief ~ U[500, 3000] m; even tiles projected 10 m,
+    odd tiles geographic 1 arc-second at a latitude ~ U[30, 60] N.  Each tile
+    is synthesised at the next power of two and cropped (five FFT sizes in
+    all, so the FFT library compiles five kernels, not hundreds)."""
+    import math
+
+    import numpy as np
+    import torch
+    from fujishadergpu_amd import synth
+    from fujishadergpu_amd.batch import TileJob
+    from fujishadergpu_amd.raster_info import meters_per_degree
+    jobs = []
+    for i in range(n_tiles):
+        r = np.random.default_rng(50000 + i)
+        side = int(round(math.exp(r.uniform(math.log(256), math.log(4096)))))
+        hurst, relief = r.uniform(0.6, 0.9), r.uniform(500.0, 3000.0)
+        big = 1 << max(0, (side - 1).bit_length())
+        dem = synth.fbm(big, big, hurst, 1.0, seed=50000 + i, device=dev)[:side, :side]
+        lo, hi = dem.min(), dem.max()
+        dem = ((dem - lo) / torch.clamp(hi - lo, min=1e-30) * float(relief)).contiguous()
+        if i % 32 == 31:
+            print(f"#   c5 tiles synthesised: {i + 1}", file=sys.stderr, flush=True)
+        p = {"radii": [2, 8, 32, 128], "weights": [8 / 15, 4 / 15, 2 / 15, 1 / 15],
+             "curvature_type": "mean", "azimuth": 315.0, "altitude": 45.0}
+        if i % 2 == 0:
+            p["pixel_size"] = 10.0
+        else:
+            m_lon, m_lat = meters_per_degree(float(r.uniform(30.0, 60.0)))
+            sx, sy = m_lon / 3600.0, m_lat / 3600.0
+            p.update(pixel_scale_x=sx, pixel_scale_y=-sy, pixel_size=0.5 * (sx + sy))
+        e = lambda: torch.empty((side, side), dtype=torch.float32, device=dev)  # noqa: E731
+        jobs.append(TileJob(dem=dem, params=p, hillshade=e(), slope=e(), curvature=e(),
+                            topousm=e(), nan_free=True))
+    return jobs
+
+
+def bench_c5(dev):
+    """Fused batch (one persistent launch, hipGraph replay) vs per-tile,
+    per-algorithm launches over the 256 ragged tiles."""
+    import torch
+    from fujishadergpu_amd.batch import FusedBatch, run_unfused, tune
+    from fujishadergpu_amd.stats import norm_stats_prepass
+    jobs = c5_jobs(dev)
+    # TopoUSM normalisation: one p99 scale (pre-pass over the first tile) for all
+    st = norm_stats_prepass(jobs[0].dem, "topousm_fast",
+                            {"radii": [2, 8, 32, 128], "weights": jobs[0].params["weights"],
+                             "pixel_size": jobs[0].params["pixel_size"]})
+    for j in jobs:
+        j.params["global_stats"] = st
+    px = sum(j.dem.numel() for j in jobs)
+    print("#   c5 unfused per-tile runs", file=sys.stderr, flush=True)
+    run_unfused(jobs)
+    t_unf = _gpu_time(lambda: run_unfused(jobs), 2)
+    sample = [0, 1, 2, 3, len(jobs) // 2, len(jobs) - 1]
+    ref_out = {i: [t.clone() for t in (jobs[i].hillshade, jobs[i].slope, jobs[i].curvature,
+                                       jobs[i].topousm)] for i in sample}
+    print("#   c5 fused batch: plan, tune, capture", file=sys.stderr, flush=True)
+    b = FusedBatch(jobs)
+    info = b.info()
+    best, times = tune(b, reps=2)
+    t_eager = times[best]
+    s = torch.cuda.Stream()
+    b.capture(s)
+    for j in (jobs[i] for i in sample):
+        for t in (j.hillshade, j.slope, j.curvature, j.topousm):
+            t.fill_(-7.0)
+    b.replay(s)
+    s.synchronize()
+    same = all(torch.equal(a.view(torch.int32), c.view(torch.int32))
+               for i in sample for a, c in zip(ref_out[i], (jobs[i].hillshade, jobs[i].slope,
+                                                            jobs[i].curvature, jobs[i].topousm)))
+
+    def replay():
+        b.replay(s)
+        s.synchronize()
+    t_graph = _gpu_time(replay, 5)
+    b.close()
+    res = {"workload": "256 tiles, side log-U[256, 4096] (seeds 50000+i), fBm H~U[0.6,0.9], "
+                       "half projected 10 m, half geographic 1 arc-second at 30-60 N (per-tile "
+                       "latitude); hillshade+slope+curvature(mean)+topousm{2,8,32,128}, f32",
+           "tiles": len(jobs), "mpx": round(px / 1e6, 1),
+           "gpix_s": round(px / t_graph / 1e9, 3), "ms_graph": round(t_graph * 1e3, 3),
+           "hbm_frac": round(20 * px / t_graph / (HBM_PEAK_GBS * 1e9), 4), "alg_bytes_per_px": 20,
+           "ms_fused_eager": round(t_eager * 1e3, 3), "ms_unfused_per_tile": round(t_unf * 1e3, 3),
+           "speedup_vs_unfused": round(t_unf / t_graph, 2),
+           "tuned_config": {"wg_per_cu": best[0], "order": best[1], "chunk": best[2]},
+           "tuner_ms": {"/".join(map(str, k)): round(v * 1e3, 3) for k, v in times.items()},
+           "batch": info, "identical_to_unfused": bool(same)}
+    del jobs, ref_out
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -209,6 +389,17 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(dem, plan, stats_scale, out_f32, args.cpu_tiles)
+    if not args.only_c2:
+        del dem, out_u8, out_f32
+        torch.cuda.empty_cache()
+        line["per_algorithm"] = {"topousm_fast_c2": {
+            "gpix_s": line["value"], "hbm_frac_whole_job": round(
+                9 * n * n / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
+            "kernel_hbm_frac": line["roofline"]["frac"]}}
+        for name, fn in (("hillshade_slope_c1", bench_c1), ("batch_fused_c5", bench_c5),
+                         ("curvature_openness_c3", bench_c3)):
+            print(f"# {name} ...", file=sys.stderr, flush=True)
+            line["per_algorithm"][name] = fn(dev)
     print(json.dumps(line), flush=True)
 
 

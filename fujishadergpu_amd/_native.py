@@ -100,7 +100,7 @@ class fsg_batch_tile(C.Structure):
 
 class fsg_batch_config(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int32), ("order", C.c_int32), ("n_cu", C.c_int32),
-                ("pad0", C.c_int32)]
+                ("chunk", C.c_int32)]
 
 
 P = C.POINTER

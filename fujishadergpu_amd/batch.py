@@ -94,7 +94,7 @@ class FusedBatch:
         N.check(N.lib().fsg_batch_create(arr, len(self.jobs), C.byref(h)))
         self._h = h
         self._arr = arr
-        self.config = N.fsg_batch_config(3, 0, 0, 0)
+        self.config = N.fsg_batch_config(3, 0, 0, 8)
 
     def info(self) -> Dict[str, int]:
         v = (C.c_int64 * 8)()
@@ -108,8 +108,9 @@ class FusedBatch:
             return self.config
         if isinstance(config, N.fsg_batch_config):
             return config
-        wg, order = config
-        return N.fsg_batch_config(int(wg), int(order), 0, 0)
+        wg, order = config[0], config[1]
+        chunk = config[2] if len(config) > 2 else 8
+        return N.fsg_batch_config(int(wg), int(order), 0, int(chunk))
 
     def run(self, config=None, stream=None) -> None:
         s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
@@ -165,15 +166,15 @@ def _time(fn, reps: int) -> float:
     return (time.perf_counter() - t0) / reps
 
 
-def tune(batch: FusedBatch, candidates: Sequence[Tuple[int, int]] = ((2, 0), (3, 0), (4, 0),
-                                                                       (3, 1), (6, 0)),
-         reps: int = 3) -> Tuple[Tuple[int, int], Dict[Tuple[int, int], float]]:
-    """Measure each (workgroups per CU, item order) on the batch itself and keep
-    the fastest as the batch's configuration."""
+def tune(batch: FusedBatch, candidates: Sequence[Tuple[int, int, int]] = (
+        (2, 0, 8), (4, 0, 8), (4, 0, 32), (4, 1, 8), (8, 0, 16)),
+         reps: int = 3) -> Tuple[Tuple[int, ...], Dict[Tuple[int, ...], float]]:
+    """Measure each (workgroups per CU, item order, items per queue claim) on
+    the batch itself and keep the fastest as the batch's configuration."""
     times = {}
     for c in candidates:
         batch.run(c)                       # warm
         times[tuple(c)] = _time(lambda: batch.run(c), reps)
     best = min(times, key=times.get)
-    batch.config = N.fsg_batch_config(int(best[0]), int(best[1]), 0, 0)
+    batch.config = batch._cfg(best)
     return best, times

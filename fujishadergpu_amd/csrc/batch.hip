@@ -9,7 +9,9 @@
 // Work items are TopoUSM main-pass blocks (SW x FAST_SH of a tile region,
 // the heavier items) and stencil blocks (64 x 16 output pixels).  A stencil
 // item stages the block's filled window values with a 3-pixel halo in LDS
-// once and derives hillshade, slope and curvature from them; both item kinds
+// once, tabulates the first derivatives and derives hillshade, slope and
+// curvature from them (stencil_tile_block, the per-window kernels' block
+// function too); both item kinds
 // run the same device functions as the per-window kernels (stencil.hpp,
 // usm.hpp), so fused and unfused outputs are identical bit for bit.
 // Per-tile coarse-grid preparation (decimation, coarse box means) is
@@ -24,68 +26,6 @@
 
 namespace fsg {
 
-constexpr int ST_TW = 64, ST_TH = 16, ST_HALO = 3;
-constexpr int ST_LT = ST_TW + 2 * ST_HALO + 1;  // LDS row stride (floats)
-constexpr int ST_ROWS = ST_TH + 2 * ST_HALO;
-
-struct StencilTile {
-  WinDev W;
-  int r0, c0, rh, rw;
-  int nbx;  // stencil blocks per block row
-  int nanaw;
-  int want_hs, want_sl, want_cv;
-  ShadeK shade;
-  CurvK curv;
-  OutDev hs, sl, cv;
-};
-
-template <bool NANAW>
-__device__ __forceinline__ void stencil_block(const StencilTile& S, int blk, const Gauss9& G,
-                                              float* lds) {
-  const int by = blk / S.nbx, bx = blk - by * S.nbx;
-  const int rl = S.r0 + by * ST_TH, cl = S.c0 + bx * ST_TW;  // window coords of the block
-  const int tr0 = max(rl - ST_HALO, 0), tr1 = min(rl + ST_TH + ST_HALO, S.W.h);
-  const int tc0 = max(cl - ST_HALO, 0), tc1 = min(cl + ST_TW + ST_HALO, S.W.w);
-  const int nr = tr1 - tr0, nc = tc1 - tc0;
-  const int t = threadIdx.x;
-  for (int e = t; e < nr * nc; e += 256) {
-    const int rr = e / nc, cc = e - rr * nc;
-    lds[rr * ST_LT + cc] = filled<NANAW>(S.W, tr0 + rr, tc0 + cc, G);
-  }
-  __syncthreads();
-  // every derivative footprint of a block pixel lies within 3 rows/columns
-  // of it and inside the window, i.e. inside the staged tile
-  auto f = [&](int r, int c) { return lds[(r - tr0) * ST_LT + (c - tc0)]; };
-  const int c = t & 63;
-  const PostK phs = load_post(S.hs), psl = load_post(S.sl), pcv = load_post(S.cv);
-  for (int k = t >> 6; k < ST_TH; k += 4) {
-    const int rr = by * ST_TH + k, cc = bx * ST_TW + c;  // region coordinates
-    if (rr < S.rh && cc < S.rw) {
-      const int wr = S.r0 + rr, wc = S.c0 + cc;
-      const float z = NANAW ? raw(S.W, wr, wc) : f(wr, wc);
-      if (isnan(z)) {
-        const float qn = __builtin_nanf("");
-        if (S.want_hs) store_out(S.hs, rr, cc, apply_post(phs, qn));
-        if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, qn));
-        if (S.want_cv) store_out(S.cv, rr, cc, apply_post(pcv, qn));
-      } else {
-        if (S.want_hs || S.want_sl) {
-          float dx, dy;
-          grad_at(S.shade.ax, S.shade.ay, wr, wc, S.W.h, S.W.w, f, dx, dy);
-          if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, slope_from(S.shade, dx, dy)));
-          if (S.want_hs)
-            store_out(S.hs, rr, cc,
-                      apply_post(phs, hillshade_from(S.shade, wr, wc, S.W.h, S.W.w, f, dx, dy)));
-        }
-        if (S.want_cv)
-          store_out(S.cv, rr, cc,
-                    apply_post(pcv, curvature_at(S.curv, wr, wc, S.W.h, S.W.w, f)));
-      }
-    }
-  }
-  __syncthreads();
-}
-
 struct BatchDev {
   const UsmArgs* usm;
   const int* usm_pre;  // prefix of main-pass blocks per fused TopoUSM tile (n + 1)
@@ -95,6 +35,8 @@ struct BatchDev {
   int n_st_tiles, n_st_items;
   int* queue;
   int order;           // 0: TopoUSM items first, 1: stencil items first
+  int chunk;           // items claimed per queue access
+  int pre_off;         // LDS byte offset of the staged prefix arrays
   Gauss9 G;
 };
 
@@ -113,34 +55,49 @@ __global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
   extern __shared__ float lds[];
   __shared__ int s_item;
   const int total = B.n_usm_items + B.n_st_items;
+  // both prefix arrays staged in LDS once: tile lookups stay on chip
+  int* upre = (int*)((char*)lds + B.pre_off);
+  int* spre = upre + B.n_usm_tiles + 1;
+  for (int e = threadIdx.x; e <= B.n_usm_tiles; e += 256) upre[e] = B.usm_pre[e];
+  for (int e = threadIdx.x; e <= B.n_st_tiles; e += 256) spre[e] = B.st_pre[e];
   // every workgroup leaves once the queue passes the item count
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(B.queue, 1);
+    if (threadIdx.x == 0) s_item = atomicAdd(B.queue, B.chunk);
     __syncthreads();
-    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    const int base = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
-    if (item >= total) break;
-    bool is_usm;
-    int it;
-    if (B.order == 0) {
-      is_usm = item < B.n_usm_items;
-      it = is_usm ? item : item - B.n_usm_items;
-    } else {
-      is_usm = item >= B.n_st_items;
-      it = is_usm ? item - B.n_st_items : item;
-    }
-    if (is_usm) {
-      const int tl = find_tile(B.usm_pre, B.n_usm_tiles, it);
-      usm_fast_block<NR, false>(B.usm[tl], it - B.usm_pre[tl], lds);
-    } else {
-      const int tl = find_tile(B.st_pre, B.n_st_tiles, it);
-      const StencilTile& S = B.st[tl];
-      if (S.nanaw)
-        stencil_block<true>(S, it - B.st_pre[tl], B.G, lds);
-      else
-        stencil_block<false>(S, it - B.st_pre[tl], B.G, lds);
+    if (base >= total) break;
+    const int end = min(base + B.chunk, total);
+    for (int item = base; item < end; item++) {
+      bool is_usm;
+      int it;
+      if (B.order == 0) {
+        is_usm = item < B.n_usm_items;
+        it = is_usm ? item : item - B.n_usm_items;
+      } else {
+        is_usm = item >= B.n_st_items;
+        it = is_usm ? item - B.n_st_items : item;
+      }
+      if (is_usm) {
+        const int tl = __builtin_amdgcn_readfirstlane(find_tile(upre, B.n_usm_tiles, it));
+        usm_fast_block<NR, false>(B.usm[tl], it - upre[tl], lds);
+      } else {
+        const int tl = __builtin_amdgcn_readfirstlane(find_tile(spre, B.n_st_tiles, it));
+        const StencilTile& S = B.st[tl];
+        const int blk = it - spre[tl];
+        const int by = blk / S.nbx, bx = blk - by * S.nbx;
+        // NaN-free tiles only: the NaN-aware fill (a call) stays out of this
+        // kernel's code; those tiles take their own launches
+        stencil_tile_block<false>(S, by, bx, B.G, lds);
+      }
     }
   }
+}
+
+// NaN-aware stencil tiles of a batch, one launch per tile.
+__global__ void __launch_bounds__(256) k_stencil_nanaw(StencilTile S, Gauss9 G) {
+  __shared__ float lds[ST_LDS_FLOATS];
+  stencil_tile_block<true>(S, blockIdx.y, blockIdx.x, G, lds);
 }
 
 __global__ void k_batch_reset(int* __restrict__ q) {
@@ -156,6 +113,7 @@ struct fsg_batch {
   std::vector<size_t> ws_off;         // per tile: TopoUSM workspace offset, or SIZE_MAX
   std::vector<int> usm_fused;         // tile indices in the fused kernel
   std::vector<int> usm_fallback;      // tile indices run with per-tile launches
+  std::vector<StencilTile> st_fallback;  // NaN-aware stencil tiles (own launches)
   int nr = -1;
   size_t lds = 0;
   int n_usm_items = 0, n_st_items = 0, n_st_tiles = 0;
@@ -262,7 +220,7 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   b->nr = best;
   std::vector<UsmArgs> fused_args;
   std::vector<int> usm_pre{0};
-  size_t lds = sizeof(float) * ST_ROWS * ST_LT;
+  size_t lds = sizeof(float) * ST_LDS_FLOATS;
   for (int i = 0; i < n; i++) {
     if (!wants_usm(b->tiles[i])) continue;
     if (fast_ok[i] && args[i].nr == best) {
@@ -295,9 +253,15 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
               (S.want_cv ? t.curv.nan_hint != 0 : 0);
     if (S.want_hs || S.want_sl) S.shade = make_shadek(&t.shade, S.want_hs, S.want_sl);
     if (S.want_cv) S.curv = make_curvk(&t.curv);
+    S.shade_from_tables = !S.want_cv || (same_axis(S.shade.ax, S.curv.ax) &&
+                                         same_axis(S.shade.ay, S.curv.ay));
     S.hs = S.want_hs ? make_out(&t.hillshade) : OutDev{};
     S.sl = S.want_sl ? make_out(&t.slope) : OutDev{};
     S.cv = S.want_cv ? make_out(&t.curvature) : OutDev{};
+    if (S.nanaw) {
+      b->st_fallback.push_back(S);
+      continue;
+    }
     st.push_back(S);
     st_pre.push_back(st_pre.back() + S.nbx * ((t.reg.h + ST_TH - 1) / ST_TH));
   }
@@ -350,6 +314,11 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
                                s);
     if (rc) return rc;
   }
+  for (const StencilTile& S : b->st_fallback) {
+    dim3 grd(S.nbx, (S.rh + ST_TH - 1) / ST_TH);
+    hipLaunchKernelGGL(k_stencil_nanaw, grd, dim3(256), 0, s, S, make_gauss9());
+    FSG_CHECK_LAUNCH("k_stencil_nanaw");
+  }
   const int total = b->n_usm_items + b->n_st_items;
   if (total == 0) return FSG_OK;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_queue);
@@ -375,17 +344,23 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
   D.n_st_items = b->n_st_items;
   D.queue = b->d_queue;
   D.order = cfg ? cfg->order : 0;
+  D.chunk = (cfg && cfg->chunk > 0) ? cfg->chunk : 8;
+  D.pre_off = (int)((b->lds + 15) & ~(size_t)15);
+  const size_t lds_total =
+      (size_t)D.pre_off + sizeof(int) * ((size_t)D.n_usm_tiles + D.n_st_tiles + 2);
+  if (lds_total > 150 * 1024)
+    return fail(FSG_E_UNSUPPORTED, "fsg_batch_run: too many tiles for the LDS tile index");
   D.G = make_gauss9();
   const int nr = b->usm_fused.empty() ? 0 : b->nr;
   TimedScope ts("k_batch_fused", s);
   switch (nr) {
-    case 0: launch_fused<0>(D, grid, b->lds, s); break;
-    case 1: launch_fused<1>(D, grid, b->lds, s); break;
-    case 2: launch_fused<2>(D, grid, b->lds, s); break;
-    case 3: launch_fused<3>(D, grid, b->lds, s); break;
-    case 4: launch_fused<4>(D, grid, b->lds, s); break;
-    case 5: launch_fused<5>(D, grid, b->lds, s); break;
-    case 6: launch_fused<6>(D, grid, b->lds, s); break;
+    case 0: launch_fused<0>(D, grid, lds_total, s); break;
+    case 1: launch_fused<1>(D, grid, lds_total, s); break;
+    case 2: launch_fused<2>(D, grid, lds_total, s); break;
+    case 3: launch_fused<3>(D, grid, lds_total, s); break;
+    case 4: launch_fused<4>(D, grid, lds_total, s); break;
+    case 5: launch_fused<5>(D, grid, lds_total, s); break;
+    case 6: launch_fused<6>(D, grid, lds_total, s); break;
     default: return fail(FSG_E_UNSUPPORTED, "fsg_batch_run: more than 6 box radii");
   }
   FSG_CHECK_LAUNCH("k_batch_fused");
@@ -434,7 +409,7 @@ extern "C" int fsg_batch_info(const fsg_batch* b, int64_t* info) {
   info[1] = b->n_st_items;
   info[2] = (int64_t)b->usm_fused.size();
   info[3] = (int64_t)b->usm_fallback.size();
-  info[4] = b->n_st_tiles;
+  info[4] = b->n_st_tiles + (int64_t)b->st_fallback.size();
   info[5] = (int64_t)b->lds;
   info[6] = b->nr;
   info[7] = (int64_t)b->dev_bytes;

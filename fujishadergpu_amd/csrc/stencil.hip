@@ -1,5 +1,6 @@
 // Gradient stencils: fused hillshade + slope, and curvature, one window per
-// launch.  The arithmetic lives in stencil.hpp.
+// launch, one workgroup per 64 x 16 block; the block function and the
+// arithmetic live in stencil.hpp (shared with the batched launch).
 #include <algorithm>
 #include <cmath>
 
@@ -7,45 +8,33 @@
 
 namespace fsg {
 
+// One workgroup per 64 x 16 block of the region (stencil.hpp).
 template <bool NANAW>
-__global__ void __launch_bounds__(256)
-k_hillshade_slope(WinDev W, int r0, int c0, int rh, int rw, ShadeK K, Gauss9 G, OutDev ohs,
-                  OutDev osl) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y;
-  if (c >= rw || r >= rh) return;
-  const int wr = r0 + r, wc = c0 + c;
-  const float zc = raw(W, wr, wc);
-  const PostK phs = load_post(ohs), psl = load_post(osl);
-  if (isnan(zc)) {
-    const float qn = __builtin_nanf("");
-    if (K.need_hs) store_out(ohs, r, c, apply_post(phs, qn));
-    if (K.need_sl) store_out(osl, r, c, apply_post(psl, qn));
-    return;
-  }
-  auto f = [&](int i, int j) { return filled<NANAW>(W, i, j, G); };
-  float dx, dy;
-  grad_at(K.ax, K.ay, wr, wc, W.h, W.w, f, dx, dy);
-  if (K.need_sl) store_out(osl, r, c, apply_post(psl, slope_from(K, dx, dy)));
-  if (K.need_hs)
-    store_out(ohs, r, c, apply_post(phs, hillshade_from(K, wr, wc, W.h, W.w, f, dx, dy)));
+__global__ void __launch_bounds__(256) k_stencil_tiles(StencilTile S, Gauss9 G) {
+  __shared__ float lds[ST_LDS_FLOATS];
+  stencil_tile_block<NANAW>(S, blockIdx.y, blockIdx.x, G, lds);
 }
 
-template <bool NANAW>
-__global__ void __launch_bounds__(256)
-k_curvature(WinDev W, int r0, int c0, int rh, int rw, CurvK K, Gauss9 G, OutDev out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y;
-  if (c >= rw || r >= rh) return;
-  const int i = r0 + r, j = c0 + c;
-  const PostK pp = load_post(out);
-  const float zc = raw(W, i, j);
-  if (isnan(zc)) {
-    store_out(out, r, c, apply_post(pp, __builtin_nanf("")));
-    return;
-  }
-  auto f = [&](int a, int b) { return filled<NANAW>(W, a, b, G); };
-  store_out(out, r, c, apply_post(pp, curvature_at(K, i, j, W.h, W.w, f)));
+static int launch_stencil(const StencilTile& S, const char* name, hipStream_t s) {
+  dim3 grd(S.nbx, (S.rh + ST_TH - 1) / ST_TH);
+  TimedScope ts(name, s);
+  if (S.nanaw)
+    hipLaunchKernelGGL(k_stencil_tiles<true>, grd, dim3(256), 0, s, S, make_gauss9());
+  else
+    hipLaunchKernelGGL(k_stencil_tiles<false>, grd, dim3(256), 0, s, S, make_gauss9());
+  FSG_CHECK_LAUNCH(name);
+  return FSG_OK;
+}
+
+static StencilTile stencil_tile(const fsg_window* win, const fsg_region* reg) {
+  StencilTile S{};
+  S.W = WinDev{win->data, win->ld, win->h, win->w};
+  S.r0 = reg->r0;
+  S.c0 = reg->c0;
+  S.rh = reg->h;
+  S.rw = reg->w;
+  S.nbx = (reg->w + ST_TW - 1) / ST_TW;
+  return S;
 }
 
 }  // namespace fsg
@@ -70,20 +59,15 @@ extern "C" int fsg_hillshade_slope(const fsg_window* win, const fsg_region* reg,
     return fail(FSG_E_INVALID, "fsg_hillshade_slope: steps must be positive");
   if (!hillshade && !slope) return FSG_OK;
   if (reg->h == 0 || reg->w == 0) return FSG_OK;
-  const ShadeK K = make_shadek(p, hillshade != nullptr, slope != nullptr);
-  WinDev W{win->data, win->ld, win->h, win->w};
-  dim3 blk(64, 4), grd((reg->w + 63) / 64, (reg->h + 3) / 4);
-  TimedScope ts("k_hillshade_slope", as_stream(stream));
-  if (p->nan_hint != 0)
-    hipLaunchKernelGGL(k_hillshade_slope<true>, grd, blk, 0, as_stream(stream), W, reg->r0,
-                       reg->c0, reg->h, reg->w, K, make_gauss9(), make_out(hillshade),
-                       make_out(slope));
-  else
-    hipLaunchKernelGGL(k_hillshade_slope<false>, grd, blk, 0, as_stream(stream), W, reg->r0,
-                       reg->c0, reg->h, reg->w, K, make_gauss9(), make_out(hillshade),
-                       make_out(slope));
-  FSG_CHECK_LAUNCH("k_hillshade_slope");
-  return FSG_OK;
+  StencilTile S = stencil_tile(win, reg);
+  S.shade = make_shadek(p, hillshade != nullptr, slope != nullptr);
+  S.want_hs = hillshade != nullptr;
+  S.want_sl = slope != nullptr;
+  S.shade_from_tables = 1;
+  S.nanaw = p->nan_hint != 0;
+  if (hillshade) S.hs = make_out(hillshade);
+  if (slope) S.sl = make_out(slope);
+  return launch_stencil(S, "k_hillshade_slope", as_stream(stream));
 }
 
 extern "C" int fsg_curvature(const fsg_window* win, const fsg_region* reg,
@@ -104,16 +88,10 @@ extern "C" int fsg_curvature(const fsg_window* win, const fsg_region* reg,
   if (p->step_x == 0.0 || p->step_y == 0.0)
     return fail(FSG_E_INVALID, "fsg_curvature: zero step");
   if (reg->h == 0 || reg->w == 0) return FSG_OK;
-  const CurvK K = make_curvk(p);
-  WinDev W{win->data, win->ld, win->h, win->w};
-  dim3 blk(64, 4), grd((reg->w + 63) / 64, (reg->h + 3) / 4);
-  TimedScope ts("k_curvature", as_stream(stream));
-  if (p->nan_hint != 0)
-    hipLaunchKernelGGL(k_curvature<true>, grd, blk, 0, as_stream(stream), W, reg->r0, reg->c0,
-                       reg->h, reg->w, K, make_gauss9(), make_out(out));
-  else
-    hipLaunchKernelGGL(k_curvature<false>, grd, blk, 0, as_stream(stream), W, reg->r0, reg->c0,
-                       reg->h, reg->w, K, make_gauss9(), make_out(out));
-  FSG_CHECK_LAUNCH("k_curvature");
-  return FSG_OK;
+  StencilTile S = stencil_tile(win, reg);
+  S.curv = make_curvk(p);
+  S.want_cv = 1;
+  S.nanaw = p->nan_hint != 0;
+  S.cv = make_out(out);
+  return launch_stencil(S, "k_curvature", as_stream(stream));
 }

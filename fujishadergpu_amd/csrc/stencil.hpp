@@ -196,6 +196,9 @@ __device__ __forceinline__ float hillshade_from(const ShadeK& K, int i, int j, i
   return (float)hs;
 }
 
+__device__ __forceinline__ float curvature_from(const CurvK& K, float p, float q, float dxx,
+                                                float dxy, float dyy, float dyx);
+
 // compute_curvature_block (_impl_curvature.py:19-57) at window (i, j):
 // first derivatives with |step|, second derivatives of those with the signed
 // steps, the curvature formula, tanh(100 c), (x + 1) / 2, gamma.
@@ -216,6 +219,13 @@ __device__ __forceinline__ float curvature_at(const CurvK& K, int i, int j, int 
   const float dyx = deriv(K.sx, j, w, [&](int k) {
     return deriv(K.ay, i, h, [&](int m) { return f(m, k); });
   });
+  return curvature_from(K, p, q, dxx, dxy, dyy, dyx);
+}
+
+// The curvature formula, tanh(100 c), (x + 1) / 2 and gamma from the first
+// (p = d/dcol, q = d/drow) and second derivatives.
+__device__ __forceinline__ float curvature_from(const CurvK& K, float p, float q, float dxx,
+                                                float dxy, float dyy, float dyx) {
   float cv;
   const float one = 1.0f, two = 2.0f;
   if (K.type == 0) {
@@ -236,6 +246,114 @@ __device__ __forceinline__ float curvature_at(const CurvK& K, int i, int j, int 
   const float cn = tanhf(cv * 100.0f);
   float res = (cn + one) / two;
   return powf(res, K.gamma);
+}
+
+// ------------------------------------------------------------ tiled block
+// One 64 x 16 block of a window region: hillshade, slope and curvature from
+// one staged tile.  Pass 1 stages the filled values (3-pixel halo, clipped to
+// the window); pass 2 computes the first derivatives P = d/dcol and Q = d/drow
+// once per position over the block plus a 2-pixel ring; pass 3 forms the
+// second derivatives from those tables.  Every value equals the one the
+// nested per-pixel formulas give (a first derivative at a position does not
+// depend on which pixel asks for it), so the outputs are the same bits.
+constexpr int ST_TW = 64, ST_TH = 16, ST_HALO = 3, ST_RING = 2;
+constexpr int ST_LT = ST_TW + 2 * ST_HALO + 1;      // staged row stride
+constexpr int ST_ROWS = ST_TH + 2 * ST_HALO;
+constexpr int ST_PW = ST_TW + 2 * ST_RING;          // derivative table width
+constexpr int ST_PH = ST_TH + 2 * ST_RING;
+constexpr int ST_LDS_FLOATS = ST_ROWS * ST_LT + 2 * ST_PH * ST_PW;
+
+struct StencilTile {
+  WinDev W;
+  int r0, c0, rh, rw;
+  int nbx;         // blocks per block row of the region
+  int nanaw;
+  int want_hs, want_sl, want_cv;
+  int shade_from_tables;  // hillshade/slope gradient equals the curvature tables' (same |steps|, z 1)
+  ShadeK shade;
+  CurvK curv;
+  OutDev hs, sl, cv;
+};
+
+inline bool same_axis(const Axis& a, const Axis& b) {
+  return a.two_h == b.two_h && a.a0 == b.a0 && a.b0 == b.b0 && a.c0 == b.c0 && a.a1 == b.a1 &&
+         a.b1 == b.b1 && a.c1 == b.c1;
+}
+
+template <bool NANAW>
+__device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by, int bx,
+                                                   const Gauss9& G, float* lds) {
+  const int rl = S.r0 + by * ST_TH, cl = S.c0 + bx * ST_TW;  // window coords of the block
+  const int H = S.W.h, Wd = S.W.w;
+  const int tr0 = max(rl - ST_HALO, 0), tr1 = min(rl + ST_TH + ST_HALO, H);
+  const int tc0 = max(cl - ST_HALO, 0), tc1 = min(cl + ST_TW + ST_HALO, Wd);
+  const int nr = tr1 - tr0, nc = tc1 - tc0;
+  float* F = lds;
+  float* Pt = lds + ST_ROWS * ST_LT;
+  float* Qt = Pt + ST_PH * ST_PW;
+  const int t = threadIdx.x;
+  for (int e = t; e < nr * nc; e += 256) {
+    const int rr = e / nc, cc = e - rr * nc;
+    F[rr * ST_LT + cc] = filled<NANAW>(S.W, tr0 + rr, tc0 + cc, G);
+  }
+  __syncthreads();
+  auto f = [&](int r, int c) { return F[(r - tr0) * ST_LT + (c - tc0)]; };
+  // first-derivative tables over rows rl-2 .. rl+TH+1, cols cl-2 .. cl+TW+1
+  const bool need_tab = S.want_cv || (S.shade_from_tables && (S.want_hs || S.want_sl));
+  const Axis ax = S.want_cv ? S.curv.ax : S.shade.ax;
+  const Axis ay = S.want_cv ? S.curv.ay : S.shade.ay;
+  if (need_tab) {
+    for (int e = t; e < ST_PH * ST_PW; e += 256) {
+      const int pr = e / ST_PW, pc = e - pr * ST_PW;
+      const int i = rl - ST_RING + pr, j = cl - ST_RING + pc;
+      if (i >= 0 && i < H && j >= 0 && j < Wd) {
+        Pt[e] = deriv(ax, j, Wd, [&](int k) { return f(i, k); });
+        Qt[e] = deriv(ay, i, H, [&](int k) { return f(k, j); });
+      }
+    }
+  }
+  __syncthreads();
+  auto P = [&](int i, int j) { return Pt[(i - rl + ST_RING) * ST_PW + (j - cl + ST_RING)]; };
+  auto Q = [&](int i, int j) { return Qt[(i - rl + ST_RING) * ST_PW + (j - cl + ST_RING)]; };
+  const int c = t & 63;
+  const PostK phs = load_post(S.hs), psl = load_post(S.sl), pcv = load_post(S.cv);
+  for (int k = t >> 6; k < ST_TH; k += 4) {
+    const int rr = by * ST_TH + k, cc = bx * ST_TW + c;  // region coordinates
+    if (rr < S.rh && cc < S.rw) {
+      const int wr = S.r0 + rr, wc = S.c0 + cc;
+      const float z = NANAW ? raw(S.W, wr, wc) : f(wr, wc);
+      if (isnan(z)) {
+        const float qn = __builtin_nanf("");
+        if (S.want_hs) store_out(S.hs, rr, cc, apply_post(phs, qn));
+        if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, qn));
+        if (S.want_cv) store_out(S.cv, rr, cc, apply_post(pcv, qn));
+        continue;
+      }
+      if (S.want_hs || S.want_sl) {
+        float dx, dy;
+        if (S.shade_from_tables) {
+          dx = P(wr, wc);
+          dy = Q(wr, wc);
+        } else {
+          grad_at(S.shade.ax, S.shade.ay, wr, wc, H, Wd, f, dx, dy);
+        }
+        if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, slope_from(S.shade, dx, dy)));
+        if (S.want_hs)
+          store_out(S.hs, rr, cc,
+                    apply_post(phs, hillshade_from(S.shade, wr, wc, H, Wd, f, dx, dy)));
+      }
+      if (S.want_cv) {
+        const CurvK& K = S.curv;
+        const float p = P(wr, wc), q = Q(wr, wc);
+        const float dxx = deriv(K.sx, wc, Wd, [&](int m) { return P(wr, m); });
+        const float dxy = deriv(K.sy, wr, H, [&](int m) { return P(m, wc); });
+        const float dyy = deriv(K.sy, wr, H, [&](int m) { return Q(m, wc); });
+        const float dyx = deriv(K.sx, wc, Wd, [&](int m) { return Q(wr, m); });
+        store_out(S.cv, rr, cc, apply_post(pcv, curvature_from(K, p, q, dxx, dxy, dyy, dyx)));
+      }
+    }
+  }
+  __syncthreads();
 }
 
 }  // namespace fsg

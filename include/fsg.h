@@ -384,7 +384,7 @@ typedef struct fsg_batch_config {
   int32_t wg_per_cu;  /* persistent workgroups per CU (0: 3) */
   int32_t order;      /* 0: TopoUSM blocks first, 1: stencil blocks first */
   int32_t n_cu;       /* CUs to size the grid for (0: the device's) */
-  int32_t pad0;
+  int32_t chunk;      /* work items claimed per queue access (0: 8) */
 } fsg_batch_config;
 
 typedef struct fsg_batch fsg_batch;

@@ -68,7 +68,7 @@ def test_fused_batch_equals_per_tile_launches(cuda):
     print("batch info", info, flush=True)
     assert info["fallback_usm_tiles"] == 2              # unsorted radii + the NaN tile
     assert info["fused_usm_tiles"] == 6 and info["box_radii"] == 3
-    for cfg in [(3, 0), (1, 1), (8, 0)]:
+    for cfg in [(3, 0, 8), (1, 1, 1), (8, 0, 32)]:
         b.run(cfg)
         torch.cuda.synchronize()
         got = _outs(jobs)
@@ -86,7 +86,7 @@ def test_fused_batch_graph_replays(cuda):
     torch.cuda.synchronize()
     ref = _outs(jobs)
     b = FusedBatch(jobs)
-    best, times = tune(b, candidates=((2, 0), (3, 1)), reps=2)
+    best, times = tune(b, candidates=((2, 0, 4), (3, 1, 16)), reps=2)
     assert best in times
     s = torch.cuda.Stream()
     b.capture(s)
