@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 3  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 4  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -67,7 +67,7 @@ class fsg_ray_sample(C.Structure):
 
 class fsg_openness_params(C.Structure):
     _fields_ = [("openness_type", C.c_int32), ("n_dirs", C.c_int32), ("n_dist", C.c_int32),
-                ("pad0", C.c_int32), ("samples", C.POINTER(fsg_ray_sample))]
+                ("raw_degrees", C.c_int32), ("samples", C.POINTER(fsg_ray_sample))]
 
 
 class fsg_ao_sample(C.Structure):
@@ -96,6 +96,17 @@ class fsg_batch_tile(C.Structure):
         ("curvature", fsg_output), ("topousm", fsg_output), ("topousm2", fsg_output),
         ("shade", fsg_shade_params), ("curv", fsg_curvature_params), ("usm", fsg_topousm_params),
     ]
+
+
+class fsg_cnn_conv(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("wpack", C.c_void_p), ("bias", C.c_void_p),
+                ("pre_in", C.c_void_p), ("pre_out", C.c_void_p), ("y", C.c_void_p),
+                ("yf", C.c_void_p), ("batch", C.c_int32), ("h", C.c_int32), ("w", C.c_int32),
+                ("cin", C.c_int32), ("kpad", C.c_int32), ("cout", C.c_int32),
+                ("epilogue", C.c_int32), ("pad0", C.c_int32)]
+
+
+CNN_GELU, CNN_LINEAR, CNN_GELU_GRAD = 0, 1, 2
 
 
 class fsg_batch_config(C.Structure):
@@ -129,6 +140,20 @@ _SIGS = {
     "fsg_topousm_coarse_means_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32]),
     "fsg_topousm_coarse_means": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, P(C.c_int32),
                                            P(vp), C.c_int64, vp, C.c_size_t, vp]),
+    "fsg_openness_points": (C.c_int, [P(fsg_window), vp, C.c_int32, P(fsg_openness_params), vp,
+                                      vp]),
+    "fsg_cnn_conv3x3": (C.c_int, [P(fsg_cnn_conv), vp]),
+    "fsg_cnn_conv3x3_wgrad": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp,
+                                        vp, vp]),
+    "fsg_cnn_pack": (C.c_int, [vp, C.c_int32, C.c_int32, vp, C.c_int32, vp, C.c_int32, vp]),
+    "fsg_cnn_adam": (C.c_int, [vp, vp, vp, vp, C.c_int32, C.c_float, C.c_float, C.c_float,
+                               C.c_float, C.c_int32, vp]),
+    "fsg_cnn_to_bf16": (C.c_int, [vp, vp, C.c_int64, vp]),
+    "fsg_cnn_features": (C.c_int, [vp, C.c_int64, P(vp), C.c_int64, P(C.c_float), C.c_int32,
+                                   C.c_int32, vp, vp]),
+    "fsg_cnn_upsample": (C.c_int, [vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp,
+                                   C.c_int64, C.c_int32, C.c_int32, vp]),
+    "fsg_cnn_mse": (C.c_int, [vp, vp, vp, C.c_int64, C.c_float, vp, vp, vp]),
     "fsg_batch_create": (C.c_int, [P(fsg_batch_tile), C.c_int32, P(vp)]),
     "fsg_batch_run": (C.c_int, [vp, P(fsg_batch_config), vp]),
     "fsg_batch_capture": (C.c_int, [vp, P(fsg_batch_config), vp]),

@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <string>
+
 #include "fsg_common.hpp"
 
 namespace fsg {
@@ -29,24 +31,18 @@ constexpr int OPEN_MAX = 320;  // n_dirs * n_dist entries passed by value
 struct OpenTab {
   int n_dirs, n_dist;
   int negative;
+  int raw;         // 1: mean angle in degrees instead of the display value
   float half_pi;   // f32(pi/2)
   float gamma;     // f32(1/2.2)
   short2 off[OPEN_MAX];
   float inv[OPEN_MAX];
 };
 
-__global__ void __launch_bounds__(256)
-k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y;
-  if (c >= rw || r >= rh) return;
-  const int i = r0 + r, j = c0 + c;
-  const PostK pp = load_post(out);
-  const float z = W.p[(int64_t)i * W.ld + j];
-  if (isnan(z)) {
-    store_out(out, r, c, apply_post(pp, __builtin_nanf("")));
-    return;
-  }
+// Mean openness angle (radians) at window (i, j): per direction the extreme
+// elevation ratio over its samples, one atan, pi/2 -/+ that angle, averaged
+// over the directions that saw a valid sample (_impl_openness.py:78-126).
+__device__ __forceinline__ float openness_angle(const WinDevR& W, int i, int j, float z,
+                                                const OpenTab& T) {
   float angle_sum = 0.0f, dir_count = 0.0f;
   for (int d = 0; d < T.n_dirs; d++) {
     float ext = T.negative ? INFINITY : -INFINITY;
@@ -70,11 +66,41 @@ k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
       dir_count = dir_count + 1.0f;
     }
   }
-  float op = angle_sum / fmaxf(dir_count, 1.0f);
-  op = op / T.half_pi;
+  return angle_sum / fmaxf(dir_count, 1.0f);
+}
+
+// display value (op / (pi/2), clipped, gamma) or, with T.raw, the angle in degrees
+__device__ __forceinline__ float openness_value(const OpenTab& T, float angle) {
+  if (T.raw) return (float)((double)angle * (180.0 / M_PI));
+  float op = angle / T.half_pi;
   op = fminf(fmaxf(op, 0.0f), 1.0f);
-  const float res = powf(op, T.gamma);
-  store_out(out, r, c, apply_post(pp, res));
+  return powf(op, T.gamma);
+}
+
+__global__ void __launch_bounds__(256)
+k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (c >= rw || r >= rh) return;
+  const int i = r0 + r, j = c0 + c;
+  const PostK pp = load_post(out);
+  const float z = W.p[(int64_t)i * W.ld + j];
+  if (isnan(z)) {
+    store_out(out, r, c, apply_post(pp, __builtin_nanf("")));
+    return;
+  }
+  store_out(out, r, c, apply_post(pp, openness_value(T, openness_angle(W, i, j, z, T))));
+}
+
+// The same at a list of window pixels (row, col), float32 out[n].
+__global__ void __launch_bounds__(256)
+k_openness_points(WinDevR W, const int2* __restrict__ pts, int n, OpenTab T,
+                  float* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int2 q = pts[e];
+  const float z = W.p[(int64_t)q.x * W.ld + q.y];
+  out[e] = isnan(z) ? __builtin_nanf("") : openness_value(T, openness_angle(W, q.x, q.y, z, T));
 }
 
 constexpr int AO_MAX = 256;
@@ -172,23 +198,17 @@ static void expanded(const fsg_window* win, const fsg_region* reg, int* er0, int
 
 using namespace fsg;
 
-extern "C" int fsg_openness(const fsg_window* win, const fsg_region* reg,
-                            const fsg_openness_params* p, const fsg_output* out, void* stream) {
-  int rc = validate_window(win, "fsg_openness");
-  if (rc) return rc;
-  rc = validate_region(win, reg, "fsg_openness");
-  if (rc) return rc;
-  if (!p || !out) return fail(FSG_E_INVALID, "fsg_openness: params/output NULL");
-  if ((rc = validate_output(out, reg, "fsg_openness"))) return rc;
+static int openness_table(const fsg_openness_params* p, OpenTab& T, const char* who) {
+  if (!p) return fail(FSG_E_INVALID, std::string(who) + ": params NULL");
   if (p->n_dirs < 0 || p->n_dist < 0 || (p->n_dirs * p->n_dist > 0 && !p->samples))
-    return fail(FSG_E_INVALID, "fsg_openness: bad ray table");
+    return fail(FSG_E_INVALID, std::string(who) + ": bad ray table");
   if (p->n_dirs * p->n_dist > OPEN_MAX)
-    return fail(FSG_E_UNSUPPORTED, "fsg_openness: ray table larger than 320 samples");
-  if (reg->h == 0 || reg->w == 0) return FSG_OK;
-  OpenTab T{};
+    return fail(FSG_E_UNSUPPORTED, std::string(who) + ": ray table larger than 320 samples");
+  T = OpenTab{};
   T.n_dirs = p->n_dirs;
   T.n_dist = p->n_dist;
   T.negative = p->openness_type != 0;
+  T.raw = p->raw_degrees != 0;
   T.half_pi = (float)(M_PI / 2);
   T.gamma = (float)(1.0 / 2.2);
   for (int e = 0; e < p->n_dirs * p->n_dist; e++) {
@@ -199,16 +219,47 @@ extern "C" int fsg_openness(const fsg_window* win, const fsg_region* reg,
       continue;
     }
     if (std::abs(s.dx) > 32767 || std::abs(s.dy) > 32767)
-      return fail(FSG_E_UNSUPPORTED, "fsg_openness: offset beyond int16");
+      return fail(FSG_E_UNSUPPORTED, std::string(who) + ": offset beyond int16");
     T.off[e] = make_short2((short)s.dx, (short)s.dy);
     T.inv[e] = 1.0f / (float)s.phys;
   }
+  return FSG_OK;
+}
+
+extern "C" int fsg_openness(const fsg_window* win, const fsg_region* reg,
+                            const fsg_openness_params* p, const fsg_output* out, void* stream) {
+  int rc = validate_window(win, "fsg_openness");
+  if (rc) return rc;
+  rc = validate_region(win, reg, "fsg_openness");
+  if (rc) return rc;
+  if (!out) return fail(FSG_E_INVALID, "fsg_openness: output NULL");
+  if ((rc = validate_output(out, reg, "fsg_openness"))) return rc;
+  OpenTab T;
+  if ((rc = openness_table(p, T, "fsg_openness"))) return rc;
+  if (reg->h == 0 || reg->w == 0) return FSG_OK;
   WinDevR W{win->data, win->ld, win->h, win->w};
   dim3 blk(64, 4), grd((reg->w + 63) / 64, (reg->h + 3) / 4);
   TimedScope ts("k_openness", as_stream(stream));
   hipLaunchKernelGGL(k_openness, grd, blk, 0, as_stream(stream), W, reg->r0, reg->c0, reg->h,
                      reg->w, T, make_out(out));
   FSG_CHECK_LAUNCH("k_openness");
+  return FSG_OK;
+}
+
+extern "C" int fsg_openness_points(const fsg_window* win, const int32_t* points, int32_t n,
+                                   const fsg_openness_params* p, float* out, void* stream) {
+  int rc = validate_window(win, "fsg_openness_points");
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!points || !out)))
+    return fail(FSG_E_INVALID, "fsg_openness_points: bad point list");
+  OpenTab T;
+  if ((rc = openness_table(p, T, "fsg_openness_points"))) return rc;
+  if (n == 0) return FSG_OK;
+  WinDevR W{win->data, win->ld, win->h, win->w};
+  TimedScope ts("k_openness_points", as_stream(stream));
+  hipLaunchKernelGGL(k_openness_points, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
+                     W, (const int2*)points, n, T, out);
+  FSG_CHECK_LAUNCH("k_openness_points");
   return FSG_OK;
 }
 

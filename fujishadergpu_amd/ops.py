@@ -128,8 +128,9 @@ def curvature(win_t, reg, *, step_x: float, step_y: float, curvature_type: str,
 
 # ------------------------------------------------------------------ rays
 
-def openness(win_t, reg, *, table, openness_type: str, out: N.fsg_output) -> None:
-    """table: list over directions of lists over distances of (ox, oy, phys)|None."""
+def openness_params(table, openness_type: str, raw_degrees: bool = False):
+    """(params, keep-alive array) for a ray table: list over directions of
+    lists over distances of (ox, oy, phys)|None."""
     n_dirs = len(table)
     n_dist = max((len(r) for r in table), default=0)
     arr = (N.fsg_ray_sample * max(1, n_dirs * n_dist))()
@@ -141,10 +142,30 @@ def openness(win_t, reg, *, table, openness_type: str, out: N.fsg_output) -> Non
                 e.skip = 1
             else:
                 e.dx, e.dy, e.phys = int(entry[0]), int(entry[1]), float(entry[2])
-    p = N.fsg_openness_params(0 if openness_type == "positive" else 1, n_dirs, n_dist, 0,
-                              C.cast(arr, C.POINTER(N.fsg_ray_sample)))
+    p = N.fsg_openness_params(0 if openness_type == "positive" else 1, n_dirs, n_dist,
+                              1 if raw_degrees else 0, C.cast(arr, C.POINTER(N.fsg_ray_sample)))
+    return p, arr
+
+
+def openness(win_t, reg, *, table, openness_type: str, out: N.fsg_output,
+             raw_degrees: bool = False) -> None:
+    p, _arr = openness_params(table, openness_type, raw_degrees)
     w = window(win_t)
     N.check(_lib().fsg_openness(C.byref(w), C.byref(reg), C.byref(p), C.byref(out), _stream()))
+
+
+def openness_points(win_t, points: torch.Tensor, *, table, openness_type: str,
+                    raw_degrees: bool = True) -> torch.Tensor:
+    """Openness at window pixels points (int32 [n, 2] = row, col, on the GPU)."""
+    if points.dtype != torch.int32 or points.dim() != 2 or points.shape[1] != 2:
+        raise ValueError("points must be an int32 [n, 2] tensor")
+    pts = points.contiguous()
+    out = torch.empty(pts.shape[0], dtype=torch.float32, device=win_t.device)
+    p, _arr = openness_params(table, openness_type, raw_degrees)
+    w = window(win_t)
+    N.check(_lib().fsg_openness_points(C.byref(w), pts.data_ptr(), int(pts.shape[0]),
+                                       C.byref(p), out.data_ptr(), _stream()))
+    return out
 
 
 def ambient_occlusion(win_t, reg, *, table, intensity: float, pad_d: int,

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 3
+#define FSG_ABI_VERSION 4
 
 enum {
   FSG_OK = 0,
@@ -188,14 +188,21 @@ typedef struct fsg_ray_sample {
 typedef struct fsg_openness_params {
   int32_t openness_type; /* 0 positive, 1 negative */
   int32_t n_dirs, n_dist;
-  int32_t pad0;
+  int32_t raw_degrees;   /* 0: display value (reference); 1: mean angle in degrees */
   const fsg_ray_sample* samples; /* host pointer, n_dirs*n_dist entries */
 } fsg_openness_params;
 
-/* Replaces compute_openness_vectorized (_impl_openness.py:31-132). */
+/* Replaces compute_openness_vectorized (_impl_openness.py:31-132).  With
+   raw_degrees the output is the mean openness angle in degrees, before the
+   reference's / (pi/2), [0, 1] clip and gamma (the surrogate's target). */
 int fsg_openness(const fsg_window* win, const fsg_region* reg,
                  const fsg_openness_params* p, const fsg_output* out,
                  void* stream);
+
+/* The same openness at n window pixels (device int32 [n][2] = row, col) into
+   float32 out[n] (the surrogate's sampled exact check). */
+int fsg_openness_points(const fsg_window* win, const int32_t* points, int32_t n,
+                        const fsg_openness_params* p, float* out, void* stream);
 
 /*
  * Ambient-occlusion sample table (_impl_ambient_occlusion.py:288-339): ring
@@ -359,6 +366,68 @@ int fsg_encode(const fsg_window* src, const fsg_region* reg,
 /* Gather the region of a window into a dense f32 pool (stats pre-pass). */
 int fsg_copy_region(const fsg_window* win, const fsg_region* reg, float* dst,
                     void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Openness surrogate: BF16 CNN on MFMA                                 */
+/* ------------------------------------------------------------------ */
+
+/*
+ * No reference counterpart (the reference has no learned component): the
+ * building blocks of the learned openness surrogate -- 3x3 "same" convolutions
+ * with zero padding on NHWC bfloat16 activations, float32 accumulation on
+ * MFMA, bias + GELU epilogues, the backward passes and Adam.
+ *
+ * fsg_cnn_conv3x3: y = epi(conv(x, w) + bias).  x is [batch][h][w][cin]
+ * bf16 with cin 8 or 32; wpack is [32][kpad] bf16 from fsg_cnn_pack (k =
+ * tap * cin + ci, kpad = 9 * cin rounded up to 16).  Epilogues:
+ *   FSG_CNN_GELU       y = bf16(gelu(z)), z = acc + bias; pre_out (optional,
+ *                      f32 [..][32]) receives z for the backward pass;
+ *   FSG_CNN_LINEAR     yf = acc + bias, f32 [..][cout] (cout <= 32);
+ *   FSG_CNN_GELU_GRAD  y = bf16(acc * gelu'(pre_in)) -- with the flipped
+ *                      weights from fsg_cnn_pack, the gradient of the previous
+ *                      layer's pre-activation (backward through conv + GELU).
+ * fsg_cnn_conv3x3_wgrad accumulates dw [32][cin][9] and db [32] (f32; zero
+ * them first) from the layer input x and the pre-activation gradient dz
+ * [..][32] bf16.  fsg_cnn_pack writes the forward and (wt != NULL) backward
+ * bf16 layouts of float32 weights w [cout][cin][3][3].  fsg_cnn_adam is one
+ * bias-corrected Adam step on float32 parameters (step counts from 1).
+ */
+enum { FSG_CNN_GELU = 0, FSG_CNN_LINEAR = 1, FSG_CNN_GELU_GRAD = 2 };
+
+typedef struct fsg_cnn_conv {
+  const void* x;
+  const void* wpack;
+  const float* bias;
+  const float* pre_in;
+  float* pre_out;
+  void* y;
+  float* yf;
+  int32_t batch, h, w, cin, kpad, cout, epilogue, pad0;
+} fsg_cnn_conv;
+
+int fsg_cnn_conv3x3(const fsg_cnn_conv* c, void* stream);
+int fsg_cnn_conv3x3_wgrad(const void* x, const void* dz, int32_t batch, int32_t h, int32_t w,
+                          int32_t cin, float* dw, float* db, void* stream);
+int fsg_cnn_pack(const float* w, int32_t cout, int32_t cin, void* wf, int32_t kpad, void* wt,
+                 int32_t kpad_t, void* stream);
+int fsg_cnn_adam(float* p, const float* g, float* m, float* v, int32_t n, float lr,
+                 float beta1, float beta2, float eps, int32_t step, void* stream);
+int fsg_cnn_to_bf16(const float* a, void* b, int64_t n, void* stream);
+
+/* Surrogate plumbing.  fsg_cnn_features: the 4-level detrended elevation
+ * pyramid, channel k = bf16((means[k] - z) * inv[k]) for k < 4 and zero
+ * channels 4..7, NHWC [h][w][8].  fsg_cnn_upsample: bilinear, pixel-centre
+ * aligned (sample at ((i + .5) / factor - .5, (j + .5) / factor - .5),
+ * clamped) of src (+ add, optional) into out [h][w].  fsg_cnn_mse: pred =
+ * yf + base against label (NaN: no label) -- loss[0] += sum of squares,
+ * dz[p] = (bf16(2 * scale * (pred - label)), 0, ..., 0) over 32 channels. */
+int fsg_cnn_features(const float* z, int64_t ldz, const float* const* means, int64_t ldm,
+                     const float* inv, int32_t h, int32_t w, void* out, void* stream);
+int fsg_cnn_upsample(const float* src, const float* add, int64_t lds, int32_t hs, int32_t ws,
+                     int32_t factor, float* out, int64_t ldo, int32_t h, int32_t w,
+                     void* stream);
+int fsg_cnn_mse(const float* yf, const float* base, const float* label, int64_t n, float scale,
+                void* dz, float* loss, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched fused launch                                                 */

@@ -452,3 +452,57 @@ def test_meters_per_degree_at_35_north():
     from fujishadergpu_amd.raster_info import meters_per_degree
     m_lon, m_lat = meters_per_degree(35.0)
     assert m_lat == pytest.approx(110940.5, abs=15) and m_lon == pytest.approx(91288.0, abs=15)
+
+
+# ------------------------------------------------ surrogate network (A18)
+
+def test_cnn_oracle_backward_matches_finite_differences():
+    from oracle import cnn
+    rng = np.random.default_rng(0)
+    params = cnn.init_params(rng, [3, 5, 4, 1])
+    x = rng.normal(size=(2, 6, 7, 3))
+    base = rng.normal(size=(2, 6, 7))
+    label = rng.normal(size=(2, 6, 7))
+    label[0, 0, 0] = np.nan                      # unlabelled cell
+    out, cache = cnn.forward(params, x)
+    grads = cnn.backward(params, cache, out, base, label, scale=0.3)
+
+    def f(ps):
+        return cnn.loss(cnn.forward(ps, x)[0], base, label, scale=0.3)
+    eps = 1e-6
+    for li in range(len(params)):
+        for which in (0, 1):
+            arr = params[li][which]
+            flat = arr.reshape(-1)
+            for idx in rng.choice(flat.size, size=min(6, flat.size), replace=False):
+                old = flat[idx]
+                flat[idx] = old + eps
+                up = f(params)
+                flat[idx] = old - eps
+                dn = f(params)
+                flat[idx] = old
+                num = (up - dn) / (2 * eps)
+                ana = grads[li][which].reshape(-1)[idx]
+                assert ana == pytest.approx(num, rel=1e-5, abs=1e-7), (li, which, idx)
+
+
+def test_cnn_oracle_conv_is_a_same_correlation():
+    from oracle import cnn
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(1, 5, 6, 2))
+    w = rng.normal(size=(3, 2, 3, 3))
+    b = rng.normal(size=3)
+    out = cnn.conv3x3(x, w, b)
+    # brute force at every pixel, zero outside the image
+    for y in range(5):
+        for xx in range(6):
+            for co in range(3):
+                s = b[co]
+                for dy in range(3):
+                    for dx in range(3):
+                        yy, xc = y + dy - 1, xx + dx - 1
+                        if 0 <= yy < 5 and 0 <= xc < 6:
+                            s += float(np.dot(x[0, yy, xc], w[co, :, dy, dx]))
+                assert out[0, y, xx, co] == pytest.approx(s, abs=1e-12)
+    assert cnn.gelu(np.array([0.0]))[0] == 0.0
+    assert cnn.gelu(np.array([3.0]))[0] == pytest.approx(3.0 * 0.5 * (1 + math.erf(3 / math.sqrt(2))))
