@@ -278,15 +278,17 @@ __global__ void k_cnn_upsample(const float* __restrict__ src, const float* __res
                                int64_t lds, int hs, int ws, int f, float* __restrict__ out,
                                int64_t ldo, int h, int w) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (j >= w || i >= h) return;
+  if (j >= w) return;
   const float inv = 1.0f / (float)f;
-  float y = ((float)i + 0.5f) * inv - 0.5f, x = ((float)j + 0.5f) * inv - 0.5f;
-  y = fminf(fmaxf(y, 0.0f), (float)(hs - 1));
+  for (int i = blockIdx.y; i < h; i += gridDim.y) {
+  float x = ((float)j + 0.5f) * inv - 0.5f;
   x = fminf(fmaxf(x, 0.0f), (float)(ws - 1));
-  const int y0 = min((int)y, hs - 1), x0 = min((int)x, ws - 1);
-  const int y1 = min(y0 + 1, hs - 1), x1 = min(x0 + 1, ws - 1);
-  const float ty = y - (float)y0, tx = x - (float)x0;
+  const int x0 = min((int)x, ws - 1), x1 = min(x0 + 1, ws - 1);
+  const float tx = x - (float)x0;
+  float y = ((float)i + 0.5f) * inv - 0.5f;
+  y = fminf(fmaxf(y, 0.0f), (float)(hs - 1));
+  const int y0 = min((int)y, hs - 1), y1 = min(y0 + 1, hs - 1);
+  const float ty = y - (float)y0;
   auto s = [&](int a, int b) {
     const int64_t o = (int64_t)a * lds + b;
     return src[o] + (add ? add[o] : 0.0f);
@@ -294,6 +296,7 @@ __global__ void k_cnn_upsample(const float* __restrict__ src, const float* __res
   const float top = s(y0, x0) * (1.0f - tx) + s(y0, x1) * tx;
   const float bot = s(y1, x0) * (1.0f - tx) + s(y1, x1) * tx;
   out[(int64_t)i * ldo + j] = top * (1.0f - ty) + bot * ty;
+  }
 }
 
 // Masked squared error of pred = yf + base against label (NaN = no label):
@@ -354,7 +357,8 @@ extern "C" int fsg_cnn_upsample(const float* src, const float* add, int64_t lds,
   if (!src || !out || hs <= 0 || ws <= 0 || factor <= 0 || h <= 0 || w <= 0 || lds < ws ||
       ldo < w)
     return fail(FSG_E_INVALID, "fsg_cnn_upsample: bad arguments");
-  hipLaunchKernelGGL(k_cnn_upsample, dim3((w + 255) / 256, h), dim3(256), 0, as_stream(stream),
+  hipLaunchKernelGGL(k_cnn_upsample, dim3((w + 255) / 256, std::min(h, 16384)), dim3(256), 0,
+                     as_stream(stream),
                      src, add, lds, hs, ws, factor, out, ldo, h, w);
   FSG_CHECK_LAUNCH("k_cnn_upsample");
   return FSG_OK;

@@ -168,9 +168,10 @@ class Net:
             pre = torch.empty((bsz, h, w, 32), dtype=torch.float32, device=self.device) \
                 if keep else None
             self._conv(li, a, h, w, bsz, N.CNN_GELU, y=y, pre_out=pre)
-            acts.append(y)
-            pres.append(pre)
-            a = y
+            if keep:
+                acts.append(y)
+                pres.append(pre)
+            a = y          # without the cache only the live activation is held
         yf = torch.empty((bsz, h, w), dtype=torch.float32, device=self.device)
         self._conv(len(self.shapes) - 1, a, h, w, bsz, N.CNN_LINEAR, yf=yf)
         return (yf, acts, pres) if keep else yf
