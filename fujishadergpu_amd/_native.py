@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 4  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 5  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -98,6 +98,11 @@ class fsg_batch_tile(C.Structure):
     ]
 
 
+class fsg_cnn_net(C.Structure):
+    _fields_ = [("wpack", C.c_void_p * 6), ("bias", C.c_void_p * 6), ("cin", C.c_int32),
+                ("cout", C.c_int32)]
+
+
 class fsg_cnn_conv(C.Structure):
     _fields_ = [("x", C.c_void_p), ("wpack", C.c_void_p), ("bias", C.c_void_p),
                 ("pre_in", C.c_void_p), ("pre_out", C.c_void_p), ("y", C.c_void_p),
@@ -153,7 +158,15 @@ _SIGS = {
                                    C.c_int32, vp, vp]),
     "fsg_cnn_upsample": (C.c_int, [vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp,
                                    C.c_int64, C.c_int32, C.c_int32, vp]),
-    "fsg_cnn_mse": (C.c_int, [vp, vp, vp, C.c_int64, C.c_float, vp, vp, vp]),
+    "fsg_cnn_mse": (C.c_int, [vp, C.c_int32, vp, vp, vp, vp, C.c_int64, C.c_float, vp, vp, vp]),
+    "fsg_cnn_forward": (C.c_int, [P(fsg_cnn_net), vp, C.c_int32, C.c_int32, C.c_int32, vp,
+                                  C.c_int32, vp]),
+    "fsg_cnn_compose": (C.c_int, [vp, C.c_int32, vp, vp, C.c_int64, C.c_int32, C.c_int32,
+                                  C.c_int32, vp, C.c_int64, C.c_float, vp, C.c_int64, C.c_int32,
+                                  C.c_int32, vp]),
+    "fsg_cnn_detail_label": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp,
+                                       C.c_int64, C.c_float, vp, vp, vp]),
     "fsg_batch_create": (C.c_int, [P(fsg_batch_tile), C.c_int32, P(vp)]),
     "fsg_batch_run": (C.c_int, [vp, P(fsg_batch_config), vp]),
     "fsg_batch_capture": (C.c_int, [vp, P(fsg_batch_config), vp]),

@@ -7,11 +7,12 @@
 // activations, FP32 accumulation) is fixed by the build's own specification.
 //
 // Convolution as implicit GEMM on v_mfma_f32_32x32x16_bf16 (wave64):
-//   M = 32 consecutive pixels of one image row, N = 32 output channels,
+//   M = 32 output channels, N = 32 consecutive pixels of one image row,
 //   K = 9 taps x Cin (tap-major, channel-minor), padded to a multiple of 16.
-// Lane l holds A row (pixel) l%32, k (l/32)*8..+7 -- one tap's 8 consecutive
-// channels, one 16-byte load -- and B column (channel) l%32 over the same k;
-// the accumulator holds column l%32, rows 8i + (l/32)*4 + j at acc[4i + j].
+// Lane l holds A row (channel) l%32, k (l/32)*8..+7 and B column (pixel) l%32
+// over the same k -- one tap's 8 consecutive channels, one 16-byte load; the
+// accumulator holds pixel l%32 and channels 8i + (l/32)*4 + j at acc[4i + j],
+// so each lane stores 4 x 4 consecutive channels of one pixel.
 // Weights are staged in LDS once per workgroup; each wave walks 32-pixel
 // tiles grid-stride.
 #include <algorithm>
@@ -30,21 +31,34 @@ constexpr int CNN_MAX_KPAD = 288;  // 9 taps x 32 channels
 
 __device__ __forceinline__ float bf2f(u16 v) { return __uint_as_float((unsigned)v << 16); }
 
-// float -> bfloat16, round to nearest even (NaN kept quiet)
-__device__ __forceinline__ u16 f2bf(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (u16)(u >> 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// float -> bfloat16, round to nearest even: gfx950's v_cvt_pk_bf16_f32
+__device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
+// two floats -> packed bfloat16 pair (a in the low half), one instruction
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
 }
 
-__device__ __forceinline__ float gelu(float x) {
-  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+// GELU in its tanh form (Hendrycks & Gimpel 2016):
+//   gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))) = x * sigmoid(u),
+//   u = 2 sqrt(2/pi) (x + 0.044715 x^3);
+// one v_exp_f32 and one v_rcp_f32 per value (exp overflow gives -0 for x << 0).
+constexpr float GELU_K0 = 1.5957691216057308f;   // 2 sqrt(2/pi)
+constexpr float GELU_K1 = 0.07135481627260025f;  // 2 sqrt(2/pi) 0.044715
+constexpr float LOG2E = 1.4426950408889634f;
+
+__device__ __forceinline__ float sigmoid_of_u(float x) {
+  // exp(-u) = exp2(x * (x^2 * (-K1 log2 e) + (-K0 log2 e)))
+  const float m = x * fmaf(x * x, -GELU_K1 * LOG2E, -GELU_K0 * LOG2E);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(m));
 }
+__device__ __forceinline__ float gelu(float x) { return x * sigmoid_of_u(x); }
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-  return cdf + x * pdf;
+  const float s = sigmoid_of_u(x);
+  return s + x * s * (1.0f - s) * fmaf(3.0f * GELU_K1, x * x, GELU_K0);
 }
 
 enum { EPI_GELU = 0, EPI_LINEAR = 1, EPI_GELU_GRAD = 2 };
@@ -73,13 +87,20 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
   const int tiles_x = (A.W + 31) >> 5;
   const int n_tiles = A.B * A.H * tiles_x;
   const int ksteps = A.kpad >> 4;
-  const float bias = (EPI != EPI_GELU_GRAD) ? A.bias[n] : 0.0f;
+  // C = W (32 out channels x K) * X (K x 32 pixels): lane holds pixel n and
+  // channels 8i + half*4 + j at acc[4i + j]
+  float bias[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++)
+    bias[e] = (EPI != EPI_GELU_GRAD && (8 * (e >> 2) + half * 4 + (e & 3)) < A.cout)
+                  ? A.bias[8 * (e >> 2) + half * 4 + (e & 3)]
+                  : 0.0f;
   for (int tile = blockIdx.x * 4 + wave; tile < n_tiles; tile += gridDim.x * 4) {
     const int tx = tile % tiles_x;
     const int row = tile / tiles_x;  // b * H + y
     const int b = row / A.H, y = row - b * A.H;
     const int x0 = tx * 32;
-    const int px = x0 + n;           // this lane's A-row pixel
+    const int px = x0 + n;           // this lane's pixel (B column)
     f32x16 acc = {};
     for (int ks = 0; ks < ksteps; ks++) {
       const int k = ks * 16 + half * 8;
@@ -93,26 +114,29 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
         a = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
       }
       const bf16x8 w = __builtin_bit_cast(bf16x8, *(const uint4*)(wl + n * A.kpad + k));
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, a, acc, 0, 0, 0);
     }
-    // epilogue: lane holds channel n for pixels x0 + 8i + half*4 + j
+    if (px >= A.W) continue;
+    const int64_t p = ((int64_t)b * A.H + y) * A.W + px;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
+      const int c0 = 8 * i + half * 4;  // channels c0 .. c0 + 3
+      if (EPI == EPI_GELU) {
+        float z[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int xo = x0 + 8 * i + half * 4 + j;
-        if (xo >= A.W) continue;
-        const int64_t p = ((int64_t)b * A.H + y) * A.W + xo;
-        const float v = acc[4 * i + j];
-        if (EPI == EPI_GELU) {
-          const float z = v + bias;
-          if (A.pre_out) A.pre_out[p * CNN_C + n] = z;
-          A.y[p * CNN_C + n] = f2bf(gelu(z));
-        } else if (EPI == EPI_LINEAR) {
-          if (n < A.cout) A.yf[p * A.cout + n] = v + bias;
-        } else {
-          A.y[p * CNN_C + n] = f2bf(v * gelu_grad(A.pre_in[p * CNN_C + n]));
-        }
+        for (int j = 0; j < 4; j++) z[j] = acc[4 * i + j] + bias[4 * i + j];
+        if (A.pre_out) *(float4*)(A.pre_out + p * CNN_C + c0) = make_float4(z[0], z[1], z[2], z[3]);
+        *(uint2*)(A.y + p * CNN_C + c0) =
+            make_uint2(pk_bf16(gelu(z[0]), gelu(z[1])), pk_bf16(gelu(z[2]), gelu(z[3])));
+      } else if (EPI == EPI_LINEAR) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (c0 + j < A.cout) A.yf[p * A.cout + c0 + j] = acc[4 * i + j] + bias[4 * i + j];
+      } else {
+        const float4 pz = *(const float4*)(A.pre_in + p * CNN_C + c0);
+        *(uint2*)(A.y + p * CNN_C + c0) =
+            make_uint2(pk_bf16(acc[4 * i + 0] * gelu_grad(pz.x), acc[4 * i + 1] * gelu_grad(pz.y)),
+                       pk_bf16(acc[4 * i + 2] * gelu_grad(pz.z), acc[4 * i + 3] * gelu_grad(pz.w)));
       }
     }
   }
@@ -299,30 +323,399 @@ __global__ void k_cnn_upsample(const float* __restrict__ src, const float* __res
   }
 }
 
-// Masked squared error of pred = yf + base against label (NaN = no label):
-// loss += sum d^2, dz[p][0] = bf16(scale * 2 d), dz[p][1..31] = 0.
-__global__ void k_cnn_mse(const float* __restrict__ yf, const float* __restrict__ base,
-                          const float* __restrict__ label, int64_t n, float scale,
+// Masked squared error of the network output yf [n][cout]:
+//   angle term     d = yf[p][0] + base[p] - label[p]   (label NaN: no label)
+//   detail term    e = yf[p][1] - label_s[p], weighted by weight_s[p]
+//                  (cout >= 2 and label_s given; label_s NaN: no label)
+// loss[0] += sum d^2, loss[1] += sum w e^2; dz[p] = bf16 of (2 scale d,
+// 2 scale w e, 0, ..., 0) over 32 channels.
+__global__ void k_cnn_mse(const float* __restrict__ yf, int cout, const float* __restrict__ base,
+                          const float* __restrict__ label, const float* __restrict__ label_s,
+                          const float* __restrict__ weight_s, int64_t n, float scale,
                           u16* __restrict__ dz, float* __restrict__ loss) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  float d2 = 0.0f;
+  float d2 = 0.0f, e2 = 0.0f;
   if (p < n) {
     const float lb = label[p];
-    float g = 0.0f;
+    float g0 = 0.0f, g1 = 0.0f;
     if (!isnan(lb)) {
-      const float d = yf[p] + base[p] - lb;
+      const float d = yf[p * cout] + base[p] - lb;
       d2 = d * d;
-      g = scale * 2.0f * d;
+      g0 = scale * 2.0f * d;
+    }
+    if (label_s) {
+      const float ls = label_s[p];
+      if (!isnan(ls)) {
+        const float w = weight_s[p];
+        const float e = yf[p * cout + 1] - ls;
+        e2 = w * e * e;
+        g1 = scale * 2.0f * w * e;
+      }
     }
     uint4* row = (uint4*)(dz + p * CNN_C);
-    row[0] = make_uint4(f2bf(g), 0, 0, 0);
+    row[0] = make_uint4(pk_bf16(g0, g1), 0, 0, 0);
     row[1] = make_uint4(0, 0, 0, 0);
     row[2] = make_uint4(0, 0, 0, 0);
     row[3] = make_uint4(0, 0, 0, 0);
   }
-  // wave sum, one atomic per wave
-  for (int o = 32; o > 0; o >>= 1) d2 += __shfl_down(d2, o);
-  if ((threadIdx.x & 63) == 0) atomicAdd(loss, d2);
+  // wave sums, one atomic per wave and term
+  for (int o = 32; o > 0; o >>= 1) {
+    d2 += __shfl_down(d2, o);
+    e2 += __shfl_down(e2, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(loss, d2);
+    if (label_s) atomicAdd(loss + 1, e2);
+  }
+}
+
+// ------------------------------------------------------------ fused forward
+//
+// All six layers in one launch.  A workgroup owns a strip of FZ_CW = 96
+// computed columns (FZ_SW = 84 output columns plus the 6-column receptive-
+// field halo on each side) and a block of SH output rows, and streams down
+// the rows: every layer keeps a ring of 4 rows of its input in LDS (bf16, 32
+// channels = 64 bytes per pixel, 16-byte chunks XOR-swizzled by pixel so that
+// 8 consecutive pixels of one chunk hit 8 different bank groups).  Layer l at
+// step t produces row y0 - 6 + t - 2(l + 1) of its output: each layer reads
+// only rows written in earlier steps, so one barrier per step separates all
+// producers from all consumers.
+//
+// Waves are specialised and hold their layers' weights in VGPRs (the MFMA A
+// operand, 16 bytes per lane and k-step), balanced so that every SIMD carries
+// about the same GELU (VALU) and MFMA load:
+//   waves 0..3  layers 1..4 (32 -> 32), all three 32-pixel tiles of the row,
+//               the GELU epilogue of tile i interleaved with the MFMAs of i+1;
+//   wave  4     loads input rows (8 channels) and runs layer 0 (8 -> 32) on tile 0;
+//   waves 5, 6  layer 0 on tiles 1, 2 and layer 5 (32 -> cout) on tiles 0, 1;
+//   wave  7     layer 5 on tile 2.
+// Activations outside the image are zero in every layer (the "same" zero
+// padding of the layer-by-layer kernels), so both paths issue identical MFMA
+// sequences on identical operands and agree bit for bit.
+constexpr int FZ_TILES = 3;
+constexpr int FZ_CW = 32 * FZ_TILES;     // computed columns per strip
+constexpr int FZ_SW = FZ_CW - 12;        // output columns per strip
+constexpr int FZ_PW = FZ_CW + 2;         // ring row: one zero pixel each side
+constexpr int FZ_ROW32 = FZ_PW * 64;     // bytes per 32-channel ring row
+constexpr int FZ_ROW8 = FZ_PW * 16;      // bytes per 8-channel ring row
+constexpr int FZ_RING0 = 4 * FZ_ROW8;
+constexpr int FZ_RING32 = 4 * FZ_ROW32;
+constexpr int FZ_LDS = FZ_RING0 + 5 * FZ_RING32;
+
+struct FusedArgs {
+  const u16* x;         // [B][H][W][8] bf16
+  const u16* w[6];      // [32][80] (layer 0), [32][288] (layers 1..5) bf16
+  const float* bias[6]; // [32] each
+  float* yf;            // [B][H][W][cout] f32
+  int B, H, W, cout, SH, n_strips, n_rb;
+};
+
+__device__ __forceinline__ int fz_off32(int pc, int chunk) {
+  return pc * 64 + ((chunk ^ ((pc >> 1) & 3)) << 4);
+}
+__device__ __forceinline__ int fz_slot(int r, int y0) { return (r - y0 + 8) & 3; }
+
+// One 32-channel 3x3 layer over the three tiles of row r (MFMA accumulators
+// per tile, ds_read_b128 operands).  Returns the accumulators of tile nt.
+__device__ __forceinline__ f32x16 fz_mma32(const char* rin, const bf16x8 (&w)[18], int r, int y0,
+                                           int nt, int n, int half) {
+  const char* rows[3] = {rin + fz_slot(r - 1, y0) * FZ_ROW32, rin + fz_slot(r, y0) * FZ_ROW32,
+                         rin + fz_slot(r + 1, y0) * FZ_ROW32};
+  f32x16 acc = {};
+#pragma unroll
+  for (int ks = 0; ks < 18; ks++) {
+    const int tap = ks >> 1, dy = tap / 3, dx = tap % 3;
+    const int pc = nt * 32 + n + dx;
+    const bf16x8 b = *(const bf16x8*)(rows[dy] + fz_off32(pc, (ks & 1) * 2 + half));
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[ks], b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// GELU of accumulator values e, e + 1 (channels of one 4-group) as a packed pair
+__device__ __forceinline__ unsigned fz_gelu2(const f32x16& acc, const float (&bias)[16], int e) {
+  return pk_bf16(gelu(acc[e] + bias[e]), gelu(acc[e + 1] + bias[e + 1]));
+}
+
+// GELU epilogue into a 32-channel ring row (zeros outside the image).
+__device__ __forceinline__ void fz_store_gelu(char* dst, const f32x16& acc, const float (&bias)[16],
+                                              bool ok, int pc, int half) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    uint2 v = make_uint2(0, 0);
+    if (ok) v = make_uint2(fz_gelu2(acc, bias, 4 * i), fz_gelu2(acc, bias, 4 * i + 2));
+    *(uint2*)(dst + fz_off32(pc, i) + half * 8) = v;
+  }
+}
+
+// Layer 0 (8 -> 32) on tile nt of row r: cin 8 puts one tap in each half-wave.
+__device__ __forceinline__ void fz_layer0(const char* lds, const bf16x8 (&w0)[5],
+                                          const float (&bias)[16], int r, int y0, int H, int W,
+                                          int xs, int nt, int n, int half) {
+  const bool row_ok = r >= 0 && r < H;
+  f32x16 acc = {};
+  if (row_ok) {
+#pragma unroll
+    for (int ks = 0; ks < 5; ks++) {
+      const int tap = 2 * ks + half;
+      bf16x8 bv = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+      if (tap < 9) {
+        const int dy = tap / 3, dx = tap % 3;
+        bv = *(const bf16x8*)(lds + fz_slot(r + dy - 1, y0) * FZ_ROW8 + (nt * 32 + n + dx) * 16);
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0[ks], bv, acc, 0, 0, 0);
+    }
+  }
+  const int x = xs + nt * 32 + n;
+  fz_store_gelu((char*)lds + FZ_RING0 + fz_slot(r, y0) * FZ_ROW32, acc, bias,
+                row_ok && x >= 0 && x < W, nt * 32 + n + 1, half);
+}
+
+// Layer 5 (32 -> cout) on tile nt of output row r, stored to global memory.
+__device__ __forceinline__ void fz_layer5(const FusedArgs& A, const char* lds,
+                                          const bf16x8 (&w)[18], const float (&bias)[16], int r,
+                                          int y0, int b, int xs, int nt, int n, int half) {
+  const f32x16 acc = fz_mma32(lds + FZ_RING0 + 4 * FZ_RING32, w, r, y0, nt, n, half);
+  const int c = nt * 32 + n, x = xs + c;
+  if (half == 0 && c >= 6 && c < 6 + FZ_SW && x < A.W) {
+    float* o = A.yf + (((int64_t)b * A.H + r) * A.W + x) * A.cout;
+    if (A.cout == 2) {
+      *(float2*)o = make_float2(acc[0] + bias[0], acc[1] + bias[1]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (j < A.cout) o[j] = acc[j] + bias[j];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(512) k_cnn_fused(FusedArgs A) {
+  extern __shared__ __attribute__((aligned(16))) char fz_lds[];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 31, half = lane >> 5;
+  int blk = blockIdx.x;
+  const int rb = blk % A.n_rb;
+  blk /= A.n_rb;
+  const int strip = blk % A.n_strips;
+  const int b = blk / A.n_strips;
+  const int y0 = rb * A.SH;
+  const int y_end = min(y0 + A.SH, A.H);
+  const int xs = strip * FZ_SW - 6;   // image column of computed column 0
+
+  // zero the pad pixels of every ring row (never written afterwards)
+  for (int e = tid; e < 4 * 6 * 2; e += 512) {
+    const int ring = e / 8, rest = e % 8, slot = rest >> 1, side = rest & 1;
+    if (ring == 0) {
+      *(uint4*)(fz_lds + slot * FZ_ROW8 + side * (FZ_PW - 1) * 16) = make_uint4(0, 0, 0, 0);
+    } else {
+      char* row = fz_lds + FZ_RING0 + (ring - 1) * FZ_RING32 + slot * FZ_ROW32 +
+                  side * (FZ_PW - 1) * 64;
+      for (int q = 0; q < 4; q++) *(uint4*)(row + q * 16) = make_uint4(0, 0, 0, 0);
+    }
+  }
+
+  // weights: w = layer wave + 1 (waves 0..3) or layer 5 (waves 5..7); w0 = layer 0
+  const int big = wave < 4 ? wave + 1 : 5;
+  bf16x8 w[18], w0[5];
+  float bias[16], bias0[16];
+#pragma unroll
+  for (int ks = 0; ks < 18; ks++)
+    w[ks] = __builtin_bit_cast(bf16x8, *(const uint4*)(A.w[big] + n * 288 + ks * 16 + half * 8));
+#pragma unroll
+  for (int ks = 0; ks < 5; ks++)
+    w0[ks] = __builtin_bit_cast(bf16x8, *(const uint4*)(A.w[0] + n * 80 + ks * 16 + half * 8));
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    bias[e] = A.bias[big][8 * (e >> 2) + half * 4 + (e & 3)];
+    bias0[e] = A.bias[0][8 * (e >> 2) + half * 4 + (e & 3)];
+  }
+  __syncthreads();
+
+  const int T = A.SH + 18;
+  for (int t = 0; t < T; t++) {
+    const int r1 = y0 - 8 + t;                          // layer 0 output row
+    const bool l0 = r1 >= y0 - 5 && r1 < y0 + A.SH + 5;
+    const int r6 = y0 + t - 18;                         // layer 5 output row
+    const bool l5 = r6 >= y0 && r6 < y_end;
+    if (wave < 4) {
+      // layer l = wave + 1 computes row r of a_{l+1}; rows needed [y0 - (5 - l), y0 + SH + 5 - l)
+      const int l = wave + 1;
+      const int r = y0 - 6 + t - 2 * (l + 1);
+      if (r >= y0 - (5 - l) && r < y0 + A.SH + 5 - l) {
+        const bool row_ok = r >= 0 && r < A.H;
+        const char* rin = fz_lds + FZ_RING0 + (l - 1) * FZ_RING32;
+        char* dst = fz_lds + FZ_RING0 + l * FZ_RING32 + fz_slot(r, y0) * FZ_ROW32;
+        if (!row_ok) {
+          const f32x16 zero = {};
+          for (int nt = 0; nt < FZ_TILES; nt++)
+            fz_store_gelu(dst, zero, bias, false, nt * 32 + n + 1, half);
+        } else {
+          const char* rows[3] = {rin + fz_slot(r - 1, y0) * FZ_ROW32,
+                                 rin + fz_slot(r, y0) * FZ_ROW32,
+                                 rin + fz_slot(r + 1, y0) * FZ_ROW32};
+          f32x16 prev = fz_mma32(rin, w, r, y0, 0, n, half);
+#pragma unroll
+          for (int nt = 1; nt <= FZ_TILES; nt++) {
+            // MFMAs of tile nt with the epilogue of tile nt - 1 in between
+            const int pp = (nt - 1) * 32 + n + 1;
+            const int xp = xs + (nt - 1) * 32 + n;
+            const bool okp = xp >= 0 && xp < A.W;
+            f32x16 acc = {};
+            unsigned pk[8];
+#pragma unroll
+            for (int ks = 0; ks < 18; ks++) {
+              if (nt < FZ_TILES) {
+                const int tap = ks >> 1, dy = tap / 3, dx = tap % 3;
+                const bf16x8 bv = *(const bf16x8*)(rows[dy] +
+                                                   fz_off32(nt * 32 + n + dx, (ks & 1) * 2 + half));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[ks], bv, acc, 0, 0, 0);
+              }
+              if (ks < 8) pk[ks] = okp ? fz_gelu2(prev, bias, 2 * ks) : 0u;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              *(uint2*)(dst + fz_off32(pp, i) + half * 8) = make_uint2(pk[2 * i], pk[2 * i + 1]);
+            prev = acc;
+          }
+        }
+      }
+    } else if (wave == 4) {
+      // input row r0 into registers (global loads in flight during layer 0)
+      const int r0 = y0 - 6 + t;
+      const bool load = r0 < y0 + A.SH + 6;
+      uint4 in0 = make_uint4(0, 0, 0, 0), in1 = make_uint4(0, 0, 0, 0);
+      if (load && r0 >= 0 && r0 < A.H) {
+        const u16* src = A.x + ((int64_t)b * A.H + r0) * A.W * 8;
+        const int xa = xs + lane, xb = xs + 64 + lane;
+        if (xa >= 0 && xa < A.W) in0 = *(const uint4*)(src + (int64_t)xa * 8);
+        if (lane < FZ_CW - 64 && xb >= 0 && xb < A.W) in1 = *(const uint4*)(src + (int64_t)xb * 8);
+      }
+      if (l0) fz_layer0(fz_lds, w0, bias0, r1, y0, A.H, A.W, xs, 0, n, half);
+      // write after the barrier of the previous step: the slot of r0 held row
+      // r0 - 4, last read by layer 0 in the previous step
+      if (load) {
+        char* row = fz_lds + fz_slot(r0, y0) * FZ_ROW8;
+        *(uint4*)(row + (lane + 1) * 16) = in0;
+        if (lane < FZ_CW - 64) *(uint4*)(row + (64 + lane + 1) * 16) = in1;
+      }
+    } else {
+      const int nt = wave - 5;   // 0, 1, 2
+      if (nt < 2 && l0) fz_layer0(fz_lds, w0, bias0, r1, y0, A.H, A.W, xs, nt + 1, n, half);
+      if (l5) fz_layer5(A, fz_lds, w, bias, r6, y0, b, xs, nt, n, half);
+    }
+    __syncthreads();
+  }
+}
+
+// Bilinear sample (pixel-centre aligned, clamped) of a grid field at full-
+// resolution pixel (i, j): the weights and the four source offsets.
+struct Bilerp {
+  int64_t o00, o01, o10, o11;
+  float tx, ty;
+};
+__device__ __forceinline__ Bilerp bilerp_at(int i, int j, float inv, int hs, int ws, int64_t ld) {
+  float x = ((float)j + 0.5f) * inv - 0.5f;
+  x = fminf(fmaxf(x, 0.0f), (float)(ws - 1));
+  const int x0 = min((int)x, ws - 1), x1 = min(x0 + 1, ws - 1);
+  float y = ((float)i + 0.5f) * inv - 0.5f;
+  y = fminf(fmaxf(y, 0.0f), (float)(hs - 1));
+  const int y0 = min((int)y, hs - 1), y1 = min(y0 + 1, hs - 1);
+  Bilerp b;
+  b.tx = x - (float)x0;
+  b.ty = y - (float)y0;
+  b.o00 = (int64_t)y0 * ld + x0;
+  b.o01 = (int64_t)y0 * ld + x1;
+  b.o10 = (int64_t)y1 * ld + x0;
+  b.o11 = (int64_t)y1 * ld + x1;
+  return b;
+}
+__device__ __forceinline__ float bilerp(const Bilerp& b, float v00, float v01, float v10,
+                                        float v11) {
+  const float top = v00 * (1.0f - b.tx) + v01 * b.tx;
+  const float bot = v10 * (1.0f - b.tx) + v11 * b.tx;
+  return top * (1.0f - b.ty) + bot * b.ty;
+}
+
+// Full-resolution surrogate angle from the network-grid output:
+//   out = bilerp(yf[.][0] + base) + s_scale * yf[cell][1] * (z - bilerp(zg))
+// (the second term only when cout >= 2 and z is given).  yf is [hs][ld][cout],
+// base and zg [hs][ld]; cell = (i / f, j / f).  Four output pixels per lane.
+__global__ void k_cnn_compose(const float* __restrict__ yf, int cout,
+                              const float* __restrict__ base, const float* __restrict__ zg,
+                              int64_t ldg, int hs, int ws, int f, const float* __restrict__ z,
+                              int64_t ldz, float s_scale, float* __restrict__ out, int64_t ldo,
+                              int h, int w) {
+  const int j0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (j0 >= w) return;
+  const float inv = 1.0f / (float)f;
+  for (int i = blockIdx.y; i < h; i += gridDim.y) {
+    float r[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int j = min(j0 + q, w - 1);
+      const Bilerp b = bilerp_at(i, j, inv, hs, ws, ldg);
+      float v = bilerp(b, yf[b.o00 * cout] + base[b.o00], yf[b.o01 * cout] + base[b.o01],
+                       yf[b.o10 * cout] + base[b.o10], yf[b.o11 * cout] + base[b.o11]);
+      if (z) {
+        const float zb = bilerp(b, zg[b.o00], zg[b.o01], zg[b.o10], zg[b.o11]);
+        const int64_t c = (int64_t)min(i / f, hs - 1) * ldg + min(j / f, ws - 1);
+        v += s_scale * yf[c * cout + 1] * (z[(int64_t)i * ldz + j] - zb);
+      }
+      r[q] = v;
+    }
+    float* o = out + (int64_t)i * ldo + j0;
+    if (j0 + 3 < w && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
+      *(float4*)o = make_float4(r[0], r[1], r[2], r[3]);
+    } else {
+      for (int q = 0; q < 4 && j0 + q < w; q++) o[q] = r[q];
+    }
+  }
+}
+
+// Training label of the detail channel for each labelled grid cell: the
+// least-squares slope s* of r = exact - bilerp(lab) on dz = z - bilerp(zg)
+// over the cell's f x f pixels, label_s = s_unit * s*, weight_s = sum dz^2 /
+// (f^2 s_unit^2).  Cells without a label, with a pixel outside the exact
+// rectangle [ey0, ey0 + eh) x [ex0, ex0 + ew), with a NaN in the bilinear
+// stencil of lab, or with sum dz^2 == 0 get label_s = NaN, weight_s = 0.
+__global__ void k_cnn_detail_label(const float* __restrict__ exact, int64_t lde, int ey0,
+                                   int ex0, int eh, int ew, const float* __restrict__ lab,
+                                   const float* __restrict__ zg, int64_t ldg, int hs, int ws,
+                                   int f, const float* __restrict__ z, int64_t ldz,
+                                   float s_unit, float* __restrict__ label_s,
+                                   float* __restrict__ weight_s) {
+  const int cx = blockIdx.x * blockDim.x + threadIdx.x, cy = blockIdx.y;
+  if (cx >= ws || cy >= hs) return;
+  const int64_t co = (int64_t)cy * ldg + cx;
+  float ls = __int_as_float(0x7fc00000), wt = 0.0f;
+  const int i0 = cy * f, j0 = cx * f;
+  if (!isnan(lab[co]) && i0 >= ey0 && i0 + f <= ey0 + eh && j0 >= ex0 && j0 + f <= ex0 + ew) {
+    const float inv = 1.0f / (float)f;
+    double srd = 0.0, sdd = 0.0;
+    bool ok = true;
+    for (int i = i0; i < i0 + f && ok; i++) {
+      for (int j = j0; j < j0 + f; j++) {
+        const Bilerp b = bilerp_at(i, j, inv, hs, ws, ldg);
+        const float bl = bilerp(b, lab[b.o00], lab[b.o01], lab[b.o10], lab[b.o11]);
+        if (isnan(bl)) {
+          ok = false;
+          break;
+        }
+        const float zb = bilerp(b, zg[b.o00], zg[b.o01], zg[b.o10], zg[b.o11]);
+        const double r = (double)exact[(int64_t)(i - ey0) * lde + (j - ex0)] - (double)bl;
+        const double d = (double)z[(int64_t)i * ldz + j] - (double)zb;
+        srd += r * d;
+        sdd += d * d;
+      }
+    }
+    if (ok && sdd > 0.0) {
+      ls = (float)(s_unit * srd / sdd);
+      wt = (float)(sdd / ((double)f * f * (double)s_unit * s_unit));
+    }
+  }
+  label_s[co] = ls;
+  weight_s[co] = wt;
 }
 
 }  // namespace fsg
@@ -364,13 +757,84 @@ extern "C" int fsg_cnn_upsample(const float* src, const float* add, int64_t lds,
   return FSG_OK;
 }
 
-extern "C" int fsg_cnn_mse(const float* yf, const float* base, const float* label, int64_t n,
-                           float scale, void* dz, float* loss, void* stream) {
-  if (!yf || !base || !label || !dz || !loss || n <= 0)
+extern "C" int fsg_cnn_mse(const float* yf, int32_t cout, const float* base, const float* label,
+                           const float* label_s, const float* weight_s, int64_t n, float scale,
+                           void* dz, float* loss, void* stream) {
+  if (!yf || !base || !label || !dz || !loss || n <= 0 || cout < 1 || cout > 32)
     return fail(FSG_E_INVALID, "fsg_cnn_mse: bad arguments");
+  if (label_s && (cout < 2 || !weight_s))
+    return fail(FSG_E_INVALID, "fsg_cnn_mse: the detail term needs cout >= 2 and weight_s");
   hipLaunchKernelGGL(k_cnn_mse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), yf, base, label, n, scale, (u16*)dz, loss);
+                     as_stream(stream), yf, cout, base, label, label_s, weight_s, n, scale,
+                     (u16*)dz, loss);
   FSG_CHECK_LAUNCH("k_cnn_mse");
+  return FSG_OK;
+}
+
+extern "C" int fsg_cnn_forward(const fsg_cnn_net* net, const void* x, int32_t batch, int32_t h,
+                               int32_t w, float* yf, int32_t rows_per_block, void* stream) {
+  if (!net || !x || !yf || batch <= 0 || h <= 0 || w <= 0)
+    return fail(FSG_E_INVALID, "fsg_cnn_forward: bad arguments");
+  if (net->cin != 8 || net->cout < 1 || net->cout > 4)
+    return fail(FSG_E_INVALID, "fsg_cnn_forward: the fused network is 8 -> 32 x 5 -> cout <= 4");
+  FusedArgs A;
+  for (int l = 0; l < 6; l++) {
+    if (!net->wpack[l] || !net->bias[l]) return fail(FSG_E_INVALID, "fsg_cnn_forward: NULL layer");
+    A.w[l] = (const u16*)net->wpack[l];
+    A.bias[l] = net->bias[l];
+  }
+  A.x = (const u16*)x;
+  A.yf = yf;
+  A.B = batch;
+  A.H = h;
+  A.W = w;
+  A.cout = net->cout;
+  int sh = rows_per_block > 0 ? rows_per_block : 256;
+  A.n_rb = (h + sh - 1) / sh;
+  A.SH = (h + A.n_rb - 1) / A.n_rb;   // balanced blocks
+  A.n_strips = (w + FZ_SW - 1) / FZ_SW;
+  const int64_t blocks = (int64_t)batch * A.n_strips * A.n_rb;
+  if (blocks > 0x7fffffff) return fail(FSG_E_INVALID, "fsg_cnn_forward: too many blocks");
+  hipStream_t s = as_stream(stream);
+  TimedScope ts("k_cnn_fused", s);
+  hipLaunchKernelGGL(k_cnn_fused, dim3((unsigned)blocks), dim3(512), FZ_LDS, s, A);
+  FSG_CHECK_LAUNCH("k_cnn_fused");
+  return FSG_OK;
+}
+
+extern "C" int fsg_cnn_compose(const float* yf, int32_t cout, const float* base, const float* zg,
+                               int64_t ldg, int32_t hs, int32_t ws, int32_t factor,
+                               const float* z, int64_t ldz, float s_scale, float* out,
+                               int64_t ldo, int32_t h, int32_t w, void* stream) {
+  if (!yf || !base || !out || cout < 1 || cout > 32 || hs <= 0 || ws <= 0 || factor <= 0 ||
+      h <= 0 || w <= 0 || ldg < ws || ldo < w)
+    return fail(FSG_E_INVALID, "fsg_cnn_compose: bad arguments");
+  if (z && (cout < 2 || !zg || ldz < w))
+    return fail(FSG_E_INVALID, "fsg_cnn_compose: the detail term needs cout >= 2, zg and z");
+  if ((int64_t)hs * factor < h || (int64_t)ws * factor < w)
+    return fail(FSG_E_INVALID, "fsg_cnn_compose: output larger than the grid covers");
+  hipStream_t s = as_stream(stream);
+  TimedScope ts("k_cnn_compose", s);
+  hipLaunchKernelGGL(k_cnn_compose, dim3((w + 1023) / 1024, std::min(h, 16384)), dim3(256), 0,
+                     s, yf, cout, base, zg, ldg, hs, ws, factor, z, ldz, s_scale, out, ldo, h,
+                     w);
+  FSG_CHECK_LAUNCH("k_cnn_compose");
+  return FSG_OK;
+}
+
+extern "C" int fsg_cnn_detail_label(const float* exact, int64_t lde, int32_t ey0, int32_t ex0,
+                                    int32_t eh, int32_t ew, const float* lab, const float* zg,
+                                    int64_t ldg, int32_t hs, int32_t ws, int32_t factor,
+                                    const float* z, int64_t ldz, float s_unit, float* label_s,
+                                    float* weight_s, void* stream) {
+  if (!exact || !lab || !zg || !z || !label_s || !weight_s || hs <= 0 || ws <= 0 ||
+      factor <= 0 || ldg < ws || lde < ew || ldz < (int64_t)ws * factor || s_unit <= 0.0f ||
+      ey0 < 0 || ex0 < 0 || eh < 0 || ew < 0)
+    return fail(FSG_E_INVALID, "fsg_cnn_detail_label: bad arguments");
+  hipLaunchKernelGGL(k_cnn_detail_label, dim3((ws + 63) / 64, hs), dim3(64), 0,
+                     as_stream(stream), exact, lde, ey0, ex0, eh, ew, lab, zg, ldg, hs, ws,
+                     factor, z, ldz, s_unit, label_s, weight_s);
+  FSG_CHECK_LAUNCH("k_cnn_detail_label");
   return FSG_OK;
 }
 

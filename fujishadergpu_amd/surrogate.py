@@ -8,12 +8,15 @@ angle, in degrees, before the reference's / (pi/2), [0, 1] clip and gamma):
 1. base: the exact openness computed on the 16x block-mean grid (distances
    and pixel size scaled by 16), bilinearly upsampled -- the method without a
    network (the reference's decimated "spatial" style);
-2. the network: 6 conv3x3 layers, 32 channels, GELU, BF16 weights and
-   activations with FP32 accumulation (csrc/cnn.hip), run on the 4x grid.  Its
-   input is a 4-level detrended elevation pyramid -- (box mean_r - z) / (r px)
-   for r = 16, 64, 256, 1024 px -- and its output is a residual added to the
-   base on that grid;
-3. the sum is bilinearly upsampled to full resolution.
+2. the network: 6 conv3x3 layers, 32 channels, GELU (tanh form), BF16
+   weights and activations with FP32 accumulation (csrc/cnn.hip), run on the
+   GRID-times block-mean grid.  Its input is a 4-level detrended elevation
+   pyramid -- (box mean_r - z) / (r px) for r = 16, 64, 256, 1024 px -- and it
+   outputs two channels per grid cell: a residual added to the base, and the
+   sensitivity s of the angle to the pixel's own elevation;
+3. compose: the bilinear upsampling of base + residual to full resolution,
+   plus s * (z - bilinear(grid mean of z)) -- the sub-cell relief the grid
+   cannot carry.
 
 The exact kernel stays the fallback and the judge: ``judge`` re-computes 1%
 of each tile's pixels exactly and replaces every tile whose mean absolute
@@ -40,11 +43,12 @@ from . import _native as N
 from . import ops, synth
 from .algorithms import openness_table
 
-WIDTHS = (8, 32, 32, 32, 32, 32, 1)      # input channels (4 real + 4 zero) ... output
-GRID = 4                                   # network grid factor
+WIDTHS = (8, 32, 32, 32, 32, 32, 2)      # input channels (4 real + 4 zero) ... output
+GRID = 8                                   # network grid factor
 BASE = 16                                  # base exact-openness grid factor
-LEVELS = (4, 16, 64, 256)                  # box radii on the network grid
+LEVELS_PX = (16, 64, 256, 1024)            # box radii of the pyramid (full-resolution px)
 FEAT_GAIN = 10.0
+DETAIL_UNIT = 20.0                         # network units per (degree / metre) of s
 CONTEXT = 1024                             # training window halo (px)
 TILE = 512                                 # labelled centre (px)
 MARGIN = 8                                 # network-grid cells around the labelled centre
@@ -172,9 +176,25 @@ class Net:
                 acts.append(y)
                 pres.append(pre)
             a = y          # without the cache only the live activation is held
-        yf = torch.empty((bsz, h, w), dtype=torch.float32, device=self.device)
+        yf = torch.empty((bsz, h, w, self.shapes[-1][0]), dtype=torch.float32, device=self.device)
         self._conv(len(self.shapes) - 1, a, h, w, bsz, N.CNN_LINEAR, yf=yf)
         return (yf, acts, pres) if keep else yf
+
+    def infer(self, x: torch.Tensor, rows_per_block: int = 0) -> torch.Tensor:
+        """Inference through the fused 6-layer kernel (bit-identical to
+        ``forward``): x bf16 [B, h, w, 8] -> f32 [B, h, w, cout]."""
+        bsz, h, w, _ = x.shape
+        if not x.is_contiguous():
+            raise ValueError("infer needs a contiguous input")
+        net = N.fsg_cnn_net()
+        for li in range(len(self.shapes)):
+            net.wpack[li] = self.wf[li].data_ptr()
+            net.bias[li] = self.b(li).data_ptr()
+        net.cin, net.cout = WIDTHS[0], WIDTHS[-1]
+        yf = torch.empty((bsz, h, w, WIDTHS[-1]), dtype=torch.float32, device=self.device)
+        N.check(N.lib().fsg_cnn_forward(C.byref(net), x.data_ptr(), bsz, h, w, yf.data_ptr(),
+                                        rows_per_block, _s()))
+        return yf
 
     def backward(self, acts, pres, dz_last: torch.Tensor):
         bsz, h, w, _ = dz_last.shape
@@ -225,6 +245,7 @@ def exact_openness(win: torch.Tensor, reg, radius: int, n_dirs: int, pixel_size:
 class Inputs:
     feat: torch.Tensor    # bf16 [hc, wc, 8] on the network grid
     base: torch.Tensor    # f32 [hc, wc]: upsampled exact openness of the 16x grid
+    zg: torch.Tensor      # f32 [hc, wc]: block mean of the elevation
     hc: int
     wc: int
 
@@ -235,18 +256,33 @@ def inputs(win: torch.Tensor, radius: int, n_dirs: int, pixel_size: float) -> In
     if H % BASE or W % BASE:
         raise ValueError("surrogate windows need sides that are multiples of 16")
     hc, wc = H // GRID, W // GRID
+    levels = [r // GRID for r in LEVELS_PX]
     zc = ops.block_mean(win, GRID)
-    means = ops.topousm_coarse_means(win, GRID, list(LEVELS))
+    means = ops.topousm_coarse_means(win, GRID, levels)
     feat = torch.empty((hc, wc, 8), dtype=torch.int16, device=win.device)
     mp = (C.c_void_p * 4)(*[means[k].data_ptr() for k in range(4)])
-    inv = (C.c_float * 4)(*[FEAT_GAIN / (r * GRID * pixel_size) for r in LEVELS])
+    inv = (C.c_float * 4)(*[FEAT_GAIN / (r * pixel_size) for r in LEVELS_PX])
     N.check(N.lib().fsg_cnn_features(zc.data_ptr(), zc.stride(0), mp, means.stride(1), inv, hc,
                                      wc, feat.data_ptr(), _s()))
     z16 = ops.block_mean(win, BASE)
     th16 = exact_openness(z16, ops.full_region(z16), max(1, radius // BASE), n_dirs,
                           pixel_size * BASE)
     base = _up(th16, BASE // GRID, hc, wc)
-    return Inputs(feat, base, hc, wc)
+    return Inputs(feat, base, zc, hc, wc)
+
+
+def compose(yf: torch.Tensor, base: torch.Tensor, zg: torch.Tensor, z: Optional[torch.Tensor],
+            out: torch.Tensor, detail: bool = True) -> torch.Tensor:
+    """Full-resolution angle from the grid output yf [hs, ws, 2]: bilinear of
+    residual + base, plus s * (z - bilinear(zg)) when ``detail``."""
+    hs, ws = base.shape
+    H, W = out.shape
+    zp = z.data_ptr() if (detail and z is not None) else None
+    N.check(N.lib().fsg_cnn_compose(yf.data_ptr(), yf.shape[-1], base.data_ptr(), zg.data_ptr(),
+                                    ws, hs, ws, GRID, zp, z.stride(0) if zp else 0,
+                                    1.0 / DETAIL_UNIT, out.data_ptr(), out.stride(0), H, W,
+                                    _s()))
+    return out
 
 
 # -------------------------------------------------------------- training set
@@ -255,7 +291,11 @@ def inputs(win: torch.Tensor, radius: int, n_dirs: int, pixel_size: float) -> In
 class TrainSet:
     feat: torch.Tensor     # [n, P, P, 8] bf16
     base: torch.Tensor     # [n, P, P] f32
+    zg: torch.Tensor       # [n, P, P] f32 block-mean elevation
     label: torch.Tensor    # [n, P, P] f32, NaN outside the labelled centre
+    label_s: torch.Tensor  # [n, P, P] f32 detail label (NaN: none)
+    weight_s: torch.Tensor  # [n, P, P] f32 detail weight
+    z: torch.Tensor        # [n, P*GRID, P*GRID] f32 full-resolution elevation of the patch
     exact: torch.Tensor    # [n, TILE, TILE] f32 full-resolution exact angles
     seeds: List[int]
 
@@ -266,12 +306,18 @@ def make_tiles(seeds: Sequence[int], device, radius=1024, n_dirs=16, pixel_size=
     log-U[100, 4000] m); the centre TILE^2 is labelled by the exact kernel."""
     n = len(seeds)
     P = TILE // GRID + 2 * MARGIN
+    f32 = dict(dtype=torch.float32, device=device)
     feat = torch.empty((n, P, P, 8), dtype=torch.int16, device=device)
-    base = torch.empty((n, P, P), dtype=torch.float32, device=device)
-    label = torch.full((n, P, P), float("nan"), dtype=torch.float32, device=device)
-    exact = torch.empty((n, TILE, TILE), dtype=torch.float32, device=device)
+    base = torch.empty((n, P, P), **f32)
+    zg = torch.empty((n, P, P), **f32)
+    label = torch.full((n, P, P), float("nan"), **f32)
+    label_s = torch.empty((n, P, P), **f32)
+    weight_s = torch.empty((n, P, P), **f32)
+    zp = torch.empty((n, P * GRID, P * GRID), **f32)
+    exact = torch.empty((n, TILE, TILE), **f32)
     S = TILE + 2 * CONTEXT
     c0 = CONTEXT // GRID - MARGIN
+    o = MARGIN * GRID
     for i, sd in enumerate(seeds):
         r = np.random.default_rng(int(sd))
         hurst = float(r.uniform(0.6, 0.9))
@@ -280,13 +326,20 @@ def make_tiles(seeds: Sequence[int], device, radius=1024, n_dirs=16, pixel_size=
         inp = inputs(win, radius, n_dirs, pixel_size)
         feat[i] = inp.feat[c0:c0 + P, c0:c0 + P]
         base[i] = inp.base[c0:c0 + P, c0:c0 + P]
+        zg[i] = inp.zg[c0:c0 + P, c0:c0 + P]
+        zp[i] = win[c0 * GRID:(c0 + P) * GRID, c0 * GRID:(c0 + P) * GRID]
         reg = ops.region(CONTEXT, CONTEXT, TILE, TILE)
         exact_openness(win, reg, radius, n_dirs, pixel_size, out=exact[i])
         lab = ops.block_mean(exact[i], GRID)
         label[i, MARGIN:MARGIN + TILE // GRID, MARGIN:MARGIN + TILE // GRID] = lab
+        N.check(N.lib().fsg_cnn_detail_label(exact[i].data_ptr(), TILE, o, o, TILE, TILE,
+                                             label[i].data_ptr(), zg[i].data_ptr(), P, P, P,
+                                             GRID, zp[i].data_ptr(), P * GRID, DETAIL_UNIT,
+                                             label_s[i].data_ptr(), weight_s[i].data_ptr(),
+                                             _s()))
         if log and (i + 1) % 64 == 0:
             log(f"tiles labelled: {i + 1}/{n}")
-    return TrainSet(feat, base, label, exact, list(seeds))
+    return TrainSet(feat, base, zg, label, label_s, weight_s, zp, exact, list(seeds))
 
 
 def train(net: Net, ts: TrainSet, steps: int = 1500, batch: int = 16, lr: float = 2e-3,
@@ -295,7 +348,7 @@ def train(net: Net, ts: TrainSet, steps: int = 1500, batch: int = 16, lr: float 
     block-mean exact angle, cosine learning-rate decay."""
     rng = np.random.default_rng(seed)
     n, P = ts.feat.shape[0], ts.feat.shape[1]
-    loss = torch.zeros(1, dtype=torch.float32, device=net.device)
+    loss = torch.zeros(2, dtype=torch.float32, device=net.device)
     losses = []
     dz = torch.empty((batch, P, P, 32), dtype=torch.int16, device=net.device)
     n_lab = batch * (TILE // GRID) ** 2
@@ -304,37 +357,44 @@ def train(net: Net, ts: TrainSet, steps: int = 1500, batch: int = 16, lr: float 
         x = ts.feat.index_select(0, idx).contiguous()
         b = ts.base.index_select(0, idx).contiguous()
         y = ts.label.index_select(0, idx).contiguous()
+        ys = ts.label_s.index_select(0, idx).contiguous()
+        ws = ts.weight_s.index_select(0, idx).contiguous()
         yf, acts, pres = net.forward(x, keep=True)
         loss.zero_()
-        N.check(N.lib().fsg_cnn_mse(yf.data_ptr(), b.data_ptr(), y.data_ptr(), yf.numel(),
-                                    1.0 / n_lab, dz.data_ptr(), loss.data_ptr(), _s()))
+        N.check(N.lib().fsg_cnn_mse(yf.data_ptr(), yf.shape[-1], b.data_ptr(), y.data_ptr(),
+                                    ys.data_ptr(), ws.data_ptr(), b.numel(), 1.0 / n_lab,
+                                    dz.data_ptr(), loss.data_ptr(), _s()))
         net.backward(acts, pres, dz)
         cur = lr * 0.5 * (1.0 + math.cos(math.pi * it / max(1, steps)))
         net.adam(cur)
         if it % 100 == 0 or it == steps - 1:
-            lv = float(loss.item()) / n_lab
-            losses.append(lv)
+            lv = loss.cpu().numpy() / n_lab
+            losses.append(float(lv.sum()))
             if log:
-                log(f"step {it}: mse {lv:.4f} deg^2")
+                log(f"step {it}: mse {lv[0]:.4f} + detail {lv[1]:.4f} deg^2")
     return losses
 
 
-def predict_patches(net: Net, feat: torch.Tensor, base: torch.Tensor) -> torch.Tensor:
+def predict_patches(net: Net, ts: TrainSet, detail: bool = True) -> torch.Tensor:
     """Full-resolution angles (degrees) of the labelled centres of patches."""
-    yf = net.forward(feat.contiguous())
-    out = []
-    for i in range(feat.shape[0]):
-        up = _up(yf[i], GRID, feat.shape[1] * GRID, feat.shape[2] * GRID, add=base[i])
-        o = MARGIN * GRID
-        out.append(up[o:o + TILE, o:o + TILE])
-    return torch.stack(out)
+    yf = net.infer(ts.feat.contiguous())
+    P = ts.feat.shape[1]
+    o = MARGIN * GRID
+    out = torch.empty((ts.feat.shape[0], TILE, TILE), dtype=torch.float32, device=yf.device)
+    full = torch.empty((P * GRID, P * GRID), dtype=torch.float32, device=yf.device)
+    for i in range(ts.feat.shape[0]):
+        compose(yf[i], ts.base[i], ts.zg[i], ts.z[i], full, detail=detail)
+        out[i] = full[o:o + TILE, o:o + TILE]
+    return out
 
 
-def base_patches(base: torch.Tensor) -> torch.Tensor:
+def base_patches(ts: TrainSet) -> torch.Tensor:
+    """The method without the network: the upsampled base alone."""
+    P = ts.feat.shape[1]
+    o = MARGIN * GRID
     out = []
-    for i in range(base.shape[0]):
-        up = _up(base[i], GRID, base.shape[1] * GRID, base.shape[2] * GRID)
-        o = MARGIN * GRID
+    for i in range(ts.feat.shape[0]):
+        up = _up(ts.base[i], GRID, P * GRID, P * GRID)
         out.append(up[o:o + TILE, o:o + TILE])
     return torch.stack(out)
 
@@ -344,10 +404,11 @@ def evaluate(net: Net, ts: TrainSet, threshold: float = 1.5, sample_frac: float 
     """Held-out accuracy (MAE in degrees vs the exact kernel) of the surrogate,
     of the method without the network (base), and after the sampled-pixel
     judge's exact fallback."""
-    pred = predict_patches(net, ts.feat, ts.base)
-    basep = base_patches(ts.base)
+    pred = predict_patches(net, ts)
+    basep = base_patches(ts)
     err = (pred - ts.exact).abs()
     err_b = (basep - ts.exact).abs()
+    err_nd = (predict_patches(net, ts, detail=False) - ts.exact).abs()
     # judge: 1% of each tile's pixels re-computed exactly (here: read from the
     # exact labels of the same pixels), tile replaced when their MAE > threshold
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -363,6 +424,7 @@ def evaluate(net: Net, ts: TrainSet, threshold: float = 1.5, sample_frac: float 
             final.append(err[i])
     final = torch.stack(final)
     return {"mae_surrogate_deg": float(err.mean()), "mae_base_deg": float(err_b.mean()),
+            "mae_without_detail_deg": float(err_nd.mean()),
             "p99_abs_err_surrogate_deg": float(torch.quantile(err.reshape(-1)[::97], 0.99)),
             "mae_after_fallback_deg": float(final.mean()), "tiles_flagged": flagged,
             "tiles": int(err.shape[0])}
@@ -374,13 +436,11 @@ def surrogate_openness(net: Net, win: torch.Tensor, radius: int, n_dirs: int,
                        pixel_size: float, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Full-resolution surrogate angles (degrees) over a whole window."""
     inp = inputs(win, radius, n_dirs, pixel_size)
-    yf = net.forward(inp.feat.view(1, inp.hc, inp.wc, 8))
+    yf = net.infer(inp.feat.view(1, inp.hc, inp.wc, 8))
     H, W = win.shape
     if out is None:
         out = torch.empty((H, W), dtype=torch.float32, device=win.device)
-    N.check(N.lib().fsg_cnn_upsample(yf.data_ptr(), inp.base.data_ptr(), inp.wc, inp.hc, inp.wc,
-                                     GRID, out.data_ptr(), out.stride(0), H, W, _s()))
-    return out
+    return compose(yf[0], inp.base, inp.zg, win, out)
 
 
 def judge(win: torch.Tensor, approx: torch.Tensor, radius: int, n_dirs: int, pixel_size: float,

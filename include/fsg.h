@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 4
+#define FSG_ABI_VERSION 5
 
 enum {
   FSG_OK = 0,
@@ -414,20 +414,60 @@ int fsg_cnn_adam(float* p, const float* g, float* m, float* v, int32_t n, float 
                  float beta1, float beta2, float eps, int32_t step, void* stream);
 int fsg_cnn_to_bf16(const float* a, void* b, int64_t n, void* stream);
 
+/* fsg_cnn_forward: the whole 6-layer network (8 -> 32 x 5 -> cout, GELU after
+ * layers 0..4, linear layer 5) in one launch, identical bit for bit to the
+ * layer-by-layer fsg_cnn_conv3x3 chain.  x [batch][h][w][8] bf16, yf
+ * [batch][h][w][cout] f32 (cout <= 4); wpack[l] from fsg_cnn_pack (kpad 80
+ * for layer 0, 288 for the others); rows_per_block <= 0 picks the default
+ * (256 rows, balanced). */
+typedef struct fsg_cnn_net {
+  const void* wpack[6];
+  const float* bias[6];
+  int32_t cin, cout;
+} fsg_cnn_net;
+
+int fsg_cnn_forward(const fsg_cnn_net* net, const void* x, int32_t batch, int32_t h, int32_t w,
+                    float* yf, int32_t rows_per_block, void* stream);
+
 /* Surrogate plumbing.  fsg_cnn_features: the 4-level detrended elevation
  * pyramid, channel k = bf16((means[k] - z) * inv[k]) for k < 4 and zero
  * channels 4..7, NHWC [h][w][8].  fsg_cnn_upsample: bilinear, pixel-centre
  * aligned (sample at ((i + .5) / factor - .5, (j + .5) / factor - .5),
- * clamped) of src (+ add, optional) into out [h][w].  fsg_cnn_mse: pred =
- * yf + base against label (NaN: no label) -- loss[0] += sum of squares,
- * dz[p] = (bf16(2 * scale * (pred - label)), 0, ..., 0) over 32 channels. */
+ * clamped) of src (+ add, optional) into out [h][w].
+ *
+ * fsg_cnn_mse: loss of the network output yf [n][cout] -- the angle term
+ * d = yf[p][0] + base[p] - label[p] (label NaN: no label) and, when label_s
+ * is given (cout >= 2), the detail term e = yf[p][1] - label_s[p] weighted by
+ * weight_s[p]; loss[0] += sum d^2, loss[1] += sum w e^2, dz[p] = bf16 of
+ * (2 scale d, 2 scale w e, 0, ..., 0) over 32 channels.
+ *
+ * fsg_cnn_compose: the full-resolution angle out [h][w] = bilinear(yf[.][0]
+ * + base) + s_scale * yf[cell][1] * (z - bilinear(zg)), cell = (i / factor,
+ * j / factor); yf [hs][ldg][cout], base and zg [hs][ldg]; z == NULL drops the
+ * detail term.
+ *
+ * fsg_cnn_detail_label: per grid cell of lab/zg [hs][ldg], the least-squares
+ * slope s* of exact - bilinear(lab) on z - bilinear(zg) over the cell's
+ * factor^2 pixels; label_s = s_unit * s*, weight_s = sum dz^2 / (factor^2
+ * s_unit^2).  exact covers full-resolution rows [ey0, ey0 + eh) and columns
+ * [ex0, ex0 + ew) of the grid's pixel frame, z covers the whole frame; cells
+ * without a full label get label_s = NaN, weight_s = 0. */
 int fsg_cnn_features(const float* z, int64_t ldz, const float* const* means, int64_t ldm,
                      const float* inv, int32_t h, int32_t w, void* out, void* stream);
 int fsg_cnn_upsample(const float* src, const float* add, int64_t lds, int32_t hs, int32_t ws,
                      int32_t factor, float* out, int64_t ldo, int32_t h, int32_t w,
                      void* stream);
-int fsg_cnn_mse(const float* yf, const float* base, const float* label, int64_t n, float scale,
-                void* dz, float* loss, void* stream);
+int fsg_cnn_mse(const float* yf, int32_t cout, const float* base, const float* label,
+                const float* label_s, const float* weight_s, int64_t n, float scale, void* dz,
+                float* loss, void* stream);
+int fsg_cnn_compose(const float* yf, int32_t cout, const float* base, const float* zg,
+                    int64_t ldg, int32_t hs, int32_t ws, int32_t factor, const float* z,
+                    int64_t ldz, float s_scale, float* out, int64_t ldo, int32_t h, int32_t w,
+                    void* stream);
+int fsg_cnn_detail_label(const float* exact, int64_t lde, int32_t ey0, int32_t ex0, int32_t eh,
+                         int32_t ew, const float* lab, const float* zg, int64_t ldg, int32_t hs,
+                         int32_t ws, int32_t factor, const float* z, int64_t ldz, float s_unit,
+                         float* label_s, float* weight_s, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched fused launch                                                 */

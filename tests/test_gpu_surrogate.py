@@ -55,7 +55,7 @@ def test_conv_layers_match_the_oracle(cuda):
         w, b = params[li]
         bb = np.zeros(w.shape[0]) + b[:w.shape[0]]
         z = ocnn.conv3x3(a, w, bb)
-        got = pres[li].cpu().numpy() if li < 5 else yf.cpu().numpy()[..., None]
+        got = pres[li].cpu().numpy() if li < 5 else yf.cpu().numpy()
         got = got[..., :w.shape[0]]
         err = np.abs(got - z).max() / max(1e-6, np.abs(z).max())
         assert err < 2e-5, f"layer {li}: relative error {err:.2e}"
@@ -69,15 +69,21 @@ def test_backward_matches_oracle_gradients(cuda):
     net, x = _net_and_input(cuda, seed=3)
     bsz, h, w = x.shape[:3]
     rng = np.random.default_rng(9)
-    base = torch.from_numpy(rng.normal(0, 1, (bsz, h, w)).astype(np.float32)).to(cuda)
-    label = torch.from_numpy(rng.normal(0, 1, (bsz, h, w)).astype(np.float32)).to(cuda)
+    f32 = lambda a: torch.from_numpy(np.asarray(a, np.float32)).to(cuda)  # noqa: E731
+    base = f32(rng.normal(0, 1, (bsz, h, w)))
+    label = f32(rng.normal(0, 1, (bsz, h, w)))
     label[0, :2] = float("nan")
+    label_s = f32(rng.normal(0, 1, (bsz, h, w)))
+    label_s[1, 3:5] = float("nan")
+    weight_s = f32(rng.uniform(0.2, 2.0, (bsz, h, w)))
     yf, acts, pres = net.forward(x, keep=True)
+    assert yf.shape == (bsz, h, w, 2)
     dz = torch.empty((bsz, h, w, 32), dtype=torch.int16, device=cuda)
-    loss = torch.zeros(1, dtype=torch.float32, device=cuda)
+    loss = torch.zeros(2, dtype=torch.float32, device=cuda)
     n = int((~torch.isnan(label)).sum())
-    N.check(N.lib().fsg_cnn_mse(yf.data_ptr(), base.data_ptr(), label.data_ptr(), yf.numel(),
-                                1.0 / n, dz.data_ptr(), loss.data_ptr(),
+    N.check(N.lib().fsg_cnn_mse(yf.data_ptr(), 2, base.data_ptr(), label.data_ptr(),
+                                label_s.data_ptr(), weight_s.data_ptr(), base.numel(), 1.0 / n,
+                                dz.data_ptr(), loss.data_ptr(),
                                 torch.cuda.current_stream().cuda_stream))
     net.backward(acts, pres, dz)
     torch.cuda.synchronize()
@@ -85,10 +91,10 @@ def test_backward_matches_oracle_gradients(cuda):
     xa = _bf16_to_f32(x).cpu().numpy().astype(np.float64)
     out, cache = ocnn.forward([(w_, np.zeros(w_.shape[0]) + b_[:w_.shape[0]]) for w_, b_ in params],
                               xa)
-    lab = label.cpu().numpy().astype(np.float64)
-    assert float(loss.item()) / n == pytest.approx(
-        ocnn.loss(out, base.cpu().numpy(), lab, 1.0 / n), rel=2e-2)
-    grads = ocnn.backward(params, cache, out, base.cpu().numpy(), lab, 1.0 / n)
+    lab, ls, ws = (a.cpu().numpy().astype(np.float64) for a in (label, label_s, weight_s))
+    assert float(loss.sum().item()) / n == pytest.approx(
+        ocnn.loss(out, base.cpu().numpy(), lab, 1.0 / n, ls, ws), rel=2e-2)
+    grads = ocnn.backward(params, cache, out, base.cpu().numpy(), lab, 1.0 / n, ls, ws)
     for li in range(6):
         co = net.shapes[li][0]
         gw = net.gw(li).cpu().numpy().astype(np.float64)
@@ -100,6 +106,60 @@ def test_backward_matches_oracle_gradients(cuda):
             cos = float(np.dot(g, r) / (np.linalg.norm(g) * np.linalg.norm(r)))
             rel = float(np.linalg.norm(g - r) / np.linalg.norm(r))
             assert cos > 0.999 and rel < 0.05, f"layer {li} d{name}: cos {cos:.5f} rel {rel:.3f}"
+
+
+def test_compose_and_detail_label_match_the_oracle(cuda):
+    from fujishadergpu_amd import _native as N
+    rng = np.random.default_rng(4)
+    hs, ws, f, unit = 9, 13, 8, 20.0
+    H, W = hs * f - 5, ws * f - 3                # ragged: not a multiple of the grid
+    yf = rng.normal(size=(hs, ws, 2))
+    base = rng.normal(size=(hs, ws))
+    zg = rng.normal(size=(hs, ws)) * 30
+    z = ocnn.bilinear(zg, f, hs * f, ws * f) + rng.normal(size=(hs * f, ws * f)) * 3
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(cuda)  # noqa: E731
+    s = torch.cuda.current_stream().cuda_stream
+    out = torch.empty((H, W), dtype=torch.float32, device=cuda)
+    yt, bt, zgt, zt = t(yf), t(base), t(zg), t(z)
+    N.check(N.lib().fsg_cnn_compose(yt.data_ptr(), 2, bt.data_ptr(), zgt.data_ptr(), ws, hs, ws, f,
+                                    zt.data_ptr(), zt.stride(0), 1.0 / unit, out.data_ptr(), W, H,
+                                    W, s))
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
+    want = ocnn.compose(f32(yf), f32(base), f32(zg), f, z=f32(z)[:H, :W], s_scale=1.0 / unit)
+    assert np.abs(out.cpu().numpy() - want).max() < 2e-4
+    N.check(N.lib().fsg_cnn_compose(yt.data_ptr(), 2, bt.data_ptr(), zgt.data_ptr(), ws, hs, ws, f,
+                                    None, 0, 1.0 / unit, out.data_ptr(), W, H, W, s))
+    want = ocnn.bilinear(f32(yf)[..., 0] + f32(base), f, H, W)
+    assert np.abs(out.cpu().numpy() - want).max() < 2e-5
+    # detail labels: exact known on a sub-rectangle, lab NaN outside its cells
+    lab = rng.normal(size=(hs, ws))
+    lab[:2] = np.nan
+    ey0, ex0, eh, ew = 2 * f, f, 5 * f, 9 * f
+    exact = rng.normal(size=(eh, ew))
+    ls = torch.empty((hs, ws), dtype=torch.float32, device=cuda)
+    wt = torch.empty_like(ls)
+    et, lt = t(exact), t(lab)
+    N.check(N.lib().fsg_cnn_detail_label(et.data_ptr(), ew, ey0, ex0, eh, ew, lt.data_ptr(),
+                                         zgt.data_ptr(), ws, hs, ws, f, zt.data_ptr(),
+                                         zt.stride(0), unit, ls.data_ptr(), wt.data_ptr(), s))
+    rls, rwt = ocnn.detail_label(f32(exact), ey0, ex0, f32(lab), f32(zg), f, f32(z), unit)
+    gls, gwt = ls.cpu().numpy(), wt.cpu().numpy()
+    assert np.array_equal(np.isnan(gls), np.isnan(rls))
+    m = ~np.isnan(rls)
+    assert m.sum() > 20
+    assert np.allclose(gls[m], rls[m], rtol=1e-3, atol=1e-4)
+    assert np.allclose(gwt, rwt, rtol=1e-4, atol=1e-7)
+
+
+@pytest.mark.parametrize("bsz,h,w,rows", [(2, 37, 200, 16), (1, 9, 70, 0), (3, 5, 84, 2),
+                                           (1, 130, 85, 64)])
+def test_fused_forward_is_bit_identical_to_the_layer_chain(cuda, bsz, h, w, rows):
+    net, x = _net_and_input(cuda, seed=5, bsz=bsz, h=h, w=w)
+    want = net.forward(x)
+    got = net.infer(x, rows_per_block=rows)
+    torch.cuda.synchronize()
+    assert got.shape == want.shape
+    assert torch.equal(got, want), float((got - want).abs().max())
 
 
 def test_adam_step_matches_the_oracle(cuda):
@@ -130,3 +190,4 @@ def test_short_training_beats_the_method_without_the_network(cuda):
     print("surrogate before", before, "after", after, "losses", losses, flush=True)
     assert losses[-1] < losses[0]
     assert after["mae_surrogate_deg"] < after["mae_base_deg"]
+    assert after["mae_surrogate_deg"] < after["mae_without_detail_deg"]

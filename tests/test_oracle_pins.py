@@ -459,16 +459,21 @@ def test_meters_per_degree_at_35_north():
 def test_cnn_oracle_backward_matches_finite_differences():
     from oracle import cnn
     rng = np.random.default_rng(0)
-    params = cnn.init_params(rng, [3, 5, 4, 1])
+    params = cnn.init_params(rng, [3, 5, 4, 2])
     x = rng.normal(size=(2, 6, 7, 3))
     base = rng.normal(size=(2, 6, 7))
     label = rng.normal(size=(2, 6, 7))
     label[0, 0, 0] = np.nan                      # unlabelled cell
+    label_s = rng.normal(size=(2, 6, 7))
+    label_s[1, 2, 3] = np.nan
+    weight_s = rng.uniform(0.1, 2.0, size=(2, 6, 7))
     out, cache = cnn.forward(params, x)
-    grads = cnn.backward(params, cache, out, base, label, scale=0.3)
+    grads = cnn.backward(params, cache, out, base, label, scale=0.3, label_s=label_s,
+                         weight_s=weight_s)
 
     def f(ps):
-        return cnn.loss(cnn.forward(ps, x)[0], base, label, scale=0.3)
+        return cnn.loss(cnn.forward(ps, x)[0], base, label, scale=0.3, label_s=label_s,
+                        weight_s=weight_s)
     eps = 1e-6
     for li in range(len(params)):
         for which in (0, 1):
@@ -484,6 +489,57 @@ def test_cnn_oracle_backward_matches_finite_differences():
                 num = (up - dn) / (2 * eps)
                 ana = grads[li][which].reshape(-1)[idx]
                 assert ana == pytest.approx(num, rel=1e-5, abs=1e-7), (li, which, idx)
+
+
+def test_cnn_oracle_gelu_is_the_tanh_form():
+    from oracle import cnn
+    x = np.linspace(-8, 8, 4001)
+    # the published tanh form, and its distance from the erf form
+    tanh_form = 0.5 * x * (1 + np.tanh(math.sqrt(2 / math.pi) * (x + 0.044715 * x ** 3)))
+    assert np.abs(cnn.gelu(x) - tanh_form).max() < 1e-12
+    assert np.abs(cnn.gelu(x) - cnn.gelu_erf(x)).max() < 5e-4
+    assert cnn.gelu(np.array([0.0]))[0] == 0.0
+    h = 1e-6
+    num = (cnn.gelu(x + h) - cnn.gelu(x - h)) / (2 * h)
+    assert np.abs(cnn.gelu_grad(x) - num).max() < 1e-7
+
+
+def test_cnn_oracle_bilinear_reproduces_affine_fields():
+    from oracle import cnn
+    hs, ws, f = 7, 9, 4
+    gy, gx = np.meshgrid(np.arange(hs, dtype=float), np.arange(ws, dtype=float), indexing="ij")
+    src = 3.0 + 0.7 * gy - 1.3 * gx
+    up = cnn.bilinear(src, f, hs * f, ws * f)
+    ys = (np.arange(hs * f) + 0.5) / f - 0.5
+    xs = (np.arange(ws * f) + 0.5) / f - 0.5
+    want = 3.0 + 0.7 * np.clip(ys, 0, hs - 1)[:, None] - 1.3 * np.clip(xs, 0, ws - 1)[None, :]
+    assert np.abs(up - want).max() < 1e-12
+    # pixel centres of the grid cells' own positions are reproduced exactly at f = 1
+    assert np.array_equal(cnn.bilinear(src, 1, hs, ws), src)
+
+
+def test_cnn_oracle_detail_label_recovers_a_known_slope():
+    from oracle import cnn
+    rng = np.random.default_rng(5)
+    hs, ws, f, s0, unit = 6, 8, 4, 0.037, 20.0
+    lab = rng.normal(size=(hs, ws))
+    zg = rng.normal(size=(hs, ws)) * 10
+    dz = rng.normal(size=(hs * f, ws * f))
+    z = cnn.bilinear(zg, f, hs * f, ws * f) + dz
+    full = cnn.bilinear(lab, f, hs * f, ws * f) + s0 * dz
+    ey0, ex0 = f, 2 * f                          # exact known on a sub-rectangle
+    exact = full[ey0:ey0 + 3 * f, ex0:ex0 + 4 * f]
+    ls, wt = cnn.detail_label(exact, ey0, ex0, lab, zg, f, z, unit)
+    inside = np.zeros((hs, ws), bool)
+    inside[1:4, 2:6] = True
+    assert np.allclose(ls[inside], unit * s0, rtol=1e-10)
+    assert np.isnan(ls[~inside]).all() and (wt[~inside] == 0).all()
+    cells = dz.reshape(hs, f, ws, f).transpose(0, 2, 1, 3)
+    assert np.allclose(wt[inside], (cells ** 2).sum(axis=(2, 3))[inside] / (f * f * unit ** 2))
+    # compose with the recovered slope reproduces the field exactly
+    out = np.stack([lab, np.where(inside, ls, 0.0)], axis=-1)
+    comp = cnn.compose(out, np.zeros((hs, ws)), zg, f, z=z, s_scale=1.0 / unit)
+    assert np.allclose(comp[ey0:ey0 + 3 * f, ex0:ex0 + 4 * f], exact, atol=1e-12)
 
 
 def test_cnn_oracle_conv_is_a_same_correlation():
@@ -504,5 +560,5 @@ def test_cnn_oracle_conv_is_a_same_correlation():
                         if 0 <= yy < 5 and 0 <= xc < 6:
                             s += float(np.dot(x[0, yy, xc], w[co, :, dy, dx]))
                 assert out[0, y, xx, co] == pytest.approx(s, abs=1e-12)
-    assert cnn.gelu(np.array([0.0]))[0] == 0.0
-    assert cnn.gelu(np.array([3.0]))[0] == pytest.approx(3.0 * 0.5 * (1 + math.erf(3 / math.sqrt(2))))
+    assert cnn.gelu_erf(np.array([3.0]))[0] == pytest.approx(
+        3.0 * 0.5 * (1 + math.erf(3 / math.sqrt(2))))
