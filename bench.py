@@ -357,9 +357,10 @@ def bench_c4(dev, n=65536, weights=None):
            "speedup_with_judge": round(t_exact / (t_sur + t_judge), 2),
            "raster_mae_deg_after_judge": round(mae, 4), "raster_p99_abs_err_deg": round(p99, 4),
            "judge": jd, "held_out": held,
-           "network": "6 conv3x3, 32 ch, GELU, bf16 on MFMA (fp32 acc), on the 4x grid; "
-                      "input: 4-level detrended elevation pyramid; residual over exact "
-                      "openness of the 16x grid"}
+           "network": f"6 conv3x3, 32 ch, GELU (tanh form), bf16 on MFMA (fp32 acc), one "
+                      f"fused launch on the {S.GRID}x grid; input: 4-level detrended elevation "
+                      "pyramid; outputs a residual over the exact openness of the 16x grid and "
+                      "the sensitivity to the pixel's own elevation, composed at full resolution"}
     del dem, ex, sur, err
     torch.cuda.empty_cache()
     return res

@@ -641,6 +641,9 @@ __device__ __forceinline__ float bilerp(const Bilerp& b, float v00, float v01, f
 //   out = bilerp(yf[.][0] + base) + s_scale * yf[cell][1] * (z - bilerp(zg))
 // (the second term only when cout >= 2 and z is given).  yf is [hs][ld][cout],
 // base and zg [hs][ld]; cell = (i / f, j / f).  Four output pixels per lane.
+// When f is a multiple of 8, the four pixels j0 .. j0 + 3 (j0 % 4 == 0) share
+// their bilinear columns and their cell, so the column pair is interpolated
+// along y once and each pixel costs one interpolation along x.
 __global__ void k_cnn_compose(const float* __restrict__ yf, int cout,
                               const float* __restrict__ base, const float* __restrict__ zg,
                               int64_t ldg, int hs, int ws, int f, const float* __restrict__ z,
@@ -649,20 +652,53 @@ __global__ void k_cnn_compose(const float* __restrict__ yf, int cout,
   const int j0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (j0 >= w) return;
   const float inv = 1.0f / (float)f;
+  const bool grouped = (f & 7) == 0;
   for (int i = blockIdx.y; i < h; i += gridDim.y) {
     float r[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int j = min(j0 + q, w - 1);
-      const Bilerp b = bilerp_at(i, j, inv, hs, ws, ldg);
-      float v = bilerp(b, yf[b.o00 * cout] + base[b.o00], yf[b.o01 * cout] + base[b.o01],
-                       yf[b.o10 * cout] + base[b.o10], yf[b.o11 * cout] + base[b.o11]);
+    if (grouped) {
+      const Bilerp b = bilerp_at(i, j0, inv, hs, ws, ldg);
+      const float a0 = (yf[b.o00 * cout] + base[b.o00]) * (1.0f - b.ty) +
+                       (yf[b.o10 * cout] + base[b.o10]) * b.ty;
+      const float a1 = (yf[b.o01 * cout] + base[b.o01]) * (1.0f - b.ty) +
+                       (yf[b.o11 * cout] + base[b.o11]) * b.ty;
+      float z0 = 0.0f, z1 = 0.0f, sc = 0.0f;
+      float zz[4] = {0.0f, 0.0f, 0.0f, 0.0f};
       if (z) {
-        const float zb = bilerp(b, zg[b.o00], zg[b.o01], zg[b.o10], zg[b.o11]);
-        const int64_t c = (int64_t)min(i / f, hs - 1) * ldg + min(j / f, ws - 1);
-        v += s_scale * yf[c * cout + 1] * (z[(int64_t)i * ldz + j] - zb);
+        z0 = zg[b.o00] * (1.0f - b.ty) + zg[b.o10] * b.ty;
+        z1 = zg[b.o01] * (1.0f - b.ty) + zg[b.o11] * b.ty;
+        const int64_t c = (int64_t)min(i / f, hs - 1) * ldg + min(j0 / f, ws - 1);
+        sc = s_scale * yf[c * cout + 1];
+        const float* zr = z + (int64_t)i * ldz + j0;
+        if (j0 + 3 < w && ((reinterpret_cast<uintptr_t>(zr) & 15) == 0)) {
+          const float4 v = *(const float4*)zr;
+          zz[0] = v.x; zz[1] = v.y; zz[2] = v.z; zz[3] = v.w;
+        } else {
+          for (int q = 0; q < 4; q++) zz[q] = zr[min(q, w - 1 - j0)];
+        }
       }
-      r[q] = v;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        float x = ((float)(j0 + q) + 0.5f) * inv - 0.5f;
+        x = fminf(fmaxf(x, 0.0f), (float)(ws - 1));
+        const float tx = x - floorf(x);
+        float v = a0 + tx * (a1 - a0);
+        if (z) v += sc * (zz[q] - (z0 + tx * (z1 - z0)));
+        r[q] = v;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = min(j0 + q, w - 1);
+        const Bilerp b = bilerp_at(i, j, inv, hs, ws, ldg);
+        float v = bilerp(b, yf[b.o00 * cout] + base[b.o00], yf[b.o01 * cout] + base[b.o01],
+                         yf[b.o10 * cout] + base[b.o10], yf[b.o11 * cout] + base[b.o11]);
+        if (z) {
+          const float zb = bilerp(b, zg[b.o00], zg[b.o01], zg[b.o10], zg[b.o11]);
+          const int64_t c = (int64_t)min(i / f, hs - 1) * ldg + min(j / f, ws - 1);
+          v += s_scale * yf[c * cout + 1] * (z[(int64_t)i * ldz + j] - zb);
+        }
+        r[q] = v;
+      }
     }
     float* o = out + (int64_t)i * ldo + j0;
     if (j0 + 3 < w && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {

@@ -103,6 +103,71 @@ k_openness_points(WinDevR W, const int2* __restrict__ pts, int n, OpenTab T,
   out[e] = isnan(z) ? __builtin_nanf("") : openness_value(T, openness_angle(W, q.x, q.y, z, T));
 }
 
+// Sampled exact check of an approximate openness-angle raster (the learned
+// surrogate's judge; no reference counterpart).  Tile t owns segments
+// [tile_start[t], tile_start[t + 1]) of seg_len pixels along one window row,
+// starting at (row, col0); one wave per segment, lane = pixel, so the ray
+// gathers coalesce as in the dense kernel.  Block (t, split) takes the tile's
+// segments split*4 + wave, + JUDGE_SPLIT*4, ... and writes the sum and count
+// of |exact - approx| over pixels where both are numbers; the reduction
+// order is fixed, so the result is deterministic.
+constexpr int JUDGE_SPLIT = 16;
+
+__global__ void __launch_bounds__(256)
+k_openness_judge(WinDevR W, OpenTab T, const float* __restrict__ approx, int64_t lda,
+                 const int2* __restrict__ seg, int seg_len, const int* __restrict__ tile_start,
+                 double* __restrict__ part) {
+  const int tile = blockIdx.x / JUDGE_SPLIT, split = blockIdx.x % JUDGE_SPLIT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+  float sum = 0.0f, cnt = 0.0f;
+  for (int s = s0 + split * 4 + wave; s < s1; s += JUDGE_SPLIT * 4) {
+    const int2 q = seg[s];
+    const int i = q.x, j = q.y + lane;
+    if (lane < seg_len && i >= 0 && i < W.h && j >= 0 && j < W.w) {
+      const float z = W.p[(int64_t)i * W.ld + j];
+      const float a = approx[(int64_t)i * lda + j];
+      if (!isnan(z) && !isnan(a)) {
+        const float v = openness_value(T, openness_angle(W, i, j, z, T));
+        sum += fabsf(v - a);
+        cnt += 1.0f;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_down(sum, o);
+    cnt += __shfl_down(cnt, o);
+  }
+  __shared__ double red[4][2];
+  if (lane == 0) {
+    red[wave][0] = sum;
+    red[wave][1] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ps = 0.0, pc = 0.0;
+    for (int w = 0; w < 4; w++) {
+      ps += red[w][0];
+      pc += red[w][1];
+    }
+    part[(int64_t)blockIdx.x * 2] = ps;
+    part[(int64_t)blockIdx.x * 2 + 1] = pc;
+  }
+}
+
+// mean absolute error per tile (NaN for a tile without a valid sample)
+__global__ void k_openness_judge_finish(const double* __restrict__ part, int n_tiles,
+                                        float* __restrict__ tile_mae) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  double s = 0.0, c = 0.0;
+  for (int k = 0; k < JUDGE_SPLIT; k++) {
+    s += part[((int64_t)t * JUDGE_SPLIT + k) * 2];
+    c += part[((int64_t)t * JUDGE_SPLIT + k) * 2 + 1];
+  }
+  tile_mae[t] = c > 0.0 ? (float)(s / c) : __builtin_nanf("");
+}
+
 constexpr int AO_MAX = 256;
 
 struct AoTab {
@@ -260,6 +325,39 @@ extern "C" int fsg_openness_points(const fsg_window* win, const int32_t* points,
   hipLaunchKernelGGL(k_openness_points, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
                      W, (const int2*)points, n, T, out);
   FSG_CHECK_LAUNCH("k_openness_points");
+  return FSG_OK;
+}
+
+extern "C" size_t fsg_openness_judge_workspace_size(int32_t n_tiles) {
+  return n_tiles > 0 ? sizeof(double) * 2 * JUDGE_SPLIT * (size_t)n_tiles : 0;
+}
+
+extern "C" int fsg_openness_judge(const fsg_window* win, const fsg_openness_params* p,
+                                  const float* approx, int64_t lda, const int32_t* segments,
+                                  int32_t seg_len, const int32_t* tile_start, int32_t n_tiles,
+                                  float* tile_mae, void* workspace, size_t workspace_bytes,
+                                  void* stream) {
+  int rc = validate_window(win, "fsg_openness_judge");
+  if (rc) return rc;
+  if (!approx || lda < win->w || !segments || !tile_start || !tile_mae || n_tiles <= 0)
+    return fail(FSG_E_INVALID, "fsg_openness_judge: bad arguments");
+  if (seg_len < 1 || seg_len > 64)
+    return fail(FSG_E_INVALID, "fsg_openness_judge: seg_len must be 1..64");
+  if (!workspace || workspace_bytes < fsg_openness_judge_workspace_size(n_tiles))
+    return fail(FSG_E_WORKSPACE, "fsg_openness_judge: workspace too small");
+  if ((int64_t)n_tiles * JUDGE_SPLIT > 0x7fffffff)
+    return fail(FSG_E_INVALID, "fsg_openness_judge: too many tiles");
+  OpenTab T;
+  if ((rc = openness_table(p, T, "fsg_openness_judge"))) return rc;
+  WinDevR W{win->data, win->ld, win->h, win->w};
+  hipStream_t s = as_stream(stream);
+  TimedScope ts("k_openness_judge", s);
+  hipLaunchKernelGGL(k_openness_judge, dim3(n_tiles * JUDGE_SPLIT), dim3(256), 0, s, W, T, approx,
+                     lda, (const int2*)segments, seg_len, tile_start, (double*)workspace);
+  FSG_CHECK_LAUNCH("k_openness_judge");
+  hipLaunchKernelGGL(k_openness_judge_finish, dim3((n_tiles + 255) / 256), dim3(256), 0, s,
+                     (const double*)workspace, n_tiles, tile_mae);
+  FSG_CHECK_LAUNCH("k_openness_judge_finish");
   return FSG_OK;
 }
 

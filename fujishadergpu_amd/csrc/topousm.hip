@@ -992,7 +992,7 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
 }
 
 extern "C" size_t fsg_topousm_coarse_means_workspace_size(const fsg_window* win, int32_t ds) {
-  if (!win || win->h <= 0 || win->w <= 0 || ds < 2) return 0;
+  if (!win || win->h <= 0 || win->w <= 0 || ds < 1) return 0;
   const size_t cells = (size_t)((win->h + ds - 1) / ds) * (size_t)((win->w + ds - 1) / ds);
   // decimated grid, flag, four void-fill temporaries (two reused by the box passes)
   return 5 * align256(sizeof(float) * cells) + align256(sizeof(int) * 4);
@@ -1005,7 +1005,7 @@ extern "C" int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32
   int rc = validate_window(win, "fsg_topousm_coarse_means");
   if (rc) return rc;
   if (win->h == 0 || win->w == 0) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means: empty");
-  if (ds < 2 || n_radii <= 0 || !radii_coarse || !outs)
+  if (ds < 1 || n_radii <= 0 || !radii_coarse || !outs)
     return fail(FSG_E_INVALID, "fsg_topousm_coarse_means: bad arguments");
   const int hc = (win->h + ds - 1) / ds, wc = (win->w + ds - 1) / ds;
   if (ld_out < wc) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means: ld_out < grid width");
@@ -1021,10 +1021,16 @@ extern "C" int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32
   for (int k = 0; k < 4; k++) tmp[k] = (float*)(ws + (1 + k) * g);
   int* flag = (int*)(ws + 5 * g);
   hipStream_t s = as_stream(stream);
-  rc = decimate(win, ds, grid, hc, wc, flag, tmp[0], tmp[1], tmp[2], tmp[3], false, s);
-  if (rc) return rc;
+  int64_t ldg = wc;
+  if (ds == 1) {  // the window is already the grid
+    grid = const_cast<float*>(win->data);
+    ldg = win->ld;
+  } else {
+    rc = decimate(win, ds, grid, hc, wc, flag, tmp[0], tmp[1], tmp[2], tmp[3], false, s);
+    if (rc) return rc;
+  }
   for (int i = 0; i < n_radii; i++) {
-    rc = coarse_mean(grid, wc, hc, wc, std::max(1, (int)radii_coarse[i]), outs[i], ld_out, tmp[0],
+    rc = coarse_mean(grid, ldg, hc, wc, std::max(1, (int)radii_coarse[i]), outs[i], ld_out, tmp[0],
                      tmp[1], s);
     if (rc) return rc;
   }

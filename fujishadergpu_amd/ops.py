@@ -168,6 +168,29 @@ def openness_points(win_t, points: torch.Tensor, *, table, openness_type: str,
     return out
 
 
+def openness_judge(win_t, approx: torch.Tensor, segments: torch.Tensor, seg_len: int,
+                   tile_start: torch.Tensor, *, table, openness_type: str = "positive",
+                   raw_degrees: bool = True) -> torch.Tensor:
+    """Per-tile mean |exact openness - approx| over sampled row segments
+    (segments int32 [n, 2] = row, col0; tile t owns segments
+    tile_start[t] .. tile_start[t + 1]); a float32 [n_tiles] tensor."""
+    if segments.dtype != torch.int32 or segments.dim() != 2 or segments.shape[1] != 2:
+        raise ValueError("segments must be an int32 [n, 2] tensor")
+    if tile_start.dtype != torch.int32:
+        raise ValueError("tile_start must be int32")
+    n_tiles = int(tile_start.numel()) - 1
+    out = torch.empty(n_tiles, dtype=torch.float32, device=win_t.device)
+    nbytes = _lib().fsg_openness_judge_workspace_size(n_tiles)
+    ws = torch.empty(max(1, nbytes), dtype=torch.uint8, device=win_t.device)
+    p, _arr = openness_params(table, openness_type, raw_degrees)
+    w = window(win_t)
+    N.check(_lib().fsg_openness_judge(C.byref(w), C.byref(p), approx.data_ptr(), approx.stride(0),
+                                      segments.contiguous().data_ptr(), int(seg_len),
+                                      tile_start.contiguous().data_ptr(), n_tiles,
+                                      out.data_ptr(), ws.data_ptr(), nbytes, _stream()))
+    return out
+
+
 def ambient_occlusion(win_t, reg, *, table, intensity: float, pad_d: int,
                       out: N.fsg_output) -> None:
     """table: list of (dx, dy, phys, dist_factor)|None in reference order."""

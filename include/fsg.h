@@ -204,6 +204,20 @@ int fsg_openness(const fsg_window* win, const fsg_region* reg,
 int fsg_openness_points(const fsg_window* win, const int32_t* points, int32_t n,
                         const fsg_openness_params* p, float* out, void* stream);
 
+/* No reference counterpart: the learned surrogate's judge.  For each of
+   n_tiles tiles, the mean |exact openness - approx| over its sampled
+   segments -- segments[k] = (row, col0) (device int32 [n][2]) covers seg_len
+   (1..64) pixels of one window row, tile t owns segments [tile_start[t],
+   tile_start[t + 1]) (device int32) -- into tile_mae[t] (device f32; NaN when
+   no pixel had both values).  approx is [h][lda] in window coordinates and is
+   compared with the openness of p (normally raw_degrees).  Workspace:
+   fsg_openness_judge_workspace_size(n_tiles) bytes. */
+size_t fsg_openness_judge_workspace_size(int32_t n_tiles);
+int fsg_openness_judge(const fsg_window* win, const fsg_openness_params* p, const float* approx,
+                       int64_t lda, const int32_t* segments, int32_t seg_len,
+                       const int32_t* tile_start, int32_t n_tiles, float* tile_mae,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
 /*
  * Ambient-occlusion sample table (_impl_ambient_occlusion.py:288-339): ring
  * r (0..3) sample i holds the np.round offsets, skip when both are 0, the
@@ -287,7 +301,8 @@ int fsg_topousm(const fsg_window* win, const fsg_region* reg,
  * (max(1, round(r / ds)); <= 1 selects the sigma=1 Gaussian) with 'reflect'
  * edges, as compute_topousm_fast_efficient_block does
  * (_impl_topousm_fast.py:72-91).  outs[i] are ceil(h/ds) x ceil(w/ds) grids
- * with leading dimension ld_out; ds is a power of two <= 16.
+ * with leading dimension ld_out; ds is a power of two <= 16, or 1 when the
+ * window already is the cell grid (no decimation: box means only).
  */
 size_t fsg_topousm_coarse_means_workspace_size(const fsg_window* win, int32_t ds);
 int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32_t n_radii,

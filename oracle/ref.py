@@ -466,8 +466,11 @@ def openness_ray_table(num_directions=16, max_distance=50, pixel_size=1.0,
 
 
 def openness(block, *, openness_type="positive", num_directions=16, max_distance=50,
-             pixel_size=1.0, pixel_scale_x=None, pixel_scale_y=None) -> np.ndarray:
-    """compute_openness_vectorized (_impl_openness.py:31-132)."""
+             pixel_size=1.0, pixel_scale_x=None, pixel_scale_y=None,
+             raw_degrees=False) -> np.ndarray:
+    """compute_openness_vectorized (_impl_openness.py:31-132).  With
+    raw_degrees, the mean angle op = angle_sum / max(dir_count, 1) of
+    _impl_openness.py:126 in degrees, before its / (pi/2), clip and gamma."""
     block = np.asarray(block, dtype=F32)
     h, w = block.shape
     nan_mask = np.isnan(block)
@@ -502,6 +505,9 @@ def openness(block, *, openness_type="positive", num_directions=16, max_distance
         angle_sum += np.where(dir_valid, dir_angle, F32(0.0))
         dir_count += dir_valid.astype(F32)
     op = angle_sum / np.maximum(dir_count, F32(1.0))
+    if raw_degrees:
+        deg = (op.astype(np.float64) * (180.0 / np.pi)).astype(F32)
+        return restore_nan(deg, nan_mask).astype(F32)
     op = np.clip(op / F32(np.pi / 2), F32(0), F32(1))
     res = np.power(op, F32(GAMMA))
     res = restore_nan(res.astype(F32), nan_mask)

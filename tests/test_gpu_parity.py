@@ -209,6 +209,51 @@ def test_openness_window(cuda, otype, nan_discs):
 
 
 @pytest.mark.parametrize("nan_discs", [0, 6])
+def test_openness_raw_degrees_points_and_judge(cuda, nan_discs):
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.algorithms import openness_table
+    dem = _dem(150, 170, 23, relief=2500.0, hurst=0.8, nan_discs=nan_discs)
+    g = dem.to(cuda)
+    kw = dict(openness_type="positive", num_directions=8, max_distance=48, pixel_size=10.0)
+    want = ref.openness(dem.numpy(), raw_degrees=True, **kw)
+    tab = openness_table(8, 48, 10.0, None, None)
+    out = torch.empty((150, 170), dtype=torch.float32, device=cuda)
+    ops.openness(g, ops.full_region(g), table=tab, openness_type="positive",
+                 out=ops.output(out), raw_degrees=True)
+    got = out.cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    m = ~np.isnan(want)
+    assert np.abs(got[m] - want[m]).max() < 2e-4        # degrees
+    # points: the same function at listed pixels, bit for bit
+    rng = np.random.default_rng(1)
+    pts = np.stack([rng.integers(0, 150, 500), rng.integers(0, 170, 500)], 1).astype(np.int32)
+    pv = ops.openness_points(g, torch.from_numpy(pts).to(cuda), table=tab,
+                             openness_type="positive").cpu().numpy()
+    assert np.array_equal(pv, got[pts[:, 0], pts[:, 1]], equal_nan=True)
+    # judge: per-tile mean |exact - approx| over row segments
+    approx = want + rng.normal(0, 2.0, want.shape).astype(np.float32)
+    approx[5, :] = np.nan
+    segs, starts = [], [0]
+    tiles = [(0, 0, 80, 90), (0, 90, 80, 80), (80, 0, 70, 170)]
+    L = 40
+    for (y0, x0, th, tw) in tiles:
+        n = 37
+        segs += [(int(rng.integers(y0, y0 + th)), int(rng.integers(x0, x0 + tw - L + 1)))
+                 for _ in range(n)]
+        starts.append(starts[-1] + n)
+    seg_t = torch.tensor(segs, dtype=torch.int32, device=cuda)
+    st_t = torch.tensor(starts, dtype=torch.int32, device=cuda)
+    mae = ops.openness_judge(g, torch.from_numpy(approx).to(cuda), seg_t, L, st_t,
+                             table=tab).cpu().numpy()
+    for t_i in range(len(tiles)):
+        e = []
+        for (r_, c_) in segs[starts[t_i]:starts[t_i + 1]]:
+            d = np.abs(want[r_, c_:c_ + L].astype(np.float64) - approx[r_, c_:c_ + L])
+            e += list(d[~np.isnan(d)])
+        assert mae[t_i] == pytest.approx(np.mean(e), rel=1e-4, abs=3e-4), t_i
+
+
+@pytest.mark.parametrize("nan_discs", [0, 6])
 def test_ambient_occlusion_window(cuda, nan_discs):
     from fujishadergpu_amd.algorithms import ALGORITHMS
     dem = _dem(120, 140, 19, nan_discs=nan_discs)

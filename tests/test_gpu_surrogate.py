@@ -108,10 +108,11 @@ def test_backward_matches_oracle_gradients(cuda):
             assert cos > 0.999 and rel < 0.05, f"layer {li} d{name}: cos {cos:.5f} rel {rel:.3f}"
 
 
-def test_compose_and_detail_label_match_the_oracle(cuda):
+@pytest.mark.parametrize("f", [8, 4, 16])
+def test_compose_and_detail_label_match_the_oracle(cuda, f):
     from fujishadergpu_amd import _native as N
     rng = np.random.default_rng(4)
-    hs, ws, f, unit = 9, 13, 8, 20.0
+    hs, ws, unit = 9, 13, 20.0
     H, W = hs * f - 5, ws * f - 3                # ragged: not a multiple of the grid
     yf = rng.normal(size=(hs, ws, 2))
     base = rng.normal(size=(hs, ws))

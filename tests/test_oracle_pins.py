@@ -342,6 +342,19 @@ def test_openness_at_a_cone_apex(k):
         ang ** G, abs=2e-6)
 
 
+@pytest.mark.parametrize("k", [1.0, 0.25])
+def test_openness_raw_degrees_at_a_cone_apex_and_against_the_display_value(k):
+    r = _radial(81)
+    pit = (F32(k) * r).astype(F32)
+    kw = dict(num_directions=8, max_distance=30, openness_type="positive")
+    raw = ref.openness(pit, raw_degrees=True, **kw)
+    assert raw[40, 40] == pytest.approx(90.0 - math.degrees(math.atan(k)), abs=2e-4)
+    # the display value is clip(raw / 90, 0, 1) ** (1 / 2.2) at every pixel
+    disp = ref.openness(pit, **kw)
+    want = np.clip(raw.astype(np.float64) / 90.0, 0, 1) ** G
+    assert np.abs(disp - want).max() < 2e-6
+
+
 def test_openness_uses_physical_distance():
     # pixel size 2 halves every elevation angle's tangent on the same cone
     r = _radial(81)
