@@ -147,9 +147,10 @@ _SIGS = {
                                            P(vp), C.c_int64, vp, C.c_size_t, vp]),
     "fsg_openness_points": (C.c_int, [P(fsg_window), vp, C.c_int32, P(fsg_openness_params), vp,
                                       vp]),
-    "fsg_openness_judge_workspace_size": (C.c_size_t, [C.c_int32]),
-    "fsg_openness_judge": (C.c_int, [P(fsg_window), P(fsg_openness_params), vp, C.c_int64, vp,
-                                     C.c_int32, vp, C.c_int32, vp, vp, C.c_size_t, vp]),
+    "fsg_openness_judge_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32, C.c_int32]),
+    "fsg_openness_judge": (C.c_int, [P(fsg_window), P(fsg_openness_params), vp, C.c_int64,
+                                     C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_int32,
+                                     vp, vp, C.c_size_t, vp]),
     "fsg_cnn_conv3x3": (C.c_int, [P(fsg_cnn_conv), vp]),
     "fsg_cnn_conv3x3_wgrad": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp,
                                         vp, vp]),

@@ -241,6 +241,46 @@ k_block_mean_p2(const float* __restrict__ src, int64_t ld, int h, int w, int f,
   }
 }
 
+// Power-of-two factor >= 4 on 16-byte aligned rows: each lane loads a float4
+// (four columns of one cell) from each of the f rows -- f independent 16-byte
+// loads in flight -- and groups of f/4 lanes reduce by shuffles.
+__global__ void __launch_bounds__(256)
+k_block_mean_p2v(const float* __restrict__ src, int64_t ld, int h, int w, int f,
+                 float* __restrict__ out, int64_t ldo, int oh, int ow) {
+  const int orow = blockIdx.y;
+  const int c = (blockIdx.x * 256 + threadIdx.x) * 4;
+  const int r0 = orow * f, r1 = min(r0 + f, h);
+  double tot = 0.0;
+  int cnt = 0;
+  if (c < w) {
+    for (int r = r0; r < r1; r++) {
+      const float* p = src + (int64_t)r * ld + c;
+      float v[4];
+      if (c + 3 < w) {
+        const float4 q = *(const float4*)p;
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+      } else {
+        for (int k = 0; k < 4; k++) v[k] = c + k < w ? p[k] : __builtin_nanf("");
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (isfinite(v[k])) {
+          tot += (double)v[k];
+          cnt++;
+        }
+    }
+  }
+  for (int off = 1; off < f / 4; off <<= 1) {
+    tot += __shfl_down(tot, off, 64);
+    cnt += __shfl_down(cnt, off, 64);
+  }
+  if ((threadIdx.x & (f / 4 - 1)) == 0) {
+    const int oc = c / f;
+    if (oc < ow && orow < oh)
+      out[(int64_t)orow * ldo + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
+  }
+}
+
 // -------------------------------------------------------------------- API
 
 extern "C" int fsg_abi_version(void) { return FSG_ABI_VERSION; }
@@ -359,6 +399,14 @@ extern "C" int fsg_block_mean(const fsg_window* src, int32_t factor, float* out,
   const int ow = std::max(1, (src->w + factor - 1) / factor);
   if (src->h == 0 || src->w == 0) return fail(FSG_E_INVALID, "fsg_block_mean: empty source");
   if (!out || ld_out < ow) return fail(FSG_E_INVALID, "fsg_block_mean: bad output");
+  if (factor >= 4 && factor <= 256 && (factor & (factor - 1)) == 0 && src->ld % 4 == 0 &&
+      (reinterpret_cast<uintptr_t>(src->data) & 15) == 0) {
+    dim3 blk(256), grd((src->w + 1023) / 1024, oh);
+    hipLaunchKernelGGL(k_block_mean_p2v, grd, blk, 0, as_stream(stream), src->data, src->ld,
+                       src->h, src->w, factor, out, ld_out, oh, ow);
+    FSG_CHECK_LAUNCH("k_block_mean_p2v");
+    return FSG_OK;
+  }
   if (factor <= 64 && (factor & (factor - 1)) == 0) {
     dim3 blk(256), grd((src->w + 255) / 256, oh);
     hipLaunchKernelGGL(k_block_mean_p2, grd, blk, 0, as_stream(stream), src->data, src->ld,

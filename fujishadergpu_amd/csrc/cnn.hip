@@ -33,6 +33,7 @@ __device__ __forceinline__ float bf2f(u16 v) { return __uint_as_float((unsigned)
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16, round to nearest even: gfx950's v_cvt_pk_bf16_f32
 __device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
@@ -101,7 +102,9 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
     const int b = row / A.H, y = row - b * A.H;
     const int x0 = tx * 32;
     const int px = x0 + n;           // this lane's pixel (B column)
-    f32x16 acc = {};
+    f32x16 acc;   // starts at the bias: z = bias + sum_k w x
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = bias[e];
     for (int ks = 0; ks < ksteps; ks++) {
       const int k = ks * 16 + half * 8;
       const int tap = k / A.cin, ci = k - tap * A.cin;
@@ -124,14 +127,14 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
       if (EPI == EPI_GELU) {
         float z[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) z[j] = acc[4 * i + j] + bias[4 * i + j];
+        for (int j = 0; j < 4; j++) z[j] = acc[4 * i + j];
         if (A.pre_out) *(float4*)(A.pre_out + p * CNN_C + c0) = make_float4(z[0], z[1], z[2], z[3]);
         *(uint2*)(A.y + p * CNN_C + c0) =
             make_uint2(pk_bf16(gelu(z[0]), gelu(z[1])), pk_bf16(gelu(z[2]), gelu(z[3])));
       } else if (EPI == EPI_LINEAR) {
 #pragma unroll
         for (int j = 0; j < 4; j++)
-          if (c0 + j < A.cout) A.yf[p * A.cout + c0 + j] = acc[4 * i + j] + bias[4 * i + j];
+          if (c0 + j < A.cout) A.yf[p * A.cout + c0 + j] = acc[4 * i + j];
       } else {
         const float4 pz = *(const float4*)(A.pre_in + p * CNN_C + c0);
         *(uint2*)(A.y + p * CNN_C + c0) =
@@ -417,11 +420,12 @@ __device__ __forceinline__ int fz_slot(int r, int y0) { return (r - y0 + 8) & 3;
 
 // One 32-channel 3x3 layer over the three tiles of row r (MFMA accumulators
 // per tile, ds_read_b128 operands).  Returns the accumulators of tile nt.
-__device__ __forceinline__ f32x16 fz_mma32(const char* rin, const bf16x8 (&w)[18], int r, int y0,
-                                           int nt, int n, int half) {
+__device__ __forceinline__ f32x16 fz_mma32(const char* rin, const bf16x8 (&w)[18],
+                                           const f32x16& init, int r, int y0, int nt, int n,
+                                           int half) {
   const char* rows[3] = {rin + fz_slot(r - 1, y0) * FZ_ROW32, rin + fz_slot(r, y0) * FZ_ROW32,
                          rin + fz_slot(r + 1, y0) * FZ_ROW32};
-  f32x16 acc = {};
+  f32x16 acc = init;
 #pragma unroll
   for (int ks = 0; ks < 18; ks++) {
     const int tap = ks >> 1, dy = tap / 3, dx = tap % 3;
@@ -433,27 +437,28 @@ __device__ __forceinline__ f32x16 fz_mma32(const char* rin, const bf16x8 (&w)[18
 }
 
 // GELU of accumulator values e, e + 1 (channels of one 4-group) as a packed pair
-__device__ __forceinline__ unsigned fz_gelu2(const f32x16& acc, const float (&bias)[16], int e) {
-  return pk_bf16(gelu(acc[e] + bias[e]), gelu(acc[e + 1] + bias[e + 1]));
+__device__ __forceinline__ unsigned fz_gelu2(const f32x16& acc, int e) {
+  return pk_bf16(gelu(acc[e]), gelu(acc[e + 1]));
 }
 
-// GELU epilogue into a 32-channel ring row (zeros outside the image).
-__device__ __forceinline__ void fz_store_gelu(char* dst, const f32x16& acc, const float (&bias)[16],
-                                              bool ok, int pc, int half) {
+// GELU epilogue into a 32-channel ring row (zeros outside the image); the
+// accumulators already hold bias + sum.
+__device__ __forceinline__ void fz_store_gelu(char* dst, const f32x16& acc, bool ok, int pc,
+                                              int half) {
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     uint2 v = make_uint2(0, 0);
-    if (ok) v = make_uint2(fz_gelu2(acc, bias, 4 * i), fz_gelu2(acc, bias, 4 * i + 2));
+    if (ok) v = make_uint2(fz_gelu2(acc, 4 * i), fz_gelu2(acc, 4 * i + 2));
     *(uint2*)(dst + fz_off32(pc, i) + half * 8) = v;
   }
 }
 
 // Layer 0 (8 -> 32) on tile nt of row r: cin 8 puts one tap in each half-wave.
 __device__ __forceinline__ void fz_layer0(const char* lds, const bf16x8 (&w0)[5],
-                                          const float (&bias)[16], int r, int y0, int H, int W,
+                                          const f32x16& bias, int r, int y0, int H, int W,
                                           int xs, int nt, int n, int half) {
   const bool row_ok = r >= 0 && r < H;
-  f32x16 acc = {};
+  f32x16 acc = bias;
   if (row_ok) {
 #pragma unroll
     for (int ks = 0; ks < 5; ks++) {
@@ -467,24 +472,24 @@ __device__ __forceinline__ void fz_layer0(const char* lds, const bf16x8 (&w0)[5]
     }
   }
   const int x = xs + nt * 32 + n;
-  fz_store_gelu((char*)lds + FZ_RING0 + fz_slot(r, y0) * FZ_ROW32, acc, bias,
+  fz_store_gelu((char*)lds + FZ_RING0 + fz_slot(r, y0) * FZ_ROW32, acc,
                 row_ok && x >= 0 && x < W, nt * 32 + n + 1, half);
 }
 
 // Layer 5 (32 -> cout) on tile nt of output row r, stored to global memory.
 __device__ __forceinline__ void fz_layer5(const FusedArgs& A, const char* lds,
-                                          const bf16x8 (&w)[18], const float (&bias)[16], int r,
+                                          const bf16x8 (&w)[18], const f32x16& bias, int r,
                                           int y0, int b, int xs, int nt, int n, int half) {
-  const f32x16 acc = fz_mma32(lds + FZ_RING0 + 4 * FZ_RING32, w, r, y0, nt, n, half);
+  const f32x16 acc = fz_mma32(lds + FZ_RING0 + 4 * FZ_RING32, w, bias, r, y0, nt, n, half);
   const int c = nt * 32 + n, x = xs + c;
   if (half == 0 && c >= 6 && c < 6 + FZ_SW && x < A.W) {
     float* o = A.yf + (((int64_t)b * A.H + r) * A.W + x) * A.cout;
     if (A.cout == 2) {
-      *(float2*)o = make_float2(acc[0] + bias[0], acc[1] + bias[1]);
+      *(float2*)o = make_float2(acc[0], acc[1]);
     } else {
 #pragma unroll
       for (int j = 0; j < 4; j++)
-        if (j < A.cout) o[j] = acc[j] + bias[j];
+        if (j < A.cout) o[j] = acc[j];
     }
   }
 }
@@ -518,7 +523,7 @@ __global__ void __launch_bounds__(512) k_cnn_fused(FusedArgs A) {
   // weights: w = layer wave + 1 (waves 0..3) or layer 5 (waves 5..7); w0 = layer 0
   const int big = wave < 4 ? wave + 1 : 5;
   bf16x8 w[18], w0[5];
-  float bias[16], bias0[16];
+  f32x16 bias, bias0;
 #pragma unroll
   for (int ks = 0; ks < 18; ks++)
     w[ks] = __builtin_bit_cast(bf16x8, *(const uint4*)(A.w[big] + n * 288 + ks * 16 + half * 8));
@@ -547,21 +552,20 @@ __global__ void __launch_bounds__(512) k_cnn_fused(FusedArgs A) {
         const char* rin = fz_lds + FZ_RING0 + (l - 1) * FZ_RING32;
         char* dst = fz_lds + FZ_RING0 + l * FZ_RING32 + fz_slot(r, y0) * FZ_ROW32;
         if (!row_ok) {
-          const f32x16 zero = {};
           for (int nt = 0; nt < FZ_TILES; nt++)
-            fz_store_gelu(dst, zero, bias, false, nt * 32 + n + 1, half);
+            fz_store_gelu(dst, bias, false, nt * 32 + n + 1, half);
         } else {
           const char* rows[3] = {rin + fz_slot(r - 1, y0) * FZ_ROW32,
                                  rin + fz_slot(r, y0) * FZ_ROW32,
                                  rin + fz_slot(r + 1, y0) * FZ_ROW32};
-          f32x16 prev = fz_mma32(rin, w, r, y0, 0, n, half);
+          f32x16 prev = fz_mma32(rin, w, bias, r, y0, 0, n, half);
 #pragma unroll
           for (int nt = 1; nt <= FZ_TILES; nt++) {
             // MFMAs of tile nt with the epilogue of tile nt - 1 in between
             const int pp = (nt - 1) * 32 + n + 1;
             const int xp = xs + (nt - 1) * 32 + n;
             const bool okp = xp >= 0 && xp < A.W;
-            f32x16 acc = {};
+            f32x16 acc = bias;
             unsigned pk[8];
 #pragma unroll
             for (int ks = 0; ks < 18; ks++) {
@@ -571,7 +575,7 @@ __global__ void __launch_bounds__(512) k_cnn_fused(FusedArgs A) {
                                                    fz_off32(nt * 32 + n + dx, (ks & 1) * 2 + half));
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[ks], bv, acc, 0, 0, 0);
               }
-              if (ks < 8) pk[ks] = okp ? fz_gelu2(prev, bias, 2 * ks) : 0u;
+              if (ks < 8) pk[ks] = okp ? fz_gelu2(prev, 2 * ks) : 0u;
             }
 #pragma unroll
             for (int i = 0; i < 4; i++)
@@ -670,8 +674,9 @@ __global__ void k_cnn_compose(const float* __restrict__ yf, int cout,
         sc = s_scale * yf[c * cout + 1];
         const float* zr = z + (int64_t)i * ldz + j0;
         if (j0 + 3 < w && ((reinterpret_cast<uintptr_t>(zr) & 15) == 0)) {
-          const float4 v = *(const float4*)zr;
-          zz[0] = v.x; zz[1] = v.y; zz[2] = v.z; zz[3] = v.w;
+          // streamed once: no reuse, keep it out of the caches' way
+          const f32x4 v = __builtin_nontemporal_load((const f32x4*)zr);
+          zz[0] = v[0]; zz[1] = v[1]; zz[2] = v[2]; zz[3] = v[3];
         } else {
           for (int q = 0; q < 4; q++) zz[q] = zr[min(q, w - 1 - j0)];
         }
@@ -702,7 +707,8 @@ __global__ void k_cnn_compose(const float* __restrict__ yf, int cout,
     }
     float* o = out + (int64_t)i * ldo + j0;
     if (j0 + 3 < w && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
-      *(float4*)o = make_float4(r[0], r[1], r[2], r[3]);
+      const f32x4 v = {r[0], r[1], r[2], r[3]};
+      __builtin_nontemporal_store(v, (f32x4*)o);
     } else {
       for (int q = 0; q < 4 && j0 + q < w; q++) o[q] = r[q];
     }

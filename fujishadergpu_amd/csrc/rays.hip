@@ -104,27 +104,51 @@ k_openness_points(WinDevR W, const int2* __restrict__ pts, int n, OpenTab T,
 }
 
 // Sampled exact check of an approximate openness-angle raster (the learned
-// surrogate's judge; no reference counterpart).  Tile t owns segments
-// [tile_start[t], tile_start[t + 1]) of seg_len pixels along one window row,
-// starting at (row, col0); one wave per segment, lane = pixel, so the ray
-// gathers coalesce as in the dense kernel.  Block (t, split) takes the tile's
-// segments split*4 + wave, + JUDGE_SPLIT*4, ... and writes the sum and count
-// of |exact - approx| over pixels where both are numbers; the reduction
-// order is fixed, so the result is deterministic.
+// surrogate's judge; no reference counterpart).  The window is cut into
+// tile_h x tile_w tiles (ragged at the far edges); tile t (row-major) gets
+// n_t = max(1, ceil(frac * th * tw / L)) segments of L pixels along one row,
+// segment k at
+//   h = splitmix64(splitmix64(seed ^ splitmix64(t)) + k),
+//   row = y0 + (h >> 32) % th,  col0 = x0 + (h & 0xffffffff) % (tw - L + 1).
+// One wave per segment, lane = pixel, so the ray gathers coalesce as in the
+// dense kernel.  Block (t, split) takes segments split*4 + wave, +
+// JUDGE_SPLIT*4, ... and writes the sum and count of |exact - approx| over
+// pixels where both are numbers; the reduction order is fixed, so the result
+// is deterministic.
 constexpr int JUDGE_SPLIT = 16;
 
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+struct JudgeGeo {
+  int tile_h, tile_w, n_tx, L;
+  double frac;
+  uint64_t seed;
+};
+
+__device__ __forceinline__ int judge_count(const JudgeGeo& G, int th, int tw) {
+  return max(1, (int)ceil(G.frac * (double)th * (double)tw / (double)G.L));
+}
+
 __global__ void __launch_bounds__(256)
-k_openness_judge(WinDevR W, OpenTab T, const float* __restrict__ approx, int64_t lda,
-                 const int2* __restrict__ seg, int seg_len, const int* __restrict__ tile_start,
+k_openness_judge(WinDevR W, OpenTab T, const float* __restrict__ approx, int64_t lda, JudgeGeo G,
                  double* __restrict__ part) {
   const int tile = blockIdx.x / JUDGE_SPLIT, split = blockIdx.x % JUDGE_SPLIT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+  const int y0 = (tile / G.n_tx) * G.tile_h, x0 = (tile % G.n_tx) * G.tile_w;
+  const int th = min(G.tile_h, W.h - y0), tw = min(G.tile_w, W.w - x0);
+  const int n_seg = judge_count(G, th, tw);
+  const uint64_t key = splitmix64(G.seed ^ splitmix64((uint64_t)tile));
   float sum = 0.0f, cnt = 0.0f;
-  for (int s = s0 + split * 4 + wave; s < s1; s += JUDGE_SPLIT * 4) {
-    const int2 q = seg[s];
-    const int i = q.x, j = q.y + lane;
-    if (lane < seg_len && i >= 0 && i < W.h && j >= 0 && j < W.w) {
+  for (int s = split * 4 + wave; s < n_seg; s += JUDGE_SPLIT * 4) {
+    const uint64_t h = splitmix64(key + (uint64_t)s);
+    const int i = y0 + (int)((h >> 32) % (uint64_t)th);
+    const int j = x0 + (int)((h & 0xffffffffull) % (uint64_t)(tw - G.L + 1)) + lane;
+    if (lane < G.L) {
       const float z = W.p[(int64_t)i * W.ld + j];
       const float a = approx[(int64_t)i * lda + j];
       if (!isnan(z) && !isnan(a)) {
@@ -328,35 +352,43 @@ extern "C" int fsg_openness_points(const fsg_window* win, const int32_t* points,
   return FSG_OK;
 }
 
-extern "C" size_t fsg_openness_judge_workspace_size(int32_t n_tiles) {
-  return n_tiles > 0 ? sizeof(double) * 2 * JUDGE_SPLIT * (size_t)n_tiles : 0;
+extern "C" size_t fsg_openness_judge_workspace_size(const fsg_window* win, int32_t tile_h,
+                                                    int32_t tile_w) {
+  if (!win || win->h <= 0 || win->w <= 0 || tile_h <= 0 || tile_w <= 0) return 0;
+  const size_t n_tiles = (size_t)((win->h + tile_h - 1) / tile_h) * ((win->w + tile_w - 1) / tile_w);
+  return sizeof(double) * 2 * JUDGE_SPLIT * n_tiles;
 }
 
 extern "C" int fsg_openness_judge(const fsg_window* win, const fsg_openness_params* p,
-                                  const float* approx, int64_t lda, const int32_t* segments,
-                                  int32_t seg_len, const int32_t* tile_start, int32_t n_tiles,
-                                  float* tile_mae, void* workspace, size_t workspace_bytes,
-                                  void* stream) {
+                                  const float* approx, int64_t lda, int32_t tile_h,
+                                  int32_t tile_w, double sample_frac, uint64_t seed,
+                                  int32_t seg_len, float* tile_mae, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
   int rc = validate_window(win, "fsg_openness_judge");
   if (rc) return rc;
-  if (!approx || lda < win->w || !segments || !tile_start || !tile_mae || n_tiles <= 0)
+  if (win->h == 0 || win->w == 0 || !approx || lda < win->w || !tile_mae || tile_h <= 0 ||
+      tile_w <= 0 || !(sample_frac > 0.0) || sample_frac > 1.0)
     return fail(FSG_E_INVALID, "fsg_openness_judge: bad arguments");
-  if (seg_len < 1 || seg_len > 64)
-    return fail(FSG_E_INVALID, "fsg_openness_judge: seg_len must be 1..64");
-  if (!workspace || workspace_bytes < fsg_openness_judge_workspace_size(n_tiles))
-    return fail(FSG_E_WORKSPACE, "fsg_openness_judge: workspace too small");
-  if ((int64_t)n_tiles * JUDGE_SPLIT > 0x7fffffff)
+  const int n_ty = (win->h + tile_h - 1) / tile_h, n_tx = (win->w + tile_w - 1) / tile_w;
+  const int min_tw = std::min(tile_w, win->w - (n_tx - 1) * tile_w);
+  if (seg_len < 1 || seg_len > 64 || seg_len > min_tw)
+    return fail(FSG_E_INVALID, "fsg_openness_judge: seg_len must be 1..64 and fit every tile");
+  const int64_t n_tiles = (int64_t)n_ty * n_tx;
+  if (n_tiles * JUDGE_SPLIT > 0x7fffffff)
     return fail(FSG_E_INVALID, "fsg_openness_judge: too many tiles");
+  if (!workspace || workspace_bytes < fsg_openness_judge_workspace_size(win, tile_h, tile_w))
+    return fail(FSG_E_WORKSPACE, "fsg_openness_judge: workspace too small");
   OpenTab T;
   if ((rc = openness_table(p, T, "fsg_openness_judge"))) return rc;
   WinDevR W{win->data, win->ld, win->h, win->w};
+  JudgeGeo G{tile_h, tile_w, n_tx, seg_len, sample_frac, seed};
   hipStream_t s = as_stream(stream);
   TimedScope ts("k_openness_judge", s);
-  hipLaunchKernelGGL(k_openness_judge, dim3(n_tiles * JUDGE_SPLIT), dim3(256), 0, s, W, T, approx,
-                     lda, (const int2*)segments, seg_len, tile_start, (double*)workspace);
+  hipLaunchKernelGGL(k_openness_judge, dim3((unsigned)(n_tiles * JUDGE_SPLIT)), dim3(256), 0, s,
+                     W, T, approx, lda, G, (double*)workspace);
   FSG_CHECK_LAUNCH("k_openness_judge");
-  hipLaunchKernelGGL(k_openness_judge_finish, dim3((n_tiles + 255) / 256), dim3(256), 0, s,
-                     (const double*)workspace, n_tiles, tile_mae);
+  hipLaunchKernelGGL(k_openness_judge_finish, dim3((unsigned)((n_tiles + 255) / 256)), dim3(256),
+                     0, s, (const double*)workspace, (int)n_tiles, tile_mae);
   FSG_CHECK_LAUNCH("k_openness_judge_finish");
   return FSG_OK;
 }

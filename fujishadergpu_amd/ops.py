@@ -168,25 +168,49 @@ def openness_points(win_t, points: torch.Tensor, *, table, openness_type: str,
     return out
 
 
-def openness_judge(win_t, approx: torch.Tensor, segments: torch.Tensor, seg_len: int,
-                   tile_start: torch.Tensor, *, table, openness_type: str = "positive",
+def splitmix64(x: int) -> int:
+    """splitmix64 (Steele, Lea & Flood 2014) on Python ints, 64-bit wrap."""
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+def judge_segments(h: int, w: int, tile_h: int, tile_w: int, sample_frac: float, seed: int,
+                   seg_len: int):
+    """The judge's sampled segments, tile by tile: [((y0, x0, th, tw), [(row, col0), ...])]
+    (the positions fsg_openness_judge draws)."""
+    import math
+    out = []
+    n_tx = -(-w // tile_w)
+    for t in range(-(-h // tile_h) * n_tx):
+        y0, x0 = (t // n_tx) * tile_h, (t % n_tx) * tile_w
+        th, tw = min(tile_h, h - y0), min(tile_w, w - x0)
+        n = max(1, math.ceil(sample_frac * th * tw / seg_len))
+        key = splitmix64(seed ^ splitmix64(t))
+        segs = []
+        for k in range(n):
+            hh = splitmix64((key + k) & ((1 << 64) - 1))
+            segs.append((y0 + (hh >> 32) % th, x0 + (hh & 0xFFFFFFFF) % (tw - seg_len + 1)))
+        out.append(((y0, x0, th, tw), segs))
+    return out
+
+
+def openness_judge(win_t, approx: torch.Tensor, *, tile_h: int, tile_w: int, sample_frac: float,
+                   seed: int, seg_len: int, table, openness_type: str = "positive",
                    raw_degrees: bool = True) -> torch.Tensor:
-    """Per-tile mean |exact openness - approx| over sampled row segments
-    (segments int32 [n, 2] = row, col0; tile t owns segments
-    tile_start[t] .. tile_start[t + 1]); a float32 [n_tiles] tensor."""
-    if segments.dtype != torch.int32 or segments.dim() != 2 or segments.shape[1] != 2:
-        raise ValueError("segments must be an int32 [n, 2] tensor")
-    if tile_start.dtype != torch.int32:
-        raise ValueError("tile_start must be int32")
-    n_tiles = int(tile_start.numel()) - 1
+    """Per-tile mean |exact openness - approx| over sampled row segments (a
+    float32 [n_tiles] tensor, tiles row-major)."""
+    h, w = win_t.shape
+    n_tiles = -(-h // tile_h) * -(-w // tile_w)
     out = torch.empty(n_tiles, dtype=torch.float32, device=win_t.device)
-    nbytes = _lib().fsg_openness_judge_workspace_size(n_tiles)
+    wd = window(win_t)
+    nbytes = _lib().fsg_openness_judge_workspace_size(C.byref(wd), tile_h, tile_w)
     ws = torch.empty(max(1, nbytes), dtype=torch.uint8, device=win_t.device)
     p, _arr = openness_params(table, openness_type, raw_degrees)
-    w = window(win_t)
-    N.check(_lib().fsg_openness_judge(C.byref(w), C.byref(p), approx.data_ptr(), approx.stride(0),
-                                      segments.contiguous().data_ptr(), int(seg_len),
-                                      tile_start.contiguous().data_ptr(), n_tiles,
+    N.check(_lib().fsg_openness_judge(C.byref(wd), C.byref(p), approx.data_ptr(), approx.stride(0),
+                                      tile_h, tile_w, float(sample_frac), int(seed), int(seg_len),
                                       out.data_ptr(), ws.data_ptr(), nbytes, _stream()))
     return out
 

@@ -449,30 +449,19 @@ SEG = 64   # judge segment: 64 consecutive pixels of one row (one wave)
 def judge(win: torch.Tensor, approx: torch.Tensor, radius: int, n_dirs: int, pixel_size: float,
           tile: int = 4096, threshold: float = 1.5, sample_frac: float = 0.01,
           seed: int = 11) -> Dict[str, float]:
-    """Exact openness at 1% of each tile's pixels -- random 64-pixel row
-    segments, so the exact kernel's gathers coalesce -- and every tile whose
-    mean absolute error on them exceeds the threshold recomputed exactly in
-    place."""
+    """Exact openness at 1% of each tile's pixels -- 64-pixel row segments
+    drawn on the device by a counter-based hash, so the exact kernel's gathers
+    coalesce -- and every tile whose mean absolute error on them exceeds the
+    threshold recomputed exactly in place."""
     H, W = win.shape
     tab = openness_table(n_dirs, radius, pixel_size, None, None)
-    g = torch.Generator(device=win.device).manual_seed(seed)
-    tiles, segs, starts = [], [], [0]
-    for y0 in range(0, H, tile):
-        for x0 in range(0, W, tile):
-            th, tw = min(tile, H - y0), min(tile, W - x0)
-            L = min(SEG, tw)
-            n = max(1, -(-int(sample_frac * th * tw) // L))
-            ry = torch.randint(y0, y0 + th, (n,), generator=g, device=win.device,
-                               dtype=torch.int32)
-            rx = torch.randint(x0, x0 + tw - L + 1, (n,), generator=g, device=win.device,
-                               dtype=torch.int32)
-            segs.append(torch.stack([ry, rx], dim=1))
-            starts.append(starts[-1] + n)
-            tiles.append((y0, x0, th, tw))
-    seg_len = min(SEG, W)
-    segments = torch.cat(segs)
-    tile_start = torch.tensor(starts, dtype=torch.int32, device=win.device)
-    mae = ops.openness_judge(win, approx, segments, seg_len, tile_start, table=tab).cpu().numpy()
+    tiles = [(y0, x0, min(tile, H - y0), min(tile, W - x0))
+             for y0 in range(0, H, tile) for x0 in range(0, W, tile)]
+    seg_len = min(SEG, min(tw for _, _, _, tw in tiles))
+    mae = ops.openness_judge(win, approx, tile_h=tile, tile_w=tile, sample_frac=sample_frac,
+                             seed=seed, seg_len=seg_len, table=tab).cpu().numpy()
+    n_samples = sum(max(1, math.ceil(sample_frac * th * tw / seg_len)) * seg_len
+                    for _, _, th, tw in tiles)
     flagged = 0
     for (y0, x0, th, tw), e in zip(tiles, mae):
         if not (e <= threshold):          # NaN (no valid sample) is recomputed too
@@ -480,7 +469,7 @@ def judge(win: torch.Tensor, approx: torch.Tensor, radius: int, n_dirs: int, pix
             exact_openness(win, ops.region(y0, x0, th, tw), radius, n_dirs, pixel_size,
                            out=approx[y0:y0 + th, x0:x0 + tw])
     return {"tiles_checked": len(tiles), "tiles_replaced": flagged,
-            "samples": int(segments.shape[0]) * seg_len,
+            "samples": n_samples,
             "sampled_mae_deg": float(np.nanmean(mae)) if np.isfinite(mae).any() else float("nan"),
             "max_tile_mae_deg": float(np.nanmax(mae)) if np.isfinite(mae).any() else float("nan")}
 

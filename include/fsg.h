@@ -204,19 +204,22 @@ int fsg_openness(const fsg_window* win, const fsg_region* reg,
 int fsg_openness_points(const fsg_window* win, const int32_t* points, int32_t n,
                         const fsg_openness_params* p, float* out, void* stream);
 
-/* No reference counterpart: the learned surrogate's judge.  For each of
-   n_tiles tiles, the mean |exact openness - approx| over its sampled
-   segments -- segments[k] = (row, col0) (device int32 [n][2]) covers seg_len
-   (1..64) pixels of one window row, tile t owns segments [tile_start[t],
-   tile_start[t + 1]) (device int32) -- into tile_mae[t] (device f32; NaN when
-   no pixel had both values).  approx is [h][lda] in window coordinates and is
+/* No reference counterpart: the learned surrogate's judge.  The window is
+   cut into tile_h x tile_w tiles (row-major, ragged at the far edges); tile t
+   gets n_t = max(1, ceil(sample_frac * th * tw / seg_len)) row segments of
+   seg_len (1..64, at most every tile's width) pixels, segment k at
+     h = splitmix64(splitmix64(seed ^ splitmix64(t)) + k),
+     row = y0 + (h >> 32) % th,  col0 = x0 + (h & 0xffffffff) % (tw - seg_len + 1)
+   (splitmix64: Steele, Lea & Flood 2014).  tile_mae[t] (device f32) receives
+   the mean |exact openness - approx| over the segments' pixels where both are
+   numbers (NaN when none).  approx is [h][lda] in window coordinates and is
    compared with the openness of p (normally raw_degrees).  Workspace:
-   fsg_openness_judge_workspace_size(n_tiles) bytes. */
-size_t fsg_openness_judge_workspace_size(int32_t n_tiles);
+   fsg_openness_judge_workspace_size bytes. */
+size_t fsg_openness_judge_workspace_size(const fsg_window* win, int32_t tile_h, int32_t tile_w);
 int fsg_openness_judge(const fsg_window* win, const fsg_openness_params* p, const float* approx,
-                       int64_t lda, const int32_t* segments, int32_t seg_len,
-                       const int32_t* tile_start, int32_t n_tiles, float* tile_mae,
-                       void* workspace, size_t workspace_bytes, void* stream);
+                       int64_t lda, int32_t tile_h, int32_t tile_w, double sample_frac,
+                       uint64_t seed, int32_t seg_len, float* tile_mae, void* workspace,
+                       size_t workspace_bytes, void* stream);
 
 /*
  * Ambient-occlusion sample table (_impl_ambient_occlusion.py:288-339): ring

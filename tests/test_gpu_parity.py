@@ -208,6 +208,24 @@ def test_openness_window(cuda, otype, nan_discs):
     _cmp("openness+stretch", out2, ref.display_stretch(ref.openness(dem.numpy(), **kw), st), 1e-4)
 
 
+@pytest.mark.parametrize("f", [2, 4, 8, 16, 64, 3])
+def test_block_mean_factors(cuda, f):
+    from fujishadergpu_amd import ops
+    dem = _dem(150, 171, 29, nan_discs=4).numpy()
+    got = ops.block_mean(torch.from_numpy(dem).to(cuda), f).cpu().numpy()
+    oh, ow = -(-150 // f), -(-171 // f)
+    want = np.full((oh, ow), np.nan, np.float32)
+    for i in range(oh):
+        for j in range(ow):
+            blk = dem[i * f:(i + 1) * f, j * f:(j + 1) * f].astype(np.float64)
+            v = blk[np.isfinite(blk)]
+            if v.size:
+                want[i, j] = np.float32(np.float32(v.sum()) / np.float32(v.size))
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    m = ~np.isnan(want)
+    assert np.allclose(got[m], want[m], rtol=1e-7, atol=0)
+
+
 @pytest.mark.parametrize("nan_discs", [0, 6])
 def test_openness_raw_degrees_points_and_judge(cuda, nan_discs):
     from fujishadergpu_amd import ops
@@ -230,24 +248,18 @@ def test_openness_raw_degrees_points_and_judge(cuda, nan_discs):
     pv = ops.openness_points(g, torch.from_numpy(pts).to(cuda), table=tab,
                              openness_type="positive").cpu().numpy()
     assert np.array_equal(pv, got[pts[:, 0], pts[:, 1]], equal_nan=True)
-    # judge: per-tile mean |exact - approx| over row segments
+    # judge: per-tile mean |exact - approx| over hashed row segments
     approx = want + rng.normal(0, 2.0, want.shape).astype(np.float32)
     approx[5, :] = np.nan
-    segs, starts = [], [0]
-    tiles = [(0, 0, 80, 90), (0, 90, 80, 80), (80, 0, 70, 170)]
     L = 40
-    for (y0, x0, th, tw) in tiles:
-        n = 37
-        segs += [(int(rng.integers(y0, y0 + th)), int(rng.integers(x0, x0 + tw - L + 1)))
-                 for _ in range(n)]
-        starts.append(starts[-1] + n)
-    seg_t = torch.tensor(segs, dtype=torch.int32, device=cuda)
-    st_t = torch.tensor(starts, dtype=torch.int32, device=cuda)
-    mae = ops.openness_judge(g, torch.from_numpy(approx).to(cuda), seg_t, L, st_t,
-                             table=tab).cpu().numpy()
-    for t_i in range(len(tiles)):
+    mae = ops.openness_judge(g, torch.from_numpy(approx).to(cuda), tile_h=64, tile_w=96,
+                             sample_frac=0.05, seed=1234, seg_len=L, table=tab).cpu().numpy()
+    plan = ops.judge_segments(150, 170, 64, 96, 0.05, 1234, L)
+    assert len(plan) == mae.size == 3 * 2
+    for t_i, ((y0, x0, th, tw), segs) in enumerate(plan):
         e = []
-        for (r_, c_) in segs[starts[t_i]:starts[t_i + 1]]:
+        for (r_, c_) in segs:
+            assert y0 <= r_ < y0 + th and x0 <= c_ and c_ + L <= x0 + tw
             d = np.abs(want[r_, c_:c_ + L].astype(np.float64) - approx[r_, c_:c_ + L])
             e += list(d[~np.isnan(d)])
         assert mae[t_i] == pytest.approx(np.mean(e), rel=1e-4, abs=3e-4), t_i

@@ -192,3 +192,20 @@ def test_short_training_beats_the_method_without_the_network(cuda):
     assert losses[-1] < losses[0]
     assert after["mae_surrogate_deg"] < after["mae_base_deg"]
     assert after["mae_surrogate_deg"] < after["mae_without_detail_deg"]
+
+
+def test_judge_replaces_exactly_the_tiles_over_the_threshold(cuda):
+    from fujishadergpu_amd import ops, synth
+    from fujishadergpu_amd import surrogate as S
+    win = synth.fbm(512, 512, 0.75, 1500.0, seed=3, device=cuda)
+    exact = S.exact_openness(win, ops.full_region(win), 64, 8, 10.0)
+    approx = exact.clone()
+    approx[0:256, 256:512] += 3.0          # tile (0, 1): over the threshold
+    approx[256:512, 0:256] += 0.5          # tile (1, 0): under it
+    rep = S.judge(win, approx, 64, 8, 10.0, tile=256, threshold=1.5)
+    assert rep["tiles_checked"] == 4 and rep["tiles_replaced"] == 1
+    assert rep["samples"] >= 4 * 0.01 * 256 * 256
+    assert torch.equal(approx[0:256, 256:512], exact[0:256, 256:512])
+    assert torch.equal(approx[0:256, 0:256], exact[0:256, 0:256])
+    assert torch.allclose(approx[256:512, 0:256], exact[256:512, 0:256] + 0.5)
+    assert rep["max_tile_mae_deg"] == pytest.approx(3.0, abs=1e-3)

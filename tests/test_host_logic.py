@@ -57,3 +57,19 @@ def test_gaussian_coarse_radius_uses_its_truncation_margin():
     assert not shared_coarse_ok((64, 64), (512, 512), (64, 64, 384, 384), 16, 1, (1024, 1024))
     # a box of radius 2 needs less margin than the Gaussian
     assert shared_coarse_ok((64, 64), (512, 512), (64, 64, 384, 384), 16, 2, (1024, 1024))
+
+
+def test_judge_sampling_hash_and_segments():
+    from fujishadergpu_amd import ops
+    # published splitmix64 outputs for seed 0 (first two draws)
+    assert ops.splitmix64(0) == 0xE220A8397B1DCDAF
+    assert ops.splitmix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+    plan = ops.judge_segments(1000, 700, 256, 256, 0.01, 7, 64)
+    assert len(plan) == 4 * 3
+    for (y0, x0, th, tw), segs in plan:
+        assert len(segs) == max(1, -(-int(round(0.01 * th * tw * 1e6)) // (64 * 10 ** 6)))
+        for r, c in segs:
+            assert y0 <= r < y0 + th and x0 <= c and c + 64 <= x0 + tw
+    # the draw depends on the seed and on the tile
+    assert plan[0][1] != ops.judge_segments(1000, 700, 256, 256, 0.01, 8, 64)[0][1]
+    assert plan[0][1][:3] != plan[1][1][:3]
