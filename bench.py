@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--only-c2", action="store_true",
                     help="skip the per-algorithm workloads (C1 hillshade+slope, C3 curvature and "
                          "openness, C5 fused batch)")
+    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5"], default=None,
+                    help="run only this per-algorithm workload and print its JSON (a profiling "
+                         "aid; the bench line is the default run)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy a 8192x8192 float32 region once with "
                          "k_copy_region (256 MiB read + 256 MiB written): the reference point "
@@ -377,6 +380,10 @@ def main():
     from fujishadergpu_amd import _native as N
 
     dev = torch.device("cuda:0")
+    if args.workload:
+        fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload]
+        print(json.dumps({args.workload: fn(dev)}), flush=True)
+        return
     n = args.size
     dem = make_dem(n, dev)
     out_u8 = torch.empty((n, n), dtype=torch.uint8, device=dev)
