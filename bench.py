@@ -221,7 +221,8 @@ def bench_c3(dev, n=32768):
 def c5_jobs(dev, n_tiles=256):
     """256 independent tiles, side log-uniform in [256, 4096] (seeds 50000+i),
     fBm H ~ U[0.6, 0.9], relief ~ U[500, 3000] m; even tiles projected 10 m,
-    odd tiles geographic 1 arc-second at a latitude ~ U[30, 60] N.  Each tile
+    odd tiles geographic 1 arc-second centred at a latitude ~ U[30, 60] N, every
+    row with the x cell width of its own latitude.  Each tile
     is synthesised at the next power of two and cropped (five FFT sizes in
     all, so the FFT library compiles five kernels, not hundreds)."""
     import math
@@ -230,7 +231,8 @@ def c5_jobs(dev, n_tiles=256):
     import torch
     from fujishadergpu_amd import synth
     from fujishadergpu_amd.batch import TileJob
-    from fujishadergpu_amd.raster_info import meters_per_degree
+    from fujishadergpu_amd.algorithms import RowScales
+    from fujishadergpu_amd.raster_info import meters_per_degree, row_geographic_scales_x
     jobs = []
     for i in range(n_tiles):
         r = np.random.default_rng(50000 + i)
@@ -247,9 +249,14 @@ def c5_jobs(dev, n_tiles=256):
         if i % 2 == 0:
             p["pixel_size"] = 10.0
         else:
-            m_lon, m_lat = meters_per_degree(float(r.uniform(30.0, 60.0)))
+            # 1 arc-second rows, each with the cell width of its own latitude
+            lat_c = float(r.uniform(30.0, 60.0))
+            m_lon, m_lat = meters_per_degree(lat_c)
             sx, sy = m_lon / 3600.0, m_lat / 3600.0
-            p.update(pixel_scale_x=sx, pixel_scale_y=-sy, pixel_size=0.5 * (sx + sy))
+            rows = RowScales(row_geographic_scales_x(1.0 / 3600.0, -1.0 / 3600.0,
+                                                     lat_c + 0.5 * side / 3600.0, 0, side),
+                             dev)
+            p.update(pixel_scale_x_rows=rows, pixel_scale_y=-sy, pixel_size=0.5 * (sx + sy))
         e = lambda: torch.empty((side, side), dtype=torch.float32, device=dev)  # noqa: E731
         jobs.append(TileJob(dem=dem, params=p, hillshade=e(), slope=e(), curvature=e(),
                             topousm=e(), nan_free=True))
@@ -298,8 +305,9 @@ def bench_c5(dev):
     t_graph = _gpu_time(replay, 5)
     b.close()
     res = {"workload": "256 tiles, side log-U[256, 4096] (seeds 50000+i), fBm H~U[0.6,0.9], "
-                       "half projected 10 m, half geographic 1 arc-second at 30-60 N (per-tile "
-                       "latitude); hillshade+slope+curvature(mean)+topousm{2,8,32,128}, f32",
+                       "half projected 10 m, half geographic 1 arc-second at 30-60 N (per-row "
+                       "cos(lat) cell width); hillshade+slope+curvature(mean)+topousm{2,8,32,128}, "
+                       "f32",
            "tiles": len(jobs), "mpx": round(px / 1e6, 1),
            "gpix_s": round(px / t_graph / 1e9, 3), "ms_graph": round(t_graph * 1e3, 3),
            "hbm_frac": round(20 * px / t_graph / (HBM_PEAK_GBS * 1e9), 4), "alg_bytes_per_px": 20,

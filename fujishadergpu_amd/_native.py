@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 5  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 6  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -52,12 +52,14 @@ class fsg_shade_params(C.Structure):
         ("step_x", C.c_double), ("step_y", C.c_double), ("sign_x", C.c_double),
         ("sign_y", C.c_double), ("z_factor", C.c_double), ("azimuth_deg", C.c_double),
         ("altitude_deg", C.c_double), ("slope_unit", C.c_int32), ("nan_hint", C.c_int32),
+        ("step_x_rows", C.c_void_p),
     ]
 
 
 class fsg_curvature_params(C.Structure):
     _fields_ = [("step_x", C.c_double), ("step_y", C.c_double),
-                ("curvature_type", C.c_int32), ("nan_hint", C.c_int32)]
+                ("curvature_type", C.c_int32), ("nan_hint", C.c_int32),
+                ("step_x_rows", C.c_void_p)]
 
 
 class fsg_ray_sample(C.Structure):

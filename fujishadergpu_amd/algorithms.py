@@ -59,6 +59,52 @@ def curvature_steps(pixel_size=1.0, pixel_scale_x=None, pixel_scale_y=None):
     return sx, sy
 
 
+class RowScales:
+    """Per-row signed x pixel scales resolved once for the device: the float64
+    tensor the kernels read, and the first row's value on the host (its sign
+    orients hillshade; every row shares it)."""
+
+    def __init__(self, rows, device, pixel_size: float = 1.0):
+        a = np.asarray(rows.cpu() if isinstance(rows, torch.Tensor) else rows,
+                       dtype=np.float64).reshape(-1)
+        ps = float(pixel_size or 1.0)
+        a = np.where(np.abs(a) < 1e-9, ps, a)   # the 1e-9 fallback of _nan_utils.py:69-72
+        if a.size == 0 or ((a > 0).any() and (a < 0).any()):
+            raise ValueError("pixel_scale_x_rows must be non-empty and keep one sign")
+        self.n = int(a.size)
+        self.first = float(a[0])
+        self.t = torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def x_steps_rows(p: dict, h: int, device) -> Optional[RowScales]:
+    """The optional ``pixel_scale_x_rows`` parameter -- one signed x pixel
+    scale per window row, for geographic grids whose cell width follows the
+    latitude of each row -- resolved for the device (a RowScales passes
+    through without a host round trip).  No reference counterpart: the
+    reference uses one latitude per tile (core/tile_processor.py:910-932)."""
+    rows = p.get("pixel_scale_x_rows")
+    if rows is None:
+        return None
+    if not isinstance(rows, RowScales):
+        rows = RowScales(rows, device, p.get("pixel_size", 1.0))
+    if rows.n != h:
+        raise ValueError(f"pixel_scale_x_rows has {rows.n} values for a window of {h} rows")
+    return rows
+
+
+def shade_steps(p: dict, win: torch.Tensor):
+    """(|step_x|, |step_y|, sign_x, sign_y, per-row x steps or None) of the
+    hillshade/slope gradient (_nan_utils.py:65-73, _impl_hillshade.py:39-42)."""
+    psx, psy = p.get("pixel_scale_x"), p.get("pixel_scale_y")
+    rows = x_steps_rows(p, win.shape[0], win.device)
+    if rows is not None:
+        psx = rows.first
+    sx, sy = gradient_steps(p.get("pixel_size", 1.0), psx, psy)
+    sgx = 1.0 if (psx is None or float(psx) >= 0.0) else -1.0
+    sgy = 1.0 if (psy is None or float(psy) >= 0.0) else -1.0
+    return sx, sy, sgx, sgy, (rows.t if rows is not None else None)
+
+
 def ray_steps(pixel_size=1.0, pixel_scale_x=None, pixel_scale_y=None):
     """|step| of the openness/AO physical distance (_impl_openness.py:70-75)."""
     sx = abs(float(pixel_scale_x)) if pixel_scale_x is not None else float(pixel_size)
@@ -211,15 +257,13 @@ class HillshadeAlgorithm(NativeTileAlgorithm):
             raise UnsupportedOnBackend("hillshade: multiscale radii are not built on this backend")
         z = p.get("z_factor", 1.0)
         z = 1.0 if z is None else z
-        psx, psy = p.get("pixel_scale_x"), p.get("pixel_scale_y")
-        sx, sy = gradient_steps(p.get("pixel_size", 1.0), psx, psy)
-        sgx = 1.0 if (psx is None or float(psx) >= 0.0) else -1.0
-        sgy = 1.0 if (psy is None or float(psy) >= 0.0) else -1.0
+        sx, sy, sgx, sgy, rows = shade_steps(p, win)
         for o in out.descriptors():
             ops.hillshade_slope(win, reg, step_x=sx, step_y=sy, sign_x=sgx, sign_y=sgy,
                                 z_factor=float(z), azimuth=float(p.get("azimuth", DEFAULT_AZIMUTH)),
                                 altitude=float(p.get("altitude", DEFAULT_ALTITUDE)), hs_out=o,
-                                nan_free=bool(p.get("_window_nan_free", False)))
+                                nan_free=bool(p.get("_window_nan_free", False)),
+                                step_x_rows=rows)
 
 
 class SlopeAlgorithm(NativeTileAlgorithm):
@@ -232,11 +276,11 @@ class SlopeAlgorithm(NativeTileAlgorithm):
 
     def run_region(self, win, reg, out, p):
         self.check_mode(p)
-        sx, sy = gradient_steps(p.get("pixel_size", 1.0), p.get("pixel_scale_x"),
-                                p.get("pixel_scale_y"))
+        sx, sy, _, _, rows = shade_steps(p, win)
         for o in out.descriptors():
             ops.hillshade_slope(win, reg, step_x=sx, step_y=sy, unit=p.get("unit", "degree"),
-                                sl_out=o, nan_free=bool(p.get("_window_nan_free", False)))
+                                sl_out=o, nan_free=bool(p.get("_window_nan_free", False)),
+                                step_x_rows=rows)
 
 
 class HillshadeSlopeFused(NativeTileAlgorithm):
@@ -253,10 +297,7 @@ class HillshadeSlopeFused(NativeTileAlgorithm):
         self.check_mode(p)
         z = p.get("z_factor", 1.0)
         z = 1.0 if z is None else z
-        psx, psy = p.get("pixel_scale_x"), p.get("pixel_scale_y")
-        sx, sy = gradient_steps(p.get("pixel_size", 1.0), psx, psy)
-        sgx = 1.0 if (psx is None or float(psx) >= 0.0) else -1.0
-        sgy = 1.0 if (psy is None or float(psy) >= 0.0) else -1.0
+        sx, sy, sgx, sgy, rows = shade_steps(p, win)
         hs = out.descriptors()
         sl = out_slope.descriptors() if out_slope is not None else []
         n = max(len(hs), len(sl))
@@ -267,7 +308,8 @@ class HillshadeSlopeFused(NativeTileAlgorithm):
                                 unit=p.get("unit", "degree"),
                                 hs_out=hs[i] if i < len(hs) else None,
                                 sl_out=sl[i] if i < len(sl) else None,
-                                nan_free=bool(p.get("_window_nan_free", False)))
+                                nan_free=bool(p.get("_window_nan_free", False)),
+                                step_x_rows=rows)
 
 
 class CurvatureAlgorithm(NativeTileAlgorithm):
@@ -282,10 +324,14 @@ class CurvatureAlgorithm(NativeTileAlgorithm):
         self.check_mode(p)
         sx, sy = curvature_steps(p.get("pixel_size", 1.0), p.get("pixel_scale_x"),
                                  p.get("pixel_scale_y"))
+        rows = x_steps_rows(p, win.shape[0], win.device)
+        if rows is not None:
+            sx = rows.first
         for o in out.descriptors():
             ops.curvature(win, reg, step_x=sx, step_y=sy,
                           curvature_type=p.get("curvature_type", "mean"), out=o,
-                          nan_free=bool(p.get("_window_nan_free", False)))
+                          nan_free=bool(p.get("_window_nan_free", False)),
+                          step_x_rows=rows.t if rows is not None else None)
 
 
 class OpennessAlgorithm(NativeTileAlgorithm):

@@ -23,7 +23,8 @@ import torch
 from . import _native as N
 from . import ops
 from .algorithms import (CurvatureAlgorithm, HillshadeAlgorithm, OutSpec, SlopeAlgorithm,
-                         TopoUSMFastAlgorithm, _post_from_stats, curvature_steps, gradient_steps)
+                         TopoUSMFastAlgorithm, _post_from_stats, curvature_steps, gradient_steps,
+                         x_steps_rows)
 from .params import DEFAULT_ALTITUDE, DEFAULT_AZIMUTH
 
 
@@ -45,12 +46,18 @@ def _empty_out() -> N.fsg_output:
     return N.fsg_output()
 
 
-def _tile_struct(job: TileJob) -> N.fsg_batch_tile:
+def _tile_struct(job: TileJob, keep: list) -> N.fsg_batch_tile:
+    """The tile's C descriptor; tensors it points at are appended to keep."""
     p = job.params
     t = N.fsg_batch_tile()
     t.win = ops.window(job.dem)
     t.reg = ops.full_region(job.dem)
     psx, psy = p.get("pixel_scale_x"), p.get("pixel_scale_y")
+    rs = x_steps_rows(p, job.dem.shape[0], job.dem.device)
+    rows = None
+    if rs is not None:
+        keep.append(rs)
+        psx, rows = rs.first, rs.t
     if job.hillshade is not None or job.slope is not None:
         sx, sy = gradient_steps(p.get("pixel_size", 1.0), psx, psy)
         z = p.get("z_factor", 1.0)
@@ -60,7 +67,7 @@ def _tile_struct(job: TileJob) -> N.fsg_batch_tile:
             z_factor=1.0 if z is None else float(z),
             azimuth=float(p.get("azimuth", DEFAULT_AZIMUTH)),
             altitude=float(p.get("altitude", DEFAULT_ALTITUDE)), unit=p.get("unit", "degree"),
-            nan_free=job.nan_free)
+            nan_free=job.nan_free, step_x_rows=rows)
     t.hillshade = ops.output(job.hillshade, clip01=True) if job.hillshade is not None \
         else _empty_out()
     t.slope = ops.output(job.slope) if job.slope is not None else _empty_out()
@@ -68,7 +75,7 @@ def _tile_struct(job: TileJob) -> N.fsg_batch_tile:
         cx, cy = curvature_steps(p.get("pixel_size", 1.0), psx, psy)
         t.curv = ops.curvature_params(step_x=cx, step_y=cy,
                                       curvature_type=p.get("curvature_type", "mean"),
-                                      nan_free=job.nan_free)
+                                      nan_free=job.nan_free, step_x_rows=rows)
         t.curvature = ops.output(job.curvature)
     else:
         t.curvature = _empty_out()
@@ -89,7 +96,9 @@ class FusedBatch:
 
     def __init__(self, jobs: Sequence[TileJob]):
         self.jobs = list(jobs)
-        arr = (N.fsg_batch_tile * len(self.jobs))(*[_tile_struct(j) for j in self.jobs])
+        self._keep = []
+        arr = (N.fsg_batch_tile * len(self.jobs))(*[_tile_struct(j, self._keep)
+                                                    for j in self.jobs])
         h = C.c_void_p()
         N.check(N.lib().fsg_batch_create(arr, len(self.jobs), C.byref(h)))
         self._h = h

@@ -253,6 +253,12 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
               (S.want_cv ? t.curv.nan_hint != 0 : 0);
     if (S.want_hs || S.want_sl) S.shade = make_shadek(&t.shade, S.want_hs, S.want_sl);
     if (S.want_cv) S.curv = make_curvk(&t.curv);
+    if ((S.want_hs || S.want_sl) && S.want_cv && t.shade.step_x_rows != t.curv.step_x_rows) {
+      fsg_batch_destroy(b);
+      return fail(FSG_E_INVALID, "fsg_batch_create: per-row x steps differ between the "
+                                 "shade and curvature parameters of one tile");
+    }
+    S.xrows = (S.want_hs || S.want_sl) ? t.shade.step_x_rows : t.curv.step_x_rows;
     S.shade_from_tables = !S.want_cv || (same_axis(S.shade.ax, S.curv.ax) &&
                                          same_axis(S.shade.ay, S.curv.ay));
     S.hs = S.want_hs ? make_out(&t.hillshade) : OutDev{};

@@ -64,6 +64,7 @@ extern "C" int fsg_hillshade_slope(const fsg_window* win, const fsg_region* reg,
   S.want_hs = hillshade != nullptr;
   S.want_sl = slope != nullptr;
   S.shade_from_tables = 1;
+  S.xrows = p->step_x_rows;
   S.nanaw = p->nan_hint != 0;
   if (hillshade) S.hs = make_out(hillshade);
   if (slope) S.sl = make_out(slope);
@@ -91,6 +92,7 @@ extern "C" int fsg_curvature(const fsg_window* win, const fsg_region* reg,
   StencilTile S = stencil_tile(win, reg);
   S.curv = make_curvk(p);
   S.want_cv = 1;
+  S.xrows = p->step_x_rows;
   S.nanaw = p->nan_hint != 0;
   S.cv = make_out(out);
   return launch_stencil(S, "k_curvature", as_stream(stream));

@@ -74,7 +74,7 @@ struct Axis {
   float a1, b1, c1;  // f32(0.5/h),  f32(-2/h), f32(1.5/h)
 };
 
-inline Axis make_axis(double h) {
+__host__ __device__ inline Axis make_axis(double h) {
   Axis a;
   a.two_h = (float)(2. * h);
   a.a0 = (float)(-1.5 / h);
@@ -177,12 +177,12 @@ __device__ __forceinline__ float slope_from(const ShadeK& K, float dx, float dy)
 // compute_hillshade_block's shading from the (unscaled) gradient; a z factor
 // other than 1 re-derives from the scaled values as the reference does.
 template <class F>
-__device__ __forceinline__ float hillshade_from(const ShadeK& K, int i, int j, int h, int w, F f,
-                                                float dx, float dy) {
+__device__ __forceinline__ float hillshade_from(const ShadeK& K, const Axis& axr, int i, int j,
+                                                int h, int w, F f, float dx, float dy) {
   float hdy = dy, hdx = dx;
   if (!K.z_is_one) {
     hdy = deriv(K.ay, i, h, [&](int k) { return f(k, j) * K.zf; });
-    hdx = deriv(K.ax, j, w, [&](int k) { return f(i, k) * K.zf; });
+    hdx = deriv(axr, j, w, [&](int k) { return f(i, k) * K.zf; });
   }
   const float e = hdx * K.sgx;
   const float n = hdy * K.sgy;
@@ -261,10 +261,13 @@ constexpr int ST_LT = ST_TW + 2 * ST_HALO + 1;      // staged row stride
 constexpr int ST_ROWS = ST_TH + 2 * ST_HALO;
 constexpr int ST_PW = ST_TW + 2 * ST_RING;          // derivative table width
 constexpr int ST_PH = ST_TH + 2 * ST_RING;
-constexpr int ST_LDS_FLOATS = ST_ROWS * ST_LT + 2 * ST_PH * ST_PW;
+constexpr int ST_AXIS_FLOATS = 8;                    // one Axis, padded
+constexpr int ST_LDS_FLOATS =
+    ST_ROWS * ST_LT + 2 * ST_PH * ST_PW + 2 * ST_ROWS * ST_AXIS_FLOATS;
 
 struct StencilTile {
   WinDev W;
+  const double* xrows;  // per-row signed x steps of the window, or NULL
   int r0, c0, rh, rw;
   int nbx;         // blocks per block row of the region
   int nanaw;
@@ -291,10 +294,17 @@ __device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by,
   float* F = lds;
   float* Pt = lds + ST_ROWS * ST_LT;
   float* Qt = Pt + ST_PH * ST_PW;
+  Axis* Xa = (Axis*)(Qt + ST_PH * ST_PW);     // per staged row: |x step| coefficients
+  Axis* Xs = (Axis*)((float*)Xa + ST_ROWS * ST_AXIS_FLOATS);  // signed x step
   const int t = threadIdx.x;
   for (int e = t; e < nr * nc; e += 256) {
     const int rr = e / nc, cc = e - rr * nc;
     F[rr * ST_LT + cc] = filled<NANAW>(S.W, tr0 + rr, tc0 + cc, G);
+  }
+  if (S.xrows && t < nr) {
+    const double hx = S.xrows[tr0 + t];
+    *(Axis*)((float*)Xa + t * ST_AXIS_FLOATS) = make_axis(fabs(hx));
+    *(Axis*)((float*)Xs + t * ST_AXIS_FLOATS) = make_axis(hx);
   }
   __syncthreads();
   auto f = [&](int r, int c) { return F[(r - tr0) * ST_LT + (c - tc0)]; };
@@ -302,12 +312,19 @@ __device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by,
   const bool need_tab = S.want_cv || (S.shade_from_tables && (S.want_hs || S.want_sl));
   const Axis ax = S.want_cv ? S.curv.ax : S.shade.ax;
   const Axis ay = S.want_cv ? S.curv.ay : S.shade.ay;
+  // x-derivative coefficients of window row i: the row's own step, or the constant one
+  auto axr = [&](int i, const Axis& c) -> Axis {
+    return S.xrows ? *(const Axis*)((const float*)Xa + (i - tr0) * ST_AXIS_FLOATS) : c;
+  };
+  auto sxr = [&](int i, const Axis& c) -> Axis {
+    return S.xrows ? *(const Axis*)((const float*)Xs + (i - tr0) * ST_AXIS_FLOATS) : c;
+  };
   if (need_tab) {
     for (int e = t; e < ST_PH * ST_PW; e += 256) {
       const int pr = e / ST_PW, pc = e - pr * ST_PW;
       const int i = rl - ST_RING + pr, j = cl - ST_RING + pc;
       if (i >= 0 && i < H && j >= 0 && j < Wd) {
-        Pt[e] = deriv(ax, j, Wd, [&](int k) { return f(i, k); });
+        Pt[e] = deriv(axr(i, ax), j, Wd, [&](int k) { return f(i, k); });
         Qt[e] = deriv(ay, i, H, [&](int k) { return f(k, j); });
       }
     }
@@ -331,24 +348,26 @@ __device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by,
       }
       if (S.want_hs || S.want_sl) {
         float dx, dy;
+        const Axis sax = axr(wr, S.shade.ax);
         if (S.shade_from_tables) {
           dx = P(wr, wc);
           dy = Q(wr, wc);
         } else {
-          grad_at(S.shade.ax, S.shade.ay, wr, wc, H, Wd, f, dx, dy);
+          grad_at(sax, S.shade.ay, wr, wc, H, Wd, f, dx, dy);
         }
         if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, slope_from(S.shade, dx, dy)));
         if (S.want_hs)
           store_out(S.hs, rr, cc,
-                    apply_post(phs, hillshade_from(S.shade, wr, wc, H, Wd, f, dx, dy)));
+                    apply_post(phs, hillshade_from(S.shade, sax, wr, wc, H, Wd, f, dx, dy)));
       }
       if (S.want_cv) {
         const CurvK& K = S.curv;
+        const Axis ksx = sxr(wr, K.sx);
         const float p = P(wr, wc), q = Q(wr, wc);
-        const float dxx = deriv(K.sx, wc, Wd, [&](int m) { return P(wr, m); });
+        const float dxx = deriv(ksx, wc, Wd, [&](int m) { return P(wr, m); });
         const float dxy = deriv(K.sy, wr, H, [&](int m) { return P(m, wc); });
         const float dyy = deriv(K.sy, wr, H, [&](int m) { return Q(m, wc); });
-        const float dyx = deriv(K.sx, wc, Wd, [&](int m) { return Q(wr, m); });
+        const float dyx = deriv(ksx, wc, Wd, [&](int m) { return Q(wr, m); });
         store_out(S.cv, rr, cc, apply_post(pcv, curvature_from(K, p, q, dxx, dxy, dyy, dyx)));
       }
     }

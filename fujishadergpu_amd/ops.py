@@ -89,28 +89,43 @@ def _lib():
 
 # ------------------------------------------------------------------ stencils
 
+def _rows_ptr(step_x_rows: Optional[torch.Tensor]):
+    if step_x_rows is None:
+        return None
+    if step_x_rows.dtype != torch.float64 or not step_x_rows.is_contiguous():
+        raise ValueError("step_x_rows must be a contiguous float64 tensor")
+    return step_x_rows.data_ptr()
+
+
 def shade_params(*, step_x: float, step_y: float, sign_x: float = 1.0, sign_y: float = 1.0,
                  z_factor: float = 1.0, azimuth: float = 315.0, altitude: float = 45.0,
-                 unit: str = "degree", nan_free: bool = False) -> N.fsg_shade_params:
+                 unit: str = "degree", nan_free: bool = False,
+                 step_x_rows: Optional[torch.Tensor] = None) -> N.fsg_shade_params:
+    """step_x_rows: per window row signed x steps (float64, on the device), kept
+    alive by the caller while the parameters are in use."""
     return N.fsg_shade_params(float(step_x), float(step_y), float(sign_x), float(sign_y),
                               float(z_factor), float(azimuth), float(altitude),
-                              {"degree": 0, "percent": 1}.get(unit, 2), 0 if nan_free else 1)
+                              {"degree": 0, "percent": 1}.get(unit, 2), 0 if nan_free else 1,
+                              _rows_ptr(step_x_rows))
 
 
 def curvature_params(*, step_x: float, step_y: float, curvature_type: str,
-                     nan_free: bool = False) -> N.fsg_curvature_params:
+                     nan_free: bool = False,
+                     step_x_rows: Optional[torch.Tensor] = None) -> N.fsg_curvature_params:
     ct = {"mean": 0, "gaussian": 1, "planform": 2}.get(curvature_type, 3)
-    return N.fsg_curvature_params(float(step_x), float(step_y), ct, 0 if nan_free else 1)
+    return N.fsg_curvature_params(float(step_x), float(step_y), ct, 0 if nan_free else 1,
+                                  _rows_ptr(step_x_rows))
 
 
 def hillshade_slope(win_t: torch.Tensor, reg: N.fsg_region, *, step_x: float, step_y: float,
                     sign_x: float = 1.0, sign_y: float = 1.0, z_factor: float = 1.0,
                     azimuth: float = 315.0, altitude: float = 45.0, unit: str = "degree",
                     hs_out: Optional[N.fsg_output] = None,
-                    sl_out: Optional[N.fsg_output] = None, nan_free: bool = False) -> None:
+                    sl_out: Optional[N.fsg_output] = None, nan_free: bool = False,
+                    step_x_rows: Optional[torch.Tensor] = None) -> None:
     p = shade_params(step_x=step_x, step_y=step_y, sign_x=sign_x, sign_y=sign_y,
                      z_factor=z_factor, azimuth=azimuth, altitude=altitude, unit=unit,
-                     nan_free=nan_free)
+                     nan_free=nan_free, step_x_rows=step_x_rows)
     w = window(win_t)
     N.check(_lib().fsg_hillshade_slope(C.byref(w), C.byref(reg), C.byref(p),
                                        C.byref(hs_out) if hs_out is not None else None,
@@ -119,9 +134,10 @@ def hillshade_slope(win_t: torch.Tensor, reg: N.fsg_region, *, step_x: float, st
 
 
 def curvature(win_t, reg, *, step_x: float, step_y: float, curvature_type: str,
-              out: N.fsg_output, nan_free: bool = False) -> None:
+              out: N.fsg_output, nan_free: bool = False,
+              step_x_rows: Optional[torch.Tensor] = None) -> None:
     p = curvature_params(step_x=step_x, step_y=step_y, curvature_type=curvature_type,
-                         nan_free=nan_free)
+                         nan_free=nan_free, step_x_rows=step_x_rows)
     w = window(win_t)
     N.check(_lib().fsg_curvature(C.byref(w), C.byref(reg), C.byref(p), C.byref(out), _stream()))
 

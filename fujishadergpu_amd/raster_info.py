@@ -59,3 +59,18 @@ def tile_geographic_scales(transform_a: float, transform_e: float, lat_center_ti
     px_x = math.copysign(abs(transform_a) * m_lon, transform_a if transform_a != 0 else 1.0)
     px_y = math.copysign(abs(transform_e) * m_lat, transform_e if transform_e != 0 else -1.0)
     return float(px_x), float(px_y)
+
+
+def row_geographic_scales_x(transform_a: float, transform_e: float, transform_f: float,
+                            row0: int, n_rows: int):
+    """Signed x pixel scale (metres) of raster rows row0 .. row0 + n_rows - 1 of a
+    geographic grid, each at the latitude of its own row centre
+    (f + (row + 0.5) e) with the WGS84 series of meters_per_degree.  No
+    reference counterpart: the reference evaluates one latitude per tile
+    (core/tile_processor.py:910-932)."""
+    import numpy as np
+    lat = transform_f + (np.arange(row0, row0 + n_rows, dtype=np.float64) + 0.5) * transform_e
+    r = np.radians(lat)
+    m_lon = 111412.84 * np.cos(r) - 93.5 * np.cos(3.0 * r) + 0.118 * np.cos(5.0 * r)
+    m_lon = np.maximum(m_lon, 1e-6)
+    return np.copysign(abs(transform_a) * m_lon, transform_a if transform_a != 0 else 1.0)

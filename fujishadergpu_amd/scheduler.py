@@ -29,7 +29,10 @@ from .params import (LOCAL_RADII, LOCAL_WEIGHTS, MULTISCALE_REQUIRED_ALGOS,
                      normalize_topousm_radii_weights, overview_geometry, required_padding,
                      sanitize_spatial_radii_weights_for_tile, split_radii_by_threshold,
                      tile_grid, topousm_large_radius_threshold)
-from .raster_info import metric_pixel_scales, tile_geographic_scales
+from .raster_info import metric_pixel_scales, row_geographic_scales_x, tile_geographic_scales
+
+# algorithms whose gradient takes a per-row x cell width
+ROW_SCALED = ("hillshade", "slope", "curvature", "hillshade_slope")
 from .stats import norm_stats_prepass
 
 logger = logging.getLogger(__name__)
@@ -84,11 +87,13 @@ def plan(raster: torch.Tensor, algorithm: str, *, tile_size: Optional[int] = 409
          padding: Optional[int] = None, sigma: float = 10.0, multiscale_mode: bool = True,
          pixel_size: Optional[float] = None, transform: Optional[Sequence[float]] = None,
          is_geographic: Optional[bool] = False, nodata: Optional[float] = None,
-         **algo_params) -> RunPlan:
+         geographic_rows: bool = False, **algo_params) -> RunPlan:
     """Parameter resolution of process_dem_tiles (core/tile_processor.py:1050-1590).
 
     ``transform`` = affine (a, b, c, d, e, f); without it the raster is taken as
-    projected, north-up, with ``pixel_size`` cells."""
+    projected, north-up, with ``pixel_size`` cells.  ``geographic_rows`` (no
+    reference counterpart) gives a geographic raster's gradient algorithms the
+    x cell width of every row's own latitude instead of the tile centre's."""
     if algorithm not in ALGORITHMS:
         raise ValueError(f"Algorithm {algorithm} not found or not available on this platform")
     if algorithm not in SUPPORTED:
@@ -169,6 +174,7 @@ def plan(raster: torch.Tensor, algorithm: str, *, tile_size: Optional[int] = 409
                 algo_params["_topousm_fast_full_shape"] = (H, W)
 
     algo_params.setdefault("is_geographic_dem", bool(is_geographic))
+    algo_params["_geographic_rows"] = bool(geographic_rows)
     req = required_padding(algorithm, algo_params, sigma, float(pixel_size), None, int(tile_size))
     padding = req if padding is None else max(padding, req)
 
@@ -275,6 +281,8 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
             bottom = f + (cy + ch) * e
             p["pixel_scale_x"], p["pixel_scale_y"] = tile_geographic_scales(
                 a, e, 0.5 * (top + bottom))
+            if p.get("_geographic_rows") and algorithm in ROW_SCALED:
+                p["pixel_scale_x_rows"] = row_geographic_scales_x(a, e, f, int(wy), int(wh))
         win = src[wy:wy + wh, wx:wx + ww]
         reg = ops.region(cy - wy, cx - wx, ch, cw)
         spec = OutSpec(f32=o_view if dt == torch.float32 else f_view,

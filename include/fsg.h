@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 5
+#define FSG_ABI_VERSION 6
 
 enum {
   FSG_OK = 0,
@@ -138,6 +138,11 @@ typedef struct fsg_shade_params {
   double azimuth_deg, altitude_deg;
   int32_t slope_unit;    /* 0 degree, 1 percent, 2 radian (_impl_slope.py:27-33) */
   int32_t nan_hint;      /* 0: window known NaN-free (no fill path); else NaN-aware */
+  /* NULL, or a device array with the signed x step of every window row
+     (geographic grids with per-row latitude, metres per pixel); it replaces
+     step_x row by row (|value| here).  No reference counterpart: the
+     reference uses one latitude per tile (core/tile_processor.py:910-932). */
+  const double* step_x_rows;
 } fsg_shade_params;
 
 /*
@@ -160,6 +165,9 @@ typedef struct fsg_curvature_params {
   double step_x, step_y;
   int32_t curvature_type; /* 0 mean, 1 gaussian, 2 planform, 3 profile */
   int32_t nan_hint;       /* 0: window known NaN-free (no fill path); else NaN-aware */
+  /* NULL, or the signed x step of every window row (device array), as in
+     fsg_shade_params; first derivatives use |value|, second ones the value. */
+  const double* step_x_rows;
 } fsg_curvature_params;
 
 /* Replaces compute_curvature_block (_impl_curvature.py:19-57). */

@@ -64,23 +64,43 @@ def handle_nan_with_uniform(block: np.ndarray, size: int, mode: str = "nearest")
     return mean, nan_mask
 
 
+def _row_steps(v, pixel_size, signed):
+    """A per-row x scale array (no reference counterpart: geographic grids with
+    each row's own latitude) with the 1e-9 fallback applied row by row."""
+    a = np.asarray(v, dtype=np.float64)
+    fb = float(pixel_size if pixel_size else 1.0)
+    a = np.where(np.abs(a) < 1e-9, fb, a if signed else np.abs(a))
+    return a
+
+
 def resolve_steps(pixel_size=1.0, pixel_scale_x=None, pixel_scale_y=None):
-    """|step| resolution of handle_nan_for_gradient (_nan_utils.py:67-72)."""
+    """|step| resolution of handle_nan_for_gradient (_nan_utils.py:67-72).
+    pixel_scale_x may also be one value per row (returned as an array)."""
     step_y = abs(float(pixel_scale_y if pixel_scale_y is not None else pixel_size))
-    step_x = abs(float(pixel_scale_x if pixel_scale_x is not None else pixel_size))
     if step_y < 1e-9:
         step_y = float(pixel_size if pixel_size else 1.0)
+    if pixel_scale_x is not None and np.ndim(pixel_scale_x) == 1:
+        return _row_steps(pixel_scale_x, pixel_size, signed=False), step_y
+    step_x = abs(float(pixel_scale_x if pixel_scale_x is not None else pixel_size))
     if step_x < 1e-9:
         step_x = float(pixel_size if pixel_size else 1.0)
     return step_x, step_y
 
 
-def gradient_f32(f: np.ndarray, step_y: float, step_x: float):
+def gradient_f32(f: np.ndarray, step_y: float, step_x):
     """np/cp.gradient(f, step_y, step_x, edge_order=2) on float32 with the
     Python spacings as weak scalars (numpy function_base gradient, uniform
-    spacing branch).  Returns (d/drow, d/dcol)."""
+    spacing branch).  Returns (d/drow, d/dcol).  A 1-D step_x (one spacing per
+    row) takes each row's x derivative with its own spacing -- the same
+    formula row by row, so equal spacings give the scalar result bit for bit."""
     f = np.asarray(f, dtype=F32)
-    return tuple(np.gradient(f, float(step_y), float(step_x), edge_order=2))
+    if np.ndim(step_x) == 0:
+        return tuple(np.gradient(f, float(step_y), float(step_x), edge_order=2))
+    d_row = np.gradient(f, float(step_y), axis=0, edge_order=2)
+    d_col = np.empty_like(d_row)
+    for i in range(f.shape[0]):
+        d_col[i] = np.gradient(f[i], float(step_x[i]), edge_order=2)
+    return d_row, d_col
 
 
 def handle_nan_for_gradient(block, scale=1.0, pixel_size=1.0,
@@ -130,7 +150,9 @@ def hillshade(block, *, azimuth=DEFAULT_AZIMUTH, altitude=DEFAULT_ALTITUDE,
     dy, dx, nan_mask = handle_nan_for_gradient(
         block, scale=z_factor, pixel_size=pixel_size,
         pixel_scale_x=pixel_scale_x, pixel_scale_y=pixel_scale_y)
-    sign_x = 1.0 if (pixel_scale_x is None or float(pixel_scale_x) >= 0.0) else -1.0
+    sx0 = (pixel_scale_x if (pixel_scale_x is None or np.ndim(pixel_scale_x) == 0)
+           else np.asarray(pixel_scale_x).reshape(-1)[0])
+    sign_x = 1.0 if (sx0 is None or float(sx0) >= 0.0) else -1.0
     sign_y = 1.0 if (pixel_scale_y is None or float(pixel_scale_y) >= 0.0) else -1.0
     e = dx * F32(sign_x)
     n = dy * F32(sign_y)
@@ -164,11 +186,14 @@ def slope(block, *, unit="degree", pixel_size=1.0, pixel_scale_x=None, pixel_sca
 # ---------------------------------------------------------------------------
 
 def curvature_steps(pixel_size=1.0, pixel_scale_x=None, pixel_scale_y=None):
-    """Signed steps of compute_curvature_block (_impl_curvature.py:23-28)."""
+    """Signed steps of compute_curvature_block (_impl_curvature.py:23-28);
+    pixel_scale_x may be one value per row."""
     step_y = float(pixel_scale_y if pixel_scale_y is not None else pixel_size)
-    step_x = float(pixel_scale_x if pixel_scale_x is not None else pixel_size)
     if abs(step_y) < 1e-9:
         step_y = float(pixel_size if pixel_size else 1.0)
+    if pixel_scale_x is not None and np.ndim(pixel_scale_x) == 1:
+        return _row_steps(pixel_scale_x, pixel_size, signed=True), step_y
+    step_x = float(pixel_scale_x if pixel_scale_x is not None else pixel_size)
     if abs(step_x) < 1e-9:
         step_x = float(pixel_size if pixel_size else 1.0)
     return step_x, step_y

@@ -23,6 +23,10 @@ def _jobs(cuda, specs):
              "global_stats": (15.0 + i,)}
         if kind == "geo":
             p.update(pixel_scale_x=24.3, pixel_scale_y=-30.8)
+        if kind == "georows":   # every row's x cell width at its own latitude
+            from fujishadergpu_amd.raster_info import row_geographic_scales_x
+            p.update(pixel_scale_x_rows=row_geographic_scales_x(1 / 3600, -1 / 3600, 47.3, 0, h),
+                     pixel_scale_y=-30.8)
         if kind == "unsorted":
             p.update(radii=[32, 2, 128, 8], weights=[0.2, 0.4, 0.1, 0.3])
         e = lambda: torch.full((h, w), -7.0, dtype=torch.float32, device=cuda)   # noqa: E731
@@ -34,7 +38,7 @@ def _jobs(cuda, specs):
 
 SPECS = [(256, 256, "proj"), (300, 517, "geo"), (97, 1030, "proj"), (700, 333, "nan"),
          (64, 64, "proj"), (410, 410, "unsorted"), (129, 257, "no_usm"), (512, 96, "cv_only"),
-         (1000, 640, "geo")]
+         (1000, 640, "geo"), (333, 300, "georows")]
 
 
 def _outs(jobs):
@@ -67,7 +71,7 @@ def test_fused_batch_equals_per_tile_launches(cuda):
     info = b.info()
     print("batch info", info, flush=True)
     assert info["fallback_usm_tiles"] == 2              # unsorted radii + the NaN tile
-    assert info["fused_usm_tiles"] == 6 and info["box_radii"] == 3
+    assert info["fused_usm_tiles"] == 7 and info["box_radii"] == 3
     for cfg in [(3, 0, 8), (1, 1, 1), (8, 0, 32)]:
         b.run(cfg)
         torch.cuda.synchronize()

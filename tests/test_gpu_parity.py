@@ -60,6 +60,37 @@ def test_hillshade_slope_window(cuda, nan_discs, psx, psy):
     _cmp("slope", sl, ref.slope(a, **kw), 2e-5, 1e-6)
 
 
+@pytest.mark.parametrize("nan_discs", [0, 5])
+@pytest.mark.parametrize("lat", [31.0, 58.5])
+def test_stencils_with_per_row_x_spacing(cuda, nan_discs, lat):
+    # a geographic 1 arc-second window: every row's x cell width at its own latitude
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    from fujishadergpu_amd.raster_info import meters_per_degree, row_geographic_scales_x
+    dem = _dem(150, 170, 31, relief=1800.0, nan_discs=nan_discs)
+    g = dem.to(cuda)
+    rows = row_geographic_scales_x(1 / 3600, -1 / 3600, lat + 0.02, 0, 150)
+    psy = -meters_per_degree(lat)[1] / 3600
+    kw = dict(pixel_size=25.0, pixel_scale_y=psy)
+    a = dem.numpy()
+    hs = ALGORITHMS["hillshade"].process(g, pixel_scale_x_rows=rows, z_factor=1.5, **kw)
+    _cmp("hillshade/rows", hs, ref.hillshade(a, pixel_scale_x=rows, z_factor=1.5, **kw), 1e-6)
+    sl = ALGORITHMS["slope"].process(g, pixel_scale_x_rows=rows, **kw)
+    _cmp("slope/rows", sl, ref.slope(a, pixel_scale_x=rows, **kw), 2e-5, 1e-6)
+    for ctype in ("mean", "gaussian"):
+        cv = ALGORITHMS["curvature"].process(g, pixel_scale_x_rows=rows, curvature_type=ctype,
+                                             **kw).cpu().numpy()
+        exp = ref.curvature(a, pixel_scale_x=rows, curvature_type=ctype, **kw)
+        _cmp(f"curvature/rows/{ctype}", cv, exp, 4e-4)
+        tg = 2.0 * np.power(cv.astype(np.float64), 2.2) - 1.0
+        te = 2.0 * np.power(exp.astype(np.float64), 2.2) - 1.0
+        _cmp(f"curvature/rows/{ctype}/tanh", tg, te, 2e-6)
+    # equal rows give the scalar path's bits
+    same = np.full(150, float(rows[75]))
+    h1 = ALGORITHMS["hillshade"].process(g, pixel_scale_x_rows=same, **kw)
+    h2 = ALGORITHMS["hillshade"].process(g, pixel_scale_x=float(rows[75]), **kw)
+    assert torch.equal(h1.view(torch.int32), h2.view(torch.int32))    # NaN bits included
+
+
 @pytest.mark.parametrize("z,az,alt,unit", [(2.5, 45.0, 30.0, "percent"), (1.0, 200.0, 60.0, "radian")])
 def test_hillshade_slope_params(cuda, z, az, alt, unit):
     from fujishadergpu_amd.algorithms import ALGORITHMS

@@ -123,3 +123,35 @@ def test_scheduler_skips_all_nodata_tiles(cuda):
     assert skipped == [(0, 0), (1, 0)]
     assert int(out[:, :150].max()) == 0
     assert int(out[:, 214:].min()) >= 1
+
+
+@pytest.mark.parametrize("algorithm", ["slope", "hillshade", "curvature"])
+@pytest.mark.parametrize("rows", [False, True])
+def test_geographic_raster_tile_and_row_latitudes(cuda, algorithm, rows):
+    # a 1 arc-second geographic raster at 44-45 N: the reference's per-tile
+    # latitude (core/tile_processor.py:910-932), or each row's own latitude
+    from fujishadergpu_amd.raster_info import row_geographic_scales_x, tile_geographic_scales
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _raster(600, 700, 33, relief=900.0)
+    g = host_t.to(cuda)
+    a, e, f = 1 / 3600, -1 / 3600, 45.0
+    tr = (a, 0.0, 139.0, 0.0, e, f)
+    out, _, P = process_dem(g, algorithm, tile_size=256, transform=tr, is_geographic=True,
+                            geographic_rows=rows, mode="local", return_plan=True)
+    host = host_t.numpy()
+    want = np.full(host.shape, np.nan, np.float32)
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+        p = {k: v for k, v in P.params.items() if not isinstance(v, torch.Tensor)}
+        p["pixel_scale_x"], p["pixel_scale_y"] = tile_geographic_scales(
+            a, e, 0.5 * ((f + cy * e) + (f + (cy + ch) * e)))
+        if rows:
+            p["pixel_scale_x"] = row_geographic_scales_x(a, e, f, wy, wh)
+        res = ref.run_window(algorithm, host[wy:wy + wh, wx:wx + ww], p)
+        want[cy:cy + ch, cx:cx + cw] = ref.format_output(
+            res[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw], algorithm)
+    got = out.cpu().numpy()
+    if out.dtype != torch.float32:
+        assert np.abs(got.astype(np.int64) - want.astype(np.int64)).max() <= 1
+    else:
+        tol = {"slope": 3e-4, "hillshade": 1e-6, "curvature": 4e-4}[algorithm]
+        assert np.nanmax(np.abs(got - want)) <= tol

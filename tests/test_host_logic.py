@@ -73,3 +73,24 @@ def test_judge_sampling_hash_and_segments():
     # the draw depends on the seed and on the tile
     assert plan[0][1] != ops.judge_segments(1000, 700, 256, 256, 0.01, 8, 64)[0][1]
     assert plan[0][1][:3] != plan[1][1][:3]
+
+
+def test_row_scales_resolution():
+    import numpy as np
+    import pytest
+    from fujishadergpu_amd.algorithms import RowScales, x_steps_rows
+    from fujishadergpu_amd.raster_info import meters_per_degree, row_geographic_scales_x
+    rs = RowScales([0.0, 21.5, 1e-12, 22.0], "cpu", pixel_size=7.0)
+    assert rs.t.dtype.is_floating_point and rs.t.tolist() == [7.0, 21.5, 7.0, 22.0]
+    assert rs.first == 7.0
+    with pytest.raises(ValueError):
+        RowScales([1.0, -1.0], "cpu")
+    with pytest.raises(ValueError):
+        x_steps_rows({"pixel_scale_x_rows": [1.0, 2.0]}, 3, "cpu")
+    assert x_steps_rows({"pixel_scale_x_rows": rs}, 4, "cpu") is rs
+    # every row at its own latitude: row r of a 1 arc-second grid from 45 N
+    v = row_geographic_scales_x(1 / 3600, -1 / 3600, 45.0, 10, 3)
+    for k in range(3):
+        assert v[k] == pytest.approx(meters_per_degree(45.0 - (10 + k + 0.5) / 3600)[0] / 3600,
+                                     rel=1e-12)
+    assert np.all(np.diff(row_geographic_scales_x(1 / 3600, -1 / 3600, 45.0, 0, 100)) > 0)

@@ -95,6 +95,61 @@ def test_planform_zero_profile_nonzero_on_a_parabolic_cylinder():
     assert np.abs(prof - ZERO_DISP).max() > 0.05
 
 
+# ---------------------------------------------- per-row x spacing (geographic rows)
+
+def _rows_dem(h=40, w=50, seed=3):
+    rng = np.random.default_rng(seed)
+    return np.cumsum(np.cumsum(rng.normal(size=(h, w)), 0), 1).astype(F32) * F32(0.7)
+
+
+def test_equal_row_spacings_reproduce_the_scalar_stencils_bit_for_bit():
+    z = _rows_dem()
+    rows = np.full(z.shape[0], 23.5)
+    for fn, kw in ((ref.hillshade, {}), (ref.slope, {}), (ref.curvature, {}),
+                   (ref.curvature, {"curvature_type": "gaussian"})):
+        a = fn(z, pixel_scale_x=rows, pixel_scale_y=-30.0, **kw)
+        b = fn(z, pixel_scale_x=23.5, pixel_scale_y=-30.0, **kw)
+        assert np.array_equal(a, b, equal_nan=True), fn.__name__
+
+
+def test_hillshade_and_slope_rows_use_their_own_spacing():
+    # each pixel's hillshade/slope depends only on its own row's x spacing:
+    # row i equals the scalar-spacing result with step rows[i]
+    z = _rows_dem()
+    rows = 20.0 + 5.0 * np.cos(np.linspace(0.5, 1.0, z.shape[0]))
+    hs = ref.hillshade(z, pixel_scale_x=rows, pixel_scale_y=-31.0)
+    sl = ref.slope(z, pixel_scale_x=rows, pixel_scale_y=-31.0)
+    for i in (0, 1, 17, z.shape[0] - 1):
+        assert np.array_equal(hs[i], ref.hillshade(z, pixel_scale_x=rows[i],
+                                                   pixel_scale_y=-31.0)[i])
+        assert np.array_equal(sl[i], ref.slope(z, pixel_scale_x=rows[i], pixel_scale_y=-31.0)[i])
+
+
+def test_slope_of_a_column_ramp_is_atan_of_the_row_gradient():
+    # z = a j: every row's metric gradient is a / sx_i exactly
+    h, w, a = 12, 20, 3.0
+    z = np.tile((a * np.arange(w)).astype(F32), (h, 1))
+    rows = np.linspace(15.0, 25.0, h)
+    sl = ref.slope(z, pixel_scale_x=rows, pixel_scale_y=-30.0)
+    want = np.degrees(np.arctan(a / rows))[:, None] * np.ones((1, w))
+    assert np.allclose(sl, want, rtol=1e-5, atol=0)   # float32 gradient and atan
+
+
+def test_mean_curvature_of_a_column_profile_is_row_local():
+    # z = g(col): dy = 0 on the interior rows (the edge_order=2 edge formula
+    # leaves a float32 residue on the first and last), so the mean curvature
+    # of interior row i is the scalar result with x step rows[i]: the mixed
+    # term is multiplied by dy = 0
+    h, w = 10, 30
+    g = (0.02 * (np.arange(w) - 13.0) ** 3).astype(F32)
+    z = np.tile(g, (h, 1))
+    rows = np.linspace(18.0, 26.0, h)
+    cv = ref.curvature(z, pixel_scale_x=rows, pixel_scale_y=-30.0)
+    for i in range(1, h - 1):
+        assert np.array_equal(cv[i], ref.curvature(z, pixel_scale_x=rows[i],
+                                                   pixel_scale_y=-30.0)[i])
+
+
 def test_dome_has_plan_and_profile_curvature():
     # reference tests/test_curvature_analytic.py:44-54
     y, x = _grid()
