@@ -18,6 +18,7 @@
 // enqueued before the persistent kernel; the whole sequence can be captured
 // once in a hipGraph and replayed.
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -126,6 +127,18 @@ struct fsg_batch {
   int* d_queue = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // coarse preparation of the fused TopoUSM tiles: batched launches (decimations,
+  // then per coarse-radius level the two box passes), or per tile when a tile's
+  // preparation has no batched form
+  struct PrepLaunch {
+    int kind;     // 0 decimation, 1 box pass 0, 2 box pass 1
+    const void* jobs;
+    const int* pre;
+    int n_jobs, n_blocks;
+  };
+  std::vector<int> prep_per_tile;
+  std::vector<PrepLaunch> prep;
+  char* d_prep = nullptr;
 };
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -137,6 +150,77 @@ static bool wants_stencil(const fsg_batch_tile& t) {
   return t.hillshade.data || t.slope.data || t.curvature.data;
 }
 static const fsg_output* opt(const fsg_output& o) { return o.data ? &o : nullptr; }
+
+// Job tables of the batched coarse preparation, in one device allocation.
+static int plan_prep(fsg_batch* b) {
+  std::vector<DecJob> dec;
+  std::vector<std::vector<BoxJob>> box0, box1;
+  for (int i : b->usm_fused) {
+    const fsg_batch_tile& t = b->tiles[i];
+    std::vector<DecJob> d1;
+    std::vector<std::vector<BoxJob>> a0, a1;
+    if (usm_prep_jobs(&t.win, &t.usm, b->dev + b->ws_off[i], d1, a0, a1) != FSG_OK) {
+      b->prep_per_tile.push_back(i);
+      continue;
+    }
+    dec.insert(dec.end(), d1.begin(), d1.end());
+    for (size_t L = 0; L < a0.size(); L++) {
+      if (box0.size() <= L) {
+        box0.emplace_back();
+        box1.emplace_back();
+      }
+      box0[L].insert(box0[L].end(), a0[L].begin(), a0[L].end());
+      box1[L].insert(box1[L].end(), a1[L].begin(), a1[L].end());
+    }
+  }
+  // host images of the tables: [jobs][prefix] per launch
+  struct Part {
+    int kind;
+    std::vector<char> jobs;
+    std::vector<int> pre;
+  };
+  std::vector<Part> parts;
+  auto add_dec = [&](const std::vector<DecJob>& v) {
+    if (v.empty()) return;
+    Part P{0, std::vector<char>(sizeof(DecJob) * v.size()), {0}};
+    std::memcpy(P.jobs.data(), v.data(), P.jobs.size());
+    for (const DecJob& J : v) P.pre.push_back(P.pre.back() + dec_blocks(J));
+    parts.push_back(std::move(P));
+  };
+  auto add_box = [&](int kind, const std::vector<BoxJob>& v) {
+    if (v.empty()) return;
+    Part P{kind, std::vector<char>(sizeof(BoxJob) * v.size()), {0}};
+    std::memcpy(P.jobs.data(), v.data(), P.jobs.size());
+    for (const BoxJob& J : v) P.pre.push_back(P.pre.back() + box_blocks(J));
+    parts.push_back(std::move(P));
+  };
+  add_dec(dec);
+  for (size_t L = 0; L < box0.size(); L++) {
+    add_box(1, box0[L]);
+    add_box(2, box1[L]);
+  }
+  if (parts.empty()) return FSG_OK;
+  size_t bytes = 0;
+  for (const Part& P : parts) bytes += al256(P.jobs.size()) + al256(sizeof(int) * P.pre.size());
+  if (int rc = check_hip(hipMalloc((void**)&b->d_prep, bytes), "fsg_batch_create")) return rc;
+  size_t off = 0;
+  for (const Part& P : parts) {
+    char* jobs = b->d_prep + off;
+    off += al256(P.jobs.size());
+    int* pre = (int*)(b->d_prep + off);
+    off += al256(sizeof(int) * P.pre.size());
+    int rc = check_hip(hipMemcpy(jobs, P.jobs.data(), P.jobs.size(), hipMemcpyHostToDevice),
+                       "fsg_batch_create");
+    if (!rc)
+      rc = check_hip(hipMemcpy(pre, P.pre.data(), sizeof(int) * P.pre.size(),
+                               hipMemcpyHostToDevice), "fsg_batch_create");
+    if (rc) return rc;
+    if (P.pre.back() <= 0) continue;
+    b->prep.push_back(fsg_batch::PrepLaunch{P.kind, jobs, pre, (int)P.pre.size() - 1,
+                                            P.pre.back()});
+  }
+  return FSG_OK;
+}
 
 extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batch** out) {
   if (!out) return fail(FSG_E_INVALID, "fsg_batch_create: out NULL");
@@ -287,6 +371,7 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   if (!rc)
     rc = check_hip(hipMemcpy(b->d_st_pre, st_pre.data(), sizeof(int) * st_pre.size(),
                              hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc) rc = plan_prep(b);
   if (rc) {
     fsg_batch_destroy(b);
     return rc;
@@ -301,8 +386,16 @@ static void launch_fused(const BatchDev& D, int grid, size_t lds, hipStream_t s)
 }
 
 static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t s) {
-  // coarse grids of the fused TopoUSM tiles (decimation, coarse box means)
-  for (int i : b->usm_fused) {
+  // coarse grids of the fused TopoUSM tiles (decimation, coarse box means):
+  // batched launches, and per-tile ones for tiles without a batched form
+  for (const fsg_batch::PrepLaunch& L : b->prep) {
+    const int rc = L.kind == 0
+                       ? launch_dec_batch((const DecJob*)L.jobs, L.pre, L.n_jobs, L.n_blocks, s)
+                       : launch_box_batch(L.kind - 1, (const BoxJob*)L.jobs, L.pre, L.n_jobs,
+                                          L.n_blocks, s);
+    if (rc) return rc;
+  }
+  for (int i : b->prep_per_tile) {
     const fsg_batch_tile& t = b->tiles[i];
     UsmArgs A;
     bool fast = false;
@@ -427,5 +520,6 @@ extern "C" void fsg_batch_destroy(fsg_batch* b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   if (b->dev) hipFree(b->dev);
+  if (b->d_prep) hipFree(b->d_prep);
   delete b;
 }

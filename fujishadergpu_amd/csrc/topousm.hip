@@ -71,10 +71,11 @@ __device__ __noinline__ float gauss9_mean(const float* p, int64_t ld, int h, int
 // 256 consecutive source columns of one coarse row (f source rows): each lane
 // sums its column, then groups of f lanes reduce by shuffles (f is a power of
 // two <= 16).  Cell sums are exact in float64, so the order does not matter.
-__global__ void __launch_bounds__(256)
-k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __restrict__ nan_flag) {
-  const int orow = blockIdx.y;
-  const int c = blockIdx.x * 256 + threadIdx.x;  // source column
+__device__ __forceinline__ void decimate_block(const WinDev& W, int f, float* __restrict__ out,
+                                               int oh, int ow, int* __restrict__ nan_flag,
+                                               int bx, int by) {
+  const int orow = by;
+  const int c = bx * 256 + threadIdx.x;  // source column
   const int r0 = orow * f, r1 = min(r0 + f, W.h);
   double tot = 0.0;
   int cnt = 0;
@@ -99,11 +100,36 @@ k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __rest
         res = (float)tot / (float)cnt;
       } else {
         res = __builtin_nanf("");
-        atomicOr(nan_flag, 1);
+        if (nan_flag) atomicOr(nan_flag, 1);
       }
       out[(int64_t)orow * ow + oc] = res;
     }
   }
+}
+
+__global__ void __launch_bounds__(256)
+k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __restrict__ nan_flag) {
+  decimate_block(W, f, out, oh, ow, nan_flag, blockIdx.x, blockIdx.y);
+}
+
+// Several independent decimations in one launch (the coarse preparation of a
+// batch of NaN-free windows): flat block index -> job by binary search over
+// the jobs' block-count prefix, then the per-window kernel's block.
+__device__ __forceinline__ int batch_job_of(const int* __restrict__ pre, int n, int g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256)
+k_decimate_batch(const DecJob* __restrict__ jobs, const int* __restrict__ pre, int n) {
+  const int j = batch_job_of(pre, n, blockIdx.x);
+  const DecJob J = jobs[j];
+  const int local = blockIdx.x - pre[j];
+  decimate_block(J.W, J.f, J.out, J.oh, J.ow, nullptr, local % J.nbx, local / J.nbx);
 }
 
 // Separable Gaussian pass of the enclosed-void fill, mode 'nearest', scipy
@@ -179,14 +205,14 @@ __global__ void k_voidfill_apply(float* __restrict__ coarse, const float* __rest
 //   pass 1: tv^T, tw^T (w x h)    -> out (h x w): axis-1 means of those and
 //           where(sw > 0, sv / sw, 0)   (_nan_utils.py:34-47 structure)
 template <int PASS>
-__global__ void __launch_bounds__(64)
-k_box_t(const float* __restrict__ a, const float* __restrict__ b, int64_t lda, int h, int w,
-        int r, double inv, float* __restrict__ oa, float* __restrict__ ob, int64_t ldo) {
-  __shared__ float ta[64][65];
-  __shared__ float tb[64][65];
+__device__ __forceinline__ void box_t_block(const float* __restrict__ a,
+                                            const float* __restrict__ b, int64_t lda, int h,
+                                            int w, int r, double inv, float* __restrict__ oa,
+                                            float* __restrict__ ob, int64_t ldo, int bx, int by,
+                                            float (*ta)[65], float (*tb)[65]) {
   const int lane = threadIdx.x;
-  const int c = blockIdx.x * 64 + lane;  // column of the (h x w) input
-  const int y0 = blockIdx.y * 64;
+  const int c = bx * 64 + lane;  // column of the (h x w) input
+  const int y0 = by * 64;
   const int y1 = min(y0 + 64, h);
   if (c < w) {
     const float* pa = a + c;
@@ -251,15 +277,37 @@ k_box_t(const float* __restrict__ a, const float* __restrict__ b, int64_t lda, i
   }
   __syncthreads();
   // transposed write: output row = input column, output column = input row
-  const int ncol = min(64, w - blockIdx.x * 64);
+  const int ncol = min(64, w - bx * 64);
   for (int cc = 0; cc < ncol; cc++) {
     const int y = y0 + lane;
     if (y < y1) {
-      const int64_t o = (int64_t)(blockIdx.x * 64 + cc) * ldo + y;
+      const int64_t o = (int64_t)(bx * 64 + cc) * ldo + y;
       oa[o] = ta[lane][cc];
       if (PASS == 0) ob[o] = tb[lane][cc];
     }
   }
+}
+
+template <int PASS>
+__global__ void __launch_bounds__(64)
+k_box_t(const float* __restrict__ a, const float* __restrict__ b, int64_t lda, int h, int w,
+        int r, double inv, float* __restrict__ oa, float* __restrict__ ob, int64_t ldo) {
+  __shared__ float ta[64][65];
+  __shared__ float tb[64][65];
+  box_t_block<PASS>(a, b, lda, h, w, r, inv, oa, ob, ldo, blockIdx.x, blockIdx.y, ta, tb);
+}
+
+// Box passes of several independent grids in one launch (see k_decimate_batch).
+template <int PASS>
+__global__ void __launch_bounds__(64)
+k_box_t_batch(const BoxJob* __restrict__ jobs, const int* __restrict__ pre, int n) {
+  __shared__ float ta[64][65];
+  __shared__ float tb[64][65];
+  const int j = batch_job_of(pre, n, blockIdx.x);
+  const BoxJob J = jobs[j];
+  const int local = blockIdx.x - pre[j];
+  box_t_block<PASS>(J.a, J.b, J.lda, J.h, J.w, J.r, J.inv, J.oa, J.ob, J.ldo, local % J.nbx,
+                    local / J.nbx, ta, tb);
 }
 
 __global__ void k_gauss9_grid(const float* __restrict__ x, int64_t ldx, int h, int w,
@@ -970,6 +1018,66 @@ int fsg::usm_prepare(const fsg_window* win, const fsg_region* reg, const fsg_top
   *A_out = A;
   *fast_out = fast;
   *n_blocks_out = A.n_strips * n_segs;
+  return FSG_OK;
+}
+
+int fsg::usm_prep_jobs(const fsg_window* win, const fsg_topousm_params* p, void* workspace,
+                       std::vector<DecJob>& dec, std::vector<std::vector<BoxJob>>& box0,
+                       std::vector<std::vector<BoxJob>>& box1) {
+  const CoarsePlan P = plan_coarse(win, p);
+  char* ws = (char*)workspace;
+  // same order and buffers as usm_prepare's launch_prep branches
+  for (int j = 0; j < P.n_ds; j++)
+    if (P.ds_val[j] > 16 || (P.ds_val[j] & (P.ds_val[j] - 1)) != 0) return FSG_E_UNSUPPORTED;
+  for (int i = 0; i < p->n_radii; i++) {
+    if (p->ds[i] <= 1 || p->mean[i]) continue;
+    const int r_small = std::max(1, (int)std::nearbyint((double)p->radii[i] / (double)p->ds[i]));
+    if (r_small <= 1) return FSG_E_UNSUPPORTED;
+  }
+  const WinDev W{win->data, win->ld, win->h, win->w};
+  for (int j = 0; j < P.n_ds; j++)
+    dec.push_back(DecJob{W, P.ds_val[j], (float*)(ws + P.grid_off[j]), P.hc[j], P.wc[j],
+                         (win->w + 255) / 256});
+  float* tv = (float*)(ws + P.tmp_off[0]);
+  float* tw = (float*)(ws + P.tmp_off[1]);
+  int level = 0;
+  for (int i = 0; i < p->n_radii; i++) {
+    if (p->ds[i] <= 1 || p->mean[i]) continue;
+    const int j = P.ds_of_radius[i];
+    const int r = std::max(1, (int)std::nearbyint((double)p->radii[i] / (double)p->ds[i]));
+    const int h = P.hc[j], w = P.wc[j];
+    const double inv = 1.0 / (double)(2 * r + 1);
+    const float* grid = (const float*)(ws + P.grid_off[j]);
+    float* m = (float*)(ws + P.mean_off[i]);
+    if ((int)box0.size() <= level) {
+      box0.emplace_back();
+      box1.emplace_back();
+    }
+    box0[level].push_back(BoxJob{grid, nullptr, (int64_t)w, h, w, r, inv, tv, tw, (int64_t)h,
+                                 (w + 63) / 64});
+    box1[level].push_back(BoxJob{tv, tw, (int64_t)h, w, h, r, inv, m, nullptr, (int64_t)w,
+                                 (h + 63) / 64});
+    level++;
+  }
+  return FSG_OK;
+}
+
+int fsg::launch_dec_batch(const DecJob* jobs, const int* pre, int n_jobs, int n_blocks,
+                          hipStream_t s) {
+  if (n_jobs <= 0 || n_blocks <= 0) return FSG_OK;
+  hipLaunchKernelGGL(k_decimate_batch, dim3(n_blocks), dim3(256), 0, s, jobs, pre, n_jobs);
+  FSG_CHECK_LAUNCH("k_decimate_batch");
+  return FSG_OK;
+}
+
+int fsg::launch_box_batch(int pass, const BoxJob* jobs, const int* pre, int n_jobs, int n_blocks,
+                          hipStream_t s) {
+  if (n_jobs <= 0 || n_blocks <= 0) return FSG_OK;
+  if (pass == 0)
+    hipLaunchKernelGGL(k_box_t_batch<0>, dim3(n_blocks), dim3(64), 0, s, jobs, pre, n_jobs);
+  else
+    hipLaunchKernelGGL(k_box_t_batch<1>, dim3(n_blocks), dim3(64), 0, s, jobs, pre, n_jobs);
+  FSG_CHECK_LAUNCH("k_box_t_batch");
   return FSG_OK;
 }
 

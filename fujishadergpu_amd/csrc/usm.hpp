@@ -4,6 +4,7 @@
 // region.
 #pragma once
 #include <cmath>
+#include <vector>
 
 #include "fsg_common.hpp"
 
@@ -343,5 +344,40 @@ int usm_prepare(const fsg_window* win, const fsg_region* reg, const fsg_topousm_
                 const fsg_output* out, const fsg_output* out2, void* workspace,
                 size_t workspace_bytes, hipStream_t s, UsmArgs* A, bool* fast, int* n_blocks,
                 bool launch_prep = true);
+
+// Coarse preparation of many NaN-free windows as a few batched launches: the
+// cell-mean decimations, then per coarse-radius level the two box passes.
+struct DecJob {
+  WinDev W;
+  int f;
+  float* out;
+  int oh, ow;
+  int nbx;   // blocks per coarse row
+};
+struct BoxJob {
+  const float* a;
+  const float* b;
+  int64_t lda;
+  int h, w, r;
+  double inv;
+  float* oa;
+  float* ob;
+  int64_t ldo;
+  int nbx;   // blocks per 64-row band
+};
+inline int dec_blocks(const DecJob& J) { return J.nbx * J.oh; }
+inline int box_blocks(const BoxJob& J) { return J.nbx * ((J.h + 63) / 64); }
+
+// The jobs the coarse preparation of this NaN-free window launches one by one
+// (decimations in dec; level L's box passes in box0[L] / box1[L]).  Returns
+// FSG_E_UNSUPPORTED when a coarse radius takes the Gaussian branch (r <= 1),
+// which stays on the per-window path.
+int usm_prep_jobs(const fsg_window* win, const fsg_topousm_params* p, void* workspace,
+                  std::vector<DecJob>& dec, std::vector<std::vector<BoxJob>>& box0,
+                  std::vector<std::vector<BoxJob>>& box1);
+int launch_dec_batch(const DecJob* jobs, const int* pre, int n_jobs, int n_blocks,
+                     hipStream_t s);
+int launch_box_batch(int pass, const BoxJob* jobs, const int* pre, int n_jobs, int n_blocks,
+                     hipStream_t s);
 
 }  // namespace fsg
