@@ -32,6 +32,7 @@ struct OpenTab {
   int n_dirs, n_dist;
   int negative;
   int raw;         // 1: mean angle in degrees instead of the display value
+  int dmax_y, dmax_x;  // largest |offset| on each axis (interior test)
   float half_pi;   // f32(pi/2)
   float gamma;     // f32(1/2.2)
   short2 off[OPEN_MAX];
@@ -69,6 +70,85 @@ __device__ __forceinline__ float openness_angle(const WinDevR& W, int i, int j, 
   return angle_sum / fmaxf(dir_count, 1.0f);
 }
 
+// The same angle with the distances of a direction unrolled (n_dist <= 10,
+// the reference's table): all loads of a direction are issued before any is
+// used, and validity is a select, not a branch.  CHECK = false is the
+// interior variant (every sample inside the window, no bounds test); with
+// CHECK the sample address is clamped and the sample masked out.  The float32
+// operations and their order are those of openness_angle, so the results are
+// the same bits.
+constexpr int OPEN_UNROLL = 10;
+
+template <int ND, bool CHECK>
+__device__ __forceinline__ float openness_angle_u(const WinDevR& W, int i, int j, float z,
+                                                  const OpenTab& T) {
+  // ND = 10: the table row of a direction is a fixed 10 entries (wide scalar
+  // loads, no guards); ND = 0: guarded by the runtime n_dist (<= 10)
+  constexpr int NU = ND > 0 ? ND : OPEN_UNROLL;
+  const int nd = ND > 0 ? ND : T.n_dist;
+  const float* pc = W.p + (int64_t)i * W.ld + j;   // this pixel
+  float angle_sum = 0.0f, dir_count = 0.0f;
+  for (int d = 0; d < T.n_dirs; d++) {
+    float v[NU];
+    bool ok[NU];
+#pragma unroll
+    for (int k = 0; k < NU; k++) {
+      ok[k] = false;
+      v[k] = 0.0f;
+      if (k < nd) {
+        const short2 o = T.off[d * nd + k];
+        bool in = !(o.x == 0 && o.y == 0);
+        if (CHECK) {
+          const int si = i + o.y, sj = j + o.x;
+          in = in && si >= 0 && si < W.h && sj >= 0 && sj < W.w;
+          v[k] = W.p[(int64_t)min(max(si, 0), W.h - 1) * W.ld + min(max(sj, 0), W.w - 1)];
+        } else {
+          // wave-uniform element offset: scalar arithmetic, one address add per lane
+          v[k] = pc[(int64_t)o.y * W.ld + o.x];
+        }
+        ok[k] = in;
+      }
+    }
+    float ext = T.negative ? INFINITY : -INFINITY;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NU; k++) {
+      if (k < nd) {
+        const bool valid = ok[k] && !isnan(v[k]);
+        const float q = (v[k] - z) * T.inv[d * nd + k];
+        const float nx = T.negative ? fminf(ext, q) : fmaxf(ext, q);
+        ext = valid ? nx : ext;
+        any = any || valid;
+      }
+    }
+    if (any) {
+      const float a = atanf(ext);
+      const float da = T.negative ? (T.half_pi + a) : (T.half_pi - a);
+      angle_sum = angle_sum + da;
+      dir_count = dir_count + 1.0f;
+    }
+  }
+  return angle_sum / fmaxf(dir_count, 1.0f);
+}
+
+// Mean angle at (i, j): the unrolled variant when the table allows it.
+// interior: the caller knows every sample of (i, j) lies inside the window.
+__device__ __forceinline__ float openness_angle_at(const WinDevR& W, int i, int j, float z,
+                                                   const OpenTab& T, bool interior) {
+  if (T.n_dist == OPEN_UNROLL)
+    return interior ? openness_angle_u<OPEN_UNROLL, false>(W, i, j, z, T)
+                    : openness_angle_u<OPEN_UNROLL, true>(W, i, j, z, T);
+  if (T.n_dist > OPEN_UNROLL) return openness_angle(W, i, j, z, T);
+  return interior ? openness_angle_u<0, false>(W, i, j, z, T)
+                  : openness_angle_u<0, true>(W, i, j, z, T);
+}
+
+// every sample of pixels in rows [i0, i1] x cols [j0, j1] lies inside the window
+__device__ __forceinline__ bool openness_interior(const WinDevR& W, const OpenTab& T, int i0,
+                                                  int i1, int j0, int j1) {
+  return i0 - T.dmax_y >= 0 && i1 + T.dmax_y < W.h && j0 - T.dmax_x >= 0 && j1 + T.dmax_x < W.w;
+}
+
 // display value (op / (pi/2), clipped, gamma) or, with T.raw, the angle in degrees
 __device__ __forceinline__ float openness_value(const OpenTab& T, float angle) {
   if (T.raw) return (float)((double)angle * (180.0 / M_PI));
@@ -81,6 +161,10 @@ __global__ void __launch_bounds__(256)
 k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y * blockDim.y + threadIdx.y;
+  // block-uniform: do all samples of the block's pixels stay inside the window?
+  const int bi0 = r0 + blockIdx.y * blockDim.y, bj0 = c0 + blockIdx.x * blockDim.x;
+  const bool interior = openness_interior(W, T, bi0, bi0 + blockDim.y - 1, bj0,
+                                          bj0 + blockDim.x - 1);
   if (c >= rw || r >= rh) return;
   const int i = r0 + r, j = c0 + c;
   const PostK pp = load_post(out);
@@ -89,7 +173,8 @@ k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
     store_out(out, r, c, apply_post(pp, __builtin_nanf("")));
     return;
   }
-  store_out(out, r, c, apply_post(pp, openness_value(T, openness_angle(W, i, j, z, T))));
+  store_out(out, r, c,
+            apply_post(pp, openness_value(T, openness_angle_at(W, i, j, z, T, interior))));
 }
 
 // The same at a list of window pixels (row, col), float32 out[n].
@@ -100,7 +185,8 @@ k_openness_points(WinDevR W, const int2* __restrict__ pts, int n, OpenTab T,
   if (e >= n) return;
   const int2 q = pts[e];
   const float z = W.p[(int64_t)q.x * W.ld + q.y];
-  out[e] = isnan(z) ? __builtin_nanf("") : openness_value(T, openness_angle(W, q.x, q.y, z, T));
+  out[e] = isnan(z) ? __builtin_nanf("")
+                    : openness_value(T, openness_angle_at(W, q.x, q.y, z, T, false));
 }
 
 // Sampled exact check of an approximate openness-angle raster (the learned
@@ -147,12 +233,14 @@ k_openness_judge(WinDevR W, OpenTab T, const float* __restrict__ approx, int64_t
   for (int s = split * 4 + wave; s < n_seg; s += JUDGE_SPLIT * 4) {
     const uint64_t h = splitmix64(key + (uint64_t)s);
     const int i = y0 + (int)((h >> 32) % (uint64_t)th);
-    const int j = x0 + (int)((h & 0xffffffffull) % (uint64_t)(tw - G.L + 1)) + lane;
+    const int jseg = x0 + (int)((h & 0xffffffffull) % (uint64_t)(tw - G.L + 1));
+    const int j = jseg + lane;
+    const bool interior = openness_interior(W, T, i, i, jseg, jseg + G.L - 1);
     if (lane < G.L) {
       const float z = W.p[(int64_t)i * W.ld + j];
       const float a = approx[(int64_t)i * lda + j];
       if (!isnan(z) && !isnan(a)) {
-        const float v = openness_value(T, openness_angle(W, i, j, z, T));
+        const float v = openness_value(T, openness_angle_at(W, i, j, z, T, interior));
         sum += fabsf(v - a);
         cnt += 1.0f;
       }
@@ -311,6 +399,8 @@ static int openness_table(const fsg_openness_params* p, OpenTab& T, const char* 
       return fail(FSG_E_UNSUPPORTED, std::string(who) + ": offset beyond int16");
     T.off[e] = make_short2((short)s.dx, (short)s.dy);
     T.inv[e] = 1.0f / (float)s.phys;
+    T.dmax_x = std::max(T.dmax_x, std::abs(s.dx));
+    T.dmax_y = std::max(T.dmax_y, std::abs(s.dy));
   }
   return FSG_OK;
 }
