@@ -165,6 +165,18 @@ def bench_c1(dev):
                                   out_slope=OutSpec(f32=sl))
     run()
     t = _gpu_time(run, 50)
+    # the launch's own duration (HIP events on its stream), without host overhead
+    from fujishadergpu_amd import _native as N
+    L = N.lib()
+    tot, cnt = ctypes.c_double(), ctypes.c_int64()
+    L.fsg_timing_read(b"k_hillshade_slope", ctypes.byref(tot), ctypes.byref(cnt))
+    L.fsg_timing_enable(1)
+    for _ in range(50):
+        run()
+    torch.cuda.synchronize()
+    L.fsg_timing_enable(0)
+    N.check(L.fsg_timing_read(b"k_hillshade_slope", ctypes.byref(tot), ctypes.byref(cnt)))
+    tk = tot.value / 1e3 / max(1, cnt.value)
     from oracle import ref
     host = dem.cpu().numpy()
     t0 = time.perf_counter()
@@ -176,7 +188,9 @@ def bench_c1(dev):
     return {"workload": "hillshade+slope fused, local, 2048x2048 f32 fBm (H=0.8, 0-3000 m, 10 m, "
                         "seed 1), no NoData", "gpix_s": round(px / t / 1e9, 3),
             "ms": round(t * 1e3, 4), "hbm_frac": round(12 * px / t / (HBM_PEAK_GBS * 1e9), 4),
-            "alg_bytes_per_px": 12,
+            "alg_bytes_per_px": 12, "kernel_ms": round(tk * 1e3, 4),
+            "kernel_gpix_s": round(px / tk / 1e9, 3),
+            "kernel_hbm_frac": round(12 * px / tk / (HBM_PEAK_GBS * 1e9), 4),
             "cpu_baseline": {"value": round(px / tc / 1e9, 4), "unit": "Gpix/s", "cores": 1,
                              "kind": "port", "sample": "whole 2048^2 raster, oracle hillshade + "
                                                        "slope (NumPy, one thread of calls)"},

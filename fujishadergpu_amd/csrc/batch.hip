@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
 
 // NaN-aware stencil tiles of a batch, one launch per tile.
 __global__ void __launch_bounds__(256) k_stencil_nanaw(StencilTile S, Gauss9 G) {
-  __shared__ float lds[ST_LDS_FLOATS];
+  __shared__ float lds[ST_LDS_FLOATS_ROWS];
   stencil_tile_block<true>(S, blockIdx.y, blockIdx.x, G, lds);
 }
 
@@ -304,7 +304,7 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   b->nr = best;
   std::vector<UsmArgs> fused_args;
   std::vector<int> usm_pre{0};
-  size_t lds = sizeof(float) * ST_LDS_FLOATS;
+  size_t lds = sizeof(float) * ST_LDS_FLOATS_ROWS;
   for (int i = 0; i < n; i++) {
     if (!wants_usm(b->tiles[i])) continue;
     if (fast_ok[i] && args[i].nr == best) {

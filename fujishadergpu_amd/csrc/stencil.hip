@@ -8,20 +8,56 @@
 
 namespace fsg {
 
-// One workgroup per 64 x 16 block of the region (stencil.hpp).
-template <bool NANAW>
+// One workgroup per 64 x 16 block of the region (stencil.hpp); ROWS: the
+// window has per-row x steps (larger LDS for the coefficient tables).
+template <bool NANAW, bool ROWS>
 __global__ void __launch_bounds__(256) k_stencil_tiles(StencilTile S, Gauss9 G) {
-  __shared__ float lds[ST_LDS_FLOATS];
-  stencil_tile_block<NANAW>(S, blockIdx.y, blockIdx.x, G, lds);
+  __shared__ float lds[ROWS ? ST_LDS_FLOATS_ROWS : ST_LDS_FLOATS];
+  stencil_tile_block_v<NANAW, ROWS>(S, blockIdx.y, blockIdx.x, G, lds);
+}
+
+// Hillshade and/or slope without curvature and with constant steps: one lane
+// per pixel, the gradient read straight from global memory (L1/L2 serve the
+// neighbours); the same per-pixel operations as the tiled block, so the same
+// bits, without its LDS staging and barriers.
+template <bool NANAW>
+__global__ void __launch_bounds__(256) k_shade_direct(StencilTile S, Gauss9 G) {
+  const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (rr >= S.rh || cc >= S.rw) return;
+  const int H = S.W.h, Wd = S.W.w;
+  const int wr = S.r0 + rr, wc = S.c0 + cc;
+  const float z = raw(S.W, wr, wc);
+  const PostK phs = load_post(S.hs), psl = load_post(S.sl);
+  if (isnan(z)) {
+    const float qn = __builtin_nanf("");
+    if (S.want_hs) store_out(S.hs, rr, cc, apply_post(phs, qn));
+    if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, qn));
+    return;
+  }
+  auto f = [&](int r, int c) { return filled<NANAW>(S.W, r, c, G); };
+  float dx, dy;
+  grad_at(S.shade.ax, S.shade.ay, wr, wc, H, Wd, f, dx, dy);
+  if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, slope_from(S.shade, dx, dy)));
+  if (S.want_hs)
+    store_out(S.hs, rr, cc,
+              apply_post(phs, hillshade_from(S.shade, S.shade.ax, wr, wc, H, Wd, f, dx, dy)));
 }
 
 static int launch_stencil(const StencilTile& S, const char* name, hipStream_t s) {
   dim3 grd(S.nbx, (S.rh + ST_TH - 1) / ST_TH);
   TimedScope ts(name, s);
-  if (S.nanaw)
-    hipLaunchKernelGGL(k_stencil_tiles<true>, grd, dim3(256), 0, s, S, make_gauss9());
-  else
-    hipLaunchKernelGGL(k_stencil_tiles<false>, grd, dim3(256), 0, s, S, make_gauss9());
+  if (S.xrows) {
+    if (S.nanaw)
+      hipLaunchKernelGGL((k_stencil_tiles<true, true>), grd, dim3(256), 0, s, S, make_gauss9());
+    else
+      hipLaunchKernelGGL((k_stencil_tiles<false, true>), grd, dim3(256), 0, s, S, make_gauss9());
+  } else {
+    if (S.nanaw)
+      hipLaunchKernelGGL((k_stencil_tiles<true, false>), grd, dim3(256), 0, s, S, make_gauss9());
+    else
+      hipLaunchKernelGGL((k_stencil_tiles<false, false>), grd, dim3(256), 0, s, S, make_gauss9());
+  }
   FSG_CHECK_LAUNCH(name);
   return FSG_OK;
 }
@@ -68,6 +104,17 @@ extern "C" int fsg_hillshade_slope(const fsg_window* win, const fsg_region* reg,
   S.nanaw = p->nan_hint != 0;
   if (hillshade) S.hs = make_out(hillshade);
   if (slope) S.sl = make_out(slope);
+  if (!S.xrows) {
+    hipStream_t s = as_stream(stream);
+    dim3 grd((reg->w + 63) / 64, (reg->h + 3) / 4);
+    TimedScope ts("k_hillshade_slope", s);
+    if (S.nanaw)
+      hipLaunchKernelGGL(k_shade_direct<true>, grd, dim3(256), 0, s, S, make_gauss9());
+    else
+      hipLaunchKernelGGL(k_shade_direct<false>, grd, dim3(256), 0, s, S, make_gauss9());
+    FSG_CHECK_LAUNCH("k_shade_direct");
+    return FSG_OK;
+  }
   return launch_stencil(S, "k_hillshade_slope", as_stream(stream));
 }
 

@@ -262,8 +262,9 @@ constexpr int ST_ROWS = ST_TH + 2 * ST_HALO;
 constexpr int ST_PW = ST_TW + 2 * ST_RING;          // derivative table width
 constexpr int ST_PH = ST_TH + 2 * ST_RING;
 constexpr int ST_AXIS_FLOATS = 8;                    // one Axis, padded
-constexpr int ST_LDS_FLOATS =
-    ST_ROWS * ST_LT + 2 * ST_PH * ST_PW + 2 * ST_ROWS * ST_AXIS_FLOATS;
+constexpr int ST_LDS_FLOATS = ST_ROWS * ST_LT + 2 * ST_PH * ST_PW;
+// with per-row x coefficient tables (|step| and signed) for the staged rows
+constexpr int ST_LDS_FLOATS_ROWS = ST_LDS_FLOATS + 2 * ST_ROWS * ST_AXIS_FLOATS;
 
 struct StencilTile {
   WinDev W;
@@ -272,7 +273,7 @@ struct StencilTile {
   int nbx;         // blocks per block row of the region
   int nanaw;
   int want_hs, want_sl, want_cv;
-  int shade_from_tables;  // hillshade/slope gradient equals the curvature tables' (same |steps|, z 1)
+  int shade_from_tables;  // with curvature: hillshade/slope gradient equals the tables' (same |steps|)
   ShadeK shade;
   CurvK curv;
   OutDev hs, sl, cv;
@@ -283,9 +284,11 @@ inline bool same_axis(const Axis& a, const Axis& b) {
          a.b1 == b.b1 && a.c1 == b.c1;
 }
 
-template <bool NANAW>
-__device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by, int bx,
-                                                   const Gauss9& G, float* lds) {
+// ROWS: the window has per-row x steps (S.xrows); the variant without them
+// keeps the constant coefficients in registers.
+template <bool NANAW, bool ROWS>
+__device__ __forceinline__ void stencil_tile_block_v(const StencilTile& S, int by, int bx,
+                                                     const Gauss9& G, float* lds) {
   const int rl = S.r0 + by * ST_TH, cl = S.c0 + bx * ST_TW;  // window coords of the block
   const int H = S.W.h, Wd = S.W.w;
   const int tr0 = max(rl - ST_HALO, 0), tr1 = min(rl + ST_TH + ST_HALO, H);
@@ -301,7 +304,7 @@ __device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by,
     const int rr = e / nc, cc = e - rr * nc;
     F[rr * ST_LT + cc] = filled<NANAW>(S.W, tr0 + rr, tc0 + cc, G);
   }
-  if (S.xrows && t < nr) {
+  if (ROWS && t < nr) {
     const double hx = S.xrows[tr0 + t];
     *(Axis*)((float*)Xa + t * ST_AXIS_FLOATS) = make_axis(fabs(hx));
     *(Axis*)((float*)Xs + t * ST_AXIS_FLOATS) = make_axis(hx);
@@ -309,15 +312,17 @@ __device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by,
   __syncthreads();
   auto f = [&](int r, int c) { return F[(r - tr0) * ST_LT + (c - tc0)]; };
   // first-derivative tables over rows rl-2 .. rl+TH+1, cols cl-2 .. cl+TW+1
-  const bool need_tab = S.want_cv || (S.shade_from_tables && (S.want_hs || S.want_sl));
+  // the tables pay off for curvature's second derivatives; hillshade/slope
+  // alone take their gradient straight from the staged tile
+  const bool need_tab = S.want_cv;
   const Axis ax = S.want_cv ? S.curv.ax : S.shade.ax;
   const Axis ay = S.want_cv ? S.curv.ay : S.shade.ay;
   // x-derivative coefficients of window row i: the row's own step, or the constant one
   auto axr = [&](int i, const Axis& c) -> Axis {
-    return S.xrows ? *(const Axis*)((const float*)Xa + (i - tr0) * ST_AXIS_FLOATS) : c;
+    return ROWS ? *(const Axis*)((const float*)Xa + (i - tr0) * ST_AXIS_FLOATS) : c;
   };
   auto sxr = [&](int i, const Axis& c) -> Axis {
-    return S.xrows ? *(const Axis*)((const float*)Xs + (i - tr0) * ST_AXIS_FLOATS) : c;
+    return ROWS ? *(const Axis*)((const float*)Xs + (i - tr0) * ST_AXIS_FLOATS) : c;
   };
   if (need_tab) {
     for (int e = t; e < ST_PH * ST_PW; e += 256) {
@@ -349,7 +354,7 @@ __device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by,
       if (S.want_hs || S.want_sl) {
         float dx, dy;
         const Axis sax = axr(wr, S.shade.ax);
-        if (S.shade_from_tables) {
+        if (S.want_cv && S.shade_from_tables) {
           dx = P(wr, wc);
           dy = Q(wr, wc);
         } else {
@@ -373,6 +378,15 @@ __device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by,
     }
   }
   __syncthreads();
+}
+
+template <bool NANAW>
+__device__ __forceinline__ void stencil_tile_block(const StencilTile& S, int by, int bx,
+                                                   const Gauss9& G, float* lds) {
+  if (S.xrows)
+    stencil_tile_block_v<NANAW, true>(S, by, bx, G, lds);
+  else
+    stencil_tile_block_v<NANAW, false>(S, by, bx, G, lds);
 }
 
 }  // namespace fsg
