@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 6  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 7  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -114,6 +114,7 @@ class fsg_cnn_conv(C.Structure):
 
 
 CNN_GELU, CNN_LINEAR, CNN_GELU_GRAD = 0, 1, 2
+AGG_WEIGHTED, AGG_MEAN, AGG_SUM, AGG_MAX, AGG_MIN = 0, 1, 2, 3, 4
 
 
 class fsg_batch_config(C.Structure):
@@ -153,6 +154,20 @@ _SIGS = {
     "fsg_openness_judge": (C.c_int, [P(fsg_window), P(fsg_openness_params), vp, C.c_int64,
                                      C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_int32,
                                      vp, vp, C.c_size_t, vp]),
+    "fsg_pad_symmetric": (C.c_int, [P(fsg_window), C.c_int32, vp, C.c_int64, vp]),
+    "fsg_gaussian_nan_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
+    "fsg_gaussian_nan": (C.c_int, [P(fsg_window), C.c_int32, C.c_double, vp, C.c_int64, vp,
+                                   C.c_size_t, vp]),
+    "fsg_downsample_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
+    "fsg_downsample_nan_aware": (C.c_int, [P(fsg_window), C.c_int32, vp, C.c_int64, vp,
+                                           C.c_size_t, vp]),
+    "fsg_upsample_to_shape": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_int32, vp, C.c_int64,
+                                        C.c_int32, C.c_int32, vp]),
+    "fsg_sample_coarse": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32, vp, C.c_int64, vp, C.c_int64,
+                                    C.c_int32, C.c_int32, vp]),
+    "fsg_combine_fields": (C.c_int, [P(vp), C.c_int32, C.c_int64, P(C.c_float), C.c_int32,
+                                     C.c_int32, C.c_int32, P(fsg_output), vp]),
     "fsg_cnn_conv3x3": (C.c_int, [P(fsg_cnn_conv), vp]),
     "fsg_cnn_conv3x3_wgrad": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp,
                                         vp, vp]),

@@ -218,13 +218,20 @@ class NativeTileAlgorithm(TileAlgorithm):
         p.update(params)
         return p
 
+    # algorithms whose spatial mode runs here (spatial.py); the others raise
+    spatial_ok = False
+
     def check_mode(self, p: dict) -> None:
         """Spatial-mode requests take the reference's multi-radius path
-        (tile/dask_bridge.py:340-346), which this build does not provide."""
-        if str(p.get("mode", "local")).lower() == "spatial":
+        (tile/dask_bridge.py:340-346); built for hillshade, slope and
+        curvature (spatial.py), not for the other algorithms."""
+        if str(p.get("mode", "local")).lower() == "spatial" and not self.spatial_ok:
             raise UnsupportedOnBackend(
                 f"{self.name}: mode='spatial' (multi-radius smoothing) is not built on the "
                 "MI355X backend; use mode='local'")
+
+    def is_spatial(self, p: dict) -> bool:
+        return self.spatial_ok and str(p.get("mode", "local")).lower() == "spatial"
 
     def process(self, dem_gpu: torch.Tensor, **params) -> torch.Tensor:
         ops._check_window(dem_gpu, "dem_gpu")
@@ -247,14 +254,24 @@ class HillshadeAlgorithm(NativeTileAlgorithm):
                 "pixel_size": 1.0, "multiscale": False, "radii": None, "weights": None,
                 "agg": "mean", "mode": "local"}
 
+    spatial_ok = True
+
     def run_region(self, win, reg, out, p):
         self.check_mode(p)
+        if self.is_spatial(p):
+            from . import spatial
+            spatial.run("hillshade", win, reg, out.descriptors(), p)
+            return
         radii = p.get("radii", [1])
         if not isinstance(radii, (list, tuple)) or len(radii) == 0:
             radii = [1]
         radii = [max(1.0, float(r)) for r in radii]
         if (bool(p.get("multiscale", False)) or len(radii) > 1) and len(radii) > 1:
-            raise UnsupportedOnBackend("hillshade: multiscale radii are not built on this backend")
+            # _direct_hillshade's multi-radius branch (tile/dask_bridge.py:95-105)
+            from . import spatial
+            spatial.run("hillshade", win, reg, out.descriptors(), {**p, "radii": radii},
+                        direct=True)
+            return
         z = p.get("z_factor", 1.0)
         z = 1.0 if z is None else z
         sx, sy, sgx, sgy, rows = shade_steps(p, win)
@@ -274,8 +291,14 @@ class SlopeAlgorithm(NativeTileAlgorithm):
         return {"unit": "degree", "pixel_size": 1.0, "mode": "local", "radii": None,
                 "weights": None}
 
+    spatial_ok = True
+
     def run_region(self, win, reg, out, p):
         self.check_mode(p)
+        if self.is_spatial(p):
+            from . import spatial
+            spatial.run("slope", win, reg, out.descriptors(), p)
+            return
         sx, sy, _, _, rows = shade_steps(p, win)
         for o in out.descriptors():
             ops.hillshade_slope(win, reg, step_x=sx, step_y=sy, unit=p.get("unit", "degree"),
@@ -320,8 +343,14 @@ class CurvatureAlgorithm(NativeTileAlgorithm):
         return {"curvature_type": "mean", "pixel_size": 1.0, "mode": "local", "radii": None,
                 "weights": None}
 
+    spatial_ok = True
+
     def run_region(self, win, reg, out, p):
         self.check_mode(p)
+        if self.is_spatial(p):
+            from . import spatial
+            spatial.run("curvature", win, reg, out.descriptors(), p)
+            return
         sx, sy = curvature_steps(p.get("pixel_size", 1.0), p.get("pixel_scale_x"),
                                  p.get("pixel_scale_y"))
         rows = x_steps_rows(p, win.shape[0], win.device)

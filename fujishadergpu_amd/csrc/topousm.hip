@@ -1139,6 +1139,35 @@ int fsg::launch_box_batch(int pass, const BoxJob* jobs, const int* pre, int n_jo
   return FSG_OK;
 }
 
+extern "C" size_t fsg_downsample_workspace_size(int32_t h, int32_t w, int32_t factor) {
+  if (h <= 0 || w <= 0 || factor < 1) return 0;
+  const size_t cells = (size_t)((h + factor - 1) / factor) * (size_t)((w + factor - 1) / factor);
+  return 4 * align256(sizeof(float) * cells) + align256(sizeof(int) * 4);
+}
+
+extern "C" int fsg_downsample_nan_aware(const fsg_window* src, int32_t factor, float* out,
+                                        int64_t ldo, void* workspace, size_t workspace_bytes,
+                                        void* stream) {
+  int rc = validate_window(src, "fsg_downsample_nan_aware");
+  if (rc) return rc;
+  if (src->h <= 0 || src->w <= 0 || factor < 2 || !out)
+    return fail(FSG_E_INVALID, "fsg_downsample_nan_aware: bad arguments");
+  const int hc = (src->h + factor - 1) / factor, wc = (src->w + factor - 1) / factor;
+  if (ldo != wc)
+    return fail(FSG_E_INVALID, "fsg_downsample_nan_aware: out must be a dense ceil(h/f) x "
+                               "ceil(w/f) grid (ldo = ceil(w/f))");
+  const size_t need = fsg_downsample_workspace_size(src->h, src->w, factor);
+  if (!workspace || workspace_bytes < need)
+    return fail(FSG_E_WORKSPACE, "fsg_downsample_nan_aware: workspace too small");
+  char* ws = (char*)workspace;
+  const size_t g = align256(sizeof(float) * (size_t)hc * wc);
+  float* t[4];
+  for (int k = 0; k < 4; k++) t[k] = (float*)(ws + k * g);
+  int* flag = (int*)(ws + 4 * g);
+  return decimate(src, factor, out, hc, wc, flag, t[0], t[1], t[2], t[3], false,
+                  as_stream(stream));
+}
+
 extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
                            const fsg_topousm_params* p, const fsg_output* out,
                            const fsg_output* out2, void* workspace, size_t workspace_bytes,

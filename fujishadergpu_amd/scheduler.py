@@ -33,6 +33,8 @@ from .raster_info import metric_pixel_scales, row_geographic_scales_x, tile_geog
 
 # algorithms whose gradient takes a per-row x cell width
 ROW_SCALED = ("hillshade", "slope", "curvature", "hillshade_slope")
+# algorithms whose spatial mode (multi-radius smoothing) runs on the GPU
+SPATIAL_GPU = ("hillshade", "slope", "curvature")
 from .stats import norm_stats_prepass
 
 logger = logging.getLogger(__name__)
@@ -238,6 +240,21 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
         fld, scale = overview_field(src, lr, lw)
         P.params["_topousm_fast_coarse_field"] = fld
         P.extras["overview_scale"] = scale
+    spatial_tiles = (algorithm in SPATIAL_GPU and
+                     str(P.params.get("mode", "local")).lower() == "spatial")
+    if spatial_tiles and P.params.get("radii"):
+        # the unified overview coarse source of the spatial algorithms
+        # (core/tile_processor.py:1519-1545, read_overview_coarse_dem
+        # _nan_utils.py:836-871): GDAL average resampling, here the NaN-aware
+        # block mean of an integer decimation
+        scale, sh, sw = overview_geometry(W, H)
+        f = int(round(scale))
+        if abs(scale - f) <= 1e-12 and sh * f == H and sw * f == W:
+            P.params["_overview_coarse_dem"] = ops.block_mean(src, f) if f > 1 \
+                else src.contiguous().clone()
+            P.params["_overview_decimation"] = 0.5 * (W / float(sw) + H / float(sh))
+            P.params["_tile_full_shape"] = (int(H), int(W))
+            P.params["_overview_cache"] = {}
     pyramid = None
     if algorithm == "topousm_fast" and raster_nan_free:
         pyramid = CoarsePyramid(src)
@@ -269,6 +286,8 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
             continue
         p = dict(P.params)
         p["_window_nan_free"] = int(nn) == 0
+        if spatial_tiles:
+            p["_tile_origin"] = (int(wy), int(wx))
         if algorithm == "topousm_fast":
             if "_topousm_fast_coarse_field" in p:
                 p["_topousm_fast_field_offset"] = (int(wy), int(wx))
@@ -281,7 +300,7 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
             bottom = f + (cy + ch) * e
             p["pixel_scale_x"], p["pixel_scale_y"] = tile_geographic_scales(
                 a, e, 0.5 * (top + bottom))
-            if p.get("_geographic_rows") and algorithm in ROW_SCALED:
+            if p.get("_geographic_rows") and algorithm in ROW_SCALED and not spatial_tiles:
                 p["pixel_scale_x_rows"] = row_geographic_scales_x(a, e, f, int(wy), int(wh))
         win = src[wy:wy + wh, wx:wx + ww]
         reg = ops.region(cy - wy, cx - wx, ch, cw)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 6
+#define FSG_ABI_VERSION 7
 
 enum {
   FSG_OK = 0,
@@ -392,6 +392,60 @@ int fsg_encode(const fsg_window* src, const fsg_region* reg,
 /* Gather the region of a window into a dense f32 pool (stats pre-pass). */
 int fsg_copy_region(const fsg_window* win, const fsg_region* reg, float* dst,
                     void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Spatial mode (multi-radius hillshade, slope, curvature)              */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Building blocks of the spatial-mode path, which the reference's tile
+ * backend runs through Dask as one chunk (algorithms/tile/dask_bridge.py:
+ * 395-405, multiscale_response_fields _nan_utils.py:455-524):
+ *
+ * fsg_pad_symmetric: the window padded by pad on every side as dask's
+ *   map_overlap boundary='reflect' does ([d c b a | a b c d | d c b a]);
+ *   out is (h + 2 pad) x (w + 2 pad), pad <= min(h, w).
+ * fsg_gaussian_nan: handle_nan_with_gaussian (_nan_utils.py:18-31) of that
+ *   padded window without materialising it -- scipy gaussian_filter(sigma,
+ *   mode='nearest', truncate=4) of where(nan, 0, x) and of valid, axis 0 then
+ *   axis 1 with float32 intermediates, where(sw > 0, sv / sw, 0).  Workspace:
+ *   fsg_gaussian_nan_workspace_size(h, w, pad).
+ * fsg_downsample_nan_aware: _downsample_nan_aware (_nan_utils.py:604-668),
+ *   cell means with the enclosed-void fill, factor a power of two <= 16, out
+ *   ceil(h/f) x ceil(w/f).  Workspace: fsg_downsample_workspace_size.
+ * fsg_upsample_to_shape: _upsample_to_shape (_nan_utils.py:671-698).
+ * fsg_sample_coarse: _bilinear_sample_coarse (_nan_utils.py:255-281) of a
+ *   global coarse response at the window rows r0 .. r0 + h - 1 and columns
+ *   c0 .. c0 + w - 1 of a full_h x full_w raster; NaN where nan_src (the
+ *   window, optional) is NaN, as coarse_large_radius_response does.
+ * fsg_combine_fields: _combine_multiscale_dask (_nan_utils.py:182-213) and
+ *   hillshade's inline variant (_impl_hillshade.py:110-131) over n <= 16 h x w
+ *   fields with leading dimension ld, written through an output descriptor.
+ */
+enum {
+  FSG_AGG_WEIGHTED = 0, /* f0 * w0 + f1 * w1 + ... (float32, in order) */
+  FSG_AGG_MEAN = 1,     /* float32 mean */
+  FSG_AGG_SUM = 2,
+  FSG_AGG_MAX = 3,
+  FSG_AGG_MIN = 4
+};
+
+int fsg_pad_symmetric(const fsg_window* src, int32_t pad, float* out, int64_t ldo,
+                      void* stream);
+size_t fsg_gaussian_nan_workspace_size(int32_t h, int32_t w, int32_t pad);
+int fsg_gaussian_nan(const fsg_window* src, int32_t pad, double sigma, float* out, int64_t ldo,
+                     void* workspace, size_t workspace_bytes, void* stream);
+size_t fsg_downsample_workspace_size(int32_t h, int32_t w, int32_t factor);
+int fsg_downsample_nan_aware(const fsg_window* src, int32_t factor, float* out, int64_t ldo,
+                             void* workspace, size_t workspace_bytes, void* stream);
+int fsg_upsample_to_shape(const float* src, int64_t lds, int32_t hs, int32_t ws, float* out,
+                          int64_t ldo, int32_t h, int32_t w, void* stream);
+int fsg_sample_coarse(const float* coarse, int64_t ldc, int32_t hc, int32_t wc, int32_t r0,
+                      int32_t c0, int32_t full_h, int32_t full_w, const float* nan_src,
+                      int64_t ld_nan, float* out, int64_t ldo, int32_t h, int32_t w,
+                      void* stream);
+int fsg_combine_fields(const float* const* fields, int32_t n, int64_t ld, const float* weights,
+                       int32_t agg, int32_t h, int32_t w, const fsg_output* out, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Openness surrogate: BF16 CNN on MFMA                                 */

@@ -706,3 +706,169 @@ def run_window(algorithm: str, window: np.ndarray, params: dict) -> np.ndarray:
             full_shape=p.get("_topousm_fast_full_shape"),
             stats=p.get("global_stats"))
     raise ValueError(f"oracle has no algorithm {algorithm!r}")
+
+
+# ---------------------------------------------------------------------------
+# Spatial mode: multi-radius hillshade / slope / curvature (next row F1)
+# ---------------------------------------------------------------------------
+# The reference's tile backend sends a spatial-mode tile through Dask as one
+# chunk (algorithms/tile/dask_bridge.py:395-405): gpu_da = the tile window.
+
+SPATIAL_AUTO_RADII = (2, 8, 32, 128, 512, 2048)   # common/spatial_mode.py
+
+
+def clean_normalized_weights(weights):
+    """_clean_normalized_weights (_nan_utils.py:148-161)."""
+    cleaned = []
+    for w in weights:
+        try:
+            fv = float(w)
+        except (TypeError, ValueError):
+            return None
+        cleaned.append(fv if np.isfinite(fv) and fv > 0 else 0.0)
+    s = float(sum(cleaned))
+    if s <= 0:
+        return None
+    return [v / s for v in cleaned]
+
+
+def resolve_spatial_radii_weights(radii, weights, pixel_size=1.0):
+    """_resolve_spatial_radii_weights with _normalize_spatial_radii
+    (_nan_utils.py:77-129) for the explicit-radii case, the 2**n profile
+    (common/spatial_mode.py:78-84) when the weights do not fit."""
+    if radii is None:
+        radii = list(SPATIAL_AUTO_RADII)
+    out = []
+    for r in radii:
+        try:
+            rv = int(round(float(r)))
+        except (TypeError, ValueError):
+            continue
+        if rv > 0 and rv not in out:
+            out.append(rv)
+    if not out:
+        out = list(SPATIAL_AUTO_RADII)
+    if weights is not None and not isinstance(weights, (str, bytes)) and len(weights) == len(out):
+        user = clean_normalized_weights(weights)
+        if user is not None:
+            return out, user
+    raw = [2.0 ** (len(out) - 1 - i) for i in range(len(out))]
+    tot = sum(raw)
+    return out, [v / tot for v in raw]
+
+
+def dask_reflect_pad(block, depth: int) -> np.ndarray:
+    """dask.array.overlap.reflect on every side: [d c b a | a b c d | d c b a]."""
+    return np.pad(np.asarray(block, dtype=F32), int(depth), mode="symmetric")
+
+
+def smooth_for_radius(block, radius, *, pixel_size=1.0, algorithm_name="default"):
+    """_smooth_for_radius (_nan_utils.py:527-552)."""
+    block = np.asarray(block, dtype=F32)
+    r = max(1.0, float(radius))
+    if r <= 1.0:
+        return block
+    factor = radius_to_downsample_factor(r, pixel_size=pixel_size, algorithm_name=algorithm_name)
+    if factor <= 1:
+        smoothed, _ = handle_nan_with_gaussian(block, sigma=max(0.5, r / 2.0), mode="nearest")
+        return smoothed
+    reduced = downsample_nan_aware(block, factor)
+    small, _ = handle_nan_with_gaussian(reduced, sigma=max(0.5, (r / factor) / 2.0), mode="nearest")
+    return upsample_to_shape(small, block.shape)
+
+
+def spatial_depth(algorithm: str, r: float) -> int:
+    """depth_for_scale of the three spatial algorithms (_impl_hillshade.py:102,
+    _impl_slope.py:64, _impl_curvature.py:84)."""
+    if algorithm == "curvature":
+        return max(3, int(float(r) * 2 + 2))
+    return max(2, int(float(r) * 2 + 1))
+
+
+def spatial_block(algorithm, block, radius, params):
+    """compute_{hillshade,slope}_spatial_block (_impl_hillshade.py:59-70,
+    _impl_slope.py:38-45) and curvature's _curv_spatial (_impl_curvature.py:72-78):
+    the local block function of the smoothed block."""
+    sm = smooth_for_radius(block, radius, pixel_size=params.get("pixel_size", 1.0),
+                           algorithm_name=algorithm)
+    return run_window(algorithm, sm, params)
+
+
+def coarsen_factor_for_shape(shape, coarse_max=2048) -> int:
+    """_nan_utils.py:231-236."""
+    longest = max(int(shape[0]), int(shape[1]))
+    if longest <= int(coarse_max):
+        return 1
+    return 1 << int(np.ceil(np.log2(longest / float(coarse_max))))
+
+
+def spatial_window(algorithm, window, params, *, overview=None, decimation=None,
+                   tile_origin=None, full_shape=None) -> np.ndarray:
+    """{Hillshade,Slope,Curvature}Algorithm.process in spatial mode on a
+    single-chunk Dask array (the tile window): multiscale_response_fields
+    (_nan_utils.py:455-524) and the combine of _impl_hillshade.py:110-131 /
+    _combine_multiscale_dask (_nan_utils.py:182-213)."""
+    window = np.asarray(window, dtype=F32)
+    h, w = window.shape
+    px = params.get("pixel_size", 1.0)
+    agg = str(params.get("agg", "mean") or "mean").lower()
+    user_w = params.get("weights")
+    radii, resolved_w = resolve_spatial_radii_weights(params.get("radii"), user_w, px)
+    if algorithm == "hillshade":
+        weights = resolved_w if user_w is None else user_w
+    else:
+        weights = resolved_w
+    thr = int(max(256, min(h, w) // 16))                         # large_radius_threshold
+    F = coarsen_factor_for_shape(window.shape)
+    coarse_ok = overview is not None and tile_origin is not None
+    chunk_cap = max(1, min(h, w) - 1)
+    base = {k: v for k, v in params.items() if k not in ("radii", "weights", "mode", "agg")}
+    fields = []
+    for r in [float(v) for v in radii]:
+        large = int(round(r)) > thr
+        if large and (F > 1 or coarse_ok):
+            if overview is None:
+                raise NotImplementedError("window-coarsen branch without an overview")
+            fac = float(decimation)
+            kw = dict(base)
+            kw["pixel_size"] = float(px) * fac
+            if kw.get("pixel_scale_x") is not None:
+                kw["pixel_scale_x"] = float(kw["pixel_scale_x"]) * fac
+            if kw.get("pixel_scale_y") is not None:
+                kw["pixel_scale_y"] = float(kw["pixel_scale_y"]) * fac
+            resp = spatial_block(algorithm, overview, max(1, int(round(r / fac))), kw).astype(F32)
+            r0, c0 = int(tile_origin[0]), int(tile_origin[1])
+            fh, fw = (int(full_shape[0]), int(full_shape[1])) if full_shape else (h, w)
+            up = bilinear_sample_coarse(resp, r0, r0 + h, c0, c0 + w, fh, fw)
+            fields.append(np.where(np.isnan(window), F32(np.nan), up).astype(F32))
+        else:
+            d = max(1, min(spatial_depth(algorithm, r), chunk_cap))
+            res = spatial_block(algorithm, dask_reflect_pad(window, d), r, base)
+            fields.append(np.asarray(res, dtype=F32)[d:d + h, d:d + w])
+    if agg == "stack":
+        return np.stack(fields, axis=0)
+    if algorithm != "hillshade" and len(fields) == 1:
+        return fields[0]
+    if agg == "max":
+        return np.max(np.stack(fields), axis=0)
+    if agg == "min":
+        return np.min(np.stack(fields), axis=0)
+    if agg == "sum" and algorithm != "hillshade":
+        return np.sum(np.stack(fields), axis=0, dtype=F32)
+    if algorithm == "hillshade":
+        if (agg == "mean" and isinstance(weights, (list, tuple)) and len(weights) == len(fields)):
+            wv = np.asarray(weights, dtype=F32)
+            if np.isfinite(wv).all() and wv.sum() > 0:
+                wv = wv / wv.sum()
+                out = fields[0] * F32(wv[0])
+                for i in range(1, len(fields)):
+                    out = out + fields[i] * F32(wv[i])
+                return out.astype(F32)
+    elif weights is not None and len(weights) == len(fields):
+        clean = clean_normalized_weights(weights)
+        if clean is not None:
+            out = fields[0] * F32(clean[0])
+            for i in range(1, len(fields)):
+                out = out + fields[i] * F32(clean[i])
+            return out.astype(F32)
+    return (np.sum(np.stack(fields), axis=0, dtype=F32) / F32(len(fields))).astype(F32)

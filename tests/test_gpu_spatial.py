@@ -1,0 +1,95 @@
+"""Spatial mode (multi-radius hillshade, slope, curvature) against the oracle's
+restatement of the reference's single-chunk Dask path and overview path."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _dem(h, w, seed, nan_discs=0, relief=1800.0, hurst=0.8):
+    from fujishadergpu_amd import synth
+    d = synth.fbm(h, w, hurst, relief, seed=seed, device="cpu")
+    if nan_discs:
+        discs = synth.nodata_discs(h, w, 0.01, seed=seed + 1)[:nan_discs]
+        synth.apply_discs(d, discs, float("nan"))
+    return d
+
+
+def _cmp(name, got, exp, atol, rtol=0.0):
+    got = got.detach().cpu().numpy() if isinstance(got, torch.Tensor) else got
+    nm_g, nm_e = np.isnan(got), np.isnan(exp)
+    assert np.array_equal(nm_g, nm_e), f"{name}: NaN mask differs ({(nm_g != nm_e).sum()} px)"
+    d = np.abs(got.astype(np.float64) - exp.astype(np.float64))[~nm_e]
+    lim = atol + rtol * np.abs(exp[~nm_e])
+    bad = d > lim
+    assert not bad.any(), f"{name}: {bad.sum()} px over tolerance, max |d| = {d.max():.3g}"
+    return float(d.max()) if d.size else 0.0
+
+
+TOL = {"hillshade": (2e-5, 0.0), "slope": (2e-4, 2e-5), "curvature": (4e-4, 0.0)}
+
+
+@pytest.mark.parametrize("algorithm", ["hillshade", "slope", "curvature"])
+@pytest.mark.parametrize("radii,weights,nan_discs", [
+    ([3, 12, 30], [0.5, 0.3, 0.2], 0),          # Gaussian sigma r / 2 on the padded window
+    ([2, 64], None, 4),                        # r = 64: decimated by 2 (2**n weights), NoData
+    ([40], None, 0),                           # one radius: the field itself
+])
+def test_spatial_window_matches_the_oracle(cuda, algorithm, radii, weights, nan_discs):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(300, 340, 41, nan_discs=nan_discs)
+    kw = dict(mode="spatial", radii=radii, weights=weights, pixel_size=10.0, pixel_scale_x=10.0,
+              pixel_scale_y=-10.0)
+    if algorithm == "curvature":
+        kw["curvature_type"] = "mean"
+    got = ALGORITHMS[algorithm].process(dem.to(cuda), **kw)
+    exp = ref.spatial_window(algorithm, dem.numpy(), kw)
+    mx = _cmp(f"{algorithm}/spatial", got, exp, *TOL[algorithm])
+    print(algorithm, radii, "max |d|", mx, flush=True)
+
+
+@pytest.mark.parametrize("agg", ["max", "min", "mean"])
+def test_hillshade_local_multiscale_matches_the_direct_path(cuda, agg):
+    # _direct_hillshade: the spatial block on the window itself, _combine_direct
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(200, 230, 43)
+    kw = dict(mode="local", multiscale=True, radii=[2, 9], weights=None, agg=agg,
+              pixel_size=10.0)
+    got = ALGORITHMS["hillshade"].process(dem.to(cuda), **kw).cpu().numpy()
+    a = dem.numpy()
+    fields = [ref.spatial_block("hillshade", a, r, {"pixel_size": 10.0}) for r in (2.0, 9.0)]
+    if agg == "max":
+        exp = np.maximum(fields[0], fields[1])
+    elif agg == "min":
+        exp = np.minimum(fields[0], fields[1])
+    else:
+        inv = np.float32(1.0 / 2.0)
+        exp = (np.zeros_like(fields[0]) + fields[0] * inv) + fields[1] * inv
+    _cmp("hillshade/direct", got, exp.astype(np.float32), 2e-5)
+
+
+@pytest.mark.parametrize("algorithm", ["hillshade", "slope"])
+def test_scheduler_spatial_with_the_overview(cuda, algorithm):
+    # 2048^2 raster, tiles of 1024: r = 300 exceeds the large-radius threshold
+    # (256) and comes from the overview response sampled at each tile window
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _dem(2048, 2048, 47, relief=2500.0)
+    kw = dict(mode="spatial", radii=[4, 40, 300], weights=[0.5, 0.3, 0.2])
+    out, _, P = process_dem(host_t.to(cuda), algorithm, tile_size=1024, pixel_size=10.0,
+                            return_plan=True, **kw)
+    host = host_t.numpy()
+    ov = host.copy()          # overview decimation 1 at 2048 px (sample_max 2048)
+    want = np.full(host.shape, np.nan, np.float32)
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+        p = {k: v for k, v in P.params.items()
+             if not isinstance(v, (torch.Tensor, dict)) and not str(k).startswith("_overview")}
+        res = ref.spatial_window(algorithm, host[wy:wy + wh, wx:wx + ww], p, overview=ov,
+                                 decimation=1.0, tile_origin=(wy, wx), full_shape=host.shape)
+        want[cy:cy + ch, cx:cx + cw] = ref.format_output(
+            res[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw], algorithm)
+    got = out.cpu().numpy() if out.dtype == torch.float32 else None
+    assert got is not None
+    _cmp(f"{algorithm}/scheduler", got, want, *TOL[algorithm])

@@ -630,3 +630,45 @@ def test_cnn_oracle_conv_is_a_same_correlation():
                 assert out[0, y, xx, co] == pytest.approx(s, abs=1e-12)
     assert cnn.gelu_erf(np.array([3.0]))[0] == pytest.approx(
         3.0 * 0.5 * (1 + math.erf(3 / math.sqrt(2))))
+
+
+# ------------------------------------------------ spatial mode (multi-radius)
+
+def test_dask_reflect_is_the_symmetric_pad():
+    # dask.array.overlap.reflect: [d c b a | a b c d | d c b a]
+    a = np.array([[1, 2, 3, 4]], dtype=F32)
+    p = ref.dask_reflect_pad(np.repeat(a, 4, axis=0), 3)
+    assert p[3].tolist() == [3, 2, 1, 1, 2, 3, 4, 4, 3, 2]
+
+
+@pytest.mark.parametrize("radius,algo", [(6.0, "hillshade"), (30.0, "slope"), (64.0, "hillshade"),
+                                         (150.0, "curvature")])
+def test_smooth_for_radius_keeps_a_constant(radius, algo):
+    z = np.full((200, 230), 123.25, F32)
+    s = ref.smooth_for_radius(z, radius, pixel_size=10.0, algorithm_name=algo)
+    assert s.shape == z.shape
+    assert np.abs(s - 123.25).max() < 1e-4
+
+
+def test_spatial_hillshade_of_a_plane_interior_is_the_closed_form():
+    # a Gaussian keeps a plane away from the padded edges; radii below the
+    # decimation threshold (r / 24 < 2 for hillshade at 10 m)
+    h, w, a, b = 260, 300, 0.12, -0.07
+    yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    z = (a * 10.0 * xx + b * 10.0 * yy).astype(F32)
+    kw = dict(pixel_size=10.0, pixel_scale_x=10.0, pixel_scale_y=10.0, radii=[3, 12, 30],
+              weights=[0.5, 0.3, 0.2])
+    out = ref.spatial_window("hillshade", z, kw)
+    want = ref.hillshade(z, pixel_size=10.0, pixel_scale_x=10.0, pixel_scale_y=10.0)
+    c = out[100:160, 100:200]
+    assert np.abs(c - want[100:160, 100:200]).max() < 2e-5
+
+
+def test_spatial_single_radius_is_the_spatial_block():
+    rng = np.random.default_rng(2)
+    z = np.cumsum(rng.normal(size=(120, 140)), axis=1).astype(F32)
+    kw = dict(pixel_size=10.0, radii=[5], weights=None)
+    out = ref.spatial_window("slope", z, kw)
+    d = ref.spatial_depth("slope", 5)
+    blk = ref.spatial_block("slope", ref.dask_reflect_pad(z, d), 5, {"pixel_size": 10.0})
+    assert np.array_equal(out, blk[d:-d, d:-d])
