@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--only-c2", action="store_true",
                     help="skip the per-algorithm workloads (C1 hillshade+slope, C3 curvature and "
                          "openness, C5 fused batch)")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5"], default=None,
+    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "f1"], default=None,
                     help="run only this per-algorithm workload and print its JSON (a profiling "
                          "aid; the bench line is the default run)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -335,6 +335,44 @@ def bench_c5(dev):
     return res
 
 
+def bench_f1(dev, n=16384):
+    """Spatial-mode hillshade (the CLI default mode) over a 16384^2 raster with
+    the automatic radii of that raster (2, 8, 32, 128, 512; 2**n weights):
+    Gaussian radii, a decimated radius and an overview radius, tiled 4096 with
+    the reference's padding rule, as a whole job; the oracle on one bounded
+    window for the CPU rate."""
+    import numpy as np
+    import torch
+    from fujishadergpu_amd.scheduler import process_dem
+    dem = make_dem(n, dev)
+    kw = dict(mode="spatial", tile_size=4096, pixel_size=5.0)
+    out, _, P = process_dem(dem, "hillshade", return_plan=True, **kw)
+    torch.cuda.synchronize()
+    t = _gpu_time(lambda: process_dem(dem, "hillshade", out=out, **kw), 3)
+    # CPU: the oracle's spatial window for one 1024^2 core of the same radii
+    from oracle import ref
+    pad = P.padding
+    host = dem[:1024 + 2 * pad, :1024 + 2 * pad].cpu().numpy()
+    prm = {k: v for k, v in P.params.items()
+           if not isinstance(v, (torch.Tensor, dict)) and not str(k).startswith("_")}
+    small = [r for r in P.params["radii"] if r <= 256]
+    prm_s = dict(prm, radii=small, weights=None)
+    t0 = time.perf_counter()
+    ref.spatial_window("hillshade", host, prm_s)
+    tc = time.perf_counter() - t0
+    del dem, out
+    torch.cuda.empty_cache()
+    return {"workload": f"spatial-mode hillshade, {n}x{n} f32 fBm (H=0.75, 0-3776 m, 5 m, "
+                        f"seed 2), automatic radii {P.params['radii']}, tile 4096 + halo "
+                        f"{P.padding}, overview decimation "
+                        f"{P.params.get('_overview_decimation')}",
+            "gpix_s": round(n * n / t / 1e9, 3), "ms": round(t * 1e3, 2),
+            "cpu_baseline": {"value": round(1024 * 1024 / tc / 1e9, 5), "unit": "Gpix/s",
+                             "cores": 1, "kind": "port",
+                             "sample": f"oracle spatial window, one 1024^2 core (+{pad} halo), "
+                                       f"the Gaussian/decimated radii {small}"}}
+
+
 def bench_c4(dev, n=65536, weights=None):
     """Learned openness surrogate at radius 1024 (16 directions, 10 m) on an
     n^2 fBm (H 0.7, 0-4000 m, seed 4) against the exact kernel: time of each,
@@ -403,7 +441,8 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.workload:
-        fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload]
+        fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+              "f1": bench_f1}[args.workload]
         print(json.dumps({args.workload: fn(dev)}), flush=True)
         return
     n = args.size
@@ -483,7 +522,8 @@ def main():
             "kernel_hbm_frac": line["roofline"]["frac"]}}
         for name, fn in (("hillshade_slope_c1", bench_c1), ("batch_fused_c5", bench_c5),
                          ("curvature_openness_c3", bench_c3),
-                         ("openness_surrogate_c4", bench_c4)):
+                         ("openness_surrogate_c4", bench_c4),
+                         ("spatial_hillshade_f1", bench_f1)):
             print(f"# {name} ...", file=sys.stderr, flush=True)
             line["per_algorithm"][name] = fn(dev)
     print(json.dumps(line), flush=True)

@@ -45,47 +45,196 @@ struct PadSrc {
   }
 };
 
-// Pass 0 (axis 0) of the NaN-aware Gaussian over the padded extent:
-// tv = f32(sum g * where(nan, 0, x)), tw = f32(sum g * valid), scipy's
-// symmetric order (centre, then pairs at distance r .. 1) in float64.
-__global__ void k_gauss_nan_rows(PadSrc S, GaussW G, float* __restrict__ tv,
-                                 float* __restrict__ tw, int64_t ldt) {
-  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
-  const int hp = S.h + 2 * S.pad, wp = S.w + 2 * S.pad;
-  if (pc >= wp) return;
-  for (int pr = blockIdx.y; pr < hp; pr += gridDim.y) {
-    const float x0 = S.at(pr, pc);
-    double sv = (isnan(x0) ? 0.0 : (double)x0) * G.g[0];
-    double sw = (isnan(x0) ? 0.0 : 1.0) * G.g[0];
-    for (int d = G.r; d >= 1; d--) {
-      const float a = S.at(pr - d, pc), b = S.at(pr + d, pc);
-      sv += ((isnan(a) ? 0.0 : (double)a) + (isnan(b) ? 0.0 : (double)b)) * G.g[d];
-      sw += ((isnan(a) ? 0.0 : 1.0) + (isnan(b) ? 0.0 : 1.0)) * G.g[d];
+// The NaN-aware Gaussian runs as two LDS-tiled passes over 64 x 64 output
+// tiles.  Each of the 4 waves of a block owns GN_M consecutive outputs along
+// the filter axis for 64 lines across it, so one sample loaded from LDS into
+// a register serves GN_M outputs: the pair (x[-d], x[+d]) of output m at
+// distance d is x[m - d], x[m + d], and a block of GN_M distances reads two
+// register windows of 2 GN_M - 1 samples that slide by GN_M per block.  Every
+// output still accumulates in scipy's symmetric order, in float64: the
+// centre, then acc + (x[-d] + x[+d]) * g[d] for d = r .. 1, each operation
+// rounded on its own (the library builds with -ffp-contract=off).
+constexpr int GN_T = 64;   // output tile edge
+constexpr int GN_M = 16;   // outputs per lane along the filter axis (4 waves x 16 = 64)
+
+// sample as the accumulation sees it: 0 plain, 1 NaN -> 0, 2 validity (0 / 1)
+template <int MODE>
+__device__ __forceinline__ double gn_val(float f) {
+  if constexpr (MODE == 0) return (double)f;
+  else if constexpr (MODE == 1) return isnan(f) ? 0.0 : (double)f;
+  else return isnan(f) ? 0.0 : 1.0;
+}
+
+// acc[m] = sum_k g[|k|] v(s[(m + k) ST]), k in [-r, r], in the order above.
+// s points at the lane's first output; the LDS line holds r samples before
+// it and r + GN_M - 1 after it.
+template <int MODE, int ST>
+__device__ __forceinline__ void gn_line(const float* s, const GaussW& G, double (&acc)[GN_M]) {
+  const double g0 = G.g[0];
+#pragma unroll
+  for (int m = 0; m < GN_M; m++) acc[m] = gn_val<MODE>(s[m * ST]) * g0;
+  int d = G.r;
+  for (; d % GN_M != 0; d--) {   // the top r mod GN_M distances one at a time
+    const double g = G.g[d];
+#pragma unroll
+    for (int m = 0; m < GN_M; m++)
+      acc[m] = acc[m] + (gn_val<MODE>(s[(m - d) * ST]) + gn_val<MODE>(s[(m + d) * ST])) * g;
+  }
+  if (d == 0) return;
+  // distances d .. d - GN_M + 1: L[k] = x[k - d], R[k] = x[k + d - GN_M + 1]
+  double L[2 * GN_M - 1], R[2 * GN_M - 1];
+#pragma unroll
+  for (int k = 0; k < 2 * GN_M - 1; k++) {
+    L[k] = gn_val<MODE>(s[(k - d) * ST]);
+    R[k] = gn_val<MODE>(s[(k + d - (GN_M - 1)) * ST]);
+  }
+  for (;;) {
+#pragma unroll
+    for (int dd = 0; dd < GN_M; dd++) {
+      const double g = G.g[d - dd];
+#pragma unroll
+      for (int m = 0; m < GN_M; m++) acc[m] = acc[m] + (L[m + dd] + R[m - dd + GN_M - 1]) * g;
     }
-    tv[(int64_t)pr * ldt + pc] = (float)sv;
-    tw[(int64_t)pr * ldt + pc] = (float)sw;
+    d -= GN_M;
+    if (d == 0) break;
+#pragma unroll
+    for (int k = 0; k < GN_M - 1; k++) L[k] = L[k + GN_M];
+#pragma unroll
+    for (int k = GN_M - 1; k < 2 * GN_M - 1; k++) L[k] = gn_val<MODE>(s[(k - d) * ST]);
+#pragma unroll
+    for (int k = 2 * GN_M - 2; k >= GN_M; k--) R[k] = R[k - GN_M];
+#pragma unroll
+    for (int k = 0; k < GN_M; k++) R[k] = gn_val<MODE>(s[(k + d - (GN_M - 1)) * ST]);
   }
 }
 
-// Pass 1 (axis 1) and where(sw > 0, sv / sw, 0) in float32 ('nearest' along
-// the padded rows).
-__global__ void k_gauss_nan_cols(const float* __restrict__ tv, const float* __restrict__ tw,
-                                 int64_t ldt, int hp, int wp, GaussW G, float* __restrict__ out,
-                                 int64_t ldo) {
-  const int pc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pc >= wp) return;
-  for (int pr = blockIdx.y; pr < hp; pr += gridDim.y) {
-    const float* rv = tv + (int64_t)pr * ldt;
-    const float* rw = tw + (int64_t)pr * ldt;
-    double sv = (double)rv[pc] * G.g[0];
-    double sw = (double)rw[pc] * G.g[0];
-    for (int d = G.r; d >= 1; d--) {
-      const int a = max(pc - d, 0), b = min(pc + d, wp - 1);
-      sv += ((double)rv[a] + (double)rv[b]) * G.g[d];
-      sw += ((double)rw[a] + (double)rw[b]) * G.g[d];
+// Pass 0 (axis 0) over the padded extent: tv = f32(sum g * where(nan, 0, x)),
+// tw = f32(sum g * valid).  A tile whose input span holds no NaN has tw =
+// f32(wsum) everywhere, wsum being the same accumulation over all-valid
+// samples (computed on the host by the same operation sequence).
+// LDS: (64 + 2r) rows x 64 columns.
+__global__ __launch_bounds__(256, 2) void k_gauss_nan_rows(PadSrc S, GaussW G,
+                                                           float* __restrict__ tv,
+                                                           float* __restrict__ tw, int64_t ldt,
+                                                           float wconst) {
+  extern __shared__ float gl[];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int hp = S.h + 2 * S.pad, wp = S.w + 2 * S.pad;
+  const int tx0 = blockIdx.x * GN_T, ty0 = blockIdx.y * GN_T;
+  const int nrow = GN_T + 2 * G.r;
+  const int pc = min(tx0 + lane, wp - 1);
+  bool nan = false;
+  // staging in batches of 8 rows per wave (8 loads in flight per lane)
+  for (int i0 = q; i0 < nrow; i0 += 32) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + 4 * u;
+      v[u] = i < nrow ? S.at(ty0 - G.r + i, pc) : 0.0f;
     }
-    const float fv = (float)sv, fw = (float)sw;
-    out[(int64_t)pr * ldo + pc] = fw > 0.0f ? fv / fw : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + 4 * u;
+      if (i < nrow) {
+        gl[i * GN_T + lane] = v[u];
+        nan |= isnan(v[u]);
+      }
+    }
+  }
+  const bool any_nan = __syncthreads_or(nan);
+  const float* s = gl + (q * GN_M + G.r) * GN_T + lane;
+  const int x = tx0 + lane;
+  const int y0 = ty0 + q * GN_M;
+  const int nm = x < wp ? min(GN_M, hp - y0) : 0;   // rows this lane stores
+  double acc[GN_M];
+  if (any_nan) gn_line<1, GN_T>(s, G, acc);
+  else gn_line<0, GN_T>(s, G, acc);
+#pragma unroll
+  for (int m = 0; m < GN_M; m++)
+    if (m < nm) tv[(int64_t)(y0 + m) * ldt + x] = (float)acc[m];
+  if (!any_nan) {
+#pragma unroll
+    for (int m = 0; m < GN_M; m++)
+      if (m < nm) tw[(int64_t)(y0 + m) * ldt + x] = wconst;
+    return;
+  }
+  gn_line<2, GN_T>(s, G, acc);
+#pragma unroll
+  for (int m = 0; m < GN_M; m++)
+    if (m < nm) tw[(int64_t)(y0 + m) * ldt + x] = (float)acc[m];
+}
+
+// Pass 1 (axis 1, 'nearest' along the padded rows) and where(sw > 0, sv / sw,
+// 0) in float32.  A tile whose tw span is all f32(wsum) divides by the
+// constant f32 of that accumulation (wdiv, from the host).  LDS: 64 rows x
+// (64 + 2r + 1) columns (odd pitch: lanes walk rows without bank conflicts),
+// reused for tw, tv and the transposed output tile.
+__global__ __launch_bounds__(256, 2) void k_gauss_nan_cols(const float* __restrict__ tv,
+                                                        const float* __restrict__ tw, int64_t ldt,
+                                                        int hp, int wp, GaussW G,
+                                                        float* __restrict__ out, int64_t ldo,
+                                                        float wconst, float wdiv) {
+  extern __shared__ float gl[];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int tx0 = blockIdx.x * GN_T, ty0 = blockIdx.y * GN_T;
+  const int ncol = GN_T + 2 * G.r, P = ncol + 1;
+  // staging: column chunks of 64, each in batches of 8 rows per wave (8 loads in flight)
+  auto stage = [&](const float* src) {
+    bool off = false;
+    for (int c = 0; c < ncol; c += 64) {
+      const int j = c + lane;
+      const bool jok = j < ncol;
+      const int col = min(max(tx0 - G.r + j, 0), wp - 1);
+#pragma unroll
+      for (int k0 = 0; k0 < GN_T / 4; k0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = q + 4 * (k0 + u);
+          v[u] = jok ? src[(int64_t)min(ty0 + i, hp - 1) * ldt + col] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = q + 4 * (k0 + u);
+          if (jok) {
+            gl[i * P + j] = v[u];
+            off |= __float_as_uint(v[u]) != __float_as_uint(wconst);
+          }
+        }
+      }
+    }
+    return off;
+  };
+  const float* s = gl + lane * P + q * GN_M + G.r;
+  double acc[GN_M];
+  float fw[GN_M], fo[GN_M];
+  const bool varied = __syncthreads_or(stage(tw));
+  if (varied) {
+    gn_line<0, 1>(s, G, acc);
+#pragma unroll
+    for (int m = 0; m < GN_M; m++) fw[m] = (float)acc[m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < GN_M; m++) fw[m] = wdiv;
+  }
+  __syncthreads();
+  stage(tv);
+  __syncthreads();
+  gn_line<0, 1>(s, G, acc);
+#pragma unroll
+  for (int m = 0; m < GN_M; m++) {
+    const float v = (float)acc[m];
+    fo[m] = fw[m] > 0.0f ? v / fw[m] : 0.0f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < GN_M; m++) gl[lane * (GN_T + 1) + q * GN_M + m] = fo[m];
+  __syncthreads();
+  const int x = tx0 + lane;
+  if (x >= wp) return;
+  for (int i = q; i < GN_T; i += 4) {
+    const int y = ty0 + i;
+    if (y < hp) out[(int64_t)y * ldo + x] = gl[i * (GN_T + 1) + lane];
   }
 }
 
@@ -231,16 +380,26 @@ extern "C" int fsg_gaussian_nan(const fsg_window* src, int32_t pad, double sigma
   std::vector<double> full(2 * G.r + 1);
   gaussian_weights(sigma, G.r, full.data());
   for (int d = 0; d <= G.r; d++) G.g[d] = full[G.r + d];
+  // the all-valid accumulations: pass 0 over validity 1, pass 1 over f32 of that
+  double w0 = 1.0 * G.g[0];
+  for (int d = G.r; d >= 1; d--) w0 = w0 + (1.0 + 1.0) * G.g[d];
+  const float wconst = (float)w0;
+  double w1 = (double)wconst * G.g[0];
+  for (int d = G.r; d >= 1; d--) w1 = w1 + ((double)wconst + (double)wconst) * G.g[d];
+  const float wdiv = (float)w1;
   float* tv = (float*)workspace;
   float* tw = (float*)((char*)workspace + need / 2);
   hipStream_t s = as_stream(stream);
   PadSrc S{src->data, src->ld, src->h, src->w, pad};
+  const dim3 grid((wp + GN_T - 1) / GN_T, (hp + GN_T - 1) / GN_T);
+  const size_t lds_r = sizeof(float) * (GN_T + 2 * G.r) * GN_T;
+  const size_t lds_c = sizeof(float) * GN_T * (GN_T + 2 * G.r + 1);
   TimedScope ts("k_gauss_nan", s);
-  hipLaunchKernelGGL(k_gauss_nan_rows, grid2(wp, hp), dim3(256), 0, s, S, G, tv, tw,
-                     (int64_t)wp);
+  hipLaunchKernelGGL(k_gauss_nan_rows, grid, dim3(256), lds_r, s, S, G, tv, tw, (int64_t)wp,
+                     wconst);
   FSG_CHECK_LAUNCH("k_gauss_nan_rows");
-  hipLaunchKernelGGL(k_gauss_nan_cols, grid2(wp, hp), dim3(256), 0, s, tv, tw, (int64_t)wp, hp,
-                     wp, G, out, ldo);
+  hipLaunchKernelGGL(k_gauss_nan_cols, grid, dim3(256), lds_c, s, tv, tw, (int64_t)wp, hp, wp, G,
+                     out, ldo, wconst, wdiv);
   FSG_CHECK_LAUNCH("k_gauss_nan_cols");
   return FSG_OK;
 }

@@ -243,6 +243,8 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
     origin = p.get("_tile_origin")
     coarse_ok = overview is not None and origin is not None
     chunk_cap = max(1, min(h, w) - 1)
+    # a NaN-free window keeps every smoothed field NaN-free (all-valid block
+    # means, Gaussian weights > 0, upsampling denominators of 1)
     nan_free = bool(p.get("_window_nan_free", False))
     base = {k: v for k, v in p.items() if k not in ("radii", "weights", "mode", "agg")}
     fields: List[torch.Tensor] = []
@@ -251,7 +253,7 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
         wsub = win[reg.r0:reg.r0 + reg.h, reg.c0:reg.c0 + reg.w]
         if direct:
             sm, may_nan = smooth_for_radius(win, 0, r, px, algorithm)
-            _local(algorithm, sm, reg, field, base, nan_free=not may_nan)
+            _local(algorithm, sm, reg, field, base, nan_free=nan_free or not may_nan)
         elif int(round(r)) > thr and (F > 1 or coarse_ok):
             if coarse_ok:
                 fac = float(p["_overview_decimation"])
@@ -274,13 +276,13 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
                 sm, may_nan = smooth_for_radius(coarse, d, r_c, kw["pixel_size"], algorithm)
                 resp = torch.empty_like(coarse)
                 _local(algorithm, sm, ops.region(d, d, coarse.shape[0], coarse.shape[1]), resp,
-                       kw, nan_free=not may_nan)
+                       kw, nan_free=nan_free or not may_nan)
                 _sample(resp, reg.r0, reg.c0, h, w, wsub, field)
         else:
             d = max(1, min(depth_for(algorithm, r), chunk_cap))
             sm, may_nan = smooth_for_radius(win, d, r, px, algorithm)
             _local(algorithm, sm, ops.region(d + reg.r0, d + reg.c0, reg.h, reg.w), field, base,
-                   nan_free=not may_nan)
+                   nan_free=nan_free or not may_nan)
         fields.append(field)
     # combine (the scalar weights are float32 in the products)
     n = len(fields)

@@ -93,3 +93,26 @@ def test_scheduler_spatial_with_the_overview(cuda, algorithm):
     got = out.cpu().numpy() if out.dtype == torch.float32 else None
     assert got is not None
     _cmp(f"{algorithm}/scheduler", got, want, *TOL[algorithm])
+
+
+@pytest.mark.parametrize("sigma", [1.0, 4.5, 7.3, 16.0, 40.0])
+@pytest.mark.parametrize("nan_discs", [0, 1])
+def test_gaussian_nan_matches_the_oracle(cuda, sigma, nan_discs):
+    # LDS-tiled two-pass Gaussian: r = 4, 18 (r mod 16 = 2), 29, 64, 160 taps
+    # per side, tiles with and without NaN in their span, symmetric padding
+    from fujishadergpu_amd import spatial
+    dem = _dem(333, 290, 53, nan_discs=nan_discs, relief=2500.0)
+    if nan_discs:
+        dem[:6, :9] = float("nan")        # one corner tile holds NaN, the others none
+    pad = 37
+    got = spatial.gaussian_nan(dem.to(cuda), pad, sigma).cpu().numpy()
+    exp, _ = ref.handle_nan_with_gaussian(ref.dask_reflect_pad(dem.numpy(), pad), sigma,
+                                          mode="nearest")
+    assert got.shape == exp.shape
+    nm = np.isnan(exp)
+    assert not np.isnan(got).any() and not nm.any()
+    d = np.abs(got.astype(np.float64) - exp.astype(np.float64))
+    lim = 2.4e-7 * np.abs(exp.astype(np.float64)) + 1e-30
+    print(f"sigma {sigma} nan {nan_discs}: max |d| {d.max():.3g}, "
+          f"identical {np.mean(got == exp):.6f}", flush=True)
+    assert (d <= lim).all(), f"{(d > lim).sum()} px beyond 2 ulp"
