@@ -223,8 +223,8 @@ class NativeTileAlgorithm(TileAlgorithm):
 
     def check_mode(self, p: dict) -> None:
         """Spatial-mode requests take the reference's multi-radius path
-        (tile/dask_bridge.py:340-346); built for hillshade, slope and
-        curvature (spatial.py), not for the other algorithms."""
+        (tile/dask_bridge.py:340-346); built for hillshade, slope, curvature,
+        openness and ambient occlusion (spatial.py), not for the others."""
         if str(p.get("mode", "local")).lower() == "spatial" and not self.spatial_ok:
             raise UnsupportedOnBackend(
                 f"{self.name}: mode='spatial' (multi-radius smoothing) is not built on the "
@@ -371,12 +371,18 @@ class OpennessAlgorithm(NativeTileAlgorithm):
         return {"openness_type": "positive", "num_directions": 16, "max_distance": 50,
                 "pixel_size": 1.0, "mode": "local", "radii": None, "weights": None}
 
+    spatial_ok = True
+
     def run_region(self, win, reg, out, p, stretch=True):
         self.check_mode(p)
+        post = _post_from_stats("stretch", p.get("global_stats") if stretch else None)
+        if self.is_spatial(p):
+            from . import spatial
+            spatial.run("openness", win, reg, out.descriptors(*post), p)
+            return
         table = openness_table(p.get("num_directions", 16), p.get("max_distance", 50),
                                p.get("pixel_size", 1.0), p.get("pixel_scale_x"),
                                p.get("pixel_scale_y"))
-        post = _post_from_stats("stretch", p.get("global_stats") if stretch else None)
         for o in out.descriptors(*post):
             ops.openness(win, reg, table=table,
                          openness_type=p.get("openness_type", "positive"), out=o)
@@ -390,12 +396,18 @@ class AmbientOcclusionAlgorithm(NativeTileAlgorithm):
         return {"num_samples": 16, "radius": 10.0, "intensity": 1.0, "pixel_size": 1.0,
                 "mode": "local", "radii": None, "weights": None}
 
+    spatial_ok = True
+
     def run_region(self, win, reg, out, p, stretch=True):
         self.check_mode(p)
+        post = _post_from_stats("stretch", p.get("global_stats") if stretch else None)
+        if self.is_spatial(p):
+            from . import spatial
+            spatial.run("ambient_occlusion", win, reg, out.descriptors(*post), p)
+            return
         radius = p.get("radius", 10.0)
         table = ao_table(p.get("num_samples", 16), radius, p.get("pixel_size", 1.0),
                          p.get("pixel_scale_x"), p.get("pixel_scale_y"))
-        post = _post_from_stats("stretch", p.get("global_stats") if stretch else None)
         for o in out.descriptors(*post):
             ops.ambient_occlusion(win, reg, table=table, intensity=p.get("intensity", 1.0),
                                   pad_d=max(1, int(round(float(radius)))), out=o)

@@ -34,7 +34,7 @@ from .raster_info import metric_pixel_scales, row_geographic_scales_x, tile_geog
 # algorithms whose gradient takes a per-row x cell width
 ROW_SCALED = ("hillshade", "slope", "curvature", "hillshade_slope")
 # algorithms whose spatial mode (multi-radius smoothing) runs on the GPU
-SPATIAL_GPU = ("hillshade", "slope", "curvature")
+SPATIAL_GPU = ("hillshade", "slope", "curvature", "openness", "ambient_occlusion")
 from .stats import norm_stats_prepass
 
 logger = logging.getLogger(__name__)

@@ -1,4 +1,5 @@
-"""Spatial mode: multi-radius hillshade, slope and curvature on the GPU.
+"""Spatial mode: multi-radius hillshade, slope, curvature, openness and
+ambient occlusion on the GPU.
 
 The reference's tile backend runs a spatial-mode tile through Dask as one
 chunk (algorithms/tile/dask_bridge.py:395-405).  For every radius r,
@@ -17,6 +18,13 @@ multiscale_response_fields (algorithms/_nan_utils.py:455-524) either
 
 then the per-radius fields are combined (_impl_hillshade.py:110-131 for
 hillshade, _combine_multiscale_dask _nan_utils.py:182-213 otherwise).
+
+Openness and ambient occlusion take no Gaussian: their block functions
+(compute_openness_spatial_block, _impl_openness.py:135-164;
+compute_ambient_occlusion_spatial_block, _impl_ambient_occlusion.py:123-157)
+run the local ray kernel on the block, or on its NaN-aware f-decimated copy
+with the radius and the pixel scales scaled by f, upsampled back to the
+block's shape.
 
 Here each step is a kernel of csrc/spatial.hip (or the local stencil kernels on
 the smoothed, padded window); only the region the caller asks for is formed.
@@ -86,9 +94,15 @@ def resolve_radii_weights(radii, weights) -> Tuple[List[int], List[float]]:
     return out, [v / tot for v in raw]
 
 
+RAY_ALGORITHMS = ("openness", "ambient_occlusion")
+SPATIAL_ALGORITHMS = ("hillshade", "slope", "curvature") + RAY_ALGORITHMS
+
+
 def depth_for(algorithm: str, r: float) -> int:
     """depth_for_scale (_impl_hillshade.py:102, _impl_slope.py:64,
-    _impl_curvature.py:84)."""
+    _impl_curvature.py:84, _impl_openness.py:195, _impl_ambient_occlusion.py:180)."""
+    if algorithm in RAY_ALGORITHMS:
+        return int(r) + 1
     if algorithm == "curvature":
         return max(3, int(float(r) * 2 + 2))
     return max(2, int(float(r) * 2 + 1))
@@ -185,6 +199,45 @@ def _local(algorithm: str, src: torch.Tensor, reg: N.fsg_region, out: torch.Tens
                       nan_free=nan_free)
 
 
+def _ray_local(algorithm: str, src: torch.Tensor, reg: N.fsg_region, out: torch.Tensor,
+               r: float, p: dict) -> None:
+    """compute_openness_vectorized / compute_ambient_occlusion_block of src at
+    radius r (max_distance / radius), region reg into out (no display stretch)."""
+    from .algorithms import ao_table, openness_table
+    px = p.get("pixel_size", 1.0)
+    psx, psy = p.get("pixel_scale_x"), p.get("pixel_scale_y")
+    o = ops.output(out)
+    if algorithm == "openness":
+        table = openness_table(p.get("num_directions", 16), r, px, psx, psy)
+        ops.openness(src, reg, table=table, openness_type=p.get("openness_type", "positive"),
+                     out=o)
+    else:
+        table = ao_table(p.get("num_samples", 16), r, px, psx, psy)
+        ops.ambient_occlusion(src, reg, table=table, intensity=p.get("intensity", 1.0),
+                              pad_d=max(1, int(round(float(r)))), out=o)
+
+
+def _ray_block(algorithm: str, src: torch.Tensor, reg: N.fsg_region, out: torch.Tensor,
+               r: float, p: dict) -> None:
+    """compute_openness_spatial_block (_impl_openness.py:135-164) /
+    compute_ambient_occlusion_spatial_block (_impl_ambient_occlusion.py:123-157)
+    of the whole array src, region reg into out."""
+    f = radius_to_downsample_factor(r, pixel_size=p.get("pixel_size", 1.0),
+                                    algorithm_name=algorithm)
+    if f <= 1:
+        _ray_local(algorithm, src, reg, out, r, p)
+        return
+    small = downsample_nan_aware(src, f)
+    if algorithm == "openness":
+        r_s = float(max(2, int(round(float(r) / float(f)))))
+    else:
+        r_s = max(1.0, float(r) / float(f))
+    resp = torch.empty_like(small)
+    _ray_local(algorithm, small, ops.full_region(small), resp, r_s, _scaled(p, float(f)))
+    up = upsample_to_shape(resp, src.shape[0], src.shape[1])
+    out.copy_(up[reg.r0:reg.r0 + reg.h, reg.c0:reg.c0 + reg.w])
+
+
 def _scaled(p: dict, fac: float) -> dict:
     kw = dict(p)
     kw["pixel_size"] = float(p.get("pixel_size", 1.0)) * fac
@@ -196,6 +249,10 @@ def _scaled(p: dict, fac: float) -> dict:
 
 def _coarse_response(algorithm: str, coarse: torch.Tensor, r_c: int, p: dict) -> torch.Tensor:
     """block_fn on a whole coarse array (no halo) at radius r_c."""
+    if algorithm in RAY_ALGORITHMS:
+        resp = torch.empty_like(coarse)
+        _ray_block(algorithm, coarse, ops.full_region(coarse), resp, float(r_c), p)
+        return resp
     sm, may_nan = smooth_for_radius(coarse, 0, r_c, p.get("pixel_size", 1.0), algorithm)
     resp = torch.empty_like(sm)
     _local(algorithm, sm, ops.full_region(sm), resp, p, nan_free=not may_nan)
@@ -219,7 +276,7 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
     (_direct_hillshade, tile/dask_bridge.py:72-111): the spatial block function
     on the window itself (no halo, no overview), radii max(1, r) as given, and
     _combine_direct (tile/dask_bridge.py:28-69)."""
-    if algorithm not in ("hillshade", "slope", "curvature"):
+    if algorithm not in SPATIAL_ALGORITHMS or (direct and algorithm != "hillshade"):
         raise ValueError(f"no spatial path for {algorithm}")
     if p.get("pixel_scale_x_rows") is not None:
         raise ValueError("per-row x scales are not defined for the spatial mode")
@@ -247,6 +304,11 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
     # means, Gaussian weights > 0, upsampling denominators of 1)
     nan_free = bool(p.get("_window_nan_free", False))
     base = {k: v for k, v in p.items() if k not in ("radii", "weights", "mode", "agg")}
+    ray = algorithm in RAY_ALGORITHMS
+    if algorithm == "openness":          # _impl_openness.py:193
+        radii = [int(max(2, round(float(r)))) for r in radii]
+    elif algorithm == "ambient_occlusion":   # _impl_ambient_occlusion.py:178
+        radii = [max(1, int(round(float(r)))) for r in radii]
     fields: List[torch.Tensor] = []
     for r in [float(v) for v in radii]:
         field = torch.empty((reg.h, reg.w), dtype=torch.float32, device=win.device)
@@ -273,11 +335,18 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
                 r_c = max(1, int(round(r / F)))
                 kw = _scaled(base, float(F))
                 d = max(1, min(depth_for(algorithm, r_c), max(1, min(coarse.shape) - 1)))
-                sm, may_nan = smooth_for_radius(coarse, d, r_c, kw["pixel_size"], algorithm)
+                creg = ops.region(d, d, coarse.shape[0], coarse.shape[1])
                 resp = torch.empty_like(coarse)
-                _local(algorithm, sm, ops.region(d, d, coarse.shape[0], coarse.shape[1]), resp,
-                       kw, nan_free=nan_free or not may_nan)
+                if ray:
+                    _ray_block(algorithm, pad_symmetric(coarse, d), creg, resp, float(r_c), kw)
+                else:
+                    sm, may_nan = smooth_for_radius(coarse, d, r_c, kw["pixel_size"], algorithm)
+                    _local(algorithm, sm, creg, resp, kw, nan_free=nan_free or not may_nan)
                 _sample(resp, reg.r0, reg.c0, h, w, wsub, field)
+        elif ray:
+            d = max(1, min(depth_for(algorithm, r), chunk_cap))
+            _ray_block(algorithm, pad_symmetric(win, d),
+                       ops.region(d + reg.r0, d + reg.c0, reg.h, reg.w), field, r, base)
         else:
             d = max(1, min(depth_for(algorithm, r), chunk_cap))
             sm, may_nan = smooth_for_radius(win, d, r, px, algorithm)

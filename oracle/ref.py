@@ -777,18 +777,59 @@ def smooth_for_radius(block, radius, *, pixel_size=1.0, algorithm_name="default"
     return upsample_to_shape(small, block.shape)
 
 
+RAY_SPATIAL = ("openness", "ambient_occlusion")
+
+
 def spatial_depth(algorithm: str, r: float) -> int:
-    """depth_for_scale of the three spatial algorithms (_impl_hillshade.py:102,
-    _impl_slope.py:64, _impl_curvature.py:84)."""
+    """depth_for_scale of the spatial algorithms (_impl_hillshade.py:102,
+    _impl_slope.py:64, _impl_curvature.py:84, _impl_openness.py:195,
+    _impl_ambient_occlusion.py:180)."""
+    if algorithm in RAY_SPATIAL:
+        return int(r) + 1
     if algorithm == "curvature":
         return max(3, int(float(r) * 2 + 2))
     return max(2, int(float(r) * 2 + 1))
 
 
+def ray_spatial_block(algorithm, block, radius, params):
+    """compute_openness_spatial_block (_impl_openness.py:135-164) and
+    compute_ambient_occlusion_spatial_block (_impl_ambient_occlusion.py:123-157):
+    the local ray function of the block, or of its NaN-aware decimated copy
+    (radius and pixel scales scaled by the factor) upsampled to the block."""
+    block = np.asarray(block, dtype=F32)
+    px = params.get("pixel_size", 1.0)
+    psx, psy = params.get("pixel_scale_x"), params.get("pixel_scale_y")
+
+    def local(b, r, px_, psx_, psy_):
+        if algorithm == "openness":
+            return openness(b, openness_type=params.get("openness_type", "positive"),
+                            num_directions=params.get("num_directions", 16), max_distance=r,
+                            pixel_size=px_, pixel_scale_x=psx_, pixel_scale_y=psy_)
+        return ambient_occlusion(b, num_samples=params.get("num_samples", 16), radius=r,
+                                 intensity=params.get("intensity", 1.0), pixel_size=px_,
+                                 pixel_scale_x=psx_, pixel_scale_y=psy_)
+
+    f = radius_to_downsample_factor(float(radius), pixel_size=px, algorithm_name=algorithm)
+    if f <= 1:
+        return local(block, radius, px, psx, psy)
+    small = downsample_nan_aware(block, f)
+    dpsx = float(abs(float(psx)) * f) if psx is not None else None
+    dpsy = float(abs(float(psy)) * f) if psy is not None else None
+    if algorithm == "openness":
+        r_s = max(2, int(round(float(radius) / float(f))))
+    else:
+        r_s = max(1.0, float(radius) / float(f))
+    res = local(small, r_s, float(px) * float(f), dpsx, dpsy)
+    return upsample_to_shape(res, block.shape)
+
+
 def spatial_block(algorithm, block, radius, params):
     """compute_{hillshade,slope}_spatial_block (_impl_hillshade.py:59-70,
     _impl_slope.py:38-45) and curvature's _curv_spatial (_impl_curvature.py:72-78):
-    the local block function of the smoothed block."""
+    the local block function of the smoothed block; the ray algorithms'
+    spatial blocks for openness and ambient occlusion."""
+    if algorithm in RAY_SPATIAL:
+        return ray_spatial_block(algorithm, block, radius, params)
     sm = smooth_for_radius(block, radius, pixel_size=params.get("pixel_size", 1.0),
                            algorithm_name=algorithm)
     return run_window(algorithm, sm, params)
@@ -804,8 +845,9 @@ def coarsen_factor_for_shape(shape, coarse_max=2048) -> int:
 
 def spatial_window(algorithm, window, params, *, overview=None, decimation=None,
                    tile_origin=None, full_shape=None) -> np.ndarray:
-    """{Hillshade,Slope,Curvature}Algorithm.process in spatial mode on a
-    single-chunk Dask array (the tile window): multiscale_response_fields
+    """{Hillshade,Slope,Curvature,Openness,AmbientOcclusion}Algorithm.process in
+    spatial mode (before the display stretch) on a single-chunk Dask array (the
+    tile window): multiscale_response_fields
     (_nan_utils.py:455-524) and the combine of _impl_hillshade.py:110-131 /
     _combine_multiscale_dask (_nan_utils.py:182-213)."""
     window = np.asarray(window, dtype=F32)
@@ -823,6 +865,10 @@ def spatial_window(algorithm, window, params, *, overview=None, decimation=None,
     coarse_ok = overview is not None and tile_origin is not None
     chunk_cap = max(1, min(h, w) - 1)
     base = {k: v for k, v in params.items() if k not in ("radii", "weights", "mode", "agg")}
+    if algorithm == "openness":                       # _impl_openness.py:193
+        radii = [int(max(2, round(float(r)))) for r in radii]
+    elif algorithm == "ambient_occlusion":            # _impl_ambient_occlusion.py:178
+        radii = [max(1, int(round(float(r)))) for r in radii]
     fields = []
     for r in [float(v) for v in radii]:
         large = int(round(r)) > thr

@@ -116,3 +116,55 @@ def test_gaussian_nan_matches_the_oracle(cuda, sigma, nan_discs):
     print(f"sigma {sigma} nan {nan_discs}: max |d| {d.max():.3g}, "
           f"identical {np.mean(got == exp):.6f}", flush=True)
     assert (d <= lim).all(), f"{(d > lim).sum()} px beyond 2 ulp"
+
+
+RAY_TOL = 2e-5
+
+
+@pytest.mark.parametrize("algorithm", ["openness", "ambient_occlusion"])
+@pytest.mark.parametrize("radii,weights,nan_discs", [
+    ([3, 12], [0.6, 0.4], 0),                   # direct: the local ray kernel on the pad
+    ([5, 64], None, 3),                         # r = 64 decimated (2 / 4), NoData
+    ([40], None, 0),                            # one radius: the field itself
+])
+def test_ray_spatial_window_matches_the_oracle(cuda, algorithm, radii, weights, nan_discs):
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(300, 330, 61, nan_discs=nan_discs)
+    kw = dict(mode="spatial", radii=radii, weights=weights, pixel_size=10.0,
+              pixel_scale_x=10.0, pixel_scale_y=-10.0)
+    if algorithm == "openness":
+        kw["num_directions"] = 8
+    else:
+        kw["num_samples"] = 8
+    got = ALGORITHMS[algorithm].process(dem.to(cuda), **kw)
+    exp = ref.spatial_window(algorithm, dem.numpy(), kw)
+    mx = _cmp(f"{algorithm}/spatial", got, exp, RAY_TOL)
+    print(algorithm, radii, "max |d|", mx, flush=True)
+
+
+@pytest.mark.parametrize("algorithm", ["openness", "ambient_occlusion"])
+def test_ray_scheduler_spatial_with_the_overview_and_stretch(cuda, algorithm):
+    # 2048^2 raster, tiles of 1024: r = 300 comes from the overview response;
+    # the display stretch of the given [p1, p99] is the combine's epilogue
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _dem(2048, 2048, 67, relief=2500.0)
+    stats = (0.62, 0.97)
+    kw = dict(mode="spatial", radii=[6, 48, 300], weights=[0.5, 0.3, 0.2], global_stats=stats)
+    if algorithm == "openness":
+        kw["num_directions"] = 8
+    else:
+        kw["num_samples"] = 8
+    out, _, P = process_dem(host_t.to(cuda), algorithm, tile_size=1024, pixel_size=10.0,
+                            return_plan=True, **kw)
+    host = host_t.numpy()
+    want = np.full(host.shape, np.nan, np.float32)
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+        p = {k: v for k, v in P.params.items()
+             if not isinstance(v, (torch.Tensor, dict)) and not str(k).startswith("_overview")}
+        res = ref.spatial_window(algorithm, host[wy:wy + wh, wx:wx + ww], p, overview=host.copy(),
+                                 decimation=1.0, tile_origin=(wy, wx), full_shape=host.shape)
+        res = ref.display_stretch(res, stats)
+        want[cy:cy + ch, cx:cx + cw] = ref.format_output(
+            res[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw], algorithm)
+    assert out.dtype == torch.float32
+    _cmp(f"{algorithm}/scheduler", out, want, RAY_TOL)

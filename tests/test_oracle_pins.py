@@ -672,3 +672,59 @@ def test_spatial_single_radius_is_the_spatial_block():
     d = ref.spatial_depth("slope", 5)
     blk = ref.spatial_block("slope", ref.dask_reflect_pad(z, d), 5, {"pixel_size": 10.0})
     assert np.array_equal(out, blk[d:-d, d:-d])
+
+
+# ------------------------------------- spatial mode of openness and AO (ray blocks)
+
+@pytest.mark.parametrize("algorithm,radius,factor", [("openness", 64, 2),
+                                                     ("ambient_occlusion", 64, 4)])
+def test_ray_spatial_block_of_a_flat_raster_is_one(algorithm, radius, factor):
+    # the decimated branch (factor from the reference's table) keeps a flat
+    # raster flat: openness 90 deg -> 1, AO no occlusion -> 1
+    assert ref.radius_to_downsample_factor(radius, pixel_size=10.0,
+                                           algorithm_name=algorithm) == factor
+    z = np.full((150, 170), 812.5, F32)
+    out = ref.ray_spatial_block(algorithm, z, radius, {"pixel_size": 10.0})
+    assert out.shape == z.shape
+    assert np.abs(out - 1.0).max() < 1e-6
+
+
+def test_ray_spatial_ao_of_a_plane_matches_the_closed_form():
+    # AO of the plane z = a * x (metres) in the decimated branch (f = 4): the
+    # ring offsets are those of radius 64 / 4 = 16, and dz / dist is the same
+    # on the coarse grid only when both the pixel size and the scales are
+    # multiplied by f.  Closed form per sample, written out here.
+    a, px, r, f = 0.35, 10.0, 64.0, 4
+    h, w = 256, 256
+    xx = np.arange(w, dtype=np.float64)[None, :].repeat(h, 0)
+    z = (a * px * xx).astype(F32)
+    p = {"pixel_size": px, "pixel_scale_x": px, "pixel_scale_y": -px, "num_samples": 16,
+         "intensity": 1.0}
+    out = ref.ray_spatial_block("ambient_occlusion", z, r, p)
+    rs = r / f
+    ang = np.linspace(0, 2 * np.pi, 16, endpoint=False)
+    tot = n = 0.0
+    for rf in (0.25, 0.5, 0.75, 1.0):
+        for t in ang:
+            dx, dy = int(np.round(rs * rf * np.cos(t))), int(np.round(rs * rf * np.sin(t)))
+            if dx == 0 and dy == 0:
+                continue
+            dist = np.hypot(dx * px * f, dy * px * f)
+            occ = min(max(np.arctan(a * dx * px * f / dist), 0.0) / (np.pi / 4), 1.0)
+            tot += occ * (1.0 - rf * 0.3)
+            n += 1
+    want = (1.0 - tot / n) ** (1 / 2.2)
+    c = out[80:176, 80:176]                 # rays and the 1-sigma blur stay inside
+    assert np.abs(c - want).max() < 2e-5
+
+
+def test_spatial_openness_single_radius_is_the_ray_block():
+    rng = np.random.default_rng(5)
+    z = np.cumsum(rng.normal(size=(140, 150)), axis=0).astype(F32) * F32(3.0)
+    kw = dict(pixel_size=10.0, radii=[12], weights=None, num_directions=8)
+    out = ref.spatial_window("openness", z, kw)
+    d = ref.spatial_depth("openness", 12)
+    assert d == 13
+    blk = ref.ray_spatial_block("openness", ref.dask_reflect_pad(z, d), 12,
+                                {"pixel_size": 10.0, "num_directions": 8})
+    assert np.array_equal(out, blk[d:-d, d:-d])
