@@ -336,19 +336,29 @@ def bench_c5(dev):
 
 
 def bench_f1(dev, n=16384):
-    """Spatial-mode hillshade (the CLI default mode) over a 16384^2 raster with
-    the automatic radii of that raster (2, 8, 32, 128, 512; 2**n weights):
-    Gaussian radii, a decimated radius and an overview radius, tiled 4096 with
-    the reference's padding rule, as a whole job; the oracle on one bounded
-    window for the CPU rate."""
-    import numpy as np
+    """Spatial mode (the CLI default mode) over a 16384^2 raster with the
+    automatic radii of that raster (2, 8, 32, 128, 512; 2**n weights), tiled
+    4096 with the reference's padding rule, as whole jobs: hillshade (the
+    headline line: Gaussian radii, a decimated radius and an overview radius),
+    then slope, curvature, openness and ambient occlusion (the last two with
+    their p1/p99 display-stretch pre-pass); the oracle on one bounded window for
+    the CPU rate."""
     import torch
     from fujishadergpu_amd.scheduler import process_dem
     dem = make_dem(n, dev)
     kw = dict(mode="spatial", tile_size=4096, pixel_size=5.0)
-    out, _, P = process_dem(dem, "hillshade", return_plan=True, **kw)
-    torch.cuda.synchronize()
-    t = _gpu_time(lambda: process_dem(dem, "hillshade", out=out, **kw), 3)
+    per = {}
+    P = None
+    for algo in ("hillshade", "slope", "curvature", "openness", "ambient_occlusion"):
+        out, _, Pa = process_dem(dem, algo, return_plan=True, **kw)
+        torch.cuda.synchronize()
+        ta = _gpu_time(lambda: process_dem(dem, algo, out=out, **kw), 3)
+        per[algo] = {"ms": round(ta * 1e3, 2), "gpix_s": round(n * n / ta / 1e9, 3),
+                     "halo": Pa.padding}
+        print(f"#   f1 {algo}: {ta * 1e3:.2f} ms", file=sys.stderr, flush=True)
+        if algo == "hillshade":
+            P, t = Pa, ta
+        del out
     # CPU: the oracle's spatial window for one 1024^2 core of the same radii
     from oracle import ref
     pad = P.padding
@@ -360,13 +370,14 @@ def bench_f1(dev, n=16384):
     t0 = time.perf_counter()
     ref.spatial_window("hillshade", host, prm_s)
     tc = time.perf_counter() - t0
-    del dem, out
+    del dem
     torch.cuda.empty_cache()
     return {"workload": f"spatial-mode hillshade, {n}x{n} f32 fBm (H=0.75, 0-3776 m, 5 m, "
                         f"seed 2), automatic radii {P.params['radii']}, tile 4096 + halo "
                         f"{P.padding}, overview decimation "
                         f"{P.params.get('_overview_decimation')}",
             "gpix_s": round(n * n / t / 1e9, 3), "ms": round(t * 1e3, 2),
+            "algorithms": per,
             "cpu_baseline": {"value": round(1024 * 1024 / tc / 1e9, 5), "unit": "Gpix/s",
                              "cores": 1, "kind": "port",
                              "sample": f"oracle spatial window, one 1024^2 core (+{pad} halo), "

@@ -157,6 +157,9 @@ __device__ __forceinline__ float openness_value(const OpenTab& T, float angle) {
   return powf(op, T.gamma);
 }
 
+// ND: the table's distances per direction when it is 1 .. 10 (fixed-length
+// rows: wide scalar table loads, no guards), 0 for any other count.
+template <int ND>
 __global__ void __launch_bounds__(256)
 k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -173,8 +176,13 @@ k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
     store_out(out, r, c, apply_post(pp, __builtin_nanf("")));
     return;
   }
-  store_out(out, r, c,
-            apply_post(pp, openness_value(T, openness_angle_at(W, i, j, z, T, interior))));
+  float a;
+  if constexpr (ND > 0)
+    a = interior ? openness_angle_u<ND, false>(W, i, j, z, T)
+                 : openness_angle_u<ND, true>(W, i, j, z, T);
+  else
+    a = openness_angle_at(W, i, j, z, T, interior);
+  store_out(out, r, c, apply_post(pp, openness_value(T, a)));
 }
 
 // The same at a list of window pixels (row, col), float32 out[n].
@@ -419,8 +427,21 @@ extern "C" int fsg_openness(const fsg_window* win, const fsg_region* reg,
   WinDevR W{win->data, win->ld, win->h, win->w};
   dim3 blk(64, 4), grd((reg->w + 63) / 64, (reg->h + 3) / 4);
   TimedScope ts("k_openness", as_stream(stream));
-  hipLaunchKernelGGL(k_openness, grd, blk, 0, as_stream(stream), W, reg->r0, reg->c0, reg->h,
-                     reg->w, T, make_out(out));
+  const OutDev od = make_out(out);
+  hipStream_t s = as_stream(stream);
+  switch (T.n_dist) {
+#define FSG_OPEN_ND(N)                                                                       \
+  case N:                                                                                    \
+    hipLaunchKernelGGL(k_openness<N>, grd, blk, 0, s, W, reg->r0, reg->c0, reg->h, reg->w, T, \
+                       od);                                                                  \
+    break;
+    FSG_OPEN_ND(1) FSG_OPEN_ND(2) FSG_OPEN_ND(3) FSG_OPEN_ND(4) FSG_OPEN_ND(5)
+    FSG_OPEN_ND(6) FSG_OPEN_ND(7) FSG_OPEN_ND(8) FSG_OPEN_ND(9) FSG_OPEN_ND(10)
+#undef FSG_OPEN_ND
+    default:
+      hipLaunchKernelGGL(k_openness<0>, grd, blk, 0, s, W, reg->r0, reg->c0, reg->h, reg->w, T,
+                         od);
+  }
   FSG_CHECK_LAUNCH("k_openness");
   return FSG_OK;
 }
