@@ -295,10 +295,55 @@ struct AoTab {
   float intensity;  // f32(intensity)
   float quarter_pi; // f32(pi/4)
   float gamma;
+  int dmax_y, dmax_x;  // largest |offset| on each axis (interior test)
   short2 off[AO_MAX];
   float inv[AO_MAX];
   float df[AO_MAX];
 };
+
+// Occlusion sum and valid-sample count at window (i, j), samples in table
+// order, 8 loads issued before any is used.  CHECK = false: every sample
+// lies inside the window (wave-uniform element offsets, no clamp); with CHECK
+// the sample coordinates are clamped to the window ('edge' padding).  The
+// float32 operations and their order are those of the per-sample loop.
+template <bool CHECK>
+__device__ __forceinline__ void ao_sum(const WinDevR& W, int i, int j, float z, const AoTab& T,
+                                       float& total, float& count) {
+  const float* pc = W.p + (int64_t)i * W.ld + j;
+  total = 0.0f;
+  count = 0.0f;
+  for (int e0 = 0; e0 < T.n; e0 += 8) {
+    float v[8];
+    bool ok[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int e = e0 + k;
+      ok[k] = false;
+      v[k] = 0.0f;
+      if (e < T.n) {
+        const short2 o = T.off[e];
+        ok[k] = !(o.x == 0 && o.y == 0);
+        if (CHECK)
+          v[k] = W.p[(int64_t)clampi(i + o.y, W.h) * W.ld + clampi(j + o.x, W.w)];
+        else
+          v[k] = pc[(int64_t)o.y * W.ld + o.x];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int e = e0 + k;
+      if (e < T.n) {
+        const bool valid = ok[k] && !isnan(v[k]);
+        const float a = atanf((v[k] - z) * T.inv[e]);
+        float occ = fmaxf(0.0f, a) / T.quarter_pi;
+        occ = fminf(occ, 1.0f);
+        const float nt = total + occ * T.df[e];
+        total = valid ? nt : total;
+        count = valid ? count + 1.0f : count;
+      }
+    }
+  }
+}
 
 // Pass 1: AO before the Gaussian on the expanded region, stored as
 // where(isnan(z), 1, ao) -- the array the reference smooths.
@@ -308,24 +353,20 @@ k_ao_raw(WinDevR W, int er0, int ec0, int eh, int ew, AoTab T, float* __restrict
   const int r = blockIdx.y * blockDim.y + threadIdx.y;
   if (c >= ew || r >= eh) return;
   const int i = er0 + r, j = ec0 + c;
+  // block-uniform: do all samples of the block's pixels stay inside the window?
+  const int bi0 = er0 + blockIdx.y * blockDim.y, bj0 = ec0 + blockIdx.x * blockDim.x;
+  const bool interior = bi0 - T.dmax_y >= 0 && bi0 + (int)blockDim.y - 1 + T.dmax_y < W.h &&
+                        bj0 - T.dmax_x >= 0 && bj0 + (int)blockDim.x - 1 + T.dmax_x < W.w;
   const float z = W.p[(int64_t)i * W.ld + j];
   float res;
   if (isnan(z)) {
     res = 1.0f;
   } else {
-    float total = 0.0f, count = 0.0f;
-    for (int e = 0; e < T.n; e++) {
-      const short2 o = T.off[e];
-      if (o.x == 0 && o.y == 0) continue;
-      const int si = clampi(i + o.y, W.h), sj = clampi(j + o.x, W.w);
-      const float s = W.p[(int64_t)si * W.ld + sj];
-      if (isnan(s)) continue;
-      const float a = atanf((s - z) * T.inv[e]);
-      float occ = fmaxf(0.0f, a) / T.quarter_pi;
-      occ = fminf(occ, 1.0f);
-      total = total + occ * T.df[e];
-      count = count + 1.0f;
-    }
+    float total, count;
+    if (interior)
+      ao_sum<false>(W, i, j, z, T, total, count);
+    else
+      ao_sum<true>(W, i, j, z, T, total, count);
     count = fmaxf(count, 1.0f);
     const float mean = total / count;
     float ao = 1.0f - mean * T.intensity;
@@ -546,6 +587,8 @@ extern "C" int fsg_ambient_occlusion(const fsg_window* win, const fsg_region* re
     T.off[e] = make_short2((short)s.dx, (short)s.dy);
     T.inv[e] = 1.0f / (float)s.phys;
     T.df[e] = (float)s.dist_factor;
+    T.dmax_y = std::max(T.dmax_y, std::abs(s.dy));
+    T.dmax_x = std::max(T.dmax_x, std::abs(s.dx));
   }
   int er0, ec0, eh, ew;
   expanded(win, reg, &er0, &ec0, &eh, &ew);
