@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 7  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 8  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -117,6 +117,15 @@ CNN_GELU, CNN_LINEAR, CNN_GELU_GRAD = 0, 1, 2
 AGG_WEIGHTED, AGG_MEAN, AGG_SUM, AGG_MAX, AGG_MIN = 0, 1, 2, 3, 4
 
 
+class fsg_mode_stats(C.Structure):
+    _fields_ = [("mode", C.c_double), ("count", C.c_int64), ("n_valid", C.c_int64),
+                ("vmin", C.c_double), ("vmax", C.c_double), ("n_distinct", C.c_int64)]
+
+
+FILL_NONE, FILL_ENCLOSED, FILL_ALL = 0, 1, 2
+PREPARE_MAX_NODATA = 16
+
+
 class fsg_batch_config(C.Structure):
     _fields_ = [("wg_per_cu", C.c_int32), ("order", C.c_int32), ("n_cu", C.c_int32),
                 ("chunk", C.c_int32)]
@@ -168,6 +177,18 @@ _SIGS = {
                                     C.c_int32, C.c_int32, vp]),
     "fsg_combine_fields": (C.c_int, [P(vp), C.c_int32, C.c_int64, P(C.c_float), C.c_int32,
                                      C.c_int32, C.c_int32, P(fsg_output), vp]),
+    "fsg_prepare_sample": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, vp, C.c_int64, vp]),
+    "fsg_value_mode_workspace_size": (C.c_size_t, [C.c_int64]),
+    "fsg_value_mode": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_float, vp, vp, C.c_size_t, vp]),
+    "fsg_count_equal": (C.c_int, [vp, C.c_int64, C.c_float, vp, vp]),
+    "fsg_coarse_average": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                     P(C.c_float), C.c_int32, vp, C.c_int64, vp, vp, vp]),
+    "fsg_pushpull_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "fsg_pushpull_fill": (C.c_int, [vp, C.c_int64, vp, C.c_int32, C.c_int32, C.c_int32, vp,
+                                    C.c_int64, vp, C.c_size_t, vp]),
+    "fsg_edge_connected": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp, P(C.c_int32), vp]),
+    "fsg_prepare_fill": (C.c_int, [P(fsg_window), P(C.c_float), C.c_int32, C.c_int32, C.c_int32,
+                                   vp, vp, C.c_int32, C.c_int32, vp, C.c_int64, vp]),
     "fsg_cnn_conv3x3": (C.c_int, [P(fsg_cnn_conv), vp]),
     "fsg_cnn_conv3x3_wgrad": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp,
                                         vp, vp]),

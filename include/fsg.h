@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 7
+#define FSG_ABI_VERSION 8
 
 enum {
   FSG_OK = 0,
@@ -603,6 +603,77 @@ int fsg_batch_capture(fsg_batch* b, const fsg_batch_config* cfg, void* stream);
 int fsg_batch_replay(fsg_batch* b, void* stream);
 int fsg_batch_info(const fsg_batch* b, int64_t* info);
 void fsg_batch_destroy(fsg_batch* b);
+
+/* ------------------------------------------------------------------ */
+/* prepare: NoData inference and push-pull void fill                    */
+/* ------------------------------------------------------------------ */
+
+/*
+ * preprocess_dem_to_cog (io/dem_preprocess.py:462-727) for a raster already
+ * in device memory; the file reads and the COG write stay with the caller.
+ *
+ * fsg_prepare_sample: src.read(out_shape=(oh, ow), resampling=nearest)
+ *   (io/dem_preprocess.py:536-542): source index floor((i + 0.5) n_src / n_dst)
+ *   per axis.
+ * fsg_value_mode: the np.unique(..., return_counts=True) + argmax step of
+ *   _detect_sentinel_nodata (io/dem_preprocess.py:241-275) and
+ *   _detect_border_nodata (:191-238) over the values that are finite and not
+ *   the declared NoData: the most frequent value (the smallest on ties), its
+ *   count, the number of such values, the smallest and largest.  stats is
+ *   device memory, written when the stream reaches it.  Workspace:
+ *   fsg_value_mode_workspace_size(n); n < 2^31.
+ * fsg_count_equal: number of the n values equal to value (the
+ *   np.mean(sample == cand) of _detect_border_nodata); count is device memory.
+ * fsg_coarse_average: _nan_aware_coarse_average (io/dem_preprocess.py:135-188)
+ *   of the (ch k) x (cw k) sample: a cell is the mean of its sub-cells that
+ *   are finite and equal to none of the n_nodata values (<= 16), summed in
+ *   numpy's order; cvalid (ch x cw bytes) is 1 for cells with a finite mean;
+ *   n_void (device) counts the others.
+ * fsg_pushpull_fill: pushpull_fill (algorithms/_pyramid_fill.py:24-87) of the
+ *   coarse grid: scipy zoom (order 1, corner-aligned) pyramid, valid cells
+ *   kept; n_valid_cells = h w - n_void.  Synchronises the stream once per
+ *   level.  Workspace: fsg_pushpull_workspace_size(h, w).
+ * fsg_edge_connected: _edge_connected_mask (utils/nodata_handler.py:13-25):
+ *   cells of mask (1 = void) 4-connected to the grid edge.  flag is one
+ *   device int32; synchronises the stream once per sweep; sweeps (optional,
+ *   host) receives the number of sweeps.
+ * fsg_prepare_fill: _apply_band_fill and _sample_coarse (io/dem_preprocess.py:
+ *   729-826) over the whole raster: NoData (non-finite or equal to one of the
+ *   n_nodata values) -> NaN, then with do_fill the bilinear sample of the
+ *   coarse surface at the pixel centre (global coarse coordinates) where the
+ *   sampled exterior is <= 0.5 (FSG_FILL_ENCLOSED) or everywhere
+ *   (FSG_FILL_ALL).  out may alias src.data.
+ */
+enum { FSG_FILL_NONE = 0, FSG_FILL_ENCLOSED = 1, FSG_FILL_ALL = 2 };
+#define FSG_PREPARE_MAX_NODATA 16
+
+typedef struct fsg_mode_stats {
+  double mode;        /* most frequent valid value */
+  int64_t count;      /* its count */
+  int64_t n_valid;    /* finite values that are not the declared NoData */
+  double vmin, vmax;  /* smallest and largest of them */
+  int64_t n_distinct; /* distinct values among them */
+} fsg_mode_stats;
+
+int fsg_prepare_sample(const fsg_window* src, int32_t oh, int32_t ow, float* out, int64_t ldo,
+                       void* stream);
+size_t fsg_value_mode_workspace_size(int64_t n);
+int fsg_value_mode(const float* values, int64_t n, int32_t has_declared, float declared,
+                   fsg_mode_stats* stats, void* workspace, size_t workspace_bytes, void* stream);
+int fsg_count_equal(const float* values, int64_t n, float value, uint64_t* count, void* stream);
+int fsg_coarse_average(const float* sample, int64_t lds, int32_t ch, int32_t cw, int32_t k,
+                       const float* nodata, int32_t n_nodata, float* coarse, int64_t ldc,
+                       uint8_t* cvalid, uint64_t* n_void, void* stream);
+size_t fsg_pushpull_workspace_size(int32_t h, int32_t w);
+int fsg_pushpull_fill(const float* coarse, int64_t ldc, const uint8_t* valid, int32_t h,
+                      int32_t w, int32_t n_valid_cells, float* out, int64_t ldo, void* workspace,
+                      size_t workspace_bytes, void* stream);
+int fsg_edge_connected(const uint8_t* mask, int32_t h, int32_t w, uint8_t* ext, int32_t* flag,
+                       int32_t* sweeps, void* stream);
+int fsg_prepare_fill(const fsg_window* src, const float* nodata, int32_t n_nodata,
+                     int32_t fill_mode, int32_t do_fill, const float* surface,
+                     const uint8_t* exterior, int32_t ch, int32_t cw, float* out, int64_t ldo,
+                     void* stream);
 
 #ifdef __cplusplus
 }

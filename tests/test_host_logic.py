@@ -94,3 +94,57 @@ def test_row_scales_resolution():
         assert v[k] == pytest.approx(meters_per_degree(45.0 - (10 + k + 0.5) / 3600)[0] / 3600,
                                      rel=1e-12)
     assert np.all(np.diff(row_geographic_scales_x(1 / 3600, -1 / 3600, 45.0, 0, 100)) > 0)
+
+
+# --------------------------------------------- prepare: detection decisions
+
+def _np_stats(vals, src_nodata):
+    import numpy as np
+    v = np.asarray(vals, dtype=np.float32).reshape(-1)
+    v = v[np.isfinite(v)]
+    if src_nodata is not None and np.isfinite(src_nodata):
+        v = v[v != np.float32(src_nodata)]
+    if v.size == 0:
+        return {"mode": 0.0, "count": 0, "n_valid": 0, "vmin": 0.0, "vmax": 0.0,
+                "n_distinct": 0}
+    u, c = np.unique(v, return_counts=True)
+    i = int(np.argmax(c))
+    return {"mode": float(u[i]), "count": int(c[i]), "n_valid": int(v.size),
+            "vmin": float(u[0]), "vmax": float(u[-1]), "n_distinct": int(u.size)}
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_prepare_detection_decisions_follow_the_oracle(monkeypatch, seed):
+    # the host decisions on the device statistics (mode, counts, range) give
+    # the oracle's answers; the statistics themselves are GPU-tested
+    import numpy as np
+    import torch
+    from fujishadergpu_amd import prepare as P
+    from oracle import prepare as prep
+    monkeypatch.setattr(P, "value_mode", lambda v, nd: _np_stats(v.numpy(), nd))
+    monkeypatch.setattr(P, "count_equal",
+                        lambda v, x: int((v.numpy() == np.float32(x)).sum()))
+    rng = np.random.default_rng(seed)
+    z = np.round(rng.random((48, 56)) * 300 + 50).astype(np.float32)
+    kind = seed % 3
+    if kind == 0:
+        z[:3] = z[-3:] = z[:, :3] = z[:, -3:] = -32768.0
+    elif kind == 1:
+        z[rng.random(z.shape) < 0.12] = z.max()
+    else:
+        z[rng.random(z.shape) < 0.04] = 0.0
+    decl = None if seed < 3 else -32768.0
+    valid = np.isfinite(z) & ~(prep.declared_mask(z, decl))
+    t = torch.from_numpy(z)
+    assert P.detect_sentinel_nodata(t, decl) == prep.detect_sentinel_nodata(z, valid)
+    assert P.detect_border_nodata(t, decl) == prep.detect_border_nodata(z, valid)
+
+
+def test_prepare_coarse_geometry_matches_the_oracle():
+    from fujishadergpu_amd import prepare as P
+    from oracle import prepare as prep
+    for w, h, cm in ((240000, 220000, 2048), (16384, 16384, 2048), (700, 600, 160),
+                     (5000, 300, 2048), (9000, 9001, 1000)):
+        ch, cw = P.coarse_shape(w, h, cm)
+        assert (ch, cw) == prep.coarse_shape(w, h, cm)
+        assert P.coarse_oversample(ch, cw) == prep.coarse_oversample(ch, cw)

@@ -23,7 +23,7 @@ def test_binding_table_covers_header():
 
 def test_loader_validates_abi_and_symbols():
     L = N.lib()
-    assert L.fsg_abi_version() == N.ABI_VERSION == 7
+    assert L.fsg_abi_version() == N.ABI_VERSION == 8
     assert L.fsg_launch_count() >= 0
 
 
@@ -34,6 +34,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(N.fsg_output) == 72
     assert ctypes.sizeof(N.fsg_ray_sample) == 24
     assert ctypes.sizeof(N.fsg_ao_sample) == 32
+    assert ctypes.sizeof(N.fsg_mode_stats) == 48
 
 
 def test_invalid_arguments_fail_without_gpu():

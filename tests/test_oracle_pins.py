@@ -728,3 +728,156 @@ def test_spatial_openness_single_radius_is_the_ray_block():
     blk = ref.ray_spatial_block("openness", ref.dask_reflect_pad(z, d), 12,
                                 {"pixel_size": 10.0, "num_directions": 8})
     assert np.array_equal(out, blk[d:-d, d:-d])
+
+
+# ------------------------------------------------------ prepare (NoData + fill)
+
+from oracle import prepare as prep  # noqa: E402
+
+
+def test_prepare_fill_coarse_surface_fills_voids_and_preserves_valid():
+    # reference tests/test_dem_preprocess.py:37-49
+    coarse = np.arange(64, dtype=F32).reshape(8, 8)
+    valid = np.ones((8, 8), dtype=bool)
+    valid[3:5, 3:5] = False
+    out = prep.fill_coarse_surface(coarse, valid)
+    assert np.isfinite(out).all()
+    assert np.allclose(out[valid], coarse[valid])
+    assert float(out[~valid].min()) >= float(coarse[valid].min()) - 1e-3
+    assert float(out[~valid].max()) <= float(coarse[valid].max()) + 1e-3
+
+
+def test_prepare_fill_coarse_surface_flat_region_stays_flat():
+    # reference tests/test_dem_preprocess.py:52-65
+    coarse = np.full((128, 128), 100.0, dtype=F32)
+    valid = np.ones((128, 128), dtype=bool)
+    valid[40:90, 40:90] = False
+    out = prep.fill_coarse_surface(coarse, valid)
+    assert np.isfinite(out).all()
+    assert np.allclose(out[~valid], 100.0, atol=1e-2)
+    assert float(out[~valid].std()) < 1e-2
+
+
+def test_prepare_fill_coarse_surface_all_invalid_returns_zeros():
+    # reference tests/test_dem_preprocess.py:68-75
+    out = prep.fill_coarse_surface(np.full((4, 4), np.nan, F32), np.zeros((4, 4), bool))
+    assert np.isfinite(out).all() and np.all(out == 0.0)
+
+
+def test_prepare_nan_aware_coarse_average_no_boundary_blend():
+    # reference tests/test_dem_preprocess.py:78-103 (the raster as the NEAREST
+    # read of itself: 256^2 at coarse_max 64 is a 4x oversample of 64^2)
+    H = W = 256
+    data = np.zeros((H, W), F32)
+    data[:, W // 2:] = 5000.0
+    ch, cw = prep.coarse_shape(W, H, 64)
+    k = prep.coarse_oversample(ch, cw)
+    assert (ch, cw, k) == (64, 64, 4)
+    nn = prep.nearest_sample(data, ch * k, cw * k)
+    assert np.array_equal(nn, data)
+    coarse, cmask = prep.nan_aware_coarse_average(nn, np.zeros_like(nn, bool), ch, cw, k,
+                                                  None, [0.0])
+    valid = coarse[~cmask]
+    assert valid.size > 0 and valid.min() >= 4999.0 and valid.max() <= 5001.0
+    assert cmask.any()
+    assert not ((coarse > 1.0) & (coarse < 4000.0)).any()
+
+
+def test_prepare_coarse_shape_caps_longest_side():
+    # reference tests/test_dem_preprocess.py:106-110
+    ch, cw = prep.coarse_shape(240000, 220000, 2048)
+    assert max(ch, cw) <= 2048
+    assert abs((cw / ch) - (240000 / 220000)) < 0.05
+
+
+def test_prepare_nearest_sample_integer_ratios():
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(12, 20)).astype(F32)
+    up = prep.nearest_sample(a, 36, 60)                       # x3 up: each pixel repeated
+    assert np.array_equal(up, np.repeat(np.repeat(a, 3, 0), 3, 1))
+    dn = prep.nearest_sample(a, 6, 5)                         # /2 rows, /4 cols: centre cells
+    assert np.array_equal(dn, a[1::2][:, 2::4])
+
+
+def _bfs_exterior(mask):
+    h, w = mask.shape
+    ext = np.zeros_like(mask)
+    stack = [(i, j) for i in range(h) for j in range(w)
+             if mask[i, j] and (i in (0, h - 1) or j in (0, w - 1))]
+    for i, j in stack:
+        ext[i, j] = True
+    while stack:
+        i, j = stack.pop()
+        for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+            u, v = i + di, j + dj
+            if 0 <= u < h and 0 <= v < w and mask[u, v] and not ext[u, v]:
+                ext[u, v] = True
+                stack.append((u, v))
+    return ext
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_prepare_edge_connected_is_a_4_connected_flood_fill(seed):
+    rng = np.random.default_rng(seed)
+    m = rng.random((37, 41)) < 0.45
+    assert np.array_equal(prep.edge_connected_mask(m), _bfs_exterior(m))
+
+
+def test_prepare_sentinel_rule():
+    rng = np.random.default_rng(4)
+    z = (rng.random((64, 64)) * 800 + 100).astype(F32)
+    v = np.ones_like(z, bool)
+    s = z.copy()
+    s[rng.random(z.shape) < 0.1] = -9999.0                    # a known sentinel, 10 %
+    assert prep.detect_sentinel_nodata(s, v) == -9999.0
+    s = z.copy()
+    s[rng.random(z.shape) < 0.1] = 512.25                     # interior, not a sentinel
+    assert prep.detect_sentinel_nodata(s, v) is None
+    s = z.copy()
+    s[rng.random(z.shape) < 0.1] = 4321.5                     # the range maximum
+    assert prep.detect_sentinel_nodata(s, v) == 4321.5
+    s = z.copy()
+    s[rng.random(z.shape) < 0.02] = -9999.0                   # below min_fraction
+    assert prep.detect_sentinel_nodata(s, v) is None
+
+
+def test_prepare_border_rule():
+    rng = np.random.default_rng(5)
+    z = (rng.random((64, 64)) * 800 + 100).astype(F32)
+    v = np.ones_like(z, bool)
+    f = z.copy()
+    f[:6, :] = f[-6:, :] = f[:, :6] = f[:, -6:] = 0.0           # a constant frame
+    assert prep.detect_border_nodata(f, v) == 0.0
+    f = z.copy()
+    f[0, :] = 0.0                                             # one edge: under half the ring
+    assert prep.detect_border_nodata(f, v) is None
+    f = z.copy()
+    f[0, :] = f[-1, :] = f[:, 0] = f[:, -1] = 7.0            # whole ring, 6 % of the grid
+    assert prep.detect_border_nodata(f, v) == 7.0
+    assert prep.detect_border_nodata(f, v, min_total_fraction=0.07) is None
+
+
+def test_prepare_end_to_end_keeps_the_exterior_and_fills_the_holes():
+    # plane with an undeclared 0 frame (exterior, detected: 0 is a known
+    # sentinel covering 23 % of the grid) and an interior -9999 hole given as
+    # the override
+    H, W = 300, 340
+    yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    plane = (1000.0 + 0.5 * xx + 0.25 * yy).astype(F32)
+    z = plane.copy()
+    z[:20, :] = z[-20:, :] = z[:, :20] = z[:, -20:] = 0.0
+    z[140:160, 150:175] = -9999.0
+    out, info = prep.prepare(z, coarse_max=128, nodata_override=-9999.0)
+    assert info["extra_nodata"] == [-9999.0, 0.0]
+    assert info["detected"] == [("sentinel", 0.0)]
+    assert info["do_fill"] and info["exterior"] is not None and info["exterior"].any()
+    assert np.isnan(out[:10, :]).all() and np.isnan(out[:, -10:]).all()
+    inner = out[140:160, 150:175]
+    assert np.isfinite(inner).all()
+    assert np.abs(inner - plane[140:160, 150:175]).max() < 12.0   # a membrane over the hole
+    keep = (z != 0.0) & (z != -9999.0)
+    assert np.array_equal(out[keep], z[keep])
+    out_all, _ = prep.prepare(z, coarse_max=128, nodata_override=-9999.0, fill_mode="all")
+    assert np.isfinite(out_all).all()
+    out_none, _ = prep.prepare(z, coarse_max=128, nodata_override=-9999.0, fill_mode="none")
+    assert np.isnan(out_none[140:160, 150:175]).all() and np.isnan(out_none[:20]).all()
