@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--only-c2", action="store_true",
                     help="skip the per-algorithm workloads (C1 hillshade+slope, C3 curvature and "
                          "openness, C5 fused batch)")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "f1"], default=None,
+    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "f1", "f4"], default=None,
                     help="run only this per-algorithm workload and print its JSON (a profiling "
                          "aid; the bench line is the default run)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -384,6 +384,54 @@ def bench_f1(dev, n=16384):
                                        f"the Gaussian/decimated radii {small}"}}
 
 
+def _prepare_raster(n, dev, seed):
+    """fBm (H 0.8, 300-2800 m) with an undeclared 0 frame (n/25 wide on the
+    top and left edges), a declared -9999 block and 0.5 % NaN discs (r 20-200)."""
+    from fujishadergpu_amd import synth
+    z = synth.fbm(n, n, 0.8, 2500.0, seed=seed, device=dev) + 300.0
+    b = max(8, n // 25)
+    z[:b] = 0.0
+    z[:, :b] = 0.0
+    z[n // 2: n // 2 + n // 16, n // 3: n // 3 + n // 12] = -9999.0
+    synth.apply_discs(z, synth.nodata_discs(n, n, 0.005, seed=seed + 1, r_min=max(2, n // 800),
+                                            r_max=max(4, n // 80)), float("nan"))
+    return z
+
+
+def bench_f4(dev, n=16384):
+    """prepare (NoData inference + push-pull fill, fill_mode 'enclosed') of a
+    raster resident in HBM, as a whole job; the oracle's prepare on the same
+    construction at 2048^2 (coarse grid 256^2, the same ratio) for the CPU rate."""
+    import torch
+    from fujishadergpu_amd import prepare as P
+    z = _prepare_raster(n, dev, 5)
+    out = torch.empty_like(z)
+    _, info = P.preprocess_dem(z, src_nodata=-9999.0, out=out)
+    torch.cuda.synchronize()
+    t = _gpu_time(lambda: P.preprocess_dem(z, src_nodata=-9999.0, out=out), 5)
+    nodata_in = torch.isnan(z) | (z == -9999.0) | (z == 0.0)
+    filled = int((nodata_in & torch.isfinite(out)).sum())
+    from oracle import prepare as prep
+    m = 2048
+    zs = _prepare_raster(m, dev, 5).cpu().numpy()
+    t0 = time.perf_counter()
+    prep.prepare(zs, src_nodata=-9999.0, coarse_max=256)
+    tc = time.perf_counter() - t0
+    del z, out
+    torch.cuda.empty_cache()
+    return {"workload": f"prepare, {n}x{n} f32 fBm (H 0.8, 300-2800 m, seed 5) with an "
+                        f"undeclared 0 frame, a declared -9999 block and 0.5 % NaN discs; "
+                        f"fill_mode enclosed, coarse_max 2048",
+            "gpix_s": round(n * n / t / 1e9, 3), "ms": round(t * 1e3, 2),
+            "coarse_shape": list(info["coarse_shape"]), "oversample": info["oversample"],
+            "detected": info["detected"], "n_void_coarse": info["n_void"],
+            "edge_sweeps": info["sweeps"], "pixels_filled": filled,
+            "cpu_baseline": {"value": round(m * m / tc / 1e9, 5), "unit": "Gpix/s", "cores": 1,
+                             "kind": "port",
+                             "sample": f"oracle prepare of the same construction at {m}x{m} "
+                                       "(coarse_max 256)"}}
+
+
 def bench_c4(dev, n=65536, weights=None):
     """Learned openness surrogate at radius 1024 (16 directions, 10 m) on an
     n^2 fBm (H 0.7, 0-4000 m, seed 4) against the exact kernel: time of each,
@@ -453,7 +501,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload:
         fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-              "f1": bench_f1}[args.workload]
+              "f1": bench_f1, "f4": bench_f4}[args.workload]
         print(json.dumps({args.workload: fn(dev)}), flush=True)
         return
     n = args.size
@@ -534,7 +582,8 @@ def main():
         for name, fn in (("hillshade_slope_c1", bench_c1), ("batch_fused_c5", bench_c5),
                          ("curvature_openness_c3", bench_c3),
                          ("openness_surrogate_c4", bench_c4),
-                         ("spatial_hillshade_f1", bench_f1)):
+                         ("spatial_hillshade_f1", bench_f1),
+                         ("prepare_f4", bench_f4)):
             print(f"# {name} ...", file=sys.stderr, flush=True)
             line["per_algorithm"][name] = fn(dev)
     print(json.dumps(line), flush=True)

@@ -86,7 +86,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 __global__ void k_mode_finish(const float* __restrict__ sorted, const float* __restrict__ uniq,
                               const int* __restrict__ n_runs,
                               const hipcub::KeyValuePair<int, int>* __restrict__ best,
-                              const unsigned long long* __restrict__ n_valid,
+                              const unsigned long long* __restrict__ n_valid, int64_t n,
                               fsg_mode_stats* __restrict__ st) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const unsigned long long nv = *n_valid;
@@ -103,7 +103,7 @@ __global__ void k_mode_finish(const float* __restrict__ sorted, const float* __r
   st->count = best->value;
   st->vmin = (double)sorted[0];
   st->vmax = (double)sorted[nv - 1];
-  st->n_distinct = *n_runs;
+  st->n_distinct = (int64_t)*n_runs - (n - (int64_t)nv);   // less the NaN runs of 1
 }
 
 static int mode_layout(int64_t n, void* base, size_t bytes, ModeWs* w, size_t* need) {
@@ -253,11 +253,18 @@ __global__ void k_min_nonneg(const float* __restrict__ a, int64_t n, unsigned* _
        e += (int64_t)gridDim.x * blockDim.x)
     m = min(m, __float_as_uint(a[e]));
   for (int o = 32; o > 0; o >>= 1) m = min(m, (unsigned)__shfl_down((int)m, o));
-  if ((threadIdx.x & 63) == 0) atomicMin(out, m);
+  __shared__ unsigned part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) m = min(m, part[k]);
+    atomicMin(out, m);   // one atomic per block
+  }
 }
 
-__global__ void k_set_u32(unsigned* p, unsigned v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+__global__ void k_set_u32n(unsigned* p, int n, unsigned v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // push: num = zoom(vw), den = zoom(w) to (ho, wo); wv = min(den, 1);
@@ -433,6 +440,39 @@ __global__ void k_prepare_fill(FillArgs A) {
   }
 }
 
+// four pixels per lane (float4) when rows and pointers are 16-byte aligned
+__global__ void k_prepare_fill4(FillArgs A) {
+  const int j0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (j0 >= A.w) return;
+  for (int i = blockIdx.y; i < A.h; i += gridDim.y) {
+    const float4 v = *(const float4*)(A.src + (int64_t)i * A.lds + j0);
+    float o[4] = {v.x, v.y, v.z, v.w};
+    bool m[4];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      m[k] = is_nodata(o[k], A.L);
+      any = any || m[k];
+    }
+    if (any) {
+      const float rr = ((float)i + 0.5f) * A.fy - 0.5f;
+      const Lin Y = lin_at((double)rr, A.ch);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!m[k]) continue;
+        o[k] = __builtin_nanf("");
+        if (A.do_fill) {
+          const float cc = ((float)(j0 + k) + 0.5f) * A.fx - 0.5f;
+          const Lin X = lin_at((double)cc, A.cw);
+          const bool exterior = A.enclosed && A.ext && bilin_u8(A.ext, A.cw, Y, X) > 0.5f;
+          if (!exterior) o[k] = bilin(A.surface, A.cw, Y, X);
+        }
+      }
+    }
+    *(float4*)(A.out + (int64_t)i * A.ldo + j0) = make_float4(o[0], o[1], o[2], o[3]);
+  }
+}
+
 static dim3 grid_rows(int w, int h, int bx = 256) {
   return dim3((unsigned)((w + bx - 1) / bx), (unsigned)std::max(1, std::min(h, 16384)));
 }
@@ -493,27 +533,25 @@ extern "C" int fsg_value_mode(const float* values, int64_t n, int32_t has_declar
   hipLaunchKernelGGL(k_mode_prep, dim3(blocks), dim3(256), 0, s, values, n, has_declared,
                      declared, W.keys, W.counts, W.n_valid);
   FSG_CHECK_LAUNCH("k_mode_prep");
-  unsigned long long nv = 0;
-  if (hipError_t e = read_word(&nv, W.n_valid, sizeof(nv), s))
-    return check_hip(e, "fsg_value_mode: reading the valid count");
+  // invalid keys sort last as NaN, each its own run of 1 (NaN != NaN), so the
+  // first arg-max over all n run counts is the mode of the valid values
+  // whenever there is one; no host read is needed before the finish kernel
   size_t tb = W.temp_bytes;
   const int ni = (int)n;
   if (check_hip(hipcub::DeviceRadixSort::SortKeys(W.temp, tb, W.keys, W.sorted, ni, 0, 32, s),
                 "fsg_value_mode: sort"))
     return FSG_E_HIP;
-  if (nv > 0) {
-    tb = W.temp_bytes;
-    if (check_hip(hipcub::DeviceRunLengthEncode::Encode(W.temp, tb, W.sorted, W.uniq, W.counts,
-                                                        W.n_runs, (int)nv, s),
-                  "fsg_value_mode: run-length encode"))
-      return FSG_E_HIP;
-    tb = W.temp_bytes;
-    if (check_hip(hipcub::DeviceReduce::ArgMax(W.temp, tb, W.counts, W.best, (int)nv, s),
-                  "fsg_value_mode: arg-max"))
-      return FSG_E_HIP;
-  }
+  tb = W.temp_bytes;
+  if (check_hip(hipcub::DeviceRunLengthEncode::Encode(W.temp, tb, W.sorted, W.uniq, W.counts,
+                                                      W.n_runs, ni, s),
+                "fsg_value_mode: run-length encode"))
+    return FSG_E_HIP;
+  tb = W.temp_bytes;
+  if (check_hip(hipcub::DeviceReduce::ArgMax(W.temp, tb, W.counts, W.best, ni, s),
+                "fsg_value_mode: arg-max"))
+    return FSG_E_HIP;
   hipLaunchKernelGGL(k_mode_finish, dim3(1), dim3(64), 0, s, W.sorted, W.uniq, W.n_runs, W.best,
-                     W.n_valid, stats);
+                     W.n_valid, n, stats);
   FSG_CHECK_LAUNCH("k_mode_finish");
   return FSG_OK;
 }
@@ -562,7 +600,7 @@ extern "C" size_t fsg_pushpull_workspace_size(int32_t h, int32_t w) {
     ch = std::max(1, ch / 2);
     cw = std::max(1, cw / 2);
   }
-  total += 2 * align256(sizeof(float) * (size_t)h * w) + 256;
+  total += 2 * align256(sizeof(float) * (size_t)h * w) + 256;   // filled x 2, 64 minima
   return total;
 }
 
@@ -621,23 +659,20 @@ extern "C" int fsg_pushpull_fill(const float* coarse, int64_t ldc, const uint8_t
   }
   float* fa = take(sizeof(float) * n0);
   float* fb = take(sizeof(float) * n0);
-  unsigned* minw = (unsigned*)p;
-  size_t lvl = 0;
-  for (;;) {
+  unsigned* minw = (unsigned*)p;   // one word per level (at most 64 levels)
+  const int n_lv = (int)slots.size() + 1;
+  // every level down to 1 x 1 is pushed, with each level's min weight; the
+  // reference stops at the first level whose min weight exceeds 1e-6, so the
+  // levels past it are dropped after one read of the minima
+  hipLaunchKernelGGL(k_set_u32n, dim3(1), dim3(64), 0, s, minw, n_lv, 0x7f800000u);
+  FSG_CHECK_LAUNCH("k_set_u32n");
+  for (int l = 0;; l++) {
     const int ch = hs.back(), cw = wsz.back();
-    if (std::max(ch, cw) <= 1) break;
     const int64_t n = (int64_t)ch * cw;
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, minw, 0x7f800000u);
-    FSG_CHECK_LAUNCH("k_set_u32");
-    hipLaunchKernelGGL(k_min_nonneg, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 2048)),
-                       dim3(256), 0, s, wts.back(), n, minw);
+    hipLaunchKernelGGL(k_min_nonneg, dim3((unsigned)std::min<int64_t>((n + 1023) / 1024, 512)),
+                       dim3(256), 0, s, wts.back(), n, minw + l);
     FSG_CHECK_LAUNCH("k_min_nonneg");
-    unsigned mb = 0;
-    if (hipError_t e = read_word(&mb, minw, sizeof(mb), s))
-      return check_hip(e, "fsg_pushpull_fill: reading the min weight");
-    float mf;
-    std::memcpy(&mf, &mb, sizeof(mf));
-    if (!((double)mf <= 1e-6)) break;   // float(ws.min()) <= 1e-6 in float64
+    if (std::max(ch, cw) <= 1) break;
     const int nh = std::max(1, ch / 2), nw = std::max(1, cw / 2);
     // scipy zoom: output shape round(n * (m / n)), coordinate scale (n - 1) / (m' - 1)
     const int zh = (int)std::nearbyint((double)ch * ((double)nh / (double)ch));
@@ -646,8 +681,8 @@ extern "C" int fsg_pushpull_fill(const float* coarse, int64_t ldc, const uint8_t
       return fail(FSG_E_UNSUPPORTED, "fsg_pushpull_fill: zoom shape differs from the level");
     const double zy = zh > 1 ? (double)(ch - 1) / (double)(zh - 1) : 1.0;
     const double zx = zw > 1 ? (double)(cw - 1) / (double)(zw - 1) : 1.0;
-    float* nvw = (float*)slots[lvl];
-    float* nwt = (float*)(slots[lvl] + align256(sizeof(float) * (size_t)nh * nw));
+    float* nvw = (float*)slots[l];
+    float* nwt = (float*)(slots[l] + align256(sizeof(float) * (size_t)nh * nw));
     hipLaunchKernelGGL(k_pp_push, grid_rows(nw, nh), dim3(256), 0, s, vws.back(), wts.back(), ch,
                        cw, nvw, nwt, nh, nw, zy, zx);
     FSG_CHECK_LAUNCH("k_pp_push");
@@ -655,8 +690,22 @@ extern "C" int fsg_pushpull_fill(const float* coarse, int64_t ldc, const uint8_t
     wts.push_back(nwt);
     hs.push_back(nh);
     wsz.push_back(nw);
-    lvl++;
   }
+  std::vector<unsigned> mins(n_lv);
+  if (hipError_t e = read_word(mins.data(), minw, sizeof(unsigned) * n_lv, s))
+    return check_hip(e, "fsg_pushpull_fill: reading the level minima");
+  // levels kept: 0 .. L where L is the first level that stops the push
+  int L = 0;
+  while (L + 1 < n_lv) {
+    float mf;
+    std::memcpy(&mf, &mins[L], sizeof(mf));
+    if (!((double)mf <= 1e-6)) break;   // float(ws.min()) <= 1e-6 in float64
+    L++;
+  }
+  vws.resize(L + 1);
+  wts.resize(L + 1);
+  hs.resize(L + 1);
+  wsz.resize(L + 1);
   // pull from the top level down
   int fh = hs.back(), fw = wsz.back();
   float* cur = fa;
@@ -695,11 +744,14 @@ extern "C" int fsg_edge_connected(const uint8_t* mask, int32_t h, int32_t w, uin
   const int64_t max_sweeps = (int64_t)h * w + 2;
   int64_t n = 0;
   for (;;) {
-    hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(64), 0, s, flag, 0);
-    FSG_CHECK_LAUNCH("k_set_i32");
-    hipLaunchKernelGGL(k_ec_sweep, grid, dim3(256), 0, s, mask, h, w, ext, flag);
-    FSG_CHECK_LAUNCH("k_ec_sweep");
-    n++;
+    // four sweeps per host read; the flag read is the last sweep's
+    for (int b = 0; b < 4; b++) {
+      hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(64), 0, s, flag, 0);
+      FSG_CHECK_LAUNCH("k_set_i32");
+      hipLaunchKernelGGL(k_ec_sweep, grid, dim3(256), 0, s, mask, h, w, ext, flag);
+      FSG_CHECK_LAUNCH("k_ec_sweep");
+      n++;
+    }
     int f = 0;
     if (hipError_t e = read_word(&f, flag, sizeof(f), s))
       return check_hip(e, "fsg_edge_connected: reading the sweep flag");
@@ -738,8 +790,16 @@ extern "C" int fsg_prepare_fill(const fsg_window* src, const float* nodata, int3
   A.out = out;
   A.ldo = ldo;
   TimedScope ts("k_prepare_fill", as_stream(stream));
-  hipLaunchKernelGGL(k_prepare_fill, grid_rows(src->w, src->h), dim3(256), 0, as_stream(stream),
-                     A);
-  FSG_CHECK_LAUNCH("k_prepare_fill");
+  const bool vec = src->w % 4 == 0 && src->ld % 4 == 0 && ldo % 4 == 0 &&
+                   ((uintptr_t)src->data & 15) == 0 && ((uintptr_t)out & 15) == 0;
+  if (vec) {
+    hipLaunchKernelGGL(k_prepare_fill4, grid_rows(src->w / 4, src->h), dim3(256), 0,
+                       as_stream(stream), A);
+    FSG_CHECK_LAUNCH("k_prepare_fill4");
+  } else {
+    hipLaunchKernelGGL(k_prepare_fill, grid_rows(src->w, src->h), dim3(256), 0,
+                       as_stream(stream), A);
+    FSG_CHECK_LAUNCH("k_prepare_fill");
+  }
   return FSG_OK;
 }
