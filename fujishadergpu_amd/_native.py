@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 8  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 9  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -164,6 +164,8 @@ _SIGS = {
                                      C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_int32,
                                      vp, vp, C.c_size_t, vp]),
     "fsg_pad_symmetric": (C.c_int, [P(fsg_window), C.c_int32, vp, C.c_int64, vp]),
+    "fsg_pad_reflect": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp,
+                                  C.c_int64, vp]),
     "fsg_gaussian_nan_workspace_size": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
     "fsg_gaussian_nan": (C.c_int, [P(fsg_window), C.c_int32, C.c_double, vp, C.c_int64, vp,
                                    C.c_size_t, vp]),

@@ -418,6 +418,33 @@ extern "C" int fsg_pad_symmetric(const fsg_window* src, int32_t pad, float* out,
   return FSG_OK;
 }
 
+// dask map_overlap boundary='reflect' at the raster edges of a window: each
+// side padded by its own amount with numpy's 'symmetric' rule
+__global__ void k_pad_sides(const float* __restrict__ src, int64_t lds, int h, int w, int top,
+                            int left, float* __restrict__ out, int64_t ldo, int ho, int wo) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= wo) return;
+  const int sc = sym(c - left, w);
+  for (int r = blockIdx.y; r < ho; r += gridDim.y)
+    out[(int64_t)r * ldo + c] = src[(int64_t)sym(r - top, h) * lds + sc];
+}
+
+extern "C" int fsg_pad_reflect(const fsg_window* src, int32_t top, int32_t bottom, int32_t left,
+                               int32_t right, float* out, int64_t ldo, void* stream) {
+  int rc = validate_window(src, "fsg_pad_reflect");
+  if (rc) return rc;
+  if (src->h <= 0 || src->w <= 0 || !out || top < 0 || bottom < 0 || left < 0 || right < 0 ||
+      std::max(top, bottom) > src->h || std::max(left, right) > src->w ||
+      ldo < (int64_t)src->w + left + right)
+    return fail(FSG_E_INVALID,
+                "fsg_pad_reflect: bad arguments (each pad at most the window extent)");
+  const int ho = src->h + top + bottom, wo = src->w + left + right;
+  hipLaunchKernelGGL(k_pad_sides, grid2(wo, ho), dim3(256), 0, as_stream(stream), src->data,
+                     src->ld, src->h, src->w, top, left, out, ldo, ho, wo);
+  FSG_CHECK_LAUNCH("k_pad_sides");
+  return FSG_OK;
+}
+
 extern "C" int fsg_upsample_to_shape(const float* src, int64_t lds, int32_t hs, int32_t ws,
                                      float* out, int64_t ldo, int32_t h, int32_t w,
                                      void* stream) {

@@ -163,6 +163,18 @@ def openness_params(table, openness_type: str, raw_degrees: bool = False):
     return p, arr
 
 
+def pad_reflect(win_t: torch.Tensor, top: int, bottom: int, left: int,
+                right: int) -> torch.Tensor:
+    """The window padded per side with the symmetric ('reflect' in dask) rule."""
+    h, w = win_t.shape
+    out = torch.empty((h + top + bottom, w + left + right), dtype=torch.float32,
+                      device=win_t.device)
+    wd = window(win_t)
+    N.check(_lib().fsg_pad_reflect(C.byref(wd), int(top), int(bottom), int(left), int(right),
+                                   out.data_ptr(), out.stride(0), _stream()))
+    return out
+
+
 def openness(win_t, reg, *, table, openness_type: str, out: N.fsg_output,
              raw_degrees: bool = False) -> None:
     p, _arr = openness_params(table, openness_type, raw_degrees)

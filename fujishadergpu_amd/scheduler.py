@@ -219,15 +219,47 @@ def overview_field(raster: torch.Tensor, large_radii, large_weights) -> Tuple[to
     return ops.topousm_coarse_field(coarse, rc, large_weights), scale
 
 
+EDGE_MODES = ("tile", "reflect")
+
+
+def dask_edge_depth(algorithm: str, params: dict) -> int:
+    """The map_overlap depth of the Dask backend's local-mode path
+    (_impl_hillshade.py:133-134, _impl_slope.py:71-72, _impl_curvature.py:92-93,
+    _impl_openness.py:204-207, _impl_ambient_occlusion.py:191-194)."""
+    if algorithm in ("hillshade", "slope", "hillshade_slope"):
+        return 1
+    if algorithm == "curvature":
+        return 2
+    if algorithm == "openness":
+        return int(params.get("max_distance", 50)) + 1
+    if algorithm == "ambient_occlusion":
+        return int(float(params.get("radius", 10.0)) + 1)
+    raise UnsupportedOnBackend(
+        f"{algorithm}: edge_mode='reflect' is not defined here (the Dask backend aligns its "
+        "decimation grids to its own chunks, which the tile windows do not reproduce)")
+
+
 def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float] = None,
                 nodata_threshold: float = 1.0, out: Optional[torch.Tensor] = None,
                 out_f32: Optional[torch.Tensor] = None, global_stats=None,
-                copy_input: bool = True, return_plan: bool = False, **kwargs):
+                copy_input: bool = True, return_plan: bool = False, edge_mode: str = "tile",
+                **kwargs):
     """Run one algorithm over a device-resident raster, tile by tile.
 
     Returns the output raster (float32 with NaN NoData, or int16/uint8 with 0
     NoData), and the per-tile TileResult list.  ``out_f32`` optionally also
-    receives the float32 result when the main output is integer-encoded."""
+    receives the float32 result when the main output is integer-encoded.
+
+    edge_mode: "tile" (the tile backend: windows end at the raster edge and
+    each block function applies its own edge rule there) or "reflect" (the
+    Dask backend's outer ring: a local-mode window that touches a raster edge
+    is extended beyond it by the algorithm's map_overlap depth with the
+    symmetric rule of boundary='reflect', core/dask_processor.py:1473-1488).
+    Spatial mode already reflects at every window edge (each tile is one
+    map_overlap chunk), so there both modes are the same."""
+    edge_mode = str(edge_mode).lower()
+    if edge_mode not in EDGE_MODES:
+        raise ValueError(f"edge_mode must be one of {EDGE_MODES}, got {edge_mode!r}")
     P = plan(raster, algorithm, nodata=nodata, **kwargs)
     H, W = raster.shape
     src = raster.clone() if copy_input else raster
@@ -259,6 +291,12 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     if algorithm == "topousm_fast" and raster_nan_free:
         pyramid = CoarsePyramid(src)
         pyramid.require(CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(P.params)))
+    reflect_d = 0
+    if edge_mode == "reflect" and not spatial_tiles:
+        reflect_d = dask_edge_depth(algorithm, P.params)
+        if P.params.get("_geographic_rows"):
+            raise ValueError("edge_mode='reflect' takes one pixel scale per tile "
+                             "(geographic_rows is not defined for it)")
     stats = global_stats if global_stats is not None else P.params.get("global_stats")
     if stats is None:
         stats = norm_stats_prepass(src, algorithm, P.params, raster_nan_free=raster_nan_free,
@@ -304,6 +342,14 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
                 p["pixel_scale_x_rows"] = row_geographic_scales_x(a, e, f, int(wy), int(wh))
         win = src[wy:wy + wh, wx:wx + ww]
         reg = ops.region(cy - wy, cx - wx, ch, cw)
+        if reflect_d:
+            top = min(reflect_d, wh) if wy == 0 else 0
+            left = min(reflect_d, ww) if wx == 0 else 0
+            bottom = min(reflect_d, wh) if wy + wh == H else 0
+            right = min(reflect_d, ww) if wx + ww == W else 0
+            if top or left or bottom or right:
+                win = ops.pad_reflect(win, top, bottom, left, right)
+                reg = ops.region(cy - wy + top, cx - wx + left, ch, cw)
         spec = OutSpec(f32=o_view if dt == torch.float32 else f_view,
                        q=o_view if dt != torch.float32 else None, qp=P.qp,
                        clip01=(algorithm == "hillshade"))

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 8
+#define FSG_ABI_VERSION 9
 
 enum {
   FSG_OK = 0,
@@ -432,6 +432,16 @@ enum {
 
 int fsg_pad_symmetric(const fsg_window* src, int32_t pad, float* out, int64_t ldo,
                       void* stream);
+/*
+ * fsg_pad_reflect: the window padded by its own amount on each side with the
+ * same symmetric rule -- the Dask backend's map_overlap(boundary='reflect')
+ * at the raster edges (core/dask_processor.py:1473-1488 runs each algorithm's
+ * map_overlap over the whole raster), for windows that touch them.  Each pad
+ * is at most the window's extent on its axis; out is (h + top + bottom) x
+ * (w + left + right).
+ */
+int fsg_pad_reflect(const fsg_window* src, int32_t top, int32_t bottom, int32_t left,
+                    int32_t right, float* out, int64_t ldo, void* stream);
 size_t fsg_gaussian_nan_workspace_size(int32_t h, int32_t w, int32_t pad);
 int fsg_gaussian_nan(const fsg_window* src, int32_t pad, double sigma, float* out, int64_t ldo,
                      void* workspace, size_t workspace_bytes, void* stream);

@@ -777,6 +777,31 @@ def smooth_for_radius(block, radius, *, pixel_size=1.0, algorithm_name="default"
     return upsample_to_shape(small, block.shape)
 
 
+def dask_edge_depth(algorithm: str, params: dict) -> int:
+    """map_overlap depth of the Dask backend's local path (_impl_hillshade.py:133-134,
+    _impl_slope.py:71-72, _impl_curvature.py:92-93, _impl_openness.py:204-207,
+    _impl_ambient_occlusion.py:191-194)."""
+    if algorithm in ("hillshade", "slope"):
+        return 1
+    if algorithm == "curvature":
+        return 2
+    if algorithm == "openness":
+        return int(params.get("max_distance", 50)) + 1
+    if algorithm == "ambient_occlusion":
+        return int(float(params.get("radius", 10.0)) + 1)
+    raise ValueError(algorithm)
+
+
+def dask_single_chunk(algorithm: str, raster, params: dict) -> np.ndarray:
+    """The Dask backend's local-mode result on a raster held as one chunk:
+    map_overlap(block_fn, depth=D, boundary='reflect') pads the chunk with the
+    symmetric rule (dask.array.overlap.reflect), runs the block function and
+    trims D (core/dask_processor.py:1473-1488)."""
+    d = dask_edge_depth(algorithm, params)
+    res = run_window(algorithm, dask_reflect_pad(raster, d), params)
+    return np.asarray(res, dtype=F32)[d:-d, d:-d]
+
+
 RAY_SPATIAL = ("openness", "ambient_occlusion")
 
 

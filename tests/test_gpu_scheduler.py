@@ -155,3 +155,44 @@ def test_geographic_raster_tile_and_row_latitudes(cuda, algorithm, rows):
     else:
         tol = {"slope": 3e-4, "hillshade": 1e-6, "curvature": 4e-4}[algorithm]
         assert np.nanmax(np.abs(got - want)) <= tol
+
+
+@pytest.mark.parametrize("algorithm,kw,tol", [
+    ("hillshade", dict(mode="local"), 2e-6),
+    ("slope", dict(mode="local"), 1e-4),
+    ("curvature", dict(mode="local", curvature_type="mean"), 4e-4),
+    ("openness", dict(mode="local", num_directions=8, max_distance=20), 1e-5),
+    ("ambient_occlusion", dict(mode="local", radius=6.0, num_samples=8), 1e-5),
+])
+def test_reflect_edge_mode_matches_the_dask_single_chunk(cuda, algorithm, kw, tol):
+    # edge_mode='reflect': windows touching the raster edge are extended by the
+    # Dask map_overlap depth with the symmetric rule; over a multi-tile raster
+    # the result equals the Dask backend's single-chunk result everywhere
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _raster(300, 340, 23)
+    stats = (0.0, 1.0)                       # identity stretch for openness / AO
+    out, _, P = process_dem(host_t.to(cuda), algorithm, tile_size=128, pixel_size=10.0,
+                            edge_mode="reflect", global_stats=stats, return_plan=True, **kw)
+    p = {k: v for k, v in P.params.items() if not isinstance(v, torch.Tensor)}
+    p["global_stats"] = stats
+    exp = ref.format_output(ref.dask_single_chunk(algorithm, host_t.numpy(), p), algorithm)
+    got = out.cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(exp))
+    d = np.abs(got.astype(np.float64) - exp)
+    print(algorithm, "max |d|", float(np.nanmax(d)), "ring max |d|",
+          float(max(np.nanmax(d[:3]), np.nanmax(d[-3:]), np.nanmax(d[:, :3]),
+                    np.nanmax(d[:, -3:]))), flush=True)
+    assert np.nanmax(d) <= tol
+    # and the tile backend's own edge rule differs on the outer ring
+    tile, _ = process_dem(host_t.to(cuda), algorithm, tile_size=128, pixel_size=10.0,
+                          global_stats=stats, **kw)
+    assert not torch.equal(tile[:, 0], out[:, 0])
+
+
+def test_reflect_edge_mode_rejects_topousm(cuda):
+    from fujishadergpu_amd.algorithms import UnsupportedOnBackend
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _raster(256, 256, 3)
+    with pytest.raises(UnsupportedOnBackend):
+        process_dem(host_t.to(cuda), "topousm_fast", tile_size=128, pixel_size=10.0,
+                    edge_mode="reflect", radii=[4, 16], weights=[0.5, 0.5])

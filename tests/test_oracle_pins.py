@@ -881,3 +881,38 @@ def test_prepare_end_to_end_keeps_the_exterior_and_fills_the_holes():
     assert np.isfinite(out_all).all()
     out_none, _ = prep.prepare(z, coarse_max=128, nodata_override=-9999.0, fill_mode="none")
     assert np.isnan(out_none[140:160, 150:175]).all() and np.isnan(out_none[:20]).all()
+
+
+# ------------------------------------------- Dask edge semantics (reflect)
+
+def test_dask_edge_hillshade_of_a_plane_halves_the_edge_gradient():
+    # boundary='reflect' mirrors [a | a b ..]: the central difference at the
+    # first column is (z1 - z0) / (2 h), half the plane's slope, where the
+    # tile backend's one-sided edge_order=2 stencil gives the full slope
+    h, w, a, px = 40, 50, 0.3, 10.0
+    z = np.tile((a * px * np.arange(w)).astype(F32), (h, 1))
+    kw = dict(pixel_size=px, pixel_scale_x=px, pixel_scale_y=px)
+    dk = ref.dask_single_chunk("hillshade", z, kw)
+    tile = ref.run_window("hillshade", z, kw)
+    half = np.tile((0.5 * a * px * np.arange(w)).astype(F32), (h, 1))
+    want_edge = ref.run_window("hillshade", half, kw)[5, 10]    # interior value of slope a / 2
+    assert abs(float(dk[5, 0]) - float(want_edge)) < 1e-6
+    assert abs(float(dk[5, -1]) - float(want_edge)) < 1e-6
+    assert abs(float(tile[5, 0]) - float(tile[5, 10])) < 1e-6   # one-sided: full slope
+    assert np.array_equal(dk[2:-2, 2:-2], tile[2:-2, 2:-2])        # the interior is the same
+
+
+def test_dask_edge_slope_closed_form_at_the_corner():
+    h, w, a, b, px = 30, 30, 0.2, 0.1, 5.0
+    yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    z = (a * px * xx + b * px * yy).astype(F32)
+    dk = ref.dask_single_chunk("slope", z, dict(pixel_size=px))
+    # at (0, 0) both axes see half the step: atan(hypot(a, b) / 2) in degrees
+    assert abs(float(dk[0, 0]) - np.degrees(np.arctan(np.hypot(a, b) / 2))) < 2e-5
+    assert abs(float(dk[15, 15]) - np.degrees(np.arctan(np.hypot(a, b)))) < 2e-5
+
+
+def test_dask_edge_depths():
+    assert ref.dask_edge_depth("curvature", {}) == 2
+    assert ref.dask_edge_depth("openness", {"max_distance": 20}) == 21
+    assert ref.dask_edge_depth("ambient_occlusion", {"radius": 7.5}) == 8
