@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 9  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 10  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -122,6 +122,34 @@ class fsg_mode_stats(C.Structure):
                 ("vmin", C.c_double), ("vmax", C.c_double), ("n_distinct", C.c_int64)]
 
 
+class fsg_cog_raster(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("ld", C.c_int64), ("h", C.c_int32), ("w", C.c_int32),
+                ("dtype", C.c_int32), ("has_geotransform", C.c_int32),
+                ("geotransform", C.c_double * 6), ("epsg", C.c_int32),
+                ("epsg_geographic", C.c_int32), ("has_nodata", C.c_int32), ("pad0", C.c_int32),
+                ("nodata", C.c_double)]
+
+
+class fsg_cog_options(C.Structure):
+    _fields_ = [("block", C.c_int32), ("zstd_level", C.c_int32), ("overview_count", C.c_int32),
+                ("threads", C.c_int32)]
+
+
+class fsg_cog_report(C.Structure):
+    _fields_ = [("n_levels", C.c_int32), ("pad0", C.c_int32), ("data_start", C.c_uint64),
+                ("compressed_bytes", C.c_uint64), ("raw_bytes", C.c_uint64)]
+
+
+class fsg_cog_info_t(C.Structure):
+    _fields_ = [("n_levels", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("tile_w", C.c_int32), ("tile_h", C.c_int32), ("dtype", C.c_int32),
+                ("compression", C.c_int32), ("predictor", C.c_int32),
+                ("has_geotransform", C.c_int32), ("pad0", C.c_int32),
+                ("geotransform", C.c_double * 6), ("epsg", C.c_int32),
+                ("epsg_geographic", C.c_int32), ("has_nodata", C.c_int32), ("pad1", C.c_int32),
+                ("nodata", C.c_double)]
+
+
 FILL_NONE, FILL_ENCLOSED, FILL_ALL = 0, 1, 2
 PREPARE_MAX_NODATA = 16
 
@@ -179,6 +207,13 @@ _SIGS = {
                                     C.c_int32, C.c_int32, vp]),
     "fsg_combine_fields": (C.c_int, [P(vp), C.c_int32, C.c_int64, P(C.c_float), C.c_int32,
                                      C.c_int32, C.c_int32, P(fsg_output), vp]),
+    "fsg_cog_write": (C.c_int, [C.c_char_p, P(fsg_cog_raster), P(fsg_cog_options),
+                                P(fsg_cog_report), vp]),
+    "fsg_cog_open": (C.c_int, [C.c_char_p, P(vp)]),
+    "fsg_cog_info": (C.c_int, [vp, C.c_int32, P(fsg_cog_info_t)]),
+    "fsg_cog_read": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp,
+                               C.c_int64, C.c_int32, vp]),
+    "fsg_cog_close": (None, [vp]),
     "fsg_prepare_sample": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, vp, C.c_int64, vp]),
     "fsg_value_mode_workspace_size": (C.c_size_t, [C.c_int64]),
     "fsg_value_mode": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_float, vp, vp, C.c_size_t, vp]),

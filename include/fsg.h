@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 9
+#define FSG_ABI_VERSION 10
 
 enum {
   FSG_OK = 0,
@@ -684,6 +684,82 @@ int fsg_prepare_fill(const fsg_window* src, const float* nodata, int32_t n_nodat
                      int32_t fill_mode, int32_t do_fill, const float* surface,
                      const uint8_t* exterior, int32_t ch, int32_t cw, float* out, int64_t ldo,
                      void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Cloud-optimized GeoTIFF output and windowed input                    */
+/* ------------------------------------------------------------------ */
+
+/*
+ * The on-disk side of the tile pipeline: the output COG the reference builds
+ * with GDAL (io/cog_builder.py:658-716; options core/dask_processor.py:201-228:
+ * COMPRESS=ZSTD, LEVEL=1, BLOCKSIZE=512, PREDICTOR=3 for float32 and 2 for
+ * int16 (none for uint8), OVERVIEW_RESAMPLING=AVERAGE, OVERVIEW_COUNT=8,
+ * BIGTIFF=YES) and the windowed read of read_tile_window (core/tile_io.py:
+ * 49-51).
+ *
+ * fsg_cog_write: a BigTIFF COG of the device raster r (float32, int16 or
+ *   uint8, row stride ld): every IFD first, then the tile data from the
+ *   smallest overview to full resolution.  Overviews are 2 x 2 AVERAGE of the
+ *   valid pixels (NaN and the NoData value excluded), level by level, while
+ *   the level exceeds one block and up to overview_count; the GPU builds them
+ *   and applies the predictor, host threads compress with ZSTD.  GeoTIFF tags
+ *   (north-up geotransform x0, a, 0, y0, 0, e; EPSG code) and GDAL_NODATA are
+ *   written when given.  opt may be NULL (512, level 1, 8 overviews, all
+ *   host threads).  The call synchronises the stream.
+ * fsg_cog_open / fsg_cog_info / fsg_cog_read / fsg_cog_close: a tiled
+ *   single-band float32 / int16 / uint8 TIFF or BigTIFF (ZSTD, DEFLATE or
+ *   none; predictor 1, 2 or 3); level 0 is full resolution, 1.. the overviews
+ *   in file order.  fsg_cog_read writes the window rows r0 .. r0 + h - 1 and
+ *   columns c0 .. c0 + w - 1 of a level to device memory out (row stride
+ *   ldo, the file's sample type) and synchronises the stream.
+ */
+typedef struct fsg_cog_raster {
+  const void* data;      /* device */
+  int64_t ld;
+  int32_t h, w;
+  int32_t dtype;         /* FSG_OUT_F32 / FSG_OUT_I16 / FSG_OUT_U8 */
+  int32_t has_geotransform;
+  double geotransform[6];
+  int32_t epsg;          /* 0: none */
+  int32_t epsg_geographic;
+  int32_t has_nodata;
+  int32_t pad0;
+  double nodata;
+} fsg_cog_raster;
+
+typedef struct fsg_cog_options {
+  int32_t block;          /* tile edge, multiple of 16, <= 4096 */
+  int32_t zstd_level;
+  int32_t overview_count;
+  int32_t threads;        /* 0: all host threads */
+} fsg_cog_options;
+
+typedef struct fsg_cog_report {
+  int32_t n_levels;
+  int32_t pad0;
+  uint64_t data_start;
+  uint64_t compressed_bytes;
+  uint64_t raw_bytes;
+} fsg_cog_report;
+
+typedef struct fsg_cog_info_t {
+  int32_t n_levels, width, height, tile_w, tile_h, dtype, compression, predictor;
+  int32_t has_geotransform;
+  int32_t pad0;
+  double geotransform[6];
+  int32_t epsg, epsg_geographic, has_nodata, pad1;
+  double nodata;
+} fsg_cog_info_t;
+
+typedef struct fsg_cog fsg_cog;
+
+int fsg_cog_write(const char* path, const fsg_cog_raster* r, const fsg_cog_options* opt,
+                  fsg_cog_report* report, void* stream);
+int fsg_cog_open(const char* path, fsg_cog** out);
+int fsg_cog_info(const fsg_cog* c, int32_t level, fsg_cog_info_t* info);
+int fsg_cog_read(fsg_cog* c, int32_t level, int32_t r0, int32_t c0, int32_t h, int32_t w,
+                 void* out, int64_t ldo, int32_t threads, void* stream);
+void fsg_cog_close(fsg_cog* c);
 
 #ifdef __cplusplus
 }

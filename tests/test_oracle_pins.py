@@ -916,3 +916,46 @@ def test_dask_edge_depths():
     assert ref.dask_edge_depth("curvature", {}) == 2
     assert ref.dask_edge_depth("openness", {"max_distance": 20}) == 21
     assert ref.dask_edge_depth("ambient_occlusion", {"radius": 7.5}) == 8
+
+
+# ------------------------------------------------------- COG reader oracle
+
+def test_tiff_oracle_decodes_libtiff_zstd_predictor3(tmp_path):
+    # Pillow's libtiff writes the file; the oracle's pure-Python reader
+    # (predictor 3 restated from Adobe TN3) must return the exact input
+    from PIL import Image
+    from oracle import tiff
+    rng = np.random.default_rng(8)
+    a = (rng.normal(size=(37, 53)) * 1000).astype(F32)
+    a[3, 4] = np.nan
+    p = tmp_path / "f.tif"
+    Image.fromarray(a).save(p, compression="zstd", tiffinfo={317: 3})
+    P = tiff.parse(str(p))
+    assert P["ifds"][0]["tags"][317] == [3] and P["ifds"][0]["tags"][259] == [50000]
+    got = tiff.decode(str(p), 0, P)
+    assert np.array_equal(got, a, equal_nan=True)
+
+
+def test_tiff_oracle_decodes_libtiff_deflate_predictor2(tmp_path):
+    from PIL import Image
+    from oracle import tiff
+    a = (np.arange(40 * 61) % 251).astype(np.uint8).reshape(40, 61)
+    p = tmp_path / "u.tif"
+    Image.fromarray(a).save(p, compression="tiff_adobe_deflate", tiffinfo={317: 2})
+    P = tiff.parse(str(p))
+    assert P["ifds"][0]["tags"][317] == [2]
+    assert np.array_equal(tiff.decode(str(p), 0, P), a)
+
+
+def test_tiff_oracle_overview_average_closed_forms():
+    from oracle import tiff
+    a = np.array([[1, 2, 3], [4, 5, 6], [7, 8, 9]], F32)
+    o = tiff.overview_average(a)
+    assert o.tolist() == [[3.0, 4.5], [7.5, 9.0]]            # clipped blocks at the edge
+    b = a.copy()
+    b[0, 0] = np.nan
+    assert tiff.overview_average(b)[0, 0] == np.float32((2 + 4 + 5) / 3)
+    c = np.array([[-9999, 10], [-9999, -9999]], np.int16)
+    assert tiff.overview_average(c, nodata=-9999).tolist() == [[10]]
+    assert tiff.overview_average(np.array([[1, 2]], np.uint8)).tolist() == [[2]]   # 1.5 -> 2
+    assert tiff.overview_levels(16384, 16384) == [(16384 >> k,) * 2 for k in range(6)]
