@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--only-c2", action="store_true",
                     help="skip the per-algorithm workloads (C1 hillshade+slope, C3 curvature and "
                          "openness, C5 fused batch)")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "f1", "f4"], default=None,
+    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "f1", "f2", "f4"],
+                    default=None,
                     help="run only this per-algorithm workload and print its JSON (a profiling "
                          "aid; the bench line is the default run)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -384,6 +385,57 @@ def bench_f1(dev, n=16384):
                                        f"the Gaussian/decimated radii {small}"}}
 
 
+def bench_f2(dev, n=16384):
+    """COG output of a 16384^2 float32 result raster resident in HBM (ZSTD level
+    1, 512 tiles, predictor 3, AVERAGE overviews while a level exceeds one
+    tile, BigTIFF) to local disk, then 4096^2 window reads back to the GPU;
+    the oracle's pure-Python decoder on one level-2 overview for the CPU rate."""
+    import os
+    import tempfile
+    import torch
+    from fujishadergpu_amd import cogio
+    z = make_dem(n, dev)
+    d = tempfile.mkdtemp(prefix="fsg_cog_")
+    path = os.path.join(d, "out.tif")
+    rep = cogio.write_cog(path, z, transform=(5.0, 0.0, 300000.0, 0.0, -5.0, 4000000.0),
+                          epsg=32654, nodata=float("nan"))
+    t_w = _gpu_time(lambda: cogio.write_cog(path, z, transform=(5.0, 0.0, 300000.0, 0.0, -5.0,
+                                                                4000000.0),
+                                            epsg=32654, nodata=float("nan")), 2)
+    size = os.path.getsize(path)
+    rd = cogio.CogReader(path)
+    win = torch.empty((4096, 4096), dtype=torch.float32, device=dev)
+    rd.read((0, 0, 4096, 4096), out=win)
+    reps = 4
+    t_r = _gpu_time(lambda: [rd.read((4096 * (k % 4), 4096 * (k // 4 % 4), 4096, 4096), out=win)
+                             for k in range(reps)], 1) / reps
+    ok = bool(torch.equal(rd.read((4096, 8192, 4096, 4096)), z[8192:12288, 4096:8192]))
+    rd.close()
+    from oracle import tiff
+    t0 = time.perf_counter()
+    P = tiff.parse(path)
+    tiff.decode(path, 2, P)
+    tc = time.perf_counter() - t0
+    ov2 = P["ifds"][2]["tags"][256][0] * P["ifds"][2]["tags"][257][0]
+    os.remove(path)
+    os.rmdir(d)
+    raw = n * n * 4
+    del z, win
+    torch.cuda.empty_cache()
+    return {"workload": f"COG write of a {n}x{n} float32 fBm raster in HBM (ZSTD 1, 512 tiles, "
+                        f"predictor 3, {rep['levels'] - 1} AVERAGE overviews, BigTIFF) to local "
+                        "disk; 4096^2 window reads back to HBM",
+            "write_ms": round(t_w * 1e3, 1), "write_gb_s": round(raw / t_w / 1e9, 3),
+            "write_gpix_s": round(n * n / t_w / 1e9, 3),
+            "file_mb": round(size / 2 ** 20, 1), "ratio": round(rep["raw_bytes"] / size, 3),
+            "read_window_ms": round(t_r * 1e3, 2),
+            "read_gpix_s": round(4096 * 4096 / t_r / 1e9, 3), "read_exact": ok,
+            "host_threads": os.cpu_count(),
+            "cpu_baseline": {"value": round(ov2 / tc / 1e9, 5), "unit": "Gpix/s (decode)",
+                             "cores": 1, "kind": "port",
+                             "sample": "oracle pure-Python decode of the level-2 overview"}}
+
+
 def _prepare_raster(n, dev, seed):
     """fBm (H 0.8, 300-2800 m) with an undeclared 0 frame (n/25 wide on the
     top and left edges), a declared -9999 block and 0.5 % NaN discs (r 20-200)."""
@@ -501,7 +553,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload:
         fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-              "f1": bench_f1, "f4": bench_f4}[args.workload]
+              "f1": bench_f1, "f2": bench_f2, "f4": bench_f4}[args.workload]
         print(json.dumps({args.workload: fn(dev)}), flush=True)
         return
     n = args.size
@@ -583,6 +635,7 @@ def main():
                          ("curvature_openness_c3", bench_c3),
                          ("openness_surrogate_c4", bench_c4),
                          ("spatial_hillshade_f1", bench_f1),
+                         ("cog_io_f2", bench_f2),
                          ("prepare_f4", bench_f4)):
             print(f"# {name} ...", file=sys.stderr, flush=True)
             line["per_algorithm"][name] = fn(dev)
