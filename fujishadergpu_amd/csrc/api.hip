@@ -155,9 +155,17 @@ k_count_nan(const float* __restrict__ ras, int64_t ld, WinChunk wins, int n_wind
   if (wi >= n_windows) return;
   const int4 wd = wins.w[wi];
   unsigned long long local = 0;
+  // 16-byte loads when the window's rows start 16-byte aligned (block-uniform)
+  const bool vec = ((ld | wd.y) & 3) == 0 && ((uintptr_t)ras & 15) == 0;
+  const int w4 = vec ? (wd.w & ~3) : 0;
   for (int r = blockIdx.y; r < wd.z; r += gridDim.y) {
     const float* row = ras + (int64_t)(wd.x + r) * ld + wd.y;
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < wd.w; c += gridDim.x * 256)
+    for (int c = 4 * (blockIdx.x * 256 + threadIdx.x); c < w4; c += 4 * gridDim.x * 256) {
+      const float4 q = *(const float4*)(row + c);
+      local += (isnan(q.x) ? 1ull : 0ull) + (isnan(q.y) ? 1ull : 0ull) +
+               (isnan(q.z) ? 1ull : 0ull) + (isnan(q.w) ? 1ull : 0ull);
+    }
+    for (int c = w4 + blockIdx.x * 256 + threadIdx.x; c < wd.w; c += gridDim.x * 256)
       local += isnan(row[c]) ? 1ull : 0ull;
   }
   for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
@@ -351,7 +359,7 @@ extern "C" int fsg_nodata_to_nan(float* raster, int64_t ld, int32_t h, int32_t w
       maxw = std::max(maxw, chunk.w[i].w);
       maxh = std::max(maxh, chunk.w[i].z);
     }
-    dim3 blk(256), grd(std::min(16, (maxw + 255) / 256), std::min(256, maxh), n);
+    dim3 blk(256), grd(std::min(4, (maxw + 1023) / 1024), std::min(512, maxh), n);
     hipLaunchKernelGGL(k_count_nan, grd, blk, 0, s, raster, ld, chunk, n,
                        (unsigned long long*)(counts + base));
     FSG_CHECK_LAUNCH("k_count_nan");
