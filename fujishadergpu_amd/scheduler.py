@@ -264,14 +264,15 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     H, W = raster.shape
     src = raster.clone() if copy_input else raster
     counts = ops.nodata_to_nan(src, nodata, [(t[7], t[6], t[9], t[8]) for t in P.tiles])
-    nan_counts = counts.cpu().numpy()
-    # the tile windows cover the raster: all zero means the raster holds no NaN
-    raster_nan_free = bool((nan_counts == 0).all())
     if "large" in P.extras:
+        # queued before the counts are read back, so the read overlaps it
         lr, lw = P.extras["large"]
         fld, scale = overview_field(src, lr, lw)
         P.params["_topousm_fast_coarse_field"] = fld
         P.extras["overview_scale"] = scale
+    nan_counts = counts.cpu().numpy()
+    # the tile windows cover the raster: all zero means the raster holds no NaN
+    raster_nan_free = bool((nan_counts == 0).all())
     spatial_tiles = (algorithm in SPATIAL_GPU and
                      str(P.params.get("mode", "local")).lower() == "spatial")
     if spatial_tiles and P.params.get("radii"):

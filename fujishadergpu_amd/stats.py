@@ -50,14 +50,15 @@ def valid_bbox(raster: torch.Tensor):
                      .floor().long(), max=H - 1)
     ci = torch.clamp(((torch.arange(ow, device=raster.device, dtype=torch.float64) + 0.5) * (W / ow))
                      .floor().long(), max=W - 1)
-    ov = raster.index_select(0, ri).index_select(1, ci)
-    vm = torch.isfinite(ov)
-    if not bool(vm.any()):
+    # the <= 512^2 overview samples are gathered on the device and read back
+    # once; the bounding box of their finite cells is host logic
+    ov = raster.index_select(0, ri).index_select(1, ci).cpu().numpy()
+    vm = np.isfinite(ov)
+    ys, xs = np.nonzero(vm.any(axis=1))[0], np.nonzero(vm.any(axis=0))[0]
+    if ys.size == 0:
         return None
-    ys = torch.nonzero(vm.any(dim=1)).flatten()
-    xs = torch.nonzero(vm.any(dim=0)).flatten()
-    by0, by1 = int(ys.min()) * cov, min(H, (int(ys.max()) + 1) * cov)
-    bx0, bx1 = int(xs.min()) * cov, min(W, (int(xs.max()) + 1) * cov)
+    by0, by1 = int(ys[0]) * cov, min(H, (int(ys[-1]) + 1) * cov)
+    bx0, bx1 = int(xs[0]) * cov, min(W, (int(xs[-1]) + 1) * cov)
     return by0, by1, bx0, bx1
 
 

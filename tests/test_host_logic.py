@@ -148,3 +148,28 @@ def test_prepare_coarse_geometry_matches_the_oracle():
         ch, cw = P.coarse_shape(w, h, cm)
         assert (ch, cw) == prep.coarse_shape(w, h, cm)
         assert P.coarse_oversample(ch, cw) == prep.coarse_oversample(ch, cw)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_valid_bbox_matches_a_brute_force_overview(seed):
+    # _norm_stats.py:254-264: the valid bounding box from a ~512 px nearest
+    # overview, recomputed here with numpy loops over the overview samples
+    import numpy as np
+    import torch
+    from fujishadergpu_amd.stats import valid_bbox
+    rng = np.random.default_rng(seed)
+    H, W = (int(v) for v in rng.integers(40, 2600, 2))
+    a = np.full((H, W), np.nan, np.float32)
+    if seed % 5:
+        y0, x0 = int(rng.integers(0, H)), int(rng.integers(0, W))
+        a[y0:int(rng.integers(y0, H)) + 1, x0:int(rng.integers(x0, W)) + 1] = 1.0
+    cov = max(1, max(W, H) // 512)
+    oh, ow = max(1, H // cov), max(1, W // cov)
+    ri = [min(int(np.floor((i + 0.5) * (H / oh))), H - 1) for i in range(oh)]
+    ci = [min(int(np.floor((j + 0.5) * (W / ow))), W - 1) for j in range(ow)]
+    ok = np.isfinite(a[np.ix_(ri, ci)])
+    want = None
+    if ok.any():
+        ys, xs = np.nonzero(ok.any(axis=1))[0], np.nonzero(ok.any(axis=0))[0]
+        want = (ys[0] * cov, min(H, (ys[-1] + 1) * cov), xs[0] * cov, min(W, (xs[-1] + 1) * cov))
+    assert valid_bbox(torch.from_numpy(a)) == want
