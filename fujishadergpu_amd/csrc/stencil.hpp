@@ -26,33 +26,43 @@ __device__ __forceinline__ float raw(const WinDev& W, int r, int c) {
 // filled*valid and of valid, axis 0 then axis 1 with float32 intermediates,
 // symmetric accumulation order (centre, then distance 4..1), then
 // where(sw > 0, sv / sw, 0) in float32 (_nan_utils.py:18-31).
-static __device__ __noinline__ float nan_fill(const WinDev W, int r, int c, const Gauss9 G) {
-  float vsv[9], vsw[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const int cc = clampi(c + k - 4, W.w);
-    const float x0 = raw(W, r, cc);
-    const bool v0 = !isnan(x0);
-    double sv = (v0 ? (double)x0 : 0.0) * G.g[0];
-    double sw = (v0 ? 1.0 : 0.0) * G.g[0];
-#pragma unroll
-    for (int d = 4; d >= 1; d--) {
-      const float a = raw(W, clampi(r - d, W.h), cc);
-      const float b = raw(W, clampi(r + d, W.h), cc);
-      const double fa = isnan(a) ? 0.0 : (double)a, fb = isnan(b) ? 0.0 : (double)b;
-      const double wa = isnan(a) ? 0.0 : 1.0, wb = isnan(b) ? 0.0 : 1.0;
-      sv += (fa + fb) * G.g[d];
-      sw += (wa + wb) * G.g[d];
-    }
-    vsv[k] = (float)sv;
-    vsw[k] = (float)sw;
-  }
-  double sv = (double)vsv[4] * G.g[0];
-  double sw = (double)vsw[4] * G.g[0];
+// axis-0 pass of that fill at column cc: (sv, sw) rounded to float32
+__device__ __forceinline__ void nan_fill_col(const WinDev& W, int r, int cc, const Gauss9& G,
+                                             float& fv, float& fw) {
+  const float x0 = raw(W, r, cc);
+  const bool v0 = !isnan(x0);
+  double sv = (v0 ? (double)x0 : 0.0) * G.g[0];
+  double sw = (v0 ? 1.0 : 0.0) * G.g[0];
 #pragma unroll
   for (int d = 4; d >= 1; d--) {
-    sv += ((double)vsv[4 - d] + (double)vsv[4 + d]) * G.g[d];
-    sw += ((double)vsw[4 - d] + (double)vsw[4 + d]) * G.g[d];
+    const float a = raw(W, clampi(r - d, W.h), cc);
+    const float b = raw(W, clampi(r + d, W.h), cc);
+    const double fa = isnan(a) ? 0.0 : (double)a, fb = isnan(b) ? 0.0 : (double)b;
+    const double wa = isnan(a) ? 0.0 : 1.0, wb = isnan(b) ? 0.0 : 1.0;
+    sv += (fa + fb) * G.g[d];
+    sw += (wa + wb) * G.g[d];
+  }
+  fv = (float)sv;
+  fw = (float)sw;
+}
+
+// NaN-aware local fill at (r, c): scipy gaussian_filter(mode='nearest') of
+// filled*valid and of valid, axis 0 then axis 1 with float32 intermediates,
+// symmetric accumulation order (centre, then distance 4..1), then
+// where(sw > 0, sv / sw, 0) in float32 (_nan_utils.py:18-31).  The axis-1
+// pass takes the column pairs in that order, so only two columns are live.
+static __device__ __forceinline__ float nan_fill(const WinDev& W, int r, int c, const Gauss9& G) {
+  float cv, cw;
+  nan_fill_col(W, r, clampi(c, W.w), G, cv, cw);
+  double sv = (double)cv * G.g[0];
+  double sw = (double)cw * G.g[0];
+#pragma unroll 1
+  for (int d = 4; d >= 1; d--) {
+    float av, aw, bv, bw;
+    nan_fill_col(W, r, clampi(c - d, W.w), G, av, aw);
+    nan_fill_col(W, r, clampi(c + d, W.w), G, bv, bw);
+    sv += ((double)av + (double)bv) * G.g[d];
+    sw += ((double)aw + (double)bw) * G.g[d];
   }
   const float fsv = (float)sv, fsw = (float)sw;
   return fsw > 0.0f ? fsv / fsw : 0.0f;
@@ -300,9 +310,34 @@ __device__ __forceinline__ void stencil_tile_block_v(const StencilTile& S, int b
   Axis* Xa = (Axis*)(Qt + ST_PH * ST_PW);     // per staged row: |x step| coefficients
   Axis* Xs = (Axis*)((float*)Xa + ST_ROWS * ST_AXIS_FLOATS);  // signed x step
   const int t = threadIdx.x;
-  for (int e = t; e < nr * nc; e += 256) {
-    const int rr = e / nc, cc = e - rr * nc;
-    F[rr * ST_LT + cc] = filled<NANAW>(S.W, tr0 + rr, tc0 + cc, G);
+  {
+    // lanes 0..63 of each wave walk columns, the 4 waves walk rows; all of a
+    // lane's loads are issued before any is used
+    constexpr int NRW = (ST_ROWS + 3) / 4, NCW = (ST_TW + 2 * ST_HALO + 63) / 64;
+    float v[NRW][NCW];
+#pragma unroll
+    for (int i = 0; i < NRW; i++)
+#pragma unroll
+      for (int j = 0; j < NCW; j++) {
+        const int rr = (t >> 6) + 4 * i, cc = (t & 63) + 64 * j;
+        v[i][j] = (rr < nr && cc < nc) ? raw(S.W, tr0 + rr, tc0 + cc) : 0.0f;
+      }
+#pragma unroll
+    for (int i = 0; i < NRW; i++)
+#pragma unroll
+      for (int j = 0; j < NCW; j++) {
+        const int rr = (t >> 6) + 4 * i, cc = (t & 63) + 64 * j;
+        if (rr < nr && cc < nc) F[rr * ST_LT + cc] = v[i][j];
+      }
+    if (NANAW) {
+      // NaN cells (rare) take the local fill; one call site, the lane's own cells
+#pragma unroll 1
+      for (int e = 0; e < NRW * NCW; e++) {
+        const int rr = (t >> 6) + 4 * (e / NCW), cc = (t & 63) + 64 * (e % NCW);
+        if (rr < nr && cc < nc && isnan(F[rr * ST_LT + cc]))
+          F[rr * ST_LT + cc] = nan_fill(S.W, tr0 + rr, tc0 + cc, G);
+      }
+    }
   }
   if (ROWS && t < nr) {
     const double hx = S.xrows[tr0 + t];
