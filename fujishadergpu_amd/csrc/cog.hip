@@ -357,14 +357,26 @@ static int encode_and_write(int fd, const std::vector<Level>& L, const fsg_cog_o
   }
   const int threads = n_workers(o.threads);
   const size_t cap = ZSTD_compressBound(tile_bytes);
-  std::vector<std::vector<uint8_t>> comp(batch, std::vector<uint8_t>(cap));
-  std::vector<size_t> csize(batch);
+  // two sets of compressed tiles: one batch is written by a writer thread while
+  // the next is compressed
+  std::vector<std::vector<uint8_t>> comp[2];
+  std::vector<size_t> csize[2];
+  for (int k = 0; k < 2; k++) {
+    comp[k].assign(batch, std::vector<uint8_t>(cap));
+    csize[k].assign(batch, 0);
+  }
   std::vector<ZSTD_CCtx*> ctx(threads, nullptr);
   for (auto& c : ctx) c = ZSTD_createCCtx();
   int rc = FSG_OK;
   for (auto* c : ctx)
     if (!c) rc = fail(FSG_E_HIP, "fsg_cog_write: ZSTD context");
   uint64_t pos = data_start;
+  std::atomic<int> wbad{0};
+  std::thread writer;
+  auto join_writer = [&]() {
+    if (writer.joinable()) writer.join();
+  };
+  int set = 0;
   for (int li = (int)L.size() - 1; li >= 0 && rc == FSG_OK; li--) {   // smallest level first
     const Level& lv = L[li];
     const int nt = lv.ntx * lv.nty;
@@ -384,28 +396,37 @@ static int encode_and_write(int fd, const std::vector<Level>& L, const fsg_cog_o
         break;
       }
       std::atomic<int> bad{0};
+      auto& cs = comp[set];
+      auto& sz = csize[set];
       parallel_for(nb, threads, [&](int i, int wid) {
-        const size_t z = ZSTD_compressCCtx(ctx[wid], comp[i].data(), cap,
-                                           hstage + tile_bytes * i, tile_bytes, o.zstd_level);
+        const size_t z = ZSTD_compressCCtx(ctx[wid], cs[i].data(), cap, hstage + tile_bytes * i,
+                                           tile_bytes, o.zstd_level);
         if (ZSTD_isError(z)) bad = 1;
-        else csize[i] = z;
+        else sz[i] = z;
       });
       if (bad) {
         rc = fail(FSG_E_HIP, "fsg_cog_write: ZSTD compression failed");
         break;
       }
-      for (int i = 0; i < nb; i++) {
-        if (write_all(fd, comp[i].data(), csize[i], pos)) {
-          rc = fail(FSG_E_INVALID, "fsg_cog_write: write failed (disk full?)");
-          break;
+      join_writer();   // the previous batch is on disk; its set is free again
+      if (wbad) break;
+      writer = std::thread([&, li, t0, nb, set]() {
+        for (int i = 0; i < nb; i++) {
+          if (write_all(fd, comp[set][i].data(), csize[set][i], pos)) {
+            wbad = 1;
+            return;
+          }
+          offs[li][t0 + i] = pos;
+          cnts[li][t0 + i] = csize[set][i];
+          pos += csize[set][i];
+          *t_bytes += (double)csize[set][i];
         }
-        offs[li][t0 + i] = pos;
-        cnts[li][t0 + i] = csize[i];
-        pos += csize[i];
-        *t_bytes += (double)csize[i];
-      }
+      });
+      set ^= 1;
     }
   }
+  join_writer();
+  if (wbad && rc == FSG_OK) rc = fail(FSG_E_INVALID, "fsg_cog_write: write failed (disk full?)");
   for (auto& c : ctx) ZSTD_freeCCtx(c);
   (void)hipHostFree(hstage);
   (void)hipFree(dstage);
