@@ -132,6 +132,11 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
   // vertical running sums over rows wy0-1-r .. wy0-1+r (16 loads in flight)
   const bool has_col = t < NC;
   const int jw = has_col ? refl(wx0 - A.R + t, A.W.w) : 0;
+  // block-uniform: every row this block reads (wy0 - R - 1 .. wy0 + FAST_SH - 1 + R)
+  // lies inside the window, so no row index needs the reflection and row
+  // addresses advance by ld (scalar adds instead of a reflect and a 64-bit
+  // multiply per row)
+  const bool rows_in = wy0 - A.R - 1 >= 0 && wy0 + FAST_SH - 1 + A.R <= H - 1;
   double S[NRA];
   int Cn[NRA];
 #pragma unroll
@@ -141,9 +146,18 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
     const int r = A.rad[i];
     for (int d0 = -r; d0 <= r; d0 += 16) {
       float v[16];
+      if (rows_in) {
+        const float* rp = Wp + (int64_t)(wy0 - 1 + d0) * ld;
 #pragma unroll
-      for (int e = 0; e < 16; e++)
-        v[e] = urow(Wp, refl1(wy0 - 1 + min(d0 + e, r), H), ld)[jw];
+        for (int e = 0; e < 16; e++) {
+          v[e] = rp[jw];
+          if (d0 + e < r) rp += ld;   // rows past r repeat row r, as min(d0 + e, r) does
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 16; e++)
+          v[e] = urow(Wp, refl1(wy0 - 1 + min(d0 + e, r), H), ld)[jw];
+      }
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         if (d0 + e > r) break;
@@ -171,14 +185,28 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
 #pragma unroll
     for (int kb = 0; kb < K; kb += KB) {
       float vz[KB], vn[NRA][KB], vo[NRA][KB];
+      if (rows_in) {
+        const float* rz = Wp + (int64_t)(wyc + kb) * ld;
 #pragma unroll
-      for (int k = 0; k < KB; k++) {
-        const int yw = wyc + kb + k;
-        vz[k] = urow(Wp, min(yw, H - 1), ld)[jw];
+        for (int k = 0; k < KB; k++) {
+          vz[k] = rz[jw];
 #pragma unroll
-        for (int i = 0; i < NR; i++) {
-          vn[i][k] = urow(Wp, refl1(yw + A.rad[i], H), ld)[jw];
-          vo[i][k] = urow(Wp, refl1(yw - A.rad[i] - 1, H), ld)[jw];
+          for (int i = 0; i < NR; i++) {
+            vn[i][k] = (rz + (int64_t)A.rad[i] * ld)[jw];
+            vo[i][k] = (rz - (int64_t)(A.rad[i] + 1) * ld)[jw];
+          }
+          rz += ld;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+          const int yw = wyc + kb + k;
+          vz[k] = urow(Wp, min(yw, H - 1), ld)[jw];
+#pragma unroll
+          for (int i = 0; i < NR; i++) {
+            vn[i][k] = urow(Wp, refl1(yw + A.rad[i], H), ld)[jw];
+            vo[i][k] = urow(Wp, refl1(yw - A.rad[i] - 1, H), ld)[jw];
+          }
         }
       }
       if (has_col) {
@@ -291,8 +319,17 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
       const PostK p1 = load_post(A.o1);
       const bool two = A.o2.data != nullptr;
       float zq[K];
+      if (rows_in) {
+        const float* rz = Wp + (int64_t)wyc * ld + wx0 + t;
 #pragma unroll
-      for (int k = 0; k < K; k++) zq[k] = urow(Wp, min(wyc + k, H - 1), ld)[wx0 + t];
+        for (int k = 0; k < K; k++) {
+          zq[k] = *rz;
+          rz += ld;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) zq[k] = urow(Wp, min(wyc + k, H - 1), ld)[wx0 + t];
+      }
 #pragma unroll
       for (int k = 0; k < K; k++) {
         if (k < nrow) {
