@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 10  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 11  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -89,6 +89,17 @@ class fsg_topousm_params(C.Structure):
         ("field_w", C.c_int32), ("full_h", C.c_int32), ("full_w", C.c_int32),
         ("win_r0", C.c_int32), ("win_c0", C.c_int32), ("w_large", C.c_float), ("nan_hint", C.c_int32),
         ("mean", C.c_void_p * MAX_RADII), ("mean_ld", C.c_int64 * MAX_RADII),
+    ]
+
+
+COARSE_JOB_MAX_RADII = 4
+
+
+class fsg_coarse_job(C.Structure):
+    _fields_ = [
+        ("win", fsg_window), ("ds", C.c_int32), ("n_radii", C.c_int32),
+        ("radii", C.c_int32 * COARSE_JOB_MAX_RADII), ("outs", C.c_void_p * COARSE_JOB_MAX_RADII),
+        ("ld_out", C.c_int64),
     ]
 
 
@@ -185,6 +196,10 @@ _SIGS = {
     "fsg_topousm_coarse_means_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32]),
     "fsg_topousm_coarse_means": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, P(C.c_int32),
                                            P(vp), C.c_int64, vp, C.c_size_t, vp]),
+    "fsg_topousm_coarse_means_batch_workspace_size": (C.c_size_t, [P(fsg_coarse_job),
+                                                                    C.c_int32]),
+    "fsg_topousm_coarse_means_batch": (C.c_int, [P(fsg_coarse_job), C.c_int32, vp, C.c_size_t,
+                                                 vp]),
     "fsg_openness_points": (C.c_int, [P(fsg_window), vp, C.c_int32, P(fsg_openness_params), vp,
                                       vp]),
     "fsg_openness_judge_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32, C.c_int32]),

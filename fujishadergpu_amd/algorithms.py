@@ -438,11 +438,17 @@ class CoarsePyramid:
         for ds, rs in pairs:
             if (ds, rs) not in self.grids and rs not in by_ds.setdefault(ds, []):
                 by_ds[ds].append(rs)
-        for ds, rss in by_ds.items():
-            if rss:
-                g = ops.topousm_coarse_means(self.raster, ds, rss)
-                for i, rs in enumerate(rss):
-                    self.grids[(ds, rs)] = g[i]
+        todo = [(ds, rss) for ds, rss in by_ds.items() if rss]
+        # one batched call (two factors 2x / 4x apart share one read of the
+        # raster) where every level fits it, else one call per level
+        if todo and all(ds in (4, 8, 16) and len(rss) <= N.COARSE_JOB_MAX_RADII
+                        and min(rss) >= 2 for ds, rss in todo):
+            res = ops.topousm_coarse_means_batch([(self.raster, ds, rss) for ds, rss in todo])
+        else:
+            res = [ops.topousm_coarse_means(self.raster, ds, rss) for ds, rss in todo]
+        for (ds, rss), g in zip(todo, res):
+            for i, rs in enumerate(rss):
+                self.grids[(ds, rs)] = g[i]
 
     def attach(self, prm, win_origin, win_shape, reg) -> int:
         """Point prm.mean[i] at the shared grid wherever that is exact;
@@ -515,6 +521,11 @@ class TopoUSMFastAlgorithm(NativeTileAlgorithm):
         pyr = p.get("_topousm_fast_pyramid")
         if pyr is not None and p.get("_window_origin") is not None:
             pyr.attach(prm, p["_window_origin"], tuple(win.shape), reg)
+        # this window's own coarse means, built beforehand for many windows at
+        # once (ops.topousm_coarse_means_batch): radius index -> grid
+        for i, g in (p.get("_topousm_fast_means") or {}).items():
+            prm.mean[int(i)] = g.data_ptr()
+            prm.mean_ld[int(i)] = int(g.stride(0))
         if prm.field and p.get("_topousm_fast_full_shape") is None:
             prm.full_h, prm.full_w = int(win.shape[0]), int(win.shape[1])
         stats = p.get("global_stats", None) if normalize else None

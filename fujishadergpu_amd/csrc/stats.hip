@@ -64,6 +64,61 @@ __global__ void k_pct_zero(unsigned long long* p, size_t n) {
     p[i] = 0ull;
 }
 
+// Visit every element of x[0, n) once across the grid: float4 loads, four in
+// flight per lane, when x is 16-byte aligned; scalar loads for the rest.
+template <class F>
+__device__ __forceinline__ void pct_visit(const float* __restrict__ x, int64_t n, F&& f) {
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t S = (int64_t)gridDim.x * 256;
+  int64_t head = 0;
+  if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+    const float4* __restrict__ x4 = (const float4*)x;
+    const int64_t n4 = n >> 2;
+    int64_t i = tid;
+    for (; i + 3 * S < n4; i += 4 * S) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = x4[i + u * S];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        f(v[u].x);
+        f(v[u].y);
+        f(v[u].z);
+        f(v[u].w);
+      }
+    }
+    for (; i < n4; i += S) {
+      const float4 v = x4[i];
+      f(v.x);
+      f(v.y);
+      f(v.z);
+      f(v.w);
+    }
+    head = n4 << 2;
+  }
+  for (int64_t i = head + tid; i < n; i += S) f(x[i]);
+}
+
+// hist[bin] += 1 for every lane with `on`.  Lanes of one wave that share the
+// bin of the lowest remaining lane add together (one atomic for the group),
+// twice; the rest add one by one.  Level-1 bins are few and crowded (exponent
+// and two mantissa bits), so most of a wave lands in one or two of them.
+__device__ __forceinline__ void hist_add_grouped(unsigned int* hist, unsigned int bin, bool on) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long act = __ballot(on);
+#pragma unroll
+  for (int round = 0; round < 2; round++) {
+    if (act == 0) return;
+    const int lead = __ffsll((long long)act) - 1;
+    const unsigned int b = __shfl(bin, lead, 64);
+    const unsigned long long grp = __ballot(on && bin == b) & act;
+    if (lane == lead) atomicAdd(&hist[b], (unsigned int)__popcll(grp));
+    if ((grp >> lane) & 1ull) on = false;
+    act &= ~grp;
+  }
+  if (on) atomicAdd(&hist[bin], 1u);
+}
+
 // Level-1 histogram (bits 31..21), valid count and FP64 moments.
 __global__ void __launch_bounds__(256)
 k_pct_hist1(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict__ ws) {
@@ -72,13 +127,14 @@ k_pct_hist1(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict
   for (int b = threadIdx.x; b < NB1; b += 256) hist[b] = 0;
   __syncthreads();
   double s = 0.0, s2 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const float v = x[i];
-    if (!pct_valid(v, flags)) continue;
-    atomicAdd(&hist[pct_key(v, flags) >> 21], 1u);
-    s += (double)v;
-    s2 += (double)v * (double)v;
-  }
+  pct_visit(x, n, [&](float v) {
+    const bool ok = pct_valid(v, flags);
+    hist_add_grouped(hist, ok ? pct_key(v, flags) >> 21 : 0u, ok);
+    if (ok) {
+      s += (double)v;
+      s2 += (double)v * (double)v;
+    }
+  });
   for (int off = 32; off > 0; off >>= 1) {
     s += __shfl_down(s, off, 64);
     s2 += __shfl_down(s2, off, 64);
@@ -96,7 +152,8 @@ k_pct_hist1(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict
     if (hist[b]) atomicAdd(&ws->h1[b], (unsigned long long)hist[b]);
 }
 
-// Level-2/3 histograms restricted to each target's current prefix.
+// Level-2/3 histograms restricted to each target's current prefix (targets
+// sharing a prefix are histogrammed once, under the first of them).
 template <int LEVEL>
 __global__ void __launch_bounds__(256)
 k_pct_hist(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict__ ws) {
@@ -107,21 +164,24 @@ k_pct_hist(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict_
   __shared__ unsigned int hist[PCT_MAX_T][NB];
   const int nt = ws->st.n_targets;
   unsigned int pre[PCT_MAX_T];
-  for (int t = 0; t < PCT_MAX_T; t++) pre[t] = (t < nt) ? ws->st.prefix[t] : 0xFFFFFFFFu;
+  bool act[PCT_MAX_T];
+  for (int t = 0; t < PCT_MAX_T; t++) {
+    pre[t] = (t < nt) ? ws->st.prefix[t] : 0xFFFFFFFFu;
+    bool dup = false;
+    for (int u = 0; u < t; u++) dup |= (pre[u] == pre[t]);
+    act[t] = t < nt && !dup;
+  }
   for (int t = 0; t < nt; t++)
     for (int b = threadIdx.x; b < NB; b += 256) hist[t][b] = 0;
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const float v = x[i];
-    if (!pct_valid(v, flags)) continue;
+  pct_visit(x, n, [&](float v) {
+    if (!pct_valid(v, flags)) return;
     const unsigned int k = pct_key(v, flags);
     const unsigned int km = k & MASK;
-    for (int t = 0; t < nt; t++) {
-      bool dup = false;
-      for (int u = 0; u < t; u++) dup |= (pre[u] == pre[t]);
-      if (!dup && km == pre[t]) atomicAdd(&hist[t][(k >> SHIFT) & BM], 1u);
-    }
-  }
+#pragma unroll
+    for (int t = 0; t < PCT_MAX_T; t++)
+      if (act[t] && km == pre[t]) atomicAdd(&hist[t][(k >> SHIFT) & BM], 1u);
+  });
   __syncthreads();
   for (int t = 0; t < nt; t++)
     for (int b = threadIdx.x; b < NB; b += 256)

@@ -360,6 +360,196 @@ k_box_t_batch(const BoxJob* __restrict__ jobs, const int* __restrict__ pre, int 
                     local / J.nbx, ta, tb);
 }
 
+// ---- coarse means of many NaN-free windows (fsg_topousm_coarse_means_batch)
+//
+// Job tables travel as kernel arguments.  A decimation job reads its window
+// once for one factor fa, or for two factors fa < fb (fb = 2 fa or 4 fa, both
+// powers of two in 4..16) when two jobs share a window.
+constexpr int CMB_MAX_DEC = 16;
+constexpr int CMB_MAX_BOX = 24;
+
+struct DualDecJob {
+  WinDev W;
+  float* oa;       // fa grid (ceil(h/fa) x ceil(w/fa), dense)
+  float* ob;       // fb grid, or nullptr
+  int fa, fb;      // fb = 0: one factor
+  int oha, owa, ohb, owb;
+  int nbx;         // blocks per band (1024 source columns each)
+  int vec;         // 16-byte aligned rows: float4 loads
+};
+struct DecTab {
+  DualDecJob j[CMB_MAX_DEC];
+  int pre[CMB_MAX_DEC + 1];
+  int n;
+};
+struct BoxTab {
+  BoxJob j[CMB_MAX_BOX];
+  int pre[CMB_MAX_BOX + 1];
+  int n;
+};
+
+__device__ __forceinline__ int tab_job_of(const int* pre, int n, int g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// float64 sum and count of the finite values of four columns c..c+3 over rows
+// r0..r1-1 (columns past w count as absent)
+__device__ __forceinline__ void dec_rows4(const WinDev& W, int c, int r0, int r1, bool vec,
+                                          double& tot, int& cnt) {
+  for (int r = r0; r < r1; r++) {
+    const float* p = W.p + (int64_t)r * W.ld + c;
+    float v[4];
+    if (vec && c + 3 < W.w) {
+      const float4 q = *(const float4*)p;
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[k] = c + k < W.w ? p[k] : __builtin_nanf("");
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (isfinite(v[k])) {
+        tot += (double)v[k];
+        cnt++;
+      }
+  }
+}
+
+// reduce over groups of g lanes (g a power of two <= 4), result in the group's first lane
+__device__ __forceinline__ void dec_reduce(double& tot, int& cnt, int g) {
+  for (int off = 1; off < g; off <<= 1) {
+    tot += __shfl_down(tot, off, 64);
+    cnt += __shfl_down(cnt, off, 64);
+  }
+}
+
+// Block (bx, by): source columns bx*1024 + 4 lane .. +3; source rows of coarse
+// row `by` of the largest factor.  Cell sums are exact in float64, so a cell's
+// value does not depend on how its rows and columns are grouped: the cells
+// equal k_decimate4's bit for bit.
+__global__ void __launch_bounds__(256) k_decimate_tab(DecTab T) {
+  const int j = tab_job_of(T.pre, T.n, blockIdx.x);
+  const DualDecJob& J = T.j[j];
+  const int local = blockIdx.x - T.pre[j];
+  const int bx = local % J.nbx, by = local / J.nbx;
+  const int c = (bx * 256 + threadIdx.x) * 4;
+  const bool vec = J.vec != 0;
+  if (J.fb == 0) {
+    const int f = J.fa;
+    const int r0 = by * f, r1 = min(r0 + f, J.W.h);
+    double tot = 0.0;
+    int cnt = 0;
+    if (c < J.W.w) dec_rows4(J.W, c, r0, r1, vec, tot, cnt);
+    dec_reduce(tot, cnt, f / 4);
+    if ((threadIdx.x & (f / 4 - 1)) == 0) {
+      const int oc = c / f;
+      if (oc < J.owa && by < J.oha)
+        J.oa[(int64_t)by * J.owa + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
+    }
+    return;
+  }
+  const int fa = J.fa, fb = J.fb, m = fb / fa;
+  double tb = 0.0;
+  int cb = 0;
+  for (int g = 0; g < m; g++) {
+    const int r0 = by * fb + g * fa, r1 = min(r0 + fa, J.W.h);
+    double ta = 0.0;
+    int ca = 0;
+    if (c < J.W.w && r0 < r1) dec_rows4(J.W, c, r0, r1, vec, ta, ca);
+    tb += ta;
+    cb += ca;
+    dec_reduce(ta, ca, fa / 4);
+    const int ra = by * m + g;
+    if ((threadIdx.x & (fa / 4 - 1)) == 0) {
+      const int oc = c / fa;
+      if (oc < J.owa && ra < J.oha)
+        J.oa[(int64_t)ra * J.owa + oc] = ca > 0 ? (float)ta / (float)ca : __builtin_nanf("");
+    }
+  }
+  dec_reduce(tb, cb, fb / 4);
+  if ((threadIdx.x & (fb / 4 - 1)) == 0) {
+    const int oc = c / fb;
+    if (oc < J.owb && by < J.ohb)
+      J.ob[(int64_t)by * J.owb + oc] = cb > 0 ? (float)tb / (float)cb : __builtin_nanf("");
+  }
+}
+
+// Box means along axis 0 of a grid WITHOUT NaN, written transposed: one
+// 64-column x 64-row tile per one-wave block.  Both passes of a NaN-free grid
+// are this operation (pass 0 on the grid, pass 1 on its transposed means):
+// with no NaN the NaN-aware passes' count planes are exactly 1.0f, so
+// fv / fw = fv and the results equal box_t_block's bit for bit (each pass
+// updates its running sum in the order box_t_block's pass does), from half the
+// traffic and half the LDS.
+template <int PASS>
+__device__ __forceinline__ void box_plain_block(const float* __restrict__ a, int64_t lda, int h,
+                                                int w, int r, double inv, float* __restrict__ o,
+                                                int64_t ldo, int bx, int by, float (*ta)[65]) {
+  const int lane = threadIdx.x;
+  const int c = bx * 64 + lane;
+  const int y0 = by * 64;
+  const int y1 = min(y0 + 64, h);
+  if (c < w) {
+    const float* pa = a + c;
+    double s = 0.0;
+    // window rows y0-r .. y0+r, 32 independent loads per batch
+    for (int d0 = -r; d0 <= r; d0 += 32) {
+      float va[32];
+#pragma unroll
+      for (int e = 0; e < 32; e++) va[e] = pa[(int64_t)refl(y0 + min(d0 + e, r), h) * lda];
+#pragma unroll
+      for (int e = 0; e < 32; e++) {
+        if (d0 + e > r) break;
+        s += (double)va[e];
+      }
+    }
+    for (int yb = y0; yb < y1; yb += 16) {
+      float na[16], ov[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int y = min(yb + e, y1 - 1);
+        na[e] = pa[(int64_t)refl(y + r, h) * lda];
+        ov[e] = pa[(int64_t)refl(y - r - 1, h) * lda];
+      }
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int y = yb + e;
+        if (y >= y1) break;
+        if (y > y0) {
+          if (PASS == 0) {
+            s += (double)na[e];
+            s -= (double)ov[e];
+          } else {
+            s += (double)na[e] - (double)ov[e];
+          }
+        }
+        ta[y - y0][lane] = (float)(s * inv);
+      }
+    }
+  }
+  __syncthreads();
+  const int ncol = min(64, w - bx * 64);
+  for (int cc = 0; cc < ncol; cc++) {
+    const int y = y0 + lane;
+    if (y < y1) o[(int64_t)(bx * 64 + cc) * ldo + y] = ta[lane][cc];
+  }
+}
+
+template <int PASS>
+__global__ void __launch_bounds__(64) k_box_plain_tab(BoxTab T) {
+  __shared__ float ta[64][65];
+  const int j = tab_job_of(T.pre, T.n, blockIdx.x);
+  const BoxJob& J = T.j[j];
+  const int local = blockIdx.x - T.pre[j];
+  box_plain_block<PASS>(J.a, J.lda, J.h, J.w, J.r, J.inv, J.oa, J.ldo, local % J.nbx, local / J.nbx,
+                  ta);
+}
+
 __global__ void k_gauss9_grid(const float* __restrict__ x, int64_t ldx, int h, int w,
                               float* __restrict__ out, int64_t ldo, Gauss9 G) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1228,6 +1418,167 @@ extern "C" int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32
     rc = coarse_mean(grid, ldg, hc, wc, std::max(1, (int)radii_coarse[i]), outs[i], ld_out, tmp[0],
                      tmp[1], s);
     if (rc) return rc;
+  }
+  return FSG_OK;
+}
+
+// Workspace of the batched coarse means: per job its decimated grid, per
+// (job, radius) the transposed axis-0 means of box pass 0 (the passes of
+// every job run in the same launches, so nothing is shared).
+static size_t cmb_job_bytes(const fsg_coarse_job& J) {
+  const size_t g = align256(sizeof(float) * (size_t)((J.win.h + J.ds - 1) / J.ds) *
+                            (size_t)((J.win.w + J.ds - 1) / J.ds));
+  return g * (1 + (size_t)std::max(0, (int)J.n_radii));
+}
+
+extern "C" size_t fsg_topousm_coarse_means_batch_workspace_size(const fsg_coarse_job* jobs,
+                                                                int32_t n) {
+  if (!jobs || n <= 0) return 0;
+  size_t b = 0;
+  for (int i = 0; i < n; i++) {
+    if (jobs[i].win.h <= 0 || jobs[i].win.w <= 0 || jobs[i].ds < 1) return 0;
+    b += cmb_job_bytes(jobs[i]);
+  }
+  return b;
+}
+
+static bool same_window(const fsg_window& a, const fsg_window& b) {
+  return a.data == b.data && a.ld == b.ld && a.h == b.h && a.w == b.w;
+}
+
+extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_t n,
+                                              void* workspace, size_t workspace_bytes,
+                                              void* stream) {
+  if (!jobs || n <= 0) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: no jobs");
+  for (int i = 0; i < n; i++) {
+    const fsg_coarse_job& J = jobs[i];
+    int rc = validate_window(&J.win, "fsg_topousm_coarse_means_batch");
+    if (rc) return rc;
+    if (J.win.h == 0 || J.win.w == 0)
+      return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: empty window");
+    if (J.ds < 4 || J.ds > 16 || (J.ds & (J.ds - 1)) != 0)
+      return fail(FSG_E_UNSUPPORTED,
+                  "fsg_topousm_coarse_means_batch: ds must be 4, 8 or 16 (use "
+                  "fsg_topousm_coarse_means)");
+    if (J.n_radii < 1 || J.n_radii > FSG_COARSE_JOB_MAX_RADII)
+      return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: n_radii out of range");
+    const int wc = (J.win.w + J.ds - 1) / J.ds;
+    if (J.ld_out < wc)
+      return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: ld_out < grid width");
+    for (int k = 0; k < J.n_radii; k++) {
+      if (!J.outs[k]) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: output NULL");
+      if (J.radii[k] <= 1)
+        return fail(FSG_E_UNSUPPORTED,
+                    "fsg_topousm_coarse_means_batch: coarse radius <= 1 (the Gaussian branch; "
+                    "use fsg_topousm_coarse_means)");
+    }
+  }
+  const size_t need = fsg_topousm_coarse_means_batch_workspace_size(jobs, n);
+  if (!workspace || workspace_bytes < need)
+    return fail(FSG_E_WORKSPACE, "fsg_topousm_coarse_means_batch: workspace too small");
+  hipStream_t s = as_stream(stream);
+  // workspace carve-up
+  std::vector<float*> grid(n);
+  std::vector<std::vector<float*>> tv(n);
+  {
+    char* p = (char*)workspace;
+    for (int i = 0; i < n; i++) {
+      const fsg_coarse_job& J = jobs[i];
+      const size_t g = align256(sizeof(float) * (size_t)((J.win.h + J.ds - 1) / J.ds) *
+                                (size_t)((J.win.w + J.ds - 1) / J.ds));
+      grid[i] = (float*)p;
+      p += g;
+      for (int k = 0; k < J.n_radii; k++) {
+        tv[i].push_back((float*)p);
+        p += g;
+      }
+    }
+  }
+  // decimations: a job whose window another job shares at 2x or 4x its factor
+  // rides along with that job's read
+  std::vector<DualDecJob> dec;
+  std::vector<int> paired(n, 0);
+  for (int i = 0; i < n; i++) {
+    if (paired[i]) continue;
+    const fsg_coarse_job& A = jobs[i];
+    int partner = -1;
+    for (int k = 0; k < n && partner < 0; k++) {
+      if (k == i || paired[k] || !same_window(A.win, jobs[k].win)) continue;
+      const int fa = A.ds, fb = jobs[k].ds;
+      if (fb == 2 * fa || fb == 4 * fa) partner = k;
+    }
+    // a larger factor paired with a smaller one later is handled from that side
+    if (partner < 0) {
+      bool later = false;
+      for (int k = 0; k < n; k++)
+        if (k != i && !paired[k] && same_window(A.win, jobs[k].win) &&
+            (A.ds == 2 * jobs[k].ds || A.ds == 4 * jobs[k].ds))
+          later = true;
+      if (later) continue;
+    }
+    DualDecJob D{};
+    D.W = WinDev{A.win.data, A.win.ld, A.win.h, A.win.w};
+    D.oa = grid[i];
+    D.fa = A.ds;
+    D.oha = (A.win.h + A.ds - 1) / A.ds;
+    D.owa = (A.win.w + A.ds - 1) / A.ds;
+    D.nbx = (A.win.w + 1023) / 1024;
+    D.vec = (A.win.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(A.win.data) & 15) == 0) ? 1 : 0;
+    paired[i] = 1;
+    if (partner >= 0) {
+      const fsg_coarse_job& B = jobs[partner];
+      D.ob = grid[partner];
+      D.fb = B.ds;
+      D.ohb = (B.win.h + B.ds - 1) / B.ds;
+      D.owb = (B.win.w + B.ds - 1) / B.ds;
+      paired[partner] = 1;
+    }
+    dec.push_back(D);
+  }
+  for (int i = 0; i < n; i++)
+    if (!paired[i]) return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: unplanned job");
+  for (size_t a = 0; a < dec.size(); a += CMB_MAX_DEC) {
+    DecTab T{};
+    T.n = (int)std::min<size_t>(CMB_MAX_DEC, dec.size() - a);
+    T.pre[0] = 0;
+    for (int k = 0; k < T.n; k++) {
+      T.j[k] = dec[a + k];
+      const int rows = T.j[k].fb ? T.j[k].ohb : T.j[k].oha;  // bands of the largest factor
+      T.pre[k + 1] = T.pre[k] + T.j[k].nbx * rows;
+    }
+    hipLaunchKernelGGL(k_decimate_tab, dim3(T.pre[T.n]), dim3(256), 0, s, T);
+    FSG_CHECK_LAUNCH("k_decimate_tab");
+  }
+  // box passes (NaN-free: means only): every (job, radius) in the same two launches
+  std::vector<BoxJob> b0, b1;
+  for (int i = 0; i < n; i++) {
+    const fsg_coarse_job& J = jobs[i];
+    const int h = (J.win.h + J.ds - 1) / J.ds, w = (J.win.w + J.ds - 1) / J.ds;
+    for (int k = 0; k < J.n_radii; k++) {
+      const int r = J.radii[k];
+      const double inv = 1.0 / (double)(2 * r + 1);
+      b0.push_back(BoxJob{grid[i], nullptr, (int64_t)w, h, w, r, inv, tv[i][k], nullptr,
+                          (int64_t)h, (w + 63) / 64});
+      b1.push_back(BoxJob{tv[i][k], nullptr, (int64_t)h, w, h, r, inv, J.outs[k], nullptr,
+                          (int64_t)J.ld_out, (h + 63) / 64});
+    }
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    const std::vector<BoxJob>& v = pass ? b1 : b0;
+    for (size_t a = 0; a < v.size(); a += CMB_MAX_BOX) {
+      BoxTab T{};
+      T.n = (int)std::min<size_t>(CMB_MAX_BOX, v.size() - a);
+      T.pre[0] = 0;
+      for (int k = 0; k < T.n; k++) {
+        T.j[k] = v[a + k];
+        T.pre[k + 1] = T.pre[k] + box_blocks(T.j[k]);
+      }
+      if (pass == 0)
+        hipLaunchKernelGGL(k_box_plain_tab<0>, dim3(T.pre[T.n]), dim3(64), 0, s, T);
+      else
+        hipLaunchKernelGGL(k_box_plain_tab<1>, dim3(T.pre[T.n]), dim3(64), 0, s, T);
+      FSG_CHECK_LAUNCH("k_box_plain_tab");
+    }
   }
   return FSG_OK;
 }

@@ -105,15 +105,16 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
   constexpr int NRA = NR > 0 ? NR : 1;
   const int NC = A.SW + 2 * A.R;
   const int LDC = NC + 1;
+  const int LDZ = A.SW + 1;                          // output columns only
   float* Vs = lds;                                   // [NR][K][LDC]
   float* Ws = Vs + NR * K * LDC;                     // [NR][K][LDC] (NaN-aware only)
-  float* Zs = NANAW ? (Ws + NR * K * LDC) : Ws;      // [K][LDC] centre -> partial sum
+  float* Zs = NANAW ? (Ws + NR * K * LDC) : Ws;      // [K][LDZ] centre -> partial sum
   // float64 tables from a 16-byte boundary: per source, rows_max staged
   // coarse rows of horizontally interpolated values [row][SW]; then per source
   // K row entries (staged-row offsets, ty, 1 - ty)
-  double* Hs_t = (double*)(lds + ((NR * K * LDC * (NANAW ? 2 : 1) + K * LDC + 3) & ~3));
+  double* Hs_t = (double*)(lds + ((NR * K * LDC * (NANAW ? 2 : 1) + K * LDZ + 3) & ~3));
   int2* RowI = (int2*)(Hs_t + A.src_floats);
-  double2* RowT = (double2*)(RowI + MAX_SRC * K);
+  double2* RowT = (double2*)(RowI + A.n_src * K);
 
   const int t = threadIdx.x;
   const int strip = tile % A.n_strips;
@@ -224,7 +225,7 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
             }
             Vs[(i * K + kb + k) * LDC + t] = (float)(S[i] * A.inv[i]);
           }
-          Zs[(kb + k) * LDC + t] = vz[k];
+          if (t >= A.R && t < A.R + A.SW) Zs[(kb + k) * LDZ + t - A.R] = vz[k];
         }
       }
     }
@@ -283,7 +284,7 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
         }
         for (int m = 0; m < L; m++) {
           const int lc = cbase + m;
-          const float z = Zs[k * LDC + lc];
+          const float z = Zs[k * LDZ + lc - A.R];
           float acc = 0.0f;
 #pragma unroll
           for (int i = 0; i < NR; i++) {
@@ -308,7 +309,7 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
             acc = (i == 0) ? diff : acc + diff;
           }
           // each output column belongs to exactly one thread: in-place is safe
-          Zs[k * LDC + lc] = acc;
+          Zs[k * LDZ + lc - A.R] = acc;
         }
       }
       __syncthreads();
@@ -334,7 +335,7 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
       for (int k = 0; k < K; k++) {
         if (k < nrow) {
           const float z = zq[k];
-          float acc = (NR > 0) ? Zs[k * LDC + A.R + t] : 0.0f;
+          float acc = (NR > 0) ? Zs[k * LDZ + t] : 0.0f;
           for (int j = 0; j < nz; j++) {
             const int2 ri = RowI[j * K + k];
             const double2 ty = RowT[j * K + k];
@@ -367,9 +368,9 @@ __device__ __forceinline__ void usm_fast_block(const UsmArgs& A, int tile, float
 inline size_t fast_lds_bytes(const UsmArgs& A, bool nanaw) {
   const int LDC = A.SW + 2 * A.R + 1;
   size_t lds = sizeof(float) * (((size_t)A.nr * FAST_K * LDC * (nanaw ? 2 : 1) +
-                                 (size_t)FAST_K * LDC + 3) & ~(size_t)3);
+                                 (size_t)FAST_K * (A.SW + 1) + 3) & ~(size_t)3);
   lds += sizeof(double) * (size_t)A.src_floats;  // staged rows (float64)
-  return lds + (sizeof(int2) + sizeof(double2)) * (size_t)MAX_SRC * FAST_K;
+  return lds + (sizeof(int2) + sizeof(double2)) * (size_t)A.n_src * FAST_K;
 }
 
 // Validation, coarse-grid preparation (decimation, void fill, coarse box

@@ -7,7 +7,7 @@ libfsg kernel.  Windows are 2-D float32 tensors whose rows may be strided
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -340,6 +340,42 @@ def topousm_coarse_means(win_t: torch.Tensor, ds: int, radii_coarse: Sequence[in
     N.check(L.fsg_topousm_coarse_means(C.byref(win), int(ds), n, rr, outs, wc, ws.data_ptr(),
                                        nbytes, _stream()))
     return out
+
+
+def topousm_coarse_means_batch(jobs: Sequence[Tuple[torch.Tensor, int, Sequence[int]]]
+                               ) -> List[torch.Tensor]:
+    """topousm_coarse_means for many NaN-free windows in a few launches.
+
+    jobs: (window tensor, ds in {4, 8, 16}, coarse radii >= 2, at most 4);
+    returns one (len(radii), ceil(h/ds), ceil(w/ds)) float32 tensor per job."""
+    n = len(jobs)
+    if n == 0:
+        return []
+    arr = (N.fsg_coarse_job * n)()
+    outs = []
+    for k, (win_t, ds, radii) in enumerate(jobs):
+        radii = [int(r) for r in radii]
+        if not 1 <= len(radii) <= N.COARSE_JOB_MAX_RADII:
+            raise ValueError("1 to 4 coarse radii per job")
+        h, w = win_t.shape
+        hc, wc = -(-h // int(ds)), -(-w // int(ds))
+        o = torch.empty((len(radii), hc, wc), dtype=torch.float32, device=win_t.device)
+        outs.append(o)
+        J = arr[k]
+        J.win = window(win_t)
+        J.ds = int(ds)
+        J.n_radii = len(radii)
+        for i, r in enumerate(radii):
+            J.radii[i] = r
+            J.outs[i] = o[i].data_ptr()
+        J.ld_out = wc
+    L = _lib()
+    nbytes = int(L.fsg_topousm_coarse_means_batch_workspace_size(arr, n))
+    if nbytes <= 0:
+        raise RuntimeError("fsg_topousm_coarse_means_batch_workspace_size failed")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=jobs[0][0].device)
+    N.check(L.fsg_topousm_coarse_means_batch(arr, n, ws.data_ptr(), nbytes, _stream()))
+    return outs
 
 
 def topousm_coarse_field(coarse_t: torch.Tensor, radii_coarse: Sequence[int],

@@ -24,7 +24,7 @@ from . import ops
 from .algorithms import (ALGORITHMS, SUPPORTED, CoarsePyramid, OutSpec, TopoUSMFastAlgorithm,
                          UnsupportedOnBackend, load_algorithm)
 from .encoding import SUPPORTED_OUTPUT_DTYPES, quantize_params, resolve_output_range
-from .params import (LOCAL_RADII, LOCAL_WEIGHTS, MULTISCALE_REQUIRED_ALGOS,
+from .params import (LOCAL_RADII, LOCAL_WEIGHTS, MULTISCALE_REQUIRED_ALGOS, NORM_STAT_ALGOS,
                      RADII_DRIVEN_ALGOS, SPATIAL_TILE_ALGORITHMS, auto_spatial_profile,
                      normalize_topousm_radii_weights, overview_geometry, required_padding,
                      sanitize_spatial_radii_weights_for_tile, split_radii_by_threshold,
@@ -35,7 +35,7 @@ from .raster_info import metric_pixel_scales, row_geographic_scales_x, tile_geog
 ROW_SCALED = ("hillshade", "slope", "curvature", "hillshade_slope")
 # algorithms whose spatial mode (multi-radius smoothing) runs on the GPU
 SPATIAL_GPU = ("hillshade", "slope", "curvature", "openness", "ambient_occlusion")
-from .stats import norm_stats_prepass
+from .stats import norm_stats_prepass, prepass_pyramid_pairs, valid_bbox_begin
 
 logger = logging.getLogger(__name__)
 
@@ -270,6 +270,11 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
         fld, scale = overview_field(src, lr, lw)
         P.params["_topousm_fast_coarse_field"] = fld
         P.extras["overview_scale"] = scale
+    stats = global_stats if global_stats is not None else P.params.get("global_stats")
+    bbox_handle = None
+    if stats is None and algorithm in NORM_STAT_ALGOS:
+        # the pre-pass overview, read back with the counts (one synchronisation)
+        bbox_handle = valid_bbox_begin(src)
     nan_counts = counts.cpu().numpy()
     # the tile windows cover the raster: all zero means the raster holds no NaN
     raster_nan_free = bool((nan_counts == 0).all())
@@ -291,17 +296,20 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     pyramid = None
     if algorithm == "topousm_fast" and raster_nan_free:
         pyramid = CoarsePyramid(src)
-        pyramid.require(CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(P.params)))
+        pairs = CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(P.params))
+        if stats is None:
+            # the pre-pass windows' levels too: every level in one batched build
+            pairs = pairs + prepass_pyramid_pairs(P.params)
+        pyramid.require(pairs)
     reflect_d = 0
     if edge_mode == "reflect" and not spatial_tiles:
         reflect_d = dask_edge_depth(algorithm, P.params)
         if P.params.get("_geographic_rows"):
             raise ValueError("edge_mode='reflect' takes one pixel scale per tile "
                              "(geographic_rows is not defined for it)")
-    stats = global_stats if global_stats is not None else P.params.get("global_stats")
     if stats is None:
         stats = norm_stats_prepass(src, algorithm, P.params, raster_nan_free=raster_nan_free,
-                                   pyramid=pyramid)
+                                   pyramid=pyramid, bbox_handle=bbox_handle)
     if stats is not None:
         P.params["global_stats"] = stats
     P.stats = stats

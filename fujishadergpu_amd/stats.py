@@ -16,10 +16,12 @@ from typing import Optional
 import numpy as np
 import torch
 
+from . import _native as N
 from . import ops
-from .algorithms import (AmbientOcclusionAlgorithm, OpennessAlgorithm, OutSpec,
+from .algorithms import (AmbientOcclusionAlgorithm, CoarsePyramid, OpennessAlgorithm, OutSpec,
                          TopoUSMFastAlgorithm)
-from .params import NORM_STAT_ALGOS, norm_stat_window_geometry, stratified_windows
+from .params import (NORM_STAT_ALGOS, coarse_radius, norm_stat_window_geometry,
+                     stratified_windows)
 
 logger = logging.getLogger(__name__)
 
@@ -39,10 +41,11 @@ _BLOCK_KEYS = {
 }
 
 
-def valid_bbox(raster: torch.Tensor):
-    """Valid-data bounding box from a ~512 px nearest-neighbour overview
-    (_norm_stats.py:254-264).  GDAL's nearest pick for non-integer ratios is
-    unpinned; the overview samples pixel centres floor((i + 0.5) * H / oh)."""
+def valid_bbox_begin(raster: torch.Tensor):
+    """Start valid_bbox: the <= 512^2 overview samples are gathered on the
+    device and copied into pinned host memory on the current stream; the
+    returned handle is finished by valid_bbox_end once the stream got there
+    (e.g. after any later synchronisation)."""
     H, W = raster.shape
     cov = max(1, max(W, H) // 512)
     oh, ow = max(1, H // cov), max(1, W // cov)
@@ -50,10 +53,21 @@ def valid_bbox(raster: torch.Tensor):
                      .floor().long(), max=H - 1)
     ci = torch.clamp(((torch.arange(ow, device=raster.device, dtype=torch.float64) + 0.5) * (W / ow))
                      .floor().long(), max=W - 1)
-    # the <= 512^2 overview samples are gathered on the device and read back
-    # once; the bounding box of their finite cells is host logic
-    ov = raster.index_select(0, ri).index_select(1, ci).cpu().numpy()
-    vm = np.isfinite(ov)
+    ov = raster.index_select(0, ri).index_select(1, ci)
+    if not raster.is_cuda:
+        return ov, None, cov, H, W
+    host = torch.empty((oh, ow), dtype=raster.dtype, pin_memory=True)
+    host.copy_(ov, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record()
+    return host, ev, cov, H, W
+
+
+def valid_bbox_end(handle):
+    host, ev, cov, H, W = handle
+    if ev is not None:
+        ev.synchronize()
+    vm = np.isfinite(host.numpy())
     ys, xs = np.nonzero(vm.any(axis=1))[0], np.nonzero(vm.any(axis=0))[0]
     if ys.size == 0:
         return None
@@ -62,27 +76,78 @@ def valid_bbox(raster: torch.Tensor):
     return by0, by1, bx0, bx1
 
 
-def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
-                       grid: int = 3, max_tile: int = 4096, min_valid_frac: float = 0.02,
-                       return_pool: bool = False, raster_nan_free: bool = False,
-                       pyramid=None):
-    """Device-side _compute_norm_stats_tiled.  ``raster`` holds NaN at NoData.
-    Returns a float64 GPU tensor (stats) or None (no valid window).
+def valid_bbox(raster: torch.Tensor):
+    """Valid-data bounding box from a ~512 px nearest-neighbour overview
+    (_norm_stats.py:254-264).  GDAL's nearest pick for non-integer ratios is
+    unpinned; the overview samples pixel centres floor((i + 0.5) * H / oh).
+    The bounding box of the overview's finite cells is host logic."""
+    return valid_bbox_end(valid_bbox_begin(raster))
 
-    ``raster_nan_free`` (known from the NoData scan) skips the per-window NaN
-    count; ``pyramid`` (a CoarsePyramid of this raster, NaN-free rasters only)
-    lets topousm_fast windows share raster-wide coarse means where exact."""
-    if algorithm not in NORM_STAT_ALGOS:
-        return None
+
+def prepass_params(algorithm: str, params: dict):
+    """(merged parameters, block-function keywords) of the pre-pass windows:
+    the defaults under the caller's parameters, topousm_fast with its full
+    radius list (the large radii included)."""
     merged = {**_DEFAULTS[algorithm](), **(params or {})}
     if merged.get("_topousm_fast_full_radii"):
         merged["radii"] = list(merged["_topousm_fast_full_radii"])
         if merged.get("_topousm_fast_full_weights"):
             merged["weights"] = list(merged["_topousm_fast_full_weights"])
     kw = {k: merged[k] for k in _BLOCK_KEYS[algorithm] if merged.get(k) is not None}
+    return merged, kw
+
+
+def prepass_pyramid_pairs(params: dict):
+    """(ds, coarse radius) pairs the topousm_fast pre-pass windows read."""
+    _, kw = prepass_params("topousm_fast", params)
+    return CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(kw))
+
+
+def _own_coarse_means(raster, algo, bp, regions, pyramid):
+    """Coarse means of the pre-pass windows that the raster-wide grids do not
+    cover, built for all windows in a few batched launches: per region index,
+    {radius index: grid}."""
+    jobs, keys = [], []
+    for ri, (wy, wx, th, tw, m, nan_free) in enumerate(regions):
+        if not nan_free:
+            continue
+        prm = algo.params_struct(bp)
+        pyramid.attach(prm, (wy, wx), (th, tw), ops.region(m, m, th - 2 * m, tw - 2 * m))
+        by_ds = {}
+        for i in range(prm.n_radii):
+            ds = int(prm.ds[i])
+            if ds > 1 and not prm.mean[i]:
+                by_ds.setdefault(ds, []).append(i)
+        for ds, idx in by_ds.items():
+            rs = [coarse_radius(int(prm.radii[i]), ds) for i in idx]
+            if ds in (4, 8, 16) and min(rs) >= 2 and len(rs) <= N.COARSE_JOB_MAX_RADII:
+                jobs.append((raster[wy:wy + th, wx:wx + tw], ds, rs))
+                keys.append((ri, idx))
+    own = {}
+    for (ri, idx), g in zip(keys, ops.topousm_coarse_means_batch(jobs)):
+        for k, i in enumerate(idx):
+            own.setdefault(ri, {})[i] = g[k]
+    return own
+
+
+def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
+                       grid: int = 3, max_tile: int = 4096, min_valid_frac: float = 0.02,
+                       return_pool: bool = False, raster_nan_free: bool = False,
+                       pyramid=None, bbox_handle=None):
+    """Device-side _compute_norm_stats_tiled.  ``raster`` holds NaN at NoData.
+    Returns a float64 GPU tensor (stats) or None (no valid window).
+
+    ``raster_nan_free`` (known from the NoData scan) skips the per-window NaN
+    count; ``pyramid`` (a CoarsePyramid of this raster, NaN-free rasters only)
+    lets topousm_fast windows share raster-wide coarse means where exact;
+    ``bbox_handle`` (valid_bbox_begin of this raster, queued earlier) saves
+    the overview read its own synchronisation."""
+    if algorithm not in NORM_STAT_ALGOS:
+        return None
+    merged, kw = prepass_params(algorithm, params)
     margin, tile = norm_stat_window_geometry(algorithm, merged, max_tile)
     H, W = raster.shape
-    bbox = valid_bbox(raster)
+    bbox = valid_bbox_end(bbox_handle) if bbox_handle is not None else valid_bbox(raster)
     if bbox is None:
         return None
     wins = stratified_windows(W, H, *bbox, grid=grid, tile=min(tile, max(W, H)))
@@ -105,10 +170,12 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
     algo = {"topousm_fast": TopoUSMFastAlgorithm, "openness": OpennessAlgorithm,
             "ambient_occlusion": AmbientOcclusionAlgorithm}[algorithm]()
     bp = dict(kw)
+    own = {}
     if pyramid is not None and algorithm == "topousm_fast":
         pyramid.require(pyramid.pairs(algo.params_struct(bp)))
+        own = _own_coarse_means(raster, algo, bp, regions, pyramid)
     off = 0
-    for wy, wx, th, tw, m, nan_free in regions:
+    for ri, (wy, wx, th, tw, m, nan_free) in enumerate(regions):
         win = raster[wy:wy + th, wx:wx + tw]
         rh, rw = th - 2 * m, tw - 2 * m
         dst = pool[off:off + rh * rw].view(rh, rw)
@@ -118,6 +185,8 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
             if pyramid is not None:
                 wp["_topousm_fast_pyramid"] = pyramid
                 wp["_window_origin"] = (wy, wx)
+            if ri in own:
+                wp["_topousm_fast_means"] = own[ri]
             algo.run_region(win, reg, OutSpec(f32=dst), wp, normalize=False)
         else:
             algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "mode": "local"}, stretch=False)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 10
+#define FSG_ABI_VERSION 11
 
 enum {
   FSG_OK = 0,
@@ -320,6 +320,32 @@ int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32_t n_radii,
                              const int32_t* radii_coarse, float* const* outs,
                              int64_t ld_out, void* workspace, size_t workspace_bytes,
                              void* stream);
+
+/*
+ * The same coarse means for many NaN-free windows in a few launches: every
+ * decimation in one launch (two jobs on one window whose factors differ by 2x
+ * or 4x share one read of it), then every (job, radius) box pass 0 in one
+ * launch and every pass 1 in one more.  Outputs equal
+ * fsg_topousm_coarse_means' per job bit for bit.  The caller vouches that each
+ * window holds no NaN (the enclosed-void fill is not run); ds is 4, 8 or 16
+ * and every coarse radius >= 2 (FSG_E_UNSUPPORTED otherwise: those take
+ * fsg_topousm_coarse_means).  Replaces, for the windows of the stratified
+ * statistics pre-pass (algorithms/_norm_stats.py:176-350) and the raster-wide
+ * grids, the per-window decimate + uniform_filter sequence of
+ * compute_topousm_fast_efficient_block (_impl_topousm_fast.py:72-91).
+ */
+#define FSG_COARSE_JOB_MAX_RADII 4
+typedef struct fsg_coarse_job {
+  fsg_window win;
+  int32_t ds;
+  int32_t n_radii;
+  int32_t radii[FSG_COARSE_JOB_MAX_RADII];   /* coarse radii (max(1, round(r / ds))) */
+  float* outs[FSG_COARSE_JOB_MAX_RADII];     /* ceil(h/ds) x ceil(w/ds) grids */
+  int64_t ld_out;
+} fsg_coarse_job;
+size_t fsg_topousm_coarse_means_batch_workspace_size(const fsg_coarse_job* jobs, int32_t n);
+int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_t n, void* workspace,
+                                   size_t workspace_bytes, void* stream);
 
 /*
  * NaN-aware block mean (area average for an integer factor), the restatement

@@ -224,6 +224,54 @@ def test_topousm_shared_coarse_means(cuda):
     _cmp("topousm/shared", shared, exp, 2e-4)
 
 
+def test_topousm_coarse_means_batch_equals_per_window(cuda):
+    """The batched coarse means (one decimation launch with two factors sharing
+    a read, one launch per box pass) equal the per-window entry point bit for
+    bit: aligned and unaligned window views, ragged edges, single and paired
+    factors, several radii per factor; and a pre-pass window reading them
+    equals the same window building its own."""
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.algorithms import OutSpec, TopoUSMFastAlgorithm
+    dem = _dem(1500, 1700, 37, relief=3776.0, hurst=0.75)
+    g = dem.to(cuda)
+    w1 = g[3:1403, 5:1617]          # unaligned origin, 1400 x 1612
+    w2 = g[64:1364, 128:1152]       # float4-aligned origin, 1300 x 1024
+    w3 = g[0:1037, 0:1501]
+    jobs = [(w1, 16, [32, 128]), (w1, 4, [32]), (w2, 4, [2, 9]), (w2, 8, [5]),
+            (w3, 8, [3, 17]), (w3, 16, [2])]
+    got = ops.topousm_coarse_means_batch(jobs)
+    for (w, ds, rs), t in zip(jobs, got):
+        want = ops.topousm_coarse_means(w, ds, rs)
+        assert t.shape == want.shape
+        assert torch.equal(t, want), (ds, rs)
+    # a window of 1400 x 1612 reading them as its own means
+    algo = TopoUSMFastAlgorithm()
+    p = dict(radii=[2, 8, 32, 128, 512, 2048], weights=[0.3, 0.2, 0.15, 0.15, 0.1, 0.1],
+             pixel_size=5.0, _window_nan_free=True)
+    prm = algo.params_struct(p)
+    assert [int(prm.ds[i]) for i in range(prm.n_radii)] == [1, 1, 1, 4, 16, 16]
+    own = ops.topousm_coarse_means_batch([(w1, 16, [32, 128]), (w1, 4, [32])])
+    reg = ops.region(300, 300, 800, 1012)
+    a = torch.empty((800, 1012), dtype=torch.float32, device=cuda)
+    b = torch.empty_like(a)
+    algo.run_region(w1, reg, OutSpec(f32=a), p, normalize=False)
+    algo.run_region(w1, reg, OutSpec(f32=b),
+                    {**p, "_topousm_fast_means": {3: own[1][0], 4: own[0][0], 5: own[0][1]}},
+                    normalize=False)
+    assert torch.equal(a, b)
+
+
+def test_topousm_coarse_means_batch_rejects_what_it_does_not_batch(cuda):
+    from fujishadergpu_amd import _native as N, ops
+    g = _dem(300, 300, 5).to(cuda)
+    with pytest.raises(N.FsgError):
+        ops.topousm_coarse_means_batch([(g, 2, [8])])      # ds 2: per-window entry point
+    with pytest.raises(N.FsgError):
+        ops.topousm_coarse_means_batch([(g, 4, [1])])      # Gaussian branch
+    with pytest.raises(ValueError):
+        ops.topousm_coarse_means_batch([(g, 4, [2, 3, 4, 5, 6])])
+
+
 @pytest.mark.parametrize("otype", ["positive", "negative"])
 @pytest.mark.parametrize("nan_discs", [0, 6])
 def test_openness_window(cuda, otype, nan_discs):
