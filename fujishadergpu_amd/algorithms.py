@@ -424,8 +424,11 @@ class CoarsePyramid:
     raster-wide grid per (ds, coarse radius) instead.
     """
 
-    def __init__(self, raster: torch.Tensor):
+    def __init__(self, raster: torch.Tensor, decimated: Optional[Dict[int, torch.Tensor]] = None):
+        """decimated: {ds: the raster's NaN-free ds x ds cell means}, already
+        computed (e.g. by ops.raster_scan); those levels skip their own read."""
         self.raster = raster
+        self.decimated = dict(decimated or {})
         self.grids: Dict[Tuple[int, int], torch.Tensor] = {}
 
     @staticmethod
@@ -443,7 +446,9 @@ class CoarsePyramid:
         # raster) where every level fits it, else one call per level
         if todo and all(ds in (4, 8, 16) and len(rss) <= N.COARSE_JOB_MAX_RADII
                         and min(rss) >= 2 for ds, rss in todo):
-            res = ops.topousm_coarse_means_batch([(self.raster, ds, rss) for ds, rss in todo])
+            res = ops.topousm_coarse_means_batch(
+                [(self.decimated[ds], 1, rss) if ds in self.decimated else (self.raster, ds, rss)
+                 for ds, rss in todo])
         else:
             res = [ops.topousm_coarse_means(self.raster, ds, rss) for ds, rss in todo]
         for (ds, rss), g in zip(todo, res):

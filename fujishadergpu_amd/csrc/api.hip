@@ -289,6 +289,100 @@ k_block_mean_p2v(const float* __restrict__ src, int64_t ld, int h, int w, int f,
   }
 }
 
+// One read of a raster for its NaN count and NaN-aware cell means at up to
+// three power-of-two factors 4..16 (fsg_raster_scan).  A block covers 1024
+// source columns (four per lane) of F source rows, F the largest factor:
+// each lane keeps float64 sums and counts of its four columns per 4-row
+// group, then a cell of factor f adds f/4 groups and f/4 lanes.  Cell sums
+// are exact in float64, so every cell equals k_block_mean_p2v's (and the
+// topousm decimation's) bit for bit.
+constexpr int SCAN_MAX_OUT = 3;
+struct ScanOut {
+  float* p;
+  int64_t ld;
+  int f, oh, ow;
+};
+struct ScanArgs {
+  const float* x;
+  int64_t ld;
+  int h, w;
+  int n_out;
+  ScanOut o[SCAN_MAX_OUT];
+  unsigned long long* nan_count;
+  int vec;
+};
+
+template <int F>
+__global__ void __launch_bounds__(256) k_raster_scan(ScanArgs A) {
+  constexpr int G = F / 4;  // 4-row groups per band
+  const int c = (blockIdx.x * 256 + threadIdx.x) * 4;
+  const int rb = blockIdx.y * F;
+  double sg[G];
+  int cg[G];
+  unsigned long long nn = 0;
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    sg[g] = 0.0;
+    cg[g] = 0;
+  }
+  if (c < A.w) {
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      float v[4][4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int r = min(rb + 4 * g + e, A.h - 1);
+        const float* p = A.x + (int64_t)r * A.ld + c;
+        if (A.vec && c + 3 < A.w) {
+          const float4 q = *(const float4*)p;
+          v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; k++) v[e][k] = c + k < A.w ? p[k] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if (rb + 4 * g + e >= A.h) break;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (c + k >= A.w) break;
+          const float x = v[e][k];
+          if (isnan(x)) nn++;
+          if (isfinite(x)) {
+            sg[g] += (double)x;
+            cg[g]++;
+          }
+        }
+      }
+    }
+  }
+  for (int k = 0; k < A.n_out; k++) {
+    const ScanOut& O = A.o[k];
+    const int gpc = O.f / 4;  // groups (and lanes) per cell
+    for (int q = 0; q < G / gpc; q++) {
+      double tot = 0.0;
+      int cnt = 0;
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (g / gpc == q) {
+          tot += sg[g];
+          cnt += cg[g];
+        }
+      for (int off = 1; off < gpc; off <<= 1) {
+        tot += __shfl_down(tot, off, 64);
+        cnt += __shfl_down(cnt, off, 64);
+      }
+      const int orow = (rb + q * O.f) / O.f;
+      const int oc = c / O.f;
+      if ((threadIdx.x & (gpc - 1)) == 0 && oc < O.ow && orow < O.oh && c < A.w)
+        O.p[(int64_t)orow * O.ld + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) nn += __shfl_down(nn, off, 64);
+  if ((threadIdx.x & 63) == 0 && nn) atomicAdd(A.nan_count, nn);
+}
+
 // -------------------------------------------------------------------- API
 
 extern "C" int fsg_abi_version(void) { return FSG_ABI_VERSION; }
@@ -395,6 +489,45 @@ extern "C" int fsg_encode(const fsg_window* src, const fsg_region* reg, const fs
   hipLaunchKernelGGL(k_encode, grd, blk, 0, as_stream(stream), src->data, src->ld, reg->r0,
                      reg->c0, reg->h, reg->w, make_out(out));
   FSG_CHECK_LAUNCH("k_encode");
+  return FSG_OK;
+}
+
+extern "C" int fsg_raster_scan(const fsg_window* raster, int32_t n_out,
+                               const fsg_scan_out* outs, uint64_t* nan_count, void* stream) {
+  int rc = validate_window(raster, "fsg_raster_scan");
+  if (rc) return rc;
+  if (raster->h == 0 || raster->w == 0) return fail(FSG_E_INVALID, "fsg_raster_scan: empty");
+  if (!nan_count) return fail(FSG_E_INVALID, "fsg_raster_scan: nan_count NULL");
+  if (n_out < 0 || n_out > SCAN_MAX_OUT || (n_out > 0 && !outs))
+    return fail(FSG_E_INVALID, "fsg_raster_scan: 0..3 outputs");
+  ScanArgs A{};
+  A.x = raster->data;
+  A.ld = raster->ld;
+  A.h = raster->h;
+  A.w = raster->w;
+  A.n_out = n_out;
+  A.nan_count = (unsigned long long*)nan_count;
+  A.vec = (raster->ld % 4 == 0 && (reinterpret_cast<uintptr_t>(raster->data) & 15) == 0) ? 1 : 0;
+  int F = 4;
+  for (int k = 0; k < n_out; k++) {
+    const int f = outs[k].factor;
+    if (f != 4 && f != 8 && f != 16)
+      return fail(FSG_E_UNSUPPORTED, "fsg_raster_scan: factors are 4, 8 or 16");
+    const int oh = (raster->h + f - 1) / f, ow = (raster->w + f - 1) / f;
+    if (!outs[k].out || outs[k].ld_out < ow)
+      return fail(FSG_E_INVALID, "fsg_raster_scan: bad output");
+    A.o[k] = ScanOut{outs[k].out, outs[k].ld_out, f, oh, ow};
+    F = std::max(F, f);
+  }
+  hipStream_t s = as_stream(stream);
+  dim3 blk(256), grd((raster->w + 1023) / 1024, (raster->h + F - 1) / F);
+  if (F == 4)
+    hipLaunchKernelGGL(k_raster_scan<4>, grd, blk, 0, s, A);
+  else if (F == 8)
+    hipLaunchKernelGGL(k_raster_scan<8>, grd, blk, 0, s, A);
+  else
+    hipLaunchKernelGGL(k_raster_scan<16>, grd, blk, 0, s, A);
+  FSG_CHECK_LAUNCH("k_raster_scan");
   return FSG_OK;
 }
 

@@ -489,54 +489,63 @@ __global__ void __launch_bounds__(256) k_decimate_tab(DecTab T) {
 template <int PASS>
 __device__ __forceinline__ void box_plain_block(const float* __restrict__ a, int64_t lda, int h,
                                                 int w, int r, double inv, float* __restrict__ o,
-                                                int64_t ldo, int bx, int by, float (*ta)[65]) {
+                                                int64_t ldo, int nsub, int bx, int by,
+                                                float (*ta)[65]) {
+  // a band of nsub 64-row sub-bands: the window sum is built once per band
+  // (2r + 1 rows) and runs on across its sub-bands, each transposed via LDS
   const int lane = threadIdx.x;
   const int c = bx * 64 + lane;
-  const int y0 = by * 64;
-  const int y1 = min(y0 + 64, h);
+  const int yb0 = by * 64 * nsub;
+  const float* pa = a + c;
+  double s = 0.0;
   if (c < w) {
-    const float* pa = a + c;
-    double s = 0.0;
-    // window rows y0-r .. y0+r, 32 independent loads per batch
-    for (int d0 = -r; d0 <= r; d0 += 32) {
-      float va[32];
+    // window rows yb0-r .. yb0+r, 64 independent loads per batch
+    for (int d0 = -r; d0 <= r; d0 += 64) {
+      float va[64];
 #pragma unroll
-      for (int e = 0; e < 32; e++) va[e] = pa[(int64_t)refl(y0 + min(d0 + e, r), h) * lda];
+      for (int e = 0; e < 64; e++) va[e] = pa[(int64_t)refl(yb0 + min(d0 + e, r), h) * lda];
 #pragma unroll
-      for (int e = 0; e < 32; e++) {
-        if (d0 + e > r) break;
-        s += (double)va[e];
-      }
-    }
-    for (int yb = y0; yb < y1; yb += 16) {
-      float na[16], ov[16];
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        const int y = min(yb + e, y1 - 1);
-        na[e] = pa[(int64_t)refl(y + r, h) * lda];
-        ov[e] = pa[(int64_t)refl(y - r - 1, h) * lda];
-      }
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        const int y = yb + e;
-        if (y >= y1) break;
-        if (y > y0) {
-          if (PASS == 0) {
-            s += (double)na[e];
-            s -= (double)ov[e];
-          } else {
-            s += (double)na[e] - (double)ov[e];
-          }
-        }
-        ta[y - y0][lane] = (float)(s * inv);
-      }
+      for (int e = 0; e < 64; e++)
+        if (d0 + e <= r) s += (double)va[e];
     }
   }
-  __syncthreads();
-  const int ncol = min(64, w - bx * 64);
-  for (int cc = 0; cc < ncol; cc++) {
-    const int y = y0 + lane;
-    if (y < y1) o[(int64_t)(bx * 64 + cc) * ldo + y] = ta[lane][cc];
+  for (int sb = 0; sb < nsub; sb++) {
+    const int y0 = yb0 + sb * 64;
+    if (y0 >= h) break;  // block-uniform
+    const int y1 = min(y0 + 64, h);
+    if (c < w) {
+      for (int yb = y0; yb < y1; yb += 32) {
+        float na[32], ov[32];
+#pragma unroll
+        for (int e = 0; e < 32; e++) {
+          const int y = min(yb + e, y1 - 1);
+          na[e] = pa[(int64_t)refl(y + r, h) * lda];
+          ov[e] = pa[(int64_t)refl(y - r - 1, h) * lda];
+        }
+#pragma unroll
+        for (int e = 0; e < 32; e++) {
+          const int y = yb + e;
+          if (y < y1) {
+            if (y > yb0) {
+              if (PASS == 0) {
+                s += (double)na[e];
+                s -= (double)ov[e];
+              } else {
+                s += (double)na[e] - (double)ov[e];
+              }
+            }
+            ta[y - y0][lane] = (float)(s * inv);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int ncol = min(64, w - bx * 64);
+    for (int cc = 0; cc < ncol; cc++) {
+      const int y = y0 + lane;
+      if (y < y1) o[(int64_t)(bx * 64 + cc) * ldo + y] = ta[lane][cc];
+    }
+    __syncthreads();
   }
 }
 
@@ -546,8 +555,8 @@ __global__ void __launch_bounds__(64) k_box_plain_tab(BoxTab T) {
   const int j = tab_job_of(T.pre, T.n, blockIdx.x);
   const BoxJob& J = T.j[j];
   const int local = blockIdx.x - T.pre[j];
-  box_plain_block<PASS>(J.a, J.lda, J.h, J.w, J.r, J.inv, J.oa, J.ldo, local % J.nbx, local / J.nbx,
-                  ta);
+  box_plain_block<PASS>(J.a, J.lda, J.h, J.w, J.r, J.inv, J.oa, J.ldo, J.nsub > 1 ? J.nsub : 1,
+                        local % J.nbx, local / J.nbx, ta);
 }
 
 __global__ void k_gauss9_grid(const float* __restrict__ x, int64_t ldx, int h, int w,
@@ -1456,9 +1465,9 @@ extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_
     if (rc) return rc;
     if (J.win.h == 0 || J.win.w == 0)
       return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: empty window");
-    if (J.ds < 4 || J.ds > 16 || (J.ds & (J.ds - 1)) != 0)
+    if (J.ds != 1 && (J.ds < 4 || J.ds > 16 || (J.ds & (J.ds - 1)) != 0))
       return fail(FSG_E_UNSUPPORTED,
-                  "fsg_topousm_coarse_means_batch: ds must be 4, 8 or 16 (use "
+                  "fsg_topousm_coarse_means_batch: ds must be 1, 4, 8 or 16 (use "
                   "fsg_topousm_coarse_means)");
     if (J.n_radii < 1 || J.n_radii > FSG_COARSE_JOB_MAX_RADII)
       return fail(FSG_E_INVALID, "fsg_topousm_coarse_means_batch: n_radii out of range");
@@ -1479,6 +1488,7 @@ extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_
   hipStream_t s = as_stream(stream);
   // workspace carve-up
   std::vector<float*> grid(n);
+  std::vector<int64_t> ldg(n);
   std::vector<std::vector<float*>> tv(n);
   {
     char* p = (char*)workspace;
@@ -1487,7 +1497,12 @@ extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_
       const size_t g = align256(sizeof(float) * (size_t)((J.win.h + J.ds - 1) / J.ds) *
                                 (size_t)((J.win.w + J.ds - 1) / J.ds));
       grid[i] = (float*)p;
+      ldg[i] = (J.win.w + J.ds - 1) / J.ds;
       p += g;
+      if (J.ds == 1) {  // the window already is the grid
+        grid[i] = const_cast<float*>(J.win.data);
+        ldg[i] = J.win.ld;
+      }
       for (int k = 0; k < J.n_radii; k++) {
         tv[i].push_back((float*)p);
         p += g;
@@ -1498,6 +1513,8 @@ extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_
   // rides along with that job's read
   std::vector<DualDecJob> dec;
   std::vector<int> paired(n, 0);
+  for (int i = 0; i < n; i++)
+    if (jobs[i].ds == 1) paired[i] = 1;  // nothing to decimate
   for (int i = 0; i < n; i++) {
     if (paired[i]) continue;
     const fsg_coarse_job& A = jobs[i];
@@ -1557,10 +1574,13 @@ extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_
     for (int k = 0; k < J.n_radii; k++) {
       const int r = J.radii[k];
       const double inv = 1.0 / (double)(2 * r + 1);
-      b0.push_back(BoxJob{grid[i], nullptr, (int64_t)w, h, w, r, inv, tv[i][k], nullptr,
-                          (int64_t)h, (w + 63) / 64});
+      // one 64-row sub-band per band: the launch lasts as long as its
+      // slowest block's chain of loads, which taller bands lengthen
+      const int nsub = 1;
+      b0.push_back(BoxJob{grid[i], nullptr, ldg[i], h, w, r, inv, tv[i][k], nullptr,
+                          (int64_t)h, (w + 63) / 64, nsub});
       b1.push_back(BoxJob{tv[i][k], nullptr, (int64_t)h, w, h, r, inv, J.outs[k], nullptr,
-                          (int64_t)J.ld_out, (h + 63) / 64});
+                          (int64_t)J.ld_out, (h + 63) / 64, nsub});
     }
   }
   for (int pass = 0; pass < 2; pass++) {

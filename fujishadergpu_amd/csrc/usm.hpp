@@ -401,10 +401,12 @@ struct BoxJob {
   float* oa;
   float* ob;
   int64_t ldo;
-  int nbx;   // blocks per 64-row band
+  int nbx;   // blocks per band
+  int nsub;  // 64-row sub-bands per band (0 or 1: one; the NaN-free passes only)
 };
 inline int dec_blocks(const DecJob& J) { return J.nbx * J.oh; }
-inline int box_blocks(const BoxJob& J) { return J.nbx * ((J.h + 63) / 64); }
+inline int box_band(const BoxJob& J) { return 64 * (J.nsub > 1 ? J.nsub : 1); }
+inline int box_blocks(const BoxJob& J) { return J.nbx * ((J.h + box_band(J) - 1) / box_band(J)); }
 
 // The jobs the coarse preparation of this NaN-free window launches one by one
 // (decimations in dec; level L's box passes in box0[L] / box1[L]).  Returns

@@ -342,6 +342,22 @@ def topousm_coarse_means(win_t: torch.Tensor, ds: int, radii_coarse: Sequence[in
     return out
 
 
+def raster_scan(raster: torch.Tensor, factors: Sequence[int]):
+    """One read of the raster: (device int64 NaN count, [NaN-aware block mean
+    per factor]); factors are 4, 8 or 16, at most three."""
+    h, w = raster.shape
+    factors = [int(f) for f in factors]
+    outs = [torch.empty((-(-h // f), -(-w // f)), dtype=torch.float32, device=raster.device)
+            for f in factors]
+    arr = (N.fsg_scan_out * max(1, len(factors)))()
+    for k, (f, o) in enumerate(zip(factors, outs)):
+        arr[k] = N.fsg_scan_out(f, o.data_ptr(), int(o.stride(0)))
+    cnt = torch.zeros(1, dtype=torch.int64, device=raster.device)
+    win = window(raster)
+    N.check(_lib().fsg_raster_scan(C.byref(win), len(factors), arr, cnt.data_ptr(), _stream()))
+    return cnt, outs
+
+
 def topousm_coarse_means_batch(jobs: Sequence[Tuple[torch.Tensor, int, Sequence[int]]]
                                ) -> List[torch.Tensor]:
     """topousm_coarse_means for many NaN-free windows in a few launches.

@@ -203,10 +203,21 @@ def plan(raster: torch.Tensor, algorithm: str, *, tile_size: Optional[int] = 409
                    qp, extras=extras)
 
 
-def overview_field(raster: torch.Tensor, large_radii, large_weights) -> Tuple[torch.Tensor, float]:
+def overview_factor(H: int, W: int) -> Optional[int]:
+    """The overview's integer decimation factor, or None when it is fractional."""
+    scale, sh, sw = overview_geometry(W, H)
+    f = int(round(scale))
+    if abs(scale - f) > 1e-12 or sh * f != H or sw * f != W:
+        return None
+    return f
+
+
+def overview_field(raster: torch.Tensor, large_radii, large_weights,
+                   coarse: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, float]:
     """Large-radius coarse field from the overview (core/tile_processor.py:552-602,
     _impl_topousm_fast.py:133-155).  The overview read is GDAL average
-    resampling; for an integer decimation that is the NaN-aware block mean."""
+    resampling; for an integer decimation that is the NaN-aware block mean
+    (``coarse``, when the caller already has it)."""
     H, W = raster.shape
     scale, sh, sw = overview_geometry(W, H)
     f = int(round(scale))
@@ -214,9 +225,39 @@ def overview_field(raster: torch.Tensor, large_radii, large_weights) -> Tuple[to
         raise UnsupportedOnBackend(
             f"overview decimation {scale:.4f} for a {W}x{H} raster is not an integer block "
             "mean; GDAL's fractional average resampling is not restated")
-    coarse = ops.block_mean(raster, f) if f > 1 else raster.contiguous().clone()
+    if coarse is None:
+        coarse = ops.block_mean(raster, f) if f > 1 else raster.contiguous().clone()
     rc = [max(1, int(round(float(r) / max(scale, 1.0)))) for r in large_radii]
     return ops.topousm_coarse_field(coarse, rc, large_weights), scale
+
+
+def _pyramid_pairs(P, with_prepass: bool):
+    """(ds, coarse radius) levels of the raster-wide pyramid the tiles (and,
+    when the statistics pre-pass runs, its windows) read."""
+    pairs = CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(P.params))
+    if with_prepass:
+        pairs = pairs + prepass_pyramid_pairs(P.params)
+    return pairs
+
+
+def _fused_scan(src: torch.Tensor, P, algorithm: str, with_prepass: bool):
+    """ops.raster_scan for the factors this job reads (the integer overview
+    and the pyramid's decimations), or None when it has none of them."""
+    H, W = src.shape
+    factors = []
+    f_ov = overview_factor(H, W) if "large" in P.extras else None
+    if f_ov in (4, 8, 16):
+        factors.append(f_ov)
+    if algorithm == "topousm_fast":
+        for ds, _ in _pyramid_pairs(P, with_prepass):
+            if ds in (4, 8, 16) and ds not in factors and len(factors) < 3:
+                factors.append(ds)
+    if not factors:
+        return None
+    cnt, grids = ops.raster_scan(src, factors)
+    by_f = dict(zip(factors, grids))
+    return {"nan": cnt, "overview": by_f.get(f_ov) if f_ov in (4, 8, 16) else None,
+            "grids": {f: g for f, g in by_f.items()}}
 
 
 EDGE_MODES = ("tile", "reflect")
@@ -263,19 +304,30 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     P = plan(raster, algorithm, nodata=nodata, **kwargs)
     H, W = raster.shape
     src = raster.clone() if copy_input else raster
-    counts = ops.nodata_to_nan(src, nodata, [(t[7], t[6], t[9], t[8]) for t in P.tiles])
+    windows = [(t[7], t[6], t[9], t[8]) for t in P.tiles]
+    stats = global_stats if global_stats is not None else P.params.get("global_stats")
+    # Without a NoData sentinel to convert, one fused read of the raster gives
+    # its NaN count, the overview and the coarse-pyramid cell grids; the
+    # per-window counts are then only needed when the raster holds NaN.
+    scan = _fused_scan(src, P, algorithm, stats is None) if nodata is None else None
+    counts = ops.nodata_to_nan(src, nodata, windows) if scan is None else None
     if "large" in P.extras:
         # queued before the counts are read back, so the read overlaps it
         lr, lw = P.extras["large"]
-        fld, scale = overview_field(src, lr, lw)
+        fld, scale = overview_field(src, lr, lw, coarse=scan["overview"] if scan else None)
         P.params["_topousm_fast_coarse_field"] = fld
         P.extras["overview_scale"] = scale
-    stats = global_stats if global_stats is not None else P.params.get("global_stats")
     bbox_handle = None
     if stats is None and algorithm in NORM_STAT_ALGOS:
         # the pre-pass overview, read back with the counts (one synchronisation)
         bbox_handle = valid_bbox_begin(src)
-    nan_counts = counts.cpu().numpy()
+    if scan is not None:
+        if int(scan["nan"].item()) == 0:
+            nan_counts = np.zeros(len(windows), dtype=np.int64)
+        else:
+            nan_counts = ops.nodata_to_nan(src, None, windows).cpu().numpy()
+    else:
+        nan_counts = counts.cpu().numpy()
     # the tile windows cover the raster: all zero means the raster holds no NaN
     raster_nan_free = bool((nan_counts == 0).all())
     spatial_tiles = (algorithm in SPATIAL_GPU and
@@ -295,12 +347,8 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
             P.params["_overview_cache"] = {}
     pyramid = None
     if algorithm == "topousm_fast" and raster_nan_free:
-        pyramid = CoarsePyramid(src)
-        pairs = CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(P.params))
-        if stats is None:
-            # the pre-pass windows' levels too: every level in one batched build
-            pairs = pairs + prepass_pyramid_pairs(P.params)
-        pyramid.require(pairs)
+        pyramid = CoarsePyramid(src, decimated=scan["grids"] if scan else None)
+        pyramid.require(_pyramid_pairs(P, stats is None))
     reflect_d = 0
     if edge_mode == "reflect" and not spatial_tiles:
         reflect_d = dask_edge_depth(algorithm, P.params)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 11
+#define FSG_ABI_VERSION 12
 
 enum {
   FSG_OK = 0,
@@ -327,8 +327,9 @@ int fsg_topousm_coarse_means(const fsg_window* win, int32_t ds, int32_t n_radii,
  * or 4x share one read of it), then every (job, radius) box pass 0 in one
  * launch and every pass 1 in one more.  Outputs equal
  * fsg_topousm_coarse_means' per job bit for bit.  The caller vouches that each
- * window holds no NaN (the enclosed-void fill is not run); ds is 4, 8 or 16
- * and every coarse radius >= 2 (FSG_E_UNSUPPORTED otherwise: those take
+ * window holds no NaN (the enclosed-void fill is not run); ds is 4, 8 or 16,
+ * or 1 when the window already is the decimated grid (box passes only), and
+ * every coarse radius >= 2 (FSG_E_UNSUPPORTED otherwise: those take
  * fsg_topousm_coarse_means).  Replaces, for the windows of the stratified
  * statistics pre-pass (algorithms/_norm_stats.py:176-350) and the raster-wide
  * grids, the per-window decimate + uniform_filter sequence of
@@ -346,6 +347,25 @@ typedef struct fsg_coarse_job {
 size_t fsg_topousm_coarse_means_batch_workspace_size(const fsg_coarse_job* jobs, int32_t n);
 int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_t n, void* workspace,
                                    size_t workspace_bytes, void* stream);
+
+/*
+ * One read of a raster for what the tile pipeline needs of it before its
+ * first tile: the number of NaN pixels (added to *nan_count, device memory,
+ * which the caller zeroes) -- the all-zero test of the per-window NoData
+ * counts (core/tile_processor.py:864-874) -- and NaN-aware block means at up
+ * to three factors 4, 8 or 16: the integer-decimation overview
+ * (core/tile_processor.py:575-590) and the decimated grids of
+ * _downsample_nan_aware (_nan_utils.py:620-643) before their void fill.
+ * outs[k].out is ceil(h/f) x ceil(w/f) with leading dimension ld_out; every
+ * cell equals fsg_block_mean's.
+ */
+typedef struct fsg_scan_out {
+  int32_t factor;
+  float* out;
+  int64_t ld_out;
+} fsg_scan_out;
+int fsg_raster_scan(const fsg_window* raster, int32_t n_out, const fsg_scan_out* outs,
+                    uint64_t* nan_count, void* stream);
 
 /*
  * NaN-aware block mean (area average for an integer factor), the restatement

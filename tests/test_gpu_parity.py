@@ -261,6 +261,37 @@ def test_topousm_coarse_means_batch_equals_per_window(cuda):
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("shape,view", [((1500, 1700), False), ((1037, 2050), True),
+                                        ((64, 100), False)])
+def test_raster_scan_equals_block_means_and_nan_count(cuda, shape, view):
+    """One read gives the NaN count and the block means at 4, 8 and 16, each
+    equal to fsg_block_mean's bit for bit (ragged edges; an unaligned view
+    takes the scalar loads); pre-decimated grids (ds 1 jobs) give the same
+    coarse means as decimating the raster."""
+    from fujishadergpu_amd import ops
+    H, W = shape
+    dem = _dem(H + 3, W + 5, 41, relief=3776.0, hurst=0.75, nan_discs=5)
+    g = dem.to(cuda)
+    src = g[1:1 + H, 3:3 + W] if view else g[:H, :W].contiguous()
+    n_nan = int(torch.isnan(src).sum())
+    assert n_nan > 0
+    for factors in ([4, 8, 16], [16], [8, 4]):
+        cnt, grids = ops.raster_scan(src, factors)
+        assert int(cnt.item()) == n_nan
+        for f, t in zip(factors, grids):
+            want = ops.block_mean(src, f)
+            assert torch.equal(torch.isnan(t), torch.isnan(want)), f
+            m = ~torch.isnan(want)
+            assert torch.equal(t[m], want[m]), f
+    clean = torch.nan_to_num(src, nan=1000.0)
+    if min(H, W) >= 1000:
+        _, (g4, g16) = ops.raster_scan(clean, [4, 16])
+        a = ops.topousm_coarse_means_batch([(g4, 1, [32]), (g16, 1, [32, 9])])
+        b = ops.topousm_coarse_means_batch([(clean, 4, [32]), (clean, 16, [32, 9])])
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)
+
+
 def test_topousm_coarse_means_batch_rejects_what_it_does_not_batch(cuda):
     from fujishadergpu_amd import _native as N, ops
     g = _dem(300, 300, 5).to(cuda)
