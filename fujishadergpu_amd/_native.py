@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 12  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 13  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -200,6 +200,8 @@ _SIGS = {
     "fsg_topousm_coarse_means_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32]),
     "fsg_topousm_coarse_means": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, P(C.c_int32),
                                            P(vp), C.c_int64, vp, C.c_size_t, vp]),
+    "fsg_topousm_field_from_means": (C.c_int, [P(vp), P(C.c_int64), P(C.c_float), C.c_int32,
+                                               C.c_int32, C.c_int32, vp, C.c_int64, vp]),
     "fsg_raster_scan": (C.c_int, [P(fsg_window), C.c_int32, P(fsg_scan_out), vp, vp]),
     "fsg_topousm_coarse_means_batch_workspace_size": (C.c_size_t, [P(fsg_coarse_job),
                                                                     C.c_int32]),

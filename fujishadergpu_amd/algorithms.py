@@ -518,7 +518,10 @@ class TopoUSMFastAlgorithm(NativeTileAlgorithm):
         return ops.topousm_params(radii, weights, ds)
 
     def run_region(self, win, reg, out, p, normalize=True):
-        prm = self.params_struct(p)
+        # a caller running many windows with the same parameters builds the
+        # struct once (params_struct of p) and passes it in
+        base = p.get("_topousm_fast_prm")
+        prm = type(base).from_buffer_copy(base) if base is not None else self.params_struct(p)
         # the scheduler knows each window's NoData count: a NaN-free window
         # may use the plain-mean kernel variant (identical results)
         if p.get("_window_nan_free", False):

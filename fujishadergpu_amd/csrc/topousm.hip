@@ -1603,6 +1603,50 @@ extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_
   return FSG_OK;
 }
 
+// field = sum_i f32(w_i) * mean_i in radius order, the first term without an
+// addition (k_field_accum's arithmetic, all terms in one pass)
+constexpr int FIELD_MAX_TERMS = 8;
+struct FieldTerms {
+  const float* m[FIELD_MAX_TERMS];
+  int64_t ld[FIELD_MAX_TERMS];
+  float wt[FIELD_MAX_TERMS];
+  int n;
+};
+
+__global__ void k_field_sum(FieldTerms T, float* __restrict__ out, int64_t ldo, int h, int w) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (c >= w || r >= h) return;
+  float acc = T.wt[0] * T.m[0][(int64_t)r * T.ld[0] + c];
+  for (int i = 1; i < T.n; i++) {
+    const float t = T.wt[i] * T.m[i][(int64_t)r * T.ld[i] + c];
+    acc = acc + t;
+  }
+  out[(int64_t)r * ldo + c] = acc;
+}
+
+extern "C" int fsg_topousm_field_from_means(const float* const* means, const int64_t* ld_means,
+                                            const float* weights, int32_t n, int32_t h,
+                                            int32_t w, float* out, int64_t ld_out,
+                                            void* stream) {
+  if (n < 1 || n > FIELD_MAX_TERMS || !means || !ld_means || !weights || !out || h <= 0 ||
+      w <= 0 || ld_out < w)
+    return fail(FSG_E_INVALID, "fsg_topousm_field_from_means: bad arguments");
+  FieldTerms T{};
+  T.n = n;
+  for (int i = 0; i < n; i++) {
+    if (!means[i] || ld_means[i] < w)
+      return fail(FSG_E_INVALID, "fsg_topousm_field_from_means: bad mean grid");
+    T.m[i] = means[i];
+    T.ld[i] = ld_means[i];
+    T.wt[i] = weights[i];
+  }
+  dim3 blk(256), grd((w + 255) / 256, h);
+  hipLaunchKernelGGL(k_field_sum, grd, blk, 0, as_stream(stream), T, out, ld_out, h, w);
+  FSG_CHECK_LAUNCH("k_field_sum");
+  return FSG_OK;
+}
+
 extern "C" size_t fsg_topousm_field_workspace_size(int32_t h, int32_t w) {
   if (h <= 0 || w <= 0) return 0;
   return 3 * align256(sizeof(float) * (size_t)h * w);

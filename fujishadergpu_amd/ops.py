@@ -409,6 +409,20 @@ def topousm_coarse_field(coarse_t: torch.Tensor, radii_coarse: Sequence[int],
     return out
 
 
+def topousm_field_from_means(means: Sequence[torch.Tensor], weights: Sequence[float]
+                             ) -> torch.Tensor:
+    """sum_i f32(w_i) * means[i] with topousm_coarse_field's arithmetic."""
+    n = len(means)
+    h, w = means[0].shape
+    out = torch.empty((h, w), dtype=torch.float32, device=means[0].device)
+    ptrs = (C.c_void_p * n)(*[m.data_ptr() for m in means])
+    lds = (C.c_int64 * n)(*[int(m.stride(0)) for m in means])
+    ww = (C.c_float * n)(*[float(np.float32(x)) for x in weights])
+    N.check(_lib().fsg_topousm_field_from_means(ptrs, lds, ww, n, h, w, out.data_ptr(), w,
+                                                _stream()))
+    return out
+
+
 # ------------------------------------------------------------------ stats
 
 PCT_ABS, PCT_FINITE = 1, 2

@@ -20,6 +20,7 @@ from typing import Any, Dict, List, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
+from . import _native as N
 from . import ops
 from .algorithms import (ALGORITHMS, SUPPORTED, CoarsePyramid, OutSpec, TopoUSMFastAlgorithm,
                          UnsupportedOnBackend, load_algorithm)
@@ -213,11 +214,13 @@ def overview_factor(H: int, W: int) -> Optional[int]:
 
 
 def overview_field(raster: torch.Tensor, large_radii, large_weights,
-                   coarse: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, float]:
+                   coarse: Optional[torch.Tensor] = None,
+                   nan_free: bool = False) -> Tuple[torch.Tensor, float]:
     """Large-radius coarse field from the overview (core/tile_processor.py:552-602,
     _impl_topousm_fast.py:133-155).  The overview read is GDAL average
     resampling; for an integer decimation that is the NaN-aware block mean
-    (``coarse``, when the caller already has it)."""
+    (``coarse``, when the caller already has it).  ``nan_free``: the raster
+    holds no NaN (so neither does the overview)."""
     H, W = raster.shape
     scale, sh, sw = overview_geometry(W, H)
     f = int(round(scale))
@@ -228,6 +231,12 @@ def overview_field(raster: torch.Tensor, large_radii, large_weights,
     if coarse is None:
         coarse = ops.block_mean(raster, f) if f > 1 else raster.contiguous().clone()
     rc = [max(1, int(round(float(r) / max(scale, 1.0)))) for r in large_radii]
+    if nan_free and min(rc) >= 2 and len(rc) <= N.COARSE_JOB_MAX_RADII:
+        # no NaN in the overview: the box means of every radius in one batched
+        # pair of launches, then the weighted sum in one pass (same values)
+        means = ops.topousm_coarse_means_batch([(coarse, 1, rc)])[0]
+        return ops.topousm_field_from_means([means[i] for i in range(len(rc))],
+                                            large_weights), scale
     return ops.topousm_coarse_field(coarse, rc, large_weights), scale
 
 
@@ -311,10 +320,10 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     # per-window counts are then only needed when the raster holds NaN.
     scan = _fused_scan(src, P, algorithm, stats is None) if nodata is None else None
     counts = ops.nodata_to_nan(src, nodata, windows) if scan is None else None
-    if "large" in P.extras:
+    if "large" in P.extras and scan is None:
         # queued before the counts are read back, so the read overlaps it
         lr, lw = P.extras["large"]
-        fld, scale = overview_field(src, lr, lw, coarse=scan["overview"] if scan else None)
+        fld, scale = overview_field(src, lr, lw)
         P.params["_topousm_fast_coarse_field"] = fld
         P.extras["overview_scale"] = scale
     bbox_handle = None
@@ -330,6 +339,14 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
         nan_counts = counts.cpu().numpy()
     # the tile windows cover the raster: all zero means the raster holds no NaN
     raster_nan_free = bool((nan_counts == 0).all())
+    if "large" in P.extras and scan is not None:
+        # queued first after the synchronisation: it runs while the host
+        # prepares the rest (only the main tiles read it)
+        lr, lw = P.extras["large"]
+        fld, scale = overview_field(src, lr, lw, coarse=scan["overview"],
+                                    nan_free=raster_nan_free)
+        P.params["_topousm_fast_coarse_field"] = fld
+        P.extras["overview_scale"] = scale
     spatial_tiles = (algorithm in SPATIAL_GPU and
                      str(P.params.get("mode", "local")).lower() == "spatial")
     if spatial_tiles and P.params.get("radii"):

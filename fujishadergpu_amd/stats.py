@@ -103,15 +103,15 @@ def prepass_pyramid_pairs(params: dict):
     return CoarsePyramid.pairs(TopoUSMFastAlgorithm().params_struct(kw))
 
 
-def _own_coarse_means(raster, algo, bp, regions, pyramid):
+def _own_coarse_means(raster, base, regions, pyramid):
     """Coarse means of the pre-pass windows that the raster-wide grids do not
     cover, built for all windows in a few batched launches: per region index,
-    {radius index: grid}."""
+    {radius index: grid}.  ``base`` is the windows' parameter struct."""
     jobs, keys = [], []
     for ri, (wy, wx, th, tw, m, nan_free) in enumerate(regions):
         if not nan_free:
             continue
-        prm = algo.params_struct(bp)
+        prm = type(base).from_buffer_copy(base)
         pyramid.attach(prm, (wy, wx), (th, tw), ops.region(m, m, th - 2 * m, tw - 2 * m))
         by_ds = {}
         for i in range(prm.n_radii):
@@ -171,9 +171,10 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
             "ambient_occlusion": AmbientOcclusionAlgorithm}[algorithm]()
     bp = dict(kw)
     own = {}
+    base = algo.params_struct(bp) if algorithm == "topousm_fast" else None
     if pyramid is not None and algorithm == "topousm_fast":
-        pyramid.require(pyramid.pairs(algo.params_struct(bp)))
-        own = _own_coarse_means(raster, algo, bp, regions, pyramid)
+        pyramid.require(pyramid.pairs(base))
+        own = _own_coarse_means(raster, base, regions, pyramid)
     off = 0
     for ri, (wy, wx, th, tw, m, nan_free) in enumerate(regions):
         win = raster[wy:wy + th, wx:wx + tw]
@@ -181,7 +182,7 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
         dst = pool[off:off + rh * rw].view(rh, rw)
         reg = ops.region(m, m, rh, rw)
         if algorithm == "topousm_fast":
-            wp = {**bp, "_window_nan_free": nan_free}
+            wp = {**bp, "_window_nan_free": nan_free, "_topousm_fast_prm": base}
             if pyramid is not None:
                 wp["_topousm_fast_pyramid"] = pyramid
                 wp["_window_origin"] = (wy, wx)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 12
+#define FSG_ABI_VERSION 13
 
 enum {
   FSG_OK = 0,
@@ -347,6 +347,17 @@ typedef struct fsg_coarse_job {
 size_t fsg_topousm_coarse_means_batch_workspace_size(const fsg_coarse_job* jobs, int32_t n);
 int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_t n, void* workspace,
                                    size_t workspace_bytes, void* stream);
+
+/*
+ * The large-radius field of fsg_topousm_coarse_field from box means already
+ * computed (e.g. by fsg_topousm_coarse_means_batch over a NaN-free overview,
+ * ds = 1): out = sum_i f32(weights[i]) * means[i] in radius order, with
+ * fsg_topousm_coarse_field's arithmetic (_impl_topousm_fast.py:133-155).
+ * n <= 8 grids of h x w.
+ */
+int fsg_topousm_field_from_means(const float* const* means, const int64_t* ld_means,
+                                 const float* weights, int32_t n, int32_t h, int32_t w,
+                                 float* out, int64_t ld_out, void* stream);
 
 /*
  * One read of a raster for what the tile pipeline needs of it before its

@@ -292,6 +292,18 @@ def test_raster_scan_equals_block_means_and_nan_count(cuda, shape, view):
             assert torch.equal(x, y)
 
 
+def test_field_from_batched_means_equals_coarse_field(cuda):
+    """The NaN-free overview field (batched box means, one weighted-sum pass)
+    equals fsg_topousm_coarse_field bit for bit."""
+    from fujishadergpu_amd import ops
+    ov = _dem(700, 900, 43, relief=3776.0, hurst=0.75).to(cuda)
+    rc, wts = [64, 256, 9], [0.0317, 0.0159, 0.21]
+    want = ops.topousm_coarse_field(ov, rc, wts)
+    means = ops.topousm_coarse_means_batch([(ov, 1, rc)])[0]
+    got = ops.topousm_field_from_means([means[i] for i in range(3)], wts)
+    assert torch.equal(got, want)
+
+
 def test_topousm_coarse_means_batch_rejects_what_it_does_not_batch(cuda):
     from fujishadergpu_amd import _native as N, ops
     g = _dem(300, 300, 5).to(cuda)
