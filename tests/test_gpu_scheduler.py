@@ -196,3 +196,34 @@ def test_reflect_edge_mode_rejects_topousm(cuda):
     with pytest.raises(UnsupportedOnBackend):
         process_dem(host_t.to(cuda), "topousm_fast", tile_size=128, pixel_size=10.0,
                     edge_mode="reflect", radii=[4, 16], weights=[0.5, 0.5])
+
+
+@pytest.mark.parametrize("nan_discs", [0, 30])
+def test_fused_scan_path_equals_the_count_path(cuda, nan_discs):
+    """Without a NoData sentinel to convert, the scheduler reads the raster once
+    (NaN count, factor-4 overview, pyramid cells) and builds the overview
+    field after the synchronisation; with a sentinel it converts and counts per
+    window first and builds the field the NaN-aware way.  An 8192^2 raster run
+    both ways (the sentinel absent from the data) gives the same statistics and
+    outputs bit for bit: NaN-free (pyramid, batched pre-pass means) and with
+    NaN (per-window counts, no pyramid)."""
+    from fujishadergpu_amd.scheduler import process_dem
+    H = W = 8192
+    host_t = _raster(H, W, 53, relief=3776.0, hurst=0.75)
+    if nan_discs:
+        from fujishadergpu_amd import synth
+        synth.apply_discs(host_t, synth.nodata_discs(H, W, 0.01, seed=54)[:nan_discs],
+                          float("nan"))
+    g = host_t.to(cuda)
+    radii = [2, 8, 32, 128, 512, 2048]
+    weights = [2.0 ** (5 - i) / 63.0 for i in range(6)]
+    kw = dict(tile_size=2048, pixel_size=5.0, mode="spatial", radii=radii, weights=weights,
+              output_dtype="float32", copy_input=True, return_plan=True)
+    a, ra, Pa = process_dem(g, "topousm_fast", nodata=None, **kw)
+    b, rb, Pb = process_dem(g, "topousm_fast", nodata=-1.0e30, **kw)
+    assert Pa.extras["overview_scale"] == Pb.extras["overview_scale"] == 4.0
+    assert torch.equal(Pa.stats, Pb.stats)
+    na, nb = torch.isnan(a), torch.isnan(b)
+    assert torch.equal(na, nb)
+    assert bool((na.any()) == bool(nan_discs))
+    assert torch.equal(a[~na], b[~nb])
