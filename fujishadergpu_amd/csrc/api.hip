@@ -316,67 +316,69 @@ template <int F>
 __global__ void __launch_bounds__(256) k_raster_scan(ScanArgs A) {
   constexpr int G = F / 4;  // 4-row groups per band
   const int c = (blockIdx.x * 256 + threadIdx.x) * 4;
-  const int rb = blockIdx.y * F;
-  double sg[G];
-  int cg[G];
   unsigned long long nn = 0;
-#pragma unroll
-  for (int g = 0; g < G; g++) {
-    sg[g] = 0.0;
-    cg[g] = 0;
-  }
-  if (c < A.w) {
+  // bands of F rows, grid-stride over gridDim.y (block-uniform)
+  for (int rb = blockIdx.y * F; rb < A.h; rb += gridDim.y * F) {
+    double sg[G];
+    int cg[G];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      float v[4][4];
+      sg[g] = 0.0;
+      cg[g] = 0;
+    }
+    if (c < A.w) {
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int r = min(rb + 4 * g + e, A.h - 1);
-        const float* p = A.x + (int64_t)r * A.ld + c;
-        if (A.vec && c + 3 < A.w) {
-          const float4 q = *(const float4*)p;
-          v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
-        } else {
+      for (int g = 0; g < G; g++) {
+        float v[4][4];
 #pragma unroll
-          for (int k = 0; k < 4; k++) v[e][k] = c + k < A.w ? p[k] : 0.0f;
+        for (int e = 0; e < 4; e++) {
+          const int r = min(rb + 4 * g + e, A.h - 1);
+          const float* p = A.x + (int64_t)r * A.ld + c;
+          if (A.vec && c + 3 < A.w) {
+            const float4 q = *(const float4*)p;
+            v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[e][k] = c + k < A.w ? p[k] : 0.0f;
+          }
         }
-      }
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        if (rb + 4 * g + e >= A.h) break;
+        for (int e = 0; e < 4; e++) {
+          if (rb + 4 * g + e >= A.h) break;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          if (c + k >= A.w) break;
-          const float x = v[e][k];
-          if (isnan(x)) nn++;
-          if (isfinite(x)) {
-            sg[g] += (double)x;
-            cg[g]++;
+          for (int k = 0; k < 4; k++) {
+            if (c + k >= A.w) break;
+            const float x = v[e][k];
+            if (isnan(x)) nn++;
+            if (isfinite(x)) {
+              sg[g] += (double)x;
+              cg[g]++;
+            }
           }
         }
       }
     }
-  }
-  for (int k = 0; k < A.n_out; k++) {
-    const ScanOut& O = A.o[k];
-    const int gpc = O.f / 4;  // groups (and lanes) per cell
-    for (int q = 0; q < G / gpc; q++) {
-      double tot = 0.0;
-      int cnt = 0;
+    for (int k = 0; k < A.n_out; k++) {
+      const ScanOut& O = A.o[k];
+      const int gpc = O.f / 4;  // groups (and lanes) per cell
+      for (int q = 0; q < G / gpc; q++) {
+        double tot = 0.0;
+        int cnt = 0;
 #pragma unroll
-      for (int g = 0; g < G; g++)
-        if (g / gpc == q) {
-          tot += sg[g];
-          cnt += cg[g];
+        for (int g = 0; g < G; g++)
+          if (g / gpc == q) {
+            tot += sg[g];
+            cnt += cg[g];
+          }
+        for (int off = 1; off < gpc; off <<= 1) {
+          tot += __shfl_down(tot, off, 64);
+          cnt += __shfl_down(cnt, off, 64);
         }
-      for (int off = 1; off < gpc; off <<= 1) {
-        tot += __shfl_down(tot, off, 64);
-        cnt += __shfl_down(cnt, off, 64);
+        const int orow = (rb + q * O.f) / O.f;
+        const int oc = c / O.f;
+        if ((threadIdx.x & (gpc - 1)) == 0 && oc < O.ow && orow < O.oh && c < A.w)
+          O.p[(int64_t)orow * O.ld + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
       }
-      const int orow = (rb + q * O.f) / O.f;
-      const int oc = c / O.f;
-      if ((threadIdx.x & (gpc - 1)) == 0 && oc < O.ow && orow < O.oh && c < A.w)
-        O.p[(int64_t)orow * O.ld + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
     }
   }
   for (int off = 32; off > 0; off >>= 1) nn += __shfl_down(nn, off, 64);
@@ -520,7 +522,7 @@ extern "C" int fsg_raster_scan(const fsg_window* raster, int32_t n_out,
     F = std::max(F, f);
   }
   hipStream_t s = as_stream(stream);
-  dim3 blk(256), grd((raster->w + 1023) / 1024, (raster->h + F - 1) / F);
+  dim3 blk(256), grd((raster->w + 1023) / 1024, std::min((raster->h + F - 1) / F, 32768));
   if (F == 4)
     hipLaunchKernelGGL(k_raster_scan<4>, grd, blk, 0, s, A);
   else if (F == 8)

@@ -262,7 +262,7 @@ def test_topousm_coarse_means_batch_equals_per_window(cuda):
 
 
 @pytest.mark.parametrize("shape,view", [((1500, 1700), False), ((1037, 2050), True),
-                                        ((64, 100), False)])
+                                        ((64, 100), False), ((140001, 37), False)])
 def test_raster_scan_equals_block_means_and_nan_count(cuda, shape, view):
     """One read gives the NaN count and the block means at 4, 8 and 16, each
     equal to fsg_block_mean's bit for bit (ragged edges; an unaligned view
