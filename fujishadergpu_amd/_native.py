@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 13  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 14  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -182,6 +182,8 @@ _SIGS = {
     "fsg_launch_count": (C.c_int64, []),
     "fsg_timing_enable": (C.c_int, [C.c_int32]),
     "fsg_timing_read": (C.c_int, [C.c_char_p, P(C.c_double), P(C.c_int64)]),
+    "fsg_timing_read_union": (C.c_int, [C.c_char_p, P(C.c_double), P(C.c_int64),
+                                        P(C.c_double)]),
     "fsg_nodata_to_nan": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                     P(C.c_int32), C.c_int32, vp, vp]),
     "fsg_hillshade_slope": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_shade_params),

@@ -398,8 +398,11 @@ extern "C" int fsg_timing_enable(int32_t on) {
   return FSG_OK;
 }
 
-extern "C" int fsg_timing_read(const char* name, double* total_ms, int64_t* count) {
-  if (!name || !total_ms || !count) return fail(FSG_E_INVALID, "fsg_timing_read: NULL argument");
+// Sum of the recorded launch durations of `name`, their count and, when
+// union_ms is given, the length of the union of their [start, end] intervals
+// (launches on several streams may overlap; times relative to the first start).
+static int timing_collect(const char* name, double* total_ms, int64_t* count,
+                          double* union_ms) {
   std::vector<EvPair> evs;
   {
     std::lock_guard<std::mutex> lk(g_tmu);
@@ -410,18 +413,55 @@ extern "C" int fsg_timing_read(const char* name, double* total_ms, int64_t* coun
     }
   }
   double tot = 0.0;
+  std::vector<std::pair<double, double>> iv;
+  hipError_t err = hipSuccess;
   for (auto& p : evs) {
-    float ms = 0.f;
+    float ms = 0.f, t0 = 0.f, t1 = 0.f;
     hipError_t e = hipEventSynchronize(p.b);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
-    hipEventDestroy(p.a);
-    hipEventDestroy(p.b);
-    if (e != hipSuccess) return check_hip(e, "fsg_timing_read");
+    if (e == hipSuccess && union_ms) {
+      e = hipEventElapsedTime(&t0, evs[0].a, p.a);
+      if (e == hipSuccess) e = hipEventElapsedTime(&t1, evs[0].a, p.b);
+      iv.emplace_back((double)t0, (double)t1);
+    }
+    if (e != hipSuccess && err == hipSuccess) err = e;
     tot += ms;
   }
+  for (auto& p : evs) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  if (err != hipSuccess) return check_hip(err, "fsg_timing_read");
   *total_ms = tot;
   *count = (int64_t)evs.size();
+  if (union_ms) {
+    std::sort(iv.begin(), iv.end());
+    double u = 0.0, a = 0.0, b = -1.0;
+    for (auto& x : iv) {
+      if (x.first > b) {
+        if (b > a) u += b - a;
+        a = x.first;
+        b = x.second;
+      } else if (x.second > b) {
+        b = x.second;
+      }
+    }
+    if (b > a) u += b - a;
+    *union_ms = u;
+  }
   return FSG_OK;
+}
+
+extern "C" int fsg_timing_read(const char* name, double* total_ms, int64_t* count) {
+  if (!name || !total_ms || !count) return fail(FSG_E_INVALID, "fsg_timing_read: NULL argument");
+  return timing_collect(name, total_ms, count, nullptr);
+}
+
+extern "C" int fsg_timing_read_union(const char* name, double* total_ms, int64_t* count,
+                                     double* union_ms) {
+  if (!name || !total_ms || !count || !union_ms)
+    return fail(FSG_E_INVALID, "fsg_timing_read_union: NULL argument");
+  return timing_collect(name, total_ms, count, union_ms);
 }
 
 extern "C" int fsg_nodata_to_nan(float* raster, int64_t ld, int32_t h, int32_t w,

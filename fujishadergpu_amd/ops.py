@@ -292,6 +292,42 @@ def topousm_params(radii: Sequence[int], weights: Sequence[float], ds: Sequence[
     return p
 
 
+_SIDE_STREAMS = {}
+
+
+class StreamRing:
+    """Independent launches spread round-robin over the caller's current stream
+    and n - 1 side streams (default priority, one set per device, reused), so
+    that one launch's last workgroups overlap the next launch's first.  The
+    side streams start after everything already queued on the caller's
+    stream; join() makes the caller's stream wait for all of them.  Work on
+    stream k must use resources of its own (workspaces) or read-only ones."""
+
+    def __init__(self, n: int = 2):
+        cur = torch.cuda.current_stream()
+        side = _SIDE_STREAMS.setdefault(cur.device, [])
+        while len(side) < n - 1:
+            side.append(torch.cuda.Stream(device=cur.device))
+        self.streams = [cur] + side[:n - 1]
+        ev = torch.cuda.Event()
+        ev.record(cur)
+        for st in self.streams[1:]:
+            st.wait_event(ev)
+
+    def __len__(self):
+        return len(self.streams)
+
+    def stream(self, k: int):
+        return self.streams[k % len(self.streams)]
+
+    def join(self) -> None:
+        cur = self.streams[0]
+        for st in self.streams[1:]:
+            ev = torch.cuda.Event()
+            ev.record(st)
+            cur.wait_event(ev)
+
+
 class Workspace:
     """Grow-only device scratch buffer (one per caller/stream)."""
 

@@ -249,18 +249,18 @@ def _pyramid_pairs(P, with_prepass: bool):
     return pairs
 
 
-def _fused_scan(src: torch.Tensor, P, algorithm: str, with_prepass: bool):
+def _fused_scan(src: torch.Tensor, P, pairs):
     """ops.raster_scan for the factors this job reads (the integer overview
-    and the pyramid's decimations), or None when it has none of them."""
+    and the decimations of the pyramid levels ``pairs``), or None when it has
+    none of them."""
     H, W = src.shape
     factors = []
     f_ov = overview_factor(H, W) if "large" in P.extras else None
     if f_ov in (4, 8, 16):
         factors.append(f_ov)
-    if algorithm == "topousm_fast":
-        for ds, _ in _pyramid_pairs(P, with_prepass):
-            if ds in (4, 8, 16) and ds not in factors and len(factors) < 3:
-                factors.append(ds)
+    for ds, _ in pairs:
+        if ds in (4, 8, 16) and ds not in factors and len(factors) < 3:
+            factors.append(ds)
     if not factors:
         return None
     cnt, grids = ops.raster_scan(src, factors)
@@ -318,7 +318,8 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     # Without a NoData sentinel to convert, one fused read of the raster gives
     # its NaN count, the overview and the coarse-pyramid cell grids; the
     # per-window counts are then only needed when the raster holds NaN.
-    scan = _fused_scan(src, P, algorithm, stats is None) if nodata is None else None
+    pairs = _pyramid_pairs(P, stats is None) if algorithm == "topousm_fast" else []
+    scan = _fused_scan(src, P, pairs) if nodata is None else None
     counts = ops.nodata_to_nan(src, nodata, windows) if scan is None else None
     if "large" in P.extras and scan is None:
         # queued before the counts are read back, so the read overlaps it
@@ -365,7 +366,7 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     pyramid = None
     if algorithm == "topousm_fast" and raster_nan_free:
         pyramid = CoarsePyramid(src, decimated=scan["grids"] if scan else None)
-        pyramid.require(_pyramid_pairs(P, stats is None))
+        pyramid.require(pairs)
     reflect_d = 0
     if edge_mode == "reflect" and not spatial_tiles:
         reflect_d = dask_edge_depth(algorithm, P.params)
@@ -381,10 +382,15 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     dt = {"float32": torch.float32, "int16": torch.int16, "uint8": torch.uint8}[P.output_dtype]
     if out is None:
         out = torch.empty((H, W), dtype=dt, device=raster.device)
-    algo = load_algorithm(algorithm)
+    # topousm_fast tiles alternate between two streams (independent windows,
+    # one algorithm object, hence workspace, per stream): each tile's last
+    # workgroups overlap the next tile's first
+    ring = ops.StreamRing(2) if algorithm == "topousm_fast" and len(P.tiles) > 1 else None
+    algos = [load_algorithm(algorithm) for _ in range(len(ring) if ring else 1)]
     results: List[TileResult] = []
     transform = P.extras.get("transform")
-    for t, nn in zip(P.tiles, nan_counts):
+    for k_tile, (t, nn) in enumerate(zip(P.tiles, nan_counts)):
+        algo = algos[k_tile % len(algos)]
         ty, tx, cx, cy, cw, ch, wx, wy, ww, wh = t
         ratio = float(nn) / float(ww * wh) if ww * wh else 1.0
         o_view = out[cy:cy + ch, cx:cx + cw]
@@ -428,13 +434,21 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
                        q=o_view if dt != torch.float32 else None, qp=P.qp,
                        clip01=(algorithm == "hillshade"))
         try:
-            algo.run_region(win, reg, spec, p)
+            if ring is not None:
+                with torch.cuda.stream(ring.stream(k_tile)):
+                    algo.run_region(win, reg, spec, p)
+            else:
+                algo.run_region(win, reg, spec, p)
             results.append(TileResult(ty, tx, True))
         except UnsupportedOnBackend:
+            if ring is not None:
+                ring.join()
             raise
         except Exception as exc:  # per-tile failure record (core/tile_processor.py:1022-1024)
             logger.error("Tile (%d, %d) processing failed: %s", ty, tx, exc)
             results.append(TileResult(ty, tx, False, error_message=str(exc)))
+    if ring is not None:
+        ring.join()
     errors = [r for r in results if not r.success and not r.skipped_reason]
     if errors:
         detail = "\n".join(f"Tile ({r.tile_y}, {r.tile_x}): {r.error_message}" for r in errors[:3])

@@ -175,23 +175,36 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
     if pyramid is not None and algorithm == "topousm_fast":
         pyramid.require(pyramid.pairs(base))
         own = _own_coarse_means(raster, base, regions, pyramid)
-    off = 0
-    for ri, (wy, wx, th, tw, m, nan_free) in enumerate(regions):
-        win = raster[wy:wy + th, wx:wx + tw]
-        rh, rw = th - 2 * m, tw - 2 * m
-        dst = pool[off:off + rh * rw].view(rh, rw)
-        reg = ops.region(m, m, rh, rw)
-        if algorithm == "topousm_fast":
-            wp = {**bp, "_window_nan_free": nan_free, "_topousm_fast_prm": base}
-            if pyramid is not None:
-                wp["_topousm_fast_pyramid"] = pyramid
-                wp["_window_origin"] = (wy, wx)
-            if ri in own:
-                wp["_topousm_fast_means"] = own[ri]
-            algo.run_region(win, reg, OutSpec(f32=dst), wp, normalize=False)
-        else:
-            algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "mode": "local"}, stretch=False)
-        off += rh * rw
+    # topousm_fast windows alternate between two streams (one algorithm object,
+    # hence workspace, per stream; disjoint pool slices)
+    ring = ops.StreamRing(2) if algorithm == "topousm_fast" and len(regions) > 1 else None
+    algos = [algo] + [type(algo)() for _ in range(len(ring) - 1)] if ring else [algo]
+    try:
+        off = 0
+        for ri, (wy, wx, th, tw, m, nan_free) in enumerate(regions):
+            win = raster[wy:wy + th, wx:wx + tw]
+            rh, rw = th - 2 * m, tw - 2 * m
+            dst = pool[off:off + rh * rw].view(rh, rw)
+            reg = ops.region(m, m, rh, rw)
+            if algorithm == "topousm_fast":
+                wp = {**bp, "_window_nan_free": nan_free, "_topousm_fast_prm": base}
+                if pyramid is not None:
+                    wp["_topousm_fast_pyramid"] = pyramid
+                    wp["_window_origin"] = (wy, wx)
+                if ri in own:
+                    wp["_topousm_fast_means"] = own[ri]
+                if ring is not None:
+                    with torch.cuda.stream(ring.stream(ri)):
+                        algos[ri % len(algos)].run_region(win, reg, OutSpec(f32=dst), wp,
+                                                          normalize=False)
+                else:
+                    algo.run_region(win, reg, OutSpec(f32=dst), wp, normalize=False)
+            else:
+                algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "mode": "local"}, stretch=False)
+            off += rh * rw
+    finally:
+        if ring is not None:
+            ring.join()
     if algorithm == "topousm_fast":
         pct, nval, mom = ops.percentile(pool, [99.0], ops.PCT_ABS)
         stats = ops.finalize_stats(0, pct, mom, nval)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 13
+#define FSG_ABI_VERSION 14
 
 enum {
   FSG_OK = 0,
@@ -104,6 +104,11 @@ int64_t fsg_launch_count(void);
  */
 int fsg_timing_enable(int32_t on);
 int fsg_timing_read(const char* name, double* total_ms, int64_t* count);
+/* The same, plus the length of the union of the launches' [start, end]
+   intervals: launches on several streams may run at the same time, and then
+   the union, not the sum, is the time the GPU spent on them. */
+int fsg_timing_read_union(const char* name, double* total_ms, int64_t* count,
+                          double* union_ms);
 
 /* ------------------------------------------------------------------ */
 /* Input preparation                                                   */
