@@ -382,10 +382,11 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     dt = {"float32": torch.float32, "int16": torch.int16, "uint8": torch.uint8}[P.output_dtype]
     if out is None:
         out = torch.empty((H, W), dtype=dt, device=raster.device)
-    # topousm_fast tiles alternate between two streams (independent windows,
-    # one algorithm object, hence workspace, per stream): each tile's last
-    # workgroups overlap the next tile's first
-    ring = ops.StreamRing(2) if algorithm == "topousm_fast" and len(P.tiles) > 1 else None
+    # tiles alternate between two streams (independent windows, one algorithm
+    # object, hence workspace, per stream): each tile's last workgroups overlap
+    # the next tile's first.  Spatial-mode tiles fill a shared overview cache
+    # as they go, so they stay on one stream.
+    ring = ops.StreamRing(2) if not spatial_tiles and len(P.tiles) > 1 else None
     algos = [load_algorithm(algorithm) for _ in range(len(ring) if ring else 1)]
     results: List[TileResult] = []
     transform = P.extras.get("transform")

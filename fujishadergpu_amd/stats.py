@@ -175,9 +175,9 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
     if pyramid is not None and algorithm == "topousm_fast":
         pyramid.require(pyramid.pairs(base))
         own = _own_coarse_means(raster, base, regions, pyramid)
-    # topousm_fast windows alternate between two streams (one algorithm object,
-    # hence workspace, per stream; disjoint pool slices)
-    ring = ops.StreamRing(2) if algorithm == "topousm_fast" and len(regions) > 1 else None
+    # the windows alternate between two streams (one algorithm object, hence
+    # workspace, per stream; disjoint pool slices)
+    ring = ops.StreamRing(2) if len(regions) > 1 else None
     algos = [algo] + [type(algo)() for _ in range(len(ring) - 1)] if ring else [algo]
     try:
         off = 0
@@ -199,6 +199,10 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
                                                           normalize=False)
                 else:
                     algo.run_region(win, reg, OutSpec(f32=dst), wp, normalize=False)
+            elif ring is not None:
+                with torch.cuda.stream(ring.stream(ri)):
+                    algos[ri % len(algos)].run_region(win, reg, OutSpec(f32=dst),
+                                                      {**bp, "mode": "local"}, stretch=False)
             else:
                 algo.run_region(win, reg, OutSpec(f32=dst), {**bp, "mode": "local"}, stretch=False)
             off += rh * rw
