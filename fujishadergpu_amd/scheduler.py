@@ -384,9 +384,9 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
         out = torch.empty((H, W), dtype=dt, device=raster.device)
     # tiles alternate between two streams (independent windows, one algorithm
     # object, hence workspace, per stream): each tile's last workgroups overlap
-    # the next tile's first.  Spatial-mode tiles fill a shared overview cache
-    # as they go, so they stay on one stream.
-    ring = ops.StreamRing(2) if not spatial_tiles and len(P.tiles) > 1 else None
+    # the next tile's first.  Spatial-mode tiles share the overview responses
+    # they build as they go; each entry carries an event the readers wait on.
+    ring = ops.StreamRing(2) if len(P.tiles) > 1 else None
     algos = [load_algorithm(algorithm) for _ in range(len(ring) if ring else 1)]
     results: List[TileResult] = []
     transform = P.extras.get("transform")

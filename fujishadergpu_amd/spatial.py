@@ -322,10 +322,16 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
                 cache: Dict = p.setdefault("_overview_cache", {})
                 key = (algorithm, int(round(r)), id(overview))
                 if key not in cache:
-                    cache[key] = _coarse_response(algorithm, overview,
-                                                  max(1, int(round(r / fac))), _scaled(base, fac))
+                    resp = _coarse_response(algorithm, overview,
+                                            max(1, int(round(r / fac))), _scaled(base, fac))
+                    ready = torch.cuda.Event()
+                    ready.record()
+                    cache[key] = (resp, ready)
+                resp, ready = cache[key]
+                # tiles may run on another stream than the one that built it
+                torch.cuda.current_stream().wait_event(ready)
                 fh, fw = p.get("_tile_full_shape") or (h, w)
-                _sample(cache[key], int(origin[0]) + reg.r0, int(origin[1]) + reg.c0, int(fh),
+                _sample(resp, int(origin[0]) + reg.r0, int(origin[1]) + reg.c0, int(fh),
                         int(fw), wsub, field)
             else:
                 # da.coarsen(nanmean, trim_excess) of the window, the block
