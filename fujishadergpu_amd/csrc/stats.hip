@@ -99,41 +99,22 @@ __device__ __forceinline__ void pct_visit(const float* __restrict__ x, int64_t n
   for (int64_t i = head + tid; i < n; i += S) f(x[i]);
 }
 
-// hist[bin] += 1 for every lane with `on`.  Lanes of one wave that share the
-// bin of the lowest remaining lane add together (one atomic for the group),
-// twice; the rest add one by one.  Level-1 bins are few and crowded (exponent
-// and two mantissa bits), so most of a wave lands in one or two of them.
-__device__ __forceinline__ void hist_add_grouped(unsigned int* hist, unsigned int bin, bool on) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long act = __ballot(on);
-#pragma unroll
-  for (int round = 0; round < 2; round++) {
-    if (act == 0) return;
-    const int lead = __ffsll((long long)act) - 1;
-    const unsigned int b = __shfl(bin, lead, 64);
-    const unsigned long long grp = __ballot(on && bin == b) & act;
-    if (lane == lead) atomicAdd(&hist[b], (unsigned int)__popcll(grp));
-    if ((grp >> lane) & 1ull) on = false;
-    act &= ~grp;
-  }
-  if (on) atomicAdd(&hist[bin], 1u);
-}
-
 // Level-1 histogram (bits 31..21), valid count and FP64 moments.
 __global__ void __launch_bounds__(256)
 k_pct_hist1(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict__ ws) {
-  __shared__ unsigned int hist[NB1];
+  // one sub-histogram per wave: the level-1 bins (exponent and two mantissa
+  // bits) are few and crowded, and waves then never contend for one address
+  __shared__ unsigned int hist[4][NB1];
   __shared__ double red[2][4];
-  for (int b = threadIdx.x; b < NB1; b += 256) hist[b] = 0;
+  for (int b = threadIdx.x; b < 4 * NB1; b += 256) hist[b / NB1][b % NB1] = 0;
   __syncthreads();
+  unsigned int* mine = hist[threadIdx.x >> 6];
   double s = 0.0, s2 = 0.0;
   pct_visit(x, n, [&](float v) {
-    const bool ok = pct_valid(v, flags);
-    hist_add_grouped(hist, ok ? pct_key(v, flags) >> 21 : 0u, ok);
-    if (ok) {
-      s += (double)v;
-      s2 += (double)v * (double)v;
-    }
+    if (!pct_valid(v, flags)) return;
+    atomicAdd(&mine[pct_key(v, flags) >> 21], 1u);
+    s += (double)v;
+    s2 += (double)v * (double)v;
   });
   for (int off = 32; off > 0; off >>= 1) {
     s += __shfl_down(s, off, 64);
@@ -148,8 +129,10 @@ k_pct_hist1(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict
     atomicAdd(&ws->st.sum, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
     atomicAdd(&ws->st.sumsq, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
   }
-  for (int b = threadIdx.x; b < NB1; b += 256)
-    if (hist[b]) atomicAdd(&ws->h1[b], (unsigned long long)hist[b]);
+  for (int b = threadIdx.x; b < NB1; b += 256) {
+    const unsigned int t = hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b];
+    if (t) atomicAdd(&ws->h1[b], (unsigned long long)t);
+  }
 }
 
 // Level-2/3 histograms restricted to each target's current prefix (targets
