@@ -327,32 +327,32 @@ __global__ void __launch_bounds__(256) k_raster_scan(ScanArgs A) {
       cg[g] = 0;
     }
     if (c < A.w) {
+      // every load of the band first (F float4 per lane in flight), then the sums
+      float v[F][4];
 #pragma unroll
-      for (int g = 0; g < G; g++) {
-        float v[4][4];
+      for (int e = 0; e < F; e++) {
+        const int r = min(rb + e, A.h - 1);
+        const float* p = A.x + (int64_t)r * A.ld + c;
+        if (A.vec && c + 3 < A.w) {
+          const float4 q = *(const float4*)p;
+          v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
+        } else {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const int r = min(rb + 4 * g + e, A.h - 1);
-          const float* p = A.x + (int64_t)r * A.ld + c;
-          if (A.vec && c + 3 < A.w) {
-            const float4 q = *(const float4*)p;
-            v[e][0] = q.x; v[e][1] = q.y; v[e][2] = q.z; v[e][3] = q.w;
-          } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++) v[e][k] = c + k < A.w ? p[k] : 0.0f;
-          }
+          for (int k = 0; k < 4; k++) v[e][k] = c + k < A.w ? p[k] : 0.0f;
         }
+      }
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          if (rb + 4 * g + e >= A.h) break;
+      for (int e = 0; e < F; e++) {
+        if (rb + e < A.h) {
 #pragma unroll
           for (int k = 0; k < 4; k++) {
-            if (c + k >= A.w) break;
-            const float x = v[e][k];
-            if (isnan(x)) nn++;
-            if (isfinite(x)) {
-              sg[g] += (double)x;
-              cg[g]++;
+            if (c + k < A.w) {
+              const float x = v[e][k];
+              if (isnan(x)) nn++;
+              if (isfinite(x)) {
+                sg[e / 4] += (double)x;
+                cg[e / 4]++;
+              }
             }
           }
         }
