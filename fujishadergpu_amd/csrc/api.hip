@@ -573,6 +573,64 @@ extern "C" int fsg_raster_scan(const fsg_window* raster, int32_t n_out,
   return FSG_OK;
 }
 
+// Area-weighted NaN-aware average of a source raster onto an oh x ow grid
+// (GDAL's Resampling.average for any ratio).  Output cell (i, j) covers source
+// rows [i*ry, (i+1)*ry) and columns [j*rx, (j+1)*rx), ry = h/oh, rx = w/ow;
+// each source pixel weighs the length of its overlap on each axis.  The sums
+// run in float64 (row by row: the row's weighted sum, then times the row
+// weight); the result is (float)(sum(w*v) / sum(w)) over finite pixels, NaN
+// when none.  For an integer ratio every weight is 1.0, so a cell is the
+// float64 mean of its finite pixels rounded once.
+__device__ __forceinline__ void area_span(int i, double ratio, int n, int& k0, int& k1,
+                                          double& a0, double& a1) {
+  a0 = (double)i * ratio;
+  a1 = (double)(i + 1) * ratio;
+  if (a1 > (double)n) a1 = (double)n;
+  k0 = (int)(a0 + 1e-8);
+  k1 = (int)ceil(a1 - 1e-8);
+  if (k1 > n) k1 = n;
+  if (k1 <= k0) k1 = k0 + 1;
+}
+
+__device__ __forceinline__ double area_weight(int k, double a0, double a1) {
+  const double lo = a0 > (double)k ? a0 : (double)k;
+  const double hi = a1 < (double)(k + 1) ? a1 : (double)(k + 1);
+  return hi > lo ? hi - lo : 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+k_area_average(const float* __restrict__ src, int64_t ld, int h, int w, float* __restrict__ out,
+               int64_t ldo, int oh, int ow, double ry, double rx) {
+  const int oc = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (oc >= ow) return;
+  int kx0, kx1;
+  double x0, x1;
+  area_span(oc, rx, w, kx0, kx1, x0, x1);
+  for (int orow = blockIdx.y * 4 + (threadIdx.x >> 6); orow < oh; orow += gridDim.y * 4) {
+    int ky0, ky1;
+    double y0, y1;
+    area_span(orow, ry, h, ky0, ky1, y0, y1);
+    double tot = 0.0, wsum = 0.0;
+    for (int k = ky0; k < ky1; k++) {
+      const double wy = area_weight(k, y0, y1);
+      const float* row = src + (int64_t)k * ld;
+      double rv = 0.0, rw = 0.0;
+      for (int c = kx0; c < kx1; c++) {
+        const float v = row[c];
+        if (isfinite(v)) {
+          const double wx = area_weight(c, x0, x1);
+          rv += wx * (double)v;
+          rw += wx;
+        }
+      }
+      tot += wy * rv;
+      wsum += wy * rw;
+    }
+    out[(int64_t)orow * ldo + oc] =
+        wsum > 0.0 ? (float)(tot / wsum) : __builtin_nanf("");
+  }
+}
+
 extern "C" int fsg_block_mean(const fsg_window* src, int32_t factor, float* out,
                               int64_t ld_out, void* stream) {
   int rc = validate_window(src, "fsg_block_mean");
@@ -601,5 +659,22 @@ extern "C" int fsg_block_mean(const fsg_window* src, int32_t factor, float* out,
   hipLaunchKernelGGL(k_block_mean, grd, blk, 0, as_stream(stream), src->data, src->ld, src->h,
                      src->w, factor, out, ld_out, oh, ow);
   FSG_CHECK_LAUNCH("k_block_mean");
+  return FSG_OK;
+}
+
+
+extern "C" int fsg_overview_average(const fsg_window* src, int32_t oh, int32_t ow, float* out,
+                                    int64_t ld_out, void* stream) {
+  int rc = validate_window(src, "fsg_overview_average");
+  if (rc) return rc;
+  if (src->h == 0 || src->w == 0) return fail(FSG_E_INVALID, "fsg_overview_average: empty source");
+  if (oh < 1 || ow < 1 || oh > src->h || ow > src->w)
+    return fail(FSG_E_INVALID, "fsg_overview_average: need 1 <= oh <= h and 1 <= ow <= w");
+  if (!out || ld_out < ow) return fail(FSG_E_INVALID, "fsg_overview_average: bad output");
+  const double ry = (double)src->h / (double)oh, rx = (double)src->w / (double)ow;
+  dim3 blk(256), grd((ow + 63) / 64, std::min((oh + 3) / 4, 16384));
+  hipLaunchKernelGGL(k_area_average, grd, blk, 0, as_stream(stream), src->data, src->ld, src->h,
+                     src->w, out, ld_out, oh, ow, ry, rx);
+  FSG_CHECK_LAUNCH("k_area_average");
   return FSG_OK;
 }

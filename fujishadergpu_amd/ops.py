@@ -360,6 +360,16 @@ def block_mean(src_t: torch.Tensor, factor: int) -> torch.Tensor:
     return out
 
 
+def overview_average(src_t: torch.Tensor, oh: int, ow: int) -> torch.Tensor:
+    """Area-weighted NaN-aware average of the raster onto an oh x ow grid
+    (Resampling.average at any ratio; float64 sums and division)."""
+    out = torch.empty((int(oh), int(ow)), dtype=torch.float32, device=src_t.device)
+    win = window(src_t)
+    N.check(_lib().fsg_overview_average(C.byref(win), int(oh), int(ow), out.data_ptr(),
+                                        int(ow), _stream()))
+    return out
+
+
 def topousm_coarse_means(win_t: torch.Tensor, ds: int, radii_coarse: Sequence[int]) -> torch.Tensor:
     """Coarse means of one decimation level over a whole window (typically the
     raster): a (len(radii_coarse), ceil(h/ds), ceil(w/ds)) float32 tensor."""

@@ -213,23 +213,36 @@ def overview_factor(H: int, W: int) -> Optional[int]:
     return f
 
 
+def overview(raster: torch.Tensor) -> Tuple[torch.Tensor, float]:
+    """The raster's overview read (core/tile_processor.py:575-590,
+    read_overview_coarse_dem _nan_utils.py:836-871): scale = max(W, H)/2048
+    (at least 1), shape round(H/scale) x round(W/scale), GDAL's average
+    resampling restated as the area-weighted NaN-aware average.  Returns
+    (overview, scale)."""
+    H, W = raster.shape
+    scale, sh, sw = overview_geometry(W, H)
+    f = overview_factor(H, W)
+    if f == 1:
+        return raster.contiguous().clone(), scale
+    if f is not None:
+        return ops.block_mean(raster, f), scale
+    return ops.overview_average(raster, sh, sw), scale
+
+
 def overview_field(raster: torch.Tensor, large_radii, large_weights,
                    coarse: Optional[torch.Tensor] = None,
                    nan_free: bool = False) -> Tuple[torch.Tensor, float]:
     """Large-radius coarse field from the overview (core/tile_processor.py:552-602,
     _impl_topousm_fast.py:133-155).  The overview read is GDAL average
-    resampling; for an integer decimation that is the NaN-aware block mean
-    (``coarse``, when the caller already has it).  ``nan_free``: the raster
-    holds no NaN (so neither does the overview)."""
+    resampling onto round(H/s) x round(W/s) cells, s = max(W, H)/2048: the
+    area-weighted NaN-aware average (``overview``), which for an integer
+    decimation is the block mean (``coarse``, when the caller already has
+    it).  ``nan_free``: the raster holds no NaN (so neither does the
+    overview)."""
     H, W = raster.shape
-    scale, sh, sw = overview_geometry(W, H)
-    f = int(round(scale))
-    if abs(scale - f) > 1e-12 or sh * f != H or sw * f != W:
-        raise UnsupportedOnBackend(
-            f"overview decimation {scale:.4f} for a {W}x{H} raster is not an integer block "
-            "mean; GDAL's fractional average resampling is not restated")
+    scale, _, _ = overview_geometry(W, H)
     if coarse is None:
-        coarse = ops.block_mean(raster, f) if f > 1 else raster.contiguous().clone()
+        coarse, scale = overview(raster)
     rc = [max(1, int(round(float(r) / max(scale, 1.0)))) for r in large_radii]
     if nan_free and min(rc) >= 2 and len(rc) <= N.COARSE_JOB_MAX_RADII:
         # no NaN in the overview: the box means of every radius in one batched
@@ -289,11 +302,26 @@ def dask_edge_depth(algorithm: str, params: dict) -> int:
         "decimation grids to its own chunks, which the tile windows do not reproduce)")
 
 
+def _run_tile(algo, src: torch.Tensor, win_geom, core_geom, pads, spec, p) -> None:
+    """One tile on the current stream.  With edge_mode='reflect' the padded
+    copy of an edge window is allocated and filled here, on the same stream as
+    the kernel that reads it: the caching allocator then ties its block to that
+    stream, so neither the pad nor a later reuse of the block can race the
+    reader."""
+    wy, wx, wh, ww = win_geom
+    cy, cx, ch, cw = core_geom
+    top, bottom, left, right = pads
+    win = src[wy:wy + wh, wx:wx + ww]
+    if top or bottom or left or right:
+        win = ops.pad_reflect(win, top, bottom, left, right)
+    algo.run_region(win, ops.region(cy - wy + top, cx - wx + left, ch, cw), spec, p)
+
+
 def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float] = None,
                 nodata_threshold: float = 1.0, out: Optional[torch.Tensor] = None,
                 out_f32: Optional[torch.Tensor] = None, global_stats=None,
                 copy_input: bool = True, return_plan: bool = False, edge_mode: str = "tile",
-                **kwargs):
+                streams: int = 2, **kwargs):
     """Run one algorithm over a device-resident raster, tile by tile.
 
     Returns the output raster (float32 with NaN NoData, or int16/uint8 with 0
@@ -306,7 +334,11 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     is extended beyond it by the algorithm's map_overlap depth with the
     symmetric rule of boundary='reflect', core/dask_processor.py:1473-1488).
     Spatial mode already reflects at every window edge (each tile is one
-    map_overlap chunk), so there both modes are the same."""
+    map_overlap chunk), so there both modes are the same.
+
+    streams: the number of streams the tiles alternate over (the caller's
+    current stream plus streams - 1 side streams); 1 runs every tile on the
+    caller's stream.  The outputs are the same bits either way."""
     edge_mode = str(edge_mode).lower()
     if edge_mode not in EDGE_MODES:
         raise ValueError(f"edge_mode must be one of {EDGE_MODES}, got {edge_mode!r}")
@@ -344,7 +376,10 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
         # queued first after the synchronisation: it runs while the host
         # prepares the rest (only the main tiles read it)
         lr, lw = P.extras["large"]
-        fld, scale = overview_field(src, lr, lw, coarse=scan["overview"],
+        # the scan's cell means round their sum first; they equal the
+        # overview's float64 means wherever every cell is whole (NaN-free)
+        fld, scale = overview_field(src, lr, lw,
+                                    coarse=scan["overview"] if raster_nan_free else None,
                                     nan_free=raster_nan_free)
         P.params["_topousm_fast_coarse_field"] = fld
         P.extras["overview_scale"] = scale
@@ -353,16 +388,14 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     if spatial_tiles and P.params.get("radii"):
         # the unified overview coarse source of the spatial algorithms
         # (core/tile_processor.py:1519-1545, read_overview_coarse_dem
-        # _nan_utils.py:836-871): GDAL average resampling, here the NaN-aware
-        # block mean of an integer decimation
-        scale, sh, sw = overview_geometry(W, H)
-        f = int(round(scale))
-        if abs(scale - f) <= 1e-12 and sh * f == H and sw * f == W:
-            P.params["_overview_coarse_dem"] = ops.block_mean(src, f) if f > 1 \
-                else src.contiguous().clone()
-            P.params["_overview_decimation"] = 0.5 * (W / float(sw) + H / float(sh))
-            P.params["_tile_full_shape"] = (int(H), int(W))
-            P.params["_overview_cache"] = {}
+        # _nan_utils.py:836-871): GDAL average resampling, restated as the
+        # area-weighted NaN-aware average (the block mean at an integer ratio)
+        ov, _ = overview(src)
+        sh, sw = int(ov.shape[0]), int(ov.shape[1])
+        P.params["_overview_coarse_dem"] = ov
+        P.params["_overview_decimation"] = 0.5 * (W / float(sw) + H / float(sh))
+        P.params["_tile_full_shape"] = (int(H), int(W))
+        P.params["_overview_cache"] = {}
     pyramid = None
     if algorithm == "topousm_fast" and raster_nan_free:
         pyramid = CoarsePyramid(src, decimated=scan["grids"] if scan else None)
@@ -386,7 +419,9 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
     # object, hence workspace, per stream): each tile's last workgroups overlap
     # the next tile's first.  Spatial-mode tiles share the overview responses
     # they build as they go; each entry carries an event the readers wait on.
-    ring = ops.StreamRing(2) if len(P.tiles) > 1 else None
+    if int(streams) < 1:
+        raise ValueError(f"streams must be >= 1, got {streams!r}")
+    ring = ops.StreamRing(int(streams)) if len(P.tiles) > 1 and int(streams) > 1 else None
     algos = [load_algorithm(algorithm) for _ in range(len(ring) if ring else 1)]
     results: List[TileResult] = []
     transform = P.extras.get("transform")
@@ -421,25 +456,21 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
                 a, e, 0.5 * (top + bottom))
             if p.get("_geographic_rows") and algorithm in ROW_SCALED and not spatial_tiles:
                 p["pixel_scale_x_rows"] = row_geographic_scales_x(a, e, f, int(wy), int(wh))
-        win = src[wy:wy + wh, wx:wx + ww]
-        reg = ops.region(cy - wy, cx - wx, ch, cw)
+        pads = (0, 0, 0, 0)
         if reflect_d:
-            top = min(reflect_d, wh) if wy == 0 else 0
-            left = min(reflect_d, ww) if wx == 0 else 0
-            bottom = min(reflect_d, wh) if wy + wh == H else 0
-            right = min(reflect_d, ww) if wx + ww == W else 0
-            if top or left or bottom or right:
-                win = ops.pad_reflect(win, top, bottom, left, right)
-                reg = ops.region(cy - wy + top, cx - wx + left, ch, cw)
+            pads = (min(reflect_d, wh) if wy == 0 else 0,
+                    min(reflect_d, wh) if wy + wh == H else 0,
+                    min(reflect_d, ww) if wx == 0 else 0,
+                    min(reflect_d, ww) if wx + ww == W else 0)
         spec = OutSpec(f32=o_view if dt == torch.float32 else f_view,
                        q=o_view if dt != torch.float32 else None, qp=P.qp,
                        clip01=(algorithm == "hillshade"))
         try:
             if ring is not None:
                 with torch.cuda.stream(ring.stream(k_tile)):
-                    algo.run_region(win, reg, spec, p)
+                    _run_tile(algo, src, (wy, wx, wh, ww), (cy, cx, ch, cw), pads, spec, p)
             else:
-                algo.run_region(win, reg, spec, p)
+                _run_tile(algo, src, (wy, wx, wh, ww), (cy, cx, ch, cw), pads, spec, p)
             results.append(TileResult(ty, tx, True))
         except UnsupportedOnBackend:
             if ring is not None:

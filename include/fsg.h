@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 14
+#define FSG_ABI_VERSION 15
 
 enum {
   FSG_OK = 0,
@@ -392,6 +392,23 @@ int fsg_raster_scan(const fsg_window* raster, int32_t n_out, const fsg_scan_out*
  */
 int fsg_block_mean(const fsg_window* src, int32_t factor, float* out,
                    int64_t ld_out, void* stream);
+
+/*
+ * The overview read of the large-radius paths at any decimation: rasterio's
+ * src.read(out_shape=(oh, ow), resampling=Resampling.average, masked=True)
+ * (core/tile_processor.py:575-590, read_overview_coarse_dem
+ * _nan_utils.py:836-871), restated as an area-weighted average over the
+ * full-resolution raster: output cell (i, j) covers source rows
+ * [i*h/oh, (i+1)*h/oh) and columns [j*w/ow, (j+1)*w/ow), every source pixel
+ * weighs its overlap length on each axis, NaN pixels are left out, and a cell
+ * with no finite pixel is NaN.  Sums and the division in float64, rounded
+ * once to float32.  For an integer ratio that is the float64 mean of each
+ * cell's finite pixels (fsg_block_mean rounds its sum first, so the two can
+ * differ by one float32 ulp where a cell's finite count is not a power of
+ * two).  Needs 1 <= oh <= h and 1 <= ow <= w.
+ */
+int fsg_overview_average(const fsg_window* src, int32_t oh, int32_t ow, float* out,
+                         int64_t ld_out, void* stream);
 
 /*
  * Large-radius coarse field sum_i w_i * mean_{r_i}(coarse)

@@ -378,6 +378,56 @@ def topousm_large_coarse_field(coarse_dem, *, large_radii, large_weights, decima
     return field.astype(F32)
 
 
+def _area_weights(n_src: int, n_dst: int):
+    """n_dst x n_src sparse float64 matrix: row i holds each source pixel's
+    overlap length with [i*n_src/n_dst, (i+1)*n_src/n_dst)."""
+    from scipy import sparse
+    ratio = n_src / float(n_dst)
+    rows, cols, vals = [], [], []
+    for i in range(n_dst):
+        lo, hi = i * ratio, min((i + 1) * ratio, float(n_src))
+        for k in range(int(math.floor(lo)), min(n_src, int(math.ceil(hi)))):
+            w = min(hi, k + 1.0) - max(lo, float(k))
+            if w > 0:
+                rows.append(i)
+                cols.append(k)
+                vals.append(w)
+    return sparse.csr_matrix((vals, (rows, cols)), shape=(n_dst, n_src), dtype=np.float64)
+
+
+def overview_average(raster: np.ndarray, oh: int, ow: int) -> np.ndarray:
+    """The overview read of core/tile_processor.py:575-590 and
+    read_overview_coarse_dem (_nan_utils.py:836-871):
+    src.read(out_shape=(oh, ow), resampling=Resampling.average, masked=True),
+    restated as the area-weighted average of the finite full-resolution
+    pixels (each pixel weighs its overlap with the output cell on each axis),
+    summed and divided in float64 and rounded once to float32; NaN where a
+    cell holds no finite pixel.  GDAL's own arithmetic (and its choice of an
+    internal overview level to resample from) is not restated."""
+    a = np.asarray(raster, dtype=F32)
+    fin = np.isfinite(a)
+    v = np.where(fin, a, F32(0)).astype(np.float64)
+    wy = _area_weights(a.shape[0], int(oh))
+    wx = _area_weights(a.shape[1], int(ow))
+    num = np.asarray((wx @ np.asarray(wy @ v).T).T)
+    den = np.asarray((wx @ np.asarray(wy @ fin.astype(np.float64)).T).T)
+    with _err():
+        out = np.where(den > 0, num / np.maximum(den, 1e-300), np.nan)
+    return out.astype(F32)
+
+
+def overview(raster: np.ndarray, sample_max: int = 2048):
+    """Overview shape and scale of core/tile_processor.py:575-580
+    (scale = max(W, H)/sample_max, at least 1; shape round(H/scale) x
+    round(W/scale)) with overview_average.  Returns (coarse, scale)."""
+    H, W = raster.shape
+    scale = max(W / sample_max, H / sample_max, 1.0)
+    sw_, sh_ = max(1, int(round(W / scale))), max(1, int(round(H / scale)))
+    if sh_ == H and sw_ == W:
+        return np.asarray(raster, dtype=F32).copy(), scale
+    return overview_average(raster, sh_, sw_), scale
+
+
 def overview_block_mean(raster: np.ndarray, sample_max: int = 2048):
     """Overview read of _compute_topousm_fast_overview_coarse_field_tile
     (core/tile_processor.py:575-590) for an integer decimation: GDAL

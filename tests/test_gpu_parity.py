@@ -192,6 +192,24 @@ def test_topousm_field_and_two_outputs(cuda):
     assert np.array_equal(u8.cpu().numpy(), dn)
 
 
+@pytest.mark.parametrize("h,w,oh,ow", [(700, 1000, 287, 410), (2600, 2200, 2048, 1733),
+                                       (999, 1001, 998, 1000), (640, 480, 80, 60)])
+@pytest.mark.parametrize("nan_discs", [0, 20])
+def test_overview_average_matches_the_oracle(cuda, h, w, oh, ow, nan_discs):
+    # the area-weighted overview read at fractional (and one integer) ratios
+    from fujishadergpu_amd import ops
+    dem = _dem(h, w, 41 + h, nan_discs=nan_discs)
+    got = ops.overview_average(dem.to(cuda), oh, ow)
+    exp = ref.overview_average(dem.numpy(), oh, ow)
+    # float64 sums in another order, rounded once: at most one float32 ulp
+    mx, _ = _cmp("overview", got, exp, 0.0, rtol=1.2e-7)
+    print("overview", (h, w, oh, ow), "max |d|", mx, flush=True)
+    if h % oh == 0 and w % ow == 0 and not nan_discs:
+        # an integer ratio over whole cells is the block mean, bit for bit
+        bm = ops.block_mean(dem.to(cuda), h // oh)
+        assert torch.equal(got, bm)
+
+
 def test_topousm_shared_coarse_means(cuda):
     """A window reading raster-wide coarse means (CoarsePyramid) equals the
     same window building its own, and the oracle."""

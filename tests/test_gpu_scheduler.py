@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import geometry as geo
 from oracle import ref
 
 pytestmark = pytest.mark.gpu
@@ -29,7 +30,9 @@ def _oracle_tiles(host, P, algorithm, nodata, stats=None, field=None):
     if nodata is not None:
         src[ref.nodata_mask(src, nodata)] = np.nan
     out = np.full((H, W), np.nan, np.float32)
-    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+    tiles = geo.plan_tiles(algorithm, P.params, W, H, P.tile_size)
+    assert tiles == P.tiles
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in tiles:
         win = src[wy:wy + wh, wx:wx + ww]
         p = {k: v for k, v in P.params.items() if not isinstance(v, torch.Tensor)}
         if stats is not None:
@@ -90,13 +93,10 @@ def test_scheduler_topousm_overview_prepass_and_uint8(cuda):
     field = ref.topousm_large_coarse_field(coarse, large_radii=[300], large_weights=[0.1],
                                            decimation=scale)
     # oracle pre-pass over the same stratified windows, full radii
-    from fujishadergpu_amd.params import norm_stat_window_geometry, stratified_windows
-    margin, tile = norm_stat_window_geometry("topousm_fast", {"radii": radii})
     pool = []
-    for wy, wx, tw, th in stratified_windows(W, H, 0, H, 0, W, grid=3, tile=min(tile, max(W, H))):
+    for wy, wx, tw, th, m in geo.prepass_windows(host, "topousm_fast", {"radii": radii}):
         raw = ref.topousm_fast_block(host[wy:wy + th, wx:wx + tw], radii=radii, weights=weights,
                                      pixel_size=5.0)
-        m = int(min(margin, th // 3, tw // 3))
         raw = raw[m:-m, m:-m] if m > 0 else raw
         pool.append(raw[~np.isnan(raw)])
     stats = ref.topousm_stat(np.concatenate(pool))
@@ -107,6 +107,42 @@ def test_scheduler_topousm_overview_prepass_and_uint8(cuda):
     qp = P.qp
     dn = ref.quantize(got, qp["a_coef"], qp["b_coef"], qp["dn_min"], qp["dn_max"], "uint8")
     assert np.array_equal(out.cpu().numpy(), dn)
+
+
+def test_scheduler_topousm_fractional_overview_default_radii(cuda):
+    # 6000 x 5200 at 10 m, spatial mode, the automatic radii: [2, 8, 32, 128,
+    # 512] (short side / 10 = 520); 512 > max(256, 2048 // 16) goes through the
+    # overview at decimation 6000 / 2048 = 2.93 onto 1775 x 2048 cells, which
+    # is not an integer block mean
+    from fujishadergpu_amd.scheduler import process_dem
+    H, W = 5200, 6000
+    host_t = _raster(H, W, 71, relief=2500.0, hurst=0.8)
+    stats = (37.5,)
+    out, results, P = process_dem(host_t.to(cuda), "topousm_fast", tile_size=2048,
+                                  pixel_size=10.0, mode="spatial", global_stats=stats,
+                                  return_plan=True)
+    assert all(r.success for r in results)
+    assert P.params["radii"] == [2, 8, 32, 128] and P.extras["large"][0] == [512]
+    assert P.extras["overview_scale"] == pytest.approx(6000 / 2048)
+    host = host_t.numpy()
+    coarse, scale = ref.overview(host)
+    assert coarse.shape == (1775, 2048) and scale == pytest.approx(6000 / 2048)
+    lr, lw = P.extras["large"]
+    field = ref.topousm_large_coarse_field(coarse, large_radii=lr, large_weights=lw,
+                                           decimation=scale)
+    tiles = geo.plan_tiles("topousm_fast", P.params, W, H, P.tile_size)
+    assert tiles == P.tiles and P.padding == 160
+    got = out.cpu().numpy()
+    p = {k: v for k, v in P.params.items() if not isinstance(v, torch.Tensor)}
+    p["global_stats"] = stats
+    p["_topousm_fast_coarse_field"] = field
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in (tiles[0], tiles[4], tiles[-1]):
+        p["_topousm_fast_field_offset"] = (wy, wx)
+        res = ref.run_window("topousm_fast", host[wy:wy + wh, wx:wx + ww], p)
+        exp = ref.format_output(res[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw], "topousm_fast")
+        d = np.abs(got[cy:cy + ch, cx:cx + cw].astype(np.float64) - exp)
+        print("tile", (ty, tx), "max |d|", float(d.max()), flush=True)
+        assert d.max() <= 2e-5
 
 
 def test_scheduler_skips_all_nodata_tiles(cuda):
@@ -140,7 +176,9 @@ def test_geographic_raster_tile_and_row_latitudes(cuda, algorithm, rows):
                             geographic_rows=rows, mode="local", return_plan=True)
     host = host_t.numpy()
     want = np.full(host.shape, np.nan, np.float32)
-    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+    tiles = geo.plan_tiles(algorithm, P.params, host.shape[1], host.shape[0], P.tile_size)
+    assert tiles == P.tiles
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in tiles:
         p = {k: v for k, v in P.params.items() if not isinstance(v, torch.Tensor)}
         p["pixel_scale_x"], p["pixel_scale_y"] = tile_geographic_scales(
             a, e, 0.5 * ((f + cy * e) + (f + (cy + ch) * e)))
@@ -187,6 +225,26 @@ def test_reflect_edge_mode_matches_the_dask_single_chunk(cuda, algorithm, kw, to
     tile, _ = process_dem(host_t.to(cuda), algorithm, tile_size=128, pixel_size=10.0,
                           global_stats=stats, **kw)
     assert not torch.equal(tile[:, 0], out[:, 0])
+
+
+@pytest.mark.parametrize("algorithm,kw", [
+    ("curvature", dict(mode="local", curvature_type="mean")),
+    ("openness", dict(mode="local", num_directions=8, max_distance=40)),
+])
+def test_reflect_edge_mode_two_streams_equal_one(cuda, algorithm, kw):
+    # 3 x 3 tiles of 1024: every edge tile's padded copy is made on the stream
+    # that reads it, so the two-stream run equals the one-stream run bit for bit
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _raster(3072, 3072, 29)
+    g = host_t.to(cuda)
+    stats = (0.0, 1.0)
+    one, _ = process_dem(g, algorithm, tile_size=1024, pixel_size=10.0, edge_mode="reflect",
+                         global_stats=stats, streams=1, **kw)
+    for _ in range(2):
+        two, _ = process_dem(g, algorithm, tile_size=1024, pixel_size=10.0,
+                             edge_mode="reflect", global_stats=stats, streams=2, **kw)
+        assert torch.equal(torch.isnan(one), torch.isnan(two))
+        assert torch.equal(torch.nan_to_num(one), torch.nan_to_num(two))
 
 
 def test_reflect_edge_mode_rejects_topousm(cuda):

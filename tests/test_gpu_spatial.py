@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import geometry as geo
 from oracle import ref
 
 pytestmark = pytest.mark.gpu
@@ -83,7 +84,9 @@ def test_scheduler_spatial_with_the_overview(cuda, algorithm):
     host = host_t.numpy()
     ov = host.copy()          # overview decimation 1 at 2048 px (sample_max 2048)
     want = np.full(host.shape, np.nan, np.float32)
-    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+    tiles = geo.plan_tiles(algorithm, P.params, host.shape[1], host.shape[0], P.tile_size)
+    assert tiles == P.tiles
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in tiles:
         p = {k: v for k, v in P.params.items()
              if not isinstance(v, (torch.Tensor, dict)) and not str(k).startswith("_overview")}
         res = ref.spatial_window(algorithm, host[wy:wy + wh, wx:wx + ww], p, overview=ov,
@@ -93,6 +96,35 @@ def test_scheduler_spatial_with_the_overview(cuda, algorithm):
     got = out.cpu().numpy() if out.dtype == torch.float32 else None
     assert got is not None
     _cmp(f"{algorithm}/scheduler", got, want, *TOL[algorithm])
+
+
+@pytest.mark.parametrize("algorithm", ["hillshade", "curvature"])
+def test_scheduler_spatial_with_a_fractional_overview(cuda, algorithm):
+    # 2600 x 2200: the overview is read at decimation 2600 / 2048 = 1.27 onto
+    # 1733 x 2048 cells (area-weighted average, not a block mean); r = 300 is
+    # served from it at every tile
+    from fujishadergpu_amd.scheduler import process_dem
+    H, W = 2200, 2600
+    host_t = _dem(H, W, 83, relief=2500.0)
+    kw = dict(mode="spatial", radii=[4, 40, 300], weights=[0.5, 0.3, 0.2])
+    out, _, P = process_dem(host_t.to(cuda), algorithm, tile_size=1024, pixel_size=10.0,
+                            return_plan=True, **kw)
+    host = host_t.numpy()
+    ov, _ = ref.overview(host)
+    assert ov.shape == (1733, 2048)
+    assert tuple(P.params["_overview_coarse_dem"].shape) == ov.shape
+    dec = 0.5 * (W / 2048.0 + H / 1733.0)
+    assert P.params["_overview_decimation"] == pytest.approx(dec)
+    tiles = geo.plan_tiles(algorithm, P.params, W, H, P.tile_size)
+    assert tiles == P.tiles
+    got = out.cpu().numpy()
+    p = {k: v for k, v in P.params.items()
+         if not isinstance(v, (torch.Tensor, dict)) and not str(k).startswith("_overview")}
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in (tiles[0], tiles[4], tiles[-1]):
+        res = ref.spatial_window(algorithm, host[wy:wy + wh, wx:wx + ww], p, overview=ov,
+                                 decimation=dec, tile_origin=(wy, wx), full_shape=host.shape)
+        exp = ref.format_output(res[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw], algorithm)
+        _cmp(f"{algorithm}/tile{ty}{tx}", got[cy:cy + ch, cx:cx + cw], exp, *TOL[algorithm])
 
 
 @pytest.mark.parametrize("sigma", [1.0, 4.5, 7.3, 16.0, 40.0])
@@ -158,7 +190,9 @@ def test_ray_scheduler_spatial_with_the_overview_and_stretch(cuda, algorithm):
                             return_plan=True, **kw)
     host = host_t.numpy()
     want = np.full(host.shape, np.nan, np.float32)
-    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in P.tiles:
+    tiles = geo.plan_tiles(algorithm, P.params, host.shape[1], host.shape[0], P.tile_size)
+    assert tiles == P.tiles
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in tiles:
         p = {k: v for k, v in P.params.items()
              if not isinstance(v, (torch.Tensor, dict)) and not str(k).startswith("_overview")}
         res = ref.spatial_window(algorithm, host[wy:wy + wh, wx:wx + ww], p, overview=host.copy(),

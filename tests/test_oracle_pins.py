@@ -342,6 +342,70 @@ def test_overview_integer_decimation_is_a_block_mean():
                 r[f * i:f * i + f, f * j:f * j + f].astype(np.float64)), rel=1e-6)
 
 
+def _supersampled_block_mean(a, qy, qx, py, px):
+    """Nearest-upsample by (qy, qx), then NaN-aware py x px block means: the
+    area average at ratios py/qy and px/qx, built without overlap weights."""
+    up = np.repeat(np.repeat(a.astype(np.float64), qy, axis=0), qx, axis=1)
+    oh, ow = up.shape[0] // py, up.shape[1] // px
+    cells = up[:oh * py, :ow * px].reshape(oh, py, ow, px)
+    with np.errstate(invalid="ignore"):
+        return np.nanmean(cells, axis=(1, 3))
+
+
+@pytest.mark.parametrize("h,w,oh,ow,qy,qx", [
+    (10, 14, 4, 4, 2, 2),      # ratios 5/2 and 7/2
+    (15, 8, 6, 3, 2, 3),       # 5/2 and 8/3 (different ratio per axis)
+    (12, 20, 5, 8, 5, 2),      # 12/5 and 5/2
+])
+@pytest.mark.parametrize("with_nan", [False, True])
+def test_overview_average_equals_a_supersampled_block_mean(h, w, oh, ow, qy, qx, with_nan):
+    rng = np.random.default_rng(h * 100 + w)
+    a = (rng.random((h, w)) * 900.0 + 100.0).astype(F32)
+    if with_nan:
+        a[rng.random((h, w)) < 0.2] = np.nan
+        a[: h // 3, : w // 3] = np.nan          # a whole cell may be empty
+    py, px = h * qy // oh, w * qx // ow
+    assert py * oh == h * qy and px * ow == w * qx
+    exp = _supersampled_block_mean(a, qy, qx, py, px)
+    got = ref.overview_average(a, oh, ow)
+    assert np.array_equal(np.isnan(got), np.isnan(exp))
+    np.testing.assert_allclose(got[~np.isnan(got)], exp[~np.isnan(exp)], rtol=2e-7)
+
+
+def test_overview_average_constant_mass_and_integer_ratio():
+    rng = np.random.default_rng(5)
+    c = np.full((37, 53), 1234.5, dtype=F32)
+    c[rng.random(c.shape) < 0.3] = np.nan
+    got = ref.overview_average(c, 9, 17)
+    assert np.all(got == F32(1234.5))
+    # mass: every source pixel is covered exactly once in total
+    a = (rng.random((30, 44)) * 100).astype(F32)
+    got = ref.overview_average(a, 7, 13)
+    area = (30 / 7) * (44 / 13)
+    assert float(got.astype(np.float64).sum() * area) == pytest.approx(
+        float(a.astype(np.float64).sum()), rel=1e-6)
+    # an integer ratio is the float64 block mean of the finite pixels, rounded
+    # once, bit for bit (3 x 3 cells, so most counts are not powers of two)
+    b = (rng.random((24, 36)) * 3000).astype(F32)
+    b[rng.random(b.shape) < 0.25] = np.nan
+    cells = b.astype(np.float64).reshape(8, 3, 12, 3)
+    with np.errstate(invalid="ignore"):
+        exp = np.nanmean(cells, axis=(1, 3)).astype(F32)
+    assert np.array_equal(ref.overview_average(b, 8, 12), exp, equal_nan=True)
+    got, scale = ref.overview_block_mean(b, sample_max=12)
+    assert scale == 3.0 and np.array_equal(got, exp, equal_nan=True)
+
+
+def test_overview_geometry_at_a_fractional_decimation():
+    # core/tile_processor.py:575-580 for a 10000 x 7000 (W x H) raster
+    a = np.zeros((70, 100), dtype=F32)
+    got, scale = ref.overview(a, sample_max=20)
+    assert scale == 5.0 and got.shape == (14, 20)
+    b = np.zeros((7000 // 50, 10000 // 50), dtype=F32)
+    got, scale = ref.overview(b, sample_max=48)
+    assert scale == pytest.approx(200 / 48) and got.shape == (34, 48)   # round(140/4.1667)=34
+
+
 def test_p99_stat_reference_vectors():
     # reference tests/test_topousm_fast_normalization.py:6-33
     rng = np.random.default_rng(0)
