@@ -10,7 +10,11 @@ beside the GPU.  It is never imported by the product package
 
 Modules
 -------
-ref    float32-faithful restatement of the reference block functions
+ref       float32-faithful restatement of the reference block functions
+geometry  tile grid, tile halo, valid-data box and stratified pre-pass windows
+prepare   the prepare step (NoData inference, push-pull fill)
+cnn       FP64 reference of the openness surrogate network
+tiff      pure-Python TIFF / BigTIFF reader
 
 Pinning (tests/test_oracle_pins.py checks each function against something
 other than itself; the GPU tests then compare the HIP path with this oracle):
@@ -22,7 +26,8 @@ gradient / hillshade/slope  exact on planes with anisotropic steps and on
                             orientations; slope in every unit on planes
 curvature                   reference tests/test_curvature_analytic.py values;
                             closed-form mean (bowl) and Gaussian (elliptic)
-                            curvature at an apex
+                            curvature at an apex; the signed row step (bowl
+                            apex H = 0, saddle s = 0 on north-up rasters)
 NaN Gaussian / box means    brute-force FP64 normalised convolutions with the
                             'nearest' and 'reflect' index rules
 topousm                     z = x^2 -> -sum w r(r+1)/3; planes and constants
@@ -30,6 +35,15 @@ topousm                     z = x^2 -> -sum w r(r+1)/3; planes and constants
 decimate / zoom / sample    brute-force cell means, enclosed-void fill,
                             corner-aligned and pixel-centre bilinear loops;
                             integer overview block mean
+overview (any ratio)        supersampled block means at rational ratios,
+                            constants, mass conservation, float64 block mean
+                            at integer ratios
+large-radius field          constant -> 0, a ramp -> 0 inside (the window
+                            offset and the pixel-centre map), the r / s
+                            rounding table with its sigma = 1 branch
+spatial smoothing           sigma = r / 2 from the second moment of u^2
+geometry                    hand-written tile and pre-pass window lists for
+                            the 16384^2, 32768^2 and ragged rasters
 openness                    reference tests/test_openness_yokoyama.py
                             orderings and flat = 1; cone-apex closed forms
                             (pi/2 - atan k), physical pixel distance
@@ -46,8 +60,9 @@ the exact float32 bits of CuPy's gaussian_filter / uniform_filter / zoom
 accumulation and of its float32 cell-sum reduction in _downsample_nan_aware
 (restated with SciPy semantics: double accumulation, float32 per pass), the
 precision of the 0-d promotion in compute_hillshade_block (restated in
-float64, checked only against the closed form to 1e-6), and GDAL's
-average-resampled overview read for a non-integer decimation
-(overview_block_mean raises NotImplementedError there).  The GPU parity
-tests state the tolerance that covers each.
+float64, checked only against the closed form to 1e-6), and GDAL's own
+average resampling (overview_average restates it as the area-weighted mean
+of the full-resolution pixels; GDAL may resample from an internal overview
+level instead, and rounds in its own order).  The GPU parity tests state the
+tolerance that covers each.
 """

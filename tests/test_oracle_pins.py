@@ -171,6 +171,34 @@ def test_mean_curvature_at_a_bowl_apex():
     assert out[16, 16] == pytest.approx(_disp(-2 * c), abs=1e-6)
 
 
+def test_signed_row_step_flips_the_row_second_derivatives():
+    # compute_curvature_block takes its first derivatives with |step| but the
+    # second ones with the signed steps (_impl_curvature.py:23-33).  On a
+    # north-up raster (pixel_scale_y < 0) the row second derivatives dyy and
+    # dxy change sign, dxx and dyx do not.
+    c = 0.001
+    y, x = _grid(33, 33)
+    bowl = (F32(c) * ((x - 16) ** 2 + (y - 16) ** 2)).astype(F32)
+    # bowl apex: r = 2c, t = -2c -> H = -(r + t) / 2 = 0 (vs -2c with a positive step)
+    neg = ref.curvature(bowl, curvature_type="mean", pixel_scale_x=1.0, pixel_scale_y=-1.0)
+    pos = ref.curvature(bowl, curvature_type="mean", pixel_scale_x=1.0, pixel_scale_y=1.0)
+    assert neg[16, 16] == pytest.approx(ZERO_DISP, abs=1e-6)
+    assert pos[16, 16] == pytest.approx(_disp(-2 * c), abs=1e-6)
+    # saddle z = k (x - 16)(y - 16): p = k (y - 16), q = k (x - 16), r = t = 0,
+    # dyx = k and dxy = -k with the signed step, so s = 0 and H = 0 everywhere;
+    # with a positive step s = k and H = p q s / (1 + p^2 + q^2)^1.5
+    k = 0.0625          # a power of two: the surface and its differences are exact
+    sad = (F32(k) * (x - 16) * (y - 16)).astype(F32)
+    neg = ref.curvature(sad, curvature_type="mean", pixel_scale_x=1.0, pixel_scale_y=-1.0)
+    pos = ref.curvature(sad, curvature_type="mean", pixel_scale_x=1.0, pixel_scale_y=1.0)
+    assert np.abs(neg - ZERO_DISP).max() < 1e-6
+    for (i, j) in ((10, 20), (22, 5), (25, 27)):
+        p, q = k * (i - 16), k * (j - 16)
+        hk = p * q * k / (1 + p * p + q * q) ** 1.5
+        assert pos[i, j] == pytest.approx(_disp(hk), abs=2e-6)
+        assert abs(pos[i, j] - ZERO_DISP) > 1e-3
+
+
 def test_gaussian_curvature_at_an_elliptic_apex():
     # z = c x^2 + d y^2 at the apex: K = r t - s^2 = 4 c d
     c, d = 0.05, 0.03
@@ -251,6 +279,83 @@ def test_topousm_on_a_parabola_is_minus_weighted_r_r_plus_1_over_3():
     out = ref.topousm_fast_block(z, radii=[2, 5], weights=[0.6, 0.4], pixel_size=100.0)
     exp = -(0.6 * 2 * 3 / 3 + 0.4 * 5 * 6 / 3)
     assert np.allclose(out[:, 6:35], exp, atol=1e-3, rtol=0)
+
+
+def _large_only(block, field, w_large, off, full):
+    """The large-radius part of _direct_topousm_fast on its own
+    (tile/dask_bridge.py:237-255): no small radii, scale 1."""
+    return ref.topousm_direct(block, radii=[], weights=[], coarse_field=field, small_radii=None,
+                              w_large=w_large, field_offset=off, full_shape=full, stats=(1.0,))
+
+
+def test_topousm_large_part_of_a_constant_is_zero():
+    # constant c: every box mean of the overview is c, the field is (sum w) c
+    # and W_large * c - bilinear(field) vanishes up to the float32 rounding of
+    # the two weighted sums
+    c = 1234.75
+    r = np.full((256, 320), c, F32)
+    coarse, s = ref.overview(r, sample_max=32)          # s = 10: 26 x 32 cells
+    lw = [0.125, 0.0625]
+    field = ref.topousm_large_coarse_field(coarse, large_radii=[40, 120], large_weights=lw,
+                                           decimation=s)
+    assert np.allclose(field, sum(lw) * c, rtol=2e-7)
+    large = _large_only(r[30:200, 50:300], field, sum(lw), (30, 50), r.shape)
+    assert np.abs(large).max() <= 4 * np.spacing(F32(c))
+
+
+def test_topousm_large_part_of_a_ramp_vanishes_inside():
+    # z = a x + b y on an integer-decimated raster (s = 8): the 8 x 8 cell
+    # means are z at the cell centres, a box keeps a linear function away from
+    # the coarse edges, and the pixel-centre bilinear map u = (x + 0.5) s_c / W
+    # - 0.5 returns z itself at every pixel -- so W_large z - bilinear(field)
+    # is 0 in the interior.  A field offset at the core origin instead of the
+    # window origin, or a corner-aligned map, leaves a ramp residue.
+    a, b = 0.75, -0.5
+    yy, xx = np.mgrid[0:256, 0:256].astype(np.float64)
+    z = (a * xx + b * yy).astype(F32)
+    coarse, s = ref.overview(z, sample_max=32)
+    assert s == 8.0 and coarse.shape == (32, 32)
+    field = ref.topousm_large_coarse_field(coarse, large_radii=[40], large_weights=[0.5],
+                                           decimation=s)          # r_c = 5: box of 11
+    off = (40, 36)
+    large = _large_only(z[40:220, 36:230], field, 0.5, off, z.shape)
+    # interior: coarse samples at least 6 cells from the coarse edge -> pixels 52..203
+    inner = large[52 - 40:204 - 40, 52 - 36:204 - 36]
+    assert np.abs(inner).max() < 2e-4
+    wrong = _large_only(z[40:220, 36:230], field, 0.5, (60, 56), z.shape)
+    # the shifted sample reads 20 rows and columns further on: 0.5 (20 a + 20 b)
+    assert np.abs(wrong[52 - 40:150 - 40, 52 - 36:150 - 36] + 2.5).max() < 2e-4
+
+
+@pytest.mark.parametrize("r,s,rc", [(4, 8.0, 1), (12, 8.0, 2), (20, 8.0, 2), (28, 8.0, 4),
+                                    (44, 8.0, 6), (512, 8.0, 64), (2048, 8.0, 256),
+                                    (300, 2.9296875, 102), (7, 5.0, 1)])
+def test_topousm_coarse_radius_rounding(r, s, rc):
+    # r_c = max(1, round(r / s)) with Python's round (ties to even), and the
+    # sigma = 1 Gaussian when r_c <= 1 (_impl_topousm_fast.py:133-155)
+    assert max(1, int(round(r / s))) == rc
+    rng = np.random.default_rng(r)
+    g = (rng.random((10, 13)) * 50).astype(F32)
+    f = ref.topousm_large_coarse_field(g, large_radii=[r], large_weights=[1.0], decimation=s)
+    if rc <= 1:
+        taps, wts, mode = *_gauss_w(1.0), "nearest"
+    else:
+        taps, wts, mode = np.arange(-rc, rc + 1), np.full(2 * rc + 1, 1.0 / (2 * rc + 1)), \
+            "reflect"
+    x = g.astype(np.float64)
+    for axis in (0, 1):            # separable: one explicit 1-D pass per axis
+        n = x.shape[axis]
+        y = np.zeros_like(x)
+        for i in range(n):
+            for k, w in zip(taps, wts):
+                src = _idx(i + int(k), n, mode)
+                if axis == 0:
+                    y[i] += w * x[src]
+                else:
+                    y[:, i] += w * x[:, src]
+        x = y
+    exp = x
+    assert np.allclose(f, exp, atol=3e-5, rtol=0)
 
 
 def test_topousm_is_zero_on_planes_and_constants():
@@ -712,6 +817,26 @@ def test_smooth_for_radius_keeps_a_constant(radius, algo):
     s = ref.smooth_for_radius(z, radius, pixel_size=10.0, algorithm_name=algo)
     assert s.shape == z.shape
     assert np.abs(s - 123.25).max() < 1e-4
+
+
+@pytest.mark.parametrize("radius", [6.0, 20.0, 40.0])
+def test_smooth_for_radius_sigma_is_half_the_radius(radius):
+    # _smooth_for_radius (_nan_utils.py:527-552) smooths with sigma = r / 2
+    # (truncate 4, no decimation below r = 48 at 10 m for hillshade).  Away from
+    # the edges the Gaussian of u^2 is u^2 plus the second moment of the
+    # normalised truncated kernel, sum_k g_k (k / 8)^2 for u = (x - 115) / 8;
+    # sigma = r / 3 or r would move it by a factor 4/9 or 4
+    sigma = radius / 2.0
+    R = int(4.0 * sigma + 0.5)
+    k = np.arange(-R, R + 1, dtype=np.float64)
+    g = np.exp(-0.5 * (k / sigma) ** 2)
+    m2 = float((g * (k / 8.0) ** 2).sum() / g.sum())
+    x = np.arange(230, dtype=np.float64)
+    z = np.tile((((x - 115.0) / 8.0) ** 2).astype(F32), (60, 1))
+    s = ref.smooth_for_radius(z, radius, pixel_size=10.0, algorithm_name="hillshade")
+    inner = s[:, R + 1:230 - R - 1] - z[:, R + 1:230 - R - 1]
+    assert np.abs(inner - m2).max() < 2e-4
+    assert abs(m2 - (sigma / 8.0) ** 2) < 0.01 * (sigma / 8.0) ** 2
 
 
 def test_spatial_hillshade_of_a_plane_interior_is_the_closed_form():
