@@ -163,13 +163,31 @@ def _stats_tuple_ok(stats, n=1, tiny=1e-9) -> bool:
 
 
 class OutSpec:
-    """Where and how run_region writes: float32 and/or integer outputs."""
+    """Where and how run_region writes: float32 and/or integer outputs.
+
+    The outputs are region-shaped 2-D tensors, or band-first C x h x w stacks
+    for agg='stack' (one band per radius, the reference's stack contract,
+    _nan_utils.py:191-194, tile/dask_bridge.py:32-36)."""
 
     def __init__(self, f32: Optional[torch.Tensor] = None, q: Optional[torch.Tensor] = None,
                  qp: Optional[dict] = None, clip01: bool = False):
         self.f32, self.q, self.qp, self.clip01 = f32, q, qp, clip01
 
+    @property
+    def n_bands(self) -> Optional[int]:
+        """The band count of a C x h x w stack output, None for 2-D outputs."""
+        t = self.f32 if self.f32 is not None else self.q
+        return int(t.shape[0]) if t is not None and t.dim() == 3 else None
+
+    def band(self, i: int) -> "OutSpec":
+        return OutSpec(f32=self.f32[i] if self.f32 is not None else None,
+                       q=self.q[i] if self.q is not None else None, qp=self.qp,
+                       clip01=self.clip01)
+
     def descriptors(self, post=N.POST_NONE, host_stats=(0.0, 0.0), dev_stats=None):
+        if self.n_bands is not None:
+            raise ValueError("a band stack (C x H x W) output is written only by agg='stack' "
+                             "in spatial mode")
         out = []
         if self.f32 is not None:
             out.append(ops.output(self.f32, clip01=self.clip01, post=post,
@@ -192,6 +210,18 @@ def _post_from_stats(kind: str, stats):
     if not (isinstance(stats, (tuple, list)) and len(stats) >= 2):
         return N.POST_NONE, (0.0, 0.0), None
     return N.POST_STRETCH, (float(stats[0]), float(stats[1])), None
+
+
+def _run_spatial(name: str, win, reg, out: "OutSpec", p: dict,
+                 post=(N.POST_NONE, (0.0, 0.0), None), direct: bool = False) -> None:
+    """spatial.run through ``out``: one descriptor set for a 2-D output, one
+    per band for a band-first stack."""
+    from . import spatial
+    if out.n_bands is None:
+        spatial.run(name, win, reg, out.descriptors(*post), p, direct=direct)
+    else:
+        spatial.run(name, win, reg, None, p, direct=direct,
+                    bands=[out.band(i).descriptors(*post) for i in range(out.n_bands)])
 
 
 # ------------------------------------------------------------------- bases
@@ -233,11 +263,27 @@ class NativeTileAlgorithm(TileAlgorithm):
     def is_spatial(self, p: dict) -> bool:
         return self.spatial_ok and str(p.get("mode", "local")).lower() == "spatial"
 
+    def stack_bands(self, p: dict) -> Optional[int]:
+        """The band count when ``p`` asks for agg='stack' on a multi-radius
+        path (spatial mode, or hillshade's local multi-radius branch):
+        band-first C x H x W, one band per radius, even for one radius
+        (_nan_utils.py:191-194, tile/dask_bridge.py:32-36).  None when the
+        output is one band (the local block functions ignore agg)."""
+        if str(p.get("agg", "mean") or "mean").lower() != "stack":
+            return None
+        from . import spatial
+        if self.is_spatial(p):
+            return spatial.n_fields(self.name, p)
+        return None
+
     def process(self, dem_gpu: torch.Tensor, **params) -> torch.Tensor:
         ops._check_window(dem_gpu, "dem_gpu")
         h, w = dem_gpu.shape
-        out = torch.empty((h, w), dtype=torch.float32, device=dem_gpu.device)
-        self.run_region(dem_gpu, ops.full_region(dem_gpu), OutSpec(f32=out), self.merged(params))
+        p = self.merged(params)
+        c = self.stack_bands(p)
+        shape = (h, w) if c is None else (c, h, w)
+        out = torch.empty(shape, dtype=torch.float32, device=dem_gpu.device)
+        self.run_region(dem_gpu, ops.full_region(dem_gpu), OutSpec(f32=out), p)
         return out
 
     @abstractmethod
@@ -256,21 +302,29 @@ class HillshadeAlgorithm(NativeTileAlgorithm):
 
     spatial_ok = True
 
-    def run_region(self, win, reg, out, p):
-        self.check_mode(p)
-        if self.is_spatial(p):
-            from . import spatial
-            spatial.run("hillshade", win, reg, out.descriptors(), p)
-            return
+    @staticmethod
+    def _direct_radii(p: dict) -> List[float]:
         radii = p.get("radii", [1])
         if not isinstance(radii, (list, tuple)) or len(radii) == 0:
             radii = [1]
-        radii = [max(1.0, float(r)) for r in radii]
+        return [max(1.0, float(r)) for r in radii]
+
+    def stack_bands(self, p: dict) -> Optional[int]:
+        c = super().stack_bands(p)
+        if c is None and str(p.get("agg", "mean") or "mean").lower() == "stack" \
+                and not self.is_spatial(p) and len(self._direct_radii(p)) > 1:
+            c = len(self._direct_radii(p))        # _direct_hillshade's multi-radius branch
+        return c
+
+    def run_region(self, win, reg, out, p):
+        self.check_mode(p)
+        if self.is_spatial(p):
+            _run_spatial("hillshade", win, reg, out, p)
+            return
+        radii = self._direct_radii(p)
         if (bool(p.get("multiscale", False)) or len(radii) > 1) and len(radii) > 1:
             # _direct_hillshade's multi-radius branch (tile/dask_bridge.py:95-105)
-            from . import spatial
-            spatial.run("hillshade", win, reg, out.descriptors(), {**p, "radii": radii},
-                        direct=True)
+            _run_spatial("hillshade", win, reg, out, {**p, "radii": radii}, direct=True)
             return
         z = p.get("z_factor", 1.0)
         z = 1.0 if z is None else z
@@ -296,8 +350,7 @@ class SlopeAlgorithm(NativeTileAlgorithm):
     def run_region(self, win, reg, out, p):
         self.check_mode(p)
         if self.is_spatial(p):
-            from . import spatial
-            spatial.run("slope", win, reg, out.descriptors(), p)
+            _run_spatial("slope", win, reg, out, p)
             return
         sx, sy, _, _, rows = shade_steps(p, win)
         for o in out.descriptors():
@@ -348,8 +401,7 @@ class CurvatureAlgorithm(NativeTileAlgorithm):
     def run_region(self, win, reg, out, p):
         self.check_mode(p)
         if self.is_spatial(p):
-            from . import spatial
-            spatial.run("curvature", win, reg, out.descriptors(), p)
+            _run_spatial("curvature", win, reg, out, p)
             return
         sx, sy = curvature_steps(p.get("pixel_size", 1.0), p.get("pixel_scale_x"),
                                  p.get("pixel_scale_y"))
@@ -377,8 +429,7 @@ class OpennessAlgorithm(NativeTileAlgorithm):
         self.check_mode(p)
         post = _post_from_stats("stretch", p.get("global_stats") if stretch else None)
         if self.is_spatial(p):
-            from . import spatial
-            spatial.run("openness", win, reg, out.descriptors(*post), p)
+            _run_spatial("openness", win, reg, out, p, post=post)
             return
         table = openness_table(p.get("num_directions", 16), p.get("max_distance", 50),
                                p.get("pixel_size", 1.0), p.get("pixel_scale_x"),
@@ -402,8 +453,7 @@ class AmbientOcclusionAlgorithm(NativeTileAlgorithm):
         self.check_mode(p)
         post = _post_from_stats("stretch", p.get("global_stats") if stretch else None)
         if self.is_spatial(p):
-            from . import spatial
-            spatial.run("ambient_occlusion", win, reg, out.descriptors(*post), p)
+            _run_spatial("ambient_occlusion", win, reg, out, p, post=post)
             return
         radius = p.get("radius", 10.0)
         table = ao_table(p.get("num_samples", 16), radius, p.get("pixel_size", 1.0),

@@ -413,8 +413,19 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
         P.params["global_stats"] = stats
     P.stats = stats
     dt = {"float32": torch.float32, "int16": torch.int16, "uint8": torch.uint8}[P.output_dtype]
+    # agg='stack' in spatial mode: band-first C x H x W, one band per radius
+    # (_nan_utils.py:191-194; the tile writer keeps it as C bands,
+    # core/tile_processor.py:944-971,606-624)
+    shape: Tuple[int, ...] = (H, W)
+    if spatial_tiles and str(P.params.get("agg", "mean") or "mean").lower() == "stack":
+        from . import spatial
+        shape = (spatial.n_fields(algorithm, P.params), H, W)
     if out is None:
-        out = torch.empty((H, W), dtype=dt, device=raster.device)
+        out = torch.empty(shape, dtype=dt, device=raster.device)
+    elif tuple(out.shape) != shape:
+        raise ValueError(f"out has shape {tuple(out.shape)}, expected {shape}")
+    if out_f32 is not None and tuple(out_f32.shape) != shape:
+        raise ValueError(f"out_f32 has shape {tuple(out_f32.shape)}, expected {shape}")
     # tiles alternate between two streams (independent windows, one algorithm
     # object, hence workspace, per stream): each tile's last workgroups overlap
     # the next tile's first.  Spatial-mode tiles share the overview responses
@@ -429,8 +440,8 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
         algo = algos[k_tile % len(algos)]
         ty, tx, cx, cy, cw, ch, wx, wy, ww, wh = t
         ratio = float(nn) / float(ww * wh) if ww * wh else 1.0
-        o_view = out[cy:cy + ch, cx:cx + cw]
-        f_view = out_f32[cy:cy + ch, cx:cx + cw] if out_f32 is not None else None
+        o_view = out[..., cy:cy + ch, cx:cx + cw]
+        f_view = out_f32[..., cy:cy + ch, cx:cx + cw] if out_f32 is not None else None
         if nn > 0 and ratio >= nodata_threshold:
             o_view.fill_(float("nan") if dt == torch.float32 else 0)
             if f_view is not None:

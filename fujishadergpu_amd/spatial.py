@@ -267,15 +267,43 @@ def _sample(resp: torch.Tensor, r0: int, c0: int, full_h: int, full_w: int,
                                       out.shape[0], out.shape[1], _s()))
 
 
+def field_radii(algorithm: str, p: dict, direct: bool = False):
+    """(radii, weights) of the per-radius fields, in the reference's order:
+    _resolve_spatial_radii_weights, then openness's max(2, r) and AO's
+    max(1, r) (_impl_openness.py:193, _impl_ambient_occlusion.py:178); the
+    direct hillshade path keeps max(1, r) as given."""
+    user_w = p.get("weights")
+    if direct:
+        return [max(1.0, float(r)) for r in (p.get("radii") or [1])], user_w
+    radii, resolved_w = resolve_radii_weights(p.get("radii"), user_w)
+    weights = (resolved_w if user_w is None else user_w) if algorithm == "hillshade" \
+        else resolved_w
+    if algorithm == "openness":
+        radii = [int(max(2, round(float(r)))) for r in radii]
+    elif algorithm == "ambient_occlusion":
+        radii = [max(1, int(round(float(r)))) for r in radii]
+    return radii, weights
+
+
+def n_fields(algorithm: str, p: dict, direct: bool = False) -> int:
+    """How many per-radius fields (the bands of agg='stack') ``p`` makes."""
+    return len(field_radii(algorithm, p, direct)[0])
+
+
 def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict,
-        direct: bool = False) -> None:
+        direct: bool = False, bands=None) -> None:
     """The spatial-mode result of ``algorithm`` on the window, over region
     reg, written through the output descriptors.
 
     direct: hillshade's local-mode multi-radius path of the tile backend
     (_direct_hillshade, tile/dask_bridge.py:72-111): the spatial block function
     on the window itself (no halo, no overview), radii max(1, r) as given, and
-    _combine_direct (tile/dask_bridge.py:28-69)."""
+    _combine_direct (tile/dask_bridge.py:28-69).
+
+    bands: for agg='stack', one descriptor list per band (per radius, in
+    radius order); band i receives field i unchanged, through its output
+    epilogue (the band-first C x H x W stack of _nan_utils.py:191-194 and
+    tile/dask_bridge.py:32-36)."""
     if algorithm not in SPATIAL_ALGORITHMS or (direct and algorithm != "hillshade"):
         raise ValueError(f"no spatial path for {algorithm}")
     if p.get("pixel_scale_x_rows") is not None:
@@ -283,17 +311,16 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
     h, w = win.shape
     px = float(p.get("pixel_size", 1.0) or 1.0)
     agg = str(p.get("agg", "mean") or "mean").lower()
+    radii, weights = field_radii(algorithm, p, direct)
     if agg == "stack":
-        raise ValueError("agg='stack' returns a band stack; this path writes one band")
-    user_w = p.get("weights")
-    if direct:
-        radii = [max(1.0, float(r)) for r in (p.get("radii") or [1])]
-        weights = user_w
-        resolved_w = None
-    else:
-        radii, resolved_w = resolve_radii_weights(p.get("radii"), user_w)
-        weights = (resolved_w if user_w is None else user_w) if algorithm == "hillshade" \
-            else resolved_w
+        if bands is None:
+            raise ValueError("agg='stack' returns one band per radius: give a band-first "
+                             "C x H x W output")
+        if len(bands) != len(radii):
+            raise ValueError(f"agg='stack' makes {len(radii)} bands; the output has "
+                             f"{len(bands)}")
+    elif bands is not None:
+        raise ValueError(f"a band stack output needs agg='stack' (agg={agg!r})")
     thr = int(max(256, min(h, w) // 16))
     F = coarsen_factor_for_shape(h, w)
     overview = p.get("_overview_coarse_dem")
@@ -305,10 +332,6 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
     nan_free = bool(p.get("_window_nan_free", False))
     base = {k: v for k, v in p.items() if k not in ("radii", "weights", "mode", "agg")}
     ray = algorithm in RAY_ALGORITHMS
-    if algorithm == "openness":          # _impl_openness.py:193
-        radii = [int(max(2, round(float(r)))) for r in radii]
-    elif algorithm == "ambient_occlusion":   # _impl_ambient_occlusion.py:178
-        radii = [max(1, int(round(float(r)))) for r in radii]
     fields: List[torch.Tensor] = []
     for r in [float(v) for v in radii]:
         field = torch.empty((reg.h, reg.w), dtype=torch.float32, device=win.device)
@@ -359,8 +382,16 @@ def run(algorithm: str, win: torch.Tensor, reg: N.fsg_region, out_descs, p: dict
             _local(algorithm, sm, ops.region(d + reg.r0, d + reg.c0, reg.h, reg.w), field, base,
                    nan_free=nan_free or not may_nan)
         fields.append(field)
-    # combine (the scalar weights are float32 in the products)
     n = len(fields)
+    if agg == "stack":
+        one = (C.c_float * 1)(1.0)
+        for f, descs in zip(fields, bands):
+            ptr = (C.c_void_p * 1)(f.data_ptr())
+            for o in descs:      # a one-field sum is the field itself
+                N.check(N.lib().fsg_combine_fields(ptr, 1, f.stride(0), one, N.AGG_SUM, reg.h,
+                                                   reg.w, C.byref(o), _s()))
+        return
+    # combine (the scalar weights are float32 in the products)
     wv = None
     kind = N.AGG_MEAN
     if direct:

@@ -202,3 +202,75 @@ def test_ray_scheduler_spatial_with_the_overview_and_stretch(cuda, algorithm):
             res[cy - wy:cy - wy + ch, cx - wx:cx - wx + cw], algorithm)
     assert out.dtype == torch.float32
     _cmp(f"{algorithm}/scheduler", out, want, RAY_TOL)
+
+
+@pytest.mark.parametrize("algorithm,radii", [("hillshade", [3, 12, 30]), ("slope", [40]),
+                                             ("curvature", [2, 64]),
+                                             ("openness", [1, 2, 24]),
+                                             ("ambient_occlusion", [4, 16])])
+def test_stack_is_band_first_one_band_per_radius(cuda, algorithm, radii):
+    # agg='stack': C x H x W, band i the field of radius i unchanged, even for
+    # one radius (_nan_utils.py:191-194); openness keeps the duplicate radius
+    # max(2, r) makes of [1, 2]; openness / AO bands each take the stretch
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(260, 300, 61, nan_discs=3)
+    kw = dict(mode="spatial", radii=radii, agg="stack", pixel_size=10.0)
+    if algorithm == "openness":
+        kw.update(num_directions=8, global_stats=(0.6, 0.35))
+    if algorithm == "ambient_occlusion":
+        kw.update(num_samples=8, global_stats=(0.5, 0.4))
+    got = ALGORITHMS[algorithm].process(dem.to(cuda), **kw)
+    assert tuple(got.shape) == (len(radii), 260, 300)
+    exp = ref.spatial_window(algorithm, dem.numpy(), kw)
+    if algorithm in ("openness", "ambient_occlusion"):
+        exp = ref.display_stretch(exp, kw["global_stats"])
+    assert exp.shape == tuple(got.shape)
+    atol, rtol = TOL.get(algorithm, (RAY_TOL, 0.0))
+    for i in range(len(radii)):
+        _cmp(f"{algorithm}/band{i}", got[i], exp[i], atol, rtol)
+    if algorithm == "openness":
+        assert torch.equal(torch.nan_to_num(got[0]), torch.nan_to_num(got[1]))
+
+
+def test_stack_of_the_direct_hillshade_path(cuda):
+    # local mode, several radii: _direct_hillshade's _combine_direct stack
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    dem = _dem(200, 230, 43)
+    got = ALGORITHMS["hillshade"].process(dem.to(cuda), mode="local", radii=[2, 9], agg="stack",
+                                          pixel_size=10.0)
+    assert tuple(got.shape) == (2, 200, 230)
+    for i, r in enumerate((2.0, 9.0)):
+        exp = ref.spatial_block("hillshade", dem.numpy(), r, {"pixel_size": 10.0})
+        _cmp(f"direct/band{i}", got[i], exp, 2e-5)
+    one = ALGORITHMS["hillshade"].process(dem.to(cuda), mode="local", agg="stack",
+                                          pixel_size=10.0)
+    assert one.dim() == 2          # the local block function ignores agg
+
+
+def test_scheduler_stack_with_the_overview_and_uint8(cuda):
+    # tiles write every band; r = 300 comes from the overview; DN per band
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _dem(2048, 2048, 47, relief=2500.0)
+    kw = dict(mode="spatial", radii=[4, 40, 300], agg="stack")
+    f32 = torch.empty((3, 2048, 2048), dtype=torch.float32, device=cuda)
+    out, _, P = process_dem(host_t.to(cuda), "hillshade", tile_size=1024, pixel_size=10.0,
+                            output_dtype="uint8", out_f32=f32, return_plan=True, **kw)
+    assert tuple(out.shape) == (3, 2048, 2048) and out.dtype == torch.uint8
+    host = host_t.numpy()
+    tiles = geo.plan_tiles("hillshade", P.params, 2048, 2048, P.tile_size)
+    assert tiles == P.tiles
+    want = np.full((3, 2048, 2048), np.nan, np.float32)
+    p = {k: v for k, v in P.params.items()
+         if not isinstance(v, (torch.Tensor, dict)) and not str(k).startswith("_overview")}
+    for ty, tx, cx, cy, cw, ch, wx, wy, ww, wh in tiles:
+        res = ref.spatial_window("hillshade", host[wy:wy + wh, wx:wx + ww], p,
+                                 overview=host.copy(), decimation=1.0, tile_origin=(wy, wx),
+                                 full_shape=host.shape)
+        want[:, cy:cy + ch, cx:cx + cw] = ref.format_output(
+            res[:, cy - wy:cy - wy + ch, cx - wx:cx - wx + cw], "hillshade")
+    for i in range(3):
+        _cmp(f"stack/band{i}", f32[i], want[i], *TOL["hillshade"])
+    qp = P.qp
+    dn = ref.quantize(f32.cpu().numpy(), qp["a_coef"], qp["b_coef"], qp["dn_min"], qp["dn_max"],
+                      "uint8")
+    assert np.array_equal(out.cpu().numpy(), dn)

@@ -173,3 +173,31 @@ def test_valid_bbox_matches_a_brute_force_overview(seed):
         ys, xs = np.nonzero(ok.any(axis=1))[0], np.nonzero(ok.any(axis=0))[0]
         want = (ys[0] * cov, min(H, (ys[-1] + 1) * cov), xs[0] * cov, min(W, (xs[-1] + 1) * cov))
     assert valid_bbox(torch.from_numpy(a)) == want
+
+
+def test_stack_band_counts():
+    # agg='stack' is band-first with one band per radius (_nan_utils.py:191-194),
+    # only on the multi-radius paths; the local block functions ignore agg
+    from fujishadergpu_amd.algorithms import ALGORITHMS
+    sp = dict(mode="spatial", agg="stack")
+    assert ALGORITHMS["hillshade"].stack_bands({**sp, "radii": [2, 8, 32]}) == 3
+    assert ALGORITHMS["slope"].stack_bands({**sp, "radii": [40]}) == 1
+    assert ALGORITHMS["curvature"].stack_bands({**sp, "radii": [2, 2, 9]}) == 2   # deduplicated
+    assert ALGORITHMS["openness"].stack_bands({**sp, "radii": [1, 2]}) == 2       # max(2, r) kept
+    assert ALGORITHMS["ambient_occlusion"].stack_bands({**sp, "radii": None}) == 6
+    assert ALGORITHMS["hillshade"].stack_bands({**sp, "radii": [2, 8], "agg": "mean"}) is None
+    assert ALGORITHMS["hillshade"].stack_bands(dict(mode="local", agg="stack")) is None
+    assert ALGORITHMS["hillshade"].stack_bands(dict(mode="local", agg="stack",
+                                                    radii=[2, 9])) == 2
+    assert ALGORITHMS["slope"].stack_bands(dict(mode="local", agg="stack", radii=[2, 9])) is None
+    assert ALGORITHMS["topousm_fast"].stack_bands({**sp, "radii": [2, 8]}) is None
+
+
+def test_band_stack_output_spec():
+    import torch
+    from fujishadergpu_amd.algorithms import OutSpec
+    t = torch.zeros((3, 4, 5))
+    s = OutSpec(f32=t)
+    assert s.n_bands == 3 and s.band(1).f32.shape == (4, 5) and s.band(1).n_bands is None
+    with pytest.raises(ValueError):
+        s.descriptors()
