@@ -22,6 +22,8 @@
 // along each row segment.  Results are staged in LDS and written coalesced.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "usm.hpp"
@@ -1383,6 +1385,51 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
   // whose results are identical on NaN-free windows.
   if (fast) return dispatch_fast(A, /*nanaw=*/p->nan_hint != 0, n_blocks, s);
   return dispatch_usm(A, /*nanaw=*/p->nan_hint != 0, n_blocks, s);
+}
+
+// ------------------------------------------------ no-allocation entry point
+// The workspace of fsg_topousm_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes its name to a
+// kernel: the name is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress, once per device, kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[FSG_TOPOUSM_SCRATCH_BYTES];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  if (int rc = check_hip(hipGetDevice(&dev), "fsg_topousm_noalloc: hipGetDevice")) return rc;
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(FSG_E_UNSUPPORTED, "fsg_topousm_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (int rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)),
+                           "fsg_topousm_noalloc: hipGetSymbolAddress"))
+      return rc;
+    if (!ptr) return fail(FSG_E_HIP, "fsg_topousm_noalloc: scratch has no device address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return FSG_OK;
+}
+
+extern "C" int fsg_topousm_noalloc(const fsg_window* win, const fsg_region* reg,
+                                   const fsg_topousm_params* p, const fsg_output* out,
+                                   const fsg_output* out2, void* stream) {
+  if (!win || !p) return fail(FSG_E_INVALID, "fsg_topousm_noalloc: window/params NULL");
+  const size_t need = fsg_topousm_workspace_size(win, p);
+  if (need > (size_t)FSG_TOPOUSM_SCRATCH_BYTES)
+    return fail(FSG_E_WORKSPACE,
+                "fsg_topousm_noalloc: this window needs " + std::to_string(need) +
+                    " bytes of coarse-grid scratch, above the fixed maximum of " +
+                    std::to_string((size_t)FSG_TOPOUSM_SCRATCH_BYTES) +
+                    " (use fsg_topousm with a workspace)");
+  void* ws = nullptr;
+  if (int rc = scratch_for_current_device(&ws)) return rc;
+  return fsg_topousm(win, reg, p, out, out2, ws, (size_t)FSG_TOPOUSM_SCRATCH_BYTES, stream);
 }
 
 extern "C" size_t fsg_topousm_coarse_means_workspace_size(const fsg_window* win, int32_t ds) {

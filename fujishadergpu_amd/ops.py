@@ -351,6 +351,15 @@ def topousm(win_t, reg, params: N.fsg_topousm_params, out: N.fsg_output,
                           buf.data_ptr(), buf.numel(), _stream()))
 
 
+def topousm_noalloc(win_t, reg, params: N.fsg_topousm_params, out: N.fsg_output,
+                    out2: Optional[N.fsg_output] = None) -> None:
+    """topousm without a workspace: the library's fixed per-GPU scratch array
+    holds the coarse grids (no allocation; overlapping calls on one GPU race)."""
+    w = window(win_t)
+    N.check(_lib().fsg_topousm_noalloc(C.byref(w), C.byref(reg), C.byref(params), C.byref(out),
+                                       C.byref(out2) if out2 is not None else None, _stream()))
+
+
 def block_mean(src_t: torch.Tensor, factor: int) -> torch.Tensor:
     h, w = src_t.shape
     oh, ow = max(1, -(-h // factor)), max(1, -(-w // factor))

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 15
+#define FSG_ABI_VERSION 16
 
 enum {
   FSG_OK = 0,
@@ -308,6 +308,23 @@ int fsg_topousm(const fsg_window* win, const fsg_region* reg,
                 const fsg_topousm_params* p, const fsg_output* out,
                 const fsg_output* out2, /* optional second encoding */
                 void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * fsg_topousm without a workspace argument and without any allocation, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, hosts that forbid hipMalloc).  No counterpart in the
+ * reference.  The coarse grids live in a library-owned device array of
+ * FSG_TOPOUSM_SCRATCH_BYTES (64 MiB) per GPU, held until the process ends;
+ * a window whose fsg_topousm_workspace_size exceeds it is rejected with
+ * FSG_E_WORKSPACE.  Every call on one GPU shares that array (each call
+ * re-initialises what it reads), so calls that overlap on one GPU -- from
+ * two streams or two host threads -- race; serialise them on one stream.
+ * Results equal fsg_topousm's bit for bit.
+ */
+#define FSG_TOPOUSM_SCRATCH_BYTES (64u << 20)
+int fsg_topousm_noalloc(const fsg_window* win, const fsg_region* reg,
+                        const fsg_topousm_params* p, const fsg_output* out,
+                        const fsg_output* out2, void* stream);
 
 /*
  * The coarse means fsg_topousm builds per window for its decimated radii,
