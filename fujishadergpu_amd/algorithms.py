@@ -139,7 +139,7 @@ def openness_table(num_directions, max_distance, pixel_size=1.0, pixel_scale_x=N
 
 
 def ao_table(num_samples, radius, pixel_size=1.0, pixel_scale_x=None, pixel_scale_y=None):
-    """Ring table of compute_ambient_occlusion_block (_impl_ambient_occlusion.py:288-339)."""
+    """Ring table of compute_ambient_occlusion_block (_impl_ambient_occlusion.py:54-81)."""
     angles = np.linspace(0, 2 * np.pi, int(num_samples), endpoint=False)
     dirs = np.stack([np.cos(angles), np.sin(angles)], axis=1)
     sx, sy = ray_steps(pixel_size, pixel_scale_x, pixel_scale_y)

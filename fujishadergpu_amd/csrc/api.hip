@@ -130,15 +130,20 @@ using namespace fsg;
 
 // ------------------------------------------------------------------ kernels
 
+// Launches that map rows to blockIdx.y cap gridDim.y here and stride over the
+// rest (gridDim.y is limited to 65535).
+constexpr int kMaxGridRows = 32768;
+
 __global__ void k_nodata_to_nan(float* __restrict__ ras, int64_t ld, int h, int w,
                                 int has_nodata, double nodata) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (c >= w || r >= h) return;
-  float* p = ras + (int64_t)r * ld + c;
-  const float v = *p;
-  if (has_nodata && !isnan(v)) {
-    if (fabs((double)v - nodata) <= 1e-6) *p = __builtin_nanf("");
+  if (c >= w) return;
+  for (int r = blockIdx.y; r < h; r += gridDim.y) {   // grid-stride over rows
+    float* p = ras + (int64_t)r * ld + c;
+    const float v = *p;
+    if (has_nodata && !isnan(v)) {
+      if (fabs((double)v - nodata) <= 1e-6) *p = __builtin_nanf("");
+    }
   }
 }
 
@@ -175,18 +180,18 @@ k_count_nan(const float* __restrict__ ras, int64_t ld, WinChunk wins, int n_wind
 __global__ void k_copy_region(const float* __restrict__ src, int64_t ld, int r0, int c0,
                               int h, int w, float* __restrict__ dst) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (c >= w || r >= h) return;
-  dst[(int64_t)r * w + c] = src[(int64_t)(r0 + r) * ld + c0 + c];
+  if (c >= w) return;
+  for (int r = blockIdx.y; r < h; r += gridDim.y)
+    dst[(int64_t)r * w + c] = src[(int64_t)(r0 + r) * ld + c0 + c];
 }
 
 __global__ void k_encode(const float* __restrict__ src, int64_t ld, int r0, int c0, int h, int w,
                          OutDev out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (c >= w || r >= h) return;
+  if (c >= w) return;
   const PostK pp = load_post(out);
-  store_out(out, r, c, apply_post(pp, src[(int64_t)(r0 + r) * ld + c0 + c]));
+  for (int r = blockIdx.y; r < h; r += gridDim.y)
+    store_out(out, r, c, apply_post(pp, src[(int64_t)(r0 + r) * ld + c0 + c]));
 }
 
 // NaN-aware f x f block mean: sum in f64 (exact for cell sizes used here),
@@ -194,29 +199,24 @@ __global__ void k_encode(const float* __restrict__ src, int64_t ld, int r0, int 
 __global__ void k_block_mean(const float* __restrict__ src, int64_t ld, int h, int w, int f,
                              float* __restrict__ out, int64_t ldo, int oh, int ow) {
   const int oc = blockIdx.x * blockDim.x + threadIdx.x;
-  const int orow = blockIdx.y;
-  if (oc >= ow || orow >= oh) return;
-  const int r0 = orow * f, c0 = oc * f;
-  const int r1 = min(r0 + f, h), c1 = min(c0 + f, w);
-  double tot = 0.0;
-  int cnt = 0;
-  for (int r = r0; r < r1; r++) {
-    const float* row = src + (int64_t)r * ld;
-    for (int c = c0; c < c1; c++) {
-      const float v = row[c];
-      if (isfinite(v)) {
-        tot += (double)v;
-        cnt++;
+  if (oc >= ow) return;
+  for (int orow = blockIdx.y; orow < oh; orow += gridDim.y) {   // grid-stride over rows
+    const int r0 = orow * f, c0 = oc * f;
+    const int r1 = min(r0 + f, h), c1 = min(c0 + f, w);
+    double tot = 0.0;
+    int cnt = 0;
+    for (int r = r0; r < r1; r++) {
+      const float* row = src + (int64_t)r * ld;
+      for (int c = c0; c < c1; c++) {
+        const float v = row[c];
+        if (isfinite(v)) {
+          tot += (double)v;
+          cnt++;
+        }
       }
     }
+    out[(int64_t)orow * ldo + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
   }
-  float res;
-  if (cnt > 0) {
-    res = (float)tot / (float)cnt;
-  } else {
-    res = __builtin_nanf("");
-  }
-  out[(int64_t)orow * ldo + oc] = res;
 }
 
 // Power-of-two factor: lanes sum their column over the f rows, then groups of
@@ -224,7 +224,8 @@ __global__ void k_block_mean(const float* __restrict__ src, int64_t ld, int h, i
 __global__ void __launch_bounds__(256)
 k_block_mean_p2(const float* __restrict__ src, int64_t ld, int h, int w, int f,
                 float* __restrict__ out, int64_t ldo, int oh, int ow) {
-  const int orow = blockIdx.y;
+  // grid-stride over output rows (block-uniform, so the shuffles see every lane)
+  for (int orow = blockIdx.y; orow < oh; orow += gridDim.y) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   const int r0 = orow * f, r1 = min(r0 + f, h);
   double tot = 0.0;
@@ -247,6 +248,7 @@ k_block_mean_p2(const float* __restrict__ src, int64_t ld, int h, int w, int f,
     if (oc < ow && orow < oh)
       out[(int64_t)orow * ldo + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
   }
+  }
 }
 
 // Power-of-two factor >= 4 on 16-byte aligned rows: each lane loads a float4
@@ -255,7 +257,8 @@ k_block_mean_p2(const float* __restrict__ src, int64_t ld, int h, int w, int f,
 __global__ void __launch_bounds__(256)
 k_block_mean_p2v(const float* __restrict__ src, int64_t ld, int h, int w, int f,
                  float* __restrict__ out, int64_t ldo, int oh, int ow) {
-  const int orow = blockIdx.y;
+  // grid-stride over output rows (block-uniform, so the shuffles see every lane)
+  for (int orow = blockIdx.y; orow < oh; orow += gridDim.y) {
   const int c = (blockIdx.x * 256 + threadIdx.x) * 4;
   const int r0 = orow * f, r1 = min(r0 + f, h);
   double tot = 0.0;
@@ -286,6 +289,7 @@ k_block_mean_p2v(const float* __restrict__ src, int64_t ld, int h, int w, int f,
     const int oc = c / f;
     if (oc < ow && orow < oh)
       out[(int64_t)orow * ldo + oc] = cnt > 0 ? (float)tot / (float)cnt : __builtin_nanf("");
+  }
   }
 }
 
@@ -474,7 +478,7 @@ extern "C" int fsg_nodata_to_nan(float* raster, int64_t ld, int32_t h, int32_t w
     return fail(FSG_E_INVALID, "fsg_nodata_to_nan: bad windows");
   hipStream_t s = as_stream(stream);
   if (has_nodata && h > 0 && w > 0) {
-    dim3 blk(256), grd((w + 255) / 256, h);
+    dim3 blk(256), grd((w + 255) / 256, std::min(h, kMaxGridRows));
     if (std::isnan(nodata)) has_nodata = 0;  // NaN sentinel: isnan only
     if (has_nodata) {
       hipLaunchKernelGGL(k_nodata_to_nan, grd, blk, 0, s, raster, ld, h, w, has_nodata, nodata);
@@ -511,7 +515,7 @@ extern "C" int fsg_copy_region(const fsg_window* win, const fsg_region* reg, flo
   if (rc) return rc;
   if (reg->h == 0 || reg->w == 0) return FSG_OK;
   if (!dst) return fail(FSG_E_INVALID, "fsg_copy_region: dst is NULL");
-  dim3 blk(256), grd((reg->w + 255) / 256, reg->h);
+  dim3 blk(256), grd((reg->w + 255) / 256, std::min(reg->h, kMaxGridRows));
   hipLaunchKernelGGL(k_copy_region, grd, blk, 0, as_stream(stream), win->data, win->ld,
                      reg->r0, reg->c0, reg->h, reg->w, dst);
   FSG_CHECK_LAUNCH("k_copy_region");
@@ -527,7 +531,7 @@ extern "C" int fsg_encode(const fsg_window* src, const fsg_region* reg, const fs
   if (!out) return fail(FSG_E_INVALID, "fsg_encode: output NULL");
   if ((rc = validate_output(out, reg, "fsg_encode"))) return rc;
   if (reg->h == 0 || reg->w == 0) return FSG_OK;
-  dim3 blk(256), grd((reg->w + 255) / 256, reg->h);
+  dim3 blk(256), grd((reg->w + 255) / 256, std::min(reg->h, kMaxGridRows));
   hipLaunchKernelGGL(k_encode, grd, blk, 0, as_stream(stream), src->data, src->ld, reg->r0,
                      reg->c0, reg->h, reg->w, make_out(out));
   FSG_CHECK_LAUNCH("k_encode");
@@ -642,20 +646,20 @@ extern "C" int fsg_block_mean(const fsg_window* src, int32_t factor, float* out,
   if (!out || ld_out < ow) return fail(FSG_E_INVALID, "fsg_block_mean: bad output");
   if (factor >= 4 && factor <= 256 && (factor & (factor - 1)) == 0 && src->ld % 4 == 0 &&
       (reinterpret_cast<uintptr_t>(src->data) & 15) == 0) {
-    dim3 blk(256), grd((src->w + 1023) / 1024, oh);
+    dim3 blk(256), grd((src->w + 1023) / 1024, std::min(oh, kMaxGridRows));
     hipLaunchKernelGGL(k_block_mean_p2v, grd, blk, 0, as_stream(stream), src->data, src->ld,
                        src->h, src->w, factor, out, ld_out, oh, ow);
     FSG_CHECK_LAUNCH("k_block_mean_p2v");
     return FSG_OK;
   }
   if (factor <= 64 && (factor & (factor - 1)) == 0) {
-    dim3 blk(256), grd((src->w + 255) / 256, oh);
+    dim3 blk(256), grd((src->w + 255) / 256, std::min(oh, kMaxGridRows));
     hipLaunchKernelGGL(k_block_mean_p2, grd, blk, 0, as_stream(stream), src->data, src->ld,
                        src->h, src->w, factor, out, ld_out, oh, ow);
     FSG_CHECK_LAUNCH("k_block_mean_p2");
     return FSG_OK;
   }
-  dim3 blk(128), grd((ow + 127) / 128, oh);
+  dim3 blk(128), grd((ow + 127) / 128, std::min(oh, kMaxGridRows));
   hipLaunchKernelGGL(k_block_mean, grd, blk, 0, as_stream(stream), src->data, src->ld, src->h,
                      src->w, factor, out, ld_out, oh, ow);
   FSG_CHECK_LAUNCH("k_block_mean");

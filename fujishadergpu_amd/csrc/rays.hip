@@ -192,6 +192,11 @@ k_openness_points(WinDevR W, const int2* __restrict__ pts, int n, OpenTab T,
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int2 q = pts[e];
+  // a point outside the window reads nothing and gets NaN
+  if (q.x < 0 || q.x >= W.h || q.y < 0 || q.y >= W.w) {
+    out[e] = __builtin_nanf("");
+    return;
+  }
   const float z = W.p[(int64_t)q.x * W.ld + q.y];
   out[e] = isnan(z) ? __builtin_nanf("")
                     : openness_value(T, openness_angle_at(W, q.x, q.y, z, T, false));

@@ -111,7 +111,8 @@ __device__ __forceinline__ void decimate_block(const WinDev& W, int f, float* __
 
 __global__ void __launch_bounds__(256)
 k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __restrict__ nan_flag) {
-  decimate_block(W, f, out, oh, ow, nan_flag, blockIdx.x, blockIdx.y);
+  for (int orow = blockIdx.y; orow < oh; orow += gridDim.y)   // grid-stride over rows
+    decimate_block(W, f, out, oh, ow, nan_flag, blockIdx.x, orow);
 }
 
 // The same cell means for f >= 4 with four source columns per lane: f rows of
@@ -121,7 +122,8 @@ k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __rest
 template <bool VEC>
 __global__ void __launch_bounds__(256)
 k_decimate4(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __restrict__ nan_flag) {
-  const int orow = blockIdx.y;
+  // grid-stride over output rows (block-uniform, so the shuffles see every lane)
+  for (int orow = blockIdx.y; orow < oh; orow += gridDim.y) {
   const int c = (blockIdx.x * 256 + threadIdx.x) * 4;  // first of this lane's source columns
   const int r0 = orow * f, r1 = min(r0 + f, W.h);
   double tot = 0.0;
@@ -161,6 +163,7 @@ k_decimate4(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __res
       }
       out[(int64_t)orow * ow + oc] = res;
     }
+  }
   }
 }
 
@@ -564,20 +567,21 @@ __global__ void __launch_bounds__(64) k_box_plain_tab(BoxTab T) {
 __global__ void k_gauss9_grid(const float* __restrict__ x, int64_t ldx, int h, int w,
                               float* __restrict__ out, int64_t ldo, Gauss9 G) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (c >= w || r >= h) return;
-  out[(int64_t)r * ldo + c] = gauss9_mean(x, ldx, h, w, r, c, G);
+  if (c >= w) return;
+  for (int r = blockIdx.y; r < h; r += gridDim.y)
+    out[(int64_t)r * ldo + c] = gauss9_mean(x, ldx, h, w, r, c, G);
 }
 
 // field = (first ? 0 : field) + f32(w) * mean, in radius order
 __global__ void k_field_accum(float* __restrict__ field, int64_t ldf, const float* __restrict__ m,
                               int64_t ldm, int h, int w, float wt, int first) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (c >= w || r >= h) return;
-  const float t = wt * m[(int64_t)r * ldm + c];
-  float* p = field + (int64_t)r * ldf + c;
-  *p = first ? t : (*p + t);
+  if (c >= w) return;
+  for (int r = blockIdx.y; r < h; r += gridDim.y) {
+    const float t = wt * m[(int64_t)r * ldm + c];
+    float* p = field + (int64_t)r * ldf + c;
+    *p = first ? t : (*p + t);
+  }
 }
 
 // ------------------------------------------------------------ main kernel
@@ -1032,7 +1036,7 @@ static CoarsePlan plan_coarse(const fsg_window* win, const fsg_topousm_params* p
 static int coarse_mean(const float* x, int64_t ldx, int h, int w, int r, float* out, int64_t ldo,
                        float* tv, float* tw, hipStream_t s) {
   if (r <= 1) {
-    dim3 blk(128), grd((w + 127) / 128, h);
+    dim3 blk(128), grd((w + 127) / 128, std::min(h, 32768));
     hipLaunchKernelGGL(k_gauss9_grid, grd, blk, 0, s, x, ldx, h, w, out, ldo, make_gauss9());
     FSG_CHECK_LAUNCH("k_gauss9_grid");
     return FSG_OK;
@@ -1058,7 +1062,7 @@ static int decimate(const fsg_window* win, int ds, float* grid, int hc, int wc, 
   FSG_CHECK_LAUNCH("k_zero_int");
   WinDev W{win->data, win->ld, win->h, win->w};
   if (ds >= 4) {
-    dim3 blk(256), grd((win->w + 1023) / 1024, hc);
+    dim3 blk(256), grd((win->w + 1023) / 1024, std::min(hc, 32768));
     const bool vec = win->ld % 4 == 0 && (reinterpret_cast<uintptr_t>(win->data) & 15) == 0;
     if (vec)
       hipLaunchKernelGGL(k_decimate4<true>, grd, blk, 0, s, W, ds, grid, hc, wc, flag);
@@ -1066,7 +1070,7 @@ static int decimate(const fsg_window* win, int ds, float* grid, int hc, int wc, 
       hipLaunchKernelGGL(k_decimate4<false>, grd, blk, 0, s, W, ds, grid, hc, wc, flag);
     FSG_CHECK_LAUNCH("k_decimate4");
   } else {
-    dim3 blk(256), grd((win->w + 255) / 256, hc);
+    dim3 blk(256), grd((win->w + 255) / 256, std::min(hc, 32768));
     hipLaunchKernelGGL(k_decimate, grd, blk, 0, s, W, ds, grid, hc, wc, flag);
     FSG_CHECK_LAUNCH("k_decimate");
   }
@@ -1662,14 +1666,15 @@ struct FieldTerms {
 
 __global__ void k_field_sum(FieldTerms T, float* __restrict__ out, int64_t ldo, int h, int w) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (c >= w || r >= h) return;
-  float acc = T.wt[0] * T.m[0][(int64_t)r * T.ld[0] + c];
-  for (int i = 1; i < T.n; i++) {
-    const float t = T.wt[i] * T.m[i][(int64_t)r * T.ld[i] + c];
-    acc = acc + t;
+  if (c >= w) return;
+  for (int r = blockIdx.y; r < h; r += gridDim.y) {
+    float acc = T.wt[0] * T.m[0][(int64_t)r * T.ld[0] + c];
+    for (int i = 1; i < T.n; i++) {
+      const float t = T.wt[i] * T.m[i][(int64_t)r * T.ld[i] + c];
+      acc = acc + t;
+    }
+    out[(int64_t)r * ldo + c] = acc;
   }
-  out[(int64_t)r * ldo + c] = acc;
 }
 
 extern "C" int fsg_topousm_field_from_means(const float* const* means, const int64_t* ld_means,
@@ -1688,7 +1693,7 @@ extern "C" int fsg_topousm_field_from_means(const float* const* means, const int
     T.ld[i] = ld_means[i];
     T.wt[i] = weights[i];
   }
-  dim3 blk(256), grd((w + 255) / 256, h);
+  dim3 blk(256), grd((w + 255) / 256, std::min(h, 32768));
   hipLaunchKernelGGL(k_field_sum, grd, blk, 0, as_stream(stream), T, out, ld_out, h, w);
   FSG_CHECK_LAUNCH("k_field_sum");
   return FSG_OK;
@@ -1721,7 +1726,7 @@ extern "C" int fsg_topousm_coarse_field(const fsg_window* coarse, int32_t n_radi
     const int r = std::max(1, (int)radii_coarse[i]);
     rc = coarse_mean(coarse->data, coarse->ld, h, w, r, mean, w, tv, tw, s);
     if (rc) return rc;
-    dim3 blk(128), grd((w + 127) / 128, h);
+    dim3 blk(128), grd((w + 127) / 128, std::min(h, 32768));
     hipLaunchKernelGGL(k_field_accum, grd, blk, 0, s, out, ld_out, mean, (int64_t)w, h, w,
                        weights[i], i == 0 ? 1 : 0);
     FSG_CHECK_LAUNCH("k_field_accum");

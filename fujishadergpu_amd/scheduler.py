@@ -302,6 +302,19 @@ def dask_edge_depth(algorithm: str, params: dict) -> int:
         "decimation grids to its own chunks, which the tile windows do not reproduce)")
 
 
+def check_results(results: Sequence[TileResult]) -> None:
+    """The end-of-run policy of process_dem_tiles (core/tile_processor.py
+    :1689-1695): any failed tile aborts the run with the first three error
+    messages; a run in which no tile succeeded (all skipped) raises
+    ValueError."""
+    errors = [r for r in results if not r.success and not r.skipped_reason]
+    if errors:
+        detail = "\n".join(f"Tile ({r.tile_y}, {r.tile_x}): {r.error_message}" for r in errors[:3])
+        raise RuntimeError(f"Tile processing error:\n{detail}")
+    if not any(r.success for r in results):
+        raise ValueError("No tiles were processed")
+
+
 def _run_tile(algo, src: torch.Tensor, win_geom, core_geom, pads, spec, p) -> None:
     """One tile on the current stream.  With edge_mode='reflect' the padded
     copy of an edge window is allocated and filled here, on the same stream as
@@ -492,12 +505,7 @@ def process_dem(raster: torch.Tensor, algorithm: str, *, nodata: Optional[float]
             results.append(TileResult(ty, tx, False, error_message=str(exc)))
     if ring is not None:
         ring.join()
-    errors = [r for r in results if not r.success and not r.skipped_reason]
-    if errors:
-        detail = "\n".join(f"Tile ({r.tile_y}, {r.tile_x}): {r.error_message}" for r in errors[:3])
-        raise RuntimeError(f"Tile processing error:\n{detail}")
-    if not any(r.success for r in results):
-        raise ValueError("No tiles were processed")
+    check_results(results)
     if return_plan:
         return out, results, P
     return out, results

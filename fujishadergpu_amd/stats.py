@@ -211,8 +211,16 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
             ring.join()
     if algorithm == "topousm_fast":
         pct, nval, mom = ops.percentile(pool, [99.0], ops.PCT_ABS)
+        if not raster_nan_free and int(nval.item()) == 0:
+            # every pooled value is NoData: the reference pools nothing and
+            # returns None (_norm_stats.py:280-283), so the main pass falls
+            # back to per-tile statistics
+            return (None, pool) if return_pool else None
         stats = ops.finalize_stats(0, pct, mom, nval)
     else:
+        # openness / AO: an empty or degenerate pool gives a zero scale, the
+        # no-op stretch the reference's None leaves (_norm_stats.py:286-289,
+        # tile/dask_bridge.py:173-187)
         pct, nval, mom = ops.percentile(pool, [1.0, 99.0], ops.PCT_FINITE)
         stats = ops.finalize_stats(1, pct, mom, nval)
     if return_pool:

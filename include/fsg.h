@@ -213,7 +213,8 @@ int fsg_openness(const fsg_window* win, const fsg_region* reg,
                  void* stream);
 
 /* The same openness at n window pixels (device int32 [n][2] = row, col) into
-   float32 out[n] (the surrogate's sampled exact check). */
+   float32 out[n] (the surrogate's sampled exact check).  A point outside the
+   window (row not in [0, h) or col not in [0, w)) reads nothing and gets NaN. */
 int fsg_openness_points(const fsg_window* win, const int32_t* points, int32_t n,
                         const fsg_openness_params* p, float* out, void* stream);
 
@@ -235,7 +236,7 @@ int fsg_openness_judge(const fsg_window* win, const fsg_openness_params* p, cons
                        size_t workspace_bytes, void* stream);
 
 /*
- * Ambient-occlusion sample table (_impl_ambient_occlusion.py:288-339): ring
+ * Ambient-occlusion sample table (_impl_ambient_occlusion.py:54-81): ring
  * r (0..3) sample i holds the np.round offsets, skip when both are 0, the
  * physical distance and the ring's distance factor 1 - 0.3*r_factor.
  */

@@ -592,7 +592,7 @@ def openness(block, *, openness_type="positive", num_directions=16, max_distance
 def ao_sample_table(num_samples=16, radius=10.0, pixel_size=1.0,
                     pixel_scale_x=None, pixel_scale_y=None):
     """Ring sample table of compute_ambient_occlusion_block
-    (_impl_ambient_occlusion.py:288-339).  Returns [(dx, dy, phys, df)|None]."""
+    (_impl_ambient_occlusion.py:54-81).  Returns [(dx, dy, phys, df)|None]."""
     angles = np.linspace(0, 2 * np.pi, num_samples, endpoint=False)
     directions = np.stack([np.cos(angles), np.sin(angles)], axis=1)
     sx, sy = _abs_steps(pixel_size, pixel_scale_x, pixel_scale_y)

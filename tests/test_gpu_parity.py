@@ -474,3 +474,43 @@ def test_nodata_to_nan_and_counts(cuda):
     mask = torch.isnan(data).cpu().numpy()
     assert mask.tolist() == [[False, True, False], [True, False, False]]
     assert counts.cpu().tolist() == [2, 1]
+
+
+def test_row_mapped_launches_cover_rasters_taller_than_the_grid_limit(cuda):
+    # 140001 rows (more than the 65535 blocks gridDim.y allows): NoData
+    # conversion with a sentinel, region copy, encode and the block mean
+    # stride over the rows beyond their capped grid
+    from fujishadergpu_amd import _native as N
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.encoding import quantize_params
+    H, W = 140001, 37
+    host = torch.arange(H * W, dtype=torch.float32).reshape(H, W) % 1013.0
+    host[::97, 5] = -9999.0
+    g = host.to(cuda)
+    counts = ops.nodata_to_nan(g, -9999.0, [(0, 0, H, W), (70000, 0, 70001, W)])
+    want = host.clone()
+    want[want == -9999.0] = float("nan")
+    assert torch.equal(torch.isnan(g).cpu(), torch.isnan(want))
+    assert counts.cpu().tolist() == [int(torch.isnan(want).sum()),
+                                     int(torch.isnan(want[70000:]).sum())]
+    reg = ops.region(3, 2, H - 5, 30)
+    dst = torch.empty((H - 5, 30), dtype=torch.float32, device=cuda)
+    ops.copy_region(g, reg, dst)
+    assert torch.equal(torch.nan_to_num(dst.cpu(), nan=-1.0),
+                       torch.nan_to_num(want[3:H - 2, 2:32], nan=-1.0))
+    qp = quantize_params(0.0, 1012.0, "uint8")
+    u8 = torch.empty((H - 5, 30), dtype=torch.uint8, device=cuda)
+    ops.encode(g, reg, ops.output(u8, qp=qp))
+    exp = ref.quantize(want[3:H - 2, 2:32].numpy(), qp["a_coef"], qp["b_coef"], qp["dn_min"],
+                       qp["dn_max"], "uint8")
+    assert np.array_equal(u8.cpu().numpy(), exp)
+    bm = ops.block_mean(g, 2)                        # 70001 output rows
+    cells = want[:140000, :36].reshape(70000, 2, 18, 2).double()
+    assert bm.shape == (70001, 19)
+    m = torch.isnan(cells).logical_not()
+    tot = torch.where(m, cells, torch.zeros_like(cells)).sum(dim=(1, 3))
+    cnt = m.sum(dim=(1, 3))
+    exp_bm = torch.where(cnt > 0, tot.float() / cnt.float(), torch.tensor(float("nan")))
+    got_bm = bm[:70000, :18].cpu()
+    assert torch.equal(torch.isnan(got_bm), torch.isnan(exp_bm))
+    assert torch.equal(torch.nan_to_num(got_bm), torch.nan_to_num(exp_bm))

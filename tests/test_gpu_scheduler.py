@@ -285,3 +285,53 @@ def test_fused_scan_path_equals_the_count_path(cuda, nan_discs):
     assert torch.equal(na, nb)
     assert bool((na.any()) == bool(nan_discs))
     assert torch.equal(a[~na], b[~nb])
+
+
+def test_a_failing_tile_aborts_the_run(cuda, monkeypatch):
+    # a tile whose compute raises becomes an error record; the run then
+    # aborts with the first three messages (core/tile_processor.py:1022-1024,
+    # 1689-1692), after every other tile ran
+    from fujishadergpu_amd import algorithms
+    from fujishadergpu_amd.scheduler import process_dem
+    host_t = _raster(600, 700, 37)
+    calls = []
+    real = algorithms.SlopeAlgorithm.run_region
+
+    def flaky(self, win, reg, out, p):
+        calls.append(1)
+        if len(calls) in (2, 5, 6, 7):
+            raise RuntimeError(f"planted failure {len(calls)}")
+        return real(self, win, reg, out, p)
+
+    monkeypatch.setattr(algorithms.SlopeAlgorithm, "run_region", flaky)
+    with pytest.raises(RuntimeError) as e:
+        process_dem(host_t.to(cuda), "slope", tile_size=256, pixel_size=10.0, mode="local")
+    assert len(calls) == 9
+    lines = str(e.value).splitlines()
+    assert lines[0] == "Tile processing error:"
+    assert lines[1:] == ["Tile (0, 1): planted failure 2", "Tile (1, 1): planted failure 5",
+                         "Tile (1, 2): planted failure 6"]
+
+
+def test_prepass_with_no_valid_pooled_value_leaves_per_tile_stats(cuda):
+    # valid data only in a 40 px frame: the one stratified window (the whole
+    # raster, margin 40 + 16) holds 10 % valid pixels, but its trimmed
+    # interior is all NoData, so the
+    # reference pools nothing, returns no global stats (_norm_stats.py:280-283)
+    # and every tile estimates its own p99 (tile/dask_bridge.py:261-270)
+    from fujishadergpu_amd.scheduler import process_dem
+    H = W = 1500
+    host_t = _raster(H, W, 81, relief=1500.0)
+    host_t[40:-40, 40:-40] = -9999.0
+    radii, weights = [2, 8, 40], [0.5, 0.3, 0.2]
+    out, results, P = process_dem(host_t.to(cuda), "topousm_fast", tile_size=512,
+                                  pixel_size=5.0, mode="spatial", radii=radii, weights=weights,
+                                  nodata=-9999.0, return_plan=True)
+    assert P.stats is None and "global_stats" not in P.params
+    skipped = [r for r in results if r.skipped_reason]
+    assert [(r.tile_y, r.tile_x) for r in skipped] == [(1, 1)]
+    exp = _oracle_tiles(host_t.numpy(), P, "topousm_fast", -9999.0, stats=None)
+    got = out.cpu().numpy()
+    m = ~np.isnan(exp)
+    assert np.array_equal(np.isnan(got), ~m)
+    assert np.abs(got[m] - exp[m]).max() <= 2e-5

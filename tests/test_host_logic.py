@@ -201,3 +201,20 @@ def test_band_stack_output_spec():
     assert s.n_bands == 3 and s.band(1).f32.shape == (4, 5) and s.band(1).n_bands is None
     with pytest.raises(ValueError):
         s.descriptors()
+
+
+def test_tile_failures_abort_with_the_first_three_messages():
+    # core/tile_processor.py:1689-1695: any failed tile aborts the run; the
+    # message lists the first three failures in tile order
+    from fujishadergpu_amd.scheduler import TileResult, check_results
+    ok = [TileResult(0, 0, True), TileResult(0, 1, True)]
+    check_results(ok + [TileResult(0, 2, False, skipped_reason="NoData covers 100.0%")])
+    bad = [TileResult(1, i, False, error_message=f"boom {i}") for i in range(5)]
+    with pytest.raises(RuntimeError) as e:
+        check_results(ok + bad)
+    msg = str(e.value)
+    assert msg.startswith("Tile processing error:\n")
+    assert msg.splitlines()[1:] == ["Tile (1, 0): boom 0", "Tile (1, 1): boom 1",
+                                    "Tile (1, 2): boom 2"]
+    with pytest.raises(ValueError, match="No tiles were processed"):
+        check_results([TileResult(0, 0, False, skipped_reason="NoData covers 100.0%")])
