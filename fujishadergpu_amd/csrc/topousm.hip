@@ -21,12 +21,14 @@
 // then 256 threads re-partition as K rows x S segments and slide running sums
 // along each row segment.  Results are staged in LDS and written coalesced.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "usm.hpp"
+#include "usm_wave.hpp"
 
 namespace fsg {
 
@@ -931,6 +933,141 @@ static int launch_fast(const UsmArgs& A, int n_tiles, hipStream_t s) {
   return FSG_OK;
 }
 
+// ------------------------------------------------- single-wavefront kernel
+
+// 0: the wavefront kernel wherever it applies; 1: the workgroup kernels only
+static std::atomic<int> g_usm_variant{0};
+
+// Two launches share the blocks: FAST = true takes the blocks whose rows and
+// columns all lie inside the window (16-byte aligned rows, no reflection),
+// FAST = false the others (each exits on the other kind's blocks).  Two
+// kernels, so the interior one keeps its own, smaller register budget.
+template <int NR, int NS, bool FAST, int KR>
+__global__ void __launch_bounds__(64) k_usm_wave(WaveArgs A) {
+  __shared__ double P[4 * WAVE_PB];
+  __shared__ double Hc[(NS > 0 ? NS : 1) * 2 * WAVE_NC];
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * A.per_xcd + (b >> 3);
+  if (tile >= A.n_blocks) return;
+  const int x0 = (tile % A.n_strips) * A.SW;
+  const int y0 = (tile / A.n_strips) * A.SH;
+  const int rows = min(A.SH, A.rh - y0);
+  const int cs = A.c0 + x0 - A.R;
+  const int wy0 = A.r0 + y0;
+  const bool fast = cs >= 0 && cs + WAVE_NC <= A.W && (A.ld & 3) == 0 &&
+                    ((((uintptr_t)(A.win + cs)) & 15) == 0) && wy0 - A.R - 1 >= 0 &&
+                    wy0 + rows - 1 + A.R <= A.H - 1;
+  if (fast != FAST) return;
+  usm_wave_block<NR, NS, FAST, KR>(A, tile, P, Hc);
+}
+
+// Bilinear row coefficients of every source at every region row (the same
+// src_axis arithmetic as the workgroup kernel's staging).
+struct WaveTabArgs {
+  SrcRef src[WAVE_MAX_SRC];
+  int n_src, r0, rh;
+  int4* rowT;
+};
+
+__global__ void __launch_bounds__(256) k_wave_tables(WaveTabArgs T) {
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < T.n_src * T.rh; e += gridDim.x * 256) {
+    const int s = e / T.rh, k = e - s * T.rh;
+    int a, b;
+    double w;
+    src_axis(T.src[s], 1, T.r0 + k, a, b, w);
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, w);
+    T.rowT[e] = make_int4(a, b, (int)(unsigned)(u & 0xffffffffull), (int)(u >> 32));
+  }
+}
+
+// Rows per wavefront block: long enough to amortise the vertical start-up
+// (2R + 1 rows) and its halo re-reads, short enough to give the launch at
+// least ~4 blocks per SIMD.
+static int wave_rows(int rh, int n_strips) {
+  int sh = 128;
+  while (sh > 32 && (long long)n_strips * ((rh + sh - 1) / sh) < 4096) sh /= 2;
+  return sh;
+}
+
+template <int NR, int NS, int KR>
+static int launch_wave(const WaveArgs& A, hipStream_t s) {
+  TimedScope ts("k_usm_main", s);
+  hipLaunchKernelGGL((k_usm_wave<NR, NS, true, KR>), dim3(A.per_xcd * 8), dim3(64), 0, s, A);
+  FSG_CHECK_LAUNCH("k_usm_wave");
+  hipLaunchKernelGGL((k_usm_wave<NR, NS, false, KR>), dim3(A.per_xcd * 8), dim3(64), 0, s, A);
+  FSG_CHECK_LAUNCH("k_usm_wave");
+  return FSG_OK;
+}
+
+// The wavefront kernel handles NaN-free windows in the fast term order with
+// 1..3 box radii up to 32 and at most 2 bilinear sources (with three, its LDS
+// caches hold the occupancy below the workgroup kernel's: measured slower).
+static bool wave_eligible(const UsmArgs& A, bool nanaw) {
+  return g_usm_variant.load(std::memory_order_relaxed) == 0 && !nanaw && A.nr >= 1 &&
+         A.nr <= WAVE_MAX_NR && A.R <= WAVE_MAX_R && A.n_src <= 2;
+}
+
+// Builds the coefficient tables (in tab, the workspace's table area) and
+// launches the wavefront kernel for the arguments of a fast-path call.
+static int dispatch_wave(const UsmArgs& U, int4* tab, hipStream_t s) {
+  WaveArgs A{};
+  A.win = U.W.p;
+  A.ld = U.W.ld;
+  A.H = U.W.h;
+  A.W = U.W.w;
+  A.r0 = U.r0;
+  A.c0 = U.c0;
+  A.rh = U.rh;
+  A.rw = U.rw;
+  A.R = U.R;
+  for (int i = 0; i < U.nr; i++) {
+    A.rad[i] = U.rad[i];
+    A.inv[i] = U.inv[i];
+    A.wt[i] = U.wt[i];
+  }
+  A.nz = U.n_src - (U.field_src >= 0 ? 1 : 0);
+  A.has_field = U.field_src >= 0 ? 1 : 0;
+  for (int j = 0; j < A.nz; j++) A.wt[U.nr + j] = U.wt[U.nr + j];
+  A.wl = U.wl;
+  WaveTabArgs T{};
+  for (int j = 0; j < U.n_src; j++) {
+    A.sp[j] = U.src[j].p;
+    A.sld[j] = U.src[j].ld;
+    A.src[j] = U.src[j];
+    T.src[j] = U.src[j];
+  }
+  A.rowT = tab;
+  A.o1 = U.o1;
+  A.o2 = U.o2;
+  A.SW = WAVE_NC - 2 * U.R;
+  A.n_strips = (U.rw + A.SW - 1) / A.SW;
+  A.SH = wave_rows(U.rh, A.n_strips);
+  A.n_blocks = A.n_strips * ((U.rh + A.SH - 1) / A.SH);
+  A.per_xcd = (A.n_blocks + 7) / 8;
+  if (U.n_src > 0) {
+    T.n_src = U.n_src;
+    T.r0 = U.r0;
+    T.rh = U.rh;
+    T.rowT = tab;
+    const int n = U.n_src * U.rh;
+    hipLaunchKernelGGL(k_wave_tables, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, s, T);
+    FSG_CHECK_LAUNCH("k_wave_tables");
+  }
+  if (U.nr == 3 && U.rad[0] == 2 && U.rad[1] == 8 && U.rad[2] == 32) {
+    if (U.n_src == 0) return launch_wave<3, 0, 1>(A, s);
+    if (U.n_src == 1) return launch_wave<3, 1, 1>(A, s);
+    if (U.n_src == 2) return launch_wave<3, 2, 1>(A, s);
+    if (U.n_src == 3) return launch_wave<3, 3, 1>(A, s);
+  }
+#define FSG_WAVE_CASE(NRV, NSV) \
+  if (U.nr == NRV && U.n_src == NSV) return launch_wave<NRV, NSV, 0>(A, s);
+  FSG_WAVE_CASE(1, 0) FSG_WAVE_CASE(1, 1) FSG_WAVE_CASE(1, 2) FSG_WAVE_CASE(1, 3)
+  FSG_WAVE_CASE(2, 0) FSG_WAVE_CASE(2, 1) FSG_WAVE_CASE(2, 2) FSG_WAVE_CASE(2, 3)
+  FSG_WAVE_CASE(3, 0) FSG_WAVE_CASE(3, 1) FSG_WAVE_CASE(3, 2) FSG_WAVE_CASE(3, 3)
+#undef FSG_WAVE_CASE
+  return fail(FSG_E_UNSUPPORTED, "k_usm_wave: unsupported term counts");
+}
+
 static int dispatch_fast(const UsmArgs& A, bool nanaw, int n_tiles, hipStream_t s) {
 #define FSG_FAST_CASE(N)                                                      \
   case N:                                                                     \
@@ -982,6 +1119,7 @@ struct CoarsePlan {
   int ds_val[FSG_MAX_RADII];
   int ds_of_radius[FSG_MAX_RADII];  // index into ds_val, or -1
   int hc[FSG_MAX_RADII], wc[FSG_MAX_RADII];
+  size_t tab_off = 0;
   size_t grid_off[FSG_MAX_RADII];   // per distinct ds: decimated grid
   size_t mean_off[FSG_MAX_RADII];   // per radius: coarse mean grid
   size_t flag_off;
@@ -1028,6 +1166,10 @@ static CoarsePlan plan_coarse(const fsg_window* win, const fsg_topousm_params* p
     P.tmp_off[k] = off;
     off += align256(sizeof(float) * (size_t)P.max_cells);
   }
+  // bilinear coefficient tables of the wavefront kernel (per source, one
+  // entry per window row and per window column at most)
+  P.tab_off = off;
+  off += align256(sizeof(int4) * (size_t)MAX_SRC * ((size_t)win->h + (size_t)win->w));
   P.total = off;
   return P;
 }
@@ -1387,8 +1529,17 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
   // The NaN-aware variant is exact for every window; nan_hint == 0 (the caller
   // knows the window holds no NaN) selects the cheaper plain-mean variant,
   // whose results are identical on NaN-free windows.
+  if (fast && wave_eligible(A, p->nan_hint != 0))
+    return dispatch_wave(A, (int4*)((char*)workspace + plan_coarse(win, p).tab_off), s);
   if (fast) return dispatch_fast(A, /*nanaw=*/p->nan_hint != 0, n_blocks, s);
   return dispatch_usm(A, /*nanaw=*/p->nan_hint != 0, n_blocks, s);
+}
+
+extern "C" int fsg_topousm_set_variant(int32_t variant) {
+  if (variant != 0 && variant != 1)
+    return fail(FSG_E_INVALID, "fsg_topousm_set_variant: 0 (default) or 1 (workgroup kernels)");
+  g_usm_variant.store(variant, std::memory_order_relaxed);
+  return FSG_OK;
 }
 
 // ------------------------------------------------ no-allocation entry point

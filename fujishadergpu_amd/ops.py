@@ -351,6 +351,13 @@ def topousm(win_t, reg, params: N.fsg_topousm_params, out: N.fsg_output,
                           buf.data_ptr(), buf.numel(), _stream()))
 
 
+def set_topousm_variant(variant: int) -> None:
+    """Main-pass kernel choice of fsg_topousm for this process: 0 (default)
+    the single-wavefront kernel wherever it applies, 1 the workgroup kernels
+    everywhere (identical results; for benchmarks and equality tests)."""
+    N.check(_lib().fsg_topousm_set_variant(int(variant)))
+
+
 def topousm_noalloc(win_t, reg, params: N.fsg_topousm_params, out: N.fsg_output,
                     out2: Optional[N.fsg_output] = None) -> None:
     """topousm without a workspace: the library's fixed per-GPU scratch array

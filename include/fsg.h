@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 16
+#define FSG_ABI_VERSION 17
 
 enum {
   FSG_OK = 0,
@@ -309,6 +309,15 @@ int fsg_topousm(const fsg_window* win, const fsg_region* reg,
                 const fsg_topousm_params* p, const fsg_output* out,
                 const fsg_output* out2, /* optional second encoding */
                 void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Which main-pass kernels fsg_topousm uses (a process-wide setting, for
+ * benchmarks and equality tests; no reference counterpart): 0 (default) the
+ * single-wavefront kernel wherever it applies (NaN-free windows, terms in
+ * ascending radius order, 1-3 box radii up to 32, at most 3 bilinear
+ * sources), 1 the workgroup kernels everywhere.  Results are identical.
+ */
+int fsg_topousm_set_variant(int32_t variant);
 
 /*
  * fsg_topousm without a workspace argument and without any allocation, for

@@ -514,3 +514,60 @@ def test_row_mapped_launches_cover_rasters_taller_than_the_grid_limit(cuda):
     got_bm = bm[:70000, :18].cpu()
     assert torch.equal(torch.isnan(got_bm), torch.isnan(exp_bm))
     assert torch.equal(torch.nan_to_num(got_bm), torch.nan_to_num(exp_bm))
+
+
+@pytest.mark.parametrize("radii,weights,px,shape,reg,field", [
+    # 16384^2 tile geometry in small: 3 box radii, ds-4 radius, overview field
+    ([2, 8, 32, 128], [8 / 15, 4 / 15, 2 / 15, 1 / 15], 5.0, (700, 1100), (160, 160, 380, 780),
+     True),
+    # pre-pass geometry: three decimated radii, no field
+    ([2, 8, 32, 128, 512, 2048], [32 / 63, 16 / 63, 8 / 63, 4 / 63, 2 / 63, 1 / 63], 5.0,
+     (1200, 1300), (200, 230, 800, 840), False),
+    # one and two box radii, odd radius (lanes straddle the output edge),
+    # region at the window edge (reflected columns and rows), ragged strips
+    ([3], [1.0], 10.0, (300, 517), (0, 0, 300, 517), False),
+    ([5, 21], [0.7, 0.3], 10.0, (411, 389), (0, 13, 400, 376), True),
+    ([2, 8, 30], [0.5, 0.3, 0.2], 10.0, (257, 1000), (1, 3, 255, 990), True),
+])
+def test_wave_kernel_equals_the_workgroup_kernel(cuda, radii, weights, px, shape, reg, field):
+    """The single-wavefront main pass and the workgroup kernel give the same
+    bits (float64 prefix differences equal the running sums exactly), and
+    both match the oracle."""
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.algorithms import OutSpec, TopoUSMFastAlgorithm
+    from fujishadergpu_amd.encoding import quantize_params
+    H, W = shape
+    dem = _dem(H, W, 91 + H, relief=3776.0, hurst=0.75)
+    g = dem.to(cuda)
+    p = dict(radii=radii, weights=weights, pixel_size=px, global_stats=(19.5,),
+             _window_nan_free=True)
+    if field:
+        ov = _dem(64, 96, 7, relief=3776.0, hurst=0.75)
+        fld = ops.topousm_coarse_field(ov.to(cuda), [9], [0.03])
+        p.update(_topousm_fast_coarse_field=fld, _topousm_fast_small_radii=radii,
+                 _topousm_fast_small_weights=weights, _topousm_fast_w_large=0.03,
+                 _topousm_fast_full_shape=(H * 3, W * 2), _topousm_fast_field_offset=(H, W // 2))
+    r0, c0, rh, rw = reg
+    qp = quantize_params(-1.176, 1.176, "uint8")
+    outs = []
+    try:
+        for variant in (1, 0):
+            ops.set_topousm_variant(variant)
+            f32 = torch.full((rh, rw), -7.0, dtype=torch.float32, device=cuda)
+            u8 = torch.zeros((rh, rw), dtype=torch.uint8, device=cuda)
+            TopoUSMFastAlgorithm().run_region(g, ops.region(r0, c0, rh, rw),
+                                              OutSpec(f32=f32, q=u8, qp=qp), p)
+            torch.cuda.synchronize()
+            outs.append((f32, u8))
+    finally:
+        ops.set_topousm_variant(0)
+    (a32, a8), (b32, b8) = outs
+    assert torch.equal(a32, b32), float((a32 - b32).abs().max())
+    assert torch.equal(a8, b8)
+    kw = dict(radii=radii, weights=weights, pixel_size=px, stats=(19.5,))
+    if field:
+        kw.update(coarse_field=fld.cpu().numpy(), small_radii=radii, small_weights=weights,
+                  w_large=0.03, field_offset=(H, W // 2), full_shape=(H * 3, W * 2))
+    exp = ref.topousm_direct(dem.numpy(), **kw)[r0:r0 + rh, c0:c0 + rw]
+    mx, _ = _cmp("topousm/wave", b32, exp, 2e-5)
+    print("radii", radii, "max |d| vs oracle", mx, flush=True)
