@@ -938,11 +938,10 @@ static int launch_fast(const UsmArgs& A, int n_tiles, hipStream_t s) {
 // 0: the wavefront kernel wherever it applies; 1: the workgroup kernels only
 static std::atomic<int> g_usm_variant{0};
 
-// Two launches share the blocks: FAST = true takes the blocks whose rows and
-// columns all lie inside the window (16-byte aligned rows, no reflection),
-// FAST = false the others (each exits on the other kind's blocks).  Two
-// kernels, so the interior one keeps its own, smaller register budget.
-template <int NR, int NS, bool FAST, int KR>
+// One launch: the blocks whose rows and columns all lie inside the window
+// (16-byte aligned rows, no reflection) take the FAST = true code, the others
+// the reflecting loads.
+template <int NR, int NS, int KR>
 __global__ void __launch_bounds__(64) k_usm_wave(WaveArgs A) {
   __shared__ double P[4 * WAVE_PB];
   __shared__ double Hc[(NS > 0 ? NS : 1) * 2 * WAVE_NC];
@@ -957,8 +956,10 @@ __global__ void __launch_bounds__(64) k_usm_wave(WaveArgs A) {
   const bool fast = cs >= 0 && cs + WAVE_NC <= A.W && (A.ld & 3) == 0 &&
                     ((((uintptr_t)(A.win + cs)) & 15) == 0) && wy0 - A.R - 1 >= 0 &&
                     wy0 + rows - 1 + A.R <= A.H - 1;
-  if (fast != FAST) return;
-  usm_wave_block<NR, NS, FAST, KR>(A, tile, P, Hc);
+  if (fast)
+    usm_wave_block<NR, NS, true, KR>(A, tile, P, Hc);
+  else
+    usm_wave_block<NR, NS, false, KR>(A, tile, P, Hc);
 }
 
 // Bilinear row coefficients of every source at every region row (the same
@@ -980,31 +981,53 @@ __global__ void __launch_bounds__(256) k_wave_tables(WaveTabArgs T) {
   }
 }
 
-// Rows per wavefront block: long enough to amortise the vertical start-up
-// (2R + 1 rows) and its halo re-reads, short enough to give the launch at
-// least ~4 blocks per SIMD.
-static int wave_rows(int rh, int n_strips) {
-  int sh = 128;
-  while (sh > 32 && (long long)n_strips * ((rh + sh - 1) / sh) < 4096) sh /= 2;
-  return sh;
+// Rows per wavefront block: the fewest that still give every strip's blocks
+// a slot in the launch's first (and only) round of resident blocks -- the
+// blocks of one launch run concurrently and each one's time grows with its
+// rows (plus a 2R + 1 row start-up), so a second, partial round would cost
+// a whole block time.
+static int wave_rows(int rh, int n_strips, int capacity) {
+  const int v = g_usm_variant.load(std::memory_order_relaxed);
+  if (v >= 8) return v;
+  const int segs = std::max(1, capacity / std::max(1, n_strips));
+  return std::max(16, (rh + segs - 1) / segs);
+}
+
+// Resident blocks of one k_usm_wave instance on the current device (CUs x
+// blocks per CU under its registers and LDS).
+template <int NR, int NS, int KR>
+static int wave_capacity() {
+  static const int cap = [] {
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_usm_wave<NR, NS, KR>, 64, 0) !=
+            hipSuccess || nb <= 0) {
+      (void)hipGetLastError();
+      nb = 8;
+    }
+    return cus * nb;
+  }();
+  return cap;
 }
 
 template <int NR, int NS, int KR>
-static int launch_wave(const WaveArgs& A, hipStream_t s) {
+static int launch_wave(WaveArgs A, hipStream_t s) {
+  A.SH = wave_rows(A.rh, A.n_strips, wave_capacity<NR, NS, KR>());
+  A.n_blocks = A.n_strips * ((A.rh + A.SH - 1) / A.SH);
+  A.per_xcd = (A.n_blocks + 7) / 8;
   TimedScope ts("k_usm_main", s);
-  hipLaunchKernelGGL((k_usm_wave<NR, NS, true, KR>), dim3(A.per_xcd * 8), dim3(64), 0, s, A);
-  FSG_CHECK_LAUNCH("k_usm_wave");
-  hipLaunchKernelGGL((k_usm_wave<NR, NS, false, KR>), dim3(A.per_xcd * 8), dim3(64), 0, s, A);
+  hipLaunchKernelGGL((k_usm_wave<NR, NS, KR>), dim3(A.per_xcd * 8), dim3(64), 0, s, A);
   FSG_CHECK_LAUNCH("k_usm_wave");
   return FSG_OK;
 }
 
 // The wavefront kernel handles NaN-free windows in the fast term order with
-// 1..3 box radii up to 32 and at most 2 bilinear sources (with three, its LDS
-// caches hold the occupancy below the workgroup kernel's: measured slower).
+// 1..3 box radii up to 32 and at most 3 bilinear sources.
 static bool wave_eligible(const UsmArgs& A, bool nanaw) {
-  return g_usm_variant.load(std::memory_order_relaxed) == 0 && !nanaw && A.nr >= 1 &&
-         A.nr <= WAVE_MAX_NR && A.R <= WAVE_MAX_R && A.n_src <= 2;
+  return g_usm_variant.load(std::memory_order_relaxed) != 1 && !nanaw && A.nr >= 1 &&
+         A.nr <= WAVE_MAX_NR && A.R <= WAVE_MAX_R && A.n_src <= WAVE_MAX_SRC;
 }
 
 // Builds the coefficient tables (in tab, the workspace's table area) and
@@ -1041,9 +1064,6 @@ static int dispatch_wave(const UsmArgs& U, int4* tab, hipStream_t s) {
   A.o2 = U.o2;
   A.SW = WAVE_NC - 2 * U.R;
   A.n_strips = (U.rw + A.SW - 1) / A.SW;
-  A.SH = wave_rows(U.rh, A.n_strips);
-  A.n_blocks = A.n_strips * ((U.rh + A.SH - 1) / A.SH);
-  A.per_xcd = (A.n_blocks + 7) / 8;
   if (U.n_src > 0) {
     T.n_src = U.n_src;
     T.r0 = U.r0;
@@ -1536,8 +1556,10 @@ extern "C" int fsg_topousm(const fsg_window* win, const fsg_region* reg,
 }
 
 extern "C" int fsg_topousm_set_variant(int32_t variant) {
-  if (variant != 0 && variant != 1)
-    return fail(FSG_E_INVALID, "fsg_topousm_set_variant: 0 (default) or 1 (workgroup kernels)");
+  if (variant != 0 && variant != 1 && !(variant >= 8 && variant <= 1024))
+    return fail(FSG_E_INVALID,
+                "fsg_topousm_set_variant: 0 (default), 1 (workgroup kernels) or 8..1024 "
+                "(wavefront kernel rows per block)");
   g_usm_variant.store(variant, std::memory_order_relaxed);
   return FSG_OK;
 }

@@ -211,15 +211,14 @@ __device__ __forceinline__ void usm_wave_block(const WaveArgs& A, int tile, doub
   const int m_lo = max(0, R - 4 * t);
   const int m_hi = min(4, R + cols - 4 * t);
 
-  // vertical sums for the row before the first output row
+  // vertical sums for the row before the first output row, rows in
+  // ascending order (eight rows' loads in flight at a time)
   double S[NR][4];
 #pragma unroll
   for (int i = 0; i < NR; i++)
 #pragma unroll
     for (int m = 0; m < 4; m++) S[i][m] = 0.0;
-  for (int d = -R; d <= R; d++) {
-    float v[4];
-    load4(wy0 - 1 + d, v);
+  auto add_row = [&](int d, const float v[4]) {
 #pragma unroll
     for (int i = 0; i < NR; i++) {
       const int r = box_radius<KR>(A, i);
@@ -228,6 +227,21 @@ __device__ __forceinline__ void usm_wave_block(const WaveArgs& A, int tile, doub
         for (int m = 0; m < 4; m++) S[i][m] += (double)v[m];
       }
     }
+  };
+  int d = -R;
+#pragma unroll 1
+  for (; d + 7 <= R; d += 8) {
+    float v[8][4];
+#pragma unroll
+    for (int k = 0; k < 8; k++) load4(wy0 - 1 + d + k, v[k]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) add_row(d + k, v[k]);
+  }
+#pragma unroll 1
+  for (; d <= R; d++) {
+    float v[4];
+    load4(wy0 - 1 + d, v);
+    add_row(d, v);
   }
 
   if (t == 0) P[wave_pidx(0)] = 0.0;   // prefix entry k = 0 (the empty sum)
@@ -245,24 +259,43 @@ __device__ __forceinline__ void usm_wave_block(const WaveArgs& A, int tile, doub
   // every lane's four columns are all outputs or none (uniform)
   const bool full4 = (R & 3) == 0 && (cols & 3) == 0 && (x0 & 3) == 0;
 
+  // the rows an output row reads: its own, and the entering and leaving row
+  // of each radius; loaded one output row ahead
+  float zq[4], vq[NR][2][4];
+  auto load_rows = [&](int wy) {
+    load4(wy, zq);
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+      const int r = box_radius<KR>(A, i);
+      load4(wy + r, vq[i][0]);
+      load4(wy - r - 1, vq[i][1]);
+    }
+  };
+  load_rows(wy0);
+
   for (int yy = 0; yy < rows; yy++) {
-    const int wy = wy0 + yy;
-    float z[4];
-    load4(wy, z);
+    float z[4], vn[NR][4], vo[NR][4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      z[m] = zq[m];
+#pragma unroll
+      for (int i = 0; i < NR; i++) {
+        vn[i][m] = vq[i][0][m];
+        vo[i][m] = vq[i][1][m];
+      }
+    }
+    if (yy + 1 < rows) load_rows(wy0 + yy + 1);
     float acc[4];
 #pragma unroll
     for (int i = 0; i < NR; i++) {
       // 1. vertical running sums -> float32 means
       const int r = box_radius<KR>(A, i);
-      float vn[4], vo[4];
-      load4(wy + r, vn);
-      load4(wy - r - 1, vo);
       const double inv = A.inv[i];
       float V[4];
 #pragma unroll
       for (int m = 0; m < 4; m++) {
-        S[i][m] += (double)vn[m];
-        S[i][m] -= (double)vo[m];
+        S[i][m] += (double)vn[i][m];
+        S[i][m] -= (double)vo[i][m];
         V[m] = (float)(S[i][m] * inv);
       }
       // 2. row prefix of the means -> LDS, then the box sums

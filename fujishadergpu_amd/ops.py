@@ -354,7 +354,9 @@ def topousm(win_t, reg, params: N.fsg_topousm_params, out: N.fsg_output,
 def set_topousm_variant(variant: int) -> None:
     """Main-pass kernel choice of fsg_topousm for this process: 0 (default)
     the single-wavefront kernel wherever it applies, 1 the workgroup kernels
-    everywhere (identical results; for benchmarks and equality tests)."""
+    everywhere, 8..1024 the single-wavefront kernel with that many rows per
+    block (identical results; for benchmarks, launch-shape sweeps and
+    equality tests)."""
     N.check(_lib().fsg_topousm_set_variant(int(variant)))
 
 

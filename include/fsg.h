@@ -315,7 +315,9 @@ int fsg_topousm(const fsg_window* win, const fsg_region* reg,
  * benchmarks and equality tests; no reference counterpart): 0 (default) the
  * single-wavefront kernel wherever it applies (NaN-free windows, terms in
  * ascending radius order, 1-3 box radii up to 32, at most 3 bilinear
- * sources), 1 the workgroup kernels everywhere.  Results are identical.
+ * sources), 1 the workgroup kernels everywhere, 8..1024 the single-wavefront
+ * kernel with that many rows per block (launch-shape sweeps).  Results are
+ * identical.
  */
 int fsg_topousm_set_variant(int32_t variant);
 
