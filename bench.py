@@ -198,38 +198,86 @@ def bench_c1(dev):
             "max_abs_diff_vs_oracle": d}
 
 
-def bench_c3(dev, n=32768):
-    """Curvature (mean) and openness (positive, N=8, D=64), local mode, on a
-    32768^2 fBm (H 0.85, 0-2500 m, 10 m, seed 3) with 1% NoData discs (seed 33),
-    tiles 4096 + the local 64 px halo; the tile size is picked by measuring."""
-    import torch
+def _c3_dem(dev, h, w):
     from fujishadergpu_amd import synth
+    dem = synth.fbm(h, w, 0.85, 2500.0, seed=3, device=dev)
+    synth.apply_discs(dem, synth.nodata_discs(h, w, 0.01, seed=33), -9999.0)
+    return dem
+
+
+C3_PRODUCTS = {"curvature": {"curvature_type": "mean", "mode": "local"},
+               "openness": {"openness_type": "positive", "num_directions": 8,
+                            "max_distance": 64, "mode": "local"}}
+
+
+def bench_c3(dev, n=32768):
+    """Curvature (mean) + openness (positive, N=8, D=64), local mode, 10 m
+    cells, on a 32768^2 fBm (H 0.85, 0-2500 m, seed 3) with 1% NoData discs
+    (seed 33): the whole job -- NoData pass, openness stretch pre-pass, and
+    ONE persistent launch over every tile's curvature and openness blocks --
+    captured in a graph and replayed.  The tuner measures the tile size (the
+    halo is each product's padding rule) and the persistent launch shape.
+    The same products through the per-tile scheduler (one launch per tile and
+    product) are timed beside it, and a 30001 x 32767 raster gives the ragged
+    grid (last tile row 1329 px, last column 4095 px) at the 4096 core."""
+    import torch
+    from fujishadergpu_amd.batch import RasterJob, tune_raster_job
     from fujishadergpu_amd.scheduler import process_dem
-    from fujishadergpu_amd.tuner import tune_tile_size
-    dem = synth.fbm(n, n, 0.85, 2500.0, seed=3, device=dev)
-    discs = synth.nodata_discs(n, n, 0.01, seed=33)
-    synth.apply_discs(dem, discs, -9999.0)
-    out = torch.empty_like(dem)
     res = {}
-    for algo, kw in (("curvature", dict(curvature_type="mean")),
-                     ("openness", dict(openness_type="positive", num_directions=8,
-                                       max_distance=64))):
-        def run(ts, _a=algo, _kw=kw):
-            process_dem(dem, _a, tile_size=ts, pixel_size=10.0, nodata=-9999.0, mode="local",
-                        out=out, copy_input=False, **_kw)
-        print(f"#   c3 {algo}: tuning the tile size", file=sys.stderr, flush=True)
-        best, times = tune_tile_size(run, (2048, 4096, 8192), reps=1)
-        t = _gpu_time(lambda: run(best), 2)
-        px = n * n
-        res[algo] = {"gpix_s": round(px / t / 1e9, 3), "ms": round(t * 1e3, 2),
-                     "hbm_frac": round(8 * px / t / (HBM_PEAK_GBS * 1e9), 4),
-                     "alg_bytes_per_px": 8, "tile_size": best,
-                     "tuner_ms": {str(k): round(v * 1e3, 2) for k, v in times.items()}}
-    res["workload"] = (f"{n}x{n} f32 fBm (H=0.85, 0-2500 m, 10 m, seed 3), 1% NoData discs "
-                       "r 5-50 px (seed 33, sentinel -9999), local mode, whole job per algorithm "
-                       "(NoData scan, tiles, openness p1/p99 stretch pre-pass)")
-    del dem, out
+    dem = _c3_dem(dev, n, n)
+    print("#   c3 tuning tile size and launch shape", file=sys.stderr, flush=True)
+    job, rep = tune_raster_job(dem, C3_PRODUCTS, tile_sizes=(2048, 4096, 8192), reps=1,
+                               nodata=-9999.0, pixel_size=10.0)
+    job.capture()
+    t = _gpu_time(job.replay, 3)
+    px = n * n
+    res["graph_job"] = {"gpix_s": round(px / t / 1e9, 3), "ms": round(t * 1e3, 2),
+                        "hbm_frac": round(12 * px / t / (HBM_PEAK_GBS * 1e9), 4),
+                        "alg_bytes_per_px": 12, "tile_size": rep["tile_size"],
+                        "launch_config": list(rep["config"]) if rep["config"] else None,
+                        "tuner_tile_ms": {str(k): round(v, 2) for k, v in rep["tile_ms"].items()},
+                        "tuner_launch_ms": {k: round(v, 2) for k, v in rep["launch_ms"].items()},
+                        "items": job.info()}
+    outs = job.outputs
+    job.close()
+    del job
+    # the per-tile scheduler, product by product, at the same tile size
+    ref_ms = {}
+    for name, kw in C3_PRODUCTS.items():
+        o = torch.empty_like(dem)
+
+        def run(_n=name, _kw=kw, _o=o):
+            process_dem(dem, _n, tile_size=rep["tile_size"], pixel_size=10.0, nodata=-9999.0,
+                        out=_o, copy_input=False, **_kw)
+        run()
+        ref_ms[name] = _gpu_time(run, 2) * 1e3
+        res.setdefault("same_bits_as_per_tile", True)
+        a, b = o, outs[name]
+        same = bool(torch.equal(torch.isnan(a), torch.isnan(b)) and
+                    torch.equal(torch.nan_to_num(a, 0.0), torch.nan_to_num(b, 0.0)))
+        res["same_bits_as_per_tile"] = res["same_bits_as_per_tile"] and same
+        del o
+    tot = sum(ref_ms.values()) / 1e3
+    res["per_tile_launches"] = {"ms": {k: round(v, 2) for k, v in ref_ms.items()},
+                                "gpix_s_both": round(px / tot / 1e9, 3)}
+    del dem, outs
     torch.cuda.empty_cache()
+    # the ragged grid
+    hr, wr = 30001, 32767
+    dem = _c3_dem(dev, hr, wr)
+    job = RasterJob(dem, C3_PRODUCTS, tile_size=4096, nodata=-9999.0, pixel_size=10.0,
+                    config=rep["config"])
+    job.capture()
+    t = _gpu_time(job.replay, 3)
+    res["graph_job_ragged"] = {"shape": [hr, wr], "tile_size": 4096,
+                               "gpix_s": round(hr * wr / t / 1e9, 3), "ms": round(t * 1e3, 2),
+                               "tiles": job.info()["tiles"]}
+    job.close()
+    del dem, job
+    torch.cuda.empty_cache()
+    res["workload"] = (f"{n}x{n} f32 fBm (H=0.85, 0-2500 m, 10 m, seed 3), 1% NoData discs "
+                       "r 5-50 px (seed 33, sentinel -9999), local mode; curvature (mean) + "
+                       "openness (positive, 8 directions, 64 px) as one job")
     return res
 
 

@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 17  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 18  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -112,6 +112,7 @@ class fsg_batch_tile(C.Structure):
         ("win", fsg_window), ("reg", fsg_region), ("hillshade", fsg_output), ("slope", fsg_output),
         ("curvature", fsg_output), ("topousm", fsg_output), ("topousm2", fsg_output),
         ("shade", fsg_shade_params), ("curv", fsg_curvature_params), ("usm", fsg_topousm_params),
+        ("openness", fsg_output), ("open", fsg_openness_params),
     ]
 
 

@@ -1,5 +1,6 @@
 // Batched fused launch: many independent tiles (ragged sizes, each its own
-// window) and, per tile, hillshade + slope + curvature + TopoUSM in ONE
+// window) and, per tile, hillshade + slope + curvature + TopoUSM, or
+// hillshade + slope + curvature (NaN-aware included) + openness, in ONE
 // persistent kernel that drains a device work queue.
 //
 // The reference runs one tile at a time and one algorithm at a time
@@ -7,7 +8,8 @@
 // per-tile, per-algorithm launches are the "unfused" comparison here.
 //
 // Work items are TopoUSM main-pass blocks (SW x FAST_SH of a tile region,
-// the heavier items) and stencil blocks (64 x 16 output pixels).  A stencil
+// the heavier items), openness blocks (64 x 16 pixels, one sparse ray march
+// per pixel: rays.hpp) and stencil blocks (64 x 16 output pixels).  A stencil
 // item stages the block's filled window values with a 3-pixel halo in LDS
 // once, tabulates the first derivatives and derives hillshade, slope and
 // curvature from them (stencil_tile_block, the per-window kernels' block
@@ -22,10 +24,23 @@
 #include <map>
 #include <vector>
 
+#include "rays.hpp"
 #include "stencil.hpp"
 #include "usm.hpp"
 
 namespace fsg {
+
+// One tile's openness region: 64 x OPEN_ITEM_H pixel items.
+constexpr int OPEN_ITEM_H = 16;
+struct OpenTileD {
+  WinDevR W;
+  int r0, c0, rh, rw;
+  int nbx;             // 64-column items per item row
+  OutDev out;
+};
+
+// Item kinds a persistent kernel instance carries (template bit mask).
+constexpr int BK_USM = 1, BK_ST = 2, BK_ST_NAN = 4, BK_OPEN = 8;
 
 struct BatchDev {
   const UsmArgs* usm;
@@ -34,11 +49,15 @@ struct BatchDev {
   const StencilTile* st;
   const int* st_pre;   // prefix of stencil blocks per tile (n + 1)
   int n_st_tiles, n_st_items;
+  const OpenTileD* op;
+  const int* op_pre;   // prefix of openness items per tile (n + 1)
+  int n_op_tiles, n_op_items;
   int* queue;
-  int order;           // 0: TopoUSM items first, 1: stencil items first
+  int order;           // 0: TopoUSM, openness, stencil items; 1: stencil, openness, TopoUSM
   int chunk;           // items claimed per queue access
   int pre_off;         // LDS byte offset of the staged prefix arrays
   Gauss9 G;
+  OpenTab T;           // the batch's one openness ray table
 };
 
 // largest t with pre[t] <= item (pre[0] = 0, pre strictly covers the items)
@@ -51,16 +70,45 @@ __device__ __forceinline__ int find_tile(const int* pre, int n, int item) {
   return lo;
 }
 
-template <int NR>
+// One openness item: rows by*16 .. by*16 + 15, columns bx*64 .. bx*64 + 63 of
+// the tile region, as four 64 x 4 passes of k_openness's block (the same
+// per-block interior test, per-pixel functions and epilogue: the same bits).
+__device__ __forceinline__ void openness_item(const OpenTileD& O, const OpenTab& T, int by,
+                                              int bx) {
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const PostK pp = load_post(O.out);
+  for (int sub = 0; sub < OPEN_ITEM_H / 4; sub++) {
+    const int rb = by * OPEN_ITEM_H + sub * 4;
+    if (rb >= O.rh) break;  // item-uniform
+    const int bi0 = O.r0 + rb, bj0 = O.c0 + bx * 64;
+    const bool interior = openness_interior(O.W, T, bi0, bi0 + 3, bj0, bj0 + 63);
+    const int r = rb + ly, c = bx * 64 + lx;
+    if (r < O.rh && c < O.rw) {
+      const int i = O.r0 + r, j = O.c0 + c;
+      const float z = O.W.p[(int64_t)i * O.W.ld + j];
+      const float v = isnan(z) ? __builtin_nanf("")
+                               : openness_value(T, openness_angle_at(O.W, i, j, z, T, interior));
+      store_out(O.out, r, c, apply_post(pp, v));
+    }
+  }
+}
+
+template <int NR, int KINDS>
 __global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
   extern __shared__ float lds[];
   __shared__ int s_item;
-  const int total = B.n_usm_items + B.n_st_items;
-  // both prefix arrays staged in LDS once: tile lookups stay on chip
+  const int total = B.n_usm_items + B.n_st_items + B.n_op_items;
+  // the prefix arrays staged in LDS once: tile lookups stay on chip
   int* upre = (int*)((char*)lds + B.pre_off);
   int* spre = upre + B.n_usm_tiles + 1;
+  int* opre = spre + B.n_st_tiles + 1;
   for (int e = threadIdx.x; e <= B.n_usm_tiles; e += 256) upre[e] = B.usm_pre[e];
   for (int e = threadIdx.x; e <= B.n_st_tiles; e += 256) spre[e] = B.st_pre[e];
+  if (KINDS & BK_OPEN)
+    for (int e = threadIdx.x; e <= B.n_op_tiles; e += 256) opre[e] = B.op_pre[e];
+  // item groups in queue order: kinds 0 TopoUSM, 1 openness, 2 stencil
+  const int first = B.order ? 2 : 0, last = B.order ? 0 : 2;
+  const int n_first = B.order ? B.n_st_items : B.n_usm_items;
   // every workgroup leaves once the queue passes the item count
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(B.queue, B.chunk);
@@ -70,26 +118,36 @@ __global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
     if (base >= total) break;
     const int end = min(base + B.chunk, total);
     for (int item = base; item < end; item++) {
-      bool is_usm;
-      int it;
-      if (B.order == 0) {
-        is_usm = item < B.n_usm_items;
-        it = is_usm ? item : item - B.n_usm_items;
+      int kind, it = item;
+      if (it < n_first) {
+        kind = first;
+      } else if (it - n_first < B.n_op_items) {
+        kind = 1;
+        it -= n_first;
       } else {
-        is_usm = item >= B.n_st_items;
-        it = is_usm ? item - B.n_st_items : item;
+        kind = last;
+        it -= n_first + B.n_op_items;
       }
-      if (is_usm) {
+      if ((KINDS & BK_USM) && kind == 0) {
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(upre, B.n_usm_tiles, it));
         usm_fast_block<NR, false>(B.usm[tl], it - upre[tl], lds);
-      } else {
+      } else if ((KINDS & BK_OPEN) && kind == 1) {
+        const int tl = __builtin_amdgcn_readfirstlane(find_tile(opre, B.n_op_tiles, it));
+        const OpenTileD& O = B.op[tl];
+        const int blk = it - opre[tl];
+        const int by = blk / O.nbx;
+        openness_item(O, B.T, by, blk - by * O.nbx);
+      } else if (kind == 2) {
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(spre, B.n_st_tiles, it));
         const StencilTile& S = B.st[tl];
         const int blk = it - spre[tl];
         const int by = blk / S.nbx, bx = blk - by * S.nbx;
-        // NaN-free tiles only: the NaN-aware fill (a call) stays out of this
-        // kernel's code; those tiles take their own launches
-        stencil_tile_block<false>(S, by, bx, B.G, lds);
+        // the NaN-aware fill only in instances that carry NaN-aware tiles; in
+        // the others those tiles take their own launches
+        if ((KINDS & BK_ST_NAN) && S.nanaw)
+          stencil_tile_block<true>(S, by, bx, B.G, lds);
+        else
+          stencil_tile_block<false>(S, by, bx, B.G, lds);
       }
     }
   }
@@ -116,8 +174,13 @@ struct fsg_batch {
   std::vector<int> usm_fallback;      // tile indices run with per-tile launches
   std::vector<StencilTile> st_fallback;  // NaN-aware stencil tiles (own launches)
   int nr = -1;
+  int kinds = 0;                      // item kinds of the persistent kernel (BK_*)
   size_t lds = 0;
   int n_usm_items = 0, n_st_items = 0, n_st_tiles = 0;
+  int n_op_items = 0, n_op_tiles = 0, first_open = -1;
+  OpenTab open_tab{};
+  OpenTileD* d_op = nullptr;
+  int* d_op_pre = nullptr;
   char* dev = nullptr;                // one allocation: workspaces, descriptors, queue
   size_t dev_bytes = 0;
   UsmArgs* d_usm = nullptr;
@@ -148,6 +211,19 @@ static bool wants_usm(const fsg_batch_tile& t) {
 }
 static bool wants_stencil(const fsg_batch_tile& t) {
   return t.hillshade.data || t.slope.data || t.curvature.data;
+}
+static bool wants_open(const fsg_batch_tile& t) { return t.openness.data != nullptr; }
+
+// Two ray tables are the same table (a batch holds one).
+static bool same_open_table(const fsg_openness_params& a, const fsg_openness_params& b) {
+  if (a.n_dirs != b.n_dirs || a.n_dist != b.n_dist || a.openness_type != b.openness_type ||
+      a.raw_degrees != b.raw_degrees)
+    return false;
+  for (int e = 0; e < a.n_dirs * a.n_dist; e++) {
+    const fsg_ray_sample &x = a.samples[e], &y = b.samples[e];
+    if (x.dx != y.dx || x.dy != y.dy || x.phys != y.phys || x.skip != y.skip) return false;
+  }
+  return true;
 }
 static const fsg_output* opt(const fsg_output& o) { return o.data ? &o : nullptr; }
 
@@ -236,8 +312,17 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
     int rc = validate_window(&t.win, "fsg_batch_create");
     if (!rc) rc = validate_region(&t.win, &t.reg, "fsg_batch_create");
     for (const fsg_output* o : {opt(t.hillshade), opt(t.slope), opt(t.curvature), opt(t.topousm),
-                                opt(t.topousm2)})
+                                opt(t.topousm2), opt(t.openness)})
       if (!rc) rc = validate_output(o, &t.reg, "fsg_batch_create");
+    if (!rc && wants_open(t)) {
+      OpenTab T;
+      rc = openness_table(&t.open, T, "fsg_batch_create");
+      if (!rc && b->n_op_tiles == 0) b->open_tab = T;
+      if (!rc && b->n_op_tiles > 0 && !same_open_table(t.open, b->tiles[b->first_open].open))
+        rc = fail(FSG_E_UNSUPPORTED, "fsg_batch_create: the openness tiles of one batch share "
+                                     "one ray table");
+      if (!rc && b->n_op_tiles++ == 0) b->first_open = i;
+    }
     if (!rc && wants_stencil(t)) {
       if (t.win.h < 3 || t.win.w < 3)
         rc = fail(FSG_E_INVALID, "fsg_batch_create: window smaller than 3 x 3");
@@ -259,7 +344,8 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   const size_t ws_total = off;
   const size_t usm_bytes = al256(sizeof(UsmArgs) * n), pre_bytes = al256(sizeof(int) * (n + 1));
   const size_t st_bytes = al256(sizeof(StencilTile) * n);
-  b->dev_bytes = ws_total + usm_bytes + 2 * pre_bytes + st_bytes + 256;
+  const size_t op_bytes = al256(sizeof(OpenTileD) * n);
+  b->dev_bytes = ws_total + usm_bytes + 3 * pre_bytes + st_bytes + op_bytes + 256;
   if (int rc = check_hip(hipMalloc((void**)&b->dev, b->dev_bytes), "fsg_batch_create")) {
     delete b;
     return rc;
@@ -273,8 +359,16 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   p += st_bytes;
   b->d_st_pre = (int*)p;
   p += pre_bytes;
+  b->d_op = (OpenTileD*)p;
+  p += op_bytes;
+  b->d_op_pre = (int*)p;
+  p += pre_bytes;
   b->d_queue = (int*)p;
 
+  // The persistent kernel's item kinds: with openness tiles, stencil blocks
+  // (NaN-aware included) and openness blocks, and the TopoUSM tiles take their
+  // own launches; otherwise TopoUSM and NaN-free stencil blocks.
+  b->kinds = b->n_op_tiles > 0 ? (BK_ST | BK_ST_NAN | BK_OPEN) : (BK_USM | BK_ST);
   // TopoUSM arguments (no launches): fused when on the fast path, NaN-free,
   // and with the batch's common number of box radii
   std::vector<UsmArgs> args(n);
@@ -292,7 +386,7 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
       fsg_batch_destroy(b);
       return rc;
     }
-    if (fast && t.usm.nan_hint == 0 && blocks[i] > 0 &&
+    if ((b->kinds & BK_USM) && fast && t.usm.nan_hint == 0 && blocks[i] > 0 &&
         fast_lds_bytes(args[i], false) <= 150 * 1024) {
       fast_ok[i] = 1;
       nr_count[args[i].nr]++;
@@ -348,7 +442,7 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
     S.hs = S.want_hs ? make_out(&t.hillshade) : OutDev{};
     S.sl = S.want_sl ? make_out(&t.slope) : OutDev{};
     S.cv = S.want_cv ? make_out(&t.curvature) : OutDev{};
-    if (S.nanaw) {
+    if (S.nanaw && !(b->kinds & BK_ST_NAN)) {
       b->st_fallback.push_back(S);
       continue;
     }
@@ -357,6 +451,25 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   }
   b->n_st_tiles = (int)st.size();
   b->n_st_items = st_pre.back();
+  // openness descriptors
+  std::vector<OpenTileD> op;
+  std::vector<int> op_pre{0};
+  for (int i = 0; i < n; i++) {
+    const fsg_batch_tile& t = b->tiles[i];
+    if (!wants_open(t) || t.reg.h == 0 || t.reg.w == 0) continue;
+    OpenTileD O{};
+    O.W = WinDevR{t.win.data, t.win.ld, t.win.h, t.win.w};
+    O.r0 = t.reg.r0;
+    O.c0 = t.reg.c0;
+    O.rh = t.reg.h;
+    O.rw = t.reg.w;
+    O.nbx = (t.reg.w + 63) / 64;
+    O.out = make_out(&t.openness);
+    op.push_back(O);
+    op_pre.push_back(op_pre.back() + O.nbx * ((t.reg.h + OPEN_ITEM_H - 1) / OPEN_ITEM_H));
+  }
+  b->n_op_tiles = (int)op.size();
+  b->n_op_items = op_pre.back();
   b->lds = lds;
   int rc = FSG_OK;
   if (!fused_args.empty())
@@ -371,6 +484,12 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   if (!rc)
     rc = check_hip(hipMemcpy(b->d_st_pre, st_pre.data(), sizeof(int) * st_pre.size(),
                              hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc && !op.empty())
+    rc = check_hip(hipMemcpy(b->d_op, op.data(), sizeof(OpenTileD) * op.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc)
+    rc = check_hip(hipMemcpy(b->d_op_pre, op_pre.data(), sizeof(int) * op_pre.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
   if (!rc) rc = plan_prep(b);
   if (rc) {
     fsg_batch_destroy(b);
@@ -382,7 +501,7 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
 
 template <int NR>
 static void launch_fused(const BatchDev& D, int grid, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_fused<NR>, dim3(grid), dim3(256), lds, s, D);
+  hipLaunchKernelGGL((k_batch_fused<NR, BK_USM | BK_ST>), dim3(grid), dim3(256), lds, s, D);
 }
 
 static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t s) {
@@ -418,7 +537,7 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
     hipLaunchKernelGGL(k_stencil_nanaw, grd, dim3(256), 0, s, S, make_gauss9());
     FSG_CHECK_LAUNCH("k_stencil_nanaw");
   }
-  const int total = b->n_usm_items + b->n_st_items;
+  const int total = b->n_usm_items + b->n_st_items + b->n_op_items;
   if (total == 0) return FSG_OK;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_queue);
   FSG_CHECK_LAUNCH("k_batch_reset");
@@ -441,17 +560,28 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
   D.st_pre = b->d_st_pre;
   D.n_st_tiles = b->n_st_tiles;
   D.n_st_items = b->n_st_items;
+  D.op = b->d_op;
+  D.op_pre = b->d_op_pre;
+  D.n_op_tiles = b->n_op_tiles;
+  D.n_op_items = b->n_op_items;
+  D.T = b->open_tab;
   D.queue = b->d_queue;
   D.order = cfg ? cfg->order : 0;
   D.chunk = (cfg && cfg->chunk > 0) ? cfg->chunk : 8;
   D.pre_off = (int)((b->lds + 15) & ~(size_t)15);
   const size_t lds_total =
-      (size_t)D.pre_off + sizeof(int) * ((size_t)D.n_usm_tiles + D.n_st_tiles + 2);
+      (size_t)D.pre_off + sizeof(int) * ((size_t)D.n_usm_tiles + D.n_st_tiles + D.n_op_tiles + 3);
   if (lds_total > 150 * 1024)
     return fail(FSG_E_UNSUPPORTED, "fsg_batch_run: too many tiles for the LDS tile index");
   D.G = make_gauss9();
   const int nr = b->usm_fused.empty() ? 0 : b->nr;
   TimedScope ts("k_batch_fused", s);
+  if (b->kinds & BK_OPEN) {
+    hipLaunchKernelGGL((k_batch_fused<0, BK_ST | BK_ST_NAN | BK_OPEN>), dim3(grid), dim3(256),
+                       lds_total, s, D);
+    FSG_CHECK_LAUNCH("k_batch_fused");
+    return FSG_OK;
+  }
   switch (nr) {
     case 0: launch_fused<0>(D, grid, lds_total, s); break;
     case 1: launch_fused<1>(D, grid, lds_total, s); break;
@@ -512,6 +642,8 @@ extern "C" int fsg_batch_info(const fsg_batch* b, int64_t* info) {
   info[5] = (int64_t)b->lds;
   info[6] = b->nr;
   info[7] = (int64_t)b->dev_bytes;
+  info[8] = b->n_op_items;
+  info[9] = b->n_op_tiles;
   return FSG_OK;
 }
 

@@ -517,6 +517,36 @@ def percentile(values: torch.Tensor, qs: Sequence[float], flags: int):
     return out, nval, mom
 
 
+class PercentileBuffers:
+    """Everything fsg_percentile reads and writes for n values and the
+    percentiles qs, allocated up front, so that ``run`` neither allocates
+    nor copies from the host (it can be captured in a graph)."""
+
+    def __init__(self, n: int, qs: Sequence[float], device):
+        self.n = int(n)
+        self.q = torch.tensor(list(qs), dtype=torch.float64, device=device)
+        self.pct = torch.empty(len(qs), dtype=torch.float64, device=device)
+        self.nval = torch.empty(1, dtype=torch.int64, device=device)
+        self.mom = torch.empty(2, dtype=torch.float64, device=device)
+        self.nbytes = int(_lib().fsg_percentile_workspace_size(self.n))
+        self.ws = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        self.stats = torch.empty(2, dtype=torch.float64, device=device)
+
+    def run(self, values: torch.Tensor, flags: int, kind: int) -> torch.Tensor:
+        """The percentiles of ``values`` (n contiguous float32) and their
+        finalised statistics (fsg_finalize_stats of ``kind``) in ``stats``."""
+        if values.dtype != torch.float32 or values.numel() != self.n or not values.is_contiguous():
+            raise ValueError(f"values must be {self.n} contiguous float32 values")
+        L = _lib()
+        N.check(L.fsg_percentile(values.data_ptr(), self.n, int(flags),
+                                 C.cast(self.q.data_ptr(), C.POINTER(C.c_double)),
+                                 int(self.q.numel()), self.pct.data_ptr(), self.nval.data_ptr(),
+                                 self.mom.data_ptr(), self.ws.data_ptr(), self.nbytes, _stream()))
+        N.check(L.fsg_finalize_stats(int(kind), self.pct.data_ptr(), self.mom.data_ptr(),
+                                     self.nval.data_ptr(), self.stats.data_ptr(), _stream()))
+        return self.stats
+
+
 def finalize_stats(kind: int, pct, moments, n_valid) -> torch.Tensor:
     stats = torch.empty(2, dtype=torch.float64, device=pct.device)
     N.check(_lib().fsg_finalize_stats(int(kind), pct.data_ptr(), moments.data_ptr(),

@@ -226,3 +226,67 @@ def norm_stats_prepass(raster: torch.Tensor, algorithm: str, params: dict, *,
     if return_pool:
         return stats, pool
     return stats
+
+
+class RayPrepass:
+    """The openness / ambient_occlusion pre-pass of ``norm_stats_prepass``
+    planned once and replayable: the window list, the pool and the percentile
+    buffers are fixed at construction (one host synchronisation for the
+    valid-data box and the windows' NoData counts); ``run`` enqueues the
+    windows' block functions, the pooled p1/p99 and the stretch statistics
+    with no allocation and no host synchronisation, so a graph can capture
+    it.  The windows, trims, per-window calls and percentile are those of
+    ``norm_stats_prepass`` (_compute_norm_stats_tiled,
+    algorithms/_norm_stats.py:176-350), so ``run`` gives the same statistics
+    bit for bit."""
+
+    def __init__(self, raster: torch.Tensor, algorithm: str, params: dict, *,
+                 raster_nan_free: bool = False, grid: int = 3, max_tile: int = 4096,
+                 min_valid_frac: float = 0.02):
+        if algorithm not in ("openness", "ambient_occlusion"):
+            raise ValueError(f"RayPrepass covers openness and ambient_occlusion, not {algorithm!r}")
+        self.algorithm = algorithm
+        self.raster = raster
+        merged, kw = prepass_params(algorithm, params)
+        self.kw = {**kw, "mode": "local"}
+        margin, tile = norm_stat_window_geometry(algorithm, merged, max_tile)
+        H, W = raster.shape
+        self.regions = []
+        self.pool = None
+        self.buffers = None
+        bbox = valid_bbox(raster)
+        if bbox is None:
+            return
+        wins = stratified_windows(W, H, *bbox, grid=grid, tile=min(tile, max(W, H)))
+        if raster_nan_free:
+            nan_counts = np.zeros(len(wins), dtype=np.int64)
+        else:
+            nan_counts = ops.nodata_to_nan(
+                raster, None, [(wy, wx, th, tw) for wy, wx, tw, th in wins]).cpu().numpy()
+        for (wy, wx, tw, th), nn in zip(wins, nan_counts):
+            if (th * tw - int(nn)) / float(th * tw) < min_valid_frac:
+                continue
+            m = int(min(margin, th // 3, tw // 3))
+            self.regions.append((wy, wx, th, tw, m))
+        if not self.regions:
+            return
+        total = sum((th - 2 * m) * (tw - 2 * m) for _, _, th, tw, m in self.regions)
+        self.pool = torch.empty(total, dtype=torch.float32, device=raster.device)
+        self.buffers = ops.PercentileBuffers(total, [1.0, 99.0], raster.device)
+        self._algo = {"openness": OpennessAlgorithm,
+                      "ambient_occlusion": AmbientOcclusionAlgorithm}[algorithm]()
+
+    def run(self) -> Optional[torch.Tensor]:
+        """Enqueue the pre-pass on the current stream; the float64 device
+        statistics (or None when no window has enough valid data)."""
+        if self.pool is None:
+            return None
+        off = 0
+        for wy, wx, th, tw, m in self.regions:
+            rh, rw = th - 2 * m, tw - 2 * m
+            win = self.raster[wy:wy + th, wx:wx + tw]
+            self._algo.run_region(win, ops.region(m, m, rh, rw),
+                                  OutSpec(f32=self.pool[off:off + rh * rw].view(rh, rw)),
+                                  self.kw, stretch=False)
+            off += rh * rw
+        return self.buffers.run(self.pool, ops.PCT_FINITE, 1)

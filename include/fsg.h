@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 17
+#define FSG_ABI_VERSION 18
 
 enum {
   FSG_OK = 0,
@@ -675,7 +675,8 @@ int fsg_cnn_detail_label(const float* exact, int64_t lde, int32_t ey0, int32_t e
  * One tile of a batch: an independent window, the region computed, and the
  * outputs wanted from it (an output with data == NULL is not computed;
  * topousm needs n_radii > 0 or a field).  The parameters mean what they mean
- * for fsg_hillshade_slope, fsg_curvature and fsg_topousm.
+ * for fsg_hillshade_slope, fsg_curvature, fsg_topousm and fsg_openness (the
+ * openness tiles of one batch share one ray table).
  */
 typedef struct fsg_batch_tile {
   fsg_window win;
@@ -684,12 +685,14 @@ typedef struct fsg_batch_tile {
   fsg_shade_params shade;
   fsg_curvature_params curv;
   fsg_topousm_params usm;
+  fsg_output openness;
+  fsg_openness_params open;
 } fsg_batch_tile;
 
 /* Launch shape of the persistent kernel (what the tuner searches). */
 typedef struct fsg_batch_config {
   int32_t wg_per_cu;  /* persistent workgroups per CU (0: 3) */
-  int32_t order;      /* 0: TopoUSM blocks first, 1: stencil blocks first */
+  int32_t order;      /* 0: TopoUSM, openness, stencil blocks; 1: stencil, openness, TopoUSM */
   int32_t n_cu;       /* CUs to size the grid for (0: the device's) */
   int32_t chunk;      /* work items claimed per queue access (0: 8) */
 } fsg_batch_config;
@@ -699,21 +702,28 @@ typedef struct fsg_batch fsg_batch;
 /*
  * The batch replaces the reference's per-tile, per-algorithm loop
  * (core/tile_processor.py:1635-1683 -> run_tile_algorithm,
- * core/tile_compute.py:93-129) for hillshade, slope, curvature and
+ * core/tile_compute.py:93-129) for hillshade, slope, curvature, openness and
  * topousm_fast: every tile's work goes through ONE persistent kernel that
- * drains a device queue of TopoUSM main-pass blocks and fused stencil blocks,
- * after the per-tile coarse-grid preparation.  Outputs equal the per-tile
- * entry points' bit for bit.  TopoUSM tiles off the fast path, or with NaN
- * (nan_hint != 0), or with a box-radius count other than the batch's most
- * common one run their own fsg_topousm launches inside the same sequence.
+ * drains a device queue of work items, after the per-tile coarse-grid
+ * preparation.  Outputs equal the per-tile entry points' bit for bit.
+ *   - Without openness tiles the items are TopoUSM main-pass blocks and
+ *     stencil blocks of NaN-free tiles; stencil tiles with NaN
+ *     (nan_hint != 0) take their own launches.
+ *   - With openness tiles the items are openness blocks (64 x 16 pixels)
+ *     and stencil blocks, NaN-aware ones included; every TopoUSM tile then
+ *     takes its own fsg_topousm launches inside the same sequence.
+ * TopoUSM tiles off the fast path, or with NaN (nan_hint != 0), or with a
+ * box-radius count other than the batch's most common one also run their own
+ * fsg_topousm launches.
  *
  * fsg_batch_create plans the batch (host work, one device allocation for the
  * coarse grids and descriptors, a synchronous upload); the tiles' buffers must
  * outlive it.  fsg_batch_run enqueues one pass on a stream; fsg_batch_capture
  * records one pass into a hipGraph (non-default stream; per-kernel timing
- * off) and fsg_batch_replay launches it.  fsg_batch_info fills int64[8]:
+ * off) and fsg_batch_replay launches it.  fsg_batch_info fills int64[10]:
  * TopoUSM items, stencil items, fused TopoUSM tiles, fallback tiles, stencil
- * tiles, LDS bytes, box radii of the fused kernel, device bytes held.
+ * tiles, LDS bytes, box radii of the fused kernel, device bytes held,
+ * openness items, openness tiles.
  */
 int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batch** out);
 int fsg_batch_run(fsg_batch* b, const fsg_batch_config* cfg, void* stream);

@@ -218,3 +218,25 @@ def test_tile_failures_abort_with_the_first_three_messages():
                                     "Tile (1, 2): boom 2"]
     with pytest.raises(ValueError, match="No tiles were processed"):
         check_results([TileResult(0, 0, False, skipped_reason="NoData covers 100.0%")])
+
+
+def test_raster_job_rejects_bad_products_and_rasters_before_any_launch():
+    import torch
+    from fujishadergpu_amd.batch import RasterJob
+    with pytest.raises(ValueError, match="products"):
+        RasterJob(torch.zeros(8, 8), {"topousm_fast": {}})
+    with pytest.raises(ValueError, match="at least one"):
+        RasterJob(torch.zeros(8, 8), {})
+    with pytest.raises(ValueError, match="GPU tensor"):
+        RasterJob(torch.zeros(8, 8), {"curvature": {}})
+
+
+def test_raster_job_refuses_the_multi_radius_hillshade():
+    import torch
+    from fujishadergpu_amd.batch import HillshadeAlgorithm
+    from fujishadergpu_amd.scheduler import plan
+    # the resolved radii of a default local hillshade take the multi-radius branch
+    P = plan(torch.zeros(2600, 3100), "hillshade", tile_size=1024, pixel_size=10.0)
+    assert len(HillshadeAlgorithm._direct_radii(P.params)) > 1
+    P1 = plan(torch.zeros(2600, 3100), "hillshade", tile_size=1024, pixel_size=10.0, radii=[1])
+    assert HillshadeAlgorithm._direct_radii(P1.params) == [1.0]
