@@ -433,7 +433,7 @@ class RasterJob:
 def tune_raster_job(raster: torch.Tensor, products: Dict[str, dict], *,
                     tile_sizes: Sequence[int] = (2048, 4096, 8192), reps: int = 1,
                     launch_candidates: Sequence[Tuple[int, int, int]] = (
-                        (2, 0, 8), (3, 0, 8), (4, 0, 8), (3, 1, 8), (3, 0, 32)),
+                        (4, 0, 8), (4, 1, 8), (6, 0, 8), (8, 0, 8), (8, 1, 8), (8, 0, 32)),
                     **kw) -> Tuple[RasterJob, Dict[str, object]]:
     """The measuring tuner for a RasterJob: each tile size is planned and its
     job timed (one warm run, then ``reps``); the fastest tile size's persistent

@@ -94,7 +94,7 @@ __device__ __forceinline__ void openness_item(const OpenTileD& O, const OpenTab&
 }
 
 template <int NR, int KINDS>
-__global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
+__device__ __forceinline__ void batch_drain(const BatchDev& B) {
   extern __shared__ float lds[];
   __shared__ int s_item;
   const int total = B.n_usm_items + B.n_st_items + B.n_op_items;
@@ -152,6 +152,23 @@ __global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
     }
   }
 }
+
+template <int NR, int KINDS>
+__global__ void __launch_bounds__(256) k_batch_fused(BatchDev B) {
+  batch_drain<NR, KINDS>(B);
+}
+
+// The stencil + openness instance with a register budget of WPE waves per
+// SIMD: the openness blocks, the bulk of such a batch, want more waves in
+// flight than the stencil blocks' registers would leave.
+template <int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_batch_rays(BatchDev B) {
+  batch_drain<0, BK_ST | BK_ST_NAN | BK_OPEN>(B);
+}
+template __global__ void k_batch_rays<5>(BatchDev);
+template __global__ void k_batch_rays<6>(BatchDev);
+template __global__ void k_batch_rays<8>(BatchDev);
 
 // NaN-aware stencil tiles of a batch, one launch per tile.
 __global__ void __launch_bounds__(256) k_stencil_nanaw(StencilTile S, Gauss9 G) {
@@ -577,8 +594,17 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
   const int nr = b->usm_fused.empty() ? 0 : b->nr;
   TimedScope ts("k_batch_fused", s);
   if (b->kinds & BK_OPEN) {
-    hipLaunchKernelGGL((k_batch_fused<0, BK_ST | BK_ST_NAN | BK_OPEN>), dim3(grid), dim3(256),
-                       lds_total, s, D);
+    // more workgroups per CU than the full-register instance holds take a
+    // register-capped instance (spilling some of the stencil path)
+    if (per_cu <= 4)
+      hipLaunchKernelGGL((k_batch_fused<0, BK_ST | BK_ST_NAN | BK_OPEN>), dim3(grid), dim3(256),
+                         lds_total, s, D);
+    else if (per_cu == 5)
+      hipLaunchKernelGGL(k_batch_rays<5>, dim3(grid), dim3(256), lds_total, s, D);
+    else if (per_cu == 6)
+      hipLaunchKernelGGL(k_batch_rays<6>, dim3(grid), dim3(256), lds_total, s, D);
+    else
+      hipLaunchKernelGGL(k_batch_rays<8>, dim3(grid), dim3(256), lds_total, s, D);
     FSG_CHECK_LAUNCH("k_batch_fused");
     return FSG_OK;
   }

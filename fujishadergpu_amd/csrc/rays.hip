@@ -20,17 +20,54 @@
 
 namespace fsg {
 
+// Block order of the ray kernels: 64 x 4 pixel blocks on a 1-D grid.  With
+// at least 8 block columns the region is cut into 8 column panels, one per
+// XCD (workgroup b runs on XCD b mod 8), walked row by row inside the panel:
+// the rows an XCD's resident blocks sample (their 4 rows plus the rays'
+// vertical reach) then span one panel's width, not the whole region's, and
+// stay in that XCD's L2.  Narrower regions keep the plain row-major order.
+struct RayGrid {
+  int nbx, nby;   // 64-column and 4-row blocks of the region
+  int pw;         // blocks per panel row (0: row-major)
+};
+
+inline RayGrid ray_grid(int h, int w) {
+  RayGrid G;
+  G.nbx = (w + 63) / 64;
+  G.nby = (h + 3) / 4;
+  G.pw = G.nbx >= 8 ? (G.nbx + 7) / 8 : 0;
+  return G;
+}
+
+inline int64_t ray_grid_blocks(const RayGrid& G) {
+  return G.pw ? (int64_t)8 * G.pw * G.nby : (int64_t)G.nbx * G.nby;
+}
+
+// (bx, by) of block b; false for the padding blocks of the last panel
+__device__ __forceinline__ bool ray_block(const RayGrid& G, int b, int& bx, int& by) {
+  if (G.pw == 0) {
+    by = b / G.nbx;
+    bx = b - by * G.nbx;
+    return true;
+  }
+  const int xcd = b & 7, k = b >> 3;
+  by = k / G.pw;
+  bx = xcd * G.pw + (k - by * G.pw);
+  return bx < G.nbx && by < G.nby;
+}
+
 // ND: the table's distances per direction when it is 1 .. 10 (fixed-length
 // rows: wide scalar table loads, no guards), 0 for any other count.
 template <int ND>
 __global__ void __launch_bounds__(256)
-k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y * blockDim.y + threadIdx.y;
+k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out, RayGrid G) {
+  int bx, by;
+  if (!ray_block(G, blockIdx.x, bx, by)) return;
+  const int c = bx * 64 + threadIdx.x;
+  const int r = by * 4 + threadIdx.y;
   // block-uniform: do all samples of the block's pixels stay inside the window?
-  const int bi0 = r0 + blockIdx.y * blockDim.y, bj0 = c0 + blockIdx.x * blockDim.x;
-  const bool interior = openness_interior(W, T, bi0, bi0 + blockDim.y - 1, bj0,
-                                          bj0 + blockDim.x - 1);
+  const int bi0 = r0 + by * 4, bj0 = c0 + bx * 64;
+  const bool interior = openness_interior(W, T, bi0, bi0 + 3, bj0, bj0 + 63);
   if (c >= rw || r >= rh) return;
   const int i = r0 + r, j = c0 + c;
   const PostK pp = load_post(out);
@@ -41,8 +78,9 @@ k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out) {
   }
   float a;
   if constexpr (ND > 0)
-    a = interior ? openness_angle_u<ND, false>(W, i, j, z, T)
-                 : openness_angle_u<ND, true>(W, i, j, z, T);
+    a = !interior ? openness_angle_u<ND, true>(W, i, j, z, T)
+        : open_off32(W) ? openness_angle_u<ND, false, true>(W, i, j, z, T)
+                        : openness_angle_u<ND, false>(W, i, j, z, T);
   else
     a = openness_angle_at(W, i, j, z, T, interior);
   store_out(out, r, c, apply_post(pp, openness_value(T, a)));
@@ -304,7 +342,10 @@ extern "C" int fsg_openness(const fsg_window* win, const fsg_region* reg,
   if ((rc = openness_table(p, T, "fsg_openness"))) return rc;
   if (reg->h == 0 || reg->w == 0) return FSG_OK;
   WinDevR W{win->data, win->ld, win->h, win->w};
-  dim3 blk(64, 4), grd((reg->w + 63) / 64, (reg->h + 3) / 4);
+  const RayGrid G = ray_grid(reg->h, reg->w);
+  if (ray_grid_blocks(G) > INT32_MAX)
+    return fail(FSG_E_UNSUPPORTED, "fsg_openness: region too large for one launch");
+  dim3 blk(64, 4), grd((unsigned)ray_grid_blocks(G));
   TimedScope ts("k_openness", as_stream(stream));
   const OutDev od = make_out(out);
   hipStream_t s = as_stream(stream);
@@ -312,14 +353,14 @@ extern "C" int fsg_openness(const fsg_window* win, const fsg_region* reg,
 #define FSG_OPEN_ND(N)                                                                       \
   case N:                                                                                    \
     hipLaunchKernelGGL(k_openness<N>, grd, blk, 0, s, W, reg->r0, reg->c0, reg->h, reg->w, T, \
-                       od);                                                                  \
+                       od, G);                                                               \
     break;
     FSG_OPEN_ND(1) FSG_OPEN_ND(2) FSG_OPEN_ND(3) FSG_OPEN_ND(4) FSG_OPEN_ND(5)
     FSG_OPEN_ND(6) FSG_OPEN_ND(7) FSG_OPEN_ND(8) FSG_OPEN_ND(9) FSG_OPEN_ND(10)
 #undef FSG_OPEN_ND
     default:
       hipLaunchKernelGGL(k_openness<0>, grd, blk, 0, s, W, reg->r0, reg->c0, reg->h, reg->w, T,
-                         od);
+                         od, G);
   }
   FSG_CHECK_LAUNCH("k_openness");
   return FSG_OK;

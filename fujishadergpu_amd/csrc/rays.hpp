@@ -61,59 +61,62 @@ __device__ __forceinline__ float openness_angle(const WinDevR& W, int i, int j, 
 }
 
 // The same angle with the distances of a direction unrolled (n_dist <= 10,
-// the reference's table): all loads of a direction are issued before any is
-// used, and validity is a select, not a branch.  CHECK = false is the
-// interior variant (every sample inside the window, no bounds test); with
-// CHECK the sample address is clamped and the sample masked out.  The float32
-// operations and their order are those of openness_angle, so the results are
-// the same bits.
+// the reference's table), all loads of a direction issued before any is
+// used.  The extreme starts as NaN and takes fmaxf / fminf with every ratio:
+// IEEE maxNum / minNum drop a NaN operand, so a NaN sample, a skipped table
+// entry (its inverse distance is NaN) and, with CHECK, a sample outside the
+// window (loaded from a clamped address and replaced by NaN) change nothing,
+// and the extreme stays NaN exactly when no sample of the direction is valid.
+// That is openness_angle's "extreme over the valid samples, any valid?" with
+// no per-sample mask (max and min are exact, so the order of the valid
+// ratios does not matter): the same bits.  CHECK = false is the interior
+// variant (every sample inside the window); OFF32: the window spans fewer
+// than 2^29 elements, so sample addresses are a 32-bit element offset from
+// the window start (one vector add per sample).
 constexpr int OPEN_UNROLL = 10;
 
-template <int ND, bool CHECK>
-__device__ __forceinline__ float openness_angle_u(const WinDevR& W, int i, int j, float z,
+template <int ND, bool CHECK, bool OFF32, bool NEG>
+__device__ __forceinline__ float openness_angle_s(const WinDevR& W, int i, int j, float z,
                                                   const OpenTab& T) {
   // ND = 10: the table row of a direction is a fixed 10 entries (wide scalar
   // loads, no guards); ND = 0: guarded by the runtime n_dist (<= 10)
   constexpr int NU = ND > 0 ? ND : OPEN_UNROLL;
   const int nd = ND > 0 ? ND : T.n_dist;
   const float* pc = W.p + (int64_t)i * W.ld + j;   // this pixel
+  int pix = i * (int)W.ld + j;                     // its element offset (OFF32)
+  // opaque to the optimiser: a sample's offset stays pix + (a scalar), one
+  // vector add, instead of being refactored into a per-sample multiply
+  asm volatile("" : "+v"(pix));
+  const int ld32 = (int)W.ld;
   float angle_sum = 0.0f, dir_count = 0.0f;
   for (int d = 0; d < T.n_dirs; d++) {
-    float v[NU];
-    bool ok[NU];
+    float q[NU];
 #pragma unroll
     for (int k = 0; k < NU; k++) {
-      ok[k] = false;
-      v[k] = 0.0f;
+      q[k] = __builtin_nanf("");
       if (k < nd) {
         const short2 o = T.off[d * nd + k];
-        bool in = !(o.x == 0 && o.y == 0);
+        float v;
         if (CHECK) {
           const int si = i + o.y, sj = j + o.x;
-          in = in && si >= 0 && si < W.h && sj >= 0 && sj < W.w;
-          v[k] = W.p[(int64_t)min(max(si, 0), W.h - 1) * W.ld + min(max(sj, 0), W.w - 1)];
+          const bool in = si >= 0 && si < W.h && sj >= 0 && sj < W.w;
+          v = W.p[(int64_t)min(max(si, 0), W.h - 1) * W.ld + min(max(sj, 0), W.w - 1)];
+          v = in ? v : __builtin_nanf("");
+        } else if (OFF32) {
+          v = W.p[(unsigned)(pix + o.y * ld32 + o.x)];
         } else {
           // wave-uniform element offset: scalar arithmetic, one address add per lane
-          v[k] = pc[(int64_t)o.y * W.ld + o.x];
+          v = pc[(int64_t)o.y * W.ld + o.x];
         }
-        ok[k] = in;
+        q[k] = (v - z) * T.inv[d * nd + k];
       }
     }
-    float ext = T.negative ? INFINITY : -INFINITY;
-    bool any = false;
+    float ext = __builtin_nanf("");
 #pragma unroll
-    for (int k = 0; k < NU; k++) {
-      if (k < nd) {
-        const bool valid = ok[k] && !isnan(v[k]);
-        const float q = (v[k] - z) * T.inv[d * nd + k];
-        const float nx = T.negative ? fminf(ext, q) : fmaxf(ext, q);
-        ext = valid ? nx : ext;
-        any = any || valid;
-      }
-    }
-    if (any) {
+    for (int k = 0; k < NU; k++) ext = NEG ? fminf(ext, q[k]) : fmaxf(ext, q[k]);
+    if (!isnan(ext)) {
       const float a = atanf(ext);
-      const float da = T.negative ? (T.half_pi + a) : (T.half_pi - a);
+      const float da = NEG ? (T.half_pi + a) : (T.half_pi - a);
       angle_sum = angle_sum + da;
       dir_count = dir_count + 1.0f;
     }
@@ -121,13 +124,27 @@ __device__ __forceinline__ float openness_angle_u(const WinDevR& W, int i, int j
   return angle_sum / fmaxf(dir_count, 1.0f);
 }
 
+template <int ND, bool CHECK, bool OFF32 = false>
+__device__ __forceinline__ float openness_angle_u(const WinDevR& W, int i, int j, float z,
+                                                  const OpenTab& T) {
+  return T.negative ? openness_angle_s<ND, CHECK, OFF32, true>(W, i, j, z, T)
+                    : openness_angle_s<ND, CHECK, OFF32, false>(W, i, j, z, T);
+}
+
+// the window spans fewer than 2^29 elements (32-bit sample offsets)
+__device__ __forceinline__ bool open_off32(const WinDevR& W) {
+  return (int64_t)W.h * W.ld < ((int64_t)1 << 29);
+}
+
 // Mean angle at (i, j): the unrolled variant when the table allows it.
 // interior: the caller knows every sample of (i, j) lies inside the window.
 __device__ __forceinline__ float openness_angle_at(const WinDevR& W, int i, int j, float z,
                                                    const OpenTab& T, bool interior) {
-  if (T.n_dist == OPEN_UNROLL)
-    return interior ? openness_angle_u<OPEN_UNROLL, false>(W, i, j, z, T)
-                    : openness_angle_u<OPEN_UNROLL, true>(W, i, j, z, T);
+  if (T.n_dist == OPEN_UNROLL) {
+    if (!interior) return openness_angle_u<OPEN_UNROLL, true>(W, i, j, z, T);
+    return open_off32(W) ? openness_angle_u<OPEN_UNROLL, false, true>(W, i, j, z, T)
+                         : openness_angle_u<OPEN_UNROLL, false>(W, i, j, z, T);
+  }
   if (T.n_dist > OPEN_UNROLL) return openness_angle(W, i, j, z, T);
   return interior ? openness_angle_u<0, false>(W, i, j, z, T)
                   : openness_angle_u<0, true>(W, i, j, z, T);
@@ -164,8 +181,10 @@ inline int openness_table(const fsg_openness_params* p, OpenTab& T, const char* 
   for (int e = 0; e < p->n_dirs * p->n_dist; e++) {
     const fsg_ray_sample& s = p->samples[e];
     if (s.skip || (s.dx == 0 && s.dy == 0)) {
+      // never a sample: openness_angle skips (0, 0); the unrolled variants'
+      // NaN ratio drops out of their fmaxf / fminf
       T.off[e] = make_short2(0, 0);
-      T.inv[e] = 0.f;
+      T.inv[e] = __builtin_nanf("");
       continue;
     }
     if (std::abs(s.dx) > 32767 || std::abs(s.dy) > 32767)

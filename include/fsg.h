@@ -691,7 +691,8 @@ typedef struct fsg_batch_tile {
 
 /* Launch shape of the persistent kernel (what the tuner searches). */
 typedef struct fsg_batch_config {
-  int32_t wg_per_cu;  /* persistent workgroups per CU (0: 3) */
+  int32_t wg_per_cu;  /* persistent workgroups per CU (0: 3); with openness tiles,
+                         5..8 select kernels capped at that many waves per SIMD */
   int32_t order;      /* 0: TopoUSM, openness, stencil blocks; 1: stencil, openness, TopoUSM */
   int32_t n_cu;       /* CUs to size the grid for (0: the device's) */
   int32_t chunk;      /* work items claimed per queue access (0: 8) */
