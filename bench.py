@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--only-c2", action="store_true",
                     help="skip the per-algorithm workloads (C1 hillshade+slope, C3 curvature and "
                          "openness, C5 fused batch)")
-    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "f1", "f2", "f4"],
+    ap.add_argument("--workload", choices=["c1", "c3", "c4", "c5", "f1", "f2", "f4", "stencils"],
                     default=None,
                     help="run only this per-algorithm workload and print its JSON (a profiling "
                          "aid; the bench line is the default run)")
@@ -196,6 +196,64 @@ def bench_c1(dev):
                              "kind": "port", "sample": "whole 2048^2 raster, oracle hillshade + "
                                                        "slope (NumPy, one thread of calls)"},
             "max_abs_diff_vs_oracle": d}
+
+
+def _kernel_ms(name: bytes, fn, reps: int) -> float:
+    """Average duration of the launches timed under ``name`` (HIP events on
+    the launching stream) over ``reps`` calls of fn."""
+    import torch
+    from fujishadergpu_amd import _native as N
+    L = N.lib()
+    tot, cnt = ctypes.c_double(), ctypes.c_int64()
+    L.fsg_timing_read(name, ctypes.byref(tot), ctypes.byref(cnt))
+    L.fsg_timing_enable(1)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    L.fsg_timing_enable(0)
+    N.check(L.fsg_timing_read(name, ctypes.byref(tot), ctypes.byref(cnt)))
+    return tot.value / max(1, cnt.value)
+
+
+def bench_stencils(dev, n=32768):
+    """Hillshade + slope (one fused launch) and curvature (mean), local mode,
+    over a whole 32768^2 f32 fBm (H 0.8, 0-3000 m, 10 m, seed 1) as one
+    window, NaN-free and with 1% NoData discs (seed 11): the kernel-only
+    duration (HIP events) against the algorithmic bytes (one read of the
+    raster, one float32 write per product)."""
+    import torch
+    from fujishadergpu_amd import ops, synth
+    from fujishadergpu_amd.algorithms import CurvatureAlgorithm, HillshadeSlopeFused, OutSpec
+    dem = synth.fbm(n, n, 0.8, 3000.0, seed=1, device=dev)
+    hs = torch.empty_like(dem)
+    sl = torch.empty_like(dem)
+    res = {}
+    px = n * n
+    for label in ("nan_free", "nodata_1pct"):
+        if label == "nodata_1pct":
+            synth.apply_discs(dem, synth.nodata_discs(n, n, 0.01, seed=11))
+        nan_free = label == "nan_free"
+        reg = ops.full_region(dem)
+        p = {"pixel_size": 10.0, "azimuth": 315.0, "altitude": 45.0, "curvature_type": "mean",
+             "_window_nan_free": nan_free}
+        shade = HillshadeSlopeFused()
+        curv = CurvatureAlgorithm()
+        run_hs = lambda: shade.run_region(dem, reg, OutSpec(f32=hs, clip01=True), p,  # noqa
+                                          out_slope=OutSpec(f32=sl))
+        run_cv = lambda: curv.run_region(dem, reg, OutSpec(f32=hs), p)  # noqa
+        for name, fn, key, nbytes in ((b"k_hillshade_slope", run_hs, "hillshade_slope", 12),
+                                      (b"k_curvature", run_cv, "curvature", 8)):
+            fn()
+            ms = _kernel_ms(name, fn, 5)
+            res[f"{key}_{label}"] = {
+                "kernel_ms": round(ms, 3), "gpix_s": round(px / (ms / 1e3) / 1e9, 2),
+                "alg_bytes_per_px": nbytes,
+                "hbm_frac": round(nbytes * px / (ms / 1e3) / (HBM_PEAK_GBS * 1e9), 4)}
+    res["workload"] = (f"{n}x{n} f32 fBm (H=0.8, 0-3000 m, 10 m, seed 1) as one window, local "
+                       "mode; NoData case: 1% discs r 5-50 px (seed 11) as NaN")
+    del dem, hs, sl
+    torch.cuda.empty_cache()
+    return res
 
 
 def _c3_dem(dev, h, w):
@@ -601,6 +659,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload:
         fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+              "stencils": bench_stencils,
               "f1": bench_f1, "f2": bench_f2, "f4": bench_f4}[args.workload]
         print(json.dumps({args.workload: fn(dev)}), flush=True)
         return
@@ -688,7 +747,8 @@ def main():
             "gpix_s": line["value"], "hbm_frac_whole_job": round(
                 9 * n * n / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9), 4),
             "kernel_hbm_frac": line["roofline"]["frac"]}}
-        for name, fn in (("hillshade_slope_c1", bench_c1), ("batch_fused_c5", bench_c5),
+        for name, fn in (("hillshade_slope_c1", bench_c1), ("stencils_32768", bench_stencils),
+                         ("batch_fused_c5", bench_c5),
                          ("curvature_openness_c3", bench_c3),
                          ("openness_surrogate_c4", bench_c4),
                          ("spatial_hillshade_f1", bench_f1),

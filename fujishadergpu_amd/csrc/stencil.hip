@@ -16,12 +16,63 @@ __global__ void __launch_bounds__(256) k_stencil_tiles(StencilTile S, Gauss9 G) 
   stencil_tile_block_v<NANAW, ROWS>(S, blockIdx.y, blockIdx.x, G, lds);
 }
 
-// Hillshade and/or slope without curvature and with constant steps: one lane
-// per pixel, the gradient read straight from global memory (L1/L2 serve the
-// neighbours); the same per-pixel operations as the tiled block, so the same
-// bits, without its LDS staging and barriers.
+// Hillshade and/or slope without curvature and with constant steps: the
+// gradient read straight from global memory (L1 serves the neighbours); the
+// same per-pixel operations as the tiled block (deriv's central difference
+// inside the window, deriv itself at its edges), so the same bits, without
+// its LDS staging and barriers.
+//
+// NaN-free windows: each lane walks SHADE_R rows of one column, the column's
+// values one row ahead in registers.
+constexpr int SHADE_R = 8;
+
 template <bool NANAW>
-__global__ void __launch_bounds__(256) k_shade_direct(StencilTile S, Gauss9 G) {
+__global__ void __launch_bounds__(256) k_shade_rows(StencilTile S, Gauss9 G) {
+  const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int rr0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * SHADE_R;
+  if (rr0 >= S.rh || cc >= S.rw) return;
+  const int H = S.W.h, Wd = S.W.w;
+  const int wc = S.c0 + cc;
+  const PostK phs = load_post(S.hs), psl = load_post(S.sl);
+  auto f = [&](int r, int c) { return filled<NANAW>(S.W, r, c, G); };
+  const int n = min(SHADE_R, S.rh - rr0);
+  const int wr0 = S.r0 + rr0;
+  const bool col_in = wc > 0 && wc < Wd - 1;
+  float up = wr0 > 0 ? raw(S.W, wr0 - 1, wc) : 0.0f;
+  float mid = raw(S.W, wr0, wc);
+  for (int k = 0; k < n; k++) {
+    const int rr = rr0 + k, wr = wr0 + k;
+    const float down = wr + 1 < H ? raw(S.W, wr + 1, wc) : 0.0f;
+    if (isnan(mid)) {
+      const float qn = __builtin_nanf("");
+      if (S.want_hs) store_out(S.hs, rr, cc, apply_post(phs, qn));
+      if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, qn));
+    } else {
+      float dx, dy;
+      if (col_in && wr > 0 && wr < H - 1) {
+        // deriv's interior branch on the filled values
+        const float fu = (NANAW && isnan(up)) ? f(wr - 1, wc) : up;
+        const float fd = (NANAW && isnan(down)) ? f(wr + 1, wc) : down;
+        dy = (fd - fu) / S.shade.ay.two_h;
+        const float fr = f(wr, wc + 1), fl = f(wr, wc - 1);
+        dx = (fr - fl) / S.shade.ax.two_h;
+      } else {
+        grad_at(S.shade.ax, S.shade.ay, wr, wc, H, Wd, f, dx, dy);
+      }
+      if (S.want_sl) store_out(S.sl, rr, cc, apply_post(psl, slope_from(S.shade, dx, dy)));
+      if (S.want_hs)
+        store_out(S.hs, rr, cc,
+                  apply_post(phs, hillshade_from(S.shade, S.shade.ax, wr, wc, H, Wd, f, dx, dy)));
+    }
+    up = mid;
+    mid = down;
+  }
+}
+
+// Windows with NaN: one lane per pixel (the fill inlined at each neighbour
+// read takes the registers that a row walk would need).
+template <bool NANAW>
+__global__ void __launch_bounds__(256) k_shade_px(StencilTile S, Gauss9 G) {
   const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
   const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (rr >= S.rh || cc >= S.rw) return;
@@ -106,12 +157,14 @@ extern "C" int fsg_hillshade_slope(const fsg_window* win, const fsg_region* reg,
   if (slope) S.sl = make_out(slope);
   if (!S.xrows) {
     hipStream_t s = as_stream(stream);
-    dim3 grd((reg->w + 63) / 64, (reg->h + 3) / 4);
     TimedScope ts("k_hillshade_slope", s);
-    if (S.nanaw)
-      hipLaunchKernelGGL(k_shade_direct<true>, grd, dim3(256), 0, s, S, make_gauss9());
-    else
-      hipLaunchKernelGGL(k_shade_direct<false>, grd, dim3(256), 0, s, S, make_gauss9());
+    if (S.nanaw) {
+      dim3 grd((reg->w + 63) / 64, (reg->h + 3) / 4);
+      hipLaunchKernelGGL(k_shade_px<true>, grd, dim3(256), 0, s, S, make_gauss9());
+    } else {
+      dim3 grd((reg->w + 63) / 64, (reg->h + 4 * SHADE_R - 1) / (4 * SHADE_R));
+      hipLaunchKernelGGL(k_shade_rows<false>, grd, dim3(256), 0, s, S, make_gauss9());
+    }
     FSG_CHECK_LAUNCH("k_shade_direct");
     return FSG_OK;
   }

@@ -232,30 +232,43 @@ __device__ __forceinline__ float curvature_at(const CurvK& K, int i, int j, int 
   return curvature_from(K, p, q, dxx, dxy, dyy, dyx);
 }
 
+// x^1.5 and a / b from the hardware square root and reciprocal (about one
+// ulp each).
+__device__ __forceinline__ float pow15(float x) { return x * __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // The curvature formula, tanh(100 c), (x + 1) / 2 and gamma from the first
-// (p = d/dcol, q = d/drow) and second derivatives.
+// (p = d/dcol, q = d/drow) and second derivatives (compute_curvature_block,
+// _impl_curvature.py:35-57).  The arithmetic is float32 like the reference's,
+// with hardware transcendentals: x^1.5 = x sqrt(x); tanh(y) as
+// 2 / (1 + e^(-2y)) - 1; res^gamma as 2^(gamma log2 res).  A few float32 ulps
+// from the reference's libm values (the parity tests' display-value bound is
+// 4e-4, one saturated tanh ulp).
 __device__ __forceinline__ float curvature_from(const CurvK& K, float p, float q, float dxx,
                                                 float dxy, float dyy, float dyx) {
   float cv;
   const float one = 1.0f, two = 2.0f;
   if (K.type == 0) {
     const float s = (dxy + dyx) / two;
-    const float den = powf(one + p * p + q * q, 1.5f);
+    const float den = pow15(one + p * p + q * q);
     const float num = (one + q * q) * dxx - two * p * q * s + (one + p * p) * dyy;
-    cv = -num / (two * den + 1e-10f);
+    cv = fdiv(-num, two * den + 1e-10f);
   } else if (K.type == 1) {
     const float b = one + p * p + q * q;
-    cv = (dxx * dyy - dxy * dxy) / (b * b);
+    cv = fdiv(dxx * dyy - dxy * dxy, b * b);
   } else if (K.type == 2) {
-    cv = (-(q * q * dxx - two * p * q * dxy + p * p * dyy)) /
-         (powf(p * p + q * q, 1.5f) + 1e-10f);
+    cv = fdiv(-(q * q * dxx - two * p * q * dxy + p * p * dyy), pow15(p * p + q * q) + 1e-10f);
   } else {
-    cv = (-(p * p * dxx + two * p * q * dxy + q * q * dyy)) /
-         ((p * p + q * q) * powf(one + p * p + q * q, 1.5f) + 1e-10f);
+    cv = fdiv(-(p * p * dxx + two * p * q * dxy + q * q * dyy),
+              (p * p + q * q) * pow15(one + p * p + q * q) + 1e-10f);
   }
-  const float cn = tanhf(cv * 100.0f);
-  float res = (cn + one) / two;
-  return powf(res, K.gamma);
+  // tanh(100 cv) = 2 / (1 + e^(-200 cv)) - 1, rounded to float32 like the
+  // reference's tanh (its saturated tail rounds to -1 + k ulp), then
+  // (tanh + 1) / 2 as the reference
+  const float e = __expf(-200.0f * cv);
+  const float cn = two * __builtin_amdgcn_rcpf(one + e) - one;
+  const float res = (cn + one) / two;
+  return __builtin_amdgcn_exp2f(K.gamma * __builtin_amdgcn_logf(res));
 }
 
 // ------------------------------------------------------------ tiled block
