@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "fsg_common.hpp"
+#include "tiff_ifd.hpp"
 
 // libzstd's public API (zstd.h is not installed; the shared library is)
 extern "C" {
@@ -46,17 +47,6 @@ size_t ZSTD_decompressDCtx(ZSTD_DCtx* dctx, void* dst, size_t dstCapacity, const
 
 namespace fsg {
 
-// TIFF tags and codes (TIFF 6.0, BigTIFF, GeoTIFF 1.1, GDAL's NoData tag)
-enum : uint16_t {
-  T_NEWSUBFILE = 254, T_WIDTH = 256, T_HEIGHT = 257, T_BPS = 258, T_COMPRESSION = 259,
-  T_PHOTOMETRIC = 262, T_SPP = 277, T_PLANAR = 284, T_PREDICTOR = 317, T_TILEW = 322,
-  T_TILEH = 323, T_TILEOFF = 324, T_TILEBC = 325, T_SAMPLEFMT = 339, T_PIXSCALE = 33550,
-  T_TIEPOINT = 33922, T_GEOKEYS = 34735, T_NODATA = 42113
-};
-enum : uint16_t { TY_BYTE = 1, TY_ASCII = 2, TY_SHORT = 3, TY_LONG = 4, TY_DOUBLE = 12,
-                  TY_LONG8 = 16 };
-constexpr uint16_t COMP_NONE = 1, COMP_DEFLATE = 8, COMP_ADOBE_DEFLATE = 32946,
-                   COMP_ZSTD = 50000;
 
 static int elem_bytes(int dtype) { return dtype == FSG_OUT_F32 ? 4 : (dtype == FSG_OUT_I16 ? 2 : 1); }
 
@@ -571,11 +561,7 @@ static int write_cog(const char* path, const fsg_cog_raster& R, const fsg_cog_op
 }
 
 // --------------------------------------------------------------- the reader
-struct CogLevelR {
-  int w = 0, h = 0, tw = 0, th = 0, bps = 0, fmt = 1, comp = 1, pred = 1, spp = 1;
-  bool strips = false;   // strips: tiles of the full width, the last one short
-  std::vector<uint64_t> off, cnt;
-};
+using CogLevelR = TiffLevel;
 
 }  // namespace fsg
 
@@ -590,52 +576,6 @@ struct fsg_cog {
 };
 
 namespace fsg {
-
-static uint64_t rd(const std::vector<uint8_t>& b, size_t at, int n) {
-  uint64_t v = 0;
-  std::memcpy(&v, b.data() + at, n);
-  return v;
-}
-
-static int type_size(int ty) {
-  switch (ty) {
-    case 1: case 2: case 6: case 7: return 1;
-    case 3: case 8: return 2;
-    case 4: case 9: case 11: case 13: return 4;
-    case 5: case 10: case 12: case 16: case 17: case 18: return 8;
-  }
-  return 0;
-}
-
-// values of one IFD entry as uint64 / double / string
-static int entry_values(int fd, bool big, const std::vector<uint8_t>& ent, std::vector<uint64_t>* u,
-                        std::vector<double>* d, std::string* str) {
-  const int ty = (int)rd(ent, 2, 2);
-  const uint64_t cnt = big ? rd(ent, 4, 8) : rd(ent, 4, 4);
-  const int ts = type_size(ty);
-  if (!ts || cnt > (1ull << 28)) return 1;
-  const size_t bytes = (size_t)cnt * ts;
-  std::vector<uint8_t> v(bytes);
-  const size_t inl = big ? 8 : 4;
-  if (bytes <= inl) {
-    std::memcpy(v.data(), ent.data() + (big ? 12 : 8), bytes);
-  } else {
-    const uint64_t off = big ? rd(ent, 12, 8) : rd(ent, 8, 4);
-    if (read_all(fd, v.data(), bytes, off)) return 1;
-  }
-  for (uint64_t k = 0; k < cnt; k++) {
-    uint64_t x = 0;
-    double y = 0;
-    std::memcpy(&x, v.data() + k * ts, ts);
-    if (ty == 12) std::memcpy(&y, v.data() + k * ts, 8);
-    else if (ty == 11) { float f; std::memcpy(&f, v.data() + k * ts, 4); y = f; }
-    else y = (double)x;
-    if (u) u->push_back(x);
-    if (d) d->push_back(y);
-  }
-  if (str && ty == 2) str->assign((const char*)v.data(), strnlen((const char*)v.data(), bytes));
-  return 0;
-}
 
 template <typename T>
 static int read_window(fsg_cog* c, const CogLevelR& L, int r0, int c0, int wh, int ww, T* out,
@@ -672,7 +612,14 @@ static int read_window(fsg_cog* c, const CogLevelR& L, int r0, int c0, int wh, i
   parallel_for(n, n_workers(threads), [&](int i, int) {
     const int id = ids[i];
     if (id >= (int)L.off.size() || id >= (int)L.cnt.size()) { bad = 1; return; }
-    std::vector<uint8_t> comp(L.cnt[id]);
+    // the parse bounded every tile's bytes by the file size
+    std::vector<uint8_t> comp;
+    try {
+      comp.resize(L.cnt[id]);
+    } catch (const std::bad_alloc&) {
+      bad = 1;
+      return;
+    }
     if (read_all(c->fd, comp.data(), comp.size(), L.off[id])) { bad = 1; return; }
     uint8_t* dst = h + tile_bytes * i;
     const size_t want = expect(i);
@@ -736,105 +683,29 @@ extern "C" int fsg_cog_write(const char* path, const fsg_cog_raster* r,
 extern "C" int fsg_cog_open(const char* path, fsg_cog** out) {
   if (!path || !out) return fail(FSG_E_INVALID, "fsg_cog_open: bad arguments");
   *out = nullptr;
-  auto* c = new fsg_cog();
-  c->fd = open(path, O_RDONLY);
-  auto bail = [&](const std::string& m) {
-    if (c->fd >= 0) close(c->fd);
-    delete c;
-    return fail(FSG_E_INVALID, "fsg_cog_open: " + m + " (" + path + ")");
-  };
-  if (c->fd < 0) return bail("cannot open");
-  std::vector<uint8_t> hd(16);
-  if (read_all(c->fd, hd.data(), 16, 0)) return bail("short file");
-  if (hd[0] != 'I' || hd[1] != 'I') return bail("not a little-endian TIFF");
-  const int magic = (int)rd(hd, 2, 2);
-  if (magic != 42 && magic != 43) return bail("not a TIFF");
-  c->big = magic == 43;
-  uint64_t ifd = c->big ? rd(hd, 8, 8) : rd(hd, 4, 4);
-  const size_t es = c->big ? 20 : 12;
-  for (int guard = 0; ifd && guard < 64; guard++) {
-    std::vector<uint8_t> nb(8);
-    if (read_all(c->fd, nb.data(), c->big ? 8 : 2, ifd)) return bail("bad IFD");
-    const uint64_t n = c->big ? rd(nb, 0, 8) : rd(nb, 0, 2);
-    if (n > 4096) return bail("bad IFD entry count");
-    std::vector<uint8_t> all(n * es + (c->big ? 8 : 4));
-    if (read_all(c->fd, all.data(), all.size(), ifd + (c->big ? 8 : 2))) return bail("bad IFD");
-    CogLevelR L;
-    bool tiled = false;
-    int rows_per_strip = 0;
-    std::vector<uint64_t> soff, scnt;
-    std::vector<double> scale, tie;
-    for (uint64_t k = 0; k < n; k++) {
-      std::vector<uint8_t> ent(all.begin() + k * es, all.begin() + (k + 1) * es);
-      const int tag = (int)rd(ent, 0, 2);
-      std::vector<uint64_t> u;
-      std::vector<double> d;
-      std::string str;
-      if (entry_values(c->fd, c->big, ent, &u, &d, &str)) return bail("bad tag value");
-      if (u.empty() && str.empty()) continue;
-      switch (tag) {
-        case T_WIDTH: L.w = (int)u[0]; break;
-        case T_HEIGHT: L.h = (int)u[0]; break;
-        case T_BPS: L.bps = (int)u[0]; break;
-        case T_COMPRESSION: L.comp = (int)u[0]; break;
-        case T_PREDICTOR: L.pred = (int)u[0]; break;
-        case T_SPP: L.spp = (int)u[0]; break;
-        case T_SAMPLEFMT: L.fmt = (int)u[0]; break;
-        case T_TILEW: L.tw = (int)u[0]; tiled = true; break;
-        case T_TILEH: L.th = (int)u[0]; break;
-        case T_TILEOFF: L.off = u; break;
-        case T_TILEBC: L.cnt = u; break;
-        case 273: soff = u; break;             // StripOffsets
-        case 278: rows_per_strip = (int)u[0]; break;
-        case 279: scnt = u; break;             // StripByteCounts
-        case T_PIXSCALE:
-          if (c->lv.empty() && d.size() >= 2) { scale = d; }
-          break;
-        case T_TIEPOINT:
-          if (c->lv.empty() && d.size() >= 6) { tie = d; }
-          break;
-        case T_GEOKEYS:
-          if (c->lv.empty())
-            for (size_t q = 4; q + 3 < u.size(); q += 4) {
-              if (u[q] == 3072 || u[q] == 2048) { c->epsg = (int)u[q + 3]; c->geographic = u[q] == 2048; }
-            }
-          break;
-        case T_NODATA:
-          if (c->lv.empty() && !str.empty()) { c->has_nodata = 1; c->nodata = std::strtod(str.c_str(), nullptr); }
-          break;
-        default: break;
-      }
-    }
-    if (!tiled && !soff.empty()) {
-      L.strips = true;
-      L.tw = L.w;
-      L.th = rows_per_strip > 0 ? std::min(rows_per_strip, L.h) : L.h;
-      L.off = soff;
-      L.cnt = scnt;
-      tiled = true;
-    }
-    if (!tiled || L.w <= 0 || L.h <= 0 || L.tw <= 0 || L.th <= 0 || L.spp != 1)
-      return bail("only tiled or stripped single-band rasters are read");
-    const bool ok_type = (L.bps == 32 && L.fmt == 3) || (L.bps == 16 && L.fmt == 2) ||
-                         (L.bps == 8 && L.fmt == 1);
-    if (!ok_type) return bail("sample type is not float32, int16 or uint8");
-    if (L.pred == 3 && L.bps != 32) return bail("predictor 3 on a non-float raster");
-    const size_t nt = (size_t)((L.w + L.tw - 1) / L.tw) * ((L.h + L.th - 1) / L.th);
-    if (L.off.size() != nt || L.cnt.size() != nt) return bail("tile tables do not match the size");
-    if (L.pred == 2 && L.bps == 32) return bail("predictor 2 on a float raster");
-    if (c->lv.empty() && !scale.empty() && !tie.empty()) {
-      // tiepoint (I, J, K, X, Y, Z) at pixel (I, J); north-up scale (sx, sy)
-      c->gt[1] = scale[0];
-      c->gt[5] = -scale[1];
-      c->gt[0] = tie[3] - tie[0] * c->gt[1];
-      c->gt[3] = tie[4] - tie[1] * c->gt[5];
-      c->has_gt = 1;
-    }
-    c->lv.push_back(L);
-    std::vector<uint8_t> nx(all.end() - (c->big ? 8 : 4), all.end());
-    ifd = c->big ? rd(nx, 0, 8) : rd(nx, 0, 4);
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) return fail(FSG_E_INVALID, std::string("fsg_cog_open: cannot open (") + path + ")");
+  TiffFile F;
+  std::string why;
+  try {
+    why = tiff_parse(fd, F);
+  } catch (const std::bad_alloc&) {
+    why = "out of host memory";
   }
-  if (c->lv.empty()) return bail("no image");
+  if (!why.empty()) {
+    close(fd);
+    return fail(FSG_E_INVALID, "fsg_cog_open: " + why + " (" + path + ")");
+  }
+  auto* c = new fsg_cog();
+  c->fd = fd;
+  c->big = F.big;
+  c->lv = std::move(F.lv);
+  for (int k = 0; k < 6; k++) c->gt[k] = F.gt[k];
+  c->has_gt = F.has_gt;
+  c->has_nodata = F.has_nodata;
+  c->nodata = F.nodata;
+  c->epsg = F.epsg;
+  c->geographic = F.geographic;
   *out = c;
   return FSG_OK;
 }
@@ -865,7 +736,8 @@ extern "C" int fsg_cog_read(fsg_cog* c, int32_t level, int32_t r0, int32_t c0, i
   if (!c || level < 0 || level >= (int)c->lv.size() || !out)
     return fail(FSG_E_INVALID, "fsg_cog_read: bad handle, level or output");
   const CogLevelR& L = c->lv[level];
-  if (h <= 0 || w <= 0 || r0 < 0 || c0 < 0 || r0 + h > L.h || c0 + w > L.w || ldo < w)
+  if (h <= 0 || w <= 0 || r0 < 0 || c0 < 0 || (int64_t)r0 + h > L.h ||
+      (int64_t)c0 + w > L.w || ldo < w)
     return fail(FSG_E_INVALID, "fsg_cog_read: window outside the level");
   if (L.pred != 1 && L.pred != 2 && L.pred != 3)
     return fail(FSG_E_UNSUPPORTED, "fsg_cog_read: predictor not 1, 2 or 3");
