@@ -79,7 +79,7 @@ k_openness(WinDevR W, int r0, int c0, int rh, int rw, OpenTab T, OutDev out, Ray
   float a;
   if constexpr (ND > 0)
     a = !interior ? openness_angle_u<ND, true>(W, i, j, z, T)
-        : open_off32(W) ? openness_angle_u<ND, false, true>(W, i, j, z, T)
+        : open_off32(W, T) ? openness_angle_u<ND, false, true>(W, i, j, z, T)
                         : openness_angle_u<ND, false>(W, i, j, z, T);
   else
     a = openness_angle_at(W, i, j, z, T, interior);
@@ -115,7 +115,7 @@ k_openness_points(WinDevR W, const int2* __restrict__ pts, int n, OpenTab T,
 // JUDGE_SPLIT*4, ... and writes the sum and count of |exact - approx| over
 // pixels where both are numbers; the reduction order is fixed, so the result
 // is deterministic.
-constexpr int JUDGE_SPLIT = 16;
+constexpr int JUDGE_SPLIT = 64;
 
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;

@@ -70,9 +70,10 @@ __device__ __forceinline__ float openness_angle(const WinDevR& W, int i, int j, 
 // That is openness_angle's "extreme over the valid samples, any valid?" with
 // no per-sample mask (max and min are exact, so the order of the valid
 // ratios does not matter): the same bits.  CHECK = false is the interior
-// variant (every sample inside the window); OFF32: the window spans fewer
-// than 2^29 elements, so sample addresses are a 32-bit element offset from
-// the window start (one vector add per sample).
+// variant (every sample inside the window); OFF32 (interior only): the rows
+// a pixel's rays reach, i - dmax_y .. i + dmax_y, span fewer than 2^31
+// elements, so a sample's address is a 32-bit element offset from the first
+// of them (one vector add per sample).
 constexpr int OPEN_UNROLL = 10;
 
 template <int ND, bool CHECK, bool OFF32, bool NEG>
@@ -83,7 +84,9 @@ __device__ __forceinline__ float openness_angle_s(const WinDevR& W, int i, int j
   constexpr int NU = ND > 0 ? ND : OPEN_UNROLL;
   const int nd = ND > 0 ? ND : T.n_dist;
   const float* pc = W.p + (int64_t)i * W.ld + j;   // this pixel
-  int pix = i * (int)W.ld + j;                     // its element offset (OFF32)
+  // OFF32: the first row the rays reach, and the pixel's offset from it
+  const float* pr = W.p + (int64_t)(i - T.dmax_y) * W.ld;
+  int pix = T.dmax_y * (int)W.ld + j;
   // opaque to the optimiser: a sample's offset stays pix + (a scalar), one
   // vector add, instead of being refactored into a per-sample multiply
   asm volatile("" : "+v"(pix));
@@ -103,7 +106,7 @@ __device__ __forceinline__ float openness_angle_s(const WinDevR& W, int i, int j
           v = W.p[(int64_t)min(max(si, 0), W.h - 1) * W.ld + min(max(sj, 0), W.w - 1)];
           v = in ? v : __builtin_nanf("");
         } else if (OFF32) {
-          v = W.p[(unsigned)(pix + o.y * ld32 + o.x)];
+          v = pr[(unsigned)(pix + o.y * ld32 + o.x)];
         } else {
           // wave-uniform element offset: scalar arithmetic, one address add per lane
           v = pc[(int64_t)o.y * W.ld + o.x];
@@ -131,9 +134,10 @@ __device__ __forceinline__ float openness_angle_u(const WinDevR& W, int i, int j
                     : openness_angle_s<ND, CHECK, OFF32, false>(W, i, j, z, T);
 }
 
-// the window spans fewer than 2^29 elements (32-bit sample offsets)
-__device__ __forceinline__ bool open_off32(const WinDevR& W) {
-  return (int64_t)W.h * W.ld < ((int64_t)1 << 29);
+// the rows any pixel's rays reach (2 dmax_y + 1 of them) span fewer than
+// 2^31 elements: 32-bit sample offsets on interior blocks
+__device__ __forceinline__ bool open_off32(const WinDevR& W, const OpenTab& T) {
+  return (int64_t)(2 * T.dmax_y + 1) * W.ld < ((int64_t)1 << 31);
 }
 
 // Mean angle at (i, j): the unrolled variant when the table allows it.
@@ -142,7 +146,7 @@ __device__ __forceinline__ float openness_angle_at(const WinDevR& W, int i, int 
                                                    const OpenTab& T, bool interior) {
   if (T.n_dist == OPEN_UNROLL) {
     if (!interior) return openness_angle_u<OPEN_UNROLL, true>(W, i, j, z, T);
-    return open_off32(W) ? openness_angle_u<OPEN_UNROLL, false, true>(W, i, j, z, T)
+    return open_off32(W, T) ? openness_angle_u<OPEN_UNROLL, false, true>(W, i, j, z, T)
                          : openness_angle_u<OPEN_UNROLL, false>(W, i, j, z, T);
   }
   if (T.n_dist > OPEN_UNROLL) return openness_angle(W, i, j, z, T);

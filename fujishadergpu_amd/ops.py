@@ -426,7 +426,7 @@ def topousm_coarse_means_batch(jobs: Sequence[Tuple[torch.Tensor, int, Sequence[
                                ) -> List[torch.Tensor]:
     """topousm_coarse_means for many NaN-free windows in a few launches.
 
-    jobs: (window tensor, ds in {4, 8, 16}, coarse radii >= 2, at most 4);
+    jobs: (window tensor, ds in {1, 4, 8, 16}, coarse radii >= 2, at most 4);
     returns one (len(radii), ceil(h/ds), ceil(w/ds)) float32 tensor per job."""
     n = len(jobs)
     if n == 0:

@@ -258,16 +258,37 @@ def inputs(win: torch.Tensor, radius: int, n_dirs: int, pixel_size: float) -> In
     hc, wc = H // GRID, W // GRID
     levels = [r // GRID for r in LEVELS_PX]
     zc = ops.block_mean(win, GRID)
-    means = ops.topousm_coarse_means(zc, 1, levels)       # box means of the grid itself
+    # the base (exact openness of the 16x grid) is independent of the
+    # features: it runs on a side stream while the box means run here
+    main = torch.cuda.current_stream(win.device) if win.is_cuda else None
+    side = torch.cuda.Stream(win.device) if win.is_cuda else None
+    if side is not None:
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            z16 = ops.block_mean(zc, BASE // GRID)
+            th16 = exact_openness(z16, ops.full_region(z16), max(1, radius // BASE), n_dirs,
+                                  pixel_size * BASE)
+            base = _up(th16, BASE // GRID, hc, wc)
+    # box means of the grid itself: the plain passes on a NaN-free grid (the
+    # same values as the NaN-aware passes there, without their count planes)
+    nan_cells = int(ops.nodata_to_nan(zc, None, [(0, 0, hc, wc)])[0].item())
+    if nan_cells == 0:
+        means = ops.topousm_coarse_means_batch([(zc, 1, levels)])[0]
+    else:
+        means = ops.topousm_coarse_means(zc, 1, levels)
     feat = torch.empty((hc, wc, 8), dtype=torch.int16, device=win.device)
     mp = (C.c_void_p * 4)(*[means[k].data_ptr() for k in range(4)])
     inv = (C.c_float * 4)(*[FEAT_GAIN / (r * pixel_size) for r in LEVELS_PX])
     N.check(N.lib().fsg_cnn_features(zc.data_ptr(), zc.stride(0), mp, means.stride(1), inv, hc,
                                      wc, feat.data_ptr(), _s()))
-    z16 = ops.block_mean(zc, BASE // GRID)
-    th16 = exact_openness(z16, ops.full_region(z16), max(1, radius // BASE), n_dirs,
-                          pixel_size * BASE)
-    base = _up(th16, BASE // GRID, hc, wc)
+    if side is None:
+        z16 = ops.block_mean(zc, BASE // GRID)
+        th16 = exact_openness(z16, ops.full_region(z16), max(1, radius // BASE), n_dirs,
+                              pixel_size * BASE)
+        base = _up(th16, BASE // GRID, hc, wc)
+    else:
+        main.wait_stream(side)
+        base.record_stream(main)       # made on the side stream, read on this one
     return Inputs(feat, base, zc, hc, wc)
 
 
