@@ -444,9 +444,14 @@ def evaluate(net: Net, ts: TrainSet, threshold: float = 1.5, sample_frac: float 
         else:
             final.append(err[i])
     final = torch.stack(final)
+    tile_mae = err.reshape(err.shape[0], -1).mean(dim=1)
+    tile_mae_b = err_b.reshape(err_b.shape[0], -1).mean(dim=1)
     return {"mae_surrogate_deg": float(err.mean()), "mae_base_deg": float(err_b.mean()),
             "mae_without_detail_deg": float(err_nd.mean()),
             "p99_abs_err_surrogate_deg": float(torch.quantile(err.reshape(-1)[::97], 0.99)),
+            "p99_abs_err_base_deg": float(torch.quantile(err_b.reshape(-1)[::97], 0.99)),
+            "worst_tile_mae_surrogate_deg": float(tile_mae.max()),
+            "worst_tile_mae_base_deg": float(tile_mae_b.max()),
             "mae_after_fallback_deg": float(final.mean()), "tiles_flagged": flagged,
             "tiles": int(err.shape[0])}
 

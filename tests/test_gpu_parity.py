@@ -5,8 +5,9 @@ Tolerances (float32 outputs):
 * slope (degrees): |d| <= 2e-5 + 1e-6*|v|  (atanf implementations differ by ulps);
 * curvature display value: |d| <= 4e-4, and |d| <= 2e-6 in the tanh domain
   (one tanh ulp at saturation becomes 3.8e-4 after the 1/2.2 gamma);
-* topousm (normalised units): |d| <= 2e-4 (float32 box means of ~3000 m
-  elevations; one f32 ulp of a mean is ~2.4e-4 m);
+* topousm (normalised units): |d| <= 2e-5 (float32 box means of ~3000 m
+  elevations: one f32 ulp of a mean is ~2.4e-4 m, ~1.2e-5 after the p99
+  divide; observed 0 except for 3.8e-6 and 7.6e-6 in two cases);
 * openness / AO display values: |d| <= 1e-5.
 NaN masks must match exactly; integer encodings must equal the oracle's
 quantisation of the GPU float values bit for bit.
@@ -150,7 +151,8 @@ def test_topousm_window(cuda, radii, weights, px, nan_discs):
                                              global_stats=(17.0,))
     exp = ref.topousm_direct(dem.numpy(), radii=radii, weights=weights, pixel_size=px,
                              stats=(17.0,))
-    _cmp("topousm", out, exp, 2e-4)
+    mx_, _ = _cmp("topousm", out, exp, 2e-5)
+    print("maxdiff topousm", mx_, flush=True)
 
 
 def test_topousm_field_and_two_outputs(cuda):
@@ -167,7 +169,8 @@ def test_topousm_field_and_two_outputs(cuda):
     field = ops.topousm_coarse_field(coarse, [max(1, int(round(r / scale))) for r in (300, 512)],
                                      [0.05, 0.02])
     _cmp("overview", coarse, coarse_np, 0.0)
-    _cmp("field", field, field_np, 2e-4)
+    mx_, _ = _cmp("field", field, field_np, 2e-5)
+    print("maxdiff field", mx_, flush=True)
     # one tile window (rows 100..400, cols 60..460) with the field
     wy, wx, wh, ww = 100, 60, 300, 400
     win = g[wy:wy + wh, wx:wx + ww]
@@ -186,7 +189,8 @@ def test_topousm_field_and_two_outputs(cuda):
                              small_radii=[2, 8, 32], small_weights=[0.4, 0.3, 0.2],
                              w_large=0.07, field_offset=(wy, wx), full_shape=(H, W),
                              stats=(9.0,))
-    _cmp("topousm+field", f32, exp, 2e-4)
+    mx_, _ = _cmp("topousm+field", f32, exp, 2e-5)
+    print("maxdiff topousm+field", mx_, flush=True)
     dn = ref.quantize(f32.cpu().numpy(), qp["a_coef"], qp["b_coef"], qp["dn_min"], qp["dn_max"],
                       "uint8")
     assert np.array_equal(u8.cpu().numpy(), dn)
@@ -239,7 +243,8 @@ def test_topousm_shared_coarse_means(cuda):
     assert d <= 1e-6
     exp = ref.topousm_direct(dem.numpy()[wy:wy + wh, wx:wx + ww], radii=radii, weights=weights,
                              pixel_size=5.0, stats=(20.0,))[256:1280, 256:1280]
-    _cmp("topousm/shared", shared, exp, 2e-4)
+    mx_, _ = _cmp("topousm/shared", shared, exp, 2e-5)
+    print("maxdiff topousm/shared", mx_, flush=True)
 
 
 def test_topousm_coarse_means_batch_equals_per_window(cuda):

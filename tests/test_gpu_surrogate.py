@@ -194,6 +194,21 @@ def test_short_training_beats_the_method_without_the_network(cuda):
     assert after["mae_surrogate_deg"] < after["mae_without_detail_deg"]
 
 
+def test_trained_network_cuts_the_tail_and_the_worst_tile(cuda):
+    """The shipped weights on 8 held-out tiles: the network must lower the
+    p99 error and the worst tile's MAE against the method without it (exact
+    openness of the 16x grid, upsampled), not only the mean."""
+    from fujishadergpu_amd import surrogate as S
+    te = S.make_tiles(range(5000, 5008), cuda)
+    net = S.Net(cuda, seed=0)
+    net.load(S.WEIGHTS_FILE)
+    m = S.evaluate(net, te)
+    print("held-out", m, flush=True)
+    assert m["mae_surrogate_deg"] < 0.6 * m["mae_base_deg"]
+    assert m["p99_abs_err_surrogate_deg"] < 0.8 * m["p99_abs_err_base_deg"]
+    assert m["worst_tile_mae_surrogate_deg"] < 0.8 * m["worst_tile_mae_base_deg"]
+
+
 def test_judge_replaces_exactly_the_tiles_over_the_threshold(cuda):
     from fujishadergpu_amd import ops, synth
     from fujishadergpu_amd import surrogate as S
