@@ -5,6 +5,7 @@
 #include <cmath>
 
 #include "stencil.hpp"
+#include "stencil_rows.hpp"
 
 namespace fsg {
 
@@ -113,6 +114,201 @@ static int launch_stencil(const StencilTile& S, const char* name, hipStream_t s)
   return FSG_OK;
 }
 
+// The output descriptor moved to region coordinates (dr, dc).
+static OutDev out_shift(OutDev o, int dr, int dc) {
+  if (!o.data) return o;
+  const int64_t es = o.dtype == FSG_OUT_F32 ? 4 : (o.dtype == FSG_OUT_I16 ? 2 : 1);
+  o.data = (char*)o.data + ((int64_t)dr * o.ld + dc) * es;
+  return o;
+}
+
+struct Rect {
+  int r0, c0, h, w;  // window coordinates
+};
+
+// The interior rectangle of the region: window rows [m, H-m) and columns
+// [m, W-m), where every derivative the stencil takes is the central one.
+static Rect interior(const fsg_window* win, const fsg_region* reg, int m) {
+  const int r0 = std::max(reg->r0, m), r1 = std::min(reg->r0 + reg->h, win->h - m);
+  const int c0 = std::max(reg->c0, m), c1 = std::min(reg->c0 + reg->w, win->w - m);
+  if (r0 >= r1 || c0 >= c1) return Rect{0, 0, 0, 0};
+  return Rect{r0, c0, r1 - r0, c1 - c0};
+}
+
+// The region less the interior rectangle, as up to four strips.
+static int border_strips(const fsg_region* reg, const Rect& in, Rect* out) {
+  const int R0 = reg->r0, R1 = reg->r0 + reg->h, C0 = reg->c0, C1 = reg->c0 + reg->w;
+  int n = 0;
+  if (in.h == 0 || in.w == 0) {
+    out[n++] = Rect{R0, C0, reg->h, reg->w};
+    return n;
+  }
+  const int ir1 = in.r0 + in.h, ic1 = in.c0 + in.w;
+  if (in.r0 > R0) out[n++] = Rect{R0, C0, in.r0 - R0, reg->w};
+  if (R1 > ir1) out[n++] = Rect{ir1, C0, R1 - ir1, reg->w};
+  if (in.c0 > C0) out[n++] = Rect{in.r0, C0, in.h, in.c0 - C0};
+  if (C1 > ic1) out[n++] = Rect{in.r0, ic1, in.h, C1 - ic1};
+  return n;
+}
+
+// S restricted to the strip q of the region reg.
+static StencilTile sub_tile(const StencilTile& S, const fsg_region* reg, const Rect& q) {
+  StencilTile T = S;
+  T.r0 = q.r0;
+  T.c0 = q.c0;
+  T.rh = q.h;
+  T.rw = q.w;
+  T.nbx = (q.w + ST_TW - 1) / ST_TW;
+  const int dr = q.r0 - reg->r0, dc = q.c0 - reg->c0;
+  T.hs = out_shift(S.hs, dr, dc);
+  T.sl = out_shift(S.sl, dr, dc);
+  T.cv = out_shift(S.cv, dr, dc);
+  return T;
+}
+
+// The row-walking launch over the interior rectangle: one wave per `cols`
+// columns and SR_SEG_ROWS rows, four waves per workgroup.
+static bool rows_grid(const Rect& in, int cols, dim3& grd) {
+  const int64_t nseg = ((int64_t)in.h + SR_SEG_ROWS - 1) / SR_SEG_ROWS;
+  const int64_t gy = (nseg + 3) / 4, gx = ((int64_t)in.w + cols - 1) / cols;
+  if (gy > 65535 || gx > 0x7fffffff) return false;
+  grd = dim3((unsigned)gx, (unsigned)gy);
+  return true;
+}
+
+static RowsTile rows_tile(const StencilTile& S, const fsg_region* reg, const Rect& in) {
+  RowsTile R{};
+  R.W = S.W;
+  R.r0 = reg->r0;
+  R.c0 = reg->c0;
+  R.ir0 = in.r0;
+  R.ir1 = in.r0 + in.h;
+  R.ic0 = in.c0;
+  R.ic1 = in.c0 + in.w;
+  return R;
+}
+
+// Hillshade/slope with constant steps and NaN-free or NaN-aware window,
+// general kernels (every position, numpy's edge formulas included).
+static void launch_shade_general(const StencilTile& S, hipStream_t s) {
+  if (S.nanaw) {
+    dim3 grd((S.rw + 63) / 64, (S.rh + 3) / 4);
+    hipLaunchKernelGGL(k_shade_px<true>, grd, dim3(256), 0, s, S, make_gauss9());
+  } else {
+    dim3 grd((S.rw + 63) / 64, (S.rh + 4 * SHADE_R - 1) / (4 * SHADE_R));
+    hipLaunchKernelGGL(k_shade_rows<false>, grd, dim3(256), 0, s, S, make_gauss9());
+  }
+}
+
+static void launch_curv_general(const StencilTile& S, hipStream_t s) {
+  dim3 grd(S.nbx, (S.rh + ST_TH - 1) / ST_TH);
+  if (S.nanaw)
+    hipLaunchKernelGGL((k_stencil_tiles<true, false>), grd, dim3(256), 0, s, S, make_gauss9());
+  else
+    hipLaunchKernelGGL((k_stencil_tiles<false, false>), grd, dim3(256), 0, s, S, make_gauss9());
+}
+
+// Element offset of a pointer within its 16-byte group, or -1 when it is
+// not 4-byte aligned.
+static int quad_phase(const void* p) {
+  const uintptr_t u = (uintptr_t)p;
+  return (u & 3) ? -1 : (int)((u >> 2) & 3);
+}
+
+static int mod4(int x) { return ((x % 4) + 4) % 4; }
+
+// The four-columns-per-lane launch: needs a row stride that is a multiple of
+// 4 floats and a 4-byte aligned window; float4 stores where the outputs line
+// up with the groups.
+static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, int n_out,
+                        RowsTile4& S4, dim3& grd) {
+  const int a = quad_phase(R.W.p);
+  if (a < 0 || R.W.ld % 4 != 0) return false;
+  const int cb0 = (in.c0 - 2) - mod4(a + in.c0 - 2);
+  const int64_t n_strips = ((int64_t)(in.c0 + in.w) - (cb0 + 2) + SR4_COLS - 1) / SR4_COLS;
+  const int64_t nseg = ((int64_t)in.h + SR_SEG_ROWS - 1) / SR_SEG_ROWS;
+  if (nseg > 65535 || n_strips > 0x7ffffff) return false;
+  S4.T = R;
+  S4.cb0 = cb0;
+  S4.n_strips = (int)n_strips;
+  int vec = 1;
+  for (int k = 0; k < n_out; k++) {
+    const OutDev& o = outs[k];
+    if (!o.data) continue;
+    const int ph = quad_phase(o.data);
+    vec = vec && o.dtype == FSG_OUT_F32 && ph >= 0 && o.ld % 4 == 0 &&
+          mod4(ph + cb0 - R.c0) == 0;
+  }
+  S4.vec_out = vec;
+  grd = dim3((unsigned)((n_strips + 3) / 4), (unsigned)nseg);
+  return true;
+}
+
+// Constant-step hillshade/slope or curvature: the row-walking kernel over
+// the interior, the general kernel over the border strips.  Falls back to the
+// general kernel alone when a divisor is outside div_rn's range.
+static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_region* reg,
+                         bool curv, hipStream_t s) {
+  const Rect in = interior(win, reg, curv ? 2 : 1);
+  RowsTile R = rows_tile(S, reg, in);
+  dim3 grd;
+  bool ok = in.h > 0 && rows_grid(in, curv ? SR_CURV_COLS : SR_SHADE_COLS, grd);
+  if (curv) {
+    R.curv = S.curv;
+    R.dx = make_divk(S.curv.ax.two_h);
+    R.dy = make_divk(S.curv.ay.two_h);
+    R.sx = make_divk(S.curv.sx.two_h);
+    R.sy = make_divk(S.curv.sy.two_h);
+    R.o0 = S.cv;
+    ok = ok && fdiv_ok(R.sx.b) && fdiv_ok(R.sy.b);
+  } else {
+    R.shade = S.shade;
+    R.dx = make_divk(S.shade.ax.two_h);
+    R.dy = make_divk(S.shade.ay.two_h);
+    R.want_hs = S.want_hs;
+    R.want_sl = S.want_sl;
+    R.o0 = S.hs;
+    R.o1 = S.sl;
+  }
+  ok = ok && fdiv_ok(R.dx.b) && fdiv_ok(R.dy.b);
+  Rect strips[4];
+  int n = 0;
+  RowsTile4 S4;
+  dim3 grd4;
+  const OutDev outs[2] = {R.o0, R.o1};
+  // NaN-free windows take four columns per lane; NaN-aware ones one column
+  // (the fills stay one per lane)
+  if (ok && !S.nanaw && rows4_setup(R, in, outs, curv ? 1 : 2, S4, grd4)) {
+    if (curv)
+      hipLaunchKernelGGL(k_curv_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
+    else
+      hipLaunchKernelGGL(k_shade_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
+    n = border_strips(reg, in, strips);
+  } else if (ok) {
+    if (curv) {
+      if (S.nanaw)
+        hipLaunchKernelGGL(k_curv_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
+      else
+        hipLaunchKernelGGL(k_curv_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
+    } else {
+      if (S.nanaw)
+        hipLaunchKernelGGL(k_shade_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
+      else
+        hipLaunchKernelGGL(k_shade_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
+    }
+    n = border_strips(reg, in, strips);
+  } else {
+    strips[n++] = Rect{reg->r0, reg->c0, reg->h, reg->w};
+  }
+  for (int k = 0; k < n; k++) {
+    const StencilTile T = sub_tile(S, reg, strips[k]);
+    if (curv)
+      launch_curv_general(T, s);
+    else
+      launch_shade_general(T, s);
+  }
+}
+
 static StencilTile stencil_tile(const fsg_window* win, const fsg_region* reg) {
   StencilTile S{};
   S.W = WinDev{win->data, win->ld, win->h, win->w};
@@ -158,13 +354,7 @@ extern "C" int fsg_hillshade_slope(const fsg_window* win, const fsg_region* reg,
   if (!S.xrows) {
     hipStream_t s = as_stream(stream);
     TimedScope ts("k_hillshade_slope", s);
-    if (S.nanaw) {
-      dim3 grd((reg->w + 63) / 64, (reg->h + 3) / 4);
-      hipLaunchKernelGGL(k_shade_px<true>, grd, dim3(256), 0, s, S, make_gauss9());
-    } else {
-      dim3 grd((reg->w + 63) / 64, (reg->h + 4 * SHADE_R - 1) / (4 * SHADE_R));
-      hipLaunchKernelGGL(k_shade_rows<false>, grd, dim3(256), 0, s, S, make_gauss9());
-    }
+    launch_split(S, win, reg, false, s);
     FSG_CHECK_LAUNCH("k_shade_direct");
     return FSG_OK;
   }
@@ -195,5 +385,12 @@ extern "C" int fsg_curvature(const fsg_window* win, const fsg_region* reg,
   S.xrows = p->step_x_rows;
   S.nanaw = p->nan_hint != 0;
   S.cv = make_out(out);
+  if (!S.xrows) {
+    hipStream_t s = as_stream(stream);
+    TimedScope ts("k_curvature", s);
+    launch_split(S, win, reg, true, s);
+    FSG_CHECK_LAUNCH("k_curvature");
+    return FSG_OK;
+  }
   return launch_stencil(S, "k_curvature", as_stream(stream));
 }

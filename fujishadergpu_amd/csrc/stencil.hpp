@@ -33,7 +33,9 @@ __device__ __forceinline__ void nan_fill_col(const WinDev& W, int r, int cc, con
   const bool v0 = !isnan(x0);
   double sv = (v0 ? (double)x0 : 0.0) * G.g[0];
   double sw = (v0 ? 1.0 : 0.0) * G.g[0];
-#pragma unroll
+  // rolled: the fill is rare, and its registers would count against every
+  // kernel that inlines it
+#pragma unroll 1
   for (int d = 4; d >= 1; d--) {
     const float a = raw(W, clampi(r - d, W.h), cc);
     const float b = raw(W, clampi(r + d, W.h), cc);
@@ -184,16 +186,9 @@ __device__ __forceinline__ float slope_from(const ShadeK& K, float dx, float dy)
   return rad;
 }
 
-// compute_hillshade_block's shading from the (unscaled) gradient; a z factor
-// other than 1 re-derives from the scaled values as the reference does.
-template <class F>
-__device__ __forceinline__ float hillshade_from(const ShadeK& K, const Axis& axr, int i, int j,
-                                                int h, int w, F f, float dx, float dy) {
-  float hdy = dy, hdx = dx;
-  if (!K.z_is_one) {
-    hdy = deriv(K.ay, i, h, [&](int k) { return f(k, j) * K.zf; });
-    hdx = deriv(axr, j, w, [&](int k) { return f(i, k) * K.zf; });
-  }
+// The shading from the z-scaled gradient (hdx, hdy): surface normal against
+// the float64 light vector, clipped to [0, 1] (_impl_hillshade.py:20-54).
+__device__ __forceinline__ float shade_from_grad(const ShadeK& K, float hdx, float hdy) {
   const float e = hdx * K.sgx;
   const float n = hdy * K.sgy;
   const float norm = sqrtf(e * e + n * n + 1.0f);
@@ -204,6 +199,19 @@ __device__ __forceinline__ float hillshade_from(const ShadeK& K, const Axis& axr
   hs = hs / (double)norm;
   hs = fmin(fmax(hs, 0.0), 1.0);
   return (float)hs;
+}
+
+// compute_hillshade_block's shading from the (unscaled) gradient; a z factor
+// other than 1 re-derives from the scaled values as the reference does.
+template <class F>
+__device__ __forceinline__ float hillshade_from(const ShadeK& K, const Axis& axr, int i, int j,
+                                                int h, int w, F f, float dx, float dy) {
+  float hdy = dy, hdx = dx;
+  if (!K.z_is_one) {
+    hdy = deriv(K.ay, i, h, [&](int k) { return f(k, j) * K.zf; });
+    hdx = deriv(axr, j, w, [&](int k) { return f(i, k) * K.zf; });
+  }
+  return shade_from_grad(K, hdx, hdy);
 }
 
 __device__ __forceinline__ float curvature_from(const CurvK& K, float p, float q, float dxx,

@@ -1,0 +1,458 @@
+// Row-walking gradient kernels for the interior of a window.
+//
+// Inside the window every derivative is numpy's central difference, so a
+// lane can own one column and walk down its rows: it loads each row's value
+// once (the NaN fill applied at the load), takes its left and right
+// neighbours' values from the adjacent lanes with whole-wave DPP shifts, and
+// keeps the rows above and below in registers.  Curvature also keeps the
+// first derivatives P = d/dcol and Q = d/drow of the rows above and below, so
+// every first derivative is formed once per position, like the tiled block's
+// tables.  A wave covers 64 columns and writes the middle 62 (hillshade,
+// slope) or 60 (curvature); the window's outer rows and columns, where
+// numpy's one-sided formulas apply, go to the general kernels.
+//
+// Divisions by f32(2h) use div_rn (the correctly rounded quotient from the
+// host's RN(1/2h)), so every value is the same float32 operation as the
+// general kernels' and the outputs are the same bits.
+#pragma once
+#include "fdiv.hpp"
+#include "stencil.hpp"
+
+namespace fsg {
+
+// The value of lane L-1 (lane 0 gets 0) and of lane L+1 (lane 63 gets 0):
+// DPP wave_shr:1 and wave_shl:1 of a 32-bit value.
+__device__ __forceinline__ float lane_left(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_right(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// A divisor f32(2h) and its reciprocal RN(1 / f32(2h)) (host computed).
+struct DivK {
+  float b, y;
+};
+
+inline DivK make_divk(float two_h) { return DivK{two_h, 1.0f / two_h}; }
+
+__device__ __forceinline__ float dq(float a, const DivK& d) { return div_rn(a, d.b, d.y); }
+
+// Columns a wave writes: 64 lanes less the neighbours the stencil reads.
+constexpr int SR_SHADE_COLS = 62, SR_CURV_COLS = 60;
+constexpr int SR_SEG_ROWS = 64;  // rows one wave walks
+
+// One interior launch: the window, the region's origin (output coordinates
+// are relative to it) and the interior rectangle [ir0, ir1) x [ic0, ic1) in
+// window coordinates.
+struct RowsTile {
+  WinDev W;
+  int r0, c0;
+  int ir0, ir1, ic0, ic1;
+  DivK dx, dy;  // f32(2|step_x|), f32(2|step_y|): first derivatives
+  DivK sx, sy;  // f32(2 step_x), f32(2 step_y): curvature's second derivatives
+  ShadeK shade;
+  CurvK curv;
+  int want_hs, want_sl;
+  OutDev o0, o1;  // hillshade and slope, or curvature in o0
+};
+
+// The window value at (r, c) with the NaN fill applied; z is the raw value.
+template <bool NANAW>
+__device__ __forceinline__ float rows_fill(const WinDev& W, int r, int c, float z,
+                                           const Gauss9& G) {
+  if (NANAW && isnan(z)) return nan_fill(W, r, c, G);
+  return z;
+}
+
+// Rows a lane has in flight ahead of the row it uses: one 256-byte load per
+// wave and row is too little to cover the memory latency on its own.
+constexpr int SR_AHEAD = 4;
+
+// Whether lane L's value is valid in any of lanes L-2 .. L+2 (a lane outside
+// the wave counts as valid).
+__device__ __forceinline__ bool near_valid2(bool v) {
+  const int x = v ? 1 : 0;
+  const int l1 = __builtin_amdgcn_update_dpp(1, x, 0x138, 0xf, 0xf, false);
+  const int r1 = __builtin_amdgcn_update_dpp(1, x, 0x130, 0xf, 0xf, false);
+  const int l2 = __builtin_amdgcn_update_dpp(1, l1, 0x138, 0xf, 0xf, false);
+  const int r2 = __builtin_amdgcn_update_dpp(1, r1, 0x130, 0xf, 0xf, false);
+  return (x | l1 | r1 | l2 | r2) != 0;
+}
+
+// Hillshade and/or slope over the interior rows [1, H-2] x cols [1, W-2].
+// Rows arrive one per iteration from ra - 1 on (the first two iterations
+// only fill).  A NaN cell takes the fill only when an output that reads it,
+// (r - 1, c), (r + 1, c) or (r, c -+ 1), is valid; the others feed NaN outputs.
+template <bool NANAW>
+__global__ void __launch_bounds__(256) k_shade_inner(RowsTile S, Gauss9 G) {
+  const int lane = threadIdx.x & 63;
+  const int c = S.ic0 - 1 + blockIdx.x * SR_SHADE_COLS + lane;
+  const int ra = S.ir0 + (blockIdx.y * 4 + (threadIdx.x >> 6)) * SR_SEG_ROWS;
+  if (ra >= S.ir1) return;  // the whole wave
+  const int rb = min(ra + SR_SEG_ROWS, S.ir1);
+  const int H = S.W.h;
+  const int cl = min(c, S.W.w - 1);  // lanes past the window read its last column
+  const bool out = lane >= 1 && lane <= SR_SHADE_COLS && c < S.ic1;
+  const int64_t ld = S.W.ld;
+  const float* p = S.W.p + (int64_t)(ra - 1) * ld + cl;  // row ra - 1 (ra >= 1)
+  const PostK phs = load_post(S.o0), psl = load_post(S.o1);
+  const ShadeK& K = S.shade;
+  // iteration r holds zd = row r + 1 and zq[k] = row r + 2 + k
+  float zd = p[0], zq[SR_AHEAD];
+#pragma unroll
+  for (int k = 0; k < SR_AHEAD; k++) zq[k] = ra + k < H ? p[(1 + k) * ld] : 0.0f;
+  p += (1 + SR_AHEAD) * ld;  // row ra + SR_AHEAD
+  float fu = 0.0f, fm = 0.0f, zm = 0.0f;
+  for (int r = ra - 2; r < rb; r++) {
+    float fd = zd;
+    if (NANAW) {
+      // every lane takes part in the shifts, before any branch
+      const float zl = lane_left(zd), zr = lane_right(zd);
+      const bool need = !isnan(zm) | !isnan(zq[0]) | !isnan(zl) | !isnan(zr);
+      if (isnan(zd)) fd = need ? nan_fill(S.W, r + 1, cl, G) : 0.0f;
+    }
+    const float fl = lane_left(fm), fr = lane_right(fm);
+    if (r >= ra && out) {
+      const int orr = r - S.r0, occ = c - S.c0;
+      if (isnan(zm)) {
+        const float qn = __builtin_nanf("");
+        if (S.want_hs) store_out(S.o0, orr, occ, apply_post(phs, qn));
+        if (S.want_sl) store_out(S.o1, orr, occ, apply_post(psl, qn));
+      } else {
+        const float dy = dq(fd - fu, S.dy);
+        const float dx = dq(fr - fl, S.dx);
+        if (S.want_sl) store_out(S.o1, orr, occ, apply_post(psl, slope_from(K, dx, dy)));
+        if (S.want_hs) {
+          float hdx = dx, hdy = dy;
+          if (!K.z_is_one) {
+            hdy = dq(fd * K.zf - fu * K.zf, S.dy);
+            hdx = dq(fr * K.zf - fl * K.zf, S.dx);
+          }
+          store_out(S.o0, orr, occ, apply_post(phs, shade_from_grad(K, hdx, hdy)));
+        }
+      }
+    }
+    fu = fm;
+    fm = fd;
+    zm = zd;
+    zd = zq[0];
+#pragma unroll
+    for (int k = 0; k + 1 < SR_AHEAD; k++) zq[k] = zq[k + 1];
+    zq[SR_AHEAD - 1] = r + 2 + SR_AHEAD < H ? p[0] : 0.0f;  // row r + 2 + SR_AHEAD
+    p += ld;
+  }
+}
+
+// Curvature over the interior rows [2, H-3] x cols [2, W-3].  Rows arrive one
+// per iteration from ra - 2 on; the first four iterations only build the rows'
+// values and first derivatives.  A NaN cell takes the fill only when a valid
+// cell lies within two rows and two columns of it (every output that reads
+// it does); the others feed NaN outputs only.
+template <bool NANAW>
+__global__ void __launch_bounds__(256) k_curv_inner(RowsTile S, Gauss9 G) {
+  const int lane = threadIdx.x & 63;
+  const int c = S.ic0 - 2 + blockIdx.x * SR_CURV_COLS + lane;
+  const int ra = S.ir0 + (blockIdx.y * 4 + (threadIdx.x >> 6)) * SR_SEG_ROWS;
+  if (ra >= S.ir1) return;  // the whole wave
+  const int rb = min(ra + SR_SEG_ROWS, S.ir1);
+  const int H = S.W.h;
+  const int cl = min(c, S.W.w - 1);
+  const bool out = lane >= 2 && lane < 2 + SR_CURV_COLS && c < S.ic1;
+  const int64_t ld = S.W.ld;
+  const float* p = S.W.p + (int64_t)(ra - 2) * ld + cl;  // row ra - 2 (ra >= 2)
+  const PostK pcv = load_post(S.o0);
+  const CurvK& K = S.curv;
+  // iteration r holds zn = row r + 2 and zq[k] = row r + 3 + k
+  float zn = p[0], zq[SR_AHEAD];
+#pragma unroll
+  for (int k = 0; k < SR_AHEAD; k++) zq[k] = ra - 1 + k < H ? p[(1 + k) * ld] : 0.0f;
+  p += (1 + SR_AHEAD) * ld;  // row ra - 1 + SR_AHEAD
+  // after iteration r: fi, fi1 = rows r + 1, r + 2 (filled); pm, p0 = P(r),
+  // P(r + 1); qm, q0 = Q(r), Q(r + 1); zc, zc1 = raw rows r + 1, r + 2
+  float fi = 0.0f, fi1 = 0.0f, pm = 0.0f, p0 = 0.0f, qm = 0.0f, q0 = 0.0f;
+  float zc = 0.0f, zc1 = 0.0f;
+  for (int r = ra - 4; r < rb; r++) {
+    const float z4 = zn;  // row r + 2
+    float f4 = z4;
+    if (NANAW) {
+      const bool col = !isnan(zc) | !isnan(zc1) | !isnan(z4) | !isnan(zq[0]) |
+                       !isnan(zq[1]);  // rows r .. r + 4 of this column
+      const bool need = near_valid2(col);
+      if (isnan(z4)) f4 = need ? nan_fill(S.W, r + 2, cl, G) : 0.0f;
+    }
+    // every lane takes part in the shifts, before any branch
+    const float l1 = lane_left(fi1), r1 = lane_right(fi1);
+    const float lp = lane_left(p0), rp = lane_right(p0);
+    const float lq = lane_left(q0), rq = lane_right(q0);
+    const float pp = dq(r1 - l1, S.dx);  // P(r + 1)
+    const float qp = dq(f4 - fi, S.dy);  // Q(r + 1)
+    if (r >= ra && out) {
+      const float dxx = dq(rp - lp, S.sx);
+      const float dyx = dq(rq - lq, S.sx);
+      const float dxy = dq(pp - pm, S.sy);
+      const float dyy = dq(qp - qm, S.sy);
+      const int orr = r - S.r0, occ = c - S.c0;
+      const float v = isnan(zc) ? __builtin_nanf("")
+                                : curvature_from(K, p0, q0, dxx, dxy, dyy, dyx);
+      store_out(S.o0, orr, occ, apply_post(pcv, v));
+    }
+    pm = p0;
+    p0 = pp;
+    qm = q0;
+    q0 = qp;
+    fi = fi1;
+    fi1 = f4;
+    zc = zc1;
+    zc1 = z4;
+    zn = zq[0];
+#pragma unroll
+    for (int k = 0; k + 1 < SR_AHEAD; k++) zq[k] = zq[k + 1];
+    zq[SR_AHEAD - 1] = r + 3 + SR_AHEAD < H ? p[0] : 0.0f;  // row r + 3 + SR_AHEAD
+    p += ld;
+  }
+}
+
+// ---------------------------------------------------------------- 4 columns
+// The same walks with four adjacent columns per lane: 16-byte loads (1 KB per
+// wave and row instead of 256 bytes, three 128-byte lines for 240 useful bytes
+// before), and only the group's outer neighbours cross lanes.  A wave covers
+// 256 columns starting at a 16-byte boundary and writes the 252 from its
+// third column on; the next wave starts 252 columns later (still aligned).
+// Needs a window row stride that is a multiple of 4 floats.
+constexpr int SR4_COLS = 252;
+constexpr int SR4_AHEAD = 1;  // rows in flight ahead of the one used (more cost occupancy)
+
+struct RowsTile4 {
+  RowsTile T;
+  int cb0;      // first column of wave 0 (16-byte aligned address in every row)
+  int n_strips; // waves across the interior
+  int vec_out;  // o0 (and o1) take aligned float4 stores at the group starts
+};
+
+// The raw values of columns c4 .. c4+3 of the row at rowp; lanes whose group
+// leaves the window read clamped columns (their values feed no output).
+__device__ __forceinline__ void load_group(const float* rowp, int c4, bool full, int w,
+                                           float z[4]) {
+  if (full) {
+    const float4 t = *(const float4*)(rowp + c4);
+    z[0] = t.x;
+    z[1] = t.y;
+    z[2] = t.z;
+    z[3] = t.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) z[k] = rowp[min(max(c4 + k, 0), w - 1)];
+  }
+}
+
+// The group's values with the NaN fill applied; the (rare) NaN cells go
+// through one call site.
+template <bool NANAW>
+__device__ __forceinline__ void fill_group(const WinDev& W, int r, int c4, const float z[4],
+                                           const Gauss9& G, float f[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) f[k] = z[k];
+  if (!NANAW) return;
+  int mask = (isnan(z[0]) ? 1 : 0) | (isnan(z[1]) ? 2 : 0) | (isnan(z[2]) ? 4 : 0) |
+             (isnan(z[3]) ? 8 : 0);
+  while (mask) {
+    const int k = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const float v = nan_fill(W, r, min(max(c4 + k, 0), W.w - 1), G);
+    f[0] = k == 0 ? v : f[0];
+    f[1] = k == 1 ? v : f[1];
+    f[2] = k == 2 ? v : f[2];
+    f[3] = k == 3 ? v : f[3];
+  }
+}
+
+// Writes v[0..3] at output row orr, columns occ .. occ+3 where out_k is set.
+__device__ __forceinline__ void store_group(const OutDev& o, bool vec, int orr, int occ,
+                                            const bool ok[4], const float v[4]) {
+  if (vec && ok[0] && ok[3]) {
+    float* d = (float*)o.data + (int64_t)orr * o.ld + occ;
+    *(float4*)d = make_float4(v[0], v[1], v[2], v[3]);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (ok[k]) store_out(o, orr, occ + k, v[k]);
+}
+
+// The wave's strip and row segment; false when the wave has no work.
+__device__ __forceinline__ bool rows4_wave(const RowsTile4& S, int& c4, int& cb, int& ra,
+                                           int& rb) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * 4 + wave;
+  ra = S.T.ir0 + blockIdx.y * SR_SEG_ROWS;
+  if (strip >= S.n_strips || ra >= S.T.ir1) return false;
+  rb = min(ra + SR_SEG_ROWS, S.T.ir1);
+  cb = S.cb0 + strip * SR4_COLS;
+  c4 = cb + 4 * lane;
+  return true;
+}
+
+// Hillshade and/or slope, four columns per lane.  Rows arrive one per
+// iteration from ra - 1 on; the first two iterations only fill (so the NaN
+// fill has one call site).
+template <bool NANAW>
+__global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
+  int c4, cb, ra, rb;
+  if (!rows4_wave(S4, c4, cb, ra, rb)) return;  // the whole wave
+  const RowsTile& S = S4.T;
+  const int H = S.W.h, Wd = S.W.w;
+  const bool full = c4 >= 0 && c4 + 3 < Wd;
+  const int lo = max(cb + 2, S.ic0), hi = min(cb + 2 + SR4_COLS, S.ic1);
+  bool ok[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) ok[k] = c4 + k >= lo && c4 + k < hi;
+  const bool any = ok[0] || ok[1] || ok[2] || ok[3];
+  const int64_t ld = S.W.ld;
+  const float* rowp = S.W.p + (int64_t)(ra - 1) * ld;  // row ra - 1 (ra >= 1)
+  const PostK phs = load_post(S.o0), psl = load_post(S.o1);
+  const ShadeK& K = S.shade;
+  // iteration r holds zd = row r + 1 and zq[a] = row r + 2 + a
+  float zd[4], zq[SR4_AHEAD][4];
+  load_group(rowp, c4, full, Wd, zd);
+#pragma unroll
+  for (int a = 0; a < SR4_AHEAD; a++) {
+    if (ra + a < H)
+      load_group(rowp + (1 + a) * ld, c4, full, Wd, zq[a]);
+    else
+      zq[a][0] = zq[a][1] = zq[a][2] = zq[a][3] = 0.0f;
+  }
+  rowp += (1 + SR4_AHEAD) * ld;  // row ra + SR4_AHEAD
+  float fu[4] = {0.f, 0.f, 0.f, 0.f}, fm[4] = {0.f, 0.f, 0.f, 0.f}, zm[4];
+  for (int r = ra - 2; r < rb; r++) {
+    float fd[4];
+    fill_group<NANAW>(S.W, r + 1, c4, zd, G, fd);
+    const float fl = lane_left(fm[3]), fr = lane_right(fm[0]);
+    if (r >= ra && any) {
+      float hs[4], sl[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float left = k == 0 ? fl : fm[k - 1];
+        const float right = k == 3 ? fr : fm[k + 1];
+        hs[k] = sl[k] = __builtin_nanf("");
+        if (!isnan(zm[k])) {
+          const float dy = dq(fd[k] - fu[k], S.dy);
+          const float dx = dq(right - left, S.dx);
+          if (S.want_sl) sl[k] = slope_from(K, dx, dy);
+          if (S.want_hs) {
+            float hdx = dx, hdy = dy;
+            if (!K.z_is_one) {
+              hdy = dq(fd[k] * K.zf - fu[k] * K.zf, S.dy);
+              hdx = dq(right * K.zf - left * K.zf, S.dx);
+            }
+            hs[k] = shade_from_grad(K, hdx, hdy);
+          }
+        }
+        hs[k] = apply_post(phs, hs[k]);
+        sl[k] = apply_post(psl, sl[k]);
+      }
+      const int orr = r - S.r0, occ = c4 - S.c0;
+      if (S.want_hs) store_group(S.o0, S4.vec_out, orr, occ, ok, hs);
+      if (S.want_sl) store_group(S.o1, S4.vec_out, orr, occ, ok, sl);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      fu[k] = fm[k];
+      fm[k] = fd[k];
+      zm[k] = zd[k];
+      zd[k] = zq[0][k];
+    }
+#pragma unroll
+    for (int a = 0; a + 1 < SR4_AHEAD; a++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
+    if (r + 2 + SR4_AHEAD < H)  // row r + 2 + SR4_AHEAD
+      load_group(rowp, c4, full, Wd, zq[SR4_AHEAD - 1]);
+    rowp += ld;
+  }
+}
+
+// Curvature, four columns per lane.  Rows arrive one per iteration from
+// ra - 2 on; the first four iterations only build the rows' values and first
+// derivatives.
+template <bool NANAW>
+__global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
+  int c4, cb, ra, rb;
+  if (!rows4_wave(S4, c4, cb, ra, rb)) return;  // the whole wave
+  const RowsTile& S = S4.T;
+  const int H = S.W.h, Wd = S.W.w;
+  const bool full = c4 >= 0 && c4 + 3 < Wd;
+  const int lo = max(cb + 2, S.ic0), hi = min(cb + 2 + SR4_COLS, S.ic1);
+  bool ok[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) ok[k] = c4 + k >= lo && c4 + k < hi;
+  const bool any = ok[0] || ok[1] || ok[2] || ok[3];
+  const int64_t ld = S.W.ld;
+  const float* rowp = S.W.p + (int64_t)(ra - 2) * ld;  // row ra - 2 (ra >= 2)
+  const PostK pcv = load_post(S.o0);
+  const CurvK& K = S.curv;
+  // iteration r holds zn = row r + 2 and zq[a] = row r + 3 + a
+  float zn[4], zq[SR4_AHEAD][4];
+  load_group(rowp, c4, full, Wd, zn);
+#pragma unroll
+  for (int a = 0; a < SR4_AHEAD; a++) {
+    if (ra - 1 + a < H)
+      load_group(rowp + (1 + a) * ld, c4, full, Wd, zq[a]);
+    else
+      zq[a][0] = zq[a][1] = zq[a][2] = zq[a][3] = 0.0f;
+  }
+  rowp += (1 + SR4_AHEAD) * ld;  // row ra - 1 + SR4_AHEAD
+  // after iteration r: fi, fi1 = rows r + 1, r + 2 (filled); pm, p0 = P(r),
+  // P(r + 1); qm, q0 = Q(r), Q(r + 1); zc, zc1 = raw rows r + 1, r + 2
+  float fi[4] = {0.f, 0.f, 0.f, 0.f}, fi1[4] = {0.f, 0.f, 0.f, 0.f};
+  float pm[4] = {0.f, 0.f, 0.f, 0.f}, p0[4] = {0.f, 0.f, 0.f, 0.f};
+  float qm[4] = {0.f, 0.f, 0.f, 0.f}, q0[4] = {0.f, 0.f, 0.f, 0.f};
+  float zc[4], zc1[4];
+  for (int r = ra - 4; r < rb; r++) {
+    float f4[4], pp[4], qp[4];
+    fill_group<NANAW>(S.W, r + 2, c4, zn, G, f4);
+    const float l1 = lane_left(fi1[3]), r1 = lane_right(fi1[0]);
+    const float lp = lane_left(p0[3]), rp = lane_right(p0[0]);
+    const float lq = lane_left(q0[3]), rq = lane_right(q0[0]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      pp[k] = dq((k == 3 ? r1 : fi1[k + 1]) - (k == 0 ? l1 : fi1[k - 1]), S.dx);  // P(r + 1)
+      qp[k] = dq(f4[k] - fi[k], S.dy);                                            // Q(r + 1)
+    }
+    if (r >= ra && any) {
+      float v[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float dxx = dq((k == 3 ? rp : p0[k + 1]) - (k == 0 ? lp : p0[k - 1]), S.sx);
+        const float dyx = dq((k == 3 ? rq : q0[k + 1]) - (k == 0 ? lq : q0[k - 1]), S.sx);
+        const float dxy = dq(pp[k] - pm[k], S.sy);
+        const float dyy = dq(qp[k] - qm[k], S.sy);
+        v[k] = isnan(zc[k]) ? __builtin_nanf("")
+                            : curvature_from(K, p0[k], q0[k], dxx, dxy, dyy, dyx);
+        v[k] = apply_post(pcv, v[k]);
+      }
+      store_group(S.o0, S4.vec_out, r - S.r0, c4 - S.c0, ok, v);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      pm[k] = p0[k];
+      p0[k] = pp[k];
+      qm[k] = q0[k];
+      q0[k] = qp[k];
+      fi[k] = fi1[k];
+      fi1[k] = f4[k];
+      zc[k] = zc1[k];
+      zc1[k] = zn[k];
+      zn[k] = zq[0][k];
+    }
+#pragma unroll
+    for (int a = 0; a + 1 < SR4_AHEAD; a++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
+    if (r + 3 + SR4_AHEAD < H)  // row r + 3 + SR4_AHEAD
+      load_group(rowp, c4, full, Wd, zq[SR4_AHEAD - 1]);
+    rowp += ld;
+  }
+}
+
+}  // namespace fsg
