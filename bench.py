@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size", type=int, default=N_PX)
+    ap.add_argument("--size", type=int, default=None,
+                    help="raster side: C2 (default 16384) or the stencils workload (32768)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=0, help="tiles in the CPU sample (0 = auto)")
     ap.add_argument("--only-c2", action="store_true",
@@ -708,9 +709,12 @@ def main():
         fn = {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
               "stencils": bench_stencils,
               "f1": bench_f1, "f2": bench_f2, "f4": bench_f4}[args.workload]
-        print(json.dumps({args.workload: fn(dev)}), flush=True)
+        # --size resizes the stencils raster (its default is 32768^2); the
+        # other workloads keep their fixed sizes
+        kw = {"n": args.size} if args.workload == "stencils" and args.size else {}
+        print(json.dumps({args.workload: fn(dev, **kw)}), flush=True)
         return
-    n = args.size
+    n = args.size or N_PX
     dem = make_dem(n, dev)
     out_u8 = torch.empty((n, n), dtype=torch.uint8, device=dev)
     out_f32 = torch.empty((n, n), dtype=torch.float32, device=dev)

@@ -217,18 +217,30 @@ static int quad_phase(const void* p) {
 
 static int mod4(int x) { return ((x % 4) + 4) % 4; }
 
-// The four-columns-per-lane launch: needs a row stride that is a multiple of
-// 4 floats and a 4-byte aligned window; float4 stores where the outputs line
-// up with the groups.
+// The four-columns-per-lane launch over the interior columns [oc0, oc1) that
+// whole aligned waves cover: needs a row stride that is a multiple of 4
+// floats and a 4-byte aligned window; each wave's 256 columns lie inside the
+// window; float4-aligned outputs take 16-byte stores.  False when no wave fits.
 static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, int n_out,
-                        RowsTile4& S4, dim3& grd) {
+                        int left, RowsTile4& S4, dim3& grd, int& oc0, int& oc1) {
+  const int cols = 256 - 2 * left;
   const int a = quad_phase(R.W.p);
-  if (a < 0 || R.W.ld % 4 != 0) return false;
-  const int cb0 = (in.c0 - 2) - mod4(a + in.c0 - 2);
-  const int64_t n_strips = ((int64_t)(in.c0 + in.w) - (cb0 + 2) + SR4_COLS - 1) / SR4_COLS;
+  if (a < 0 || R.W.ld % 4 != 0 || in.w == 0) return false;
+  int cb0 = (in.c0 - left) - mod4(a + in.c0 - left);  // aligned, cb0 + left <= in.c0
+  if (cb0 < 0) cb0 += 4 * ((-cb0 + 3) / 4);      // first aligned start inside the window
+  const int cb_max = R.W.w - 256;                // a wave's last column <= W - 1
+  const int in_c1 = in.c0 + in.w;
+  if (cb0 > cb_max) return false;
+  const int64_t n_fit = (cb_max - cb0) / cols + 1;
+  const int64_t n_need = ((int64_t)in_c1 - (cb0 + left) + cols - 1) / cols;
+  const int64_t n_strips = std::min(n_fit, n_need);
+  oc0 = std::max(in.c0, cb0 + left);
+  oc1 = (int)std::min<int64_t>(in_c1, cb0 + left + n_strips * cols);
   const int64_t nseg = ((int64_t)in.h + SR_SEG_ROWS - 1) / SR_SEG_ROWS;
-  if (nseg > 65535 || n_strips > 0x7ffffff) return false;
+  if (n_strips <= 0 || oc0 >= oc1 || nseg > 65535) return false;
   S4.T = R;
+  S4.T.ic0 = oc0;
+  S4.T.ic1 = oc1;
   S4.cb0 = cb0;
   S4.n_strips = (int)n_strips;
   int vec = 1;
@@ -242,6 +254,28 @@ static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, i
   S4.vec_out = vec;
   grd = dim3((unsigned)((n_strips + 3) / 4), (unsigned)nseg);
   return true;
+}
+
+// The one-column kernel over the interior rows of R and columns [c0, c1).
+static void launch_rows1(RowsTile R, const Rect& in, int c0, int c1, bool curv, bool nanaw,
+                         hipStream_t s) {
+  if (c0 >= c1) return;
+  R.ic0 = c0;
+  R.ic1 = c1;
+  dim3 grd;
+  if (!rows_grid(Rect{in.r0, c0, in.h, c1 - c0}, curv ? SR_CURV_COLS : SR_SHADE_COLS, grd))
+    return;
+  if (curv) {
+    if (nanaw)
+      hipLaunchKernelGGL(k_curv_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
+    else
+      hipLaunchKernelGGL(k_curv_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
+  } else {
+    if (nanaw)
+      hipLaunchKernelGGL(k_shade_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
+    else
+      hipLaunchKernelGGL(k_shade_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
+  }
 }
 
 // Constant-step hillshade/slope or curvature: the row-walking kernel over
@@ -275,26 +309,22 @@ static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_
   int n = 0;
   RowsTile4 S4;
   dim3 grd4;
+  int oc0 = 0, oc1 = 0;
   const OutDev outs[2] = {R.o0, R.o1};
-  // NaN-free windows take four columns per lane; NaN-aware ones one column
-  // (the fills stay one per lane)
-  if (ok && !S.nanaw && rows4_setup(R, in, outs, curv ? 1 : 2, S4, grd4)) {
-    if (curv)
-      hipLaunchKernelGGL(k_curv_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
-    else
-      hipLaunchKernelGGL(k_shade_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
-    n = border_strips(reg, in, strips);
-  } else if (ok) {
-    if (curv) {
-      if (S.nanaw)
-        hipLaunchKernelGGL(k_curv_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
+  if (ok) {
+    // NaN-free windows take four columns per lane where whole waves fit and
+    // one column at the sides; NaN-aware ones one column (one fill per lane)
+    const int ic1 = in.c0 + in.w;
+    if (!S.nanaw && rows4_setup(R, in, outs, curv ? 1 : 2, curv ? SR4_LEFT_CURV : SR4_LEFT_SHADE,
+                                S4, grd4, oc0, oc1)) {
+      if (curv)
+        hipLaunchKernelGGL(k_curv_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
       else
-        hipLaunchKernelGGL(k_curv_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
+        hipLaunchKernelGGL(k_shade_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
+      launch_rows1(R, in, in.c0, oc0, curv, false, s);
+      launch_rows1(R, in, oc1, ic1, curv, false, s);
     } else {
-      if (S.nanaw)
-        hipLaunchKernelGGL(k_shade_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
-      else
-        hipLaunchKernelGGL(k_shade_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
+      launch_rows1(R, in, in.c0, ic1, curv, S.nanaw != 0, s);
     }
     n = border_strips(reg, in, strips);
   } else {

@@ -26,23 +26,31 @@ __device__ __forceinline__ float raw(const WinDev& W, int r, int c) {
 // filled*valid and of valid, axis 0 then axis 1 with float32 intermediates,
 // symmetric accumulation order (centre, then distance 4..1), then
 // where(sw > 0, sv / sw, 0) in float32 (_nan_utils.py:18-31).
-// axis-0 pass of that fill at column cc: (sv, sw) rounded to float32
+// axis-0 pass of that fill at column cc: (sv, sw) rounded to float32.
+// ROLLED keeps the nine loads in a loop (the per-cell fill is rare, and its
+// registers would count against every kernel that inlines it); the unrolled
+// form issues them together.
+template <bool ROLLED = true>
 __device__ __forceinline__ void nan_fill_col(const WinDev& W, int r, int cc, const Gauss9& G,
                                              float& fv, float& fw) {
   const float x0 = raw(W, r, cc);
   const bool v0 = !isnan(x0);
   double sv = (v0 ? (double)x0 : 0.0) * G.g[0];
   double sw = (v0 ? 1.0 : 0.0) * G.g[0];
-  // rolled: the fill is rare, and its registers would count against every
-  // kernel that inlines it
-#pragma unroll 1
-  for (int d = 4; d >= 1; d--) {
+  auto step = [&](int d) {
     const float a = raw(W, clampi(r - d, W.h), cc);
     const float b = raw(W, clampi(r + d, W.h), cc);
     const double fa = isnan(a) ? 0.0 : (double)a, fb = isnan(b) ? 0.0 : (double)b;
     const double wa = isnan(a) ? 0.0 : 1.0, wb = isnan(b) ? 0.0 : 1.0;
     sv += (fa + fb) * G.g[d];
     sw += (wa + wb) * G.g[d];
+  };
+  if (ROLLED) {
+#pragma unroll 1
+    for (int d = 4; d >= 1; d--) step(d);
+  } else {
+#pragma unroll
+    for (int d = 4; d >= 1; d--) step(d);
   }
   fv = (float)sv;
   fw = (float)sw;

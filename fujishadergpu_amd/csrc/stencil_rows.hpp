@@ -80,6 +80,51 @@ __device__ __forceinline__ bool near_valid2(bool v) {
   return (x | l1 | r1 | l2 | r2) != 0;
 }
 
+// Column sums the wave fill keeps per wave: its 64 columns and 4 either side.
+constexpr int SR_FILL_COLS = 72;
+
+// The NaN fill of row x for the lanes with `want`, computed by the whole wave
+// at once: every lane forms the axis-0 pass of its own column (nan_fill_col,
+// nine loads issued together), lanes 0..7 also the 4 columns either side of
+// the wave, and the axis-1 pass reads the nine column sums from LDS in
+// nan_fill's order (centre, then distance 4..1), so the value is nan_fill's
+// bits.  c is the lane's column (cb + lane), cb the wave's first column.
+// Every lane of the wave must call it (it exchanges through LDS).
+__device__ __forceinline__ float wave_fill(const WinDev& W, int x, int c, int cb, bool want,
+                                           const Gauss9& G, float2* cols) {
+  const int lane = threadIdx.x & 63;
+  float cv, cw;
+  nan_fill_col<false>(W, x, clampi(c, W.w), G, cv, cw);
+  cols[4 + lane] = make_float2(cv, cw);
+  if (lane < 8) {
+    const int col = lane < 4 ? cb - 4 + lane : cb + 64 + (lane - 4);
+    float ev, ew;
+    nan_fill_col<false>(W, x, clampi(col, W.w), G, ev, ew);
+    cols[lane < 4 ? lane : 64 + lane] = make_float2(ev, ew);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  float f = 0.0f;
+  if (want) {
+    double sv = (double)cv * G.g[0];
+    double sw = (double)cw * G.g[0];
+#pragma unroll
+    for (int d = 4; d >= 1; d--) {
+      const float2 a = cols[4 + lane - d], b = cols[4 + lane + d];
+      sv += ((double)a.x + (double)b.x) * G.g[d];
+      sw += ((double)a.y + (double)b.y) * G.g[d];
+    }
+    const float fsv = (float)sv, fsw = (float)sw;
+    f = fsw > 0.0f ? fsv / fsw : 0.0f;
+  }
+  // the next call overwrites the sums
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return f;
+}
+
 // Hillshade and/or slope over the interior rows [1, H-2] x cols [1, W-2].
 // Rows arrive one per iteration from ra - 1 on (the first two iterations
 // only fill).  A NaN cell takes the fill only when an output that reads it,
@@ -104,13 +149,18 @@ __global__ void __launch_bounds__(256) k_shade_inner(RowsTile S, Gauss9 G) {
   for (int k = 0; k < SR_AHEAD; k++) zq[k] = ra + k < H ? p[(1 + k) * ld] : 0.0f;
   p += (1 + SR_AHEAD) * ld;  // row ra + SR_AHEAD
   float fu = 0.0f, fm = 0.0f, zm = 0.0f;
+  __shared__ float2 fill_cols[4][SR_FILL_COLS];
   for (int r = ra - 2; r < rb; r++) {
     float fd = zd;
     if (NANAW) {
       // every lane takes part in the shifts, before any branch
       const float zl = lane_left(zd), zr = lane_right(zd);
-      const bool need = !isnan(zm) | !isnan(zq[0]) | !isnan(zl) | !isnan(zr);
-      if (isnan(zd)) fd = need ? nan_fill(S.W, r + 1, cl, G) : 0.0f;
+      const bool want = isnan(zd) & (!isnan(zm) | !isnan(zq[0]) | !isnan(zl) | !isnan(zr));
+      if (__ballot(want)) {  // the whole wave
+        const float f = wave_fill(S.W, r + 1, c, c - lane, want, G, fill_cols[threadIdx.x >> 6]);
+        if (want) fd = f;
+      }
+      if (isnan(fd)) fd = 0.0f;  // NaN cells no output reads
     }
     const float fl = lane_left(fm), fr = lane_right(fm);
     if (r >= ra && out) {
@@ -172,14 +222,19 @@ __global__ void __launch_bounds__(256) k_curv_inner(RowsTile S, Gauss9 G) {
   // P(r + 1); qm, q0 = Q(r), Q(r + 1); zc, zc1 = raw rows r + 1, r + 2
   float fi = 0.0f, fi1 = 0.0f, pm = 0.0f, p0 = 0.0f, qm = 0.0f, q0 = 0.0f;
   float zc = 0.0f, zc1 = 0.0f;
+  __shared__ float2 fill_cols[4][SR_FILL_COLS];
   for (int r = ra - 4; r < rb; r++) {
     const float z4 = zn;  // row r + 2
     float f4 = z4;
     if (NANAW) {
       const bool col = !isnan(zc) | !isnan(zc1) | !isnan(z4) | !isnan(zq[0]) |
                        !isnan(zq[1]);  // rows r .. r + 4 of this column
-      const bool need = near_valid2(col);
-      if (isnan(z4)) f4 = need ? nan_fill(S.W, r + 2, cl, G) : 0.0f;
+      const bool want = isnan(z4) & near_valid2(col);
+      if (__ballot(want)) {  // the whole wave
+        const float f = wave_fill(S.W, r + 2, c, c - lane, want, G, fill_cols[threadIdx.x >> 6]);
+        if (want) f4 = f;
+      }
+      if (isnan(f4)) f4 = 0.0f;  // NaN cells no output reads
     }
     // every lane takes part in the shifts, before any branch
     const float l1 = lane_left(fi1), r1 = lane_right(fi1);
@@ -217,10 +272,20 @@ __global__ void __launch_bounds__(256) k_curv_inner(RowsTile S, Gauss9 G) {
 // The same walks with four adjacent columns per lane: 16-byte loads (1 KB per
 // wave and row instead of 256 bytes, three 128-byte lines for 240 useful bytes
 // before), and only the group's outer neighbours cross lanes.  A wave covers
-// 256 columns starting at a 16-byte boundary and writes the 252 from its
-// third column on; the next wave starts 252 columns later (still aligned).
-// Needs a window row stride that is a multiple of 4 floats.
-constexpr int SR4_COLS = 252;
+// 256 columns starting at a 16-byte boundary and writes 256 - 2 LEFT of them
+// from column LEFT on; the next wave starts that many columns later (still
+// aligned).  LEFT = 4: lanes 1..62 write whole groups (16-byte stores), lanes
+// 0 and 63 only supply neighbours; LEFT = 2: every lane writes, the first
+// and last lanes half a group (8-byte stores).  Measured: hillshade/slope
+// (two outputs) run faster with LEFT = 4, curvature with LEFT = 2.
+// Needs a window row stride that is a multiple of 4 floats; waves go only
+// where their 256 columns lie inside the window, and the interior columns
+// left at its sides take the one-column kernel.
+template <int LEFT>
+constexpr int sr4_cols() {
+  return 256 - 2 * LEFT;
+}
+constexpr int SR4_LEFT_SHADE = 4, SR4_LEFT_CURV = 2;
 constexpr int SR4_AHEAD = 1;  // rows in flight ahead of the one used (more cost occupancy)
 
 struct RowsTile4 {
@@ -230,20 +295,14 @@ struct RowsTile4 {
   int vec_out;  // o0 (and o1) take aligned float4 stores at the group starts
 };
 
-// The raw values of columns c4 .. c4+3 of the row at rowp; lanes whose group
-// leaves the window read clamped columns (their values feed no output).
-__device__ __forceinline__ void load_group(const float* rowp, int c4, bool full, int w,
-                                           float z[4]) {
-  if (full) {
-    const float4 t = *(const float4*)(rowp + c4);
-    z[0] = t.x;
-    z[1] = t.y;
-    z[2] = t.z;
-    z[3] = t.w;
-  } else {
-#pragma unroll
-    for (int k = 0; k < 4; k++) z[k] = rowp[min(max(c4 + k, 0), w - 1)];
-  }
+// The raw values of columns c4 .. c4+3 of the row at rowp (16-byte aligned:
+// the launch only places waves whose 256 columns lie inside the window).
+__device__ __forceinline__ void load_group(const float* rowp, int c4, float z[4]) {
+  const float4 t = *(const float4*)(rowp + c4);
+  z[0] = t.x;
+  z[1] = t.y;
+  z[2] = t.z;
+  z[3] = t.w;
 }
 
 // The group's values with the NaN fill applied; the (rare) NaN cells go
@@ -267,13 +326,25 @@ __device__ __forceinline__ void fill_group(const WinDev& W, int r, int c4, const
   }
 }
 
-// Writes v[0..3] at output row orr, columns occ .. occ+3 where out_k is set.
+// Writes v[0..3] at output row orr, columns occ .. occ+3 where ok[k] is set.
+// vec: the group starts on a 16-byte boundary of a float32 output.  HALVES
+// stores each half as one 8-byte store (a wave's first and last lanes write
+// one half each), else a whole group as one 16-byte store; groups cut by the
+// region's first or last column, and other outputs, store per column.
+template <bool HALVES>
 __device__ __forceinline__ void store_group(const OutDev& o, bool vec, int orr, int occ,
                                             const bool ok[4], const float v[4]) {
-  if (vec && ok[0] && ok[3]) {
+  if (vec) {
     float* d = (float*)o.data + (int64_t)orr * o.ld + occ;
-    *(float4*)d = make_float4(v[0], v[1], v[2], v[3]);
-    return;
+    if (!HALVES && ok[0] && ok[3]) {
+      *(float4*)d = make_float4(v[0], v[1], v[2], v[3]);
+      return;
+    }
+    if (HALVES) {
+      if (ok[0] && ok[1]) *(float2*)d = make_float2(v[0], v[1]);
+      if (ok[2] && ok[3]) *(float2*)(d + 2) = make_float2(v[2], v[3]);
+      if (ok[0] == ok[1] && ok[2] == ok[3]) return;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 4; k++)
@@ -281,6 +352,7 @@ __device__ __forceinline__ void store_group(const OutDev& o, bool vec, int orr, 
 }
 
 // The wave's strip and row segment; false when the wave has no work.
+template <int LEFT>
 __device__ __forceinline__ bool rows4_wave(const RowsTile4& S, int& c4, int& cb, int& ra,
                                            int& rb) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -288,7 +360,7 @@ __device__ __forceinline__ bool rows4_wave(const RowsTile4& S, int& c4, int& cb,
   ra = S.T.ir0 + blockIdx.y * SR_SEG_ROWS;
   if (strip >= S.n_strips || ra >= S.T.ir1) return false;
   rb = min(ra + SR_SEG_ROWS, S.T.ir1);
-  cb = S.cb0 + strip * SR4_COLS;
+  cb = S.cb0 + strip * sr4_cols<LEFT>();
   c4 = cb + 4 * lane;
   return true;
 }
@@ -298,12 +370,12 @@ __device__ __forceinline__ bool rows4_wave(const RowsTile4& S, int& c4, int& cb,
 // fill has one call site).
 template <bool NANAW>
 __global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
+  constexpr int LEFT = SR4_LEFT_SHADE;
   int c4, cb, ra, rb;
-  if (!rows4_wave(S4, c4, cb, ra, rb)) return;  // the whole wave
+  if (!rows4_wave<LEFT>(S4, c4, cb, ra, rb)) return;  // the whole wave
   const RowsTile& S = S4.T;
-  const int H = S.W.h, Wd = S.W.w;
-  const bool full = c4 >= 0 && c4 + 3 < Wd;
-  const int lo = max(cb + 2, S.ic0), hi = min(cb + 2 + SR4_COLS, S.ic1);
+  const int H = S.W.h;
+  const int lo = max(cb + LEFT, S.ic0), hi = min(cb + LEFT + sr4_cols<LEFT>(), S.ic1);
   bool ok[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) ok[k] = c4 + k >= lo && c4 + k < hi;
@@ -314,11 +386,11 @@ __global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
   const ShadeK& K = S.shade;
   // iteration r holds zd = row r + 1 and zq[a] = row r + 2 + a
   float zd[4], zq[SR4_AHEAD][4];
-  load_group(rowp, c4, full, Wd, zd);
+  load_group(rowp, c4, zd);
 #pragma unroll
   for (int a = 0; a < SR4_AHEAD; a++) {
     if (ra + a < H)
-      load_group(rowp + (1 + a) * ld, c4, full, Wd, zq[a]);
+      load_group(rowp + (1 + a) * ld, c4, zq[a]);
     else
       zq[a][0] = zq[a][1] = zq[a][2] = zq[a][3] = 0.0f;
   }
@@ -352,8 +424,8 @@ __global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
         sl[k] = apply_post(psl, sl[k]);
       }
       const int orr = r - S.r0, occ = c4 - S.c0;
-      if (S.want_hs) store_group(S.o0, S4.vec_out, orr, occ, ok, hs);
-      if (S.want_sl) store_group(S.o1, S4.vec_out, orr, occ, ok, sl);
+      if (S.want_hs) store_group<LEFT == 2>(S.o0, S4.vec_out, orr, occ, ok, hs);
+      if (S.want_sl) store_group<LEFT == 2>(S.o1, S4.vec_out, orr, occ, ok, sl);
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -367,7 +439,7 @@ __global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
 #pragma unroll
       for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
     if (r + 2 + SR4_AHEAD < H)  // row r + 2 + SR4_AHEAD
-      load_group(rowp, c4, full, Wd, zq[SR4_AHEAD - 1]);
+      load_group(rowp, c4, zq[SR4_AHEAD - 1]);
     rowp += ld;
   }
 }
@@ -377,12 +449,12 @@ __global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
 // derivatives.
 template <bool NANAW>
 __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
+  constexpr int LEFT = SR4_LEFT_CURV;
   int c4, cb, ra, rb;
-  if (!rows4_wave(S4, c4, cb, ra, rb)) return;  // the whole wave
+  if (!rows4_wave<LEFT>(S4, c4, cb, ra, rb)) return;  // the whole wave
   const RowsTile& S = S4.T;
-  const int H = S.W.h, Wd = S.W.w;
-  const bool full = c4 >= 0 && c4 + 3 < Wd;
-  const int lo = max(cb + 2, S.ic0), hi = min(cb + 2 + SR4_COLS, S.ic1);
+  const int H = S.W.h;
+  const int lo = max(cb + LEFT, S.ic0), hi = min(cb + LEFT + sr4_cols<LEFT>(), S.ic1);
   bool ok[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) ok[k] = c4 + k >= lo && c4 + k < hi;
@@ -393,11 +465,11 @@ __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
   const CurvK& K = S.curv;
   // iteration r holds zn = row r + 2 and zq[a] = row r + 3 + a
   float zn[4], zq[SR4_AHEAD][4];
-  load_group(rowp, c4, full, Wd, zn);
+  load_group(rowp, c4, zn);
 #pragma unroll
   for (int a = 0; a < SR4_AHEAD; a++) {
     if (ra - 1 + a < H)
-      load_group(rowp + (1 + a) * ld, c4, full, Wd, zq[a]);
+      load_group(rowp + (1 + a) * ld, c4, zq[a]);
     else
       zq[a][0] = zq[a][1] = zq[a][2] = zq[a][3] = 0.0f;
   }
@@ -431,7 +503,7 @@ __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
                             : curvature_from(K, p0[k], q0[k], dxx, dxy, dyy, dyx);
         v[k] = apply_post(pcv, v[k]);
       }
-      store_group(S.o0, S4.vec_out, r - S.r0, c4 - S.c0, ok, v);
+      store_group<LEFT == 2>(S.o0, S4.vec_out, r - S.r0, c4 - S.c0, ok, v);
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -450,7 +522,7 @@ __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
 #pragma unroll
       for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
     if (r + 3 + SR4_AHEAD < H)  // row r + 3 + SR4_AHEAD
-      load_group(rowp, c4, full, Wd, zq[SR4_AHEAD - 1]);
+      load_group(rowp, c4, zq[SR4_AHEAD - 1]);
     rowp += ld;
   }
 }

@@ -132,6 +132,52 @@ def test_curvature_window(cuda, ctype, nan_discs):
     assert (d <= 1e-6).mean() >= 0.9, f"only {(d <= 1e-6).mean():.3f} within 1e-6"
 
 
+@pytest.mark.parametrize("nan_discs", [0, 6])
+@pytest.mark.parametrize("off", [0, 1, 2, 3])
+def test_stencils_wide_windows(cuda, nan_discs, off):
+    """Windows wide enough for the four-columns-per-lane interior kernels (a
+    wave spans 256 columns).  The window view starts `off` floats past a
+    16-byte boundary of a row stride that is a multiple of 4, so the aligned
+    wave starts, the one-column side strips, the 8-byte output stores
+    (off == 0) and the per-column stores (off != 0) all run; a region inside
+    the window (output origin away from the window's) runs them again."""
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.algorithms import ALGORITHMS, CurvatureAlgorithm, \
+        HillshadeSlopeFused, OutSpec
+    dem = _dem(150, 1104, 21 + off, nan_discs=nan_discs)
+    g = dem.to(cuda)[:, off:]
+    a = dem[:, off:].numpy()
+    kw = dict(pixel_size=10.0)
+    hs = ALGORITHMS["hillshade"].process(g, **kw)
+    sl = ALGORITHMS["slope"].process(g, **kw)
+    ref_hs, ref_sl = ref.hillshade(a, **kw), ref.slope(a, **kw)
+    mx, exact = _cmp("hillshade", hs, ref_hs, 1e-6)
+    if not nan_discs:
+        assert exact == 1.0, f"hillshade not bit-exact ({exact:.4f}, max {mx})"
+    _cmp("slope", sl, ref_sl, 2e-5, 1e-6)
+    cv = ALGORITHMS["curvature"].process(g, curvature_type="mean", **kw).cpu().numpy()
+    ref_cv = ref.curvature(a, curvature_type="mean", **kw)
+    _cmp("curvature", cv, ref_cv, 4e-4)
+    d = np.abs(cv - ref_cv)[~np.isnan(ref_cv)]
+    assert (d <= 1e-6).mean() >= 0.9, f"only {(d <= 1e-6).mean():.3f} within 1e-6"
+    # a region of the window: rows 5..124, columns 37 .. 37+900
+    r0, c0, rh, rw = 5, 37, 120, 900
+    reg = ops.region(r0, c0, rh, rw)
+    p = {"pixel_size": 10.0, "curvature_type": "mean", "_window_nan_free": not nan_discs}
+    o_hs = torch.empty((rh, rw), dtype=torch.float32, device=cuda)
+    o_sl = torch.empty_like(o_hs)
+    o_cv = torch.empty_like(o_hs)
+    HillshadeSlopeFused().run_region(g, reg, OutSpec(f32=o_hs, clip01=True), p,
+                                     out_slope=OutSpec(f32=o_sl))
+    CurvatureAlgorithm().run_region(g, reg, OutSpec(f32=o_cv), p)
+    sub = (slice(r0, r0 + rh), slice(c0, c0 + rw))
+    mx, exact = _cmp("hillshade/region", o_hs, ref_hs[sub], 1e-6)
+    if not nan_discs:
+        assert exact == 1.0, f"hillshade/region not bit-exact ({exact:.4f}, max {mx})"
+    _cmp("slope/region", o_sl, ref_sl[sub], 2e-5, 1e-6)
+    _cmp("curvature/region", o_cv, ref_cv[sub], 4e-4)
+
+
 @pytest.mark.parametrize("radii,weights,px", [
     ([2, 8, 32], [0.5, 0.3, 0.2], 5.0),
     ([2, 8, 32, 128], [8 / 15, 4 / 15, 2 / 15, 1 / 15], 5.0),
