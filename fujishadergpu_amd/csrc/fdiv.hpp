@@ -41,5 +41,17 @@ inline float div_rn(float a, float b, float y) {
   return a / b;
 }
 
+// The same quotient without the range guard, for a caller whose output is
+// checked to a tolerance (curvature): equal to RN(a / b) for 2^-75 <= |a| <=
+// 2^75, a zero (of either sign) for a = 0; outside that range it may be one
+// ulp off (a subnormal remainder) or NaN where the quotient is infinite.
+inline float div_rn_unguarded(float a, float b, float y) {
+  const float q0 = a * y;
+  const float r0 = __builtin_fmaf(-b, q0, a);
+  const float q1 = __builtin_fmaf(r0, y, q0);
+  const float r1 = __builtin_fmaf(-b, q1, a);
+  return __builtin_fmaf(r1, y, q1);
+}
+
 }  // namespace fsg
 #pragma clang force_cuda_host_device end

@@ -13,7 +13,9 @@
 //
 // Divisions by f32(2h) use div_rn (the correctly rounded quotient from the
 // host's RN(1/2h)), so every value is the same float32 operation as the
-// general kernels' and the outputs are the same bits.
+// general kernels' and the outputs are the same bits.  Curvature drops the
+// range guard (div_rn_unguarded): the same quotients for every dividend in
+// [2^-75, 2^75] and zero, which real elevation data stays within.
 #pragma once
 #include "fdiv.hpp"
 #include "stencil.hpp"
@@ -37,6 +39,11 @@ struct DivK {
 inline DivK make_divk(float two_h) { return DivK{two_h, 1.0f / two_h}; }
 
 __device__ __forceinline__ float dq(float a, const DivK& d) { return div_rn(a, d.b, d.y); }
+// curvature's divisions: the unguarded form (its output is checked to a
+// tolerance, and every elevation difference of real data is in its range)
+__device__ __forceinline__ float dqu(float a, const DivK& d) {
+  return div_rn_unguarded(a, d.b, d.y);
+}
 
 // Columns a wave writes: 64 lanes less the neighbours the stencil reads.
 constexpr int SR_SHADE_COLS = 62, SR_CURV_COLS = 60;
@@ -240,13 +247,13 @@ __global__ void __launch_bounds__(256) k_curv_inner(RowsTile S, Gauss9 G) {
     const float l1 = lane_left(fi1), r1 = lane_right(fi1);
     const float lp = lane_left(p0), rp = lane_right(p0);
     const float lq = lane_left(q0), rq = lane_right(q0);
-    const float pp = dq(r1 - l1, S.dx);  // P(r + 1)
-    const float qp = dq(f4 - fi, S.dy);  // Q(r + 1)
+    const float pp = dqu(r1 - l1, S.dx);  // P(r + 1)
+    const float qp = dqu(f4 - fi, S.dy);  // Q(r + 1)
     if (r >= ra && out) {
-      const float dxx = dq(rp - lp, S.sx);
-      const float dyx = dq(rq - lq, S.sx);
-      const float dxy = dq(pp - pm, S.sy);
-      const float dyy = dq(qp - qm, S.sy);
+      const float dxx = dqu(rp - lp, S.sx);
+      const float dyx = dqu(rq - lq, S.sx);
+      const float dxy = dqu(pp - pm, S.sy);
+      const float dyy = dqu(qp - qm, S.sy);
       const int orr = r - S.r0, occ = c - S.c0;
       const float v = isnan(zc) ? __builtin_nanf("")
                                 : curvature_from(K, p0, q0, dxx, dxy, dyy, dyx);
@@ -488,17 +495,17 @@ __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
     const float lq = lane_left(q0[3]), rq = lane_right(q0[0]);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      pp[k] = dq((k == 3 ? r1 : fi1[k + 1]) - (k == 0 ? l1 : fi1[k - 1]), S.dx);  // P(r + 1)
-      qp[k] = dq(f4[k] - fi[k], S.dy);                                            // Q(r + 1)
+      pp[k] = dqu((k == 3 ? r1 : fi1[k + 1]) - (k == 0 ? l1 : fi1[k - 1]), S.dx);  // P(r + 1)
+      qp[k] = dqu(f4[k] - fi[k], S.dy);                                            // Q(r + 1)
     }
     if (r >= ra && any) {
       float v[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const float dxx = dq((k == 3 ? rp : p0[k + 1]) - (k == 0 ? lp : p0[k - 1]), S.sx);
-        const float dyx = dq((k == 3 ? rq : q0[k + 1]) - (k == 0 ? lq : q0[k - 1]), S.sx);
-        const float dxy = dq(pp[k] - pm[k], S.sy);
-        const float dyy = dq(qp[k] - qm[k], S.sy);
+        const float dxx = dqu((k == 3 ? rp : p0[k + 1]) - (k == 0 ? lp : p0[k - 1]), S.sx);
+        const float dyx = dqu((k == 3 ? rq : q0[k + 1]) - (k == 0 ? lq : q0[k - 1]), S.sx);
+        const float dxy = dqu(pp[k] - pm[k], S.sy);
+        const float dyy = dqu(qp[k] - qm[k], S.sy);
         v[k] = isnan(zc[k]) ? __builtin_nanf("")
                             : curvature_from(K, p0[k], q0[k], dxx, dxy, dyy, dyx);
         v[k] = apply_post(pcv, v[k]);

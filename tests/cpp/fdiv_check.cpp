@@ -38,10 +38,17 @@ static void check(float a, float b) {
   const float got = fsg::div_rn(a, b, y);
   n_cases++;
   const bool same = bits(got) == bits(want) || (std::isnan(got) && std::isnan(want));
-  if (!same) {
+  // the unguarded form inside its range: the same value (zeros of either sign)
+  const float m = std::fabs(a);
+  bool same_u = true;
+  if (m == 0.0f || (m >= 0x1p-75f && m <= 0x1p75f)) {
+    const float gu = fsg::div_rn_unguarded(a, b, y);
+    same_u = m == 0.0f ? gu == 0.0f : bits(gu) == bits(want);
+  }
+  if (!same || !same_u) {
     if (n_bad < 10)
-      std::printf("mismatch a=%a b=%a got=%a want=%a\n", (double)a, (double)b, (double)got,
-                  (double)want);
+      std::printf("mismatch a=%a b=%a got=%a want=%a unguarded %s\n", (double)a, (double)b,
+                  (double)got, (double)want, same_u ? "ok" : "differs");
     n_bad++;
   }
 }

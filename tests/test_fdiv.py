@@ -6,7 +6,8 @@ bit-exact only if every such quotient equals the IEEE one.  tests/cpp/
 fdiv_check.cpp compiles the same header on the host (g++, no contraction) and
 compares bits over random and exhaustive dividends, the quotients of elevation
 differences by 2h for pixel sizes in use, and zeros, subnormals, infinities
-and NaN.  To show the harness discriminates, the uncorrected product
+and NaN; the unguarded form curvature uses must agree wherever its range
+holds (zeros up to their sign).  To show the harness discriminates, the uncorrected product
 a * RN(1/b) planted in a copy of the header must fail it.
 """
 import os
