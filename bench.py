@@ -336,15 +336,24 @@ def bench_c3(dev, n=32768):
     print("#   c3 tuning tile size and launch shape", file=sys.stderr, flush=True)
     job, rep = tune_raster_job(dem, C3_PRODUCTS, tile_sizes=(2048, 4096, 8192), reps=1,
                                nodata=-9999.0, pixel_size=10.0)
-    job.capture()
-    t = _gpu_time(job.replay, 3)
+    # both execution shapes captured and replayed; the job keeps the tuner's
+    shape_graph_ms = {}
+    chosen = job.split
+    for split in (not chosen, chosen):
+        job.set_split(split)
+        job.capture()
+        shape_graph_ms["per_tile" if split else "persistent"] = \
+            round(_gpu_time(job.replay, 3) * 1e3, 2)
+    t = shape_graph_ms[rep["shape"]] / 1e3
     px = n * n
     res["graph_job"] = {"gpix_s": round(px / t / 1e9, 3), "ms": round(t * 1e3, 2),
                         "hbm_frac": round(12 * px / t / (HBM_PEAK_GBS * 1e9), 4),
                         "alg_bytes_per_px": 12, "tile_size": rep["tile_size"],
+                        "shape": rep["shape"], "graph_ms_by_shape": shape_graph_ms,
                         "launch_config": list(rep["config"]) if rep["config"] else None,
                         "tuner_tile_ms": {str(k): round(v, 2) for k, v in rep["tile_ms"].items()},
                         "tuner_launch_ms": {k: round(v, 2) for k, v in rep["launch_ms"].items()},
+                        "tuner_shape_ms": {k: round(v, 2) for k, v in rep["shape_ms"].items()},
                         "items": job.info()}
     outs = job.outputs
     job.close()
@@ -375,9 +384,10 @@ def bench_c3(dev, n=32768):
     dem = _c3_dem(dev, hr, wr)
     job = RasterJob(dem, C3_PRODUCTS, tile_size=4096, nodata=-9999.0, pixel_size=10.0,
                     config=rep["config"])
+    job.set_split(rep["shape"] == "per_tile")
     job.capture()
     t = _gpu_time(job.replay, 3)
-    res["graph_job_ragged"] = {"shape": [hr, wr], "tile_size": 4096,
+    res["graph_job_ragged"] = {"raster": [hr, wr], "tile_size": 4096, "shape": rep["shape"],
                                "gpix_s": round(hr * wr / t / 1e9, 3), "ms": round(t * 1e3, 2),
                                "tiles": job.info()["tiles"]}
     job.close()

@@ -268,7 +268,9 @@ class RasterJob:
     3. NaN fills of skipped tiles;
     4. ONE persistent ``k_batch_fused`` launch over every tile's stencil
        blocks (NaN-aware ones included) and openness blocks, each tile
-       reading one window with the widest product halo.
+       reading one window with the widest product halo -- or, with
+       ``set_split(True)`` (the tuner's choice when it measures faster), the
+       per-window kernels tile by tile, the calls ``process_dem`` makes.
 
     ``capture`` records ``run`` in a graph and ``replay`` launches it.  The
     raster is converted in place (``process_dem(copy_input=False)``'s
@@ -381,7 +383,38 @@ class RasterJob:
         self.batch = FusedBatch(jobs) if jobs else None
         if self.batch is not None and config is not None:
             self.batch.config = self.batch._cfg(config)
+        self._jobs = jobs
+        self._calls = None
+        self.split = False
         self.graph = None
+
+    def set_split(self, split: bool) -> None:
+        """The job's execution shape after the shared NoData pass and
+        pre-pass: ONE persistent launch over every tile's items (False), or
+        the per-window kernels tile by tile and product by product (True) --
+        the calls ``process_dem`` makes, with this job's windows, statistics
+        and NaN flags, so the same bits either way.  The tuner measures both."""
+        self.split = bool(split)
+        self.graph = None
+
+    def _split_calls(self):
+        if self._calls is None:
+            from .algorithms import ALGORITHMS, OutSpec
+            calls = []
+            for j in self._jobs:
+                reg = ops.region(*j.region)
+                for n in RASTER_JOB_PRODUCTS:
+                    v = getattr(j, n)
+                    if v is None:
+                        continue
+                    q = dict(j.product_params[n])
+                    q["_window_nan_free"] = bool(j.nan_free)
+                    if n == "openness":
+                        q["global_stats"] = j.openness_stats
+                    spec = OutSpec(f32=v, clip01=(n == "hillshade"))
+                    calls.append((ALGORITHMS[n], j.dem, reg, spec, q))
+            self._calls = calls
+        return self._calls
 
     def run(self) -> None:
         """Enqueue the whole job on the current stream."""
@@ -396,7 +429,10 @@ class RasterJob:
             self.prepass.run()
         for v in self._fills:
             v.fill_(float("nan"))
-        if self.batch is not None:
+        if self.split:
+            for algo, dem, reg, spec, q in self._split_calls():
+                algo.run_region(dem, reg, spec, q)
+        elif self.batch is not None:
             self.batch.run()
 
     def capture(self) -> None:
@@ -419,6 +455,7 @@ class RasterJob:
 
     def info(self) -> Dict[str, int]:
         d = self.batch.info() if self.batch is not None else {}
+        d["split"] = int(self.split)
         d["tiles"] = len(self.tiles)
         d["skipped_products"] = len(self._fills)
         d["prepass_windows"] = len(self.prepass.regions) if self.prepass is not None else 0
@@ -461,6 +498,17 @@ def tune_raster_job(raster: torch.Tensor, products: Dict[str, dict], *,
         launch_ms = {str(k): v * 1e3 for k, v in times.items()}
     else:
         best_cfg = None
+    # the execution shape: the persistent launch at its best shape against
+    # the per-window kernels tile by tile (the whole job timed both ways)
+    best_job.run()
+    t_fused = _time(best_job.run, max(1, reps))
+    best_job.set_split(True)
+    best_job.run()
+    t_split = _time(best_job.run, max(1, reps))
+    if t_split >= t_fused:
+        best_job.set_split(False)
     best_ts = min(tile_ms, key=tile_ms.get)
     return best_job, {"tile_ms": tile_ms, "launch_ms": launch_ms, "tile_size": best_ts,
-                      "config": best_cfg}
+                      "config": best_cfg,
+                      "shape_ms": {"persistent": t_fused * 1e3, "per_tile": t_split * 1e3},
+                      "shape": "per_tile" if best_job.split else "persistent"}

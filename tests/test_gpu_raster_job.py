@@ -113,3 +113,32 @@ def test_raster_job_tuner_keeps_the_bits(cuda):
     for k in prods:
         _same(job.outputs[k], ref[k], k)
     job.close()
+
+
+def test_raster_job_per_tile_shape_equals_the_persistent_launch(cuda):
+    """set_split(True): the per-window kernels tile by tile -- eager and
+    through graph replays -- give the persistent launch's bits."""
+    from fujishadergpu_amd.batch import RasterJob
+    dem = _dem(cuda, 2100, 2700, seed=8)
+    job = RasterJob(dem, PRODUCTS, tile_size=1024, nodata=-9999.0, pixel_size=10.0)
+    job.run()
+    torch.cuda.synchronize()
+    fused = {k: v.clone() for k, v in job.outputs.items()}
+    job.set_split(True)
+    assert job.info()["split"] == 1
+    for v in job.outputs.values():
+        v.fill_(-5.0)
+    job.run()
+    torch.cuda.synchronize()
+    for name in PRODUCTS:
+        _same(job.outputs[name], fused[name], f"{name} eager")
+    job.capture()
+    for rep in range(2):
+        for v in job.outputs.values():
+            v.fill_(-5.0)
+        torch.cuda.synchronize()
+        job.replay()
+        torch.cuda.synchronize()
+        for name in PRODUCTS:
+            _same(job.outputs[name], fused[name], f"{name} replay {rep}")
+    job.close()
