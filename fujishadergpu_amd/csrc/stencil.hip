@@ -9,12 +9,59 @@
 
 namespace fsg {
 
-// One workgroup per 64 x 16 block of the region (stencil.hpp); ROWS: the
-// window has per-row x steps (larger LDS for the coefficient tables).
+// The general kernels run over up to six rectangles of the region in one
+// launch (the region itself, or the window's border strips and the interior
+// columns the four-column kernel leaves): a 1-D grid, blocks numbered rect by
+// rect.
+constexpr int RS_MAX = 6;
+struct RectSet {
+  int n;
+  int r0[RS_MAX], c0[RS_MAX], h[RS_MAX], w[RS_MAX];  // window coordinates
+  int nbx[RS_MAX];
+  int pre[RS_MAX + 1];  // blocks before each rectangle
+};
+
+// The output descriptor moved to region coordinates (dr, dc).
+__host__ __device__ inline OutDev out_shift(OutDev o, int dr, int dc) {
+  if (!o.data) return o;
+  const int64_t es = o.dtype == FSG_OUT_F32 ? 4 : (o.dtype == FSG_OUT_I16 ? 2 : 1);
+  o.data = (char*)o.data + ((int64_t)dr * o.ld + dc) * es;
+  return o;
+}
+
+// This block's rectangle as a stencil tile of its own, and the block's
+// position in it; false past the last rectangle's blocks.
+__device__ __forceinline__ bool rect_block(const StencilTile& S, const RectSet& R,
+                                           StencilTile& T, int& bx, int& by) {
+  int b = blockIdx.x;
+  if (b >= R.pre[R.n]) return false;
+  int k = 0;
+  while (k + 1 < R.n && b >= R.pre[k + 1]) k++;
+  b -= R.pre[k];
+  by = b / R.nbx[k];
+  bx = b - by * R.nbx[k];
+  T = S;
+  T.r0 = R.r0[k];
+  T.c0 = R.c0[k];
+  T.rh = R.h[k];
+  T.rw = R.w[k];
+  T.nbx = R.nbx[k];
+  const int dr = R.r0[k] - S.r0, dc = R.c0[k] - S.c0;
+  T.hs = out_shift(S.hs, dr, dc);
+  T.sl = out_shift(S.sl, dr, dc);
+  T.cv = out_shift(S.cv, dr, dc);
+  return true;
+}
+
+// One workgroup per 64 x 16 block (stencil.hpp); ROWS: the window has
+// per-row x steps (larger LDS for the coefficient tables).
 template <bool NANAW, bool ROWS>
-__global__ void __launch_bounds__(256) k_stencil_tiles(StencilTile S, Gauss9 G) {
+__global__ void __launch_bounds__(256) k_stencil_tiles(StencilTile S, RectSet R, Gauss9 G) {
   __shared__ float lds[ROWS ? ST_LDS_FLOATS_ROWS : ST_LDS_FLOATS];
-  stencil_tile_block_v<NANAW, ROWS>(S, blockIdx.y, blockIdx.x, G, lds);
+  StencilTile T;
+  int bx, by;
+  if (!rect_block(S, R, T, bx, by)) return;  // the whole block
+  stencil_tile_block_v<NANAW, ROWS>(T, by, bx, G, lds);
 }
 
 // Hillshade and/or slope without curvature and with constant steps: the
@@ -28,9 +75,12 @@ __global__ void __launch_bounds__(256) k_stencil_tiles(StencilTile S, Gauss9 G) 
 constexpr int SHADE_R = 8;
 
 template <bool NANAW>
-__global__ void __launch_bounds__(256) k_shade_rows(StencilTile S, Gauss9 G) {
-  const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int rr0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * SHADE_R;
+__global__ void __launch_bounds__(256) k_shade_rows(StencilTile S0, RectSet R, Gauss9 G) {
+  StencilTile S;
+  int bx, by;
+  if (!rect_block(S0, R, S, bx, by)) return;
+  const int cc = bx * 64 + (threadIdx.x & 63);
+  const int rr0 = (by * 4 + (threadIdx.x >> 6)) * SHADE_R;
   if (rr0 >= S.rh || cc >= S.rw) return;
   const int H = S.W.h, Wd = S.W.w;
   const int wc = S.c0 + cc;
@@ -73,9 +123,12 @@ __global__ void __launch_bounds__(256) k_shade_rows(StencilTile S, Gauss9 G) {
 // Windows with NaN: one lane per pixel (the fill inlined at each neighbour
 // read takes the registers that a row walk would need).
 template <bool NANAW>
-__global__ void __launch_bounds__(256) k_shade_px(StencilTile S, Gauss9 G) {
-  const int cc = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int rr = blockIdx.y * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(256) k_shade_px(StencilTile S0, RectSet R, Gauss9 G) {
+  StencilTile S;
+  int bx, by;
+  if (!rect_block(S0, R, S, bx, by)) return;
+  const int cc = bx * 64 + (threadIdx.x & 63);
+  const int rr = by * 4 + (threadIdx.x >> 6);
   if (rr >= S.rh || cc >= S.rw) return;
   const int H = S.W.h, Wd = S.W.w;
   const int wr = S.r0 + rr, wc = S.c0 + cc;
@@ -96,35 +149,48 @@ __global__ void __launch_bounds__(256) k_shade_px(StencilTile S, Gauss9 G) {
               apply_post(phs, hillshade_from(S.shade, S.shade.ax, wr, wc, H, Wd, f, dx, dy)));
 }
 
+struct Rect {
+  int r0, c0, h, w;  // window coordinates
+};
+
+// The rectangle set of up to RS_MAX rectangles with blocks of bh rows.
+static RectSet rect_set(const Rect* q, int n, int bh) {
+  RectSet R{};
+  R.n = 0;
+  R.pre[0] = 0;
+  for (int k = 0; k < n; k++) {
+    if (q[k].h <= 0 || q[k].w <= 0) continue;
+    const int i = R.n++;
+    R.r0[i] = q[k].r0;
+    R.c0[i] = q[k].c0;
+    R.h[i] = q[k].h;
+    R.w[i] = q[k].w;
+    R.nbx[i] = (q[k].w + 63) / 64;
+    R.pre[i + 1] = R.pre[i] + R.nbx[i] * ((q[k].h + bh - 1) / bh);
+  }
+  return R;
+}
+
 static int launch_stencil(const StencilTile& S, const char* name, hipStream_t s) {
-  dim3 grd(S.nbx, (S.rh + ST_TH - 1) / ST_TH);
+  const Rect q{S.r0, S.c0, S.rh, S.rw};
+  const RectSet R = rect_set(&q, 1, ST_TH);
   TimedScope ts(name, s);
+  if (R.n == 0) return FSG_OK;
+  dim3 grd(R.pre[R.n]);
   if (S.xrows) {
     if (S.nanaw)
-      hipLaunchKernelGGL((k_stencil_tiles<true, true>), grd, dim3(256), 0, s, S, make_gauss9());
+      hipLaunchKernelGGL((k_stencil_tiles<true, true>), grd, dim3(256), 0, s, S, R, make_gauss9());
     else
-      hipLaunchKernelGGL((k_stencil_tiles<false, true>), grd, dim3(256), 0, s, S, make_gauss9());
+      hipLaunchKernelGGL((k_stencil_tiles<false, true>), grd, dim3(256), 0, s, S, R, make_gauss9());
   } else {
     if (S.nanaw)
-      hipLaunchKernelGGL((k_stencil_tiles<true, false>), grd, dim3(256), 0, s, S, make_gauss9());
+      hipLaunchKernelGGL((k_stencil_tiles<true, false>), grd, dim3(256), 0, s, S, R, make_gauss9());
     else
-      hipLaunchKernelGGL((k_stencil_tiles<false, false>), grd, dim3(256), 0, s, S, make_gauss9());
+      hipLaunchKernelGGL((k_stencil_tiles<false, false>), grd, dim3(256), 0, s, S, R, make_gauss9());
   }
   FSG_CHECK_LAUNCH(name);
   return FSG_OK;
 }
-
-// The output descriptor moved to region coordinates (dr, dc).
-static OutDev out_shift(OutDev o, int dr, int dc) {
-  if (!o.data) return o;
-  const int64_t es = o.dtype == FSG_OUT_F32 ? 4 : (o.dtype == FSG_OUT_I16 ? 2 : 1);
-  o.data = (char*)o.data + ((int64_t)dr * o.ld + dc) * es;
-  return o;
-}
-
-struct Rect {
-  int r0, c0, h, w;  // window coordinates
-};
 
 // The interior rectangle of the region: window rows [m, H-m) and columns
 // [m, W-m), where every derivative the stencil takes is the central one.
@@ -151,25 +217,16 @@ static int border_strips(const fsg_region* reg, const Rect& in, Rect* out) {
   return n;
 }
 
-// S restricted to the strip q of the region reg.
-static StencilTile sub_tile(const StencilTile& S, const fsg_region* reg, const Rect& q) {
-  StencilTile T = S;
-  T.r0 = q.r0;
-  T.c0 = q.c0;
-  T.rh = q.h;
-  T.rw = q.w;
-  T.nbx = (q.w + ST_TW - 1) / ST_TW;
-  const int dr = q.r0 - reg->r0, dc = q.c0 - reg->c0;
-  T.hs = out_shift(S.hs, dr, dc);
-  T.sl = out_shift(S.sl, dr, dc);
-  T.cv = out_shift(S.cv, dr, dc);
-  return T;
+// Rows per wave for an interior of h rows and `strips` waves across.
+static int seg_rows(int64_t h, int64_t strips) {
+  const int64_t s = h * strips / SR_MIN_WAVES;
+  return (int)std::max<int64_t>(SR_SEG_MIN, std::min<int64_t>(SR_SEG_MAX, s));
 }
 
-// The row-walking launch over the interior rectangle: one wave per `cols`
-// columns and SR_SEG_ROWS rows, four waves per workgroup.
-static bool rows_grid(const Rect& in, int cols, dim3& grd) {
-  const int64_t nseg = ((int64_t)in.h + SR_SEG_ROWS - 1) / SR_SEG_ROWS;
+// The one-column launch over the interior rectangle: one wave per `cols`
+// columns and `seg` rows, four waves (segments) per workgroup.
+static bool rows_grid(const Rect& in, int cols, int seg, dim3& grd) {
+  const int64_t nseg = ((int64_t)in.h + seg - 1) / seg;
   const int64_t gy = (nseg + 3) / 4, gx = ((int64_t)in.w + cols - 1) / cols;
   if (gy > 65535 || gx > 0x7fffffff) return false;
   grd = dim3((unsigned)gx, (unsigned)gy);
@@ -188,24 +245,26 @@ static RowsTile rows_tile(const StencilTile& S, const fsg_region* reg, const Rec
   return R;
 }
 
-// Hillshade/slope with constant steps and NaN-free or NaN-aware window,
-// general kernels (every position, numpy's edge formulas included).
-static void launch_shade_general(const StencilTile& S, hipStream_t s) {
-  if (S.nanaw) {
-    dim3 grd((S.rw + 63) / 64, (S.rh + 3) / 4);
-    hipLaunchKernelGGL(k_shade_px<true>, grd, dim3(256), 0, s, S, make_gauss9());
+// The general kernels (every position, numpy's edge formulas included) over
+// up to RS_MAX rectangles of the region, one launch: hillshade/slope (one
+// lane per pixel with NaN, a row walk without) or curvature (tiled block).
+static void launch_general(const StencilTile& S, const Rect* q, int n, bool curv,
+                           hipStream_t s) {
+  const int bh = curv ? ST_TH : (S.nanaw ? 4 : 4 * SHADE_R);
+  const RectSet R = rect_set(q, n, bh);
+  if (R.n == 0) return;
+  const dim3 grd((unsigned)R.pre[R.n]);
+  if (curv) {
+    if (S.nanaw)
+      hipLaunchKernelGGL((k_stencil_tiles<true, false>), grd, dim3(256), 0, s, S, R, make_gauss9());
+    else
+      hipLaunchKernelGGL((k_stencil_tiles<false, false>), grd, dim3(256), 0, s, S, R,
+                         make_gauss9());
+  } else if (S.nanaw) {
+    hipLaunchKernelGGL(k_shade_px<true>, grd, dim3(256), 0, s, S, R, make_gauss9());
   } else {
-    dim3 grd((S.rw + 63) / 64, (S.rh + 4 * SHADE_R - 1) / (4 * SHADE_R));
-    hipLaunchKernelGGL(k_shade_rows<false>, grd, dim3(256), 0, s, S, make_gauss9());
+    hipLaunchKernelGGL(k_shade_rows<false>, grd, dim3(256), 0, s, S, R, make_gauss9());
   }
-}
-
-static void launch_curv_general(const StencilTile& S, hipStream_t s) {
-  dim3 grd(S.nbx, (S.rh + ST_TH - 1) / ST_TH);
-  if (S.nanaw)
-    hipLaunchKernelGGL((k_stencil_tiles<true, false>), grd, dim3(256), 0, s, S, make_gauss9());
-  else
-    hipLaunchKernelGGL((k_stencil_tiles<false, false>), grd, dim3(256), 0, s, S, make_gauss9());
 }
 
 // Element offset of a pointer within its 16-byte group, or -1 when it is
@@ -236,11 +295,13 @@ static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, i
   const int64_t n_strips = std::min(n_fit, n_need);
   oc0 = std::max(in.c0, cb0 + left);
   oc1 = (int)std::min<int64_t>(in_c1, cb0 + left + n_strips * cols);
-  const int64_t nseg = ((int64_t)in.h + SR_SEG_ROWS - 1) / SR_SEG_ROWS;
+  const int seg = seg_rows(in.h, n_strips);
+  const int64_t nseg = ((int64_t)in.h + seg - 1) / seg;
   if (n_strips <= 0 || oc0 >= oc1 || nseg > 65535) return false;
   S4.T = R;
   S4.T.ic0 = oc0;
   S4.T.ic1 = oc1;
+  S4.T.seg = seg;
   S4.cb0 = cb0;
   S4.n_strips = (int)n_strips;
   int vec = 1;
@@ -256,37 +317,23 @@ static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, i
   return true;
 }
 
-// The one-column kernel over the interior rows of R and columns [c0, c1).
-static void launch_rows1(RowsTile R, const Rect& in, int c0, int c1, bool curv, bool nanaw,
-                         hipStream_t s) {
-  if (c0 >= c1) return;
-  R.ic0 = c0;
-  R.ic1 = c1;
-  dim3 grd;
-  if (!rows_grid(Rect{in.r0, c0, in.h, c1 - c0}, curv ? SR_CURV_COLS : SR_SHADE_COLS, grd))
-    return;
-  if (curv) {
-    if (nanaw)
-      hipLaunchKernelGGL(k_curv_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
-    else
-      hipLaunchKernelGGL(k_curv_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
-  } else {
-    if (nanaw)
-      hipLaunchKernelGGL(k_shade_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
-    else
-      hipLaunchKernelGGL(k_shade_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
-  }
-}
+// Interiors smaller than this take the general kernel alone: one launch
+// instead of two, for tiles where the launch costs more than the kernel saves.
+constexpr int64_t SPLIT_MIN_PX = int64_t(1) << 18;
 
 // Constant-step hillshade/slope or curvature: the row-walking kernel over
-// the interior, the general kernel over the border strips.  Falls back to the
-// general kernel alone when a divisor is outside div_rn's range.
+// the interior, then ONE general launch over the border strips (and the
+// interior columns the four-column kernel leaves at the sides).  Small
+// interiors, and divisors outside div_rn's range, take the general kernel
+// over the whole region.
 static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_region* reg,
                          bool curv, hipStream_t s) {
   const Rect in = interior(win, reg, curv ? 2 : 1);
   RowsTile R = rows_tile(S, reg, in);
+  const int cols1 = curv ? SR_CURV_COLS : SR_SHADE_COLS;
+  R.seg = seg_rows(in.h, (in.w + cols1 - 1) / cols1);
   dim3 grd;
-  bool ok = in.h > 0 && rows_grid(in, curv ? SR_CURV_COLS : SR_SHADE_COLS, grd);
+  bool ok = (int64_t)in.h * in.w >= SPLIT_MIN_PX && rows_grid(in, cols1, R.seg, grd);
   if (curv) {
     R.curv = S.curv;
     R.dx = make_divk(S.curv.ax.two_h);
@@ -305,15 +352,16 @@ static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_
     R.o1 = S.sl;
   }
   ok = ok && fdiv_ok(R.dx.b) && fdiv_ok(R.dy.b);
-  Rect strips[4];
-  int n = 0;
-  RowsTile4 S4;
-  dim3 grd4;
-  int oc0 = 0, oc1 = 0;
-  const OutDev outs[2] = {R.o0, R.o1};
+  Rect gen[RS_MAX];
+  int ng = 0;
   if (ok) {
-    // NaN-free windows take four columns per lane where whole waves fit and
-    // one column at the sides; NaN-aware ones one column (one fill per lane)
+    // NaN-free windows take four columns per lane where whole waves fit (the
+    // interior columns left at the sides go to the general launch); NaN-aware
+    // ones one column per lane (one fill per lane)
+    RowsTile4 S4;
+    dim3 grd4;
+    int oc0 = 0, oc1 = 0;
+    const OutDev outs[2] = {R.o0, R.o1};
     const int ic1 = in.c0 + in.w;
     if (!S.nanaw && rows4_setup(R, in, outs, curv ? 1 : 2, curv ? SR4_LEFT_CURV : SR4_LEFT_SHADE,
                                 S4, grd4, oc0, oc1)) {
@@ -321,22 +369,24 @@ static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_
         hipLaunchKernelGGL(k_curv_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
       else
         hipLaunchKernelGGL(k_shade_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
-      launch_rows1(R, in, in.c0, oc0, curv, false, s);
-      launch_rows1(R, in, oc1, ic1, curv, false, s);
+      if (in.c0 < oc0) gen[ng++] = Rect{in.r0, in.c0, in.h, oc0 - in.c0};
+      if (oc1 < ic1) gen[ng++] = Rect{in.r0, oc1, in.h, ic1 - oc1};
+    } else if (curv) {
+      if (S.nanaw)
+        hipLaunchKernelGGL(k_curv_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
+      else
+        hipLaunchKernelGGL(k_curv_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
     } else {
-      launch_rows1(R, in, in.c0, ic1, curv, S.nanaw != 0, s);
+      if (S.nanaw)
+        hipLaunchKernelGGL(k_shade_inner<true>, grd, dim3(256), 0, s, R, make_gauss9());
+      else
+        hipLaunchKernelGGL(k_shade_inner<false>, grd, dim3(256), 0, s, R, make_gauss9());
     }
-    n = border_strips(reg, in, strips);
+    ng += border_strips(reg, in, gen + ng);
   } else {
-    strips[n++] = Rect{reg->r0, reg->c0, reg->h, reg->w};
+    gen[ng++] = Rect{reg->r0, reg->c0, reg->h, reg->w};
   }
-  for (int k = 0; k < n; k++) {
-    const StencilTile T = sub_tile(S, reg, strips[k]);
-    if (curv)
-      launch_curv_general(T, s);
-    else
-      launch_shade_general(T, s);
-  }
+  launch_general(S, gen, ng, curv, s);
 }
 
 static StencilTile stencil_tile(const fsg_window* win, const fsg_region* reg) {

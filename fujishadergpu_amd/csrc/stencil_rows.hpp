@@ -47,7 +47,10 @@ __device__ __forceinline__ float dqu(float a, const DivK& d) {
 
 // Columns a wave writes: 64 lanes less the neighbours the stencil reads.
 constexpr int SR_SHADE_COLS = 62, SR_CURV_COLS = 60;
-constexpr int SR_SEG_ROWS = 64;  // rows one wave walks
+// Rows one wave walks: at most 64, fewer on small windows so that the launch
+// still has SR_MIN_WAVES waves (a wave walks its rows one after another),
+// at least 8.
+constexpr int SR_SEG_MAX = 64, SR_SEG_MIN = 8, SR_MIN_WAVES = 8192;
 
 // One interior launch: the window, the region's origin (output coordinates
 // are relative to it) and the interior rectangle [ir0, ir1) x [ic0, ic1) in
@@ -56,6 +59,7 @@ struct RowsTile {
   WinDev W;
   int r0, c0;
   int ir0, ir1, ic0, ic1;
+  int seg;      // rows per wave
   DivK dx, dy;  // f32(2|step_x|), f32(2|step_y|): first derivatives
   DivK sx, sy;  // f32(2 step_x), f32(2 step_y): curvature's second derivatives
   ShadeK shade;
@@ -140,9 +144,9 @@ template <bool NANAW>
 __global__ void __launch_bounds__(256) k_shade_inner(RowsTile S, Gauss9 G) {
   const int lane = threadIdx.x & 63;
   const int c = S.ic0 - 1 + blockIdx.x * SR_SHADE_COLS + lane;
-  const int ra = S.ir0 + (blockIdx.y * 4 + (threadIdx.x >> 6)) * SR_SEG_ROWS;
+  const int ra = S.ir0 + (blockIdx.y * 4 + (threadIdx.x >> 6)) * S.seg;
   if (ra >= S.ir1) return;  // the whole wave
-  const int rb = min(ra + SR_SEG_ROWS, S.ir1);
+  const int rb = min(ra + S.seg, S.ir1);
   const int H = S.W.h;
   const int cl = min(c, S.W.w - 1);  // lanes past the window read its last column
   const bool out = lane >= 1 && lane <= SR_SHADE_COLS && c < S.ic1;
@@ -210,9 +214,9 @@ template <bool NANAW>
 __global__ void __launch_bounds__(256) k_curv_inner(RowsTile S, Gauss9 G) {
   const int lane = threadIdx.x & 63;
   const int c = S.ic0 - 2 + blockIdx.x * SR_CURV_COLS + lane;
-  const int ra = S.ir0 + (blockIdx.y * 4 + (threadIdx.x >> 6)) * SR_SEG_ROWS;
+  const int ra = S.ir0 + (blockIdx.y * 4 + (threadIdx.x >> 6)) * S.seg;
   if (ra >= S.ir1) return;  // the whole wave
-  const int rb = min(ra + SR_SEG_ROWS, S.ir1);
+  const int rb = min(ra + S.seg, S.ir1);
   const int H = S.W.h;
   const int cl = min(c, S.W.w - 1);
   const bool out = lane >= 2 && lane < 2 + SR_CURV_COLS && c < S.ic1;
@@ -364,9 +368,9 @@ __device__ __forceinline__ bool rows4_wave(const RowsTile4& S, int& c4, int& cb,
                                            int& rb) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int strip = blockIdx.x * 4 + wave;
-  ra = S.T.ir0 + blockIdx.y * SR_SEG_ROWS;
+  ra = S.T.ir0 + blockIdx.y * S.T.seg;
   if (strip >= S.n_strips || ra >= S.T.ir1) return false;
-  rb = min(ra + SR_SEG_ROWS, S.T.ir1);
+  rb = min(ra + S.T.seg, S.T.ir1);
   cb = S.cb0 + strip * sr4_cols<LEFT>();
   c4 = cb + 4 * lane;
   return true;

@@ -133,18 +133,21 @@ def test_curvature_window(cuda, ctype, nan_discs):
 
 
 @pytest.mark.parametrize("nan_discs", [0, 6])
-@pytest.mark.parametrize("off", [0, 1, 2, 3])
-def test_stencils_wide_windows(cuda, nan_discs, off):
+@pytest.mark.parametrize("off,ld", [(0, 1104), (1, 1104), (2, 1104), (3, 1104), (0, 1107)])
+def test_stencils_wide_windows(cuda, nan_discs, off, ld):
     """Windows wide enough for the four-columns-per-lane interior kernels (a
     wave spans 256 columns).  The window view starts `off` floats past a
     16-byte boundary of a row stride that is a multiple of 4, so the aligned
     wave starts, the one-column side strips, the 8-byte output stores
-    (off == 0) and the per-column stores (off != 0) all run; a region inside
-    the window (output origin away from the window's) runs them again."""
+    (off == 0) and the per-column stores (off != 0) all run; a row stride
+    that is not a multiple of 4 (ld 1107) takes the one-column kernel; a
+    region inside the window (output origin away from the window's) runs
+    them again."""
     from fujishadergpu_amd import ops
     from fujishadergpu_amd.algorithms import ALGORITHMS, CurvatureAlgorithm, \
         HillshadeSlopeFused, OutSpec
-    dem = _dem(150, 1104, 21 + off, nan_discs=nan_discs)
+    # 600 x 1104: interiors above the size that takes the row kernels
+    dem = _dem(600, ld, 21 + off, nan_discs=3 * nan_discs)
     g = dem.to(cuda)[:, off:]
     a = dem[:, off:].numpy()
     kw = dict(pixel_size=10.0)
@@ -160,8 +163,8 @@ def test_stencils_wide_windows(cuda, nan_discs, off):
     _cmp("curvature", cv, ref_cv, 4e-4)
     d = np.abs(cv - ref_cv)[~np.isnan(ref_cv)]
     assert (d <= 1e-6).mean() >= 0.9, f"only {(d <= 1e-6).mean():.3f} within 1e-6"
-    # a region of the window: rows 5..124, columns 37 .. 37+900
-    r0, c0, rh, rw = 5, 37, 120, 900
+    # a region of the window: rows 5..504, columns 37 .. 37+900
+    r0, c0, rh, rw = 5, 37, 500, 900
     reg = ops.region(r0, c0, rh, rw)
     p = {"pixel_size": 10.0, "curvature_type": "mean", "_window_nan_free": not nan_discs}
     o_hs = torch.empty((rh, rw), dtype=torch.float32, device=cuda)
