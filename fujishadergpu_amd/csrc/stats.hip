@@ -25,7 +25,7 @@ enum { PCT_ABS = 1, PCT_FINITE = 2 };
 
 struct PctState {
   unsigned long long n_valid;
-  double sum, sumsq;
+  double pad_sum, pad_sumsq;  // unused (the moments come from PctWs::part)
   int n_targets;
   int pad;
   unsigned long long rank[PCT_MAX_T];
@@ -35,8 +35,27 @@ struct PctState {
   double idx[PCT_MAX_Q];
 };
 
+constexpr int PCT_BLOCKS = 2048;  // at most this many workgroups per pass
+
+// (count, mean, sum of squared deviations) of a set of values.
+struct Moments {
+  double n, mean, m2;
+};
+
+// The moments of the union of two sets (Chan, Golub and LeVeque 1979): the
+// stable pairwise update, applied in a fixed order so that the result does
+// not depend on timing.
+__host__ __device__ inline Moments merge_moments(const Moments& a, const Moments& b) {
+  const double n = a.n + b.n;
+  if (b.n == 0.0) return a;
+  if (a.n == 0.0) return b;
+  const double d = b.mean - a.mean;
+  return Moments{n, a.mean + d * (b.n / n), a.m2 + b.m2 + d * d * (a.n * b.n / n)};
+}
+
 struct PctWs {
   PctState st;
+  double part[PCT_BLOCKS][3];  // per workgroup of the level-1 pass: n, mean, m2
   unsigned long long h1[NB1];
   unsigned long long h2[PCT_MAX_T][NB2];
   unsigned long long h3[PCT_MAX_T][NB3];
@@ -99,35 +118,49 @@ __device__ __forceinline__ void pct_visit(const float* __restrict__ x, int64_t n
   for (int64_t i = head + tid; i < n; i += S) f(x[i]);
 }
 
-// Level-1 histogram (bits 31..21), valid count and FP64 moments.
+// Level-1 histogram (bits 31..21), valid count and FP64 moments.  Each
+// thread sums its values' offsets from its own first value (shifted sums, no
+// cancellation for data far from zero) and turns them into (n, mean, m2);
+// waves and the workgroup merge those in a fixed order into part[block], and
+// k_pct_finish merges the workgroups in block order: the same bits on every
+// run, like the reference's two-pass std to a few ulps.
 __global__ void __launch_bounds__(256)
 k_pct_hist1(const float* __restrict__ x, int64_t n, int flags, PctWs* __restrict__ ws) {
   // one sub-histogram per wave: the level-1 bins (exponent and two mantissa
   // bits) are few and crowded, and waves then never contend for one address
   __shared__ unsigned int hist[4][NB1];
-  __shared__ double red[2][4];
+  __shared__ double red[3][4];
   for (int b = threadIdx.x; b < 4 * NB1; b += 256) hist[b / NB1][b % NB1] = 0;
   __syncthreads();
   unsigned int* mine = hist[threadIdx.x >> 6];
-  double s = 0.0, s2 = 0.0;
+  double shift = 0.0, c = 0.0, s1 = 0.0, s2 = 0.0;
   pct_visit(x, n, [&](float v) {
     if (!pct_valid(v, flags)) return;
     atomicAdd(&mine[pct_key(v, flags) >> 21], 1u);
-    s += (double)v;
-    s2 += (double)v * (double)v;
+    if (c == 0.0) shift = (double)v;
+    const double d = (double)v - shift;
+    c += 1.0;
+    s1 += d;
+    s2 += d * d;
   });
+  Moments m{c, c > 0.0 ? shift + s1 / c : 0.0, c > 0.0 ? fmax(s2 - s1 * (s1 / c), 0.0) : 0.0};
   for (int off = 32; off > 0; off >>= 1) {
-    s += __shfl_down(s, off, 64);
-    s2 += __shfl_down(s2, off, 64);
+    const Moments o{__shfl_down(m.n, off, 64), __shfl_down(m.mean, off, 64),
+                    __shfl_down(m.m2, off, 64)};
+    m = merge_moments(m, o);
   }
   if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = s;
-    red[1][threadIdx.x >> 6] = s2;
+    red[0][threadIdx.x >> 6] = m.n;
+    red[1][threadIdx.x >> 6] = m.mean;
+    red[2][threadIdx.x >> 6] = m.m2;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(&ws->st.sum, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
-    atomicAdd(&ws->st.sumsq, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    Moments b{red[0][0], red[1][0], red[2][0]};
+    for (int w = 1; w < 4; w++) b = merge_moments(b, Moments{red[0][w], red[1][w], red[2][w]});
+    ws->part[blockIdx.x][0] = b.n;
+    ws->part[blockIdx.x][1] = b.mean;
+    ws->part[blockIdx.x][2] = b.m2;
   }
   for (int b = threadIdx.x; b < NB1; b += 256) {
     const unsigned int t = hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b];
@@ -278,7 +311,23 @@ __global__ void __launch_bounds__(256) k_pct_select(PctWs* __restrict__ ws) {
 }
 
 __global__ void k_pct_finish(PctWs* __restrict__ ws, int nq, int flags, double* __restrict__ out,
-                             int64_t* __restrict__ n_valid, double* __restrict__ moments) {
+                             int64_t* __restrict__ n_valid, double* __restrict__ moments,
+                             int n_blocks) {
+  // the workgroups' moments: lane t merges its run of blocks in order, lane 0
+  // then the 64 runs in lane order (a fixed order: the same bits every run)
+  Moments m{0.0, 0.0, 0.0};
+  if (moments) {
+    const int per = (n_blocks + 63) / 64;
+    const int b0 = threadIdx.x * per, b1 = min(n_blocks, b0 + per);
+    for (int b = b0; b < b1; b++)
+      m = merge_moments(m, Moments{ws->part[b][0], ws->part[b][1], ws->part[b][2]});
+    Moments all = m;
+    for (int t = 1; t < 64; t++) {
+      const Moments o{__shfl(m.n, t, 64), __shfl(m.mean, t, 64), __shfl(m.m2, t, 64)};
+      all = merge_moments(all, o);
+    }
+    m = all;
+  }
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const unsigned long long n = ws->st.n_valid;
   for (int j = 0; j < nq; j++) {
@@ -298,10 +347,8 @@ __global__ void k_pct_finish(PctWs* __restrict__ ws, int nq, int flags, double* 
     if (n == 0) {
       moments[0] = moments[1] = __builtin_nan("");
     } else {
-      const double mean = ws->st.sum / (double)n;
-      const double var = fmax(ws->st.sumsq / (double)n - mean * mean, 0.0);
-      moments[0] = mean;
-      moments[1] = sqrt(var);
+      moments[0] = m.mean;
+      moments[1] = sqrt(m.m2 / m.n);
     }
   }
 }
@@ -360,7 +407,7 @@ extern "C" int fsg_percentile(const float* values, int64_t n, int32_t flags, con
   const size_t words = sizeof(PctWs) / sizeof(unsigned long long);
   hipLaunchKernelGGL(k_pct_zero, dim3(256), dim3(256), 0, s, (unsigned long long*)ws, words);
   FSG_CHECK_LAUNCH("k_pct_zero");
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256));
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(PCT_BLOCKS, (n + 255) / 256));
   hipLaunchKernelGGL(k_pct_hist1, dim3(blocks), dim3(256), 0, s, values, n, flags, ws);
   FSG_CHECK_LAUNCH("k_pct_hist1");
   hipLaunchKernelGGL(k_pct_select1, dim3(1), dim3(256), 0, s, ws, q, nq);
@@ -373,7 +420,8 @@ extern "C" int fsg_percentile(const float* values, int64_t n, int32_t flags, con
   FSG_CHECK_LAUNCH("k_pct_hist3");
   hipLaunchKernelGGL(k_pct_select<3>, dim3(1), dim3(256), 0, s, ws);
   FSG_CHECK_LAUNCH("k_pct_select3");
-  hipLaunchKernelGGL(k_pct_finish, dim3(1), dim3(64), 0, s, ws, nq, flags, out, n_valid, moments);
+  hipLaunchKernelGGL(k_pct_finish, dim3(1), dim3(64), 0, s, ws, nq, flags, out, n_valid, moments,
+                     blocks);
   FSG_CHECK_LAUNCH("k_pct_finish");
   return FSG_OK;
 }

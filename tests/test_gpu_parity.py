@@ -500,6 +500,33 @@ def test_percentile_ties_and_reference_vectors(cuda):
     assert pct.item() == 3.25
 
 
+def test_percentile_moments_stable_and_deterministic(cuda):
+    """The moments behind the std fallback: data far from zero (where a
+    one-pass sum of squares cancels) against numpy's two-pass float64 mean and
+    std, the same bits on a second run, and the fallback itself: p99 of |v| is
+    0 when 99.5 % of the pool is 0, so the scale is the pool's std."""
+    from fujishadergpu_amd import ops
+    rng = np.random.default_rng(5)
+    v = (1.0e4 + rng.standard_normal(3_000_001) * 0.01).astype(np.float32)
+    t = torch.from_numpy(v).to(cuda)
+    _, _, m1 = ops.percentile(t, [99.0], ops.PCT_ABS)
+    _, _, m2 = ops.percentile(t, [99.0], ops.PCT_ABS)
+    a, b = m1.cpu().numpy(), m2.cpu().numpy()
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (a, b)
+    d = v.astype(np.float64)
+    assert abs(a[0] - d.mean()) <= 1e-12 * d.mean()
+    assert abs(a[1] - d.std()) <= 1e-9 * d.std(), (a[1], d.std())
+    # the fallback: p99 = 0, scale = std
+    w = np.zeros(200_000, np.float32)
+    w[rng.choice(w.size, size=1000, replace=False)] = 1.0
+    tw = torch.from_numpy(w).to(cuda)
+    pct, nval, mom = ops.percentile(tw, [99.0], ops.PCT_ABS)
+    assert pct.item() == 0.0
+    stats = ops.finalize_stats(0, pct, mom, nval)
+    exp = float(w.astype(np.float64).std())
+    assert abs(stats[0].item() - exp) <= 1e-12 * exp, (stats[0].item(), exp)
+
+
 def test_encode_quantize_bit_exact(cuda):
     from fujishadergpu_amd import ops
     from fujishadergpu_amd.encoding import quantize_params
