@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 18  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 19  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):
@@ -202,6 +202,7 @@ _SIGS = {
     "fsg_topousm_noalloc": (C.c_int, [P(fsg_window), P(fsg_region), P(fsg_topousm_params),
                                       P(fsg_output), P(fsg_output), vp]),
     "fsg_topousm_set_variant": (C.c_int, [C.c_int32]),
+    "fsg_stencil_set_split_min": (C.c_int64, [C.c_int64]),
     "fsg_block_mean": (C.c_int, [P(fsg_window), C.c_int32, vp, C.c_int64, vp]),
     "fsg_overview_average": (C.c_int, [P(fsg_window), C.c_int32, C.c_int32, vp, C.c_int64, vp]),
     "fsg_topousm_coarse_means_workspace_size": (C.c_size_t, [P(fsg_window), C.c_int32]),

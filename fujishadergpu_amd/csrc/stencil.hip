@@ -2,6 +2,7 @@
 // launch, one workgroup per 64 x 16 block; the block function and the
 // arithmetic live in stencil.hpp (shared with the batched launch).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "stencil.hpp"
@@ -318,8 +319,10 @@ static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, i
 }
 
 // Interiors smaller than this take the general kernel alone: one launch
-// instead of two, for tiles where the launch costs more than the kernel saves.
-constexpr int64_t SPLIT_MIN_PX = int64_t(1) << 18;
+// instead of two, and no row-walking launch with too few rows per wave to
+// hide its loads (2048^2 windows ran faster as one general launch).
+constexpr int64_t SPLIT_MIN_DEFAULT = int64_t(1) << 23;
+static std::atomic<int64_t> g_split_min{SPLIT_MIN_DEFAULT};
 
 // Constant-step hillshade/slope or curvature: the row-walking kernel over
 // the interior, then ONE general launch over the border strips (and the
@@ -333,7 +336,8 @@ static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_
   const int cols1 = curv ? SR_CURV_COLS : SR_SHADE_COLS;
   R.seg = seg_rows(in.h, (in.w + cols1 - 1) / cols1);
   dim3 grd;
-  bool ok = (int64_t)in.h * in.w >= SPLIT_MIN_PX && rows_grid(in, cols1, R.seg, grd);
+  bool ok = (int64_t)in.h * in.w >= g_split_min.load(std::memory_order_relaxed) &&
+            in.h > 0 && rows_grid(in, cols1, R.seg, grd);
   if (curv) {
     R.curv = S.curv;
     R.dx = make_divk(S.curv.ax.two_h);
@@ -473,4 +477,8 @@ extern "C" int fsg_curvature(const fsg_window* win, const fsg_region* reg,
     return FSG_OK;
   }
   return launch_stencil(S, "k_curvature", as_stream(stream));
+}
+
+extern "C" int64_t fsg_stencil_set_split_min(int64_t px) {
+  return g_split_min.exchange(px < 0 ? SPLIT_MIN_DEFAULT : px, std::memory_order_relaxed);
 }

@@ -360,6 +360,15 @@ def set_topousm_variant(variant: int) -> None:
     N.check(_lib().fsg_topousm_set_variant(int(variant)))
 
 
+def set_stencil_split_min(px: int) -> int:
+    """Smallest interior (pixels) that hillshade/slope and curvature give to
+    the row-walking interior kernels (the rest of the region, and smaller
+    interiors, take the general kernel); px < 0 restores the default.  A
+    process-wide setting for equality tests and tuning (identical results);
+    returns the previous value."""
+    return int(_lib().fsg_stencil_set_split_min(int(px)))
+
+
 def topousm_noalloc(win_t, reg, params: N.fsg_topousm_params, out: N.fsg_output,
                     out2: Optional[N.fsg_output] = None) -> None:
     """topousm without a workspace: the library's fixed per-GPU scratch array

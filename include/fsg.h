@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 18
+#define FSG_ABI_VERSION 19
 
 enum {
   FSG_OK = 0,
@@ -179,6 +179,17 @@ typedef struct fsg_curvature_params {
 int fsg_curvature(const fsg_window* win, const fsg_region* reg,
                   const fsg_curvature_params* p, const fsg_output* out,
                   void* stream);
+
+/*
+ * Smallest interior (in pixels: window rows and columns where every
+ * derivative is the central one, within the region) that fsg_hillshade_slope
+ * and fsg_curvature give to the row-walking interior kernels, with one more
+ * launch for the border strips; smaller ones take the general kernel alone,
+ * one launch.  A process-wide setting with no reference counterpart, for
+ * equality tests and tuning: results are identical either way.  px < 0
+ * restores the default (2^23); returns the previous value.
+ */
+int64_t fsg_stencil_set_split_min(int64_t px);
 
 /* ------------------------------------------------------------------ */
 /* Ray marches                                                          */

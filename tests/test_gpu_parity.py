@@ -142,11 +142,20 @@ def test_stencils_wide_windows(cuda, nan_discs, off, ld):
     (off == 0) and the per-column stores (off != 0) all run; a row stride
     that is not a multiple of 4 (ld 1107) takes the one-column kernel; a
     region inside the window (output origin away from the window's) runs
-    them again."""
+    them again.  The split size is lowered so that these interiors take the
+    row kernels (by default they are small enough for the general kernel)."""
+    from fujishadergpu_amd import ops
+    prev = ops.set_stencil_split_min(0)
+    try:
+        _stencils_wide_windows(cuda, nan_discs, off, ld)
+    finally:
+        ops.set_stencil_split_min(prev)
+
+
+def _stencils_wide_windows(cuda, nan_discs, off, ld):
     from fujishadergpu_amd import ops
     from fujishadergpu_amd.algorithms import ALGORITHMS, CurvatureAlgorithm, \
         HillshadeSlopeFused, OutSpec
-    # 600 x 1104: interiors above the size that takes the row kernels
     dem = _dem(600, ld, 21 + off, nan_discs=3 * nan_discs)
     g = dem.to(cuda)[:, off:]
     a = dem[:, off:].numpy()
