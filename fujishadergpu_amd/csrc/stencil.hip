@@ -359,20 +359,26 @@ static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_
   Rect gen[RS_MAX];
   int ng = 0;
   if (ok) {
-    // NaN-free windows take four columns per lane where whole waves fit (the
-    // interior columns left at the sides go to the general launch); NaN-aware
-    // ones one column per lane (one fill per lane)
+    // four columns per lane where whole waves fit (the interior columns left
+    // at the sides go to the general launch), else one column per lane
     RowsTile4 S4;
     dim3 grd4;
     int oc0 = 0, oc1 = 0;
     const OutDev outs[2] = {R.o0, R.o1};
     const int ic1 = in.c0 + in.w;
-    if (!S.nanaw && rows4_setup(R, in, outs, curv ? 1 : 2, curv ? SR4_LEFT_CURV : SR4_LEFT_SHADE,
-                                S4, grd4, oc0, oc1)) {
-      if (curv)
-        hipLaunchKernelGGL(k_curv_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
-      else
-        hipLaunchKernelGGL(k_shade_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
+    if (rows4_setup(R, in, outs, curv ? 1 : 2, curv ? SR4_LEFT_CURV : SR4_LEFT_SHADE, S4, grd4,
+                    oc0, oc1)) {
+      if (curv) {
+        if (S.nanaw)
+          hipLaunchKernelGGL(k_curv_inner4<true>, grd4, dim3(256), 0, s, S4, make_gauss9());
+        else
+          hipLaunchKernelGGL(k_curv_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
+      } else {
+        if (S.nanaw)
+          hipLaunchKernelGGL(k_shade_inner4<true>, grd4, dim3(256), 0, s, S4, make_gauss9());
+        else
+          hipLaunchKernelGGL(k_shade_inner4<false>, grd4, dim3(256), 0, s, S4, make_gauss9());
+      }
       if (in.c0 < oc0) gen[ng++] = Rect{in.r0, in.c0, in.h, oc0 - in.c0};
       if (oc1 < ic1) gen[ng++] = Rect{in.r0, oc1, in.h, ic1 - oc1};
     } else if (curv) {

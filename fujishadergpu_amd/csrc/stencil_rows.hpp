@@ -316,25 +316,57 @@ __device__ __forceinline__ void load_group(const float* rowp, int c4, float z[4]
   z[3] = t.w;
 }
 
-// The group's values with the NaN fill applied; the (rare) NaN cells go
-// through one call site.
-template <bool NANAW>
-__device__ __forceinline__ void fill_group(const WinDev& W, int r, int c4, const float z[4],
-                                           const Gauss9& G, float f[4]) {
-#pragma unroll
-  for (int k = 0; k < 4; k++) f[k] = z[k];
-  if (!NANAW) return;
-  int mask = (isnan(z[0]) ? 1 : 0) | (isnan(z[1]) ? 2 : 0) | (isnan(z[2]) ? 4 : 0) |
-             (isnan(z[3]) ? 8 : 0);
-  while (mask) {
-    const int k = __builtin_ctz(mask);
-    mask &= mask - 1;
-    const float v = nan_fill(W, r, min(max(c4 + k, 0), W.w - 1), G);
-    f[0] = k == 0 ? v : f[0];
-    f[1] = k == 1 ? v : f[1];
-    f[2] = k == 2 ? v : f[2];
-    f[3] = k == 3 ? v : f[3];
+// Column sums the four-column wave fill keeps per wave: 256 columns and 4
+// either side.
+constexpr int SR4_FILL_COLS = 264;
+
+// wave_fill for four columns per lane: the lane's four column sums (nine
+// loads each) and lanes 0..7 the four columns either side of the wave go to
+// LDS, then each wanted cell reads its nine in nan_fill's order, so the value
+// is nan_fill's bits.  z holds the raw row, want which cells need the fill;
+// the group's values come back in f (other NaN cells as 0: no output reads
+// them).  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_fill4(const WinDev& W, int x, int c4, int cb,
+                                           const float z[4], const bool want[4],
+                                           const Gauss9& G, float2* cols, float f[4]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll 1
+  for (int k = 0; k < 4; k++) {
+    float cv, cw;
+    nan_fill_col<false>(W, x, clampi(c4 + k, W.w), G, cv, cw);
+    cols[4 + 4 * lane + k] = make_float2(cv, cw);
   }
+  if (lane < 8) {
+    const int col = lane < 4 ? cb - 4 + lane : cb + 256 + (lane - 4);
+    float ev, ew;
+    nan_fill_col<false>(W, x, clampi(col, W.w), G, ev, ew);
+    cols[lane < 4 ? lane : 256 + lane] = make_float2(ev, ew);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    float v = isnan(z[k]) ? 0.0f : z[k];
+    if (want[k]) {
+      const int e = 4 + 4 * lane + k;
+      const float2 ce = cols[e];
+      double sv = (double)ce.x * G.g[0];
+      double sw = (double)ce.y * G.g[0];
+#pragma unroll
+      for (int d = 4; d >= 1; d--) {
+        const float2 a = cols[e - d], b = cols[e + d];
+        sv += ((double)a.x + (double)b.x) * G.g[d];
+        sw += ((double)a.y + (double)b.y) * G.g[d];
+      }
+      const float fsv = (float)sv, fsw = (float)sw;
+      v = fsw > 0.0f ? fsv / fsw : 0.0f;
+    }
+    f[k] = v;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Writes v[0..3] at output row orr, columns occ .. occ+3 where ok[k] is set.
@@ -395,21 +427,42 @@ __global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
   const float* rowp = S.W.p + (int64_t)(ra - 1) * ld;  // row ra - 1 (ra >= 1)
   const PostK phs = load_post(S.o0), psl = load_post(S.o1);
   const ShadeK& K = S.shade;
+  constexpr int AHEAD = SR4_AHEAD;
   // iteration r holds zd = row r + 1 and zq[a] = row r + 2 + a
-  float zd[4], zq[SR4_AHEAD][4];
+  float zd[4], zq[AHEAD][4];
   load_group(rowp, c4, zd);
 #pragma unroll
-  for (int a = 0; a < SR4_AHEAD; a++) {
+  for (int a = 0; a < AHEAD; a++) {
     if (ra + a < H)
       load_group(rowp + (1 + a) * ld, c4, zq[a]);
     else
       zq[a][0] = zq[a][1] = zq[a][2] = zq[a][3] = 0.0f;
   }
-  rowp += (1 + SR4_AHEAD) * ld;  // row ra + SR4_AHEAD
-  float fu[4] = {0.f, 0.f, 0.f, 0.f}, fm[4] = {0.f, 0.f, 0.f, 0.f}, zm[4];
+  rowp += (1 + AHEAD) * ld;  // row ra + AHEAD
+  float fu[4] = {0.f, 0.f, 0.f, 0.f}, fm[4] = {0.f, 0.f, 0.f, 0.f};
+  float zm[4] = {0.f, 0.f, 0.f, 0.f};
+  __shared__ float2 fill_cols[4][SR4_FILL_COLS];
   for (int r = ra - 2; r < rb; r++) {
-    float fd[4];
-    fill_group<NANAW>(S.W, r + 1, c4, zd, G, fd);
+    float fd[4] = {zd[0], zd[1], zd[2], zd[3]};
+    if (NANAW) {
+      // a NaN cell of row r + 1 is filled when an output reading it is valid:
+      // (r, c), (r + 2, c), (r + 1, c -+ 1); every lane takes part in the shifts
+      const float zl = lane_left(zd[3]), zr = lane_right(zd[0]);
+      bool want[4];
+      bool any_want = false;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float left = k == 0 ? zl : zd[k - 1], right = k == 3 ? zr : zd[k + 1];
+        want[k] = isnan(zd[k]) & (!isnan(zm[k]) | !isnan(zq[0][k]) | !isnan(left) |
+                                  !isnan(right));
+        any_want |= want[k];
+      }
+      if (__ballot(any_want))  // the whole wave
+        wave_fill4(S.W, r + 1, c4, cb, zd, want, G, fill_cols[threadIdx.x >> 6], fd);
+      else
+#pragma unroll
+        for (int k = 0; k < 4; k++) fd[k] = isnan(zd[k]) ? 0.0f : zd[k];
+    }
     const float fl = lane_left(fm[3]), fr = lane_right(fm[0]);
     if (r >= ra && any) {
       float hs[4], sl[4];
@@ -446,11 +499,11 @@ __global__ void __launch_bounds__(256) k_shade_inner4(RowsTile4 S4, Gauss9 G) {
       zd[k] = zq[0][k];
     }
 #pragma unroll
-    for (int a = 0; a + 1 < SR4_AHEAD; a++)
+    for (int a = 0; a + 1 < AHEAD; a++)
 #pragma unroll
       for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
-    if (r + 2 + SR4_AHEAD < H)  // row r + 2 + SR4_AHEAD
-      load_group(rowp, c4, zq[SR4_AHEAD - 1]);
+    if (r + 2 + AHEAD < H)  // row r + 2 + AHEAD
+      load_group(rowp, c4, zq[AHEAD - 1]);
     rowp += ld;
   }
 }
@@ -474,26 +527,54 @@ __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
   const float* rowp = S.W.p + (int64_t)(ra - 2) * ld;  // row ra - 2 (ra >= 2)
   const PostK pcv = load_post(S.o0);
   const CurvK& K = S.curv;
+  // the NaN-aware variant looks two rows ahead: the fill test reads rows
+  // r .. r + 4
+  constexpr int AHEAD = NANAW ? 2 : SR4_AHEAD;
   // iteration r holds zn = row r + 2 and zq[a] = row r + 3 + a
-  float zn[4], zq[SR4_AHEAD][4];
+  float zn[4], zq[AHEAD][4];
   load_group(rowp, c4, zn);
 #pragma unroll
-  for (int a = 0; a < SR4_AHEAD; a++) {
+  for (int a = 0; a < AHEAD; a++) {
     if (ra - 1 + a < H)
       load_group(rowp + (1 + a) * ld, c4, zq[a]);
     else
       zq[a][0] = zq[a][1] = zq[a][2] = zq[a][3] = 0.0f;
   }
-  rowp += (1 + SR4_AHEAD) * ld;  // row ra - 1 + SR4_AHEAD
+  rowp += (1 + AHEAD) * ld;  // row ra - 1 + AHEAD
   // after iteration r: fi, fi1 = rows r + 1, r + 2 (filled); pm, p0 = P(r),
   // P(r + 1); qm, q0 = Q(r), Q(r + 1); zc, zc1 = raw rows r + 1, r + 2
   float fi[4] = {0.f, 0.f, 0.f, 0.f}, fi1[4] = {0.f, 0.f, 0.f, 0.f};
   float pm[4] = {0.f, 0.f, 0.f, 0.f}, p0[4] = {0.f, 0.f, 0.f, 0.f};
   float qm[4] = {0.f, 0.f, 0.f, 0.f}, q0[4] = {0.f, 0.f, 0.f, 0.f};
-  float zc[4], zc1[4];
+  float zc[4] = {0.f, 0.f, 0.f, 0.f}, zc1[4] = {0.f, 0.f, 0.f, 0.f};
+  __shared__ float2 fill_cols[4][SR4_FILL_COLS];
   for (int r = ra - 4; r < rb; r++) {
-    float f4[4], pp[4], qp[4];
-    fill_group<NANAW>(S.W, r + 2, c4, zn, G, f4);
+    float f4[4] = {zn[0], zn[1], zn[2], zn[3]}, pp[4], qp[4];
+    if (NANAW) {
+      // a NaN cell of row r + 2 is filled when a valid cell lies within two
+      // rows and two columns of it (the reach of the derivatives reading it)
+      int colv = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        colv |= (!isnan(zc[k]) | !isnan(zc1[k]) | !isnan(zn[k]) | !isnan(zq[0][k]) |
+                 !isnan(zq[1][k])) << k;
+      // the neighbour lanes' masks; outside the wave counts as valid
+      const int ml = __builtin_amdgcn_update_dpp(0xf, colv, 0x138, 0xf, 0xf, false);
+      const int mr = __builtin_amdgcn_update_dpp(0xf, colv, 0x130, 0xf, 0xf, false);
+      const int w12 = ml | (colv << 4) | (mr << 8);  // columns c4 - 4 .. c4 + 7
+      bool want[4];
+      bool any_want = false;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        want[k] = isnan(zn[k]) & (((w12 >> (k + 2)) & 0x1f) != 0);
+        any_want |= want[k];
+      }
+      if (__ballot(any_want))  // the whole wave
+        wave_fill4(S.W, r + 2, c4, cb, zn, want, G, fill_cols[threadIdx.x >> 6], f4);
+      else
+#pragma unroll
+        for (int k = 0; k < 4; k++) f4[k] = isnan(zn[k]) ? 0.0f : zn[k];
+    }
     const float l1 = lane_left(fi1[3]), r1 = lane_right(fi1[0]);
     const float lp = lane_left(p0[3]), rp = lane_right(p0[0]);
     const float lq = lane_left(q0[3]), rq = lane_right(q0[0]);
@@ -529,11 +610,11 @@ __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
       zn[k] = zq[0][k];
     }
 #pragma unroll
-    for (int a = 0; a + 1 < SR4_AHEAD; a++)
+    for (int a = 0; a + 1 < AHEAD; a++)
 #pragma unroll
       for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
-    if (r + 3 + SR4_AHEAD < H)  // row r + 3 + SR4_AHEAD
-      load_group(rowp, c4, zq[SR4_AHEAD - 1]);
+    if (r + 3 + AHEAD < H)  // row r + 3 + AHEAD
+      load_group(rowp, c4, zq[AHEAD - 1]);
     rowp += ld;
   }
 }
