@@ -291,11 +291,10 @@ static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, i
   const int cb_max = R.W.w - 256;                // a wave's last column <= W - 1
   const int in_c1 = in.c0 + in.w;
   if (cb0 > cb_max) return false;
-  const int64_t n_fit = (cb_max - cb0) / cols + 1;
-  const int64_t n_need = ((int64_t)in_c1 - (cb0 + left) + cols - 1) / cols;
-  const int64_t n_strips = std::min(n_fit, n_need);
+  const int cb_last = cb_max - mod4(a + cb_max);  // aligned like cb0, <= cb_max
+  const int64_t n_strips = ((int64_t)in_c1 - (cb0 + left) + cols - 1) / cols;
   oc0 = std::max(in.c0, cb0 + left);
-  oc1 = (int)std::min<int64_t>(in_c1, cb0 + left + n_strips * cols);
+  oc1 = std::min(in_c1, cb_last + left + cols);  // the last wave's last output + 1
   const int seg = seg_rows(in.h, n_strips);
   const int64_t nseg = ((int64_t)in.h + seg - 1) / seg;
   if (n_strips <= 0 || oc0 >= oc1 || nseg > 65535) return false;
@@ -304,6 +303,7 @@ static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, i
   S4.T.ic1 = oc1;
   S4.T.seg = seg;
   S4.cb0 = cb0;
+  S4.cb_last = cb_last;
   S4.n_strips = (int)n_strips;
   int vec = 1;
   for (int k = 0; k < n_out; k++) {

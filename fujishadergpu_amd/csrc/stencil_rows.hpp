@@ -290,8 +290,9 @@ __global__ void __launch_bounds__(256) k_curv_inner(RowsTile S, Gauss9 G) {
 // and last lanes half a group (8-byte stores).  Measured: hillshade/slope
 // (two outputs) run faster with LEFT = 4, curvature with LEFT = 2.
 // Needs a window row stride that is a multiple of 4 floats; waves go only
-// where their 256 columns lie inside the window, and the interior columns
-// left at its sides take the one-column kernel.
+// where their 256 columns lie inside the window (the last one moved left to
+// the last aligned start that fits, overlapping its neighbour), and the few
+// interior columns left at the sides take the general kernel.
 template <int LEFT>
 constexpr int sr4_cols() {
   return 256 - 2 * LEFT;
@@ -302,6 +303,8 @@ constexpr int SR4_AHEAD = 1;  // rows in flight ahead of the one used (more cost
 struct RowsTile4 {
   RowsTile T;
   int cb0;      // first column of wave 0 (16-byte aligned address in every row)
+  int cb_last;  // the last aligned start whose 256 columns fit the window:
+                // later waves start there (overlapping writes of equal values)
   int n_strips; // waves across the interior
   int vec_out;  // o0 (and o1) take aligned float4 stores at the group starts
 };
@@ -403,7 +406,7 @@ __device__ __forceinline__ bool rows4_wave(const RowsTile4& S, int& c4, int& cb,
   ra = S.T.ir0 + blockIdx.y * S.T.seg;
   if (strip >= S.n_strips || ra >= S.T.ir1) return false;
   rb = min(ra + S.T.seg, S.T.ir1);
-  cb = S.cb0 + strip * sr4_cols<LEFT>();
+  cb = min(S.cb0 + strip * sr4_cols<LEFT>(), S.cb_last);
   c4 = cb + 4 * lane;
   return true;
 }
