@@ -26,6 +26,7 @@
 
 #include "rays.hpp"
 #include "stencil.hpp"
+#include "stencil_plan.hpp"
 #include "usm.hpp"
 
 namespace fsg {
@@ -49,6 +50,9 @@ struct BatchDev {
   const StencilTile* st;
   const int* st_pre;   // prefix of stencil blocks per tile (n + 1)
   int n_st_tiles, n_st_items;
+  const RowsTile4* rt; // NaN-free tile interiors as row-walking items
+  const int* rt_pre;   // prefix of their items (n + 1)
+  int n_rt_tiles, n_rt_items;
   const OpenTileD* op;
   const int* op_pre;   // prefix of openness items per tile (n + 1)
   int n_op_tiles, n_op_items;
@@ -97,18 +101,21 @@ template <int NR, int KINDS>
 __device__ __forceinline__ void batch_drain(const BatchDev& B) {
   extern __shared__ float lds[];
   __shared__ int s_item;
-  const int total = B.n_usm_items + B.n_st_items + B.n_op_items;
+  const int n_stencil = B.n_st_items + B.n_rt_items;  // tile blocks, then row items
+  const int total = B.n_usm_items + n_stencil + B.n_op_items;
   // the prefix arrays staged in LDS once: tile lookups stay on chip
   int* upre = (int*)((char*)lds + B.pre_off);
   int* spre = upre + B.n_usm_tiles + 1;
   int* opre = spre + B.n_st_tiles + 1;
+  int* rpre = opre + B.n_op_tiles + 1;
   for (int e = threadIdx.x; e <= B.n_usm_tiles; e += 256) upre[e] = B.usm_pre[e];
   for (int e = threadIdx.x; e <= B.n_st_tiles; e += 256) spre[e] = B.st_pre[e];
   if (KINDS & BK_OPEN)
     for (int e = threadIdx.x; e <= B.n_op_tiles; e += 256) opre[e] = B.op_pre[e];
+  for (int e = threadIdx.x; e <= B.n_rt_tiles; e += 256) rpre[e] = B.rt_pre[e];
   // item groups in queue order: kinds 0 TopoUSM, 1 openness, 2 stencil
   const int first = B.order ? 2 : 0, last = B.order ? 0 : 2;
-  const int n_first = B.order ? B.n_st_items : B.n_usm_items;
+  const int n_first = B.order ? n_stencil : B.n_usm_items;
   // every workgroup leaves once the queue passes the item count
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(B.queue, B.chunk);
@@ -137,6 +144,19 @@ __device__ __forceinline__ void batch_drain(const BatchDev& B) {
         const int blk = it - opre[tl];
         const int by = blk / O.nbx;
         openness_item(O, B.T, by, blk - by * O.nbx);
+      } else if (!(KINDS & BK_OPEN) && kind == 2 && it >= B.n_st_items) {
+        // a row item: four adjacent strips of one row segment, a strip per wave
+        const int ir = it - B.n_st_items;
+        const int tl = __builtin_amdgcn_readfirstlane(find_tile(rpre, B.n_rt_tiles, ir));
+        const RowsTile4& R = B.rt[tl];
+        const int blk = ir - rpre[tl];
+        const int ngx = (R.n_strips + 3) >> 2;
+        const int sg = blk / ngx, xg = blk - sg * ngx;
+        const int strip = xg * 4 + (threadIdx.x >> 6);
+        if (R.T.xrows)
+          stencil4_wave<true>(R, strip, sg);
+        else
+          stencil4_wave<false>(R, strip, sg);
       } else if (kind == 2) {
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(spre, B.n_st_tiles, it));
         const StencilTile& S = B.st[tl];
@@ -194,6 +214,10 @@ struct fsg_batch {
   int kinds = 0;                      // item kinds of the persistent kernel (BK_*)
   size_t lds = 0;
   int n_usm_items = 0, n_st_items = 0, n_st_tiles = 0;
+  int n_rt_items = 0, n_rt_tiles = 0;
+  int n_stencil_in = 0;               // input tiles with stencil work in the kernel
+  RowsTile4* d_rt = nullptr;
+  int* d_rt_pre = nullptr;
   int n_op_items = 0, n_op_tiles = 0, first_open = -1;
   OpenTab open_tab{};
   OpenTileD* d_op = nullptr;
@@ -315,6 +339,63 @@ static int plan_prep(fsg_batch* b) {
   return FSG_OK;
 }
 
+// Interiors smaller than this keep the tile's blocks.
+constexpr int64_t BATCH_ROWS_MIN_PX = int64_t(1) << 16;
+
+// A NaN-free stencil tile's interior as row items (appended to rt, rt_pre)
+// and the rest of its region as up to six block strips; false: the tile
+// keeps its blocks (NaN-aware windows, a hillshade z factor other than 1, a
+// hillshade/slope gradient that differs from curvature's, small or unaligned
+// interiors).
+static bool rows_item_plan(const StencilTile& S, std::vector<RowsTile4>& rt,
+                           std::vector<int>& rt_pre, Rect* strips, int& n_strips) {
+  const bool shade = S.want_hs || S.want_sl;
+  if (S.nanaw || (S.want_hs && !S.shade.z_is_one) ||
+      (shade && S.want_cv && !S.shade_from_tables))
+    return false;
+  const Rect in = interior(S.W.h, S.W.w, S.r0, S.c0, S.rh, S.rw, 2);
+  if ((int64_t)in.h * in.w < BATCH_ROWS_MIN_PX) return false;
+  RowsTile R{};
+  R.W = S.W;
+  R.r0 = S.r0;
+  R.c0 = S.c0;
+  R.ir0 = in.r0;
+  R.ir1 = in.r0 + in.h;
+  R.ic0 = in.c0;
+  R.ic1 = in.c0 + in.w;
+  if (S.want_cv) {
+    R.dx = make_divk(S.curv.ax.two_h);
+    R.dy = make_divk(S.curv.ay.two_h);
+    R.sx = make_divk(S.curv.sx.two_h);
+    R.sy = make_divk(S.curv.sy.two_h);
+  } else {
+    R.dx = make_divk(S.shade.ax.two_h);
+    R.dy = make_divk(S.shade.ay.two_h);
+  }
+  R.shade = S.shade;
+  R.curv = S.curv;
+  R.want_hs = S.want_hs;
+  R.want_sl = S.want_sl;
+  R.want_cv = S.want_cv;
+  R.o0 = S.hs;
+  R.o1 = S.sl;
+  R.o2 = S.cv;
+  R.xrows = S.xrows;
+  RowsTile4 S4;
+  dim3 grd;
+  int oc0 = 0, oc1 = 0;
+  const OutDev outs[3] = {S.hs, S.sl, S.cv};
+  if (!rows4_setup(R, in, outs, 3, SR4_LEFT_CURV, S4, grd, oc0, oc1, SR_SEG_MAX)) return false;
+  n_strips = 0;
+  const int ic1 = in.c0 + in.w;
+  if (in.c0 < oc0) strips[n_strips++] = Rect{in.r0, in.c0, in.h, oc0 - in.c0};
+  if (oc1 < ic1) strips[n_strips++] = Rect{in.r0, oc1, in.h, ic1 - oc1};
+  n_strips += border_strips(S.r0, S.c0, S.rh, S.rw, in, strips + n_strips);
+  rt.push_back(S4);
+  rt_pre.push_back(rt_pre.back() + (int)grd.x * (int)grd.y);
+  return true;
+}
+
 extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batch** out) {
   if (!out) return fail(FSG_E_INVALID, "fsg_batch_create: out NULL");
   *out = nullptr;
@@ -360,9 +441,14 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   }
   const size_t ws_total = off;
   const size_t usm_bytes = al256(sizeof(UsmArgs) * n), pre_bytes = al256(sizeof(int) * (n + 1));
-  const size_t st_bytes = al256(sizeof(StencilTile) * n);
+  // a tile's stencil work: its blocks, or row items for the interior and up
+  // to four border strips of blocks
+  const size_t st_bytes = al256(sizeof(StencilTile) * 4 * n);
+  const size_t st_pre_bytes = al256(sizeof(int) * (4 * n + 1));
+  const size_t rt_bytes = al256(sizeof(RowsTile4) * n);
   const size_t op_bytes = al256(sizeof(OpenTileD) * n);
-  b->dev_bytes = ws_total + usm_bytes + 3 * pre_bytes + st_bytes + op_bytes + 256;
+  b->dev_bytes = ws_total + usm_bytes + 3 * pre_bytes + st_bytes + st_pre_bytes + rt_bytes +
+                 op_bytes + 256;
   if (int rc = check_hip(hipMalloc((void**)&b->dev, b->dev_bytes), "fsg_batch_create")) {
     delete b;
     return rc;
@@ -375,6 +461,10 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   b->d_st = (StencilTile*)p;
   p += st_bytes;
   b->d_st_pre = (int*)p;
+  p += st_pre_bytes;
+  b->d_rt = (RowsTile4*)p;
+  p += rt_bytes;
+  b->d_rt_pre = (int*)p;
   p += pre_bytes;
   b->d_op = (OpenTileD*)p;
   p += op_bytes;
@@ -431,6 +521,8 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   // stencil descriptors
   std::vector<StencilTile> st;
   std::vector<int> st_pre{0};
+  std::vector<RowsTile4> rt;
+  std::vector<int> rt_pre{0};
   for (int i = 0; i < n; i++) {
     const fsg_batch_tile& t = b->tiles[i];
     if (!wants_stencil(t) || t.reg.h == 0 || t.reg.w == 0) continue;
@@ -463,11 +555,37 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
       b->st_fallback.push_back(S);
       continue;
     }
+    b->n_stencil_in++;
+    // NaN-free tiles with a big enough interior: row items over the interior
+    // (the per-window interior kernels' walk), blocks over the border strips
+    Rect strips[6];
+    int n_strips = 0;
+    // (the instances with openness items keep the blocks: their register
+    // budget is capped for the openness items)
+    if (!(b->kinds & BK_OPEN) && rows_item_plan(S, rt, rt_pre, strips, n_strips)) {
+      for (int k = 0; k < n_strips; k++) {
+        const Rect& q = strips[k];
+        StencilTile T = S;
+        T.r0 = q.r0;
+        T.c0 = q.c0;
+        T.rh = q.h;
+        T.rw = q.w;
+        T.nbx = (q.w + ST_TW - 1) / ST_TW;
+        T.hs = out_shift(S.hs, q.r0 - S.r0, q.c0 - S.c0);
+        T.sl = out_shift(S.sl, q.r0 - S.r0, q.c0 - S.c0);
+        T.cv = out_shift(S.cv, q.r0 - S.r0, q.c0 - S.c0);
+        st.push_back(T);
+        st_pre.push_back(st_pre.back() + T.nbx * ((q.h + ST_TH - 1) / ST_TH));
+      }
+      continue;
+    }
     st.push_back(S);
     st_pre.push_back(st_pre.back() + S.nbx * ((t.reg.h + ST_TH - 1) / ST_TH));
   }
   b->n_st_tiles = (int)st.size();
   b->n_st_items = st_pre.back();
+  b->n_rt_tiles = (int)rt.size();
+  b->n_rt_items = rt_pre.back();
   // openness descriptors
   std::vector<OpenTileD> op;
   std::vector<int> op_pre{0};
@@ -497,6 +615,12 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
                              hipMemcpyHostToDevice), "fsg_batch_create");
   if (!rc && !st.empty())
     rc = check_hip(hipMemcpy(b->d_st, st.data(), sizeof(StencilTile) * st.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc && !rt.empty())
+    rc = check_hip(hipMemcpy(b->d_rt, rt.data(), sizeof(RowsTile4) * rt.size(),
+                             hipMemcpyHostToDevice), "fsg_batch_create");
+  if (!rc)
+    rc = check_hip(hipMemcpy(b->d_rt_pre, rt_pre.data(), sizeof(int) * rt_pre.size(),
                              hipMemcpyHostToDevice), "fsg_batch_create");
   if (!rc)
     rc = check_hip(hipMemcpy(b->d_st_pre, st_pre.data(), sizeof(int) * st_pre.size(),
@@ -554,7 +678,7 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
     hipLaunchKernelGGL(k_stencil_nanaw, grd, dim3(256), 0, s, S, make_gauss9());
     FSG_CHECK_LAUNCH("k_stencil_nanaw");
   }
-  const int total = b->n_usm_items + b->n_st_items + b->n_op_items;
+  const int total = b->n_usm_items + b->n_st_items + b->n_rt_items + b->n_op_items;
   if (total == 0) return FSG_OK;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_queue);
   FSG_CHECK_LAUNCH("k_batch_reset");
@@ -577,6 +701,10 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
   D.st_pre = b->d_st_pre;
   D.n_st_tiles = b->n_st_tiles;
   D.n_st_items = b->n_st_items;
+  D.rt = b->d_rt;
+  D.rt_pre = b->d_rt_pre;
+  D.n_rt_tiles = b->n_rt_tiles;
+  D.n_rt_items = b->n_rt_items;
   D.op = b->d_op;
   D.op_pre = b->d_op_pre;
   D.n_op_tiles = b->n_op_tiles;
@@ -587,7 +715,8 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
   D.chunk = (cfg && cfg->chunk > 0) ? cfg->chunk : 8;
   D.pre_off = (int)((b->lds + 15) & ~(size_t)15);
   const size_t lds_total =
-      (size_t)D.pre_off + sizeof(int) * ((size_t)D.n_usm_tiles + D.n_st_tiles + D.n_op_tiles + 3);
+      (size_t)D.pre_off +
+      sizeof(int) * ((size_t)D.n_usm_tiles + D.n_st_tiles + D.n_op_tiles + D.n_rt_tiles + 4);
   if (lds_total > 150 * 1024)
     return fail(FSG_E_UNSUPPORTED, "fsg_batch_run: too many tiles for the LDS tile index");
   D.G = make_gauss9();
@@ -661,10 +790,10 @@ extern "C" int fsg_batch_replay(fsg_batch* b, void* stream) {
 extern "C" int fsg_batch_info(const fsg_batch* b, int64_t* info) {
   if (!b || !info) return fail(FSG_E_INVALID, "fsg_batch_info: NULL argument");
   info[0] = b->n_usm_items;
-  info[1] = b->n_st_items;
+  info[1] = b->n_st_items + b->n_rt_items;
   info[2] = (int64_t)b->usm_fused.size();
   info[3] = (int64_t)b->usm_fallback.size();
-  info[4] = b->n_st_tiles + (int64_t)b->st_fallback.size();
+  info[4] = b->n_stencil_in + (int64_t)b->st_fallback.size();
   info[5] = (int64_t)b->lds;
   info[6] = b->nr;
   info[7] = (int64_t)b->dev_bytes;

@@ -6,7 +6,7 @@
 #include <cmath>
 
 #include "stencil.hpp"
-#include "stencil_rows.hpp"
+#include "stencil_plan.hpp"
 
 namespace fsg {
 
@@ -21,14 +21,6 @@ struct RectSet {
   int nbx[RS_MAX];
   int pre[RS_MAX + 1];  // blocks before each rectangle
 };
-
-// The output descriptor moved to region coordinates (dr, dc).
-__host__ __device__ inline OutDev out_shift(OutDev o, int dr, int dc) {
-  if (!o.data) return o;
-  const int64_t es = o.dtype == FSG_OUT_F32 ? 4 : (o.dtype == FSG_OUT_I16 ? 2 : 1);
-  o.data = (char*)o.data + ((int64_t)dr * o.ld + dc) * es;
-  return o;
-}
 
 // This block's rectangle as a stencil tile of its own, and the block's
 // position in it; false past the last rectangle's blocks.
@@ -150,10 +142,6 @@ __global__ void __launch_bounds__(256) k_shade_px(StencilTile S0, RectSet R, Gau
               apply_post(phs, hillshade_from(S.shade, S.shade.ax, wr, wc, H, Wd, f, dx, dy)));
 }
 
-struct Rect {
-  int r0, c0, h, w;  // window coordinates
-};
-
 // The rectangle set of up to RS_MAX rectangles with blocks of bh rows.
 static RectSet rect_set(const Rect* q, int n, int bh) {
   RectSet R{};
@@ -191,37 +179,6 @@ static int launch_stencil(const StencilTile& S, const char* name, hipStream_t s)
   }
   FSG_CHECK_LAUNCH(name);
   return FSG_OK;
-}
-
-// The interior rectangle of the region: window rows [m, H-m) and columns
-// [m, W-m), where every derivative the stencil takes is the central one.
-static Rect interior(const fsg_window* win, const fsg_region* reg, int m) {
-  const int r0 = std::max(reg->r0, m), r1 = std::min(reg->r0 + reg->h, win->h - m);
-  const int c0 = std::max(reg->c0, m), c1 = std::min(reg->c0 + reg->w, win->w - m);
-  if (r0 >= r1 || c0 >= c1) return Rect{0, 0, 0, 0};
-  return Rect{r0, c0, r1 - r0, c1 - c0};
-}
-
-// The region less the interior rectangle, as up to four strips.
-static int border_strips(const fsg_region* reg, const Rect& in, Rect* out) {
-  const int R0 = reg->r0, R1 = reg->r0 + reg->h, C0 = reg->c0, C1 = reg->c0 + reg->w;
-  int n = 0;
-  if (in.h == 0 || in.w == 0) {
-    out[n++] = Rect{R0, C0, reg->h, reg->w};
-    return n;
-  }
-  const int ir1 = in.r0 + in.h, ic1 = in.c0 + in.w;
-  if (in.r0 > R0) out[n++] = Rect{R0, C0, in.r0 - R0, reg->w};
-  if (R1 > ir1) out[n++] = Rect{ir1, C0, R1 - ir1, reg->w};
-  if (in.c0 > C0) out[n++] = Rect{in.r0, C0, in.h, in.c0 - C0};
-  if (C1 > ic1) out[n++] = Rect{in.r0, ic1, in.h, C1 - ic1};
-  return n;
-}
-
-// Rows per wave for an interior of h rows and `strips` waves across.
-static int seg_rows(int64_t h, int64_t strips) {
-  const int64_t s = h * strips / SR_MIN_WAVES;
-  return (int)std::max<int64_t>(SR_SEG_MIN, std::min<int64_t>(SR_SEG_MAX, s));
 }
 
 // The one-column launch over the interior rectangle: one wave per `cols`
@@ -266,56 +223,6 @@ static void launch_general(const StencilTile& S, const Rect* q, int n, bool curv
   } else {
     hipLaunchKernelGGL(k_shade_rows<false>, grd, dim3(256), 0, s, S, R, make_gauss9());
   }
-}
-
-// Element offset of a pointer within its 16-byte group, or -1 when it is
-// not 4-byte aligned.
-static int quad_phase(const void* p) {
-  const uintptr_t u = (uintptr_t)p;
-  return (u & 3) ? -1 : (int)((u >> 2) & 3);
-}
-
-static int mod4(int x) { return ((x % 4) + 4) % 4; }
-
-// The four-columns-per-lane launch over the interior columns [oc0, oc1) that
-// whole aligned waves cover: needs a row stride that is a multiple of 4
-// floats and a 4-byte aligned window; each wave's 256 columns lie inside the
-// window; float4-aligned outputs take 16-byte stores.  False when no wave fits.
-static bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, int n_out,
-                        int left, RowsTile4& S4, dim3& grd, int& oc0, int& oc1) {
-  const int cols = 256 - 2 * left;
-  const int a = quad_phase(R.W.p);
-  if (a < 0 || R.W.ld % 4 != 0 || in.w == 0) return false;
-  int cb0 = (in.c0 - left) - mod4(a + in.c0 - left);  // aligned, cb0 + left <= in.c0
-  if (cb0 < 0) cb0 += 4 * ((-cb0 + 3) / 4);      // first aligned start inside the window
-  const int cb_max = R.W.w - 256;                // a wave's last column <= W - 1
-  const int in_c1 = in.c0 + in.w;
-  if (cb0 > cb_max) return false;
-  const int cb_last = cb_max - mod4(a + cb_max);  // aligned like cb0, <= cb_max
-  const int64_t n_strips = ((int64_t)in_c1 - (cb0 + left) + cols - 1) / cols;
-  oc0 = std::max(in.c0, cb0 + left);
-  oc1 = std::min(in_c1, cb_last + left + cols);  // the last wave's last output + 1
-  const int seg = seg_rows(in.h, n_strips);
-  const int64_t nseg = ((int64_t)in.h + seg - 1) / seg;
-  if (n_strips <= 0 || oc0 >= oc1 || nseg > 65535) return false;
-  S4.T = R;
-  S4.T.ic0 = oc0;
-  S4.T.ic1 = oc1;
-  S4.T.seg = seg;
-  S4.cb0 = cb0;
-  S4.cb_last = cb_last;
-  S4.n_strips = (int)n_strips;
-  int vec = 1;
-  for (int k = 0; k < n_out; k++) {
-    const OutDev& o = outs[k];
-    if (!o.data) continue;
-    const int ph = quad_phase(o.data);
-    vec = vec && o.dtype == FSG_OUT_F32 && ph >= 0 && o.ld % 4 == 0 &&
-          mod4(ph + cb0 - R.c0) == 0;
-  }
-  S4.vec_out = vec;
-  grd = dim3((unsigned)((n_strips + 3) / 4), (unsigned)nseg);
-  return true;
 }
 
 // Interiors smaller than this take the general kernel alone: one launch

@@ -32,17 +32,23 @@ __device__ __forceinline__ float lane_right(float v) {
 }
 
 // A divisor f32(2h) and its reciprocal RN(1 / f32(2h)) (host computed).
+// ok: b is in div_rn's range (else the IEEE division).
 struct DivK {
   float b, y;
+  int ok;
 };
 
-inline DivK make_divk(float two_h) { return DivK{two_h, 1.0f / two_h}; }
+__host__ __device__ inline DivK make_divk(float two_h) {
+  return DivK{two_h, 1.0f / two_h, fdiv_ok(two_h) ? 1 : 0};
+}
 
-__device__ __forceinline__ float dq(float a, const DivK& d) { return div_rn(a, d.b, d.y); }
+__device__ __forceinline__ float dq(float a, const DivK& d) {
+  return d.ok ? div_rn(a, d.b, d.y) : a / d.b;
+}
 // curvature's divisions: the unguarded form (its output is checked to a
 // tolerance, and every elevation difference of real data is in its range)
 __device__ __forceinline__ float dqu(float a, const DivK& d) {
-  return div_rn_unguarded(a, d.b, d.y);
+  return d.ok ? div_rn_unguarded(a, d.b, d.y) : a / d.b;
 }
 
 // Columns a wave writes: 64 lanes less the neighbours the stencil reads.
@@ -66,6 +72,11 @@ struct RowsTile {
   CurvK curv;
   int want_hs, want_sl;
   OutDev o0, o1;  // hillshade and slope, or curvature in o0
+  // the batch's combined walk (stencil4_wave): curvature's output, and the
+  // window's per-row signed x steps (NULL: the constant dx / sx)
+  int want_cv;
+  OutDev o2;
+  const double* xrows;
 };
 
 // The window value at (r, c) with the NaN fill applied; z is the raw value.
@@ -397,6 +408,20 @@ __device__ __forceinline__ void store_group(const OutDev& o, bool vec, int orr, 
     if (ok[k]) store_out(o, orr, occ + k, v[k]);
 }
 
+// Wave `strip` of row segment `sg`: its columns and rows; false when the wave
+// has no work.
+template <int LEFT>
+__device__ __forceinline__ bool rows4_at(const RowsTile4& S, int strip, int sg, int& c4, int& cb,
+                                         int& ra, int& rb) {
+  const int lane = threadIdx.x & 63;
+  ra = S.T.ir0 + sg * S.T.seg;
+  if (strip >= S.n_strips || ra >= S.T.ir1) return false;
+  rb = min(ra + S.T.seg, S.T.ir1);
+  cb = min(S.cb0 + strip * sr4_cols<LEFT>(), S.cb_last);
+  c4 = cb + 4 * lane;
+  return true;
+}
+
 // The wave's strip and row segment; false when the wave has no work.
 template <int LEFT>
 __device__ __forceinline__ bool rows4_wave(const RowsTile4& S, int& c4, int& cb, int& ra,
@@ -617,6 +642,128 @@ __global__ void __launch_bounds__(256) k_curv_inner4(RowsTile4 S4, Gauss9 G) {
 #pragma unroll
       for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
     if (r + 3 + AHEAD < H)  // row r + 3 + AHEAD
+      load_group(rowp, c4, zq[AHEAD - 1]);
+    rowp += ld;
+  }
+}
+
+
+// ---------------------------------------------------------- batch items
+// Hillshade, slope and curvature of a NaN-free window's interior in one walk
+// (the batch's stencil items; the window interior is rows [2, H-3] x cols
+// [2, W-3]): curvature's first derivatives P(r) and Q(r) are the gradient the
+// tiled block gives hillshade and slope when all three share the steps.  With
+// ROWS, the x divisors are row r's own: f32(2|hx|) for P and hillshade,
+// f32(2 hx) for curvature's second derivatives, RN(1/b) by an IEEE division
+// per row (the same values the tiled block's per-row coefficient tables
+// hold).  Curvature geometry: 252 columns per wave, half-group stores.
+__device__ __forceinline__ DivK row_divk(double h) { return make_divk((float)(2. * h)); }
+
+template <bool ROWS>
+__device__ __forceinline__ void stencil4_wave(const RowsTile4& S4, int strip, int sg) {
+  constexpr int LEFT = SR4_LEFT_CURV;
+  int c4, cb, ra, rb;
+  if (!rows4_at<LEFT>(S4, strip, sg, c4, cb, ra, rb)) return;  // the whole wave
+  const RowsTile& S = S4.T;
+  const int H = S.W.h;
+  const int lo = max(cb + LEFT, S.ic0), hi = min(cb + LEFT + sr4_cols<LEFT>(), S.ic1);
+  bool ok[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) ok[k] = c4 + k >= lo && c4 + k < hi;
+  const bool any = ok[0] || ok[1] || ok[2] || ok[3];
+  const int64_t ld = S.W.ld;
+  const float* rowp = S.W.p + (int64_t)(ra - 2) * ld;  // row ra - 2 (ra >= 2)
+  const PostK phs = load_post(S.o0), psl = load_post(S.o1), pcv = load_post(S.o2);
+  const CurvK& K = S.curv;
+  const ShadeK& KS = S.shade;
+  constexpr int AHEAD = SR4_AHEAD;
+  float zn[4], zq[AHEAD][4];
+  load_group(rowp, c4, zn);
+#pragma unroll
+  for (int a = 0; a < AHEAD; a++) {
+    if (ra - 1 + a < H)
+      load_group(rowp + (1 + a) * ld, c4, zq[a]);
+    else
+      zq[a][0] = zq[a][1] = zq[a][2] = zq[a][3] = 0.0f;
+  }
+  rowp += (1 + AHEAD) * ld;
+  float fi[4] = {0.f, 0.f, 0.f, 0.f}, fi1[4] = {0.f, 0.f, 0.f, 0.f};
+  float pm[4] = {0.f, 0.f, 0.f, 0.f}, p0[4] = {0.f, 0.f, 0.f, 0.f};
+  float qm[4] = {0.f, 0.f, 0.f, 0.f}, q0[4] = {0.f, 0.f, 0.f, 0.f};
+  float zc[4] = {0.f, 0.f, 0.f, 0.f}, zc1[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int r = ra - 4; r < rb; r++) {
+    const float* f4 = zn;  // NaN-free window: row r + 2 as loaded
+    float pp[4], qp[4];
+    // P(r + 1) takes row r + 1's |x| step
+    const DivK dxp = ROWS ? row_divk(fabs(S.xrows[min(max(r + 1, 0), H - 1)])) : S.dx;
+    const float l1 = lane_left(fi1[3]), r1 = lane_right(fi1[0]);
+    const float lp = lane_left(p0[3]), rp = lane_right(p0[0]);
+    const float lq = lane_left(q0[3]), rq = lane_right(q0[0]);
+    // first derivatives: the guarded division when hillshade or slope reads
+    // them (hillshade's bits), the unguarded one for curvature alone
+    const bool exact = S.want_hs || S.want_sl;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float dp = (k == 3 ? r1 : fi1[k + 1]) - (k == 0 ? l1 : fi1[k - 1]);
+      pp[k] = exact ? dq(dp, dxp) : dqu(dp, dxp);
+      qp[k] = exact ? dq(f4[k] - fi[k], S.dy) : dqu(f4[k] - fi[k], S.dy);
+    }
+    if (r >= ra && any) {
+      const int orr = r - S.r0, occ = c4 - S.c0;
+      if (S.want_cv) {
+        const DivK sxr = ROWS ? row_divk(S.xrows[r]) : S.sx;
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const float dxx = dqu((k == 3 ? rp : p0[k + 1]) - (k == 0 ? lp : p0[k - 1]), sxr);
+          const float dyx = dqu((k == 3 ? rq : q0[k + 1]) - (k == 0 ? lq : q0[k - 1]), sxr);
+          const float dxy = dqu(pp[k] - pm[k], S.sy);
+          const float dyy = dqu(qp[k] - qm[k], S.sy);
+          v[k] = isnan(zc[k]) ? __builtin_nanf("")
+                              : curvature_from(K, p0[k], q0[k], dxx, dxy, dyy, dyx);
+          v[k] = apply_post(pcv, v[k]);
+        }
+        store_group<true>(S.o2, S4.vec_out, orr, occ, ok, v);
+      }
+      if (S.want_hs || S.want_sl) {
+        const DivK dxr = ROWS ? row_divk(fabs(S.xrows[r])) : S.dx;
+        float hs[4], sl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          hs[k] = sl[k] = __builtin_nanf("");
+          if (!isnan(zc[k])) {
+            // the gradient of row r: P(r), Q(r)
+            const float dx = p0[k], dy = q0[k];
+            if (S.want_sl) sl[k] = slope_from(KS, dx, dy);
+            // (z factor 1 only: the batch keeps other hillshade tiles on
+            // the tiled block, which re-derives from the scaled values)
+            if (S.want_hs) hs[k] = shade_from_grad(KS, dx, dy);
+          }
+          hs[k] = apply_post(phs, hs[k]);
+          sl[k] = apply_post(psl, sl[k]);
+        }
+        if (S.want_hs) store_group<true>(S.o0, S4.vec_out, orr, occ, ok, hs);
+        if (S.want_sl) store_group<true>(S.o1, S4.vec_out, orr, occ, ok, sl);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      pm[k] = p0[k];
+      p0[k] = pp[k];
+      qm[k] = q0[k];
+      q0[k] = qp[k];
+      fi[k] = fi1[k];
+      fi1[k] = f4[k];
+      zc[k] = zc1[k];
+      zc1[k] = zn[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) zn[k] = zq[0][k];
+#pragma unroll
+    for (int a = 0; a + 1 < AHEAD; a++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) zq[a][k] = zq[a + 1][k];
+    if (r + 3 + AHEAD < H)
       load_group(rowp, c4, zq[AHEAD - 1]);
     rowp += ld;
   }
