@@ -17,7 +17,7 @@ FSG_OK, FSG_E_INVALID, FSG_E_HIP, FSG_E_UNSUPPORTED, FSG_E_WORKSPACE = 0, 1, 2, 
 OUT_F32, OUT_I16, OUT_U8 = 0, 1, 2
 POST_NONE, POST_DIV, POST_STRETCH = 0, 1, 2
 MAX_RADII = 16
-ABI_VERSION = 19  # FSG_ABI_VERSION of include/fsg.h
+ABI_VERSION = 20  # FSG_ABI_VERSION of include/fsg.h
 
 
 class NativeLibraryError(RuntimeError):

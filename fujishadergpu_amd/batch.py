@@ -162,11 +162,11 @@ class FusedBatch:
         self.config = N.fsg_batch_config(3, 0, 0, 8)
 
     def info(self) -> Dict[str, int]:
-        v = (C.c_int64 * 10)()
+        v = (C.c_int64 * 12)()
         N.check(N.lib().fsg_batch_info(self._h, v))
         keys = ("usm_items", "stencil_items", "fused_usm_tiles", "fallback_usm_tiles",
                 "stencil_tiles", "lds_bytes", "box_radii", "device_bytes", "openness_items",
-                "openness_tiles")
+                "openness_tiles", "row_tiles", "row_items")
         return dict(zip(keys, [int(x) for x in v]))
 
     def _cfg(self, config):

@@ -64,6 +64,15 @@ struct BatchDev {
   OpenTab T;           // the batch's one openness ray table
 };
 
+// Tile descriptor i of an array the host wrote before the launch and no
+// kernel writes, read through the constant address space: uniform fields take
+// scalar loads, and no store of the kernel makes the compiler read them again
+// (through a plain pointer every output store could alias them).
+template <class T>
+__device__ __forceinline__ const T& desc(const T* p, int i) {
+  return *(const T*)((const __attribute__((address_space(4))) T*)p + i);
+}
+
 // largest t with pre[t] <= item (pre[0] = 0, pre strictly covers the items)
 __device__ __forceinline__ int find_tile(const int* pre, int n, int item) {
   int lo = 0, hi = n - 1;
@@ -137,10 +146,10 @@ __device__ __forceinline__ void batch_drain(const BatchDev& B) {
       }
       if ((KINDS & BK_USM) && kind == 0) {
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(upre, B.n_usm_tiles, it));
-        usm_fast_block<NR, false>(B.usm[tl], it - upre[tl], lds);
+        usm_fast_block<NR, false>(desc(B.usm, tl), it - upre[tl], lds);
       } else if ((KINDS & BK_OPEN) && kind == 1) {
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(opre, B.n_op_tiles, it));
-        const OpenTileD& O = B.op[tl];
+        const OpenTileD& O = desc(B.op, tl);
         const int blk = it - opre[tl];
         const int by = blk / O.nbx;
         openness_item(O, B.T, by, blk - by * O.nbx);
@@ -148,7 +157,7 @@ __device__ __forceinline__ void batch_drain(const BatchDev& B) {
         // a row item: four adjacent strips of one row segment, a strip per wave
         const int ir = it - B.n_st_items;
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(rpre, B.n_rt_tiles, ir));
-        const RowsTile4& R = B.rt[tl];
+        const RowsTile4& R = desc(B.rt, tl);
         const int blk = ir - rpre[tl];
         const int ngx = (R.n_strips + 3) >> 2;
         const int sg = blk / ngx, xg = blk - sg * ngx;
@@ -159,7 +168,7 @@ __device__ __forceinline__ void batch_drain(const BatchDev& B) {
           stencil4_wave<false>(R, strip, sg);
       } else if (kind == 2) {
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(spre, B.n_st_tiles, it));
-        const StencilTile& S = B.st[tl];
+        const StencilTile& S = desc(B.st, tl);
         const int blk = it - spre[tl];
         const int by = blk / S.nbx, bx = blk - by * S.nbx;
         // the NaN-aware fill only in instances that carry NaN-aware tiles; in
@@ -799,6 +808,8 @@ extern "C" int fsg_batch_info(const fsg_batch* b, int64_t* info) {
   info[7] = (int64_t)b->dev_bytes;
   info[8] = b->n_op_items;
   info[9] = b->n_op_tiles;
+  info[10] = b->n_rt_tiles;
+  info[11] = b->n_rt_items;
   return FSG_OK;
 }
 

@@ -71,21 +71,24 @@ inline int quad_phase(const void* p) {
 inline int mod4(int x) { return ((x % 4) + 4) % 4; }
 
 // The four-columns-per-lane launch over the interior columns [oc0, oc1) that
-// whole aligned waves cover: needs a row stride that is a multiple of 4
-// floats and a 4-byte aligned window; each wave's 256 columns lie inside the
-// window; float4-aligned outputs take 16-byte stores.  seg > 0 fixes the rows
-// per wave (else seg_rows).  False when no wave fits.
+// whole waves cover: needs a 4-byte aligned window; each wave's 256 columns
+// lie inside the window; float32 outputs take 16-byte (8-byte) stores.  seg > 0
+// fixes the rows per wave (else seg_rows).  False when no wave fits.
 inline bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, int n_out,
                         int left, RowsTile4& S4, dim3& grd, int& oc0, int& oc1, int seg = 0) {
   const int cols = 256 - 2 * left;
   const int a = quad_phase(R.W.p);
-  if (a < 0 || R.W.ld % 4 != 0 || in.w == 0) return false;
-  int cb0 = (in.c0 - left) - mod4(a + in.c0 - left);  // aligned, cb0 + left <= in.c0
-  if (cb0 < 0) cb0 += 4 * ((-cb0 + 3) / 4);      // first aligned start inside the window
+  if (a < 0 || in.w == 0) return false;
+  // wave starts on 16-byte boundaries when every row keeps the same phase
+  // (row stride a multiple of 4 floats), else anywhere (dword-aligned loads)
+  const int q = R.W.ld % 4 == 0 ? 4 : 1;
+  auto modq = [q](int x) { return q == 4 ? mod4(x) : 0; };
+  int cb0 = (in.c0 - left) - modq(a + in.c0 - left);  // aligned, cb0 + left <= in.c0
+  if (cb0 < 0) cb0 += q * ((-cb0 + q - 1) / q);  // first aligned start inside the window
   const int cb_max = R.W.w - 256;                // a wave's last column <= W - 1
   const int in_c1 = in.c0 + in.w;
   if (cb0 > cb_max) return false;
-  const int cb_last = cb_max - mod4(a + cb_max);  // aligned like cb0, <= cb_max
+  const int cb_last = cb_max - modq(a + cb_max);  // aligned like cb0, <= cb_max
   const int64_t n_strips = ((int64_t)in_c1 - (cb0 + left) + cols - 1) / cols;
   oc0 = std::max(in.c0, cb0 + left);
   oc1 = std::min(in_c1, cb_last + left + cols);  // the last wave's last output + 1
@@ -103,9 +106,7 @@ inline bool rows4_setup(const RowsTile& R, const Rect& in, const OutDev* outs, i
   for (int k = 0; k < n_out; k++) {
     const OutDev& o = outs[k];
     if (!o.data) continue;
-    const int ph = quad_phase(o.data);
-    vec = vec && o.dtype == FSG_OUT_F32 && ph >= 0 && o.ld % 4 == 0 &&
-          mod4(ph + cb0 - R.c0) == 0;
+    vec = vec && o.dtype == FSG_OUT_F32 && quad_phase(o.data) >= 0;
   }
   S4.vec_out = vec;
   grd = dim3((unsigned)((n_strips + 3) / 4), (unsigned)nseg);

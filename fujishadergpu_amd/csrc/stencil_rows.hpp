@@ -294,16 +294,17 @@ __global__ void __launch_bounds__(256) k_curv_inner(RowsTile S, Gauss9 G) {
 // The same walks with four adjacent columns per lane: 16-byte loads (1 KB per
 // wave and row instead of 256 bytes, three 128-byte lines for 240 useful bytes
 // before), and only the group's outer neighbours cross lanes.  A wave covers
-// 256 columns starting at a 16-byte boundary and writes 256 - 2 LEFT of them
-// from column LEFT on; the next wave starts that many columns later (still
-// aligned).  LEFT = 4: lanes 1..62 write whole groups (16-byte stores), lanes
-// 0 and 63 only supply neighbours; LEFT = 2: every lane writes, the first
-// and last lanes half a group (8-byte stores).  Measured: hillshade/slope
-// (two outputs) run faster with LEFT = 4, curvature with LEFT = 2.
-// Needs a window row stride that is a multiple of 4 floats; waves go only
-// where their 256 columns lie inside the window (the last one moved left to
-// the last aligned start that fits, overlapping its neighbour), and the few
-// interior columns left at the sides take the general kernel.
+// 256 columns and writes 256 - 2 LEFT of them from column LEFT on; the next
+// wave starts that many columns later.  With a row stride that is a multiple
+// of 4 floats every wave starts at a 16-byte boundary in every row; other
+// strides take dword-aligned 16-byte accesses.  LEFT = 4: lanes 1..62 write
+// whole groups (16-byte stores), lanes 0 and 63 only supply neighbours;
+// LEFT = 2: every lane writes, the first and last lanes half a group (8-byte
+// stores).  Measured: hillshade/slope (two outputs) run faster with LEFT = 4,
+// curvature with LEFT = 2.  Waves go only where their 256 columns lie inside
+// the window (the last one moved left to the last start that fits, aligned
+// like the others, overlapping its neighbour), and the few interior columns
+// left at the sides take the general kernel.
 template <int LEFT>
 constexpr int sr4_cols() {
   return 256 - 2 * LEFT;
@@ -313,17 +314,22 @@ constexpr int SR4_AHEAD = 1;  // rows in flight ahead of the one used (more cost
 
 struct RowsTile4 {
   RowsTile T;
-  int cb0;      // first column of wave 0 (16-byte aligned address in every row)
-  int cb_last;  // the last aligned start whose 256 columns fit the window:
-                // later waves start there (overlapping writes of equal values)
+  int cb0;      // first column of wave 0 (a 16-byte aligned address in every
+                // row when the row stride is a multiple of 4 floats)
+  int cb_last;  // the last start (aligned like cb0) whose 256 columns fit the
+                // window: later waves start there (overlapping writes of equal values)
   int n_strips; // waves across the interior
-  int vec_out;  // o0 (and o1) take aligned float4 stores at the group starts
+  int vec_out;  // o0 (and o1) are float32: 16-byte (8-byte) stores per group
 };
 
-// The raw values of columns c4 .. c4+3 of the row at rowp (16-byte aligned:
-// the launch only places waves whose 256 columns lie inside the window).
+// The raw values of columns c4 .. c4+3 of the row at rowp, as one 16-byte
+// load: the launch only places waves whose 256 columns lie inside the window.
+// The address is 4-byte aligned; gfx9 and later take multi-dword global
+// accesses at dword alignment, a 16-byte aligned one (row stride a multiple of
+// 4 floats) touches one fewer cache line per wave and row.
 __device__ __forceinline__ void load_group(const float* rowp, int c4, float z[4]) {
-  const float4 t = *(const float4*)(rowp + c4);
+  float4 t;
+  __builtin_memcpy(&t, rowp + c4, sizeof(t));
   z[0] = t.x;
   z[1] = t.y;
   z[2] = t.z;
@@ -384,22 +390,24 @@ __device__ __forceinline__ void wave_fill4(const WinDev& W, int x, int c4, int c
 }
 
 // Writes v[0..3] at output row orr, columns occ .. occ+3 where ok[k] is set.
-// vec: the group starts on a 16-byte boundary of a float32 output.  HALVES
-// stores each half as one 8-byte store (a wave's first and last lanes write
-// one half each), else a whole group as one 16-byte store; groups cut by the
-// region's first or last column, and other outputs, store per column.
+// vec: a float32 output (4-byte aligned, like the input row).  HALVES stores
+// each half as one 8-byte store (a wave's first and last lanes write one half
+// each), else a whole group as one 16-byte store; groups cut by the region's
+// first or last column, and other outputs, store per column.
 template <bool HALVES>
 __device__ __forceinline__ void store_group(const OutDev& o, bool vec, int orr, int occ,
                                             const bool ok[4], const float v[4]) {
   if (vec) {
     float* d = (float*)o.data + (int64_t)orr * o.ld + occ;
     if (!HALVES && ok[0] && ok[3]) {
-      *(float4*)d = make_float4(v[0], v[1], v[2], v[3]);
+      const float4 t = make_float4(v[0], v[1], v[2], v[3]);
+      __builtin_memcpy(d, &t, sizeof(t));
       return;
     }
     if (HALVES) {
-      if (ok[0] && ok[1]) *(float2*)d = make_float2(v[0], v[1]);
-      if (ok[2] && ok[3]) *(float2*)(d + 2) = make_float2(v[2], v[3]);
+      const float2 lo = make_float2(v[0], v[1]), hi = make_float2(v[2], v[3]);
+      if (ok[0] && ok[1]) __builtin_memcpy(d, &lo, sizeof(lo));
+      if (ok[2] && ok[3]) __builtin_memcpy(d + 2, &hi, sizeof(hi));
       if (ok[0] == ok[1] && ok[2] == ok[3]) return;
     }
   }

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define FSG_ABI_VERSION 19
+#define FSG_ABI_VERSION 20
 
 enum {
   FSG_OK = 0,
@@ -732,10 +732,11 @@ typedef struct fsg_batch fsg_batch;
  * coarse grids and descriptors, a synchronous upload); the tiles' buffers must
  * outlive it.  fsg_batch_run enqueues one pass on a stream; fsg_batch_capture
  * records one pass into a hipGraph (non-default stream; per-kernel timing
- * off) and fsg_batch_replay launches it.  fsg_batch_info fills int64[10]:
+ * off) and fsg_batch_replay launches it.  fsg_batch_info fills int64[12]:
  * TopoUSM items, stencil items, fused TopoUSM tiles, fallback tiles, stencil
  * tiles, LDS bytes, box radii of the fused kernel, device bytes held,
- * openness items, openness tiles.
+ * openness items, openness tiles, stencil tiles whose interior is walked by
+ * rows, and the row-walking items among the stencil items.
  */
 int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batch** out);
 int fsg_batch_run(fsg_batch* b, const fsg_batch_config* cfg, void* stream);

@@ -72,6 +72,8 @@ def test_fused_batch_equals_per_tile_launches(cuda):
     print("batch info", info, flush=True)
     assert info["fallback_usm_tiles"] == 2              # unsorted radii + the NaN tile
     assert info["fused_usm_tiles"] == 7 and info["box_radii"] == 3
+    # row-walking interiors, row strides 1030 and 410 floats among them
+    assert info["row_tiles"] >= 2 and info["row_items"] > 0
     for cfg in [(3, 0, 8), (1, 1, 1), (8, 0, 32)]:
         b.run(cfg)
         torch.cuda.synchronize()

@@ -23,7 +23,7 @@ def test_binding_table_covers_header():
 
 def test_loader_validates_abi_and_symbols():
     L = N.lib()
-    assert L.fsg_abi_version() == N.ABI_VERSION == 19
+    assert L.fsg_abi_version() == N.ABI_VERSION == 20
     assert L.fsg_launch_count() >= 0
 
 
