@@ -50,9 +50,6 @@ struct BatchDev {
   const StencilTile* st;
   const int* st_pre;   // prefix of stencil blocks per tile (n + 1)
   int n_st_tiles, n_st_items;
-  const RowsTile4* rt; // NaN-free tile interiors as row-walking items
-  const int* rt_pre;   // prefix of their items (n + 1)
-  int n_rt_tiles, n_rt_items;
   const OpenTileD* op;
   const int* op_pre;   // prefix of openness items per tile (n + 1)
   int n_op_tiles, n_op_items;
@@ -110,18 +107,16 @@ template <int NR, int KINDS>
 __device__ __forceinline__ void batch_drain(const BatchDev& B) {
   extern __shared__ float lds[];
   __shared__ int s_item;
-  const int n_stencil = B.n_st_items + B.n_rt_items;  // tile blocks, then row items
+  const int n_stencil = B.n_st_items;
   const int total = B.n_usm_items + n_stencil + B.n_op_items;
   // the prefix arrays staged in LDS once: tile lookups stay on chip
   int* upre = (int*)((char*)lds + B.pre_off);
   int* spre = upre + B.n_usm_tiles + 1;
   int* opre = spre + B.n_st_tiles + 1;
-  int* rpre = opre + B.n_op_tiles + 1;
   for (int e = threadIdx.x; e <= B.n_usm_tiles; e += 256) upre[e] = B.usm_pre[e];
   for (int e = threadIdx.x; e <= B.n_st_tiles; e += 256) spre[e] = B.st_pre[e];
   if (KINDS & BK_OPEN)
     for (int e = threadIdx.x; e <= B.n_op_tiles; e += 256) opre[e] = B.op_pre[e];
-  for (int e = threadIdx.x; e <= B.n_rt_tiles; e += 256) rpre[e] = B.rt_pre[e];
   // item groups in queue order: kinds 0 TopoUSM, 1 openness, 2 stencil
   const int first = B.order ? 2 : 0, last = B.order ? 0 : 2;
   const int n_first = B.order ? n_stencil : B.n_usm_items;
@@ -153,19 +148,6 @@ __device__ __forceinline__ void batch_drain(const BatchDev& B) {
         const int blk = it - opre[tl];
         const int by = blk / O.nbx;
         openness_item(O, B.T, by, blk - by * O.nbx);
-      } else if (!(KINDS & BK_OPEN) && kind == 2 && it >= B.n_st_items) {
-        // a row item: four adjacent strips of one row segment, a strip per wave
-        const int ir = it - B.n_st_items;
-        const int tl = __builtin_amdgcn_readfirstlane(find_tile(rpre, B.n_rt_tiles, ir));
-        const RowsTile4& R = desc(B.rt, tl);
-        const int blk = ir - rpre[tl];
-        const int ngx = (R.n_strips + 3) >> 2;
-        const int sg = blk / ngx, xg = blk - sg * ngx;
-        const int strip = xg * 4 + (threadIdx.x >> 6);
-        if (R.T.xrows)
-          stencil4_wave<true>(R, strip, sg);
-        else
-          stencil4_wave<false>(R, strip, sg);
       } else if (kind == 2) {
         const int tl = __builtin_amdgcn_readfirstlane(find_tile(spre, B.n_st_tiles, it));
         const StencilTile& S = desc(B.st, tl);
@@ -198,6 +180,30 @@ k_batch_rays(BatchDev B) {
 template __global__ void k_batch_rays<5>(BatchDev);
 template __global__ void k_batch_rays<6>(BatchDev);
 template __global__ void k_batch_rays<8>(BatchDev);
+
+// The NaN-free stencil tiles' interiors as row-walking items, one workgroup
+// per item: four adjacent strips of one row segment, a strip per wave.  A
+// launch of its own, ahead of the persistent kernel, so that the walk's
+// registers alone set its occupancy (inside the persistent kernel the TopoUSM
+// path's registers would halve it).  The items cover columns no other item
+// writes.
+__global__ void __launch_bounds__(256) k_batch_rows(const RowsTile4* rt, const int* pre, int n) {
+  const int ir = blockIdx.x;
+  int lo = 0, hi = n - 1;  // largest tile t with pre[t] <= ir
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (desc(pre, mid) <= ir) lo = mid; else hi = mid - 1;
+  }
+  const RowsTile4& R = desc(rt, lo);
+  const int blk = ir - desc(pre, lo);
+  const int ngx = (R.n_strips + 3) >> 2;
+  const int sg = blk / ngx, xg = blk - sg * ngx;
+  const int strip = xg * 4 + (threadIdx.x >> 6);
+  if (R.T.xrows)
+    stencil4_wave<true>(R, strip, sg);
+  else
+    stencil4_wave<false>(R, strip, sg);
+}
 
 // NaN-aware stencil tiles of a batch, one launch per tile.
 __global__ void __launch_bounds__(256) k_stencil_nanaw(StencilTile S, Gauss9 G) {
@@ -350,12 +356,15 @@ static int plan_prep(fsg_batch* b) {
 
 // Interiors smaller than this keep the tile's blocks.
 constexpr int64_t BATCH_ROWS_MIN_PX = int64_t(1) << 16;
+// Block strips of a tile whose interior is walked by rows: the interior
+// columns left at either side, and four border strips.
+constexpr int BATCH_TILE_STRIPS = 6;
 
 // A NaN-free stencil tile's interior as row items (appended to rt, rt_pre)
 // and the rest of its region as up to six block strips; false: the tile
 // keeps its blocks (NaN-aware windows, a hillshade z factor other than 1, a
-// hillshade/slope gradient that differs from curvature's, small or unaligned
-// interiors).
+// hillshade/slope gradient that differs from curvature's, interiors too small
+// or too narrow for one 256-column wave).
 static bool rows_item_plan(const StencilTile& S, std::vector<RowsTile4>& rt,
                            std::vector<int>& rt_pre, Rect* strips, int& n_strips) {
   const bool shade = S.want_hs || S.want_sl;
@@ -451,9 +460,10 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   const size_t ws_total = off;
   const size_t usm_bytes = al256(sizeof(UsmArgs) * n), pre_bytes = al256(sizeof(int) * (n + 1));
   // a tile's stencil work: its blocks, or row items for the interior and up
-  // to four border strips of blocks
-  const size_t st_bytes = al256(sizeof(StencilTile) * 4 * n);
-  const size_t st_pre_bytes = al256(sizeof(int) * (4 * n + 1));
+  // to BATCH_TILE_STRIPS strips of blocks
+  const size_t st_cap = (size_t)BATCH_TILE_STRIPS * n;
+  const size_t st_bytes = al256(sizeof(StencilTile) * st_cap);
+  const size_t st_pre_bytes = al256(sizeof(int) * (st_cap + 1));
   const size_t rt_bytes = al256(sizeof(RowsTile4) * n);
   const size_t op_bytes = al256(sizeof(OpenTileD) * n);
   b->dev_bytes = ws_total + usm_bytes + 3 * pre_bytes + st_bytes + st_pre_bytes + rt_bytes +
@@ -567,11 +577,9 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
     b->n_stencil_in++;
     // NaN-free tiles with a big enough interior: row items over the interior
     // (the per-window interior kernels' walk), blocks over the border strips
-    Rect strips[6];
+    Rect strips[BATCH_TILE_STRIPS];
     int n_strips = 0;
-    // (the instances with openness items keep the blocks: their register
-    // budget is capped for the openness items)
-    if (!(b->kinds & BK_OPEN) && rows_item_plan(S, rt, rt_pre, strips, n_strips)) {
+    if (rows_item_plan(S, rt, rt_pre, strips, n_strips)) {
       for (int k = 0; k < n_strips; k++) {
         const Rect& q = strips[k];
         StencilTile T = S;
@@ -615,6 +623,10 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   b->n_op_tiles = (int)op.size();
   b->n_op_items = op_pre.back();
   b->lds = lds;
+  if (st.size() > st_cap || rt.size() > (size_t)n) {
+    fsg_batch_destroy(b);
+    return fail(FSG_E_UNSUPPORTED, "fsg_batch_create: stencil descriptors exceed their buffer");
+  }
   int rc = FSG_OK;
   if (!fused_args.empty())
     rc = check_hip(hipMemcpy(b->d_usm, fused_args.data(), sizeof(UsmArgs) * fused_args.size(),
@@ -687,7 +699,12 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
     hipLaunchKernelGGL(k_stencil_nanaw, grd, dim3(256), 0, s, S, make_gauss9());
     FSG_CHECK_LAUNCH("k_stencil_nanaw");
   }
-  const int total = b->n_usm_items + b->n_st_items + b->n_rt_items + b->n_op_items;
+  if (b->n_rt_items > 0) {
+    hipLaunchKernelGGL(k_batch_rows, dim3(b->n_rt_items), dim3(256), 0, s, b->d_rt, b->d_rt_pre,
+                       b->n_rt_tiles);
+    FSG_CHECK_LAUNCH("k_batch_rows");
+  }
+  const int total = b->n_usm_items + b->n_st_items + b->n_op_items;
   if (total == 0) return FSG_OK;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_queue);
   FSG_CHECK_LAUNCH("k_batch_reset");
@@ -710,10 +727,6 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
   D.st_pre = b->d_st_pre;
   D.n_st_tiles = b->n_st_tiles;
   D.n_st_items = b->n_st_items;
-  D.rt = b->d_rt;
-  D.rt_pre = b->d_rt_pre;
-  D.n_rt_tiles = b->n_rt_tiles;
-  D.n_rt_items = b->n_rt_items;
   D.op = b->d_op;
   D.op_pre = b->d_op_pre;
   D.n_op_tiles = b->n_op_tiles;
@@ -725,7 +738,7 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
   D.pre_off = (int)((b->lds + 15) & ~(size_t)15);
   const size_t lds_total =
       (size_t)D.pre_off +
-      sizeof(int) * ((size_t)D.n_usm_tiles + D.n_st_tiles + D.n_op_tiles + D.n_rt_tiles + 4);
+      sizeof(int) * ((size_t)D.n_usm_tiles + D.n_st_tiles + D.n_op_tiles + 3);
   if (lds_total > 150 * 1024)
     return fail(FSG_E_UNSUPPORTED, "fsg_batch_run: too many tiles for the LDS tile index");
   D.G = make_gauss9();
