@@ -5,9 +5,11 @@ The reference walks its tiles one by one and runs one algorithm per call
 (``core/tile_processor.py:1635-1683`` -> ``run_tile_algorithm``,
 ``core/tile_compute.py:93-129``).  A ``FusedBatch`` plans a whole set of
 independent tiles -- ragged sizes, projected or geographic -- once, and runs
-every tile's four products through one persistent kernel that drains a device
-work queue (``fsg_batch_run``), optionally captured once in a hipGraph and
-replayed (``fsg_batch_capture`` / ``fsg_batch_replay``).  ``run_unfused`` is
+every tile's four products as one batched launch over the NaN-free stencil
+tiles' interiors (row-walking items) and one persistent kernel that drains a
+device work queue over everything else (``fsg_batch_run``), optionally
+captured once in a hipGraph and replayed (``fsg_batch_capture`` /
+``fsg_batch_replay``).  ``run_unfused`` is
 the per-tile, per-algorithm path through the same algorithm classes, and the
 two produce identical bits.  ``tune`` picks the persistent launch shape by
 measuring it.
@@ -15,7 +17,7 @@ measuring it.
 ``RasterJob`` is the whole-raster form: one device raster, several
 local-mode products (hillshade, slope, curvature, openness), its tiles --
 ragged edge tiles included -- planned once and run as one NoData pass, the
-openness stretch pre-pass and ONE persistent launch, which a graph captures
+openness stretch pre-pass and the batch's launches, which a graph captures
 and replays.  Its outputs equal ``scheduler.process_dem``'s for each product
 bit for bit.
 """
@@ -266,7 +268,8 @@ class RasterJob:
     1. the NoData sentinel -> NaN pass with the per-window counts;
     2. the openness stretch pre-pass (``stats.RayPrepass``);
     3. NaN fills of skipped tiles;
-    4. ONE persistent ``k_batch_fused`` launch over every tile's stencil
+    4. the batch: row items over the interiors of NaN-free stencil tiles,
+       then ONE persistent ``k_batch_fused`` launch over the other stencil
        blocks (NaN-aware ones included) and openness blocks, each tile
        reading one window with the widest product halo -- or, with
        ``set_split(True)`` (the tuner's choice when it measures faster), the

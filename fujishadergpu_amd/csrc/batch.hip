@@ -1,7 +1,8 @@
 // Batched fused launch: many independent tiles (ragged sizes, each its own
 // window) and, per tile, hillshade + slope + curvature + TopoUSM, or
 // hillshade + slope + curvature (NaN-aware included) + openness, in ONE
-// persistent kernel that drains a device work queue.
+// persistent kernel that drains a device work queue, after one batched launch
+// over the NaN-free stencil tiles' interiors (k_batch_rows).
 //
 // The reference runs one tile at a time and one algorithm at a time
 // (core/tile_processor.py:1635-1683 calling run_tile_algorithm per tile);
@@ -15,7 +16,10 @@
 // curvature from them (stencil_tile_block, the per-window kernels' block
 // function too); both item kinds
 // run the same device functions as the per-window kernels (stencil.hpp,
-// usm.hpp), so fused and unfused outputs are identical bit for bit.
+// usm.hpp), so fused and unfused outputs are identical bit for bit.  The row
+// items (stencil4_wave) form the same first derivatives with the same
+// correctly rounded divisions; curvature's second derivatives divide without
+// the range guard, the same quotients for dividends in [2^-75, 2^75].
 // Per-tile coarse-grid preparation (decimation, coarse box means) is
 // enqueued before the persistent kernel; the whole sequence can be captured
 // once in a hipGraph and replayed.
