@@ -117,16 +117,16 @@ k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __rest
     decimate_block(W, f, out, oh, ow, nan_flag, blockIdx.x, orow);
 }
 
-// The same cell means for f >= 4 with four source columns per lane: f rows of
-// four independent loads in flight (one float4 when VEC, i.e. 16-byte aligned
-// rows), then groups of f/4 lanes reduce by shuffles.  The sums are exact in
-// float64, so the cells equal k_decimate's bit for bit.
-template <bool VEC>
-__global__ void __launch_bounds__(256)
-k_decimate4(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __restrict__ nan_flag) {
-  // grid-stride over output rows (block-uniform, so the shuffles see every lane)
-  for (int orow = blockIdx.y; orow < oh; orow += gridDim.y) {
-  const int c = (blockIdx.x * 256 + threadIdx.x) * 4;  // first of this lane's source columns
+// The same cell means for f >= 4 (a power of two <= 16) with four source
+// columns per lane: one 16-byte load per row (dword-aligned, which gfx9 and
+// later take as one access), then groups of f/4 lanes reduce by shuffles.
+// The sums are exact in float64, so the cells equal k_decimate's bit for bit.
+// Block bx covers source columns bx*1024 .. bx*1024 + 1023 of output row orow;
+// every lane of the block must call it (the shuffles).
+__device__ __forceinline__ void decimate4_block(const WinDev& W, int f, float* __restrict__ out,
+                                                int oh, int ow, int* __restrict__ nan_flag,
+                                                int bx, int orow) {
+  const int c = (bx * 256 + threadIdx.x) * 4;  // first of this lane's source columns
   const int r0 = orow * f, r1 = min(r0 + f, W.h);
   double tot = 0.0;
   int cnt = 0;
@@ -134,8 +134,9 @@ k_decimate4(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __res
     for (int r = r0; r < r1; r++) {
       const float* p = W.p + (int64_t)r * W.ld + c;
       float v[4];
-      if (VEC && c + 3 < W.w) {
-        const float4 q = *(const float4*)p;
+      if (c + 3 < W.w) {
+        float4 q;
+        __builtin_memcpy(&q, p, sizeof(q));
         v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
       } else {
 #pragma unroll
@@ -166,7 +167,13 @@ k_decimate4(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __res
       out[(int64_t)orow * ow + oc] = res;
     }
   }
-  }
+}
+
+__global__ void __launch_bounds__(256)
+k_decimate4(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __restrict__ nan_flag) {
+  // grid-stride over output rows (block-uniform, so the shuffles see every lane)
+  for (int orow = blockIdx.y; orow < oh; orow += gridDim.y)
+    decimate4_block(W, f, out, oh, ow, nan_flag, blockIdx.x, orow);
 }
 
 // Several independent decimations in one launch (the coarse preparation of a
@@ -186,7 +193,10 @@ k_decimate_batch(const DecJob* __restrict__ jobs, const int* __restrict__ pre, i
   const int j = batch_job_of(pre, n, blockIdx.x);
   const DecJob J = jobs[j];
   const int local = blockIdx.x - pre[j];
-  decimate_block(J.W, J.f, J.out, J.oh, J.ow, nullptr, local % J.nbx, local / J.nbx);
+  if (J.f >= 4)  // job-uniform: nbx counts 1024-column blocks
+    decimate4_block(J.W, J.f, J.out, J.oh, J.ow, nullptr, local % J.nbx, local / J.nbx);
+  else
+    decimate_block(J.W, J.f, J.out, J.oh, J.ow, nullptr, local % J.nbx, local / J.nbx);
 }
 
 // Separable Gaussian pass of the enclosed-void fill, mode 'nearest', scipy
@@ -1225,11 +1235,7 @@ static int decimate(const fsg_window* win, int ds, float* grid, int hc, int wc, 
   WinDev W{win->data, win->ld, win->h, win->w};
   if (ds >= 4) {
     dim3 blk(256), grd((win->w + 1023) / 1024, std::min(hc, 32768));
-    const bool vec = win->ld % 4 == 0 && (reinterpret_cast<uintptr_t>(win->data) & 15) == 0;
-    if (vec)
-      hipLaunchKernelGGL(k_decimate4<true>, grd, blk, 0, s, W, ds, grid, hc, wc, flag);
-    else
-      hipLaunchKernelGGL(k_decimate4<false>, grd, blk, 0, s, W, ds, grid, hc, wc, flag);
+    hipLaunchKernelGGL(k_decimate4, grd, blk, 0, s, W, ds, grid, hc, wc, flag);
     FSG_CHECK_LAUNCH("k_decimate4");
   } else {
     dim3 blk(256), grd((win->w + 255) / 256, std::min(hc, 32768));
@@ -1460,9 +1466,11 @@ int fsg::usm_prep_jobs(const fsg_window* win, const fsg_topousm_params* p, void*
     if (r_small <= 1) return FSG_E_UNSUPPORTED;
   }
   const WinDev W{win->data, win->ld, win->h, win->w};
-  for (int j = 0; j < P.n_ds; j++)
+  for (int j = 0; j < P.n_ds; j++) {
+    const int cols = P.ds_val[j] >= 4 ? 1024 : 256;  // per block (decimate4_block / decimate_block)
     dec.push_back(DecJob{W, P.ds_val[j], (float*)(ws + P.grid_off[j]), P.hc[j], P.wc[j],
-                         (win->w + 255) / 256});
+                         (win->w + cols - 1) / cols});
+  }
   float* tv = (float*)(ws + P.tmp_off[0]);
   float* tw = (float*)(ws + P.tmp_off[1]);
   int level = 0;

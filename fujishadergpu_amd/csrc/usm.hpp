@@ -390,7 +390,7 @@ struct DecJob {
   int f;
   float* out;
   int oh, ow;
-  int nbx;   // blocks per coarse row
+  int nbx;   // blocks per coarse row (of 1024 source columns for f >= 4, else 256)
 };
 struct BoxJob {
   const float* a;
