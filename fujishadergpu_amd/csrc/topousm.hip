@@ -117,6 +117,53 @@ k_decimate(WinDev W, int f, float* __restrict__ out, int oh, int ow, int* __rest
     decimate_block(W, f, out, oh, ow, nan_flag, blockIdx.x, orow);
 }
 
+// Columns c..c+3 of row r (c < w): one 16-byte load where all four lie inside
+// (dword-aligned: windows start at any column), else per column with the
+// columns past w absent (NaN).
+__device__ __forceinline__ void row4(const WinDev& W, int r, int c, float v[4]) {
+  const float* p = W.p + (int64_t)r * W.ld + c;
+  if (c + 3 < W.w) {
+    float4 q;
+    __builtin_memcpy(&q, p, sizeof(q));
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = c + k < W.w ? p[k] : __builtin_nanf("");
+  }
+}
+
+// float64 sum and count of the finite values of four columns c..c+3 (c < w)
+// over rows r0..r1-1, added row by row and column by column.  The loads of
+// four rows are issued before their values are added (a runtime-length loop
+// would otherwise wait for each row's load in turn).
+__device__ __forceinline__ void dec_rows4(const WinDev& W, int c, int r0, int r1, double& tot,
+                                          int& cnt) {
+  int r = r0;
+  for (; r + 4 <= r1; r += 4) {
+    float v[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) row4(W, r + i, c, v[i]);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (isfinite(v[i][k])) {
+          tot += (double)v[i][k];
+          cnt++;
+        }
+  }
+  for (; r < r1; r++) {
+    float v[4];
+    row4(W, r, c, v);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (isfinite(v[k])) {
+        tot += (double)v[k];
+        cnt++;
+      }
+  }
+}
+
 // The same cell means for f >= 4 (a power of two <= 16) with four source
 // columns per lane: one 16-byte load per row (dword-aligned, which gfx9 and
 // later take as one access), then groups of f/4 lanes reduce by shuffles.
@@ -130,26 +177,7 @@ __device__ __forceinline__ void decimate4_block(const WinDev& W, int f, float* _
   const int r0 = orow * f, r1 = min(r0 + f, W.h);
   double tot = 0.0;
   int cnt = 0;
-  if (c < W.w) {
-    for (int r = r0; r < r1; r++) {
-      const float* p = W.p + (int64_t)r * W.ld + c;
-      float v[4];
-      if (c + 3 < W.w) {
-        float4 q;
-        __builtin_memcpy(&q, p, sizeof(q));
-        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = c + k < W.w ? p[k] : __builtin_nanf("");
-      }
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (isfinite(v[k])) {
-          tot += (double)v[k];
-          cnt++;
-        }
-    }
-  }
+  if (c < W.w) dec_rows4(W, c, r0, r1, tot, cnt);
   for (int off = 1; off < f / 4; off <<= 1) {
     tot += __shfl_down(tot, off, 64);
     cnt += __shfl_down(cnt, off, 64);
@@ -392,7 +420,6 @@ struct DualDecJob {
   int fa, fb;      // fb = 0: one factor
   int oha, owa, ohb, owb;
   int nbx;         // blocks per band (1024 source columns each)
-  int vec;         // 16-byte aligned rows: float4 loads
 };
 struct DecTab {
   DualDecJob j[CMB_MAX_DEC];
@@ -414,29 +441,6 @@ __device__ __forceinline__ int tab_job_of(const int* pre, int n, int g) {
   return lo;
 }
 
-// float64 sum and count of the finite values of four columns c..c+3 over rows
-// r0..r1-1 (columns past w count as absent)
-__device__ __forceinline__ void dec_rows4(const WinDev& W, int c, int r0, int r1, bool vec,
-                                          double& tot, int& cnt) {
-  for (int r = r0; r < r1; r++) {
-    const float* p = W.p + (int64_t)r * W.ld + c;
-    float v[4];
-    if (vec && c + 3 < W.w) {
-      const float4 q = *(const float4*)p;
-      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; k++) v[k] = c + k < W.w ? p[k] : __builtin_nanf("");
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (isfinite(v[k])) {
-        tot += (double)v[k];
-        cnt++;
-      }
-  }
-}
-
 // reduce over groups of g lanes (g a power of two <= 4), result in the group's first lane
 __device__ __forceinline__ void dec_reduce(double& tot, int& cnt, int g) {
   for (int off = 1; off < g; off <<= 1) {
@@ -455,13 +459,12 @@ __global__ void __launch_bounds__(256) k_decimate_tab(DecTab T) {
   const int local = blockIdx.x - T.pre[j];
   const int bx = local % J.nbx, by = local / J.nbx;
   const int c = (bx * 256 + threadIdx.x) * 4;
-  const bool vec = J.vec != 0;
   if (J.fb == 0) {
     const int f = J.fa;
     const int r0 = by * f, r1 = min(r0 + f, J.W.h);
     double tot = 0.0;
     int cnt = 0;
-    if (c < J.W.w) dec_rows4(J.W, c, r0, r1, vec, tot, cnt);
+    if (c < J.W.w) dec_rows4(J.W, c, r0, r1, tot, cnt);
     dec_reduce(tot, cnt, f / 4);
     if ((threadIdx.x & (f / 4 - 1)) == 0) {
       const int oc = c / f;
@@ -477,7 +480,7 @@ __global__ void __launch_bounds__(256) k_decimate_tab(DecTab T) {
     const int r0 = by * fb + g * fa, r1 = min(r0 + fa, J.W.h);
     double ta = 0.0;
     int ca = 0;
-    if (c < J.W.w && r0 < r1) dec_rows4(J.W, c, r0, r1, vec, ta, ca);
+    if (c < J.W.w && r0 < r1) dec_rows4(J.W, c, r0, r1, ta, ca);
     tb += ta;
     cb += ca;
     dec_reduce(ta, ca, fa / 4);
@@ -1772,7 +1775,6 @@ extern "C" int fsg_topousm_coarse_means_batch(const fsg_coarse_job* jobs, int32_
     D.oha = (A.win.h + A.ds - 1) / A.ds;
     D.owa = (A.win.w + A.ds - 1) / A.ds;
     D.nbx = (A.win.w + 1023) / 1024;
-    D.vec = (A.win.ld % 4 == 0 && (reinterpret_cast<uintptr_t>(A.win.data) & 15) == 0) ? 1 : 0;
     paired[i] = 1;
     if (partner >= 0) {
       const fsg_coarse_job& B = jobs[partner];
