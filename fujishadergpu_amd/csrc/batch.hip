@@ -65,15 +65,6 @@ struct BatchDev {
   OpenTab T;           // the batch's one openness ray table
 };
 
-// Tile descriptor i of an array the host wrote before the launch and no
-// kernel writes, read through the constant address space: uniform fields take
-// scalar loads, and no store of the kernel makes the compiler read them again
-// (through a plain pointer every output store could alias them).
-template <class T>
-__device__ __forceinline__ const T& desc(const T* p, int i) {
-  return *(const T*)((const __attribute__((address_space(4))) T*)p + i);
-}
-
 // largest t with pre[t] <= item (pre[0] = 0, pre strictly covers the items)
 __device__ __forceinline__ int find_tile(const int* pre, int n, int item) {
   int lo = 0, hi = n - 1;
@@ -227,6 +218,8 @@ struct fsg_batch {
   std::vector<fsg_batch_tile> tiles;
   std::vector<size_t> ws_off;         // per tile: TopoUSM workspace offset, or SIZE_MAX
   std::vector<int> usm_fused;         // tile indices in the fused kernel
+  std::vector<int> usm_wave;          // tile indices in the batched wavefront main pass
+  WaveBatch wave;
   std::vector<int> usm_fallback;      // tile indices run with per-tile launches
   std::vector<StencilTile> st_fallback;  // NaN-aware stencil tiles (own launches)
   int nr = -1;
@@ -291,7 +284,9 @@ static const fsg_output* opt(const fsg_output& o) { return o.data ? &o : nullptr
 static int plan_prep(fsg_batch* b) {
   std::vector<DecJob> dec;
   std::vector<std::vector<BoxJob>> box0, box1;
-  for (int i : b->usm_fused) {
+  std::vector<int> with_prep = b->usm_fused;
+  with_prep.insert(with_prep.end(), b->usm_wave.begin(), b->usm_wave.end());
+  for (int i : with_prep) {
     const fsg_batch_tile& t = b->tiles[i];
     std::vector<DecJob> d1;
     std::vector<std::vector<BoxJob>> a0, a1;
@@ -526,11 +521,33 @@ extern "C" int fsg_batch_create(const fsg_batch_tile* tiles, int32_t n, fsg_batc
   for (auto& kv : nr_count)
     if (kv.second > best_n) best = kv.first, best_n = kv.second;
   b->nr = best;
+  // the fused-path tiles the batched wavefront main pass takes leave the
+  // persistent kernel (its single-wavefront blocks run the main pass faster
+  // than the workgroup blocks, the same bits)
+  std::vector<WaveTileIn> wave_in;
+  std::vector<int> wave_idx;
+  for (int i = 0; i < n; i++) {
+    const fsg_batch_tile& t = b->tiles[i];
+    if (!wants_usm(t) || !fast_ok[i] || args[i].nr != best) continue;
+    wave_in.push_back(WaveTileIn{&t.win, &t.usm, b->dev + b->ws_off[i], &args[i]});
+    wave_idx.push_back(i);
+  }
+  std::vector<char> taken;
+  if (int rc = wave_batch_create(wave_in, taken, &b->wave)) {
+    fsg_batch_destroy(b);
+    return rc;
+  }
+  std::vector<char> in_wave(n, 0);
+  for (size_t k = 0; k < wave_idx.size(); k++)
+    if (taken[k]) {
+      in_wave[wave_idx[k]] = 1;
+      b->usm_wave.push_back(wave_idx[k]);
+    }
   std::vector<UsmArgs> fused_args;
   std::vector<int> usm_pre{0};
   size_t lds = sizeof(float) * ST_LDS_FLOATS_ROWS;
   for (int i = 0; i < n; i++) {
-    if (!wants_usm(b->tiles[i])) continue;
+    if (!wants_usm(b->tiles[i]) || in_wave[i]) continue;
     if (fast_ok[i] && args[i].nr == best) {
       b->usm_fused.push_back(i);
       fused_args.push_back(args[i]);
@@ -698,6 +715,7 @@ static int batch_enqueue(fsg_batch* b, const fsg_batch_config* cfg, hipStream_t 
                                s);
     if (rc) return rc;
   }
+  if (int rc = wave_batch_launch(b->wave, s)) return rc;
   for (const StencilTile& S : b->st_fallback) {
     dim3 grd(S.nbx, (S.rh + ST_TH - 1) / ST_TH);
     hipLaunchKernelGGL(k_stencil_nanaw, grd, dim3(256), 0, s, S, make_gauss9());
@@ -815,9 +833,9 @@ extern "C" int fsg_batch_replay(fsg_batch* b, void* stream) {
 
 extern "C" int fsg_batch_info(const fsg_batch* b, int64_t* info) {
   if (!b || !info) return fail(FSG_E_INVALID, "fsg_batch_info: NULL argument");
-  info[0] = b->n_usm_items;
+  info[0] = b->n_usm_items + b->wave.n_blocks;
   info[1] = b->n_st_items + b->n_rt_items;
-  info[2] = (int64_t)b->usm_fused.size();
+  info[2] = (int64_t)(b->usm_fused.size() + b->usm_wave.size());
   info[3] = (int64_t)b->usm_fallback.size();
   info[4] = b->n_stencil_in + (int64_t)b->st_fallback.size();
   info[5] = (int64_t)b->lds;
@@ -836,5 +854,6 @@ extern "C" void fsg_batch_destroy(fsg_batch* b) {
   if (b->graph) hipGraphDestroy(b->graph);
   if (b->dev) hipFree(b->dev);
   if (b->d_prep) hipFree(b->d_prep);
+  wave_batch_destroy(&b->wave);
   delete b;
 }

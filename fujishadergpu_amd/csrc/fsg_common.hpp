@@ -48,6 +48,15 @@ struct TimedScope {
   }
 };
 
+// Descriptor i of an array the host wrote before the launch and no
+// kernel writes, read through the constant address space: uniform fields take
+// scalar loads, and no store of the kernel makes the compiler read them again
+// (through a plain pointer every output store could alias them).
+template <class T>
+__device__ __forceinline__ const T& desc(const T* p, int i) {
+  return *(const T*)((const __attribute__((address_space(4))) T*)p + i);
+}
+
 int validate_window(const fsg_window* w, const char* who);
 int validate_region(const fsg_window* w, const fsg_region* r, const char* who);
 int validate_output(const fsg_output* o, const fsg_region* r, const char* who);

@@ -952,8 +952,24 @@ static int launch_fast(const UsmArgs& A, int n_tiles, hipStream_t s) {
 static std::atomic<int> g_usm_variant{0};
 
 // One launch: the blocks whose rows and columns all lie inside the window
-// (16-byte aligned rows, no reflection) take the FAST = true code, the others
-// the reflecting loads.
+// (no reflection) take the FAST = true code, the others the reflecting loads.
+template <int NR, int NS, int KR>
+__device__ __forceinline__ void wave_tile(const WaveArgs& A, int tile, double* P, double* Hc) {
+  const int x0 = (tile % A.n_strips) * A.SW;
+  const int y0 = (tile / A.n_strips) * A.SH;
+  const int rows = min(A.SH, A.rh - y0);
+  const int cs = A.c0 + x0 - A.R;
+  const int wy0 = A.r0 + y0;
+  const bool fast = cs >= 0 && cs + WAVE_NC <= A.W && wy0 - A.R - 1 >= 0 &&
+                    wy0 + rows - 1 + A.R <= A.H - 1;
+  if (fast)
+    usm_wave_block<NR, NS, true, KR>(A, tile, P, Hc);
+  else
+    usm_wave_block<NR, NS, false, KR>(A, tile, P, Hc);
+}
+
+// Block b takes tile (b & 7) * per_xcd + (b >> 3): consecutive tiles of a
+// strip (vertical neighbours, sharing halo rows) land on one XCD and its L2.
 template <int NR, int NS, int KR>
 __global__ void __launch_bounds__(64) k_usm_wave(WaveArgs A) {
   __shared__ double P[4 * WAVE_PB];
@@ -961,18 +977,30 @@ __global__ void __launch_bounds__(64) k_usm_wave(WaveArgs A) {
   const int b = blockIdx.x;
   const int tile = (b & 7) * A.per_xcd + (b >> 3);
   if (tile >= A.n_blocks) return;
-  const int x0 = (tile % A.n_strips) * A.SW;
-  const int y0 = (tile / A.n_strips) * A.SH;
-  const int rows = min(A.SH, A.rh - y0);
-  const int cs = A.c0 + x0 - A.R;
-  const int wy0 = A.r0 + y0;
-  const bool fast = cs >= 0 && cs + WAVE_NC <= A.W && (A.ld & 3) == 0 &&
-                    ((((uintptr_t)(A.win + cs)) & 15) == 0) && wy0 - A.R - 1 >= 0 &&
-                    wy0 + rows - 1 + A.R <= A.H - 1;
-  if (fast)
-    usm_wave_block<NR, NS, true, KR>(A, tile, P, Hc);
-  else
-    usm_wave_block<NR, NS, false, KR>(A, tile, P, Hc);
+  wave_tile<NR, NS, KR>(A, tile, P, Hc);
+}
+
+// Many windows' wavefront main passes in one launch (fsg_batch): block b
+// finds its window by binary search over the block prefix (each window's
+// count a multiple of 8, so b & 7 is also the window-local XCD slot), then
+// runs k_usm_wave's block on that window's arguments, read through the
+// constant address space.
+template <int NR, int NS, int KR>
+__global__ void __launch_bounds__(64) k_usm_wave_batch(const WaveArgs* args, const int* pre,
+                                                       int n) {
+  __shared__ double P[4 * WAVE_PB];
+  __shared__ double Hc[(NS > 0 ? NS : 1) * 2 * WAVE_NC];
+  const int b = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (desc(pre, mid) <= b) lo = mid; else hi = mid - 1;
+  }
+  const WaveArgs& A = desc(args, lo);
+  const int local = b - desc(pre, lo);
+  const int tile = (local & 7) * A.per_xcd + (local >> 3);
+  if (tile >= A.n_blocks) return;
+  wave_tile<NR, NS, KR>(A, tile, P, Hc);
 }
 
 // Bilinear row coefficients of every source at every region row (the same
@@ -1043,10 +1071,10 @@ static bool wave_eligible(const UsmArgs& A, bool nanaw) {
          A.nr <= WAVE_MAX_NR && A.R <= WAVE_MAX_R && A.n_src <= WAVE_MAX_SRC;
 }
 
-// Builds the coefficient tables (in tab, the workspace's table area) and
-// launches the wavefront kernel for the arguments of a fast-path call.
-static int dispatch_wave(const UsmArgs& U, int4* tab, hipStream_t s) {
-  WaveArgs A{};
+// The wavefront kernel's arguments for a fast-path call (coefficient tables
+// at tab, the workspace's table area) and launches the tables' kernel.
+static int wave_args(const UsmArgs& U, int4* tab, WaveArgs& A, hipStream_t s) {
+  A = WaveArgs{};
   A.win = U.W.p;
   A.ld = U.W.ld;
   A.H = U.W.h;
@@ -1086,6 +1114,15 @@ static int dispatch_wave(const UsmArgs& U, int4* tab, hipStream_t s) {
     hipLaunchKernelGGL(k_wave_tables, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, s, T);
     FSG_CHECK_LAUNCH("k_wave_tables");
   }
+  return FSG_OK;
+}
+
+// Builds the coefficient tables and launches the wavefront kernel for the
+// arguments of a fast-path call.
+static int dispatch_wave(const UsmArgs& U, int4* tab, hipStream_t s) {
+  WaveArgs A;
+  const int rc = wave_args(U, tab, A, s);
+  if (rc) return rc;
   if (U.nr == 3 && U.rad[0] == 2 && U.rad[1] == 8 && U.rad[2] == 32) {
     if (U.n_src == 0) return launch_wave<3, 0, 1>(A, s);
     if (U.n_src == 1) return launch_wave<3, 1, 1>(A, s);
@@ -1496,6 +1533,130 @@ int fsg::usm_prep_jobs(const fsg_window* win, const fsg_topousm_params* p, void*
     level++;
   }
   return FSG_OK;
+}
+
+// Resident blocks of one k_usm_wave_batch instance on the current device.
+template <int NS>
+static int wave_batch_capacity() {
+  static const int cap = [] {
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_usm_wave_batch<3, NS, 1>, 64, 0) !=
+            hipSuccess || nb <= 0) {
+      (void)hipGetLastError();
+      nb = 8;
+    }
+    return cus * nb;
+  }();
+  return cap;
+}
+
+int fsg::wave_batch_create(const std::vector<WaveTileIn>& in, std::vector<char>& taken,
+                           WaveBatch* wb) {
+  *wb = WaveBatch{};
+  taken.assign(in.size(), 0);
+  if (g_usm_variant.load(std::memory_order_relaxed) == 1) return FSG_OK;
+  // the instances batched here: the default radii, 0..3 bilinear sources
+  auto inst_of = [](const UsmArgs& U) {
+    if (U.nr != 3 || U.rad[0] != 2 || U.rad[1] != 8 || U.rad[2] != 32) return -1;
+    if (U.R > WAVE_MAX_R || U.n_src > WAVE_MAX_SRC || U.rh <= 0 || U.rw <= 0) return -1;
+    return U.n_src;
+  };
+  int count[WAVE_MAX_SRC + 1] = {0};
+  for (const WaveTileIn& t : in) {
+    const int k = inst_of(*t.args);
+    if (k >= 0) count[k]++;
+  }
+  int inst = -1;
+  for (int k = 0; k <= WAVE_MAX_SRC; k++)
+    if (count[k] > 0 && (inst < 0 || count[k] > count[inst])) inst = k;
+  if (inst < 0) return FSG_OK;
+  std::vector<WaveArgs> A;
+  std::vector<size_t> idx;
+  int64_t strip_rows = 0;
+  for (size_t i = 0; i < in.size(); i++) {
+    if (inst_of(*in[i].args) != inst) continue;
+    WaveArgs a;
+    int4* tab = (int4*)((char*)in[i].workspace + plan_coarse(in[i].win, in[i].p).tab_off);
+    if (int rc = wave_args(*in[i].args, tab, a, nullptr)) return rc;
+    strip_rows += (int64_t)a.n_strips * a.rh;
+    A.push_back(a);
+    idx.push_back(i);
+  }
+  // rows per block: the batch's strip rows over 32 rounds of resident blocks
+  // (at most a window's rows, at least 16), the windows with the longest
+  // blocks first, so that the short ones fill in behind them.  Each block
+  // pays a 2R + 1 row start-up, yet short blocks balance better: on 256
+  // tiles of sides 256-4096 (about 44 rows per block) 32 rounds ran 11.9 ms,
+  // 16 rounds 12.2, 4 rounds 14.9, 2 rounds 16.6 and 64 rounds 12.3.
+  int cap = 0;
+  switch (inst) {
+    case 0: cap = wave_batch_capacity<0>(); break;
+    case 1: cap = wave_batch_capacity<1>(); break;
+    case 2: cap = wave_batch_capacity<2>(); break;
+    default: cap = wave_batch_capacity<3>(); break;
+  }
+  const int64_t slots = 32 * (int64_t)std::max(1, cap);
+  const int64_t sh_all = std::max<int64_t>(16, (strip_rows + slots - 1) / slots);
+  for (WaveArgs& a : A) a.SH = (int)std::max<int64_t>(16, std::min<int64_t>(sh_all, a.rh));
+  std::stable_sort(A.begin(), A.end(),
+                   [](const WaveArgs& x, const WaveArgs& y) { return x.SH > y.SH; });
+  std::vector<int> pre{0};
+  for (WaveArgs& a : A) {
+    const int64_t nb = (int64_t)a.n_strips * ((a.rh + a.SH - 1) / a.SH);
+    a.n_blocks = (int)nb;
+    a.per_xcd = (int)((nb + 7) / 8);
+    const int64_t next = (int64_t)pre.back() + 8 * (int64_t)a.per_xcd;
+    if (next > INT32_MAX) return fail(FSG_E_UNSUPPORTED, "wave_batch_create: too many blocks");
+    pre.push_back((int)next);
+  }
+  void* d_args = nullptr;
+  int* d_pre = nullptr;
+  int rc = check_hip(hipMalloc(&d_args, sizeof(WaveArgs) * A.size()), "wave_batch_create");
+  if (!rc) rc = check_hip(hipMalloc((void**)&d_pre, sizeof(int) * pre.size()), "wave_batch_create");
+  if (!rc)
+    rc = check_hip(hipMemcpy(d_args, A.data(), sizeof(WaveArgs) * A.size(), hipMemcpyHostToDevice),
+                   "wave_batch_create");
+  if (!rc)
+    rc = check_hip(hipMemcpy(d_pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice),
+                   "wave_batch_create");
+  // the coefficient tables were queued on the null stream
+  if (!rc) rc = check_hip(hipStreamSynchronize(nullptr), "wave_batch_create");
+  if (rc) {
+    if (d_args) hipFree(d_args);
+    if (d_pre) hipFree(d_pre);
+    return rc;
+  }
+  wb->inst = inst;
+  wb->n_tiles = (int)A.size();
+  wb->n_blocks = pre.back();
+  wb->d_args = d_args;
+  wb->d_pre = d_pre;
+  for (size_t i : idx) taken[i] = 1;
+  return FSG_OK;
+}
+
+int fsg::wave_batch_launch(const WaveBatch& wb, hipStream_t s) {
+  if (wb.n_tiles <= 0 || wb.n_blocks <= 0) return FSG_OK;
+  const WaveArgs* a = (const WaveArgs*)wb.d_args;
+  TimedScope ts("k_usm_wave_batch", s);
+  switch (wb.inst) {
+    case 0: hipLaunchKernelGGL((k_usm_wave_batch<3, 0, 1>), dim3(wb.n_blocks), dim3(64), 0, s, a, wb.d_pre, wb.n_tiles); break;
+    case 1: hipLaunchKernelGGL((k_usm_wave_batch<3, 1, 1>), dim3(wb.n_blocks), dim3(64), 0, s, a, wb.d_pre, wb.n_tiles); break;
+    case 2: hipLaunchKernelGGL((k_usm_wave_batch<3, 2, 1>), dim3(wb.n_blocks), dim3(64), 0, s, a, wb.d_pre, wb.n_tiles); break;
+    case 3: hipLaunchKernelGGL((k_usm_wave_batch<3, 3, 1>), dim3(wb.n_blocks), dim3(64), 0, s, a, wb.d_pre, wb.n_tiles); break;
+    default: return fail(FSG_E_INVALID, "wave_batch_launch: no instance");
+  }
+  FSG_CHECK_LAUNCH("k_usm_wave_batch");
+  return FSG_OK;
+}
+
+void fsg::wave_batch_destroy(WaveBatch* wb) {
+  if (wb->d_args) hipFree(wb->d_args);
+  if (wb->d_pre) hipFree(wb->d_pre);
+  *wb = WaveBatch{};
 }
 
 int fsg::launch_dec_batch(const DecJob* jobs, const int* pre, int n_jobs, int n_blocks,

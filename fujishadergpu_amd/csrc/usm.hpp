@@ -415,6 +415,28 @@ inline int box_blocks(const BoxJob& J) { return J.nbx * ((J.h + box_band(J) - 1)
 int usm_prep_jobs(const fsg_window* win, const fsg_topousm_params* p, void* workspace,
                   std::vector<DecJob>& dec, std::vector<std::vector<BoxJob>>& box0,
                   std::vector<std::vector<BoxJob>>& box1);
+// Batched wavefront main pass over many windows (fsg_batch): one launch of
+// k_usm_wave's blocks for the fast-path, NaN-free windows of the most common
+// instance among those that take the default radii (2, 8, 32).
+struct WaveTileIn {
+  const fsg_window* win;
+  const fsg_topousm_params* p;
+  void* workspace;            // the window's fsg_topousm workspace (its table area)
+  const UsmArgs* args;        // usm_prepare's arguments (fast path, NaN-free)
+};
+struct WaveBatch {
+  int inst = -1;              // kernel instance (number of bilinear sources), -1: none
+  int n_tiles = 0, n_blocks = 0;
+  void* d_args = nullptr;     // the windows' wavefront-kernel arguments
+  int* d_pre = nullptr;       // block prefix (n_tiles + 1), each count a multiple of 8
+};
+// Builds the coefficient tables (once: they depend on the geometry only,
+// synchronously on the null stream) and the device descriptors; taken[i] = 1
+// for the windows it takes.  wb->n_tiles = 0 when it takes none.
+int wave_batch_create(const std::vector<WaveTileIn>& in, std::vector<char>& taken, WaveBatch* wb);
+int wave_batch_launch(const WaveBatch& wb, hipStream_t s);
+void wave_batch_destroy(WaveBatch* wb);
+
 int launch_dec_batch(const DecJob* jobs, const int* pre, int n_jobs, int n_blocks,
                      hipStream_t s);
 int launch_box_batch(int pass, const BoxJob* jobs, const int* pre, int n_jobs, int n_blocks,

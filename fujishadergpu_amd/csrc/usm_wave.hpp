@@ -127,8 +127,9 @@ __device__ __forceinline__ void store_out4(const OutDev& o, int r, int c, const 
   const int64_t off = (int64_t)r * o.ld + c;
   if (o.dtype == FSG_OUT_F32) {
     float* p = (float*)o.data + off;
-    if (full4 && (((uintptr_t)o.data | (uintptr_t)(o.ld * 4)) & 15) == 0) {
-      *(float4*)p = make_float4(w[0], w[1], w[2], w[3]);
+    if (full4) {  // one 16-byte store (dword-aligned)
+      const float4 q = make_float4(w[0], w[1], w[2], w[3]);
+      __builtin_memcpy(p, &q, sizeof(q));
     } else {
 #pragma unroll
       for (int m = 0; m < 4; m++)
@@ -161,7 +162,8 @@ __device__ __forceinline__ void store_out4(const OutDev& o, int r, int c, const 
 // WAVE_PB doubles, one radius at a time).  Hc: its LDS copy of the float64
 // coarse-row values of each source, [NS][2][WAVE_NC] (lane t uses columns
 // 4t .. 4t + 3 only).  FAST: every row and column the block reads lies inside
-// the window and the strip's rows are 16-byte aligned (no reflection).
+// the window (no reflection); a lane's four columns are one 16-byte load,
+// dword-aligned (gfx9 and later take it as one access at any column).
 // KR = 1: the box radii are {2, 8, 32} (every prefix-entry offset an
 // immediate); KR = 0: the radii of A.
 template <int KR>
@@ -193,7 +195,8 @@ __device__ __forceinline__ void usm_wave_block(const WaveArgs& A, int tile, doub
   auto load4 = [&](int wrow, float v[4]) {
     if (FAST) {
       const char* rp = (const char*)(Wp + (int64_t)wrow * ld);
-      const float4 q = *(const float4*)(rp + loff);
+      float4 q;
+      __builtin_memcpy(&q, rp + loff, sizeof(q));
       v[0] = q.x;
       v[1] = q.y;
       v[2] = q.z;
