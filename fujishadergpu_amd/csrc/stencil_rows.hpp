@@ -55,8 +55,9 @@ __device__ __forceinline__ float dqu(float a, const DivK& d) {
 constexpr int SR_SHADE_COLS = 62, SR_CURV_COLS = 60;
 // Rows one wave walks: at most 64, fewer on small windows so that the launch
 // still has SR_MIN_WAVES waves (a wave walks its rows one after another),
-// at least 8.
-constexpr int SR_SEG_MAX = 64, SR_SEG_MIN = 8, SR_MIN_WAVES = 8192;
+// at least 8.  16384 waves (about two rounds of resident waves) measured
+// about 7 % faster than 8192 on 8192^2 windows, NaN-free and NaN-aware.
+constexpr int SR_SEG_MAX = 64, SR_SEG_MIN = 8, SR_MIN_WAVES = 16384;
 
 // One interior launch: the window, the region's origin (output coordinates
 // are relative to it) and the interior rectangle [ir0, ir1) x [ic0, ic1) in
