@@ -134,16 +134,26 @@ using namespace fsg;
 // rest (gridDim.y is limited to 65535).
 constexpr int kMaxGridRows = 32768;
 
+// Four columns per lane: one 16-byte load per row (dword-aligned, any row
+// stride), a store only for the cells that match the sentinel.
 __global__ void k_nodata_to_nan(float* __restrict__ ras, int64_t ld, int h, int w,
                                 int has_nodata, double nodata) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= w) return;
+  const int c = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= w || !has_nodata) return;
   for (int r = blockIdx.y; r < h; r += gridDim.y) {   // grid-stride over rows
     float* p = ras + (int64_t)r * ld + c;
-    const float v = *p;
-    if (has_nodata && !isnan(v)) {
-      if (fabs((double)v - nodata) <= 1e-6) *p = __builtin_nanf("");
+    float v[4];
+    if (c + 3 < w) {
+      float4 q;
+      __builtin_memcpy(&q, p, sizeof(q));
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[k] = c + k < w ? p[k] : __builtin_nanf("");
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (!isnan(v[k]) && fabs((double)v[k] - nodata) <= 1e-6) p[k] = __builtin_nanf("");
   }
 }
 
@@ -160,13 +170,13 @@ k_count_nan(const float* __restrict__ ras, int64_t ld, WinChunk wins, int n_wind
   if (wi >= n_windows) return;
   const int4 wd = wins.w[wi];
   unsigned long long local = 0;
-  // 16-byte loads when the window's rows start 16-byte aligned (block-uniform)
-  const bool vec = ((ld | wd.y) & 3) == 0 && ((uintptr_t)ras & 15) == 0;
-  const int w4 = vec ? (wd.w & ~3) : 0;
+  // 16-byte loads (dword-aligned: windows at any column, any row stride)
+  const int w4 = wd.w & ~3;
   for (int r = blockIdx.y; r < wd.z; r += gridDim.y) {
     const float* row = ras + (int64_t)(wd.x + r) * ld + wd.y;
     for (int c = 4 * (blockIdx.x * 256 + threadIdx.x); c < w4; c += 4 * gridDim.x * 256) {
-      const float4 q = *(const float4*)(row + c);
+      float4 q;
+      __builtin_memcpy(&q, row + c, sizeof(q));
       local += (isnan(q.x) ? 1ull : 0ull) + (isnan(q.y) ? 1ull : 0ull) +
                (isnan(q.z) ? 1ull : 0ull) + (isnan(q.w) ? 1ull : 0ull);
     }
@@ -478,7 +488,7 @@ extern "C" int fsg_nodata_to_nan(float* raster, int64_t ld, int32_t h, int32_t w
     return fail(FSG_E_INVALID, "fsg_nodata_to_nan: bad windows");
   hipStream_t s = as_stream(stream);
   if (has_nodata && h > 0 && w > 0) {
-    dim3 blk(256), grd((w + 255) / 256, std::min(h, kMaxGridRows));
+    dim3 blk(256), grd((w + 1023) / 1024, std::min(h, kMaxGridRows));
     if (std::isnan(nodata)) has_nodata = 0;  // NaN sentinel: isnan only
     if (has_nodata) {
       hipLaunchKernelGGL(k_nodata_to_nan, grd, blk, 0, s, raster, ld, h, w, has_nodata, nodata);
