@@ -230,6 +230,10 @@ static void launch_general(const StencilTile& S, const Rect* q, int n, bool curv
 // hide its loads (2048^2 windows ran faster as one general launch).
 constexpr int64_t SPLIT_MIN_DEFAULT = int64_t(1) << 23;
 static std::atomic<int64_t> g_split_min{SPLIT_MIN_DEFAULT};
+// NaN-aware windows: the general kernel's fill is far slower than the row
+// kernels' wave fill, so their interiors take the row kernels from 2^20
+// pixels (or the process-wide setting, when smaller).
+constexpr int64_t SPLIT_MIN_NANAW = int64_t(1) << 20;
 
 // Constant-step hillshade/slope or curvature: the row-walking kernel over
 // the interior, then ONE general launch over the border strips (and the
@@ -243,7 +247,9 @@ static void launch_split(const StencilTile& S, const fsg_window* win, const fsg_
   const int cols1 = curv ? SR_CURV_COLS : SR_SHADE_COLS;
   R.seg = seg_rows(in.h, (in.w + cols1 - 1) / cols1);
   dim3 grd;
-  bool ok = (int64_t)in.h * in.w >= g_split_min.load(std::memory_order_relaxed) &&
+  const int64_t split_all = g_split_min.load(std::memory_order_relaxed);
+  const int64_t split_min = S.nanaw ? std::min(SPLIT_MIN_NANAW, split_all) : split_all;
+  bool ok = (int64_t)in.h * in.w >= split_min &&
             in.h > 0 && rows_grid(in, cols1, R.seg, grd);
   if (curv) {
     R.curv = S.curv;

@@ -187,7 +187,8 @@ int fsg_curvature(const fsg_window* win, const fsg_region* reg,
  * launch for the border strips; smaller ones take the general kernel alone,
  * one launch.  A process-wide setting with no reference counterpart, for
  * equality tests and tuning: results are identical either way.  px < 0
- * restores the default (2^23); returns the previous value.
+ * restores the default (2^23); returns the previous value.  Windows with
+ * NaN take the row-walking kernels from min(px, 2^20) pixels.
  */
 int64_t fsg_stencil_set_split_min(int64_t px);
 
