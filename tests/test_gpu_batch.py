@@ -130,3 +130,30 @@ def test_fused_batch_matches_the_oracle_on_one_tile(cuda):
     usm = ref.topousm_direct(dem, radii=p["radii"], weights=p["weights"], pixel_size=10.0,
                              stats=p["global_stats"])
     assert np.abs(j.topousm.cpu().numpy() - usm).max() <= 2e-5
+
+
+def test_fused_batch_workgroup_items_equal_per_tile_launches(cuda):
+    """With the workgroup kernels forced (variant 1), the batch's TopoUSM tiles
+    are items of the persistent kernel instead of the batched wavefront
+    launch: the same bits as the per-tile launches."""
+    from fujishadergpu_amd import ops
+    from fujishadergpu_amd.batch import FusedBatch, run_unfused
+    jobs = _jobs(cuda, SPECS)
+    run_unfused(jobs)
+    torch.cuda.synchronize()
+    ref = _outs(jobs)
+    _reset(jobs)
+    ops.set_topousm_variant(1)
+    try:
+        b = FusedBatch(jobs)
+        info = b.info()
+        assert info["fused_usm_tiles"] == 7 and info["usm_items"] > 0
+        b.run((3, 0, 8))
+        torch.cuda.synchronize()
+        got = _outs(jobs)
+        b.close()
+    finally:
+        ops.set_topousm_variant(0)
+    for i, (g, r) in enumerate(zip(got, ref)):
+        for k, (x, y) in enumerate(zip(g, r)):
+            assert _bits_equal(x, y), f"tile {i} product {k}"
