@@ -445,8 +445,9 @@ def c5_jobs(dev, n_tiles=256):
 
 
 def bench_c5(dev):
-    """Fused batch (one persistent launch, hipGraph replay) vs per-tile,
-    per-algorithm launches over the 256 ragged tiles."""
+    """Fused batch (the batched wavefront main pass, the row-walking stencil
+    interiors and one persistent launch over the rest, as one hipGraph
+    replay) vs per-tile, per-algorithm launches over the 256 ragged tiles."""
     import torch
     from fujishadergpu_amd.batch import FusedBatch, run_unfused, tune
     from fujishadergpu_amd.stats import norm_stats_prepass
